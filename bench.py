@@ -1,0 +1,136 @@
+"""Benchmark of the simulation loop on one MI355X.
+
+One bench step = one complete simulation of the workload: every (parameter
+set, stimulus) run advanced through all of its dt steps by libneurogpu's HIP
+kernel, inputs already resident in device memory.  The headline workload is
+the active pyramidal-like cell (500 compartments, Mainen channel set),
+1024 parameter sets x 3 step currents, dt = 0.02 ms, 10,000 steps.
+
+Prints one JSON line (the last line of stdout).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+_ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, _ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "compartment-steps/s at FP64 oracle <=1e-6 mV"
+
+
+def algorithmic_bytes_per_comp_step(model) -> float:
+    """B_cs = 16 (1 + S) + 8 N_rec / N_c: every state variable and v read and
+    written once per compartment-step, plus the recorded trace."""
+    return 16.0 * (1.0 + model.state_count()) + 8.0 * len(model.rec_sites) / model.tree.N
+
+
+def traffic_from_profile(name: str):
+    """HBM bytes per launch recorded by the rocprofv3 PMC passes, if present
+    (profiles/<name>_traffic.txt, line 'traffic_bytes_per_launch: <value>')."""
+    path = os.path.join(_ROOT, "profiles", f"{name}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(w, nsets: int, threads: int):
+    """The FP64 oracle on a bounded sample of the same workload."""
+    from oracle import cpu_ref
+    m = w.model
+    x = w.x[:nsets]
+    t0 = time.perf_counter()
+    ref = cpu_ref.simulate(m.tree, m.mechset_id, m.allparams(x), w.amps, w.dts, m.stim_seg, m.stim_area,
+                           m.v_init, m.rec_sites, nthreads=threads)
+    el = time.perf_counter() - t0
+    cs = nsets * w.amps.shape[0] * m.tree.N * w.dts.size
+    return ref, cs / el, el
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--nsets", type=int, default=None)
+    ap.add_argument("--nt", type=int, default=None)
+    ap.add_argument("--waves-per-run", type=int, default=0)
+    ap.add_argument("--runs-per-block", type=int, default=0)
+    ap.add_argument("--graph", type=int, default=1)
+    ap.add_argument("--cpu-sets", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (runs are independent, "
+              f"so the throughput of N GPUs is N times this figure).", file=sys.stderr)
+        sys.exit(2)
+
+    from neurogpu_amd import models
+    from neurogpu_amd._lib import Simulator
+
+    w = models.config(args.config, nsets=args.nsets, nt=args.nt)
+    m = w.model
+    print(f"bench: {w.name} N={m.tree.N} runs={w.nruns} nt={w.dts.size} S={m.state_count():.2f}", flush=True)
+    sim = Simulator(0)
+    sim.set_tree(m.tree, m.mechset_id)
+    sim.set_sim(m.v_init, m.rec_sites)
+    sim.set_params(w.params, m.ncomps, m.nparams)
+    sim.set_stimuli(w.amps, w.dts, m.stim_seg, m.stim_area)
+    sim.set_launch(args.waves_per_run, args.runs_per_block, 0, bool(args.graph))
+    for i in range(args.warmup):
+        _, st = sim.run(fetch=False)
+        print(f"warmup {i}: kernel {st['ms_kernel']:.1f} ms", flush=True)
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        _, st = sim.run(fetch=False)
+        kernel_ms.append(st["ms_kernel"])
+        print(f"step {i}: kernel {st['ms_kernel']:.1f} ms", flush=True)
+    wall = time.perf_counter() - t0
+    gpu = sim.fetch().reshape(w.x.shape[0], w.amps.shape[0], len(m.rec_sites), w.dts.size)
+    sim.close()
+
+    comp_steps = w.comp_steps
+    value = comp_steps * args.steps / wall
+    launch_ms = float(np.mean(kernel_ms)) / max(1, st["launches"])
+    bcs = algorithmic_bytes_per_comp_step(m)
+    alg_bytes_per_launch = comp_steps / max(1, st["launches"]) * bcs
+    achieved = alg_bytes_per_launch / (launch_ms * 1e-3) / 1e9
+    traffic = traffic_from_profile(f"config{args.config}")
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "bytes_per_comp_step": round(bcs, 3), "kernel_ms_per_launch": round(launch_ms, 3)}
+
+    result = {
+        "metric": METRIC, "value": value, "unit": "comp-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{w.name}: {m.tree.N} compartments, {m.mechset} mechanisms, "
+                               f"{w.x.shape[0]} sets x {w.amps.shape[0]} stimuli, dt {w.dts[0]} ms, "
+                               f"{w.dts.size} steps", "runs": w.nruns, "compartments": m.tree.N,
+                   "nt": int(w.dts.size), "launch": {k: st[k] for k in ("waves_per_run", "slots_per_lane",
+                                                                      "runs_per_block", "launches")}},
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        threads = max(1, min(16, os.cpu_count() or 1))
+        nsets = min(args.cpu_sets, w.x.shape[0])
+        ref, cps, el = cpu_baseline(w, nsets, threads)
+        d = float(np.max(np.abs(gpu[:nsets] - ref)))
+        result["cpu_baseline"] = {"value": cps, "unit": "comp-steps/s", "cores": threads, "kind": "port",
+                                  "sample": f"{nsets} sets x {w.amps.shape[0]} stimuli x {w.dts.size} steps "
+                                            f"of the same workload ({el:.1f} s)"}
+        result["parity_sample_max_abs_mV"] = d
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,122 @@
+/*
+ * neurogpu.h -- C ABI of libneurogpu, the MI355X (gfx950) engine for
+ * NeuroGPU's compartmental simulation loop.
+ *
+ * The reference has no FFI: its boundary is the `neuroGPU <i>` process plus
+ * files.  Each entry point below replaces one step of that process; the
+ * reference function it stands in for is cited (paths relative to the
+ * reference repository root, CANON = GUI/BBP_TTPC_EXAMPLE/pyNeuroGPU_unix/src).
+ *
+ * Conventions: 0 on success, a negative NGPU_E* code on failure, with the
+ * message in ngpu_last_error().  The context owns all device memory; callers
+ * own host buffers.  One context per thread; contexts are not re-entrant.
+ * No entry point retains a caller pointer after it returns.
+ */
+#ifndef NEUROGPU_H
+#define NEUROGPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NGPU_OK 0
+#define NGPU_E_ARG (-1)      /* invalid argument or inconsistent sizes        */
+#define NGPU_E_FILE (-2)     /* missing or malformed data file                */
+#define NGPU_E_HIP (-3)      /* HIP runtime error                             */
+#define NGPU_E_STATE (-4)    /* call order: model/params/stimuli not set      */
+#define NGPU_E_NOMEM (-5)    /* host or device allocation failed              */
+
+/* Mechanism sets compiled into the library. */
+#define NGPU_MECH_PASSIVE 0  /* pas                                               */
+#define NGPU_MECH_TTPC 1     /* BBP TTPC, 13 mechanisms (Code/OneKernel/src/AllModels.cu) */
+#define NGPU_MECH_MAINEN 2   /* ca cad kca km kv na pas (Figures/FigureS4_mainen/CParsed) */
+
+typedef struct ngpu_ctx ngpu_ctx;
+
+/* Tree data = the contents of BasicConstSegP.csv, i.e. the reference's HMat
+ * (CANON/Util.h:70-110) as filled by ReadParallelNeuronData (CANON/MainC.cu:335-551).
+ * Index conventions are the file's (1-based rows; e, f, ks, fidxs in Hines-row order). */
+typedef struct {
+    int N, ncomps, mechset, nmodels;
+    const double *e, *f, *cm;
+    const int *ks, *seg_to_comp, *bool_model, *son_no;
+    int depth, logn_depth, n_fathers, n_call;
+    const int *rel_starts, *rel_ends, *rel_vec, *seg_start, *seg_end, *fathers, *fidxs;
+    int comp_depth, comp_fdepth;
+    const int *comp_by_level32, *comp_by_flevel32;
+    int n_lrel, n_flrel;
+    const int *lrel_starts, *lrel_ends, *flrel_starts, *flrel_ends;
+} ngpu_tree;
+
+typedef struct {
+    double ms_total;          /* upload + kernels + download                         */
+    double ms_kernel;         /* HIP events around the simulation launches            */
+    double comp_steps;        /* sum over runs of N * Nt                              */
+    double comp_steps_per_s;  /* comp_steps / (ms_kernel / 1000)                      */
+    int waves_per_run, slots_per_lane, runs_per_block, launches;
+    int nvar;                 /* distinct run-varying parameter columns               */
+} ngpu_stats;
+
+/* Create a context on HIP device `device` (replaces cudaSetDevice in CANON/kernel.cu:24-31). */
+int ngpu_create(int device, ngpu_ctx **out);
+
+/* Model tree and mechanism placement (replaces initFrameWork's
+ * cudaMemcpyToSymbol of the tree maps, CANON/CudaStuff.cu:372-437). */
+int ngpu_set_tree(ngpu_ctx *ctx, const ngpu_tree *tree);
+
+/* v_init and record sites (replaces ReadSimData, CANON/Util.cu:394-432). */
+int ngpu_set_sim(ngpu_ctx *ctx, double v_init, int nrec, const int *rec_sites);
+
+/* Parameter matrix [nsets][nparams*ncomps], index p*ncomps + comp
+ * (replaces ReadAllParams, CANON/Util.cu:210-234). */
+int ngpu_set_params(ngpu_ctx *ctx, const double *allparams, int nsets, int ncomps, int nparams);
+
+/* Stimuli: amps [nstim][nt_amps] in nA (nt_amps >= nt), dts [nt] in ms,
+ * stim_seg is 0-based (Stim_meta loc - 1), area in um^2
+ * (replaces ReadCSVStim, CANON/Util.cu:337-390). */
+int ngpu_set_stimuli(ngpu_ctx *ctx, const double *amps, int nstim, int nt_amps,
+                     const double *dts, int nt, int stim_seg, double area);
+
+/* Launch shape (the autotuner's knobs; 0 = choose automatically):
+ * waves_per_run in 1..8, runs_per_block for waves_per_run == 1 (1..4),
+ * steps_per_launch (0 = all steps in one launch), use_graph (capture the
+ * launch sequence into a hipGraph and replay it). No reference counterpart:
+ * the reference fixes the launch at NSets x (32, NStim) (CANON/CudaStuff.cu:487-507). */
+int ngpu_set_launch(ngpu_ctx *ctx, int waves_per_run, int runs_per_block,
+                    int steps_per_launch, int use_graph);
+
+/* Run every (set, stimulus) for all Nt steps and copy the traces to vhot
+ * [nsets][nstim][nrec][nt] (caller-owned; may be NULL to keep them on the
+ * device).  Replaces callKernel/stEfork2Main (CANON/CudaStuff.cu:443-587). */
+int ngpu_run(ngpu_ctx *ctx, double *vhot, ngpu_stats *stats);
+
+/* Copy the traces of the last run to vhot (same layout as ngpu_run). */
+int ngpu_fetch_vhot(ngpu_ctx *ctx, double *vhot);
+
+/* Parse a reference data directory (BasicConstSegP.csv, Sim.csv,
+ * AllParams.csv, Stim_meta.csv, Stim_raw<suffix>.csv, times<suffix>.csv) and
+ * load it into a new context (replaces RunByModelP's Init,
+ * CANON/MainC.cu:37-80 and :592-626).  mechset < 0 infers the set from NModels. */
+int ngpu_open(const char *data_dir, const char *suffix, int mechset, int device, ngpu_ctx **out);
+
+/* Write the last traces as VHotP<dev>.dat: int32 N, int32 3, N doubles
+ * (replaces SaveArrayToFile, CANON/Util.cu:443-456). */
+int ngpu_write_vhot(ngpu_ctx *ctx, const char *path);
+
+/* Number of doubles ngpu_run writes (nsets * nstim * nrec * nt). */
+long long ngpu_vhot_size(const ngpu_ctx *ctx);
+
+/* Parameters per compartment of a mechanism set (NPARAMS of the generated
+ * AllModels.cuh), or a negative code. */
+int ngpu_mechset_nparams(int mechset);
+
+/* Message of the last failed call on ctx (or of the last failed
+ * ngpu_create / ngpu_open when ctx is NULL). */
+const char *ngpu_last_error(const ngpu_ctx *ctx);
+
+void ngpu_close(ngpu_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,227 @@
+"""ctypes binding of libneurogpu (the C ABI declared in include/neurogpu.h).
+
+The library is built in-tree (``neurogpu_amd/_build/libneurogpu.so``) by
+:func:`neurogpu_amd.build.build`.  There is no fallback: if the library is
+missing or cannot create a context, these calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from .formats import TreeData
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "libneurogpu.so")
+
+_IP = C.POINTER(C.c_int)
+_DP = C.POINTER(C.c_double)
+
+EXPORTS = ("ngpu_create", "ngpu_set_tree", "ngpu_set_sim", "ngpu_set_params", "ngpu_set_stimuli",
+           "ngpu_set_launch", "ngpu_run", "ngpu_fetch_vhot", "ngpu_open", "ngpu_write_vhot",
+           "ngpu_vhot_size", "ngpu_mechset_nparams", "ngpu_last_error", "ngpu_close")
+
+
+class NgpuError(RuntimeError):
+    """A libneurogpu call returned an error code."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libneurogpu error {code}: {msg}")
+        self.code = code
+
+
+class ngpu_tree(C.Structure):
+    _fields_ = [
+        ("N", C.c_int), ("ncomps", C.c_int), ("mechset", C.c_int), ("nmodels", C.c_int),
+        ("e", _DP), ("f", _DP), ("cm", _DP),
+        ("ks", _IP), ("seg_to_comp", _IP), ("bool_model", _IP), ("son_no", _IP),
+        ("depth", C.c_int), ("logn_depth", C.c_int), ("n_fathers", C.c_int), ("n_call", C.c_int),
+        ("rel_starts", _IP), ("rel_ends", _IP), ("rel_vec", _IP), ("seg_start", _IP),
+        ("seg_end", _IP), ("fathers", _IP), ("fidxs", _IP),
+        ("comp_depth", C.c_int), ("comp_fdepth", C.c_int),
+        ("comp_by_level32", _IP), ("comp_by_flevel32", _IP),
+        ("n_lrel", C.c_int), ("n_flrel", C.c_int),
+        ("lrel_starts", _IP), ("lrel_ends", _IP), ("flrel_starts", _IP), ("flrel_ends", _IP),
+    ]
+
+
+class ngpu_stats(C.Structure):
+    _fields_ = [("ms_total", C.c_double), ("ms_kernel", C.c_double), ("comp_steps", C.c_double),
+                ("comp_steps_per_s", C.c_double), ("waves_per_run", C.c_int), ("slots_per_lane", C.c_int),
+                ("runs_per_block", C.c_int), ("launches", C.c_int), ("nvar", C.c_int)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+
+
+def lib():
+    """Load the shared library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"libneurogpu is not built ({LIB_PATH}); run neurogpu_amd.build.build()")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.ngpu_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.ngpu_set_tree.argtypes = [vp, C.POINTER(ngpu_tree)]
+        L.ngpu_set_sim.argtypes = [vp, C.c_double, C.c_int, _IP]
+        L.ngpu_set_params.argtypes = [vp, _DP, C.c_int, C.c_int, C.c_int]
+        L.ngpu_set_stimuli.argtypes = [vp, _DP, C.c_int, C.c_int, _DP, C.c_int, C.c_int, C.c_double]
+        L.ngpu_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.ngpu_run.argtypes = [vp, _DP, C.POINTER(ngpu_stats)]
+        L.ngpu_fetch_vhot.argtypes = [vp, _DP]
+        L.ngpu_open.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.POINTER(vp)]
+        L.ngpu_write_vhot.argtypes = [vp, C.c_char_p]
+        L.ngpu_vhot_size.argtypes = [vp]
+        L.ngpu_vhot_size.restype = C.c_longlong
+        L.ngpu_mechset_nparams.argtypes = [C.c_int]
+        L.ngpu_last_error.argtypes = [vp]
+        L.ngpu_last_error.restype = C.c_char_p
+        L.ngpu_close.argtypes = [vp]
+        L.ngpu_close.restype = None
+        for name in ("ngpu_create", "ngpu_set_tree", "ngpu_set_sim", "ngpu_set_params", "ngpu_set_stimuli",
+                     "ngpu_set_launch", "ngpu_run", "ngpu_fetch_vhot", "ngpu_open", "ngpu_write_vhot",
+                     "ngpu_mechset_nparams"):
+            getattr(L, name).restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _tree_struct(t: TreeData, mechset: int):
+    keep = []
+
+    def ia(x):
+        a = np.ascontiguousarray(np.asarray(x).ravel(), dtype=np.int32)
+        keep.append(a)
+        return a.ctypes.data_as(_IP)
+
+    def da(x):
+        a = np.ascontiguousarray(np.asarray(x).ravel(), dtype=np.float64)
+        keep.append(a)
+        return a.ctypes.data_as(_DP)
+
+    s = ngpu_tree(t.N, t.ncomps, mechset, t.nmodels, da(t.e), da(t.f), da(t.cm), ia(t.ks), ia(t.seg_to_comp),
+                  ia(t.bool_model), ia(t.son_no), t.depth, t.logn_depth, t.n_fathers, t.n_call,
+                  ia(t.rel_starts), ia(t.rel_ends), ia(t.rel_vec), ia(t.seg_start), ia(t.seg_end),
+                  ia(t.fathers), ia(t.fidxs), t.comp_by_level32.shape[0], t.comp_by_flevel32.shape[0],
+                  ia(t.comp_by_level32), ia(t.comp_by_flevel32), t.lrel_starts.size, t.flrel_starts.size,
+                  ia(t.lrel_starts), ia(t.lrel_ends), ia(t.flrel_starts), ia(t.flrel_ends))
+    return s, keep
+
+
+class Simulator:
+    """One libneurogpu context on one device.
+
+    Mirrors the reference's run: tree data, Sim data, parameter matrix and
+    stimuli in, VHot traces [nsets, nstim, nrec, nt] out."""
+
+    def __init__(self, device: int = 0, _handle: Optional[int] = None):
+        self._L = lib()
+        self._h = C.c_void_p(_handle)
+        if _handle is None:
+            self._check(self._L.ngpu_create(int(device), C.byref(self._h)), None)
+        self.nsets = self.nstim = self.nrec = self.nt = 0
+
+    def _check(self, rc: int, h=...):
+        if rc != 0:
+            hh = self._h if h is ... else h
+            msg = self._L.ngpu_last_error(hh)
+            raise NgpuError(rc, msg.decode() if msg else "")
+
+    @classmethod
+    def open(cls, data_dir: str, suffix: str = "", mechset: int = -1, device: int = 0) -> "Simulator":
+        """Load a reference data directory (ngpu_open)."""
+        L = lib()
+        h = C.c_void_p()
+        rc = L.ngpu_open(data_dir.encode(), suffix.encode(), int(mechset), int(device), C.byref(h))
+        if rc != 0:
+            raise NgpuError(rc, (L.ngpu_last_error(None) or b"").decode())
+        return cls(device, _handle=h.value)
+
+    def set_tree(self, t: TreeData, mechset: int) -> None:
+        s, keep = _tree_struct(t, mechset)
+        self._check(self._L.ngpu_set_tree(self._h, C.byref(s)))
+        del keep
+
+    def set_sim(self, v_init: float, rec_sites) -> None:
+        rec = np.ascontiguousarray(rec_sites, dtype=np.int32)
+        self._check(self._L.ngpu_set_sim(self._h, float(v_init), rec.size, rec.ctypes.data_as(_IP)))
+        self.nrec = rec.size
+
+    def set_params(self, params: np.ndarray, ncomps: int, nparams: int) -> None:
+        P = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
+        self._check(self._L.ngpu_set_params(self._h, P.ctypes.data_as(_DP), P.shape[0], int(ncomps), int(nparams)))
+        self.nsets = P.shape[0]
+
+    def set_stimuli(self, amps: np.ndarray, dts: np.ndarray, stim_seg: int, area: float) -> None:
+        A = np.ascontiguousarray(np.atleast_2d(amps), dtype=np.float64)
+        D = np.ascontiguousarray(dts, dtype=np.float64)
+        self._check(self._L.ngpu_set_stimuli(self._h, A.ctypes.data_as(_DP), A.shape[0], A.shape[1],
+                                             D.ctypes.data_as(_DP), D.size, int(stim_seg), float(area)))
+        self.nstim, self.nt = A.shape[0], D.size
+
+    def set_launch(self, waves_per_run: int = 0, runs_per_block: int = 0, steps_per_launch: int = 0,
+                   use_graph: bool = False) -> None:
+        self._check(self._L.ngpu_set_launch(self._h, int(waves_per_run), int(runs_per_block),
+                                            int(steps_per_launch), int(bool(use_graph))))
+
+    def vhot_size(self) -> int:
+        return int(self._L.ngpu_vhot_size(self._h))
+
+    def run(self, fetch: bool = True):
+        """Simulate; returns (vhot or None, stats dict)."""
+        st = ngpu_stats()
+        out = None
+        if fetch:
+            out = np.empty(self.vhot_size(), dtype=np.float64)
+            self._check(self._L.ngpu_run(self._h, out.ctypes.data_as(_DP), C.byref(st)))
+        else:
+            self._check(self._L.ngpu_run(self._h, None, C.byref(st)))
+        return out, st.as_dict()
+
+    def fetch(self) -> np.ndarray:
+        out = np.empty(self.vhot_size(), dtype=np.float64)
+        self._check(self._L.ngpu_fetch_vhot(self._h, out.ctypes.data_as(_DP)))
+        return out
+
+    def write_vhot(self, path: str) -> None:
+        self._check(self._L.ngpu_write_vhot(self._h, path.encode()))
+
+    def close(self) -> None:
+        if self._h and self._h.value:
+            self._L.ngpu_close(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def simulate_workload(w, device: int = 0, launch: Optional[dict] = None, fetch: bool = True):
+    """Run a :class:`models.Workload` on the GPU; returns (vhot [nsets,nstim,nrec,nt], stats)."""
+    m = w.model
+    with Simulator(device) as sim:
+        sim.set_tree(m.tree, m.mechset_id)
+        sim.set_sim(m.v_init, m.rec_sites)
+        sim.set_params(w.params, m.ncomps, m.nparams)
+        sim.set_stimuli(w.amps, w.dts, m.stim_seg, m.stim_area)
+        if launch:
+            sim.set_launch(**launch)
+        out, st = sim.run(fetch=fetch)
+    if out is not None:
+        out = out.reshape(w.x.shape[0], w.amps.shape[0], len(m.rec_sites), w.dts.size)
+    return out, st

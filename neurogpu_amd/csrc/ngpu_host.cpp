@@ -1,0 +1,738 @@
+/*
+ * ngpu_host.cpp -- host side of libneurogpu: context, model compilation for the
+ * device layout, parameter compression, launches (optionally captured into a
+ * hipGraph), timing, and the reference's data-file boundary.
+ *
+ * Replaces the reference's host runtime:
+ *   initFrameWork / callKernel / stEfork2Main  CANON/CudaStuff.cu:372-587
+ *   ReadParallelNeuronData                     CANON/MainC.cu:335-551
+ *   ReadAllParams / ReadCSVStim / ReadSimData  CANON/Util.cu:210-432
+ *   SaveArrayToFile                            CANON/Util.cu:443-456
+ * (CANON = GUI/BBP_TTPC_EXAMPLE/pyNeuroGPU_unix/src)
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/neurogpu.h"
+#include "sim_kernel.h"
+
+/* Per-set launchers, one translation unit each (sim_<set>.hip). */
+hipError_t ngpu_launch_passive(int nslot, bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
+hipError_t ngpu_launch_ttpc(int nslot, bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
+hipError_t ngpu_launch_mainen(int nslot, bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
+
+static const int kSlotSizes[] = {1, 2, 4, 6, 8};
+static std::string g_last_error;
+
+static int mech_nmodels(int ms) { return ms == NGPU_MECH_TTPC ? SetTTPC::NMECH : ms == NGPU_MECH_MAINEN ? SetMainen::NMECH : ms == NGPU_MECH_PASSIVE ? SetPassive::NMECH : -1; }
+static int mech_nparams(int ms) { return ms == NGPU_MECH_TTPC ? SetTTPC::NPARAM : ms == NGPU_MECH_MAINEN ? SetMainen::NPARAM : ms == NGPU_MECH_PASSIVE ? SetPassive::NPARAM : -1; }
+static int mech_nstate(int ms) { return ms == NGPU_MECH_TTPC ? SetTTPC::NSTATE : ms == NGPU_MECH_MAINEN ? SetMainen::NSTATE : SetPassive::NSTATE; }
+
+template <class T> struct DevArr {
+    T *p = nullptr;
+    size_t n = 0;
+    void release() { if (p) (void)hipFree(p); p = nullptr; n = 0; }
+};
+
+struct ngpu_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    /* model */
+    bool have_tree = false;
+    int N = 0, ncomps = 0, mechset = 0, nmodels = 0, depth = 0, logn = 0;
+    std::vector<double> e, f, cm;
+    std::vector<int> par, son, seg_to_comp;
+    std::vector<unsigned> mask;
+    std::vector<int> chains, lvl_ptr, fath, flvl_ptr, fchild, fidx;
+    /* sim */
+    bool have_sim = false;
+    double v_init = -65.0;
+    std::vector<int> rec;
+    /* params */
+    bool have_params = false;
+    int nsets = 0, nparams = 0, nvar = 0;
+    std::vector<int> pref;        /* [nparams*N]: >=0 varying column, else -(uniform index)-1 */
+    std::vector<double> puni, pv;
+    /* stimuli */
+    bool have_stim = false;
+    int nstim = 0, nt_amps = 0, nt = 0, stim_seg = -1;
+    double area = 1.0;
+    std::vector<double> amps, dts;
+    /* launch configuration */
+    int cfg_w = 0, cfg_r = 0, cfg_spl = 0, cfg_graph = 0;
+    /* device state */
+    int lay_W = 0, lay_slot = 0, lay_NQ = 0;
+    bool tables_dirty = true, params_dirty = true, stim_dirty = true, graph_dirty = true;
+    DevArr<int> d_qh, d_qph, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_fidx, d_pvar, d_rec;
+    DevArr<double> d_qcm, d_e, d_f, d_pval, d_pv, d_dts, d_amps, d_states, d_carry, d_vhot;
+    int stim_q = -1;
+    std::vector<int> rec_q;
+    hipGraphExec_t gexec = nullptr;
+    bool ran = false;
+    int last_launches = 0;
+};
+
+static int fail(ngpu_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    g_last_error = buf;
+    return code;
+}
+
+#define HIPCHK(c, call)                                                                           \
+    do {                                                                                          \
+        hipError_t _e = (call);                                                                   \
+        if (_e != hipSuccess)                                                                     \
+            return fail((c), NGPU_E_HIP, "%s failed: %s", #call, hipGetErrorString(_e));          \
+    } while (0)
+
+template <class T>
+static int upload(ngpu_ctx *c, DevArr<T> &d, const std::vector<T> &h)
+{
+    size_t n = std::max<size_t>(h.size(), 1);
+    if (d.n != n) {
+        d.release();
+        HIPCHK(c, hipMalloc((void **)&d.p, n * sizeof(T)));
+        d.n = n;
+    }
+    if (!h.empty()) HIPCHK(c, hipMemcpyAsync(d.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+template <class T>
+static int alloc(ngpu_ctx *c, DevArr<T> &d, size_t n)
+{
+    n = std::max<size_t>(n, 1);
+    if (d.n == n) return 0;
+    d.release();
+    hipError_t e = hipMalloc((void **)&d.p, n * sizeof(T));
+    if (e != hipSuccess) return fail(c, NGPU_E_NOMEM, "device allocation of %zu bytes failed: %s", n * sizeof(T), hipGetErrorString(e));
+    d.n = n;
+    return 0;
+}
+
+static void drop_graph(ngpu_ctx *c)
+{
+    if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+    c->graph_dirty = true;
+}
+
+extern "C" int ngpu_mechset_nparams(int ms) { return mech_nparams(ms) < 0 ? NGPU_E_ARG : mech_nparams(ms); }
+
+extern "C" int ngpu_create(int device, ngpu_ctx **out)
+{
+    if (!out) return fail(nullptr, NGPU_E_ARG, "ngpu_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return fail(nullptr, NGPU_E_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= ndev) return fail(nullptr, NGPU_E_ARG, "device %d out of range (0..%d)", device, ndev - 1);
+    ngpu_ctx *c = new ngpu_ctx();
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        delete c;
+        return fail(nullptr, NGPU_E_HIP, "could not create stream/events on device %d", device);
+    }
+    *out = c;
+    return NGPU_OK;
+}
+
+extern "C" void ngpu_close(ngpu_ctx *c)
+{
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    for (auto *d : {&c->d_qh, &c->d_qph, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_fidx, &c->d_pvar, &c->d_rec}) d->release();
+    for (auto *d : {&c->d_qcm, &c->d_e, &c->d_f, &c->d_pval, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_carry, &c->d_vhot}) d->release();
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+extern "C" const char *ngpu_last_error(const ngpu_ctx *c) { return c ? c->err.c_str() : g_last_error.c_str(); }
+
+/* ------------------------------------------------------------------ tree */
+
+extern "C" int ngpu_set_tree(ngpu_ctx *c, const ngpu_tree *t)
+{
+    if (!c || !t) return fail(c, NGPU_E_ARG, "ngpu_set_tree: NULL argument");
+    const int N = t->N;
+    if (N < 1 || t->ncomps < 1) return fail(c, NGPU_E_ARG, "N=%d and ncomps=%d must be positive", N, t->ncomps);
+    if (mech_nmodels(t->mechset) < 0) return fail(c, NGPU_E_ARG, "unknown mechanism set %d", t->mechset);
+    if (t->nmodels != mech_nmodels(t->mechset))
+        return fail(c, NGPU_E_ARG, "mechanism set %d has %d mechanisms, tree gives NModels=%d", t->mechset, mech_nmodels(t->mechset), t->nmodels);
+    if (!t->e || !t->f || !t->cm || !t->ks || !t->seg_to_comp || !t->bool_model || !t->son_no)
+        return fail(c, NGPU_E_ARG, "ngpu_set_tree: missing array");
+    if (t->depth < 0 || t->logn_depth < 0 || t->n_lrel != t->depth + 1 || t->n_flrel != t->depth)
+        return fail(c, NGPU_E_ARG, "inconsistent Depth=%d nLRel=%d nFLRel=%d", t->depth, t->n_lrel, t->n_flrel);
+    std::vector<int> par(N, -1), son(N);
+    for (int s = 1; s < N; s++) {
+        int r = N - s, kr = t->ks[r];
+        if (kr <= r || kr > N) return fail(c, NGPU_E_ARG, "Ks[%d]=%d is not a row above %d", r, kr, r);
+        par[s] = N - kr;
+    }
+    for (int s = 0; s < N; s++) {
+        if (t->seg_to_comp[s] < 0 || t->seg_to_comp[s] >= t->ncomps) return fail(c, NGPU_E_ARG, "SegToComp[%d]=%d out of range", s, t->seg_to_comp[s]);
+        son[s] = t->son_no[s];
+    }
+    std::vector<unsigned> mask(N, 0u);
+    for (int m = 0; m < t->nmodels; m++)
+        for (int s = 0; s < N; s++)
+            if (t->bool_model[(size_t)m * N + s]) mask[s] |= 1u << m;
+    /* Level schedule: unique runs per level, unique fathers per father-level. */
+    std::vector<int> chains, lvl_ptr{0}, fath, flvl_ptr{0}, fchild;
+    std::vector<int> elim(N, 0);
+    auto row_ok = [&](int r) { return r >= 0 && r < N; };
+    for (int L = 0; L <= t->depth; L++) {
+        int lo = t->lrel_starts[L], hi = t->lrel_ends[L];
+        if (lo < 0 || hi >= t->comp_depth || lo > hi) return fail(c, NGPU_E_ARG, "LRel range %d..%d invalid", lo, hi);
+        std::vector<int> seen;
+        for (int i = lo * 32; i < (hi + 1) * 32; i++) {
+            int J = t->comp_by_level32[i] - 1;
+            if (J < 0 || J > t->n_call) return fail(c, NGPU_E_ARG, "CompByLevel32 entry %d out of range", J + 1);
+            if (std::find(seen.begin(), seen.end(), J) != seen.end()) continue;
+            seen.push_back(J);
+            int i0 = t->seg_start[J] - 1, i1 = t->seg_end[J];
+            if (i0 < 1 || i1 > N) return fail(c, NGPU_E_ARG, "run %d spans rows %d..%d outside the tree", J, i0, i1);
+            for (int r = i0; r < i1; r++) elim[r - 1]++;
+            chains.push_back(i0);
+            chains.push_back(i1);
+        }
+        lvl_ptr.push_back((int)chains.size() / 2);
+        if (L < t->depth) {
+            int flo = t->flrel_starts[L], fhi = t->flrel_ends[L];
+            if (flo < 0 || fhi >= t->comp_fdepth || flo > fhi) return fail(c, NGPU_E_ARG, "FLRel range %d..%d invalid", flo, fhi);
+            std::vector<int> fseen;
+            for (int i = flo * 32; i < (fhi + 1) * 32; i++) {
+                int B = t->comp_by_flevel32[i] - 1;
+                if (B < 0 || B >= t->n_fathers) return fail(c, NGPU_E_ARG, "CompByFLevel32 entry %d out of range", B + 1);
+                if (std::find(fseen.begin(), fseen.end(), B) != fseen.end()) continue;
+                fseen.push_back(B);
+                int J = t->fathers[B] - 1;
+                if (!row_ok(J)) return fail(c, NGPU_E_ARG, "father row %d out of range", J + 1);
+                fath.push_back(J);
+                fath.push_back((int)fchild.size());
+                for (int j = t->rel_starts[B]; j <= t->rel_ends[B]; j++) {
+                    if (j < 1 || j > t->n_call) return fail(c, NGPU_E_ARG, "RelStarts/RelEnds out of range");
+                    int tr = t->rel_vec[j - 1] - 2;
+                    if (!row_ok(tr) || tr >= J) return fail(c, NGPU_E_ARG, "related row %d of father %d invalid", tr + 1, J + 1);
+                    elim[tr]++;
+                    fchild.push_back(tr);
+                }
+                fath.push_back((int)fchild.size());
+            }
+            flvl_ptr.push_back((int)fath.size() / 3);
+        }
+    }
+    for (int r = 0; r < N - 1; r++)
+        if (elim[r] != 1) return fail(c, NGPU_E_ARG, "tree maps eliminate row %d %d times (expected once)", r + 1, elim[r]);
+    std::vector<int> fidx((size_t)t->logn_depth * N);
+    for (size_t i = 0; i < fidx.size(); i++) {
+        int a = t->fidxs[i] - 1;
+        if (a < 0 || a > N) return fail(c, NGPU_E_ARG, "FIdxs entry %d out of range", a + 1);
+        fidx[i] = a;
+    }
+    c->N = N; c->ncomps = t->ncomps; c->mechset = t->mechset; c->nmodels = t->nmodels;
+    c->depth = t->depth; c->logn = t->logn_depth;
+    c->e.assign(t->e, t->e + N); c->f.assign(t->f, t->f + N); c->cm.assign(t->cm, t->cm + N);
+    c->par = par; c->son = son; c->seg_to_comp.assign(t->seg_to_comp, t->seg_to_comp + N); c->mask = mask;
+    c->chains = chains; c->lvl_ptr = lvl_ptr; c->fath = fath; c->flvl_ptr = flvl_ptr; c->fchild = fchild; c->fidx = fidx;
+    c->have_tree = true;
+    c->have_params = false;          /* parameter layout depends on the tree */
+    c->lay_W = 0; c->tables_dirty = true;
+    drop_graph(c);
+    return NGPU_OK;
+}
+
+extern "C" int ngpu_set_sim(ngpu_ctx *c, double v_init, int nrec, const int *rec_sites)
+{
+    if (!c) return fail(c, NGPU_E_ARG, "NULL context");
+    if (!c->have_tree) return fail(c, NGPU_E_STATE, "ngpu_set_sim before ngpu_set_tree");
+    if (nrec < 0 || (nrec > 0 && !rec_sites) || nrec > 64) return fail(c, NGPU_E_ARG, "nrec=%d invalid (0..64)", nrec);
+    for (int r = 0; r < nrec; r++)
+        if (rec_sites[r] < 0 || rec_sites[r] >= c->N) return fail(c, NGPU_E_ARG, "record site %d out of range", rec_sites[r]);
+    if (!std::isfinite(v_init)) return fail(c, NGPU_E_ARG, "v_init must be finite");
+    c->v_init = v_init;
+    c->rec.assign(rec_sites, rec_sites + nrec);
+    c->have_sim = true;
+    c->tables_dirty = true;
+    drop_graph(c);
+    return NGPU_OK;
+}
+
+extern "C" int ngpu_set_params(ngpu_ctx *c, const double *P, int nsets, int ncomps, int nparams)
+{
+    if (!c) return fail(c, NGPU_E_ARG, "NULL context");
+    if (!c->have_tree) return fail(c, NGPU_E_STATE, "ngpu_set_params before ngpu_set_tree");
+    if (nsets < 1 || !P) return fail(c, NGPU_E_ARG, "nsets=%d must be positive", nsets);
+    if (ncomps != c->ncomps) return fail(c, NGPU_E_ARG, "ncomps=%d but the tree has NComps=%d", ncomps, c->ncomps);
+    if (nparams != mech_nparams(c->mechset)) return fail(c, NGPU_E_ARG, "nparams=%d but mechanism set %d has %d", nparams, c->mechset, mech_nparams(c->mechset));
+    const int N = c->N;
+    const size_t row = (size_t)nparams * ncomps;
+    /* Column (p, comp) over sets: uniform columns become constants, varying
+     * columns are de-duplicated and stored per set. */
+    std::map<std::string, int> seen;
+    std::vector<int> colref(row);
+    std::vector<double> puni;
+    std::vector<std::vector<double>> vcols;
+    std::vector<double> col(nsets);
+    for (size_t k = 0; k < row; k++) {
+        bool uni = true;
+        for (int s = 0; s < nsets; s++) {
+            col[s] = P[(size_t)s * row + k];
+            if (std::memcmp(&col[s], &col[0], sizeof(double)) != 0) uni = false;
+        }
+        if (uni) {
+            colref[k] = -(int)puni.size() - 1;
+            puni.push_back(col[0]);
+        } else {
+            std::string key((const char *)col.data(), col.size() * sizeof(double));
+            auto it = seen.find(key);
+            if (it == seen.end()) {
+                it = seen.emplace(key, (int)vcols.size()).first;
+                vcols.push_back(col);
+            }
+            colref[k] = it->second;
+        }
+    }
+    c->nvar = (int)vcols.size();
+    c->pv.assign((size_t)nsets * std::max(c->nvar, 1), 0.0);
+    for (int k = 0; k < c->nvar; k++)
+        for (int s = 0; s < nsets; s++) c->pv[(size_t)s * c->nvar + k] = vcols[k][s];
+    c->pref.assign((size_t)nparams * N, 0);
+    for (int j = 0; j < nparams; j++)
+        for (int s = 0; s < N; s++) c->pref[(size_t)j * N + s] = colref[(size_t)j * ncomps + c->seg_to_comp[s]];
+    c->puni = puni;
+    c->nsets = nsets;
+    c->nparams = nparams;
+    c->have_params = true;
+    c->params_dirty = true;
+    c->tables_dirty = true;
+    drop_graph(c);
+    return NGPU_OK;
+}
+
+extern "C" int ngpu_set_stimuli(ngpu_ctx *c, const double *amps, int nstim, int nt_amps, const double *dts, int nt, int stim_seg, double area)
+{
+    if (!c) return fail(c, NGPU_E_ARG, "NULL context");
+    if (!c->have_tree) return fail(c, NGPU_E_STATE, "ngpu_set_stimuli before ngpu_set_tree");
+    if (nstim < 1 || nt < 1 || nt_amps < nt || !amps || !dts) return fail(c, NGPU_E_ARG, "invalid stimulus sizes nstim=%d nt=%d nt_amps=%d", nstim, nt, nt_amps);
+    if (stim_seg < -1 || stim_seg >= c->N) return fail(c, NGPU_E_ARG, "stimulus segment %d out of range", stim_seg);
+    if (!(area > 0)) return fail(c, NGPU_E_ARG, "stimulus area must be positive");
+    for (int i = 0; i < nt; i++)
+        if (!(dts[i] > 0) || !std::isfinite(dts[i])) return fail(c, NGPU_E_ARG, "dt[%d]=%g must be positive and finite", i, dts[i]);
+    c->amps.assign(amps, amps + (size_t)nstim * nt_amps);
+    c->dts.assign(dts, dts + nt);
+    c->nstim = nstim; c->nt_amps = nt_amps; c->nt = nt; c->stim_seg = stim_seg; c->area = area;
+    c->have_stim = true;
+    c->stim_dirty = true;
+    c->tables_dirty = true;
+    drop_graph(c);
+    return NGPU_OK;
+}
+
+extern "C" int ngpu_set_launch(ngpu_ctx *c, int w, int r, int spl, int graph)
+{
+    if (!c) return fail(c, NGPU_E_ARG, "NULL context");
+    if (w < 0 || w > 8 || r < 0 || r > 4 || spl < 0 || (graph != 0 && graph != 1))
+        return fail(c, NGPU_E_ARG, "invalid launch configuration w=%d r=%d spl=%d graph=%d", w, r, spl, graph);
+    c->cfg_w = w; c->cfg_r = r; c->cfg_spl = spl; c->cfg_graph = graph;
+    c->tables_dirty = true;
+    drop_graph(c);
+    return NGPU_OK;
+}
+
+extern "C" long long ngpu_vhot_size(const ngpu_ctx *c)
+{
+    if (!c || !c->have_params || !c->have_stim) return 0;
+    return (long long)c->nsets * c->nstim * (long long)c->rec.size() * c->nt;
+}
+
+/* ------------------------------------------------------------------ run */
+
+static void choose_layout(const ngpu_ctx *c, int &W, int &slot)
+{
+    const int N = c->N;
+    W = c->cfg_w;
+    if (W <= 0) W = std::max(1, (N + 64 * 8 - 1) / (64 * 8));
+    W = std::min(W, 8);
+    int need = (N + 64 * W - 1) / (64 * W);
+    slot = 8;
+    for (int s : kSlotSizes) if (s >= need) { slot = s; break; }
+    while (slot * 64 * W < N && W < 8) {             /* only reached for very large trees */
+        W++;
+        need = (N + 64 * W - 1) / (64 * W);
+        for (int s : kSlotSizes) if (s >= need) { slot = s; break; }
+    }
+}
+
+static int build_tables(ngpu_ctx *c, int W, int slot)
+{
+    const int N = c->N, LW = 64 * W, NQ = slot * LW;
+    if (NQ < N) return fail(c, NGPU_E_ARG, "tree of %d segments exceeds the largest layout (%d positions)", N, NQ);
+    /* positions: segments sorted by mechanism mask, stable */
+    std::vector<int> order(N);
+    for (int s = 0; s < N; s++) order[s] = s;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return c->mask[a] < c->mask[b]; });
+    std::vector<int> pos(N), qh(NQ, -1), qph(NQ, 0), qmeta(NQ, 0);
+    std::vector<double> qcm(NQ, 0.0);
+    for (int q = 0; q < N; q++) {
+        int s = order[q];
+        pos[s] = q;
+        qh[q] = N - 1 - s;
+        qph[q] = s == 0 ? N - 1 : N - 1 - c->par[s];
+        qmeta[q] = (int)(c->mask[s] | ((unsigned)std::min(c->son[s], 255) << 24));
+        qcm[q] = c->cm[s];
+    }
+    const int np = c->nparams;
+    std::vector<int> pvar((size_t)np * NQ, -1);
+    std::vector<double> pval((size_t)np * NQ, 0.0);
+    for (int j = 0; j < np; j++)
+        for (int q = 0; q < N; q++) {
+            int ref = c->pref[(size_t)j * N + order[q]];
+            if (ref >= 0) pvar[(size_t)j * NQ + q] = ref;
+            else pval[(size_t)j * NQ + q] = c->puni[-ref - 1];
+        }
+    c->rec_q.resize(c->rec.size());
+    for (size_t r = 0; r < c->rec.size(); r++) c->rec_q[r] = pos[c->rec[r]];
+    c->stim_q = c->stim_seg >= 0 ? pos[c->stim_seg] : -1;
+    std::vector<double> re(c->e), rf(c->f);
+    int rc;
+    if ((rc = upload(c, c->d_qh, qh)) || (rc = upload(c, c->d_qph, qph)) || (rc = upload(c, c->d_qmeta, qmeta)) ||
+        (rc = upload(c, c->d_qcm, qcm)) || (rc = upload(c, c->d_e, re)) || (rc = upload(c, c->d_f, rf)) ||
+        (rc = upload(c, c->d_pvar, pvar)) || (rc = upload(c, c->d_pval, pval)) || (rc = upload(c, c->d_chains, c->chains)) ||
+        (rc = upload(c, c->d_lvl, c->lvl_ptr)) || (rc = upload(c, c->d_fath, c->fath)) || (rc = upload(c, c->d_flvl, c->flvl_ptr)) ||
+        (rc = upload(c, c->d_fchild, c->fchild)) || (rc = upload(c, c->d_fidx, c->fidx)) || (rc = upload(c, c->d_rec, c->rec_q)))
+        return rc;
+    c->lay_W = W; c->lay_slot = slot; c->lay_NQ = NQ;
+    c->tables_dirty = false;
+    return NGPU_OK;
+}
+
+static hipError_t launch_one(ngpu_ctx *c, const SimArgs &a, dim3 grid, dim3 block, size_t lds)
+{
+    bool multi = c->lay_W > 1;
+    switch (c->mechset) {
+    case NGPU_MECH_TTPC: return ngpu_launch_ttpc(c->lay_slot, multi, grid, block, lds, c->stream, a);
+    case NGPU_MECH_MAINEN: return ngpu_launch_mainen(c->lay_slot, multi, grid, block, lds, c->stream, a);
+    default: return ngpu_launch_passive(c->lay_slot, multi, grid, block, lds, c->stream, a);
+    }
+}
+
+static int enqueue(ngpu_ctx *c, int &launches)
+{
+    const int N = c->N, NQ = c->lay_NQ, LW = 64 * c->lay_W;
+    const int nruns = c->nsets * c->nstim;
+    int R = 1;
+    if (c->lay_W == 1) R = c->cfg_r > 0 ? c->cfg_r : 4;
+    const size_t per_run = (size_t)(2 * (N + 1) + 3 * NQ + (int)c->rec.size() * 64) * sizeof(double);
+    while (R > 1 && per_run * R > 160 * 1024) R--;
+    const size_t lds = per_run * R;
+    if (lds > 160 * 1024) return fail(c, NGPU_E_ARG, "tree needs %zu bytes of LDS per run (limit 160 KiB)", lds);
+    SimArgs a{};
+    a.N = N; a.NQ = NQ; a.LW = LW; a.runs_per_block = R;
+    a.nvar = std::max(c->nvar, 1); a.nrec = (int)c->rec.size(); a.nstim = c->nstim; a.nt = c->nt; a.nt_amps = c->nt_amps;
+    a.nruns = nruns; a.depth = c->depth; a.logn = c->logn; a.stim_q = c->stim_q;
+    a.stim_area = c->area; a.v_init = c->v_init;
+    a.q_h = c->d_qh.p; a.q_ph = c->d_qph.p; a.q_meta = c->d_qmeta.p; a.q_cm = c->d_qcm.p;
+    a.row_e = c->d_e.p; a.row_f = c->d_f.p; a.pvar = c->d_pvar.p; a.pval = c->d_pval.p; a.pv = c->d_pv.p;
+    a.chains = c->d_chains.p; a.lvl_ptr = c->d_lvl.p; a.fath = c->d_fath.p; a.flvl_ptr = c->d_flvl.p;
+    a.fchild = c->d_fchild.p; a.fidx = c->d_fidx.p; a.dts = c->d_dts.p; a.amps = c->d_amps.p; a.rec_q = c->d_rec.p;
+    a.states = c->d_states.p; a.carry = c->d_carry.p; a.vhot = c->d_vhot.p;
+    dim3 block(LW * R), grid((nruns + R - 1) / R);
+    int spl = c->cfg_spl > 0 ? c->cfg_spl : c->nt;
+    launches = 0;
+    for (int t0 = 0; t0 < c->nt; t0 += spl) {
+        a.t0 = t0;
+        a.t1 = std::min(c->nt, t0 + spl);
+        hipError_t e = launch_one(c, a, grid, block, lds);
+        if (e != hipSuccess) return fail(c, NGPU_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
+        launches++;
+    }
+    return NGPU_OK;
+}
+
+static int prepare(ngpu_ctx *c)
+{
+    if (!c->have_tree || !c->have_params || !c->have_stim)
+        return fail(c, NGPU_E_STATE, "ngpu_run needs ngpu_set_tree, ngpu_set_params and ngpu_set_stimuli first");
+    if (!c->have_sim) { c->rec.assign(1, 0); c->have_sim = true; }
+    HIPCHK(c, hipSetDevice(c->device));
+    int W, slot;
+    choose_layout(c, W, slot);
+    int rc;
+    if (c->tables_dirty || W != c->lay_W || slot != c->lay_slot) {
+        if ((rc = build_tables(c, W, slot))) return rc;
+        drop_graph(c);
+    }
+    if (c->params_dirty) {
+        if ((rc = upload(c, c->d_pv, c->pv))) return rc;
+        c->params_dirty = false;
+    }
+    if (c->stim_dirty) {
+        if ((rc = upload(c, c->d_dts, c->dts)) || (rc = upload(c, c->d_amps, c->amps))) return rc;
+        c->stim_dirty = false;
+    }
+    const size_t nruns = (size_t)c->nsets * c->nstim, NQ = c->lay_NQ;
+    if ((rc = alloc(c, c->d_states, nruns * mech_nstate(c->mechset) * NQ)) ||
+        (rc = alloc(c, c->d_carry, nruns * (3 * NQ + c->N + 1))) ||
+        (rc = alloc(c, c->d_vhot, nruns * c->rec.size() * c->nt)))
+        return rc;
+    return NGPU_OK;
+}
+
+extern "C" int ngpu_run(ngpu_ctx *c, double *vhot, ngpu_stats *st)
+{
+    if (!c) return fail(c, NGPU_E_ARG, "NULL context");
+    hipEvent_t h0, h1;
+    int rc = prepare(c);
+    if (rc) return rc;
+    HIPCHK(c, hipEventCreate(&h0));
+    HIPCHK(c, hipEventCreate(&h1));
+    HIPCHK(c, hipEventRecord(h0, c->stream));
+    int launches = 0;
+    if (c->cfg_graph) {
+        if (!c->gexec) {
+            hipGraph_t g;
+            HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+            rc = enqueue(c, launches);
+            hipError_t ce = hipStreamEndCapture(c->stream, &g);
+            if (rc) return rc;
+            if (ce != hipSuccess) return fail(c, NGPU_E_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+            hipError_t ie = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (ie != hipSuccess) return fail(c, NGPU_E_HIP, "graph instantiation failed: %s", hipGetErrorString(ie));
+            c->last_launches = launches;
+        }
+        launches = c->last_launches;
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    } else {
+        HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+        if ((rc = enqueue(c, launches))) return rc;
+        HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+    }
+    hipError_t se = hipStreamSynchronize(c->stream);
+    if (se != hipSuccess) return fail(c, NGPU_E_HIP, "simulation failed: %s", hipGetErrorString(se));
+    c->ran = true;
+    if (vhot) {
+        HIPCHK(c, hipMemcpy(vhot, c->d_vhot.p, (size_t)ngpu_vhot_size(c) * sizeof(double), hipMemcpyDeviceToHost));
+    }
+    HIPCHK(c, hipEventRecord(h1, c->stream));
+    HIPCHK(c, hipEventSynchronize(h1));
+    if (st) {
+        float mk = 0, mt = 0;
+        HIPCHK(c, hipEventElapsedTime(&mk, c->ev0, c->ev1));
+        HIPCHK(c, hipEventElapsedTime(&mt, h0, h1));
+        st->ms_kernel = mk;
+        st->ms_total = mt;
+        st->comp_steps = (double)c->nsets * c->nstim * (double)c->N * c->nt;
+        st->comp_steps_per_s = mk > 0 ? st->comp_steps / (mk * 1e-3) : 0.0;
+        st->waves_per_run = c->lay_W;
+        st->slots_per_lane = c->lay_slot;
+        st->runs_per_block = c->lay_W == 1 ? (c->cfg_r > 0 ? c->cfg_r : 4) : 1;
+        st->launches = launches;
+        st->nvar = c->nvar;
+    }
+    (void)hipEventDestroy(h0);
+    (void)hipEventDestroy(h1);
+    return NGPU_OK;
+}
+
+extern "C" int ngpu_fetch_vhot(ngpu_ctx *c, double *vhot)
+{
+    if (!c || !vhot) return fail(c, NGPU_E_ARG, "NULL argument");
+    if (!c->ran) return fail(c, NGPU_E_STATE, "no completed run");
+    HIPCHK(c, hipSetDevice(c->device));
+    HIPCHK(c, hipMemcpy(vhot, c->d_vhot.p, (size_t)ngpu_vhot_size(c) * sizeof(double), hipMemcpyDeviceToHost));
+    return NGPU_OK;
+}
+
+/* ------------------------------------------------------------------ files */
+
+static bool read_lines(const std::string &path, std::vector<std::string> &lines)
+{
+    std::ifstream in(path);
+    if (!in) return false;
+    std::string l;
+    while (std::getline(in, l)) lines.push_back(l);
+    return true;
+}
+
+static std::vector<double> tokens(const std::string &line)
+{
+    std::vector<double> out;
+    std::stringstream ss(line);
+    std::string tok;
+    while (std::getline(ss, tok, ',')) {
+        size_t b = tok.find_first_not_of(" \t\r\n");
+        if (b == std::string::npos) continue;
+        out.push_back(std::strtod(tok.c_str() + b, nullptr));
+    }
+    return out;
+}
+
+struct ParsedTree {
+    ngpu_tree t{};
+    std::vector<double> e, f, cm;
+    std::vector<int> ks, s2c, bm, son, rs, re, rv, ss, se, fa, fx, cbl, cbfl, lrs, lre, flrs, flre;
+};
+
+static int parse_tree(ngpu_ctx *c, const std::string &path, ParsedTree &p)
+{
+    std::vector<std::string> L;
+    if (!read_lines(path, L)) return fail(c, NGPU_E_FILE, "cannot open %s", path.c_str());
+    if (L.size() < 31) return fail(c, NGPU_E_FILE, "%s: expected at least 31 lines, found %zu", path.c_str(), L.size());
+    size_t li = 0;
+    auto dvec = [&](size_t n, std::vector<double> &out, const char *what) -> int {
+        std::vector<double> v = tokens(L[li++]);
+        if (v.size() != n) return fail(c, NGPU_E_FILE, "%s: %s has %zu values, expected %zu", path.c_str(), what, v.size(), n);
+        out = v;
+        return 0;
+    };
+    auto ivec = [&](size_t n, std::vector<int> &out, const char *what) -> int {
+        std::vector<double> v;
+        int r = dvec(n, v, what);
+        if (r) return r;
+        out.resize(n);
+        for (size_t i = 0; i < n; i++) out[i] = (int)v[i];   /* atoi truncation, as the reference */
+        return 0;
+    };
+    auto ione = [&](int &x, const char *what) -> int {
+        std::vector<int> v;
+        int r = ivec(1, v, what);
+        if (!r) x = v[0];
+        return r;
+    };
+    int N, NC, NM, depth, logn, nf, ncall, cd, cfd, nl, nfl, rc;
+    if ((rc = ione(N, "N")) || (rc = ione(NC, "NComps"))) return rc;
+    if (N < 1 || NC < 1) return fail(c, NGPU_E_FILE, "%s: N and NComps must be positive", path.c_str());
+    if ((rc = dvec(N, p.e, "e")) || (rc = dvec(N, p.f, "f")) || (rc = ivec(N, p.ks, "Ks")) || (rc = ivec(N, p.s2c, "SegToComp")) ||
+        (rc = dvec(N, p.cm, "Cms")) || (rc = ione(NM, "NModels")) || (rc = ivec((size_t)NM * N, p.bm, "boolModel")) ||
+        (rc = ivec(N, p.son, "SonNoVec")) || (rc = ione(depth, "Depth")) || (rc = ione(logn, "LognDepth")) ||
+        (rc = ione(nf, "nFathers")) || (rc = ione(ncall, "nCallForFather")) || (rc = ivec(nf, p.rs, "RelStarts")) ||
+        (rc = ivec(nf, p.re, "RelEnds")) || (rc = ivec(ncall, p.rv, "RelVec")) || (rc = ivec(ncall + 1, p.ss, "SegStartI")) ||
+        (rc = ivec(ncall + 1, p.se, "SegEndI")) || (rc = ivec(nf, p.fa, "Fathers")) || (rc = ivec((size_t)logn * N, p.fx, "FIdxs")) ||
+        (rc = ione(cd, "CompDepth")) || (rc = ione(cfd, "CompFDepth")) || (rc = ivec((size_t)cd * 32, p.cbl, "CompByLevel32")) ||
+        (rc = ivec((size_t)cfd * 32, p.cbfl, "CompByFLevel32")) || (rc = ione(nl, "nLRel")) || (rc = ivec(nl, p.lrs, "LRelStarts")) ||
+        (rc = ivec(nl, p.lre, "LRelEnds")) || (rc = ione(nfl, "nFLRel")) || (rc = ivec(nfl, p.flrs, "FLRelStarts")) ||
+        (rc = ivec(nfl, p.flre, "FLRelEnds")))
+        return rc;
+    ngpu_tree &t = p.t;
+    t.N = N; t.ncomps = NC; t.nmodels = NM; t.depth = depth; t.logn_depth = logn; t.n_fathers = nf; t.n_call = ncall;
+    t.e = p.e.data(); t.f = p.f.data(); t.cm = p.cm.data(); t.ks = p.ks.data(); t.seg_to_comp = p.s2c.data();
+    t.bool_model = p.bm.data(); t.son_no = p.son.data(); t.rel_starts = p.rs.data(); t.rel_ends = p.re.data();
+    t.rel_vec = p.rv.data(); t.seg_start = p.ss.data(); t.seg_end = p.se.data(); t.fathers = p.fa.data(); t.fidxs = p.fx.data();
+    t.comp_depth = cd; t.comp_fdepth = cfd; t.comp_by_level32 = p.cbl.data(); t.comp_by_flevel32 = p.cbfl.data();
+    t.n_lrel = nl; t.n_flrel = nfl; t.lrel_starts = p.lrs.data(); t.lrel_ends = p.lre.data();
+    t.flrel_starts = p.flrs.data(); t.flrel_ends = p.flre.data();
+    return NGPU_OK;
+}
+
+extern "C" int ngpu_open(const char *dir, const char *suffix, int mechset, int device, ngpu_ctx **out)
+{
+    if (!dir || !out) return fail(nullptr, NGPU_E_ARG, "ngpu_open: NULL argument");
+    *out = nullptr;
+    std::string d(dir), sfx(suffix ? suffix : "");
+    ngpu_ctx *c = nullptr;
+    int rc = ngpu_create(device, &c);
+    if (rc) return rc;
+    auto bail = [&](int code) { g_last_error = c->err; ngpu_close(c); return code; };
+    ParsedTree pt;
+    if ((rc = parse_tree(c, d + "/BasicConstSegP.csv", pt))) return bail(rc);
+    if (mechset < 0) {
+        int nm = pt.t.nmodels;
+        mechset = nm == SetTTPC::NMECH ? NGPU_MECH_TTPC : nm == SetMainen::NMECH ? NGPU_MECH_MAINEN : nm == 1 ? NGPU_MECH_PASSIVE : -1;
+        if (mechset < 0) { fail(c, NGPU_E_FILE, "cannot infer a mechanism set from NModels=%d", nm); return bail(NGPU_E_FILE); }
+    }
+    pt.t.mechset = mechset;
+    if ((rc = ngpu_set_tree(c, &pt.t))) return bail(rc);
+    /* Sim.csv */
+    std::vector<std::string> L;
+    if (!read_lines(d + "/Sim.csv", L) || L.size() < 3) { fail(c, NGPU_E_FILE, "cannot read %s/Sim.csv", dir); return bail(NGPU_E_FILE); }
+    std::vector<double> v0 = tokens(L[0]), nr = tokens(L[2]);
+    if (v0.size() != 1 || nr.size() != 1) { fail(c, NGPU_E_FILE, "Sim.csv malformed"); return bail(NGPU_E_FILE); }
+    std::vector<int> rec;
+    if ((int)nr[0] > 0) {
+        std::vector<double> rs = L.size() > 3 ? tokens(L[3]) : std::vector<double>();
+        if ((int)rs.size() != (int)nr[0]) { fail(c, NGPU_E_FILE, "Sim.csv: %d record sites declared, %zu given", (int)nr[0], rs.size()); return bail(NGPU_E_FILE); }
+        for (double r : rs) rec.push_back((int)r);
+    }
+    if ((rc = ngpu_set_sim(c, v0[0], (int)rec.size(), rec.data()))) return bail(rc);
+    /* AllParams.csv */
+    L.clear();
+    if (!read_lines(d + "/AllParams.csv", L) || L.empty()) { fail(c, NGPU_E_FILE, "cannot read %s/AllParams.csv", dir); return bail(NGPU_E_FILE); }
+    std::vector<double> hdr = tokens(L[0]);
+    int nsets = hdr.size() == 1 ? (int)hdr[0] : -1;
+    int np = mech_nparams(mechset);
+    if (nsets < 1 || (int)L.size() - 1 < nsets) { fail(c, NGPU_E_FILE, "AllParams.csv: NSets=%d but %zu rows", nsets, L.size() - 1); return bail(NGPU_E_FILE); }
+    std::vector<double> P;
+    P.reserve((size_t)nsets * np * pt.t.ncomps);
+    for (int s = 0; s < nsets; s++) {
+        std::vector<double> row = tokens(L[1 + s]);
+        if ((int)row.size() != np * pt.t.ncomps) { fail(c, NGPU_E_FILE, "AllParams.csv row %d has %zu values, expected %d", s, row.size(), np * pt.t.ncomps); return bail(NGPU_E_FILE); }
+        P.insert(P.end(), row.begin(), row.end());
+    }
+    if ((rc = ngpu_set_params(c, P.data(), nsets, pt.t.ncomps, np))) return bail(rc);
+    /* Stim_meta.csv, Stim_raw<sfx>.csv, times<sfx>.csv */
+    L.clear();
+    if (!read_lines(d + "/Stim_meta.csv", L) || L.size() < 5) { fail(c, NGPU_E_FILE, "cannot read %s/Stim_meta.csv", dir); return bail(NGPU_E_FILE); }
+    int nstim = (int)tokens(L[0]).at(0), nt = (int)tokens(L[1]).at(0), loc = (int)tokens(L[3]).at(0);
+    double area = tokens(L[4]).at(0);
+    std::vector<std::string> R;
+    if (!read_lines(d + "/Stim_raw" + sfx + ".csv", R) || (int)R.size() < nstim) { fail(c, NGPU_E_FILE, "cannot read Stim_raw%s.csv", sfx.c_str()); return bail(NGPU_E_FILE); }
+    std::vector<double> amps((size_t)nstim * nt);
+    for (int i = 0; i < nstim; i++) {
+        std::vector<double> row = tokens(R[i]);
+        if ((int)row.size() < nt) { fail(c, NGPU_E_FILE, "Stim_raw row %d has %zu values, expected >= %d", i, row.size(), nt); return bail(NGPU_E_FILE); }
+        std::copy(row.begin(), row.begin() + nt, amps.begin() + (size_t)i * nt);
+    }
+    std::vector<std::string> T;
+    if (!read_lines(d + "/times" + sfx + ".csv", T) && !read_lines(d + "/times.csv", T)) { fail(c, NGPU_E_FILE, "cannot read times.csv"); return bail(NGPU_E_FILE); }
+    std::vector<double> dts = T.empty() ? std::vector<double>() : tokens(T[0]);
+    if ((int)dts.size() < nt) { fail(c, NGPU_E_FILE, "times.csv has %zu values, expected >= %d", dts.size(), nt); return bail(NGPU_E_FILE); }
+    dts.resize(nt);
+    if ((rc = ngpu_set_stimuli(c, amps.data(), nstim, nt, dts.data(), nt, loc - 1, area))) return bail(rc);
+    *out = c;
+    return NGPU_OK;
+}
+
+extern "C" int ngpu_write_vhot(ngpu_ctx *c, const char *path)
+{
+    if (!c || !path) return fail(c, NGPU_E_ARG, "NULL argument");
+    if (!c->ran) return fail(c, NGPU_E_STATE, "no completed run");
+    long long n = ngpu_vhot_size(c);
+    if (n > 0x7fffffffLL) return fail(c, NGPU_E_ARG, "%lld values exceed the VHotP int32 header", n);
+    std::vector<double> h((size_t)n);
+    int rc = ngpu_fetch_vhot(c, h.data());
+    if (rc) return rc;
+    FILE *fp = std::fopen(path, "wb");
+    if (!fp) return fail(c, NGPU_E_FILE, "cannot write %s", path);
+    int n32 = (int)n, prec = 3;
+    bool ok = std::fwrite(&n32, sizeof n32, 1, fp) == 1 && std::fwrite(&prec, sizeof prec, 1, fp) == 1 &&
+              std::fwrite(h.data(), sizeof(double), h.size(), fp) == h.size();
+    ok = (std::fclose(fp) == 0) && ok;
+    return ok ? NGPU_OK : fail(c, NGPU_E_FILE, "short write to %s", path);
+}

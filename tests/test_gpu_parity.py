@@ -1,0 +1,138 @@
+"""GPU parity: libneurogpu's HIP kernels against the FP64 oracle on the same
+inputs.  The bar is max-abs trace difference <= 1e-6 mV and identical spike
+counts; with the shared transcendental functions and -ffp-contract=off on both
+sides the traces are expected to be bit-identical."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import TOL_MV, TTPC_DIR, spike_counts
+from neurogpu_amd import formats, models
+from neurogpu_amd._lib import NgpuError, Simulator, simulate_workload
+from oracle import cpu_ref
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _oracle(w, sets=None):
+    m = w.model
+    x = w.x if sets is None else w.x[sets]
+    return cpu_ref.simulate(m.tree, m.mechset_id, m.allparams(x), w.amps, w.dts, m.stim_seg, m.stim_area,
+                            m.v_init, m.rec_sites)
+
+
+def _check(gpu, ref, what):
+    d = float(np.max(np.abs(gpu - ref)))
+    _say(f"{what}: max|gpu-oracle| = {d:.3e} mV")
+    assert np.all(np.isfinite(gpu))
+    assert d <= TOL_MV, what
+    np.testing.assert_array_equal(spike_counts(gpu), spike_counts(ref))
+    return d
+
+
+def test_passive_config_full_size():
+    w = models.config(0)
+    _say(f"case passive64 full: {w.nruns} runs x {w.dts.size} steps")
+    gpu, st = simulate_workload(w)
+    _check(gpu, _oracle(w), "passive64")
+
+
+@pytest.mark.parametrize("launch", [None, dict(waves_per_run=2), dict(runs_per_block=1),
+                                    dict(steps_per_launch=700), dict(steps_per_launch=500, use_graph=True)])
+def test_active_config_small(launch):
+    w = models.config(1, nsets=6, nt=1500)
+    _say(f"case active500 small launch={launch}")
+    gpu, st = simulate_workload(w, launch=launch)
+    _check(gpu, _oracle(w), f"active500 {launch}")
+
+
+@pytest.mark.parametrize("launch", [None, dict(waves_per_run=4), dict(waves_per_run=1, runs_per_block=2)])
+def test_ttpc_config_small(launch):
+    w = models.config(2, nsets=3, nt=1200)
+    _say(f"case ttpc1200 small launch={launch}")
+    gpu, st = simulate_workload(w, launch=launch)
+    _check(gpu, _oracle(w), f"ttpc1200 {launch}")
+
+
+def test_reference_ttpc_fixture_through_file_boundary(tmp_path):
+    """The reference's own data directory, parsed by ngpu_open (C++), run, and
+    written as VHotP.dat; compared with the oracle run on the Python parse."""
+    _say("case ttpc512 fixture via ngpu_open")
+    sim = Simulator.open(TTPC_DIR)
+    out = tmp_path / "VHotP0.dat"
+    sim.run(fetch=False)
+    sim.write_vhot(str(out))
+    sim.close()
+    t = formats.read_tree(os.path.join(TTPC_DIR, "BasicConstSegP.csv"))
+    P = formats.read_allparams(os.path.join(TTPC_DIR, "AllParams.csv"), t.ncomps, 19)
+    sd = formats.read_sim(os.path.join(TTPC_DIR, "Sim.csv"))
+    st = formats.read_stim(TTPC_DIR)
+    ref = cpu_ref.simulate(t, 1, P, st.amps, st.dts, st.stim_seg, st.area, sd.v_init, sd.rec_sites)
+    gpu = formats.read_vhot(str(out)).reshape(ref.shape)
+    _check(gpu, ref, "ttpc512 fixture")
+    assert np.all(np.abs(gpu) < 1000)          # Code/test_output.py bound
+
+
+def test_active_full_length_sampled():
+    """Configuration 1 at full length (10,000 steps), 16 parameter sets, all
+    three stimuli, every run checked."""
+    w = models.config(1, nsets=16)
+    _say("case active500 full length, 16 sets")
+    gpu, st = simulate_workload(w)
+    _check(gpu, _oracle(w), "active500 full length")
+
+
+def test_determinism_and_identical_instances():
+    w = models.config(1, nsets=8, nt=800)
+    w.x[4:] = w.x[:4]
+    _say("case determinism")
+    a, _ = simulate_workload(w)
+    b, _ = simulate_workload(w)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(a[:4], a[4:])
+
+
+def test_error_paths():
+    w = models.config(1, nsets=2, nt=10)
+    m = w.model
+    with Simulator(0) as sim:
+        with pytest.raises(NgpuError):
+            sim.run()                                   # nothing set yet
+        sim.set_tree(m.tree, m.mechset_id)
+        with pytest.raises(NgpuError):
+            sim.set_params(w.params, m.ncomps + 1, m.nparams)
+        with pytest.raises(NgpuError):
+            sim.set_params(w.params, m.ncomps, m.nparams - 1)
+        bad = w.dts.copy()
+        bad[3] = 0.0
+        with pytest.raises(NgpuError):
+            sim.set_stimuli(w.amps, bad, m.stim_seg, m.stim_area)
+        with pytest.raises(NgpuError):
+            sim.set_stimuli(w.amps, w.dts, m.tree.N, m.stim_area)
+        with pytest.raises(NgpuError):
+            sim.set_sim(m.v_init, [m.tree.N])
+        with pytest.raises(NgpuError):
+            sim.set_launch(waves_per_run=9)
+
+
+def test_nan_parameter_poisons_only_its_run():
+    w = models.config(1, nsets=3, nt=300)
+    P = w.params
+    P[1, :] = np.nan
+    m = w.model
+    with Simulator(0) as sim:
+        sim.set_tree(m.tree, m.mechset_id)
+        sim.set_sim(m.v_init, m.rec_sites)
+        sim.set_params(P, m.ncomps, m.nparams)
+        sim.set_stimuli(w.amps, w.dts, m.stim_seg, m.stim_area)
+        out, _ = sim.run()
+    out = out.reshape(3, w.amps.shape[0], 1, w.dts.size)
+    assert np.all(np.isfinite(out[0])) and np.all(np.isfinite(out[2]))
+    assert not np.all(np.isfinite(out[1]))

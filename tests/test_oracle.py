@@ -1,0 +1,226 @@
+"""Pins for the FP64 CPU oracle (oracle/cpu_ref.c): dense-solve equivalence,
+closed forms, invariants, and the shared transcendental functions."""
+import ctypes as C
+import math
+
+import numpy as np
+import pytest
+
+from conftest import TREE_FILES, TTPC_DIR, spike_counts
+from neurogpu_amd import formats, models, tree
+from oracle import cpu_ref
+
+
+def _matrix(t, u):
+    """Tree matrix in Hines-row order: A[h][h] = u[h]; child c of parent p:
+    A[c][p] = f[c], A[p][c] = e[c]."""
+    A = np.diag(np.asarray(u, dtype=float))
+    for h in range(t.N - 1):
+        p = t.ks[h + 1] - 1
+        A[h, p] = t.f[h]
+        A[p, h] = t.e[h]
+    return A
+
+
+@pytest.mark.parametrize("name", ["ttpc512", "mainen384", "cc224", "passive384"])
+def test_solve_has_tiny_backward_error(name):
+    """The reference trees mix zero-area and large nodes, so the matrices are
+    badly scaled and a dense LU is no sharper a reference than the tree solve.
+    Pin every mode by its componentwise backward error instead:
+    |A x - b|_i <= 1e-13 (sum_j |A_ij x_j| + |b_i|)."""
+    t = formats.read_tree(TREE_FILES[name])
+    rng = np.random.default_rng(3)
+    A0 = _matrix(t, np.zeros(t.N))
+    u = np.abs(A0).sum(axis=0) + np.abs(A0).sum(axis=1) + rng.uniform(0.01, 1.0, t.N)
+    A = _matrix(t, u)
+    b = rng.normal(size=t.N)
+    for mode in (0, 1, 2):
+        x = cpu_ref.solve(t, u, b, mode)
+        r = np.abs(A @ x - b)
+        scale = np.abs(A) @ np.abs(x) + np.abs(b)
+        assert np.max(r / scale) < 1e-13, mode
+
+
+@pytest.mark.parametrize("idx", [0, 1, 2])
+def test_solve_matches_dense_solve_on_generated_trees(idx):
+    t = models.config(idx, nsets=1, nt=1).model.tree
+    rng = np.random.default_rng(idx)
+    u = rng.uniform(1.0, 2.0, t.N)
+    A0 = _matrix(t, np.zeros(t.N))
+    u = u + np.abs(A0).sum(axis=1)
+    A = _matrix(t, u)
+    b = rng.normal(size=t.N)
+    x_ref = np.linalg.solve(A, b)
+    for mode in (0, 1, 2):
+        np.testing.assert_allclose(cpu_ref.solve(t, u, b, mode), x_ref, rtol=1e-11, atol=1e-12)
+
+
+def test_beforelu_equals_serial_elimination_bitwise():
+    """The level schedule only reorders independent eliminations; with the
+    father folds in ascending row order it is bit-identical to eliminating
+    rows 0..N-2 into their parents in order."""
+    t = formats.read_tree(TREE_FILES["ttpc512"])
+    rng = np.random.default_rng(5)
+    u = rng.uniform(1.0, 3.0, t.N) + 2.0
+    b = rng.normal(size=t.N)
+    np.testing.assert_array_equal(cpu_ref.solve(t, u, b, 0), cpu_ref.solve(t, u, b, 1))
+
+
+def test_single_compartment_backward_euler_closed_form():
+    """N = 1 passive compartment: V_{n+1} = V_n + (I - g (V_n - E)) / (C/dt + g)."""
+    par = np.array([-1])
+    t = tree.build_tree(par, [0.0], [0.0], [1.0], [0], np.ones((1, 1)), 1)
+    g, E, dt, nt = 1e-3, -70.0, 0.025, 200
+    area = 1000.0
+    amps = np.zeros((1, nt))
+    amps[0, 20:] = 0.1
+    vh = cpu_ref.simulate(t, cpu_ref.MS_PASSIVE, np.array([[g, E]]), amps, np.full(nt, dt), 0, area, -65.0, [0])
+    V = -65.0
+    for n in range(nt):
+        assert abs(vh[0, 0, 0, n] - V) <= 1e-12 * abs(V)
+        I = 100 * amps[0, n] / area
+        V = V + (I - g * (V - E)) / (1.0 / (dt * 1000) + g)
+
+
+def test_rest_is_a_fixed_point_bitwise():
+    w = models.config(0, nsets=3, nt=400)
+    m = w.model
+    P = m.allparams(w.x)
+    vh = cpu_ref.simulate(m.tree, m.mechset_id, P, np.zeros_like(w.amps), w.dts, m.stim_seg, m.stim_area,
+                          -70.0, m.rec_sites)
+    assert np.all(vh == -70.0)
+
+
+def test_identical_instances_identical_traces():
+    w = models.config(1, nsets=2, nt=600)
+    m = w.model
+    x = np.vstack([w.x[0], w.x[0]])
+    vh = cpu_ref.simulate(m.tree, m.mechset_id, m.allparams(x), w.amps, w.dts, m.stim_seg, m.stim_area,
+                          m.v_init, m.rec_sites)
+    np.testing.assert_array_equal(vh[0], vh[1])
+
+
+def test_passive_linearity():
+    """A passive cell at rest responds linearly: trace(2I) - rest = 2 (trace(I) - rest)."""
+    w = models.config(0, nsets=2, nt=800)
+    m = w.model
+    P = m.allparams(w.x)
+    a1 = w.amps
+    v1 = cpu_ref.simulate(m.tree, 0, P, a1, w.dts, m.stim_seg, m.stim_area, -70.0, m.rec_sites)
+    v2 = cpu_ref.simulate(m.tree, 0, P, 2 * a1, w.dts, m.stim_seg, m.stim_area, -70.0, m.rec_sites)
+    np.testing.assert_allclose(v2 + 70.0, 2 * (v1 + 70.0), atol=1e-9)
+
+
+def test_dense_step_equivalence_passive():
+    """One step of the whole passive assembly equals a dense backward-Euler
+    step of the cable system (NEURON discretisation)."""
+    w = models.config(0, nsets=1, nt=2)
+    m = w.model
+    t = m.tree
+    g, E = w.x[0, 0], -70.0
+    v0 = -70.0
+    dt = w.dts[0]
+    amps = np.full((1, 2), 0.5)
+    vh, vfin = cpu_ref.simulate(t, 0, m.allparams(w.x), amps, w.dts[:1], m.stim_seg, m.stim_area, v0,
+                                m.rec_sites, want_vfinal=True)
+    N = t.N
+    par = t.parent_segments()
+    A = np.zeros((N, N))
+    rhs = np.zeros(N)
+    for s in range(N):
+        A[s, s] += t.cm[s] / (dt * 1000) + g
+        rhs[s] += -g * (v0 - E)
+    rhs[m.stim_seg] += 100 * 0.5 / m.stim_area
+    for s in range(1, N):
+        p, h = par[s], N - 1 - s
+        A[s, s] -= t.f[h]
+        A[s, p] += t.f[h]
+        A[p, p] -= t.e[h]
+        A[p, s] += t.e[h]
+    dv = np.linalg.solve(A, rhs)
+    np.testing.assert_allclose(vfin[0], v0 + dv, atol=1e-10)
+
+
+@pytest.mark.parametrize("ms,mech,points", [
+    (1, 0, [-27.0]), (1, 3, [-154.9]), (1, 7, [-38.0, -17.0, -64.4]), (1, 8, [-38.0, -66.0]),
+    (1, 9, [-32.0, -60.0]),
+])
+def test_singularity_bumps_are_continuous(ms, mech, points):
+    """At the removable singularities the rate code shifts v by 1e-4 mV; the
+    result must sit between the values on either side."""
+    for v in points:
+        mid = cpu_ref.rates(ms, mech, v)
+        lo = cpu_ref.rates(ms, mech, v - 2e-4)
+        hi = cpu_ref.rates(ms, mech, v + 2e-4)
+        assert np.all(np.isfinite(mid))
+        for k in range(4):
+            a, b = sorted((lo[k], hi[k]))
+            assert a - 1e-9 * abs(a) <= mid[k] <= b + 1e-9 * abs(b)
+
+
+@pytest.mark.parametrize("ms,mech", [(1, m) for m in (0, 1, 3, 4, 5, 6, 7, 8, 9, 12)] +
+                         [(2, 0), (2, 3), (2, 4), (2, 5)])
+def test_rates_are_gates(ms, mech):
+    P = np.zeros(40)
+    if ms == 2:
+        kin = models._MAINEN_KIN
+        names = models.MECHSETS["mainen"]["params"]
+        for i, n in enumerate(names):
+            P[i] = kin.get(n, 0.0)
+    for v in np.linspace(-100, 50, 61):
+        r = cpu_ref.rates(ms, mech, v, P)
+        assert 0.0 <= r[0] <= 1.0 and 0.0 <= r[2] <= 1.0
+        assert r[1] > 0 or (r[1] == 0 and r[0] == 0)
+        if r[3] != 0:
+            assert r[3] > 0
+
+
+def test_cnexp_exact_contraction():
+    """cnexp is the exact solution at fixed v: y1 - yinf = (y0 - yinf) exp(-dt/tau)."""
+    for y0, yinf, tau, dt in [(0.1, 0.7, 3.0, 0.025), (0.9, 0.05, 0.3, 0.1), (0.5, 0.5, 10.0, 0.02)]:
+        y1 = cpu_ref.cnexp(y0, yinf, tau, dt)
+        assert abs((y1 - yinf) - (y0 - yinf) * math.exp(-dt / tau)) < 1e-15
+
+
+def test_nernst_constant():
+    """ktf/2 ln(cao/cai) at DEF_cai reproduces the reference's def_eca2 constant
+    140.2187 mV at 34 C (Code/OneKernel/src/AllModels.cu:36)."""
+    assert abs(cpu_ref.ktf(1) / 2 * math.log(2.0 / 5e-5) - 140.218711995) < 1e-4
+
+
+def test_ttpc_fixture_runs_and_stays_in_bounds():
+    """Code/test_output.py: every VHot value lies in (-1000, 1000)."""
+    t = formats.read_tree(TREE_FILES["ttpc512"])
+    P = formats.read_allparams(TTPC_DIR + "/AllParams.csv", t.ncomps, 19)
+    sim = formats.read_sim(TTPC_DIR + "/Sim.csv")
+    st = formats.read_stim(TTPC_DIR)
+    vh = cpu_ref.simulate(t, 1, P[:4], st.amps, st.dts, st.stim_seg, st.area, sim.v_init, sim.rec_sites)
+    assert np.all(np.abs(vh) < 1000) and np.all(np.isfinite(vh))
+    assert np.all(spike_counts(vh) > 0)
+
+
+def _mathf():
+    L = cpu_ref.lib()
+    L.ref_math.restype = C.c_double
+    L.ref_math.argtypes = [C.c_int, C.c_double, C.c_double]
+    return L.ref_math
+
+
+def test_shared_exp_log_within_one_ulp_of_glibc():
+    f = _mathf()
+    rng = np.random.default_rng(0)
+    xs = np.concatenate([rng.uniform(-50, 50, 3000), rng.uniform(-740, 705, 3000)])
+    worst = max(abs(f(0, x, 0) - math.exp(x)) / math.ulp(math.exp(x)) for x in xs)
+    assert worst <= 1.0
+    ys = np.concatenate([10 ** rng.uniform(-300, 300, 3000), rng.uniform(0.5, 2, 3000)])
+    worst = max(abs(f(1, y, 0) - math.log(y)) / math.ulp(math.log(y)) for y in ys)
+    assert worst <= 1.0
+    assert f(1, 0.0, 0) == -math.inf and f(1, math.inf, 0) == math.inf and f(0, 800.0, 0) == math.inf
+    assert f(0, -800.0, 0) == 0.0 and math.isnan(f(0, math.nan, 0))
+
+
+def test_shared_pow_relative_error():
+    f = _mathf()
+    for x in 10 ** np.linspace(-3, 4, 400):
+        assert abs(f(2, x, 4.8) / x ** 4.8 - 1) < 2e-14
+    assert f(2, 3.0, 2.0) == 9.0 and f(2, 0.7, 4.0) == (0.7 * 0.7) * (0.7 * 0.7)
