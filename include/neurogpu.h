@@ -55,6 +55,7 @@ typedef struct {
     double comp_steps_per_s;  /* comp_steps / (ms_kernel / 1000)                      */
     int waves_per_run, slots_per_lane, runs_per_block, launches;
     int nvar;                 /* distinct run-varying parameter columns               */
+    int nsets, nstim, nrec, nt, N;
 } ngpu_stats;
 
 /* Create a context on HIP device `device` (replaces cudaSetDevice in CANON/kernel.cu:24-31). */

@@ -51,7 +51,8 @@ class ngpu_tree(C.Structure):
 class ngpu_stats(C.Structure):
     _fields_ = [("ms_total", C.c_double), ("ms_kernel", C.c_double), ("comp_steps", C.c_double),
                 ("comp_steps_per_s", C.c_double), ("waves_per_run", C.c_int), ("slots_per_lane", C.c_int),
-                ("runs_per_block", C.c_int), ("launches", C.c_int), ("nvar", C.c_int)]
+                ("runs_per_block", C.c_int), ("launches", C.c_int), ("nvar", C.c_int),
+                ("nsets", C.c_int), ("nstim", C.c_int), ("nrec", C.c_int), ("nt", C.c_int), ("N", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -555,6 +555,7 @@ extern "C" int ngpu_run(ngpu_ctx *c, double *vhot, ngpu_stats *st)
         st->runs_per_block = c->lay_W == 1 ? (c->cfg_r > 0 ? c->cfg_r : 4) : 1;
         st->launches = launches;
         st->nvar = c->nvar;
+        st->nsets = c->nsets; st->nstim = c->nstim; st->nrec = (int)c->rec.size(); st->nt = c->nt; st->N = c->N;
     }
     (void)hipEventDestroy(h0);
     (void)hipEventDestroy(h1);
