@@ -15,7 +15,7 @@ import numpy as np
 from .formats import TreeData
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "libneurogpu.so")
+LIB_PATH = os.environ.get("NGPU_LIB") or os.path.join(_HERE, "_build", "libneurogpu.so")
 
 _IP = C.POINTER(C.c_int)
 _DP = C.POINTER(C.c_double)

@@ -36,6 +36,19 @@ static std::string g_last_error;
 
 static int mech_nmodels(int ms) { return ms == NGPU_MECH_TTPC ? SetTTPC::NMECH : ms == NGPU_MECH_MAINEN ? SetMainen::NMECH : ms == NGPU_MECH_PASSIVE ? SetPassive::NMECH : -1; }
 static int mech_nparams(int ms) { return ms == NGPU_MECH_TTPC ? SetTTPC::NPARAM : ms == NGPU_MECH_MAINEN ? SetMainen::NPARAM : ms == NGPU_MECH_PASSIVE ? SetPassive::NPARAM : -1; }
+/* First parameter index of each mechanism of a set, plus the total (the
+ * parameter layout of the generated AllModels.cuh). */
+static const int kPBaseTTPC[14] = {0, 1, 2, 6, 8, 9, 10, 11, 12, 13, 14, 16, 18, 19};
+static const int kPBaseMainen[8] = {0, 2, 2, 6, 11, 16, 28, 30};
+static const int kPBasePassive[2] = {0, 2};
+static int param_owner(int ms, int j)
+{
+    const int *b = ms == NGPU_MECH_TTPC ? kPBaseTTPC : ms == NGPU_MECH_MAINEN ? kPBaseMainen : kPBasePassive;
+    int nm = mech_nmodels(ms);
+    for (int m = 0; m < nm; m++)
+        if (j >= b[m] && j < b[m + 1]) return m;
+    return -1;
+}
 static int mech_nstate(int ms) { return ms == NGPU_MECH_TTPC ? SetTTPC::NSTATE : ms == NGPU_MECH_MAINEN ? SetMainen::NSTATE : SetPassive::NSTATE; }
 
 template <class T> struct DevArr {
@@ -75,13 +88,16 @@ struct ngpu_ctx {
     /* device state */
     int lay_W = 0, lay_slot = 0, lay_NQ = 0;
     bool tables_dirty = true, params_dirty = true, stim_dirty = true, graph_dirty = true;
-    DevArr<int> d_qh, d_qph, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_fidx, d_pvar, d_rec;
+    DevArr<int> d_qh, d_qph, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_fidx, d_rec;
     DevArr<double> d_qcm, d_e, d_f, d_pval, d_pv, d_dts, d_amps, d_states, d_carry, d_vhot;
     int stim_q = -1;
     std::vector<int> rec_q;
+    unsigned pconst_mask = 0;
+    std::vector<double> pconst;
     hipGraphExec_t gexec = nullptr;
     bool ran = false;
     int last_launches = 0;
+    int last_R = 1;
 };
 
 static int fail(ngpu_ctx *c, int code, const char *fmt, ...)
@@ -162,7 +178,7 @@ extern "C" void ngpu_close(ngpu_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
-    for (auto *d : {&c->d_qh, &c->d_qph, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_fidx, &c->d_pvar, &c->d_rec}) d->release();
+    for (auto *d : {&c->d_qh, &c->d_qph, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_fidx, &c->d_rec}) d->release();
     for (auto *d : {&c->d_qcm, &c->d_e, &c->d_f, &c->d_pval, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_carry, &c->d_vhot}) d->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -406,13 +422,37 @@ static int build_tables(ngpu_ctx *c, int W, int slot)
         qcm[q] = c->cm[s];
     }
     const int np = c->nparams;
-    std::vector<int> pvar((size_t)np * NQ, -1);
+    /* A parameter is a kernel constant when every segment that uses it holds
+     * the same uniform value; otherwise the table holds the value, or a tagged
+     * NaN naming the run-varying column. */
+    c->pconst_mask = 0;
+    c->pconst.assign(32, 0.0);
+    for (int j = 0; j < np; j++) {
+        int owner = param_owner(c->mechset, j);
+        bool uni = true, any = false;
+        double val = 0.0;
+        for (int s = 0; s < N && uni; s++) {
+            if (owner < 0 || !(c->mask[s] & (1u << owner))) continue;
+            int ref = c->pref[(size_t)j * N + s];
+            if (ref >= 0) { uni = false; break; }
+            double x = c->puni[-ref - 1];
+            if (!any) { val = x; any = true; }
+            else if (std::memcmp(&x, &val, sizeof x) != 0) uni = false;
+        }
+        if (uni) { c->pconst_mask |= 1u << j; c->pconst[j] = val; }
+    }
     std::vector<double> pval((size_t)np * NQ, 0.0);
     for (int j = 0; j < np; j++)
         for (int q = 0; q < N; q++) {
             int ref = c->pref[(size_t)j * N + order[q]];
-            if (ref >= 0) pvar[(size_t)j * NQ + q] = ref;
-            else pval[(size_t)j * NQ + q] = c->puni[-ref - 1];
+            double x;
+            if (ref >= 0) {
+                unsigned long long bits = ((unsigned long long)NG_PTAG << 32) | (unsigned)ref;
+                std::memcpy(&x, &bits, sizeof x);
+            } else {
+                x = c->puni[-ref - 1];
+            }
+            pval[(size_t)j * NQ + q] = x;
         }
     c->rec_q.resize(c->rec.size());
     for (size_t r = 0; r < c->rec.size(); r++) c->rec_q[r] = pos[c->rec[r]];
@@ -421,7 +461,7 @@ static int build_tables(ngpu_ctx *c, int W, int slot)
     int rc;
     if ((rc = upload(c, c->d_qh, qh)) || (rc = upload(c, c->d_qph, qph)) || (rc = upload(c, c->d_qmeta, qmeta)) ||
         (rc = upload(c, c->d_qcm, qcm)) || (rc = upload(c, c->d_e, re)) || (rc = upload(c, c->d_f, rf)) ||
-        (rc = upload(c, c->d_pvar, pvar)) || (rc = upload(c, c->d_pval, pval)) || (rc = upload(c, c->d_chains, c->chains)) ||
+        (rc = upload(c, c->d_pval, pval)) || (rc = upload(c, c->d_chains, c->chains)) ||
         (rc = upload(c, c->d_lvl, c->lvl_ptr)) || (rc = upload(c, c->d_fath, c->fath)) || (rc = upload(c, c->d_flvl, c->flvl_ptr)) ||
         (rc = upload(c, c->d_fchild, c->fchild)) || (rc = upload(c, c->d_fidx, c->fidx)) || (rc = upload(c, c->d_rec, c->rec_q)))
         return rc;
@@ -445,21 +485,26 @@ static int enqueue(ngpu_ctx *c, int &launches)
     const int N = c->N, NQ = c->lay_NQ, LW = 64 * c->lay_W;
     const int nruns = c->nsets * c->nstim;
     int R = 1;
-    if (c->lay_W == 1) R = c->cfg_r > 0 ? c->cfg_r : 4;
-    const size_t per_run = (size_t)(2 * (N + 1) + 3 * NQ + (int)c->rec.size() * 64) * sizeof(double);
-    while (R > 1 && per_run * R > 160 * 1024) R--;
-    const size_t lds = per_run * R;
-    if (lds > 160 * 1024) return fail(c, NGPU_E_ARG, "tree needs %zu bytes of LDS per run (limit 160 KiB)", lds);
+    if (c->lay_W == 1) R = c->cfg_r > 0 ? c->cfg_r : 1;
+    const int npv = c->nvar <= 512 ? std::max(c->nvar, 1) : 0;
+    const size_t per_run = (size_t)(2 * (N + 1) + npv + (int)c->rec.size() * 64) * sizeof(double);
+    const size_t shared = (size_t)2 * N * sizeof(double);
+    while (R > 1 && shared + per_run * R > 160 * 1024) R--;
+    const size_t lds = shared + per_run * R;
+    if (lds > 160 * 1024) return fail(c, NGPU_E_ARG, "tree needs %zu bytes of LDS per block (limit 160 KiB)", lds);
     SimArgs a{};
     a.N = N; a.NQ = NQ; a.LW = LW; a.runs_per_block = R;
     a.nvar = std::max(c->nvar, 1); a.nrec = (int)c->rec.size(); a.nstim = c->nstim; a.nt = c->nt; a.nt_amps = c->nt_amps;
     a.nruns = nruns; a.depth = c->depth; a.logn = c->logn; a.stim_q = c->stim_q;
     a.stim_area = c->area; a.v_init = c->v_init;
     a.q_h = c->d_qh.p; a.q_ph = c->d_qph.p; a.q_meta = c->d_qmeta.p; a.q_cm = c->d_qcm.p;
-    a.row_e = c->d_e.p; a.row_f = c->d_f.p; a.pvar = c->d_pvar.p; a.pval = c->d_pval.p; a.pv = c->d_pv.p;
+    a.row_e = c->d_e.p; a.row_f = c->d_f.p; a.pval = c->d_pval.p; a.pv = c->d_pv.p;
+    a.pconst_mask = c->pconst_mask; a.pv_in_lds = npv > 0 ? 1 : 0;
+    for (int j = 0; j < 32; j++) a.pconst[j] = j < (int)c->pconst.size() ? c->pconst[j] : 0.0;
     a.chains = c->d_chains.p; a.lvl_ptr = c->d_lvl.p; a.fath = c->d_fath.p; a.flvl_ptr = c->d_flvl.p;
     a.fchild = c->d_fchild.p; a.fidx = c->d_fidx.p; a.dts = c->d_dts.p; a.amps = c->d_amps.p; a.rec_q = c->d_rec.p;
     a.states = c->d_states.p; a.carry = c->d_carry.p; a.vhot = c->d_vhot.p;
+    c->last_R = R;
     dim3 block(LW * R), grid((nruns + R - 1) / R);
     int spl = c->cfg_spl > 0 ? c->cfg_spl : c->nt;
     launches = 0;
@@ -552,7 +597,7 @@ extern "C" int ngpu_run(ngpu_ctx *c, double *vhot, ngpu_stats *st)
         st->comp_steps_per_s = mk > 0 ? st->comp_steps / (mk * 1e-3) : 0.0;
         st->waves_per_run = c->lay_W;
         st->slots_per_lane = c->lay_slot;
-        st->runs_per_block = c->lay_W == 1 ? (c->cfg_r > 0 ? c->cfg_r : 4) : 1;
+        st->runs_per_block = c->last_R;
         st->launches = launches;
         st->nvar = c->nvar;
         st->nsets = c->nsets; st->nstim = c->nstim; st->nrec = (int)c->rec.size(); st->nt = c->nt; st->N = c->N;

@@ -47,9 +47,11 @@ struct SimArgs {
     const int *q_h, *q_ph, *q_meta;      /* [NQ]: row, parent row, mask | son<<24 (-1 = padding) */
     const double *q_cm;                  /* [NQ] */
     const double *row_e, *row_f;         /* [N] by Hines row */
-    const int *pvar;                     /* [NPARAM*NQ] varying-column index or -1 */
-    const double *pval;                  /* [NPARAM*NQ] value when uniform */
+    const double *pval;                  /* [NPARAM*NQ] value, or tagged NaN = varying column */
     const double *pv;                    /* [nsets*nvar] varying parameter values */
+    unsigned pconst_mask;                /* bit j: parameter j has one value wherever it is used */
+    int pv_in_lds;                       /* stage the run's varying values in LDS */
+    double pconst[32];
     const int *chains, *lvl_ptr;         /* chains: (i0, i1) pairs; lvl_ptr[depth+2] */
     const int *fath, *flvl_ptr, *fchild; /* fath: (J, c0, c1) triples; flvl_ptr[depth+1] */
     const int *fidx;                     /* [logn*N] 0-based ancestor row, N = beyond root */
@@ -72,46 +74,106 @@ __device__ __forceinline__ void group_sync()
     }
 }
 
-struct ParamAccess {
-    const int *pvar;
-    const double *pval, *pv;
-    int q, NQ;
-    __device__ __forceinline__ double operator()(int j) const
-    {
-        int k = pvar[j * NQ + q];
-        return k < 0 ? pval[j * NQ + q] : pv[k];
-    }
+/* Parameter tables hold either the value or, for a run-varying column k, a
+ * NaN whose high word is NG_PTAG and low word k (decoded from the run's
+ * column values).  Genuine NaN parameters (quiet NaN payloads) pass through. */
+#define NG_PTAG 0x7FF40000u
+
+struct ParamRegs {
+    double v[32];
+    __device__ __forceinline__ double operator()(int j) const { return v[j]; }
 };
 
+template <int NS>
+__device__ __forceinline__ double pick(const double (&a)[NS], int c)
+{
+    double r = a[0];
+#pragma unroll
+    for (int i = 1; i < NS; i++) r = (c == i) ? a[i] : r;
+    return r;
+}
+template <int NS>
+__device__ __forceinline__ int pick(const int (&a)[NS], int c)
+{
+    int r = a[0];
+#pragma unroll
+    for (int i = 1; i < NS; i++) r = (c == i) ? a[i] : r;
+    return r;
+}
+template <int NS>
+__device__ __forceinline__ void place(double (&a)[NS], int c, double v)
+{
+#pragma unroll
+    for (int i = 0; i < NS; i++) a[i] = (c == i) ? v : a[i];
+}
+
 template <class SET, int NSLOT, bool MULTI>
-__global__ void __launch_bounds__(MULTI ? 512 : 256) ngpu_sim_kernel(SimArgs a)
+#ifndef NG_WAVES_PER_EU
+#define NG_WAVES_PER_EU 2
+#endif
+__global__ void __launch_bounds__(MULTI ? 512 : 256) __attribute__((amdgpu_waves_per_eu(NG_WAVES_PER_EU)))
+ngpu_sim_kernel(SimArgs a)
 {
     extern __shared__ double lds[];
     const int LW = a.LW;
     const int group = threadIdx.x / LW;
     const int g = threadIdx.x - group * LW;
+    const int N = a.N, NQ = a.NQ;
+    /* block-shared: coupling coefficients by Hines row */
+    double *RE = lds, *RF = lds + N;
+    for (int i = threadIdx.x; i < N; i += blockDim.x) { RE[i] = a.row_e[i]; RF[i] = a.row_f[i]; }
+    __syncthreads();
     const int run = blockIdx.x * a.runs_per_block + group;
     if (run >= a.nruns) return;
-    const int N = a.N, NQ = a.NQ;
-    const int per_run = 2 * (N + 1) + 3 * NQ + a.nrec * 64;
-    double *X = lds + (size_t)group * per_run;   /* rhs, then x (PX)          */
-    double *U = X + (N + 1);                      /* diagonal, then PF          */
-    double *V = U + (N + 1);                      /* v per position             */
-    double *DV = V + NQ;                          /* axial dv per position      */
-    double *ICA = DV + NQ;                        /* ica of the last BREAKPOINT */
-    double *REC = ICA + NQ;
+    const int npv = a.pv_in_lds ? a.nvar : 0;
+    const int per_run = 2 * (N + 1) + npv + a.nrec * 64;
+    double *X = lds + 2 * N + (size_t)group * per_run;   /* rhs, then x (PX)          */
+    double *U = X + (N + 1);                              /* diagonal, then PF          */
+    double *PV = U + (N + 1);                             /* run-varying parameters     */
+    double *REC = PV + npv;
     const int set = run / a.nstim, stim = run - set * a.nstim;
     double *S = a.states + (size_t)run * SET::NSTATE * NQ;
     double *CY = a.carry + (size_t)run * (3 * NQ + N + 1);
-    const double *pv = a.pv + (size_t)set * a.nvar;
+    const double *pvg = a.pv + (size_t)set * a.nvar;
+    const double *pvsrc = pvg;
+    if (a.pv_in_lds) {
+        for (int k = g; k < a.nvar; k += LW) PV[k] = pvg[k];
+        pvsrc = PV;
+    }
+
+    int qh[NSLOT], qph[NSLOT], qm[NSLOT];
+    double v[NSLOT], dv[NSLOT], ica[NSLOT];
+#pragma unroll
+    for (int c = 0; c < NSLOT; c++) {
+        const int q = c * LW + g;
+        qh[c] = -1; qph[c] = 0; qm[c] = 0; v[c] = 0.0; dv[c] = 0.0; ica[c] = 0.0;
+        if (q < N) { qh[c] = a.q_h[q]; qph[c] = a.q_ph[q]; qm[c] = a.q_meta[q]; }
+    }
+    auto load_params = [&](int q, ParamRegs &pr) {
+#pragma unroll
+        for (int j = 0; j < SET::NPARAM; j++) {
+            double x;
+            if (a.pconst_mask & (1u << j)) {
+                x = a.pconst[j];
+            } else {
+                x = a.pval[j * NQ + q];
+                const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
+                if ((unsigned)(bits >> 32) == NG_PTAG) x = pvsrc[(unsigned)bits];
+            }
+            pr.v[j] = x;
+        }
+    };
 
     if (a.t0 == 0) {
         for (int i = g; i <= N; i += LW) { X[i] = 0.0; U[i] = 0.0; }
-#pragma unroll 1
-        for (int q = g; q < N; q += LW) {
-            const unsigned m = (unsigned)a.q_meta[q] & 0xffffffu;
-            ParamAccess P{a.pvar, a.pval, pv, q, NQ};
-            V[q] = a.v_init; DV[q] = 0.0; ICA[q] = 0.0;
+#pragma unroll
+        for (int c = 0; c < NSLOT; c++) {
+            const int q = c * LW + g;
+            if (q >= N) continue;
+            const unsigned m = (unsigned)qm[c] & 0xffffffu;
+            ParamRegs P;
+            load_params(q, P);
+            v[c] = a.v_init;
             double st[SET::NSTATE];
 #pragma unroll
             for (int k = 0; k < SET::NSTATE; k++) st[k] = 0.0;
@@ -123,7 +185,11 @@ __global__ void __launch_bounds__(MULTI ? 512 : 256) ngpu_sim_kernel(SimArgs a)
         }
     } else {
         for (int i = g; i < N; i += LW) X[i] = CY[3 * NQ + i];
-        for (int q = g; q < N; q += LW) { V[q] = CY[q]; DV[q] = CY[NQ + q]; ICA[q] = CY[2 * NQ + q]; }
+#pragma unroll
+        for (int c = 0; c < NSLOT; c++) {
+            const int q = c * LW + g;
+            if (q < N) { v[c] = CY[q]; dv[c] = CY[NQ + q]; ica[c] = CY[2 * NQ + q]; }
+        }
     }
     if (g == 0) { X[N] = 0.0; U[N] = 1.0; }
     group_sync<MULTI>();
@@ -137,29 +203,35 @@ __global__ void __launch_bounds__(MULTI ? 512 : 256) ngpu_sim_kernel(SimArgs a)
         const double amp = amp_row[t];
         const int slot_t = (t - a.t0) & 63;
         /* record v before the step (CudaStuff.cu:257-261) */
-        for (int r = g; r < a.nrec; r += LW) REC[r * 64 + slot_t] = V[a.rec_q[r]];
+        for (int r = 0; r < a.nrec; r++) {
+            const int rq = a.rec_q[r];
+            if (rq - (rq / LW) * LW == g) REC[r * 64 + slot_t] = pick(v, rq / LW);
+        }
 
         /* dv += x[parent] - x[self] with the previous solution (CudaStuff.cu:287) */
 #pragma unroll
-        for (int c = 0; c < NSLOT; c++) {
-            const int q = c * LW + g;
-            if (q < N) DV[q] += X[a.q_ph[q]] - X[a.q_h[q]];
-        }
+        for (int c = 0; c < NSLOT; c++)
+            if (qh[c] >= 0) dv[c] += X[qph[c]] - X[qh[c]];
         group_sync<MULTI>();
 
         /* fused cnexp(t-1) + BREAKPOINT(t) + Hines row (CudaStuff.cu:263-294) */
 #pragma unroll 1
-        for (int q = g; q < N; q += LW) {
-            const int h = a.q_h[q];
-            const unsigned m = (unsigned)a.q_meta[q] & 0xffffffu;
-            const double vq = V[q];
-            ParamAccess P{a.pvar, a.pval, pv, q, NQ};
+        for (int c = 0; c < NSLOT; c++) {
+            const int q = c * LW + g;
+            if (q >= N) continue;
+            const int h = pick(qh, c);
+            const unsigned m = (unsigned)pick(qm, c) & 0xffffffu;
+            const double vq = pick(v, c);
+            const double cmq = a.q_cm[q];
+            const double fh = RF[h];
             const unsigned sb = SET::state_bits(m);
             double st[SET::NSTATE];
 #pragma unroll
             for (int k = 0; k < SET::NSTATE; k++) st[k] = (sb & (1u << k)) ? S[k * NQ + q] : 0.0;
+            ParamRegs P;
+            load_params(q, P);
             if (t > 0) {
-                SET::cnexp(m, vq, dtp, st, ICA[q], P);
+                SET::cnexp(m, vq, dtp, st, pick(ica, c), P);
 #pragma unroll
                 for (int k = 0; k < SET::NSTATE; k++)
                     if (sb & (1u << k)) S[k * NQ + q] = st[k];
@@ -167,14 +239,13 @@ __global__ void __launch_bounds__(MULTI ? 512 : 256) ngpu_sim_kernel(SimArgs a)
             double sumDv = 0.0, sum = 0.0, icaDv = 0.0, icaNow = 0.0;
             SET::current(m, vq + 0.001, st, P, sumDv, icaDv, false);
             SET::current(m, vq, st, P, sum, icaNow, true);
-            ICA[q] = icaNow;
+            place(ica, c, icaNow);
             const double istim = (q == a.stim_q) ? 100 * amp / a.stim_area : 0.0;
             const double gModel = (sumDv - sum) / NG_EPS_V;
             double rhs = istim - sum;
-            double D = gModel + a.q_cm[q] / (dt * 1000);
-            const double fh = a.row_f[h];
+            double D = gModel + cmq / (dt * 1000);
             D -= fh;
-            rhs -= fh * DV[q];
+            rhs -= fh * pick(dv, c);
             X[h] = rhs;
             U[h] = D;
         }
@@ -185,12 +256,10 @@ __global__ void __launch_bounds__(MULTI ? 512 : 256) ngpu_sim_kernel(SimArgs a)
         for (int son = 1; son <= 2; son++) {
 #pragma unroll
             for (int c = 0; c < NSLOT; c++) {
-                const int q = c * LW + g;
-                if (q >= N || (a.q_meta[q] >> 24) != son) continue;
-                const int h = a.q_h[q], ph = a.q_ph[q];
-                const double e = a.row_e[h];
-                X[ph] += e * DV[q];
-                U[ph] -= e;
+                if (qh[c] < 0 || (qm[c] >> 24) != son) continue;
+                const double e = RE[qh[c]];
+                X[qph[c]] += e * dv[c];
+                U[qph[c]] -= e;
             }
             group_sync<MULTI>();
         }
@@ -202,12 +271,15 @@ __global__ void __launch_bounds__(MULTI ? 512 : 256) ngpu_sim_kernel(SimArgs a)
                 const int i0 = a.chains[2 * k], i1 = a.chains[2 * k + 1];
                 if (i0 >= i1) continue;
                 double um1 = U[i0 - 1], bm1 = X[i0 - 1];
+                double e1 = RE[i0 - 1], f1 = RF[i0 - 1], ui = U[i0], xi = X[i0];
                 for (int i = i0; i < i1; i++) {
-                    const double e1 = a.row_e[i - 1], f1 = a.row_f[i - 1];
-                    const double ui = U[i] - f1 * (e1 / um1);
-                    const double bi = X[i] - bm1 * e1 / um1;
-                    U[i] = ui; X[i] = bi;
-                    um1 = ui; bm1 = bi;
+                    double e1n = 0.0, f1n = 0.0, un = 0.0, xn = 0.0;
+                    if (i + 1 < i1) { e1n = RE[i]; f1n = RF[i]; un = U[i + 1]; xn = X[i + 1]; }
+                    const double unew = ui - f1 * (e1 / um1);
+                    const double bnew = xi - bm1 * e1 / um1;
+                    U[i] = unew; X[i] = bnew;
+                    um1 = unew; bm1 = bnew;
+                    e1 = e1n; f1 = f1n; ui = un; xi = xn;
                 }
             }
             group_sync<MULTI>();
@@ -218,7 +290,7 @@ __global__ void __launch_bounds__(MULTI ? 512 : 256) ngpu_sim_kernel(SimArgs a)
                     double uJ = U[J], bJ = X[J];
                     for (int cc = c0; cc < c1; cc++) {
                         const int tr = a.fchild[cc];
-                        const double ut = U[tr], bt = X[tr], e1 = a.row_e[tr], f1 = a.row_f[tr];
+                        const double ut = U[tr], bt = X[tr], e1 = RE[tr], f1 = RF[tr];
                         uJ = uJ - f1 * (e1 / ut);
                         bJ = bJ - bt * e1 / ut;
                     }
@@ -236,22 +308,27 @@ __global__ void __launch_bounds__(MULTI ? 512 : 256) ngpu_sim_kernel(SimArgs a)
             if (h < N) {
                 const double u = U[h];
                 X[h] = X[h] / u;
-                U[h] = -a.row_f[h] / u;
+                U[h] = -RF[h] / u;
             }
         }
         group_sync<MULTI>();
         for (int r = 0; r < a.logn; r++) {
             double nx[NSLOT], nf[NSLOT];
+            int an[NSLOT];
             const int *fr = a.fidx + (size_t)r * N;
+#pragma unroll
+            for (int c = 0; c < NSLOT; c++) {
+                const int h = c * LW + g;
+                an[c] = h < N ? fr[h] : N;
+            }
 #pragma unroll
             for (int c = 0; c < NSLOT; c++) {
                 const int h = c * LW + g;
                 nx[c] = 0.0; nf[c] = 0.0;
                 if (h < N) {
-                    const int an = fr[h];
                     const double pxj = X[h], pfj = U[h];
-                    nx[c] = pxj + X[an] * pfj;
-                    nf[c] = pfj * U[an];
+                    nx[c] = pxj + X[an[c]] * pfj;
+                    nf[c] = pfj * U[an[c]];
                 }
             }
             group_sync<MULTI>();
@@ -265,10 +342,8 @@ __global__ void __launch_bounds__(MULTI ? 512 : 256) ngpu_sim_kernel(SimArgs a)
 
         /* v update (CudaStuff.cu:312-315) */
 #pragma unroll
-        for (int c = 0; c < NSLOT; c++) {
-            const int q = c * LW + g;
-            if (q < N) V[q] += X[a.q_h[q]];
-        }
+        for (int c = 0; c < NSLOT; c++)
+            if (qh[c] >= 0) v[c] += X[qh[c]];
         dtp = dt;
 
         /* flush the recorded chunk */
@@ -286,7 +361,11 @@ __global__ void __launch_bounds__(MULTI ? 512 : 256) ngpu_sim_kernel(SimArgs a)
     /* carry for the next launch */
     if (a.t1 < a.nt) {
         for (int i = g; i < N; i += LW) CY[3 * NQ + i] = X[i];
-        for (int q = g; q < N; q += LW) { CY[q] = V[q]; CY[NQ + q] = DV[q]; CY[2 * NQ + q] = ICA[q]; }
+#pragma unroll
+        for (int c = 0; c < NSLOT; c++) {
+            const int q = c * LW + g;
+            if (q < N) { CY[q] = v[c]; CY[NQ + q] = dv[c]; CY[2 * NQ + q] = ica[c]; }
+        }
     }
 }
 
