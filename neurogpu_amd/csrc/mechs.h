@@ -15,7 +15,14 @@
  *   S::NMECH, S::NPARAM, S::NSTATE        counts
  *   S::state_bits(mech_mask)              which state slots the present mechanisms use
  *   S::init(mask, v, st, P)               INITIAL blocks in mechanism order
- *   S::cnexp(mask, v, dt, st, ica, P)     DERIVATIVE (cnexp) blocks at the new v
+ *   S::cnexp(mask, v, dt, st, ica, P)     DERIVATIVE (cnexp) blocks at the new v; the
+ *                                         reversal eca is not touched (see S::eca_after_cnexp)
+ *   S::eca_after_cnexp(mask, st)          eca after a cnexp pass.  Every DERIVATIVE block
+ *                                         that changes cai ends with eca = f(cai), and the
+ *                                         blocks that follow read cai only, so one
+ *                                         evaluation at the end gives the same bits; where
+ *                                         no block changes cai it is constant and eca =
+ *                                         f(cai) already held.  Valid after the first step.
  *   S::current(mask, v, st, P, sum, ica)  BREAKPOINT currents; ica accumulated when asked
  * st[] holds the state slots (ion concentration cai and reversal eca included)
  * and P(j) returns parameter j (the set's parameter-matrix index).
@@ -39,6 +46,8 @@ NG_DEV double ng_cnexp(double y, double yinf, double tau, double dt)
 /* ===================================================================== pas */
 struct SetPassive {
     static constexpr int NMECH = 1, NPARAM = 2, NSTATE = 1;
+    static constexpr int ECA = 31;
+    NG_DEV static void eca_after_cnexp(unsigned, double *) {}
     NG_DEV static unsigned state_bits(unsigned) { return 0u; }
     template <class PF> NG_DEV static void init(unsigned, double, double *, const PF &) {}
     template <class PF> NG_DEV static void cnexp(unsigned, double, double, double *, double, const PF &) {}
@@ -172,6 +181,10 @@ NG_DEV void skv3_1(double v, double &mi, double &mt)
 struct SetTTPC {
     static constexpr int NMECH = 13, NPARAM = 19, NSTATE = 20;
     static constexpr int CAI = 18, ECA = 19;
+    NG_DEV static void eca_after_cnexp(unsigned m, double *st)
+    {
+        if (m & ((1u << 0) | (1u << 1) | (1u << 2) | (1u << 11))) st[ECA] = ttpc::eca_of(st[CAI]);
+    }
     NG_DEV static unsigned state_bits(unsigned m)
     {
         unsigned s = 0;
@@ -218,7 +231,6 @@ struct SetTTPC {
             double B = ((-minCai)) / decay;
             double C = (-(1.0)) / decay;
             st[CAI] = st[CAI] + (1. - ng_exp(dt * ((-(1.0)) / decay))) * ((-(A - B)) / C - st[CAI]);
-            st[ECA] = eca_of(st[CAI]);
         }
         if (m & (1u << 3)) { ih(v, a, b); st[4] = ng_cnexp(st[4], a, b, dt); }
         if (m & (1u << 4)) { im(v, a, b); st[5] = ng_cnexp(st[5], a, b, dt); }
@@ -227,7 +239,7 @@ struct SetTTPC {
         if (m & (1u << 7)) { nap_et2(v, a, b, c, d); st[10] = ng_cnexp(st[10], a, b, dt); st[11] = ng_cnexp(st[11], c, d, dt); }
         if (m & (1u << 8)) { nata_t(v, a, b, c, d); st[12] = ng_cnexp(st[12], a, b, dt); st[13] = ng_cnexp(st[13], c, d, dt); }
         if (m & (1u << 9)) { nats2_t(v, a, b, c, d); st[14] = ng_cnexp(st[14], a, b, dt); st[15] = ng_cnexp(st[15], c, d, dt); }
-        if (m & (1u << 11)) { double zi = sk_e2(st[CAI]); st[17] = ng_cnexp(st[17], zi, P(17), dt); st[ECA] = eca_of(st[CAI]); }
+        if (m & (1u << 11)) { double zi = sk_e2(st[CAI]); st[17] = ng_cnexp(st[17], zi, P(17), dt); }
         if (m & (1u << 12)) { skv3_1(v, a, b); st[16] = ng_cnexp(st[16], a, b, dt); }
     }
     template <class PF>
@@ -316,6 +328,10 @@ NG_DEV void na(double vm, const PF &P, double &mi, double &mt, double &hi, doubl
 struct SetMainen {
     static constexpr int NMECH = 7, NPARAM = 30, NSTATE = 10;
     static constexpr int CAI = 8, ECA = 9;
+    NG_DEV static void eca_after_cnexp(unsigned m, double *st)
+    {
+        if (m & 7u) st[ECA] = mainen::eca_of(st[CAI]);
+    }
     NG_DEV static unsigned state_bits(unsigned m)
     {
         unsigned s = 0;
@@ -343,15 +359,14 @@ struct SetMainen {
     {
         using namespace mainen;
         double a, b, c, d;
-        if (m & 1u) { ca(v + MN_VSHIFT_CA, a, b, c, d); st[0] = ng_cnexp(st[0], a, b, dt); st[1] = ng_cnexp(st[1], c, d, dt); st[ECA] = eca_of(st[CAI]); }
+        if (m & 1u) { ca(v + MN_VSHIFT_CA, a, b, c, d); st[0] = ng_cnexp(st[0], a, b, dt); st[1] = ng_cnexp(st[1], c, d, dt); }
         if (m & 2u) {
             double drive = -(10000.0) * ica / (2.0 * NG_FARADAY * MN_DEPTH_CAD);
             if (drive <= 0.) drive = 0.;
             st[2] = st[2] + (1. - ng_exp(dt * ((-1.0) / MN_TAUR))) * ((-(drive + (MN_CAINF) / MN_TAUR)) / ((-1.0) / MN_TAUR) - st[2]);
             st[CAI] = st[2];
-            st[ECA] = eca_of(st[CAI]);
         }
-        if (m & 4u) { kca(st[CAI], P(4), P(5), P(3), a, b); st[3] = ng_cnexp(st[3], a, b, dt); st[ECA] = eca_of(st[CAI]); }
+        if (m & 4u) { kca(st[CAI], P(4), P(5), P(3), a, b); st[3] = ng_cnexp(st[3], a, b, dt); }
         if (m & 8u) { kx(v, P(7), P(8), P(9), P(10), a, b); st[4] = ng_cnexp(st[4], a, b, dt); }
         if (m & 16u) { kx(v, P(12), P(13), P(14), P(15), a, b); st[5] = ng_cnexp(st[5], a, b, dt); }
         if (m & 32u) { na(v + MN_VSHIFT_NA, P, a, b, c, d); st[6] = ng_cnexp(st[6], a, b, dt); st[7] = ng_cnexp(st[7], c, d, dt); }

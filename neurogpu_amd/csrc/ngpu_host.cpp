@@ -89,7 +89,7 @@ struct ngpu_ctx {
     int lay_W = 0, lay_slot = 0, lay_NQ = 0;
     bool tables_dirty = true, params_dirty = true, stim_dirty = true, graph_dirty = true;
     DevArr<int> d_qh, d_qph, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_fidx, d_rec;
-    DevArr<double> d_qcm, d_e, d_f, d_pval, d_pv, d_dts, d_amps, d_states, d_carry, d_vhot;
+    DevArr<double> d_qcm, d_e, d_f, d_pval, d_pconst, d_pv, d_dts, d_amps, d_states, d_carry, d_vhot;
     int stim_q = -1;
     std::vector<int> rec_q;
     unsigned pconst_mask = 0;
@@ -179,7 +179,7 @@ extern "C" void ngpu_close(ngpu_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
     for (auto *d : {&c->d_qh, &c->d_qph, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_fidx, &c->d_rec}) d->release();
-    for (auto *d : {&c->d_qcm, &c->d_e, &c->d_f, &c->d_pval, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_carry, &c->d_vhot}) d->release();
+    for (auto *d : {&c->d_qcm, &c->d_e, &c->d_f, &c->d_pval, &c->d_pconst, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_carry, &c->d_vhot}) d->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -195,6 +195,7 @@ extern "C" int ngpu_set_tree(ngpu_ctx *c, const ngpu_tree *t)
     if (!c || !t) return fail(c, NGPU_E_ARG, "ngpu_set_tree: NULL argument");
     const int N = t->N;
     if (N < 1 || t->ncomps < 1) return fail(c, NGPU_E_ARG, "N=%d and ncomps=%d must be positive", N, t->ncomps);
+    if (N > 32000) return fail(c, NGPU_E_ARG, "N=%d exceeds the 32000-segment limit of the int16 schedule", N);
     if (mech_nmodels(t->mechset) < 0) return fail(c, NGPU_E_ARG, "unknown mechanism set %d", t->mechset);
     if (t->nmodels != mech_nmodels(t->mechset))
         return fail(c, NGPU_E_ARG, "mechanism set %d has %d mechanisms, tree gives NModels=%d", t->mechset, mech_nmodels(t->mechset), t->nmodels);
@@ -459,7 +460,7 @@ static int build_tables(ngpu_ctx *c, int W, int slot)
     c->stim_q = c->stim_seg >= 0 ? pos[c->stim_seg] : -1;
     std::vector<double> re(c->e), rf(c->f);
     int rc;
-    if ((rc = upload(c, c->d_qh, qh)) || (rc = upload(c, c->d_qph, qph)) || (rc = upload(c, c->d_qmeta, qmeta)) ||
+    if ((rc = upload(c, c->d_pconst, c->pconst)) || (rc = upload(c, c->d_qh, qh)) || (rc = upload(c, c->d_qph, qph)) || (rc = upload(c, c->d_qmeta, qmeta)) ||
         (rc = upload(c, c->d_qcm, qcm)) || (rc = upload(c, c->d_e, re)) || (rc = upload(c, c->d_f, rf)) ||
         (rc = upload(c, c->d_pval, pval)) || (rc = upload(c, c->d_chains, c->chains)) ||
         (rc = upload(c, c->d_lvl, c->lvl_ptr)) || (rc = upload(c, c->d_fath, c->fath)) || (rc = upload(c, c->d_flvl, c->flvl_ptr)) ||
@@ -488,7 +489,9 @@ static int enqueue(ngpu_ctx *c, int &launches)
     if (c->lay_W == 1) R = c->cfg_r > 0 ? c->cfg_r : 1;
     const int npv = c->nvar <= 512 ? std::max(c->nvar, 1) : 0;
     const size_t per_run = (size_t)(2 * (N + 1) + npv + (int)c->rec.size() * 64) * sizeof(double);
-    const size_t shared = (size_t)2 * N * sizeof(double);
+    const size_t nshort = (size_t)c->logn * N + c->chains.size() + c->fath.size() + c->fchild.size() +
+                          (size_t)(c->depth + 2) + (size_t)(c->depth + 1);
+    const size_t shared = (size_t)2 * N * sizeof(double) + ((nshort * sizeof(short) + 7) / 8) * 8;
     while (R > 1 && shared + per_run * R > 160 * 1024) R--;
     const size_t lds = shared + per_run * R;
     if (lds > 160 * 1024) return fail(c, NGPU_E_ARG, "tree needs %zu bytes of LDS per block (limit 160 KiB)", lds);
@@ -500,9 +503,10 @@ static int enqueue(ngpu_ctx *c, int &launches)
     a.q_h = c->d_qh.p; a.q_ph = c->d_qph.p; a.q_meta = c->d_qmeta.p; a.q_cm = c->d_qcm.p;
     a.row_e = c->d_e.p; a.row_f = c->d_f.p; a.pval = c->d_pval.p; a.pv = c->d_pv.p;
     a.pconst_mask = c->pconst_mask; a.pv_in_lds = npv > 0 ? 1 : 0;
-    for (int j = 0; j < 32; j++) a.pconst[j] = j < (int)c->pconst.size() ? c->pconst[j] : 0.0;
+    a.pconst = c->d_pconst.p;
     a.chains = c->d_chains.p; a.lvl_ptr = c->d_lvl.p; a.fath = c->d_fath.p; a.flvl_ptr = c->d_flvl.p;
-    a.fchild = c->d_fchild.p; a.fidx = c->d_fidx.p; a.dts = c->d_dts.p; a.amps = c->d_amps.p; a.rec_q = c->d_rec.p;
+    a.fchild = c->d_fchild.p; a.fidx = c->d_fidx.p;
+    a.nchains = (int)c->chains.size() / 2; a.nfath = (int)c->fath.size() / 3; a.nfchild = (int)c->fchild.size(); a.dts = c->d_dts.p; a.amps = c->d_amps.p; a.rec_q = c->d_rec.p;
     a.states = c->d_states.p; a.carry = c->d_carry.p; a.vhot = c->d_vhot.p;
     c->last_R = R;
     dim3 block(LW * R), grid((nruns + R - 1) / R);

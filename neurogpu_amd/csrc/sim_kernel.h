@@ -51,10 +51,11 @@ struct SimArgs {
     const double *pv;                    /* [nsets*nvar] varying parameter values */
     unsigned pconst_mask;                /* bit j: parameter j has one value wherever it is used */
     int pv_in_lds;                       /* stage the run's varying values in LDS */
-    double pconst[32];
+    const double *pconst;                /* [NPARAM] those values */
     const int *chains, *lvl_ptr;         /* chains: (i0, i1) pairs; lvl_ptr[depth+2] */
     const int *fath, *flvl_ptr, *fchild; /* fath: (J, c0, c1) triples; flvl_ptr[depth+1] */
     const int *fidx;                     /* [logn*N] 0-based ancestor row, N = beyond root */
+    int nchains, nfath, nfchild;         /* schedule sizes (staged into LDS as int16) */
     const double *dts, *amps;
     const int *rec_q;
     double *states;                      /* [nruns][NSTATE][NQ] */
@@ -79,9 +80,21 @@ __device__ __forceinline__ void group_sync()
  * column values).  Genuine NaN parameters (quiet NaN payloads) pass through. */
 #define NG_PTAG 0x7FF40000u
 
-struct ParamRegs {
-    double v[32];
-    __device__ __forceinline__ double operator()(int j) const { return v[j]; }
+/* Parameter j at position q, read where a mechanism uses it: a value shared
+ * by every segment comes from the constant table (a scalar load), otherwise
+ * the per-position table gives the value or the tag of a run-varying column. */
+struct ParamFetch {
+    const double *pconst, *pval, *pv;
+    unsigned mask;
+    int q, NQ;
+    __device__ __forceinline__ double operator()(int j) const
+    {
+        if (mask & (1u << j)) return pconst[j];
+        double x = pval[j * NQ + q];
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
+        if ((unsigned)(bits >> 32) == NG_PTAG) x = pv[(unsigned)bits];
+        return x;
+    }
 };
 
 template <int NS>
@@ -119,14 +132,28 @@ ngpu_sim_kernel(SimArgs a)
     const int group = threadIdx.x / LW;
     const int g = threadIdx.x - group * LW;
     const int N = a.N, NQ = a.NQ;
-    /* block-shared: coupling coefficients by Hines row */
+    /* block-shared, staged once: coupling coefficients by Hines row, the
+     * level schedule and the ancestor tables (int16 row indices) */
+    const int npv = a.pv_in_lds ? a.nvar : 0;
+    const int per_run = 2 * (N + 1) + npv + a.nrec * 64;
     double *RE = lds, *RF = lds + N;
+    short *SH = (short *)(lds + 2 * N + (size_t)a.runs_per_block * per_run);
+    short *FIDX = SH;
+    short *CH = FIDX + a.logn * N;
+    short *FA = CH + 2 * a.nchains;
+    short *FC = FA + 3 * a.nfath;
+    short *LP = FC + a.nfchild;
+    short *FLP = LP + (a.depth + 2);
     for (int i = threadIdx.x; i < N; i += blockDim.x) { RE[i] = a.row_e[i]; RF[i] = a.row_f[i]; }
+    for (int i = threadIdx.x; i < a.logn * N; i += blockDim.x) FIDX[i] = (short)a.fidx[i];
+    for (int i = threadIdx.x; i < 2 * a.nchains; i += blockDim.x) CH[i] = (short)a.chains[i];
+    for (int i = threadIdx.x; i < 3 * a.nfath; i += blockDim.x) FA[i] = (short)a.fath[i];
+    for (int i = threadIdx.x; i < a.nfchild; i += blockDim.x) FC[i] = (short)a.fchild[i];
+    for (int i = threadIdx.x; i < a.depth + 2; i += blockDim.x) LP[i] = (short)a.lvl_ptr[i];
+    for (int i = threadIdx.x; i < a.depth + 1; i += blockDim.x) FLP[i] = (short)a.flvl_ptr[i];
     __syncthreads();
     const int run = blockIdx.x * a.runs_per_block + group;
     if (run >= a.nruns) return;
-    const int npv = a.pv_in_lds ? a.nvar : 0;
-    const int per_run = 2 * (N + 1) + npv + a.nrec * 64;
     double *X = lds + 2 * N + (size_t)group * per_run;   /* rhs, then x (PX)          */
     double *U = X + (N + 1);                              /* diagonal, then PF          */
     double *PV = U + (N + 1);                             /* run-varying parameters     */
@@ -149,31 +176,15 @@ ngpu_sim_kernel(SimArgs a)
         qh[c] = -1; qph[c] = 0; qm[c] = 0; v[c] = 0.0; dv[c] = 0.0; ica[c] = 0.0;
         if (q < N) { qh[c] = a.q_h[q]; qph[c] = a.q_ph[q]; qm[c] = a.q_meta[q]; }
     }
-    auto load_params = [&](int q, ParamRegs &pr) {
-#pragma unroll
-        for (int j = 0; j < SET::NPARAM; j++) {
-            double x;
-            if (a.pconst_mask & (1u << j)) {
-                x = a.pconst[j];
-            } else {
-                x = a.pval[j * NQ + q];
-                const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
-                if ((unsigned)(bits >> 32) == NG_PTAG) x = pvsrc[(unsigned)bits];
-            }
-            pr.v[j] = x;
-        }
-    };
-
     if (a.t0 == 0) {
         for (int i = g; i <= N; i += LW) { X[i] = 0.0; U[i] = 0.0; }
-#pragma unroll
+#pragma unroll 1
         for (int c = 0; c < NSLOT; c++) {
             const int q = c * LW + g;
             if (q >= N) continue;
-            const unsigned m = (unsigned)qm[c] & 0xffffffu;
-            ParamRegs P;
-            load_params(q, P);
-            v[c] = a.v_init;
+            const unsigned m = (unsigned)pick(qm, c) & 0xffffffu;
+            const ParamFetch P{a.pconst, a.pval, pvsrc, a.pconst_mask, q, NQ};
+            place(v, c, a.v_init);
             double st[SET::NSTATE];
 #pragma unroll
             for (int k = 0; k < SET::NSTATE; k++) st[k] = 0.0;
@@ -214,24 +225,40 @@ ngpu_sim_kernel(SimArgs a)
             if (qh[c] >= 0) dv[c] += X[qph[c]] - X[qh[c]];
         group_sync<MULTI>();
 
-        /* fused cnexp(t-1) + BREAKPOINT(t) + Hines row (CudaStuff.cu:263-294) */
+        /* fused cnexp(t-1) + BREAKPOINT(t) + Hines row (CudaStuff.cu:263-294).
+         * eca is stored by INITIAL only: at t = 0 it is loaded, afterwards it is
+         * recomputed from cai after the cnexp pass.  The next slot's states are
+         * requested before the current slot is computed. */
+        const unsigned keep = (t > 0) ? ~(1u << SET::ECA) : ~0u;
+        double stn[SET::NSTATE];
+        {
+            const unsigned sb0 = SET::state_bits((unsigned)qm[0] & 0xffffffu) & keep;
+#pragma unroll
+            for (int k = 0; k < SET::NSTATE; k++) stn[k] = (g < N && (sb0 & (1u << k))) ? S[k * NQ + g] : 0.0;
+        }
 #pragma unroll 1
         for (int c = 0; c < NSLOT; c++) {
             const int q = c * LW + g;
+            double st[SET::NSTATE];
+#pragma unroll
+            for (int k = 0; k < SET::NSTATE; k++) st[k] = stn[k];
+            if (c + 1 < NSLOT) {
+                const int qn = q + LW;
+                const unsigned sbn = SET::state_bits((unsigned)pick(qm, c + 1) & 0xffffffu) & keep;
+#pragma unroll
+                for (int k = 0; k < SET::NSTATE; k++) stn[k] = (qn < N && (sbn & (1u << k))) ? S[k * NQ + qn] : 0.0;
+            }
             if (q >= N) continue;
             const int h = pick(qh, c);
             const unsigned m = (unsigned)pick(qm, c) & 0xffffffu;
             const double vq = pick(v, c);
             const double cmq = a.q_cm[q];
             const double fh = RF[h];
-            const unsigned sb = SET::state_bits(m);
-            double st[SET::NSTATE];
-#pragma unroll
-            for (int k = 0; k < SET::NSTATE; k++) st[k] = (sb & (1u << k)) ? S[k * NQ + q] : 0.0;
-            ParamRegs P;
-            load_params(q, P);
+            const unsigned sb = SET::state_bits(m) & keep;
+            const ParamFetch P{a.pconst, a.pval, pvsrc, a.pconst_mask, q, NQ};
             if (t > 0) {
                 SET::cnexp(m, vq, dtp, st, pick(ica, c), P);
+                SET::eca_after_cnexp(m, st);
 #pragma unroll
                 for (int k = 0; k < SET::NSTATE; k++)
                     if (sb & (1u << k)) S[k * NQ + q] = st[k];
@@ -266,9 +293,9 @@ ngpu_sim_kernel(SimArgs a)
 
         /* triangularisation by levels (BeforeLU, CudaStuff.cu:31-69) */
         for (int L = 0; L <= a.depth; L++) {
-            const int k1 = a.lvl_ptr[L + 1];
-            for (int k = a.lvl_ptr[L] + g; k < k1; k += LW) {
-                const int i0 = a.chains[2 * k], i1 = a.chains[2 * k + 1];
+            const int k1 = LP[L + 1];
+            for (int k = LP[L] + g; k < k1; k += LW) {
+                const int i0 = CH[2 * k], i1 = CH[2 * k + 1];
                 if (i0 >= i1) continue;
                 double um1 = U[i0 - 1], bm1 = X[i0 - 1];
                 double e1 = RE[i0 - 1], f1 = RF[i0 - 1], ui = U[i0], xi = X[i0];
@@ -284,12 +311,12 @@ ngpu_sim_kernel(SimArgs a)
             }
             group_sync<MULTI>();
             if (L < a.depth) {
-                const int f1e = a.flvl_ptr[L + 1];
-                for (int k = a.flvl_ptr[L] + g; k < f1e; k += LW) {
-                    const int J = a.fath[3 * k], c0 = a.fath[3 * k + 1], c1 = a.fath[3 * k + 2];
+                const int f1e = FLP[L + 1];
+                for (int k = FLP[L] + g; k < f1e; k += LW) {
+                    const int J = FA[3 * k], c0 = FA[3 * k + 1], c1 = FA[3 * k + 2];
                     double uJ = U[J], bJ = X[J];
                     for (int cc = c0; cc < c1; cc++) {
-                        const int tr = a.fchild[cc];
+                        const int tr = FC[cc];
                         const double ut = U[tr], bt = X[tr], e1 = RE[tr], f1 = RF[tr];
                         uJ = uJ - f1 * (e1 / ut);
                         bJ = bJ - bt * e1 / ut;
@@ -315,7 +342,7 @@ ngpu_sim_kernel(SimArgs a)
         for (int r = 0; r < a.logn; r++) {
             double nx[NSLOT], nf[NSLOT];
             int an[NSLOT];
-            const int *fr = a.fidx + (size_t)r * N;
+            const short *fr = FIDX + r * N;
 #pragma unroll
             for (int c = 0; c < NSLOT; c++) {
                 const int h = c * LW + g;
