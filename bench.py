@@ -1,10 +1,15 @@
 """Benchmark of the simulation loop on one MI355X.
 
-One bench step = one complete simulation of the workload: every (parameter
-set, stimulus) run advanced through all of its dt steps by libneurogpu's HIP
-kernel, inputs already resident in device memory.  The headline workload is
-the active pyramidal-like cell (500 compartments, Mainen channel set),
-1024 parameter sets x 3 step currents, dt = 0.02 ms, 10,000 steps.
+One bench step = one pass of the simulation over the whole batch: every
+(parameter set, stimulus) run advanced through a window of dt steps by
+libneurogpu's HIP kernel, inputs already resident in device memory.  The
+headline workload is the thick-tufted-pyramidal-like cell (1,200 compartments,
+the 13 BBP TTPC mechanisms), 8,192 parameter sets x 5 step currents,
+dt = 0.025 ms; a bench step covers its first 1,000 dt steps (the work per
+compartment-step does not depend on the window: the kernel has no
+data-dependent branches besides the rate functions' singularity bumps).
+``--config 1`` measures the active 500-compartment cell over its full 10,000
+steps.
 
 Prints one JSON line (the last line of stdout).
 """
@@ -59,13 +64,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=1)
+    ap.add_argument("--config", type=int, default=2)
     ap.add_argument("--nsets", type=int, default=None)
     ap.add_argument("--nt", type=int, default=None)
     ap.add_argument("--waves-per-run", type=int, default=0)
     ap.add_argument("--runs-per-block", type=int, default=0)
     ap.add_argument("--graph", type=int, default=1)
-    ap.add_argument("--cpu-sets", type=int, default=8)
+    ap.add_argument("--cpu-sets", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (HBM counter calibration)")
@@ -88,7 +93,8 @@ def main():
     from neurogpu_amd import models
     from neurogpu_amd._lib import Simulator
 
-    w = models.config(args.config, nsets=args.nsets, nt=args.nt)
+    nt = args.nt if args.nt is not None else (1000 if args.config == 2 else None)
+    w = models.config(args.config, nsets=args.nsets, nt=nt)
     m = w.model
     print(f"bench: {w.name} N={m.tree.N} runs={w.nruns} nt={w.dts.size} S={m.state_count():.2f}", flush=True)
     sim = Simulator(0)
@@ -134,7 +140,8 @@ def main():
     }
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
-        nsets = min(args.cpu_sets, w.x.shape[0])
+        cpu_sets = args.cpu_sets if args.cpu_sets is not None else (48 if args.config == 2 else 8)
+        nsets = min(cpu_sets, w.x.shape[0])
         ref, cps, el = cpu_baseline(w, nsets, threads)
         d = float(np.max(np.abs(gpu[:nsets] - ref)))
         result["cpu_baseline"] = {"value": cps, "unit": "comp-steps/s", "cores": threads, "kind": "port",

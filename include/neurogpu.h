@@ -91,6 +91,15 @@ int ngpu_set_launch(ngpu_ctx *ctx, int waves_per_run, int runs_per_block,
  * device).  Replaces callKernel/stEfork2Main (CANON/CudaStuff.cu:443-587). */
 int ngpu_run(ngpu_ctx *ctx, double *vhot, ngpu_stats *stats);
 
+/* Spike features of every trace of the last run (record site 0), computed on
+ * the device: out [nsets][nstim][NGPU_NFEAT] = spike count, first-spike
+ * latency (ms after t_on[stim]), mean AP amplitude above the pre-stimulus
+ * baseline (mV), mean AP width at thr (ms), mean ISI (ms); NaN where undefined.
+ * Replaces the per-individual eFEL pass of the GA evaluator (PYS/efel_ext.py:23-97,
+ * threshold at :15). */
+#define NGPU_NFEAT 5
+int ngpu_features(ngpu_ctx *ctx, const double *t_on, const double *t_off, double thr, double *out);
+
 /* Copy the traces of the last run to vhot (same layout as ngpu_run). */
 int ngpu_fetch_vhot(ngpu_ctx *ctx, double *vhot);
 
@@ -116,6 +125,32 @@ int ngpu_mechset_nparams(int mechset);
 const char *ngpu_last_error(const ngpu_ctx *ctx);
 
 void ngpu_close(ngpu_ctx *ctx);
+
+/* ---- Surrogate network (no reference counterpart: the reference GA has no
+ * surrogate).  An ensemble of `members` MLPs nin -> H -> H -> H -> nout with
+ * ReLU, BF16 weights and activations on MFMA with FP32 accumulation, FP32
+ * master weights and Adam.  Parameters of one member, in order: W1[H][nin],
+ * b1[H], W2[H][H], b2[H], W3[H][H], b3[H], W4[nout][H], b4[nout]; members are
+ * concatenated. */
+typedef struct ngpu_mlp ngpu_mlp;
+int ngpu_mlp_create(int device, int nin, int hidden, int nout, int members, ngpu_mlp **out);
+long long ngpu_mlp_nparams(const ngpu_mlp *mlp);                      /* per member */
+int ngpu_mlp_set_weights(ngpu_mlp *mlp, const float *w);              /* all members; resets Adam */
+int ngpu_mlp_get_weights(ngpu_mlp *mlp, float *w);
+/* Mean-squared-error training, mini-batch Adam (beta 0.9/0.999, eps 1e-8);
+ * losses [members][epochs] (may be NULL). */
+int ngpu_mlp_train(ngpu_mlp *mlp, const float *x, const float *t, int n, int epochs, int batch, float lr,
+                   unsigned seed, float *losses);
+/* out [members][n][nout] */
+int ngpu_mlp_predict(ngpu_mlp *mlp, const float *x, int n, float *out);
+/* Loss and gradient of member `member` on one batch (for checks against a CPU restatement). */
+int ngpu_mlp_gradients(ngpu_mlp *mlp, int member, const float *x, const float *t, int n, float *grad, float *loss);
+const char *ngpu_mlp_last_error(const ngpu_mlp *mlp);
+void ngpu_mlp_close(ngpu_mlp *mlp);
+/* C[M][N] = op(A) op(B) on the device, A/B bf16 bit patterns (op = transpose
+ * when ta/tb), FP32 result; a test entry point for the MFMA GEMM. */
+int ngpu_gemm_bf16(int device, int ta, int tb, int M, int N, int K, const unsigned short *A,
+                   const unsigned short *B, float *C);
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,11 @@ _DP = C.POINTER(C.c_double)
 
 EXPORTS = ("ngpu_create", "ngpu_set_tree", "ngpu_set_sim", "ngpu_set_params", "ngpu_set_stimuli",
            "ngpu_set_launch", "ngpu_run", "ngpu_fetch_vhot", "ngpu_open", "ngpu_write_vhot",
-           "ngpu_vhot_size", "ngpu_mechset_nparams", "ngpu_last_error", "ngpu_close")
+           "ngpu_vhot_size", "ngpu_mechset_nparams", "ngpu_last_error", "ngpu_close", "ngpu_features",
+           "ngpu_mlp_create", "ngpu_mlp_nparams", "ngpu_mlp_set_weights", "ngpu_mlp_get_weights", "ngpu_mlp_train",
+           "ngpu_mlp_predict", "ngpu_mlp_gradients", "ngpu_mlp_last_error", "ngpu_mlp_close", "ngpu_gemm_bf16")
+NFEAT = 5
+FEATURES = ("spike_count", "first_latency", "ap_amplitude", "ap_width", "mean_isi")
 
 
 class NgpuError(RuntimeError):
@@ -84,6 +88,8 @@ def lib():
         L.ngpu_mechset_nparams.argtypes = [C.c_int]
         L.ngpu_last_error.argtypes = [vp]
         L.ngpu_last_error.restype = C.c_char_p
+        L.ngpu_features.argtypes = [vp, _DP, _DP, C.c_double, _DP]
+        L.ngpu_features.restype = C.c_int
         L.ngpu_close.argtypes = [vp]
         L.ngpu_close.restype = None
         for name in ("ngpu_create", "ngpu_set_tree", "ngpu_set_sim", "ngpu_set_params", "ngpu_set_stimuli",
@@ -189,6 +195,16 @@ class Simulator:
     def fetch(self) -> np.ndarray:
         out = np.empty(self.vhot_size(), dtype=np.float64)
         self._check(self._L.ngpu_fetch_vhot(self._h, out.ctypes.data_as(_DP)))
+        return out
+
+    def features(self, t_on, t_off, thr: float = -30.0) -> np.ndarray:
+        """Spike features of the last run's traces, computed on the device:
+        [nsets, nstim, 5] (see FEATURES)."""
+        on = np.ascontiguousarray(np.broadcast_to(np.asarray(t_on, np.float64), (self.nstim,)))
+        off = np.ascontiguousarray(np.broadcast_to(np.asarray(t_off, np.float64), (self.nstim,)))
+        out = np.empty((self.nsets, self.nstim, NFEAT), dtype=np.float64)
+        self._check(self._L.ngpu_features(self._h, on.ctypes.data_as(_DP), off.ctypes.data_as(_DP), float(thr),
+                                          out.ctypes.data_as(_DP)))
         return out
 
     def write_vhot(self, path: str) -> None:

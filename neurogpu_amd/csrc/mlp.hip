@@ -1,0 +1,431 @@
+/*
+ * mlp.hip -- the surrogate network: BF16 MFMA GEMMs with FP32 accumulation,
+ * forward and backward passes, Adam, for an ensemble of MLPs
+ * nin -> H -> H -> H -> nout with ReLU.  No reference counterpart (the
+ * reference GA has no surrogate); it pre-screens GA offspring.
+ *
+ * GEMM: C[M,N] = sum_k A(m,k) B(k,n), A and B bf16 (either orientation), FP32
+ * accumulation on v_mfma_f32_32x32x16_bf16 (gfx950): a 256-thread block holds
+ * a 64x64 tile of C, each wave a 32x32 sub-tile; operands are read straight
+ * from global memory (the matrices here are at most a few hundred wide) with
+ * bounds checks, so no size needs padding.  The epilogue optionally adds a
+ * bias per column, applies ReLU, multiplies by a ReLU mask (backward), and
+ * writes FP32 and/or BF16.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/neurogpu.h"
+
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+__host__ __device__ static inline uint16_t f2bf(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x7fffffu)) return (uint16_t)((u >> 16) | 0x40); /* NaN */
+    u += 0x7fffu + ((u >> 16) & 1u);                                                              /* nearest even */
+    return (uint16_t)(u >> 16);
+}
+__host__ __device__ static inline float bf2f(uint16_t h)
+{
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+struct GemmArgs {
+    int M, N, K, lda, ldb, ldc;
+    const uint16_t *A, *B;
+    const float *bias;          /* [N] or null */
+    const uint16_t *mask;       /* [M, ldc] bf16: keep where > 0 (ReLU backward) or null */
+    float *C32;                 /* fp32 out or null */
+    uint16_t *C16;              /* bf16 out or null */
+    int relu;
+};
+
+template <bool TA, bool TB>
+__global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m0 = blockIdx.y * 64 + (wave >> 1) * 32, n0 = blockIdx.x * 64 + (wave & 1) * 32;
+    const int row = lane & 31, kh = (lane >> 5) * 8;
+    f32x16 acc;
+#pragma unroll
+    for (int i = 0; i < 16; i++) acc[i] = 0.0f;
+    const int am = m0 + row, bn = n0 + row;
+    for (int k0 = 0; k0 < g.K; k0 += 16) {
+        bf16x8 a, b;
+#pragma unroll
+        for (int e = 0; e < 8; e++) {
+            const int k = k0 + kh + e;
+            uint16_t av = 0, bv = 0;
+            if (am < g.M && k < g.K) av = TA ? g.A[(size_t)k * g.lda + am] : g.A[(size_t)am * g.lda + k];
+            if (bn < g.N && k < g.K) bv = TB ? g.B[(size_t)bn * g.ldb + k] : g.B[(size_t)k * g.ldb + bn];
+            a[e] = (short)av;
+            b[e] = (short)bv;
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    /* accumulator i of this lane: row 8*(i/4) + 4*(lane/32) + i%4, column lane%32 */
+    const int col = n0 + (lane & 31);
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const int r = m0 + 8 * (i >> 2) + 4 * (lane >> 5) + (i & 3);
+        if (r >= g.M || col >= g.N) continue;
+        float v = acc[i];
+        if (g.bias) v += g.bias[col];
+        if (g.relu && v < 0.0f) v = 0.0f;
+        if (g.mask && !(bf2f(g.mask[(size_t)r * g.ldc + col]) > 0.0f)) v = 0.0f;
+        if (g.C32) g.C32[(size_t)r * g.ldc + col] = v;
+        if (g.C16) g.C16[(size_t)r * g.ldc + col] = f2bf(v);
+    }
+}
+
+static hipError_t gemm(hipStream_t s, bool ta, bool tb, const GemmArgs &g)
+{
+    dim3 grid((g.N + 63) / 64, (g.M + 63) / 64), block(256);
+    if (!ta && tb) hipLaunchKernelGGL((gemm_bf16_kernel<false, true>), grid, block, 0, s, g);
+    else if (!ta && !tb) hipLaunchKernelGGL((gemm_bf16_kernel<false, false>), grid, block, 0, s, g);
+    else if (ta && !tb) hipLaunchKernelGGL((gemm_bf16_kernel<true, false>), grid, block, 0, s, g);
+    else hipLaunchKernelGGL((gemm_bf16_kernel<true, true>), grid, block, 0, s, g);
+    return hipGetLastError();
+}
+
+__global__ void f32_to_bf16_kernel(const float *x, uint16_t *y, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        y[i] = f2bf(x[i]);
+}
+
+/* dY = 2 (Y - T) / (rows * cols) in bf16 and fp32; loss partial sums per block */
+__global__ void mse_grad_kernel(const float *Y, const float *T, int n, float scale, float *dY32, uint16_t *dY16,
+                                float *loss)
+{
+    __shared__ float part[256];
+    float s = 0.0f;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float d = Y[i] - T[i];
+        s += d * d;
+        const float gval = 2.0f * d * scale;
+        dY32[i] = gval;
+        dY16[i] = f2bf(gval);
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) part[threadIdx.x] += part[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) atomicAdd(loss, part[0] * scale);
+}
+
+/* column sums of an [rows, cols] matrix (fp32 or bf16) -> out[cols] */
+__global__ void colsum_kernel(const float *x32, const uint16_t *x16, int rows, int cols, float *out)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= cols) return;
+    float s = 0.0f;
+    for (int r = 0; r < rows; r++) s += x32 ? x32[(size_t)r * cols + c] : bf2f(x16[(size_t)r * cols + c]);
+    out[c] = s;
+}
+
+__global__ void adam_kernel(float *w, const float *gr, float *m, float *v, uint16_t *w16, int n, float lr, float b1,
+                            float b2, float eps, float c1, float c2)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float gi = gr[i];
+        const float mi = b1 * m[i] + (1.0f - b1) * gi;
+        const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        w[i] = wi;
+        if (w16) w16[i] = f2bf(wi);
+    }
+}
+
+/* ------------------------------------------------------------------ host */
+
+struct ngpu_mlp {
+    int device = 0, nin = 0, H = 0, nout = 0, members = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    /* per member: parameters in one fp32 buffer; offsets of W1 b1 W2 b2 W3 b3 W4 b4 */
+    size_t off[8], nparam = 0;
+    float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;  /* [members * nparam] */
+    uint16_t *w16 = nullptr;                                      /* bf16 copy (biases unused) */
+    long long step = 0;
+    /* scratch for batches of up to cap rows */
+    int cap = 0;
+    uint16_t *x16 = nullptr, *a16[3] = {nullptr, nullptr, nullptr}, *d16[3] = {nullptr, nullptr, nullptr}, *dy16 = nullptr;
+    float *y32 = nullptr, *t32 = nullptr, *dy32 = nullptr, *loss = nullptr;
+};
+
+static int mfail(ngpu_mlp *p, int code, const std::string &msg)
+{
+    if (p) p->err = msg;
+    return code;
+}
+#define MCHK(p, call)                                                                              \
+    do {                                                                                           \
+        hipError_t _e = (call);                                                                    \
+        if (_e != hipSuccess) return mfail((p), NGPU_E_HIP, std::string(#call) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+static int ensure_cap(ngpu_mlp *p, int n)
+{
+    if (n <= p->cap) return 0;
+    (void)hipFree(p->x16); (void)hipFree(p->dy16); (void)hipFree(p->y32); (void)hipFree(p->t32); (void)hipFree(p->dy32);
+    for (int i = 0; i < 3; i++) { (void)hipFree(p->a16[i]); (void)hipFree(p->d16[i]); }
+    const size_t H = p->H;
+    MCHK(p, hipMalloc((void **)&p->x16, (size_t)n * p->nin * 2));
+    for (int i = 0; i < 3; i++) {
+        MCHK(p, hipMalloc((void **)&p->a16[i], (size_t)n * H * 2));
+        MCHK(p, hipMalloc((void **)&p->d16[i], (size_t)n * H * 2));
+    }
+    MCHK(p, hipMalloc((void **)&p->dy16, (size_t)n * p->nout * 2));
+    MCHK(p, hipMalloc((void **)&p->y32, (size_t)n * p->nout * 4));
+    MCHK(p, hipMalloc((void **)&p->t32, (size_t)n * p->nout * 4));
+    MCHK(p, hipMalloc((void **)&p->dy32, (size_t)n * p->nout * 4));
+    p->cap = n;
+    return 0;
+}
+
+extern "C" int ngpu_mlp_create(int device, int nin, int hidden, int nout, int members, ngpu_mlp **out)
+{
+    if (!out || nin < 1 || hidden < 1 || nout < 1 || members < 1 || nin > 4096 || hidden > 4096 || nout > 4096)
+        return NGPU_E_ARG;
+    *out = nullptr;
+    if (hipSetDevice(device) != hipSuccess) return NGPU_E_HIP;
+    ngpu_mlp *p = new ngpu_mlp();
+    p->device = device; p->nin = nin; p->H = hidden; p->nout = nout; p->members = members;
+    const size_t H = hidden;
+    const size_t sz[8] = {H * nin, H, H * H, H, H * H, H, (size_t)nout * H, (size_t)nout};
+    size_t o = 0;
+    for (int i = 0; i < 8; i++) { p->off[i] = o; o += sz[i]; }
+    p->nparam = o;
+    const size_t tot = o * members;
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void **)&p->w, tot * 4) != hipSuccess || hipMalloc((void **)&p->g, tot * 4) != hipSuccess ||
+        hipMalloc((void **)&p->m, tot * 4) != hipSuccess || hipMalloc((void **)&p->v, tot * 4) != hipSuccess ||
+        hipMalloc((void **)&p->w16, tot * 2) != hipSuccess || hipMalloc((void **)&p->loss, members * 4) != hipSuccess) {
+        delete p;
+        return NGPU_E_NOMEM;
+    }
+    (void)hipMemset(p->m, 0, tot * 4);
+    (void)hipMemset(p->v, 0, tot * 4);
+    *out = p;
+    return NGPU_OK;
+}
+
+extern "C" void ngpu_mlp_close(ngpu_mlp *p)
+{
+    if (!p) return;
+    (void)hipSetDevice(p->device);
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    for (void *q : {(void *)p->w, (void *)p->g, (void *)p->m, (void *)p->v, (void *)p->w16, (void *)p->loss, (void *)p->x16,
+                    (void *)p->dy16, (void *)p->y32, (void *)p->t32, (void *)p->dy32})
+        if (q) (void)hipFree(q);
+    for (int i = 0; i < 3; i++) { (void)hipFree(p->a16[i]); (void)hipFree(p->d16[i]); }
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+}
+
+extern "C" const char *ngpu_mlp_last_error(const ngpu_mlp *p) { return p ? p->err.c_str() : ""; }
+
+extern "C" long long ngpu_mlp_nparams(const ngpu_mlp *p) { return p ? (long long)p->nparam : 0; }
+
+extern "C" int ngpu_mlp_set_weights(ngpu_mlp *p, const float *w)
+{
+    if (!p || !w) return NGPU_E_ARG;
+    MCHK(p, hipSetDevice(p->device));
+    const size_t tot = p->nparam * p->members;
+    MCHK(p, hipMemcpy(p->w, w, tot * 4, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(256), dim3(256), 0, p->stream, p->w, p->w16, tot);
+    MCHK(p, hipMemsetAsync(p->m, 0, tot * 4, p->stream));
+    MCHK(p, hipMemsetAsync(p->v, 0, tot * 4, p->stream));
+    MCHK(p, hipStreamSynchronize(p->stream));
+    p->step = 0;
+    return NGPU_OK;
+}
+
+extern "C" int ngpu_mlp_get_weights(ngpu_mlp *p, float *w)
+{
+    if (!p || !w) return NGPU_E_ARG;
+    MCHK(p, hipSetDevice(p->device));
+    MCHK(p, hipMemcpy(w, p->w, p->nparam * p->members * 4, hipMemcpyDeviceToHost));
+    return NGPU_OK;
+}
+
+/* forward of member mem on x16 [n, nin] -> activations a16[0..2], y32 [n, nout] */
+static int forward(ngpu_mlp *p, int mem, int n)
+{
+    const float *W = p->w + mem * p->nparam;
+    const uint16_t *W16 = p->w16 + mem * p->nparam;
+    const int H = p->H;
+    GemmArgs g{};
+    const uint16_t *in = p->x16;
+    int K = p->nin;
+    for (int l = 0; l < 3; l++) {
+        g = GemmArgs{n, H, K, K, K, H, in, W16 + p->off[2 * l], W + p->off[2 * l + 1], nullptr, nullptr, p->a16[l], 1};
+        MCHK(p, gemm(p->stream, false, true, g));
+        in = p->a16[l];
+        K = H;
+    }
+    g = GemmArgs{n, p->nout, H, H, H, p->nout, in, W16 + p->off[6], W + p->off[7], nullptr, p->y32, nullptr, 0};
+    MCHK(p, gemm(p->stream, false, true, g));
+    return 0;
+}
+
+/* gradients of member mem (loss = mean squared error over n x nout) into p->g */
+static int backward(ngpu_mlp *p, int mem, int n)
+{
+    const uint16_t *W16 = p->w16 + mem * p->nparam;
+    float *G = p->g + mem * p->nparam;
+    const int H = p->H, O = p->nout;
+    const int tot = n * O;
+    MCHK(p, hipMemsetAsync(p->loss + mem, 0, 4, p->stream));
+    hipLaunchKernelGGL(mse_grad_kernel, dim3(std::min(256, (tot + 255) / 256)), dim3(256), 0, p->stream, p->y32, p->t32,
+                       tot, 1.0f / (float)tot, p->dy32, p->dy16, p->loss + mem);
+    /* layer 4: dW4 = dY^T A3, db4 = colsum(dY), dZ3 = (dY W4) * (A3 > 0) */
+    GemmArgs g{O, H, n, O, H, H, p->dy16, p->a16[2], nullptr, nullptr, G + p->off[6], nullptr, 0};
+    MCHK(p, gemm(p->stream, true, false, g));
+    hipLaunchKernelGGL(colsum_kernel, dim3((O + 63) / 64), dim3(64), 0, p->stream, p->dy32, (const uint16_t *)nullptr, n, O,
+                       G + p->off[7]);
+    g = GemmArgs{n, H, O, O, H, H, p->dy16, W16 + p->off[6], nullptr, p->a16[2], nullptr, p->d16[2], 0};
+    MCHK(p, gemm(p->stream, false, false, g));
+    for (int l = 2; l >= 0; l--) {
+        const uint16_t *prev = l > 0 ? p->a16[l - 1] : p->x16;
+        const int K = l > 0 ? H : p->nin;
+        g = GemmArgs{H, K, n, H, K, K, p->d16[l], prev, nullptr, nullptr, G + p->off[2 * l], nullptr, 0};
+        MCHK(p, gemm(p->stream, true, false, g));
+        hipLaunchKernelGGL(colsum_kernel, dim3((H + 63) / 64), dim3(64), 0, p->stream, (const float *)nullptr, p->d16[l], n,
+                           H, G + p->off[2 * l + 1]);
+        if (l > 0) {
+            g = GemmArgs{n, H, H, H, H, H, p->d16[l], W16 + p->off[2 * l], nullptr, p->a16[l - 1], nullptr, p->d16[l - 1], 0};
+            MCHK(p, gemm(p->stream, false, false, g));
+        }
+    }
+    return 0;
+}
+
+static int upload_x(ngpu_mlp *p, const float *x, int n)
+{
+    std::vector<uint16_t> h((size_t)n * p->nin);
+    for (size_t i = 0; i < h.size(); i++) h[i] = f2bf(x[i]);
+    MCHK(p, hipMemcpyAsync(p->x16, h.data(), h.size() * 2, hipMemcpyHostToDevice, p->stream));
+    MCHK(p, hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+extern "C" int ngpu_mlp_predict(ngpu_mlp *p, const float *x, int n, float *out)
+{
+    if (!p || !x || !out || n < 1) return NGPU_E_ARG;
+    MCHK(p, hipSetDevice(p->device));
+    int rc;
+    if ((rc = ensure_cap(p, n)) || (rc = upload_x(p, x, n))) return rc;
+    for (int mem = 0; mem < p->members; mem++) {
+        if ((rc = forward(p, mem, n))) return rc;
+        MCHK(p, hipMemcpyAsync(out + (size_t)mem * n * p->nout, p->y32, (size_t)n * p->nout * 4, hipMemcpyDeviceToHost,
+                               p->stream));
+    }
+    MCHK(p, hipStreamSynchronize(p->stream));
+    return NGPU_OK;
+}
+
+extern "C" int ngpu_mlp_gradients(ngpu_mlp *p, int mem, const float *x, const float *t, int n, float *grad, float *loss)
+{
+    if (!p || !x || !t || !grad || n < 1 || mem < 0 || mem >= p->members) return NGPU_E_ARG;
+    MCHK(p, hipSetDevice(p->device));
+    int rc;
+    if ((rc = ensure_cap(p, n)) || (rc = upload_x(p, x, n))) return rc;
+    MCHK(p, hipMemcpyAsync(p->t32, t, (size_t)n * p->nout * 4, hipMemcpyHostToDevice, p->stream));
+    if ((rc = forward(p, mem, n)) || (rc = backward(p, mem, n))) return rc;
+    MCHK(p, hipMemcpyAsync(grad, p->g + mem * p->nparam, p->nparam * 4, hipMemcpyDeviceToHost, p->stream));
+    if (loss) MCHK(p, hipMemcpyAsync(loss, p->loss + mem, 4, hipMemcpyDeviceToHost, p->stream));
+    MCHK(p, hipStreamSynchronize(p->stream));
+    return NGPU_OK;
+}
+
+/* Mini-batch Adam on (x [n, nin], t [n, nout]) for every member; losses_out
+ * [members * epochs] receives the mean batch loss per epoch (may be null). */
+extern "C" int ngpu_mlp_train(ngpu_mlp *p, const float *x, const float *t, int n, int epochs, int batch, float lr,
+                              unsigned seed, float *losses_out)
+{
+    if (!p || !x || !t || n < 1 || epochs < 0 || batch < 1) return NGPU_E_ARG;
+    MCHK(p, hipSetDevice(p->device));
+    batch = std::min(batch, n);
+    int rc;
+    if ((rc = ensure_cap(p, batch))) return rc;
+    std::vector<uint16_t> xb((size_t)n * p->nin);
+    for (size_t i = 0; i < xb.size(); i++) xb[i] = f2bf(x[i]);
+    std::vector<uint16_t> bx((size_t)batch * p->nin);
+    std::vector<float> bt((size_t)batch * p->nout);
+    std::vector<int> idx(n);
+    for (int i = 0; i < n; i++) idx[i] = i;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    for (int mem = 0; mem < p->members; mem++) {
+        std::mt19937 rng(seed * 1009u + (unsigned)mem);
+        long long step = 0;
+        for (int ep = 0; ep < epochs; ep++) {
+            std::shuffle(idx.begin(), idx.end(), rng);
+            double esum = 0.0;
+            int nb = 0;
+            for (int s0 = 0; s0 + batch <= n; s0 += batch) {
+                for (int r = 0; r < batch; r++) {
+                    const int i = idx[s0 + r];
+                    std::memcpy(&bx[(size_t)r * p->nin], &xb[(size_t)i * p->nin], p->nin * 2);
+                    std::memcpy(&bt[(size_t)r * p->nout], &t[(size_t)i * p->nout], p->nout * 4);
+                }
+                MCHK(p, hipMemcpyAsync(p->x16, bx.data(), bx.size() * 2, hipMemcpyHostToDevice, p->stream));
+                MCHK(p, hipMemcpyAsync(p->t32, bt.data(), bt.size() * 4, hipMemcpyHostToDevice, p->stream));
+                if ((rc = forward(p, mem, batch)) || (rc = backward(p, mem, batch))) return rc;
+                step++;
+                const float c1 = 1.0f - std::pow(b1, (float)step), c2 = 1.0f - std::pow(b2, (float)step);
+                const size_t o = (size_t)mem * p->nparam;
+                hipLaunchKernelGGL(adam_kernel, dim3(256), dim3(256), 0, p->stream, p->w + o, p->g + o, p->m + o, p->v + o,
+                                   p->w16 + o, (int)p->nparam, lr, b1, b2, eps, c1, c2);
+                float l = 0.0f;
+                MCHK(p, hipMemcpyAsync(&l, p->loss + mem, 4, hipMemcpyDeviceToHost, p->stream));
+                MCHK(p, hipStreamSynchronize(p->stream));
+                esum += l;
+                nb++;
+            }
+            if (losses_out) losses_out[(size_t)mem * epochs + ep] = nb ? (float)(esum / nb) : 0.0f;
+        }
+    }
+    return NGPU_OK;
+}
+
+/* Plain GEMM on host buffers, for tests: C[M,N] = op(A) op(B), A/B bf16 bit patterns. */
+extern "C" int ngpu_gemm_bf16(int device, int ta, int tb, int M, int N, int K, const unsigned short *A,
+                              const unsigned short *B, float *C)
+{
+    if (M < 1 || N < 1 || K < 1 || !A || !B || !C) return NGPU_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return NGPU_E_HIP;
+    uint16_t *dA = nullptr, *dB = nullptr;
+    float *dC = nullptr;
+    int rc = NGPU_OK;
+    if (hipMalloc((void **)&dA, (size_t)M * K * 2) != hipSuccess || hipMalloc((void **)&dB, (size_t)K * N * 2) != hipSuccess ||
+        hipMalloc((void **)&dC, (size_t)M * N * 4) != hipSuccess) {
+        rc = NGPU_E_NOMEM;
+    } else {
+        (void)hipMemcpy(dA, A, (size_t)M * K * 2, hipMemcpyHostToDevice);
+        (void)hipMemcpy(dB, B, (size_t)K * N * 2, hipMemcpyHostToDevice);
+        GemmArgs g{M, N, K, ta ? M : K, tb ? K : N, N, dA, dB, nullptr, nullptr, dC, nullptr, 0};
+        if (gemm(nullptr, ta != 0, tb != 0, g) != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = NGPU_E_HIP;
+        else if (hipMemcpy(C, dC, (size_t)M * N * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = NGPU_E_HIP;
+    }
+    (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC);
+    return rc;
+}

@@ -29,6 +29,8 @@
 /* Per-set launchers, one translation unit each (sim_<set>.hip). */
 hipError_t ngpu_launch_passive(int nslot, bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
 hipError_t ngpu_launch_ttpc(int nslot, bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
+hipError_t ngpu_launch_features(const double *vhot, int nrec, int nt, int nruns, int nstim, const double *times,
+                                const double *t_on, const double *t_off, double thr, double *out, hipStream_t s);
 hipError_t ngpu_launch_mainen(int nslot, bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
 
 static const int kSlotSizes[] = {1, 2, 4, 6, 8};
@@ -89,7 +91,7 @@ struct ngpu_ctx {
     int lay_W = 0, lay_slot = 0, lay_NQ = 0;
     bool tables_dirty = true, params_dirty = true, stim_dirty = true, graph_dirty = true;
     DevArr<int> d_qh, d_qph, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_fidx, d_rec;
-    DevArr<double> d_qcm, d_e, d_f, d_pval, d_pconst, d_pv, d_dts, d_amps, d_states, d_carry, d_vhot;
+    DevArr<double> d_qcm, d_e, d_f, d_pval, d_pconst, d_pv, d_dts, d_amps, d_states, d_carry, d_vhot, d_times, d_fwin, d_feat;
     int stim_q = -1;
     std::vector<int> rec_q;
     unsigned pconst_mask = 0;
@@ -179,7 +181,7 @@ extern "C" void ngpu_close(ngpu_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
     for (auto *d : {&c->d_qh, &c->d_qph, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_fidx, &c->d_rec}) d->release();
-    for (auto *d : {&c->d_qcm, &c->d_e, &c->d_f, &c->d_pval, &c->d_pconst, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_carry, &c->d_vhot}) d->release();
+    for (auto *d : {&c->d_qcm, &c->d_e, &c->d_f, &c->d_pval, &c->d_pconst, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_carry, &c->d_vhot, &c->d_times, &c->d_fwin, &c->d_feat}) d->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -608,6 +610,28 @@ extern "C" int ngpu_run(ngpu_ctx *c, double *vhot, ngpu_stats *st)
     }
     (void)hipEventDestroy(h0);
     (void)hipEventDestroy(h1);
+    return NGPU_OK;
+}
+
+extern "C" int ngpu_features(ngpu_ctx *c, const double *t_on, const double *t_off, double thr, double *out)
+{
+    if (!c || !t_on || !t_off || !out) return fail(c, NGPU_E_ARG, "ngpu_features: NULL argument");
+    if (!c->ran) return fail(c, NGPU_E_STATE, "ngpu_features needs a completed ngpu_run");
+    if (c->rec.empty()) return fail(c, NGPU_E_STATE, "no record site");
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<double> times(c->nt), win(2 * c->nstim);
+    double t = 0.0;
+    for (int i = 0; i < c->nt; i++) { times[i] = t; t += c->dts[i]; }
+    for (int k = 0; k < c->nstim; k++) { win[k] = t_on[k]; win[c->nstim + k] = t_off[k]; }
+    const int nruns = c->nsets * c->nstim;
+    int rc;
+    if ((rc = upload(c, c->d_times, times)) || (rc = upload(c, c->d_fwin, win)) || (rc = alloc(c, c->d_feat, (size_t)nruns * NGPU_NFEAT)))
+        return rc;
+    hipError_t e = ngpu_launch_features(c->d_vhot.p, (int)c->rec.size(), c->nt, nruns, c->nstim, c->d_times.p, c->d_fwin.p,
+                                        c->d_fwin.p + c->nstim, thr, c->d_feat.p, c->stream);
+    if (e != hipSuccess) return fail(c, NGPU_E_HIP, "feature kernel launch failed: %s", hipGetErrorString(e));
+    HIPCHK(c, hipMemcpyAsync(out, c->d_feat.p, (size_t)nruns * NGPU_NFEAT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return NGPU_OK;
 }
 

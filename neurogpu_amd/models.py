@@ -156,7 +156,7 @@ def binary_tree(seed: int = 1000, depth: int = 5) -> Morphology:
 
 
 def pyramidal(seed: int, n_target: int, n_basal: int = 4, seg_len: float = 20.0,
-              axon_len: float = 600.0) -> Morphology:
+              axon_len: float = 600.0, apical_bias: float = 0.0) -> Morphology:
     """Seeded pyramidal-like morphology with about ``n_target`` compartments.
 
     The soma is a chain of short pieces so that every compartment has at most
@@ -166,7 +166,8 @@ def pyramidal(seed: int, n_target: int, n_basal: int = 4, seg_len: float = 20.0,
     and ``n_basal`` Galton-Watson basal trees.  Growth is breadth-first over
     all trees with a compartment budget; each section branches into two
     children with a depth-dependent probability, and while the budget is not
-    met a random terminal section restarts the process."""
+    met a random terminal section restarts the process (an apical one with
+    probability ``apical_bias``)."""
     rng = np.random.default_rng(seed)
     b = _Builder()
     ntrees = n_basal + 2
@@ -188,7 +189,10 @@ def pyramidal(seed: int, n_target: int, n_basal: int = 4, seg_len: float = 20.0,
         if not frontier:
             if not terminals:
                 break
-            k, region, depth, d = terminals.pop(int(rng.integers(len(terminals))))
+            apical = [i for i, tt in enumerate(terminals) if tt[1] == 3]
+            pick = apical[int(rng.integers(len(apical)))] if apical and rng.random() < apical_bias \
+                else int(rng.integers(len(terminals)))
+            k, region, depth, d = terminals.pop(pick)
             frontier.append((k, region, 1, d))
             continue
         idx = int(rng.integers(len(frontier)))
@@ -368,7 +372,7 @@ def ttpc_model(seed: int = 1002, n_target: int = 1200) -> Model:
     """Thick-tufted-pyramidal-like cell with the BBP TTPC mechanisms placed as
     in the reference's cADpyr232 biophysics (Figures/Figure2_BBP_TTPC_clean/
     biophysics.hoc) and 12 free conductances around its densities."""
-    m = pyramidal(seed, n_target, n_basal=5, axon_len=400.0)
+    m = pyramidal(seed, n_target, n_basal=5, axon_len=400.0, apical_bias=0.6)
     placement = {
         "basal": ("Ih", "pas"),
         "apical": ("Im", "NaTs2_t", "SKv3_1", "Ih", "pas"),

@@ -1,0 +1,68 @@
+"""Spike features: the CPU restatement pinned by known-answer traces, and the
+on-device feature kernel against it."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import features_ref
+
+
+def _triangles(centres, dt=0.1, nt=400, peak=30.0, slope=100.0, base=-70.0):
+    t = np.arange(nt) * dt
+    v = np.full(nt, base)
+    for c in centres:
+        m = np.abs(t - c) < (peak - base) / slope
+        v[m] = np.maximum(v[m], peak - slope * np.abs(t[m] - c))
+    return v, t
+
+
+def test_known_answer_three_spikes():
+    v, t = _triangles([10.0, 20.0, 32.0])
+    f = features_ref.trace_features(v, t, t_on=5.0)
+    # crossings of -30 at c -/+ 0.6: latency 10 - 0.6 - 5, width 1.2, ISI (10 + 12) / 2
+    np.testing.assert_allclose(f, [3.0, 4.4, 100.0, 1.2, 11.0], rtol=0, atol=1e-9)
+
+
+def test_no_spike_and_single_spike():
+    v, t = _triangles([])
+    f = features_ref.trace_features(v, t, 5.0)
+    assert f[0] == 0 and all(math.isnan(x) for x in f[1:])
+    v, t = _triangles([15.0])
+    f = features_ref.trace_features(v, t, 5.0)
+    assert f[0] == 1 and abs(f[1] - 9.4) < 1e-9 and math.isnan(f[4])
+
+
+def test_spike_cut_by_the_end_counts_amplitude_not_width():
+    v, t = _triangles([39.95], nt=400)
+    f = features_ref.trace_features(v, t, 5.0)
+    assert f[0] == 1 and math.isnan(f[3]) and f[2] > 0
+
+
+def test_baseline_is_mean_before_onset():
+    v, t = _triangles([20.0])
+    v[:50] = -60.0                      # t < 5 ms
+    f = features_ref.trace_features(v, t, 5.0)
+    assert abs(f[2] - (30.0 - (-60.0))) < 1e-9
+
+
+@pytest.mark.gpu
+def test_device_features_match_oracle():
+    from neurogpu_amd import models
+    from neurogpu_amd._lib import Simulator
+    from oracle import cpu_ref
+    w = models.config(1, nsets=6, nt=2500)
+    m = w.model
+    with Simulator(0) as sim:
+        sim.set_tree(m.tree, m.mechset_id)
+        sim.set_sim(m.v_init, m.rec_sites)
+        sim.set_params(w.params, m.ncomps, m.nparams)
+        sim.set_stimuli(w.amps, w.dts, m.stim_seg, m.stim_area)
+        sim.run(fetch=False)
+        gf = sim.features(20.0, 180.0)
+        gv = sim.fetch().reshape(6, 3, 1, -1)
+    ref = cpu_ref.simulate(m.tree, m.mechset_id, w.params, w.amps, w.dts, m.stim_seg, m.stim_area, m.v_init,
+                           m.rec_sites)
+    np.testing.assert_array_equal(gv, ref)
+    np.testing.assert_array_equal(gf, features_ref.features(ref, w.dts, 20.0))
+    assert np.nansum(gf[..., 0]) > 0

@@ -1,0 +1,64 @@
+"""GA layer: variation operators (CPU) and the evaluator / surrogate screen on the GPU."""
+import numpy as np
+import pytest
+
+from neurogpu_amd import ga
+
+
+def test_sbx_and_mutation_stay_in_bounds():
+    rng = np.random.default_rng(0)
+    lo, hi = np.full(8, -3.0), np.full(8, 1.0)
+    for _ in range(200):
+        a, b = rng.uniform(lo, hi), rng.uniform(lo, hi)
+        c1, c2 = ga.sbx(rng, a, b, lo, hi)
+        m = ga.poly_mutation(rng, a, lo, hi, indpb=1.0)
+        for y in (c1, c2, m):
+            assert np.all(y >= lo) and np.all(y <= hi)
+
+
+def test_sbx_preserves_the_mean_of_each_gene():
+    """SBX children are symmetric about the parents' midpoint (Deb & Agrawal 1995)."""
+    rng = np.random.default_rng(1)
+    lo, hi = np.full(4, -100.0), np.full(4, 100.0)
+    a, b = np.array([0.0, 1.0, 2.0, 3.0]), np.array([1.0, -1.0, 5.0, 3.0])
+    for _ in range(100):
+        c1, c2 = ga.sbx(rng, a, b, lo, hi)
+        np.testing.assert_allclose(c1 + c2, a + b, atol=1e-9)
+
+
+def test_fill_and_objectives():
+    f = np.array([[[2.0, 5.0, 90.0, 1.0, 20.0], [0.0, np.nan, np.nan, np.nan, np.nan]]])
+    filled = ga.fill_features(f, 100.0)
+    assert filled[0, 1, 1] == 100.0 and filled[0, 1, 2] == 0.0 and filled[0, 1, 4] == 100.0
+    tgt = ga.Target(filled[0], np.ones(5))
+    assert np.all(ga.objectives(filled, tgt) == 0)
+
+
+@pytest.mark.gpu
+def test_ga_improves_and_surrogate_screen_beats_random():
+    """A short GA improves the best simulated score; a surrogate trained on
+    simulator features picks offspring whose simulated scores are better than a
+    random pick of the same size (the gain the network brings to the GA)."""
+    from neurogpu_amd import models
+    from neurogpu_amd.surrogate import MLPEnsemble, feature_targets, genome_to_input, make_screen
+    w = models.config(1, nsets=1, nt=2000)           # 40 ms protocol
+    m = w.model
+    a, d = models.step_stimulus([0.2, 0.5, 1.0], 2000, 0.02, 5.0, 35.0)
+    w.amps, w.dts = a, d
+    ev = ga.Evaluator(w, 5.0, 35.0, target_x=m.nominal())
+    res = ga.run_ga(ev, mu=64, lam=128, ngen=4, seed=1178)
+    assert res.history[-1][2] <= res.history[0][2]
+    lo = np.log10([fp.lo for fp in m.free])
+    hi = np.log10([fp.hi for fp in m.free])
+    rng = np.random.default_rng(0)
+    g = rng.uniform(lo, hi, size=(2048, lo.size))
+    f = ev.features(10 ** g)
+    ens = MLPEnsemble(lo.size, 3 * len(ga.GA_FEATURES), members=3, seed=1)
+    ens.fit(genome_to_input(g, lo, hi), feature_targets(f), epochs=60, batch=256, lr=2e-3)
+    cand = rng.uniform(lo, hi, size=(1000, lo.size))
+    true = ga.objectives(ev.features(10 ** cand), ev.target).sum(axis=1)
+    pick = make_screen(ens, ev.target, lo, hi, 3, keep_frac=0.1, disagree_frac=0.0)(cand)
+    rand = rng.choice(cand.shape[0], pick.size, replace=False)
+    assert np.median(true[pick]) < np.median(true[rand])
+    ens.close()
+    ev.close()

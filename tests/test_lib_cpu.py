@@ -12,7 +12,7 @@ from neurogpu_amd import _lib
 
 def _declared():
     hdr = open(os.path.join(ROOT, "include", "neurogpu.h")).read()
-    return sorted(set(re.findall(r"\b(ngpu_[a-z_]+)\s*\(", hdr)))
+    return sorted(set(re.findall(r"\b(ngpu_[a-z_0-9]+)\s*\(", hdr)))
 
 
 def test_header_declarations_match_binding():

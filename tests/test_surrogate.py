@@ -1,0 +1,57 @@
+"""Surrogate network: the BF16 MFMA GEMM, forward/backward against the CPU
+restatement, training, and what the trained network gains for the GA."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("ta,tb", [(False, True), (False, False), (True, False), (True, True)])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 16), (100, 37, 12), (257, 256, 256), (5, 300, 33)])
+def test_gemm_bf16_against_float64(ta, tb, M, N, K):
+    from neurogpu_amd.surrogate import from_bf16_bits, gemm_bf16, to_bf16_bits
+    rng = np.random.default_rng(M * 7 + N + K)
+    A = to_bf16_bits(rng.normal(size=(K, M) if ta else (M, K)).astype(np.float32))
+    B = to_bf16_bits(rng.normal(size=(N, K) if tb else (K, N)).astype(np.float32))
+    C = gemm_bf16(A, B, ta, tb, M, N, K)
+    a = from_bf16_bits(A).astype(np.float64)
+    b = from_bf16_bits(B).astype(np.float64)
+    ref = (a.T if ta else a) @ (b.T if tb else b)
+    np.testing.assert_allclose(C, ref, rtol=1e-5, atol=1e-5 * np.sqrt(K))
+
+
+def test_gradients_match_cpu_restatement():
+    from neurogpu_amd.surrogate import MLPEnsemble
+    from oracle import mlp_ref
+    rng = np.random.default_rng(1)
+    ens = MLPEnsemble(12, 20, hidden=256, members=2, seed=3)
+    x = rng.uniform(-1, 1, size=(300, 12)).astype(np.float32)
+    t = rng.normal(size=(300, 20)).astype(np.float32)
+    w = ens.get_weights()
+    for mem in range(2):
+        g, loss = ens.gradients(mem, x, t)
+        params = ens.unpack(w, mem)
+        ref, rloss = mlp_ref.gradients(params, x, t)
+        assert abs(loss - rloss) < 1e-3 * abs(rloss)
+        gd = ens.unpack(np.concatenate([np.zeros(mem * ens.nparam, np.float32), g]), mem)
+        for k, (a, b) in enumerate(zip(gd, ref)):
+            err = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+            assert err < 2e-2, (mem, k, err)
+    ens.close()
+
+
+def test_training_fits_a_smooth_function():
+    from neurogpu_amd.surrogate import MLPEnsemble
+    rng = np.random.default_rng(2)
+    x = rng.uniform(-1, 1, size=(4096, 12)).astype(np.float32)
+    y = np.stack([np.sin(2 * x[:, 0]) + x[:, 1] * x[:, 2], np.exp(x[:, 3]) - x[:, 4]], axis=1)
+    ens = MLPEnsemble(12, 2, members=2, seed=0)
+    losses = ens.fit(x, y, epochs=40, batch=256, lr=2e-3)
+    assert losses[:, -1].max() < 0.2 * losses[:, 0].min()
+    xt = rng.uniform(-1, 1, size=(1024, 12)).astype(np.float32)
+    yt = np.stack([np.sin(2 * xt[:, 0]) + xt[:, 1] * xt[:, 2], np.exp(xt[:, 3]) - xt[:, 4]], axis=1)
+    mu, sd = ens.predict(xt)
+    r2 = 1 - ((mu - yt) ** 2).sum(0) / ((yt - yt.mean(0)) ** 2).sum(0)
+    assert np.all(r2 > 0.8), r2
+    assert mu.shape == (1024, 2) and np.all(sd >= 0)
+    ens.close()
