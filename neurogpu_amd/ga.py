@@ -167,7 +167,7 @@ class GAResult:
 
 def run_ga(evaluator: Evaluator, mu: int, lam: int, ngen: int, seed: int = 1178,
            screen: Optional[Callable] = None, oversample: int = 1, time_limit: Optional[float] = None,
-           stop_score: Optional[float] = None) -> GAResult:
+           stop_score: Optional[float] = None, log: Optional[Callable] = None) -> GAResult:
     """(mu + lambda) GA in log10 parameter space.  With ``screen`` the GA draws
     ``oversample * lam`` offspring, and ``screen(genomes)`` returns the indices
     of those to simulate (the surrogate pre-screen); every score that enters
@@ -191,6 +191,8 @@ def run_ga(evaluator: Evaluator, mu: int, lam: int, ngen: int, seed: int = 1178,
         keep = np.argsort(alls, kind="stable")[:mu]
         pop, score = allx[keep], alls[keep]
         hist.append((gen, time.perf_counter() - t0, float(score.min()), evaluator.nevals))
+        if log is not None:
+            log(f"gen {gen}: best {hist[-1][2]:.4f} after {hist[-1][1]:.1f} s, {evaluator.nevals} evaluations")
         if time_limit is not None and hist[-1][1] > time_limit:
             break
         if stop_score is not None and score.min() <= stop_score:
