@@ -8,7 +8,7 @@ the 13 BBP TTPC mechanisms), 8,192 parameter sets x 5 step currents,
 dt = 0.025 ms; a bench step covers its first 1,000 dt steps (the work per
 compartment-step does not depend on the window: the kernel has no
 data-dependent branches besides the rate functions' singularity bumps).
-``--config 1`` measures the active 500-compartment cell over its full 10,000
+``--workload active500`` measures the active 500-compartment cell over its full 10,000
 steps.
 
 Prints one JSON line (the last line of stdout).
@@ -25,6 +25,7 @@ sys.path.insert(0, _ROOT)
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
 METRIC = "compartment-steps/s at FP64 oracle <=1e-6 mV"
 
 
@@ -46,6 +47,56 @@ def traffic_from_profile(name: str):
     return None
 
 
+def fp64_from_profile(name: str):
+    """Issued FP64 flops per compartment-step from the rocprofv3 FP64
+    instruction-count pass (profiles/<name>_fp64.txt, line
+    'fp64_flop_per_comp_step: <value>'), if present."""
+    path = os.path.join(_ROOT, "profiles", f"{name}_fp64.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("fp64_flop_per_comp_step:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def copy_peak_gbs(mib: int = 2048, reps: int = 10) -> float:
+    """Measured device-to-device copy bandwidth (read + write bytes / time), in
+    a child process: torch ships its own HIP runtime, which must not share a
+    process with the one libneurogpu links."""
+    import subprocess
+    out = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-peak", str(mib), "--steps", str(reps)],
+                         capture_output=True, text=True, timeout=300)
+    if out.returncode != 0:
+        raise RuntimeError(f"copy peak measurement failed: {out.stderr[-2000:]}")
+    return float(json.loads(out.stdout.strip().splitlines()[-1])["copy_peak_GBs"])
+
+
+def _copy_peak_child(mib: int, reps: int) -> None:
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    s.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e.record()
+    e.synchronize()
+    print(json.dumps({"copy_peak_GBs": 2.0 * n * 8 * reps / (s.elapsed_time(e) * 1e-3) / 1e9}))
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(w, nsets: int, threads: int):
     """The FP64 oracle on a bounded sample of the same workload."""
     from oracle import cpu_ref
@@ -59,12 +110,17 @@ def cpu_baseline(w, nsets: int, threads: int):
     return ref, cs / el, el
 
 
+def spike_counts(tr, thr: float = -30.0):
+    """Upward crossings of thr along the last axis."""
+    return np.sum((tr[..., 1:] >= thr) & (tr[..., :-1] < thr), axis=-1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--workload", default="ttpc1200", choices=("passive64", "active500", "ttpc1200"))
     ap.add_argument("--nsets", type=int, default=None)
     ap.add_argument("--nt", type=int, default=None)
     ap.add_argument("--waves-per-run", type=int, default=0)
@@ -72,9 +128,14 @@ def main():
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--cpu-sets", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-copy-peak", action="store_true")
+    ap.add_argument("--copy-peak", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_peak:
+        _copy_peak_child(args.copy_peak, args.steps)
+        return
     if args.copy_probe:
         import torch
         n = args.copy_probe * (1 << 20) // 8
@@ -93,8 +154,8 @@ def main():
     from neurogpu_amd import models
     from neurogpu_amd._lib import Simulator
 
-    nt = args.nt if args.nt is not None else (1000 if args.config == 2 else None)
-    w = models.config(args.config, nsets=args.nsets, nt=nt)
+    nt = args.nt if args.nt is not None else (1000 if args.workload == "ttpc1200" else None)
+    w = models.workload(args.workload, nsets=args.nsets, nt=nt)
     m = w.model
     print(f"bench: {w.name} N={m.tree.N} runs={w.nruns} nt={w.dts.size} S={m.state_count():.2f}", flush=True)
     sim = Simulator(0)
@@ -122,10 +183,20 @@ def main():
     bcs = algorithmic_bytes_per_comp_step(m)
     alg_bytes_per_launch = comp_steps / max(1, st["launches"]) * bcs
     achieved = alg_bytes_per_launch / (launch_ms * 1e-3) / 1e9
-    traffic = traffic_from_profile(f"config{args.config}")
+    traffic = traffic_from_profile(args.workload)
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "bytes_per_comp_step": round(bcs, 3), "kernel_ms_per_launch": round(launch_ms, 3)}
+                "bytes_per_comp_step": round(bcs, 3), "kernel_ms_per_launch": round(launch_ms, 3),
+                "comp_steps_per_launch": comp_steps / max(1, st["launches"])}
+    if traffic:
+        roofline["actual_GBs"] = round(traffic / (launch_ms * 1e-3) / 1e9, 1)
+    if not args.no_copy_peak:
+        roofline["copy_peak_GBs"] = round(copy_peak_gbs(), 1)
+    flop_cs = fp64_from_profile(args.workload)
+    if flop_cs:
+        tf = flop_cs * comp_steps / max(1, st["launches"]) / (launch_ms * 1e-3) / 1e12
+        roofline["fp64"] = {"issued_flop_per_comp_step": flop_cs, "achieved": round(tf, 2),
+                            "peak": FP64_PEAK_TFS, "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFS, 4)}
 
     result = {
         "metric": METRIC, "value": value, "unit": "comp-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -140,14 +211,23 @@ def main():
     }
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
-        cpu_sets = args.cpu_sets if args.cpu_sets is not None else (48 if args.config == 2 else 8)
+        cpu_sets = args.cpu_sets if args.cpu_sets is not None else (48 if args.workload == "ttpc1200" else 8)
         nsets = min(cpu_sets, w.x.shape[0])
         ref, cps, el = cpu_baseline(w, nsets, threads)
         d = float(np.max(np.abs(gpu[:nsets] - ref)))
+        same_spikes = bool(np.array_equal(spike_counts(gpu[:nsets]), spike_counts(ref)))
+        one_sets = max(1, nsets // threads)
+        _, cps1, el1 = cpu_baseline(w, one_sets, 1)
         result["cpu_baseline"] = {"value": cps, "unit": "comp-steps/s", "cores": threads, "kind": "port",
                                   "sample": f"{nsets} sets x {w.amps.shape[0]} stimuli x {w.dts.size} steps "
-                                            f"of the same workload ({el:.1f} s)"}
+                                            f"of the same workload ({el:.1f} s, OpenMP over runs)",
+                                  "cpu": cpu_model(), "nproc": os.cpu_count(),
+                                  "one_thread": {"value": cps1, "sample": f"{one_sets} sets ({el1:.1f} s)"}}
         result["parity_sample_max_abs_mV"] = d
+        result["parity_sample_spike_counts_equal"] = same_spikes
+        if d > 1e-6 or not same_spikes:
+            print(f"PARITY FAILURE on the sample: max |dv| {d} mV, spike counts equal: {same_spikes}",
+                  file=sys.stderr)
     print(json.dumps(result), flush=True)
 
 

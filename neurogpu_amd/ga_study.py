@@ -1,6 +1,6 @@
 """Plain GA vs surrogate-screened GA on one GPU, both scored by simulation.
 
-    python -m neurogpu_amd.ga_study [--config 2] [--pop 1024] [--gens 25] ...
+    python -m neurogpu_amd.ga_study [--workload ttpc1200] [--pop 1024] [--gens 25] ...
 
 The plain (mu + lambda) GA runs ``--gens`` generations; its best simulated
 score S* and wall time T are recorded.  The screened GA first simulates
@@ -22,13 +22,13 @@ from . import ga, models
 from .surrogate import MLPEnsemble, feature_targets, genome_to_input, make_screen
 
 
-def study(config=2, pop=1024, gens=25, train=8192, nt=2000, members=5, epochs=60, seed=1178, device=0,
+def study(name="ttpc1200", pop=1024, gens=25, train=8192, nt=2000, members=5, epochs=60, seed=1178, device=0,
           max_screen_gens=None):
-    w = models.config(config, nsets=1, nt=nt)
+    w = models.workload(name, nsets=1, nt=nt)
     m = w.model
     dt = float(w.dts[0])
     t_on, t_off = 5.0, 45.0 if nt * dt >= 45.0 else nt * dt
-    amps = {1: [0.2, 0.5, 1.0], 2: [0.1, 0.3, 0.5, 0.7, 0.9]}.get(config, [0.5])
+    amps = {"active500": [0.2, 0.5, 1.0], "ttpc1200": [0.1, 0.3, 0.5, 0.7, 0.9]}.get(name, [0.5])
     w.amps, w.dts = models.step_stimulus(amps, nt, dt, t_on, t_off)
     nstim = len(amps)
     ev = ga.Evaluator(w, t_on, t_off, device=device, target_x=m.nominal())
@@ -75,7 +75,7 @@ def study(config=2, pop=1024, gens=25, train=8192, nt=2000, members=5, epochs=60
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--config", type=int, default=2)
+    ap.add_argument("--workload", default="ttpc1200", choices=models.WORKLOADS)
     ap.add_argument("--pop", type=int, default=1024)
     ap.add_argument("--gens", type=int, default=25)
     ap.add_argument("--train", type=int, default=8192)
@@ -83,7 +83,7 @@ def main():
     ap.add_argument("--members", type=int, default=5)
     ap.add_argument("--epochs", type=int, default=60)
     args = ap.parse_args()
-    r = study(args.config, args.pop, args.gens, args.train, args.nt, args.members, args.epochs)
+    r = study(args.workload, args.pop, args.gens, args.train, args.nt, args.members, args.epochs)
     print(json.dumps(r), flush=True)
 
 

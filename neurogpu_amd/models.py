@@ -444,26 +444,31 @@ class Workload:
         return float(self.nruns) * self.model.tree.N * self.dts.size
 
 
-def config(idx: int, nsets: Optional[int] = None, nt: Optional[int] = None, seed: Optional[int] = None) -> Workload:
-    """The benchmark configurations (seed 1000 + index unless given):
-    0 passive tree, 64 compartments, 32 instances, 0.5 nA, dt 0.025 ms, 100 ms;
-    1 active ~500-compartment cell, 1024 instances x (0.2, 0.5, 1.0) nA, dt 0.02 ms, 200 ms;
-    2 TTPC-like ~1200-compartment cell, 8192 instances x 5 amplitudes 0.1-0.9 nA, dt 0.025 ms, 300 ms.
-    ``nsets`` / ``nt`` shrink a configuration for tests."""
-    seed = 1000 + idx if seed is None else seed
-    if idx == 0:
+WORKLOADS = ("passive64", "active500", "ttpc1200")
+_SEEDS = {"passive64": 1000, "active500": 1001, "ttpc1200": 1002}
+
+
+def workload(name: str, nsets: Optional[int] = None, nt: Optional[int] = None, seed: Optional[int] = None) -> Workload:
+    """The benchmark workloads (default seeds 1000, 1001, 1002):
+    passive64  passive tree, 64 compartments, 32 instances, 0.5 nA, dt 0.025 ms, 100 ms;
+    active500  active ~500-compartment cell, 1024 instances x (0.2, 0.5, 1.0) nA, dt 0.02 ms, 200 ms;
+    ttpc1200   TTPC-like ~1200-compartment cell, 8192 instances x 5 amplitudes 0.1-0.9 nA,
+               dt 0.025 ms, 300 ms.
+    ``nsets`` / ``nt`` shrink a workload for tests."""
+    if name not in _SEEDS:
+        raise ValueError(f"no workload {name!r}; known: {', '.join(WORKLOADS)}")
+    seed = _SEEDS[name] if seed is None else seed
+    if name == "passive64":
         model = passive_model(seed)
         n, T, dt, amps, on, off = 32, 4000, 0.025, [0.5], 10.0, 90.0
-    elif idx == 1:
+    elif name == "active500":
         model = active_model(seed)
         n, T, dt, amps, on, off = 1024, 10000, 0.02, [0.2, 0.5, 1.0], 20.0, 180.0
-    elif idx == 2:
+    else:
         model = ttpc_model(seed)
         n, T, dt, amps, on, off = 8192, 12000, 0.025, [0.1, 0.3, 0.5, 0.7, 0.9], 20.0, 280.0
-    else:
-        raise ValueError(f"no configuration {idx}")
     n = n if nsets is None else nsets
     T = T if nt is None else nt
     x = model.sample(n, seed)
     a, d = step_stimulus(amps, T, dt, on, off)
-    return Workload(f"config{idx}:{model.name}", model, x, a, d)
+    return Workload(name, model, x, a, d)

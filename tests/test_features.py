@@ -51,7 +51,7 @@ def test_device_features_match_oracle():
     from neurogpu_amd import models
     from neurogpu_amd._lib import Simulator
     from oracle import cpu_ref
-    w = models.config(1, nsets=6, nt=2500)
+    w = models.workload("active500", nsets=6, nt=2500)
     m = w.model
     with Simulator(0) as sim:
         sim.set_tree(m.tree, m.mechset_id)

@@ -41,7 +41,7 @@ def test_ga_improves_and_surrogate_screen_beats_random():
     random pick of the same size (the gain the network brings to the GA)."""
     from neurogpu_amd import models
     from neurogpu_amd.surrogate import MLPEnsemble, feature_targets, genome_to_input, make_screen
-    w = models.config(1, nsets=1, nt=2000)           # 40 ms protocol
+    w = models.workload("active500", nsets=1, nt=2000)           # 40 ms protocol
     m = w.model
     a, d = models.step_stimulus([0.2, 0.5, 1.0], 2000, 0.02, 5.0, 35.0)
     w.amps, w.dts = a, d

@@ -37,8 +37,8 @@ def _check(gpu, ref, what):
     return d
 
 
-def test_passive_config_full_size():
-    w = models.config(0)
+def test_passive64_full_size():
+    w = models.workload("passive64")
     _say(f"case passive64 full: {w.nruns} runs x {w.dts.size} steps")
     gpu, st = simulate_workload(w)
     _check(gpu, _oracle(w), "passive64")
@@ -46,16 +46,16 @@ def test_passive_config_full_size():
 
 @pytest.mark.parametrize("launch", [None, dict(waves_per_run=2), dict(runs_per_block=1),
                                     dict(steps_per_launch=700), dict(steps_per_launch=500, use_graph=True)])
-def test_active_config_small(launch):
-    w = models.config(1, nsets=6, nt=1500)
+def test_active500_small(launch):
+    w = models.workload("active500", nsets=6, nt=1500)
     _say(f"case active500 small launch={launch}")
     gpu, st = simulate_workload(w, launch=launch)
     _check(gpu, _oracle(w), f"active500 {launch}")
 
 
 @pytest.mark.parametrize("launch", [None, dict(waves_per_run=4), dict(waves_per_run=1, runs_per_block=2)])
-def test_ttpc_config_small(launch):
-    w = models.config(2, nsets=3, nt=1200)
+def test_ttpc1200_small(launch):
+    w = models.workload("ttpc1200", nsets=3, nt=1200)
     _say(f"case ttpc1200 small launch={launch}")
     gpu, st = simulate_workload(w, launch=launch)
     _check(gpu, _oracle(w), f"ttpc1200 {launch}")
@@ -83,14 +83,14 @@ def test_reference_ttpc_fixture_through_file_boundary(tmp_path):
 def test_active_full_length_sampled():
     """Configuration 1 at full length (10,000 steps), 16 parameter sets, all
     three stimuli, every run checked."""
-    w = models.config(1, nsets=16)
+    w = models.workload("active500", nsets=16)
     _say("case active500 full length, 16 sets")
     gpu, st = simulate_workload(w)
     _check(gpu, _oracle(w), "active500 full length")
 
 
 def test_determinism_and_identical_instances():
-    w = models.config(1, nsets=8, nt=800)
+    w = models.workload("active500", nsets=8, nt=800)
     w.x[4:] = w.x[:4]
     _say("case determinism")
     a, _ = simulate_workload(w)
@@ -100,7 +100,7 @@ def test_determinism_and_identical_instances():
 
 
 def test_error_paths():
-    w = models.config(1, nsets=2, nt=10)
+    w = models.workload("active500", nsets=2, nt=10)
     m = w.model
     with Simulator(0) as sim:
         with pytest.raises(NgpuError):
@@ -123,7 +123,7 @@ def test_error_paths():
 
 
 def test_nan_parameter_poisons_only_its_run():
-    w = models.config(1, nsets=3, nt=300)
+    w = models.workload("active500", nsets=3, nt=300)
     P = w.params
     P[1, :] = np.nan
     m = w.model

@@ -41,9 +41,9 @@ def test_solve_has_tiny_backward_error(name):
         assert np.max(r / scale) < 1e-13, mode
 
 
-@pytest.mark.parametrize("idx", [0, 1, 2])
-def test_solve_matches_dense_solve_on_generated_trees(idx):
-    t = models.config(idx, nsets=1, nt=1).model.tree
+@pytest.mark.parametrize("idx,name", list(enumerate(models.WORKLOADS)))
+def test_solve_matches_dense_solve_on_generated_trees(idx, name):
+    t = models.workload(name, nsets=1, nt=1).model.tree
     rng = np.random.default_rng(idx)
     u = rng.uniform(1.0, 2.0, t.N)
     A0 = _matrix(t, np.zeros(t.N))
@@ -83,7 +83,7 @@ def test_single_compartment_backward_euler_closed_form():
 
 
 def test_rest_is_a_fixed_point_bitwise():
-    w = models.config(0, nsets=3, nt=400)
+    w = models.workload("passive64", nsets=3, nt=400)
     m = w.model
     P = m.allparams(w.x)
     vh = cpu_ref.simulate(m.tree, m.mechset_id, P, np.zeros_like(w.amps), w.dts, m.stim_seg, m.stim_area,
@@ -92,7 +92,7 @@ def test_rest_is_a_fixed_point_bitwise():
 
 
 def test_identical_instances_identical_traces():
-    w = models.config(1, nsets=2, nt=600)
+    w = models.workload("active500", nsets=2, nt=600)
     m = w.model
     x = np.vstack([w.x[0], w.x[0]])
     vh = cpu_ref.simulate(m.tree, m.mechset_id, m.allparams(x), w.amps, w.dts, m.stim_seg, m.stim_area,
@@ -102,7 +102,7 @@ def test_identical_instances_identical_traces():
 
 def test_passive_linearity():
     """A passive cell at rest responds linearly: trace(2I) - rest = 2 (trace(I) - rest)."""
-    w = models.config(0, nsets=2, nt=800)
+    w = models.workload("passive64", nsets=2, nt=800)
     m = w.model
     P = m.allparams(w.x)
     a1 = w.amps
@@ -114,7 +114,7 @@ def test_passive_linearity():
 def test_dense_step_equivalence_passive():
     """One step of the whole passive assembly equals a dense backward-Euler
     step of the cable system (NEURON discretisation)."""
-    w = models.config(0, nsets=1, nt=2)
+    w = models.workload("passive64", nsets=1, nt=2)
     m = w.model
     t = m.tree
     g, E = w.x[0, 0], -70.0
