@@ -394,7 +394,12 @@ static void choose_layout(const ngpu_ctx *c, int &W, int &slot)
 {
     const int N = c->N;
     W = c->cfg_w;
-    if (W <= 0) W = std::max(1, (N + 64 * 8 - 1) / (64 * 8));
+    if (W <= 0) {
+        /* 1, 2, 4 or 8 waves: at 2 waves per SIMD (the register budget of the
+         * mechanism code) a CU holds 8 waves, which these divide evenly */
+        W = 1;
+        while (W < 8 && 64 * 8 * W < N) W *= 2;
+    }
     W = std::min(W, 8);
     int need = (N + 64 * W - 1) / (64 * W);
     slot = 8;
@@ -488,12 +493,13 @@ static int enqueue(ngpu_ctx *c, int &launches)
     const int N = c->N, NQ = c->lay_NQ, LW = 64 * c->lay_W;
     const int nruns = c->nsets * c->nstim;
     int R = 1;
-    if (c->lay_W == 1) R = c->cfg_r > 0 ? c->cfg_r : 1;
+    if (c->lay_W == 1) R = c->cfg_r > 0 ? c->cfg_r : 4;   /* 4 single-wave runs per workgroup */
     const int npv = c->nvar <= 512 ? std::max(c->nvar, 1) : 0;
     const size_t per_run = (size_t)(2 * (N + 1) + npv + (int)c->rec.size() * 64) * sizeof(double);
     const size_t nshort = (size_t)c->logn * N + c->chains.size() + c->fath.size() + c->fchild.size() +
                           (size_t)(c->depth + 2) + (size_t)(c->depth + 1);
-    const size_t shared = (size_t)2 * N * sizeof(double) + ((nshort * sizeof(short) + 7) / 8) * 8;
+    const size_t shared = (size_t)2 * N * sizeof(double) +
+                          (((size_t)NQ * sizeof(int) + (nshort + 2 * (size_t)NQ) * sizeof(short) + 7) / 8) * 8;
     while (R > 1 && shared + per_run * R > 160 * 1024) R--;
     const size_t lds = shared + per_run * R;
     if (lds > 160 * 1024) return fail(c, NGPU_E_ARG, "tree needs %zu bytes of LDS per block (limit 160 KiB)", lds);

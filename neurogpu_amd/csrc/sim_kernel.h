@@ -16,6 +16,13 @@
  *   4. back-substitution by pointer jumping over the ancestor tables;
  *   5. v += x.
  *
+ * Per-lane values of the lane's positions (v, dv, ica) live in registers and
+ * are only ever indexed with compile-time constants: the slot loop of phase 1
+ * is not unrolled (the mechanism code is large), so it works on index 0 and
+ * rotates the three arrays by one position per slot.  A run-time index would
+ * put the arrays in scratch memory.  Position tables (row, parent row,
+ * mechanism mask) are block-shared in LDS.
+ *
  * Levels are synchronised at wavefront scope when W == 1 (no barrier
  * instruction: a wave's LDS operations complete in issue order, so only the
  * compiler is fenced) and at workgroup scope when W > 1.
@@ -97,29 +104,6 @@ struct ParamFetch {
     }
 };
 
-template <int NS>
-__device__ __forceinline__ double pick(const double (&a)[NS], int c)
-{
-    double r = a[0];
-#pragma unroll
-    for (int i = 1; i < NS; i++) r = (c == i) ? a[i] : r;
-    return r;
-}
-template <int NS>
-__device__ __forceinline__ int pick(const int (&a)[NS], int c)
-{
-    int r = a[0];
-#pragma unroll
-    for (int i = 1; i < NS; i++) r = (c == i) ? a[i] : r;
-    return r;
-}
-template <int NS>
-__device__ __forceinline__ void place(double (&a)[NS], int c, double v)
-{
-#pragma unroll
-    for (int i = 0; i < NS; i++) a[i] = (c == i) ? v : a[i];
-}
-
 template <class SET, int NSLOT, bool MULTI>
 #ifndef NG_WAVES_PER_EU
 #define NG_WAVES_PER_EU 2
@@ -133,18 +117,27 @@ ngpu_sim_kernel(SimArgs a)
     const int g = threadIdx.x - group * LW;
     const int N = a.N, NQ = a.NQ;
     /* block-shared, staged once: coupling coefficients by Hines row, the
-     * level schedule and the ancestor tables (int16 row indices) */
+     * position tables, the level schedule and the ancestor tables (int16 row
+     * indices; -1 marks a padding position) */
     const int npv = a.pv_in_lds ? a.nvar : 0;
     const int per_run = 2 * (N + 1) + npv + a.nrec * 64;
     double *RE = lds, *RF = lds + N;
-    short *SH = (short *)(lds + 2 * N + (size_t)a.runs_per_block * per_run);
-    short *FIDX = SH;
+    int *QM = (int *)(lds + 2 * N + (size_t)a.runs_per_block * per_run);
+    short *QH = (short *)(QM + NQ);
+    short *QP = QH + NQ;
+    short *FIDX = QP + NQ;
     short *CH = FIDX + a.logn * N;
     short *FA = CH + 2 * a.nchains;
     short *FC = FA + 3 * a.nfath;
     short *LP = FC + a.nfchild;
     short *FLP = LP + (a.depth + 2);
     for (int i = threadIdx.x; i < N; i += blockDim.x) { RE[i] = a.row_e[i]; RF[i] = a.row_f[i]; }
+    for (int i = threadIdx.x; i < NQ; i += blockDim.x) {
+        const bool in = i < N;
+        QM[i] = in ? a.q_meta[i] : 0;
+        QH[i] = (short)(in ? a.q_h[i] : -1);
+        QP[i] = (short)(in ? a.q_ph[i] : 0);
+    }
     for (int i = threadIdx.x; i < a.logn * N; i += blockDim.x) FIDX[i] = (short)a.fidx[i];
     for (int i = threadIdx.x; i < 2 * a.nchains; i += blockDim.x) CH[i] = (short)a.chains[i];
     for (int i = threadIdx.x; i < 3 * a.nfath; i += blockDim.x) FA[i] = (short)a.fath[i];
@@ -168,23 +161,17 @@ ngpu_sim_kernel(SimArgs a)
         pvsrc = PV;
     }
 
-    int qh[NSLOT], qph[NSLOT], qm[NSLOT];
     double v[NSLOT], dv[NSLOT], ica[NSLOT];
 #pragma unroll
-    for (int c = 0; c < NSLOT; c++) {
-        const int q = c * LW + g;
-        qh[c] = -1; qph[c] = 0; qm[c] = 0; v[c] = 0.0; dv[c] = 0.0; ica[c] = 0.0;
-        if (q < N) { qh[c] = a.q_h[q]; qph[c] = a.q_ph[q]; qm[c] = a.q_meta[q]; }
-    }
+    for (int c = 0; c < NSLOT; c++) { v[c] = (c * LW + g < N) ? a.v_init : 0.0; dv[c] = 0.0; ica[c] = 0.0; }
     if (a.t0 == 0) {
         for (int i = g; i <= N; i += LW) { X[i] = 0.0; U[i] = 0.0; }
 #pragma unroll 1
         for (int c = 0; c < NSLOT; c++) {
             const int q = c * LW + g;
             if (q >= N) continue;
-            const unsigned m = (unsigned)pick(qm, c) & 0xffffffu;
+            const unsigned m = (unsigned)QM[q] & 0xffffffu;
             const ParamFetch P{a.pconst, a.pval, pvsrc, a.pconst_mask, q, NQ};
-            place(v, c, a.v_init);
             double st[SET::NSTATE];
 #pragma unroll
             for (int k = 0; k < SET::NSTATE; k++) st[k] = 0.0;
@@ -216,65 +203,65 @@ ngpu_sim_kernel(SimArgs a)
         /* record v before the step (CudaStuff.cu:257-261) */
         for (int r = 0; r < a.nrec; r++) {
             const int rq = a.rec_q[r];
-            if (rq - (rq / LW) * LW == g) REC[r * 64 + slot_t] = pick(v, rq / LW);
+#pragma unroll
+            for (int c = 0; c < NSLOT; c++)
+                if (c * LW + g == rq) REC[r * 64 + slot_t] = v[c];
         }
 
         /* dv += x[parent] - x[self] with the previous solution (CudaStuff.cu:287) */
 #pragma unroll
-        for (int c = 0; c < NSLOT; c++)
-            if (qh[c] >= 0) dv[c] += X[qph[c]] - X[qh[c]];
+        for (int c = 0; c < NSLOT; c++) {
+            const int q = c * LW + g, h = QH[q];
+            if (h >= 0) dv[c] += X[QP[q]] - X[h];
+        }
         group_sync<MULTI>();
 
         /* fused cnexp(t-1) + BREAKPOINT(t) + Hines row (CudaStuff.cu:263-294).
          * eca is stored by INITIAL only: at t = 0 it is loaded, afterwards it is
-         * recomputed from cai after the cnexp pass.  The next slot's states are
-         * requested before the current slot is computed. */
+         * recomputed from cai after the cnexp pass. */
         const unsigned keep = (t > 0) ? ~(1u << SET::ECA) : ~0u;
-        double stn[SET::NSTATE];
-        {
-            const unsigned sb0 = SET::state_bits((unsigned)qm[0] & 0xffffffu) & keep;
-#pragma unroll
-            for (int k = 0; k < SET::NSTATE; k++) stn[k] = (g < N && (sb0 & (1u << k))) ? S[k * NQ + g] : 0.0;
-        }
 #pragma unroll 1
         for (int c = 0; c < NSLOT; c++) {
+            /* slot c's values are at index 0: the arrays rotate by one per slot */
             const int q = c * LW + g;
+            const double vq = v[0], dvq = dv[0];
+            double icaq = ica[0];
             double st[SET::NSTATE];
+            {
+                const unsigned sbn = SET::state_bits((unsigned)QM[q] & 0xffffffu) & keep;
 #pragma unroll
-            for (int k = 0; k < SET::NSTATE; k++) st[k] = stn[k];
-            if (c + 1 < NSLOT) {
-                const int qn = q + LW;
-                const unsigned sbn = SET::state_bits((unsigned)pick(qm, c + 1) & 0xffffffu) & keep;
-#pragma unroll
-                for (int k = 0; k < SET::NSTATE; k++) stn[k] = (qn < N && (sbn & (1u << k))) ? S[k * NQ + qn] : 0.0;
+                for (int k = 0; k < SET::NSTATE; k++) st[k] = (q < N && (sbn & (1u << k))) ? S[k * NQ + q] : 0.0;
             }
-            if (q >= N) continue;
-            const int h = pick(qh, c);
-            const unsigned m = (unsigned)pick(qm, c) & 0xffffffu;
-            const double vq = pick(v, c);
-            const double cmq = a.q_cm[q];
-            const double fh = RF[h];
-            const unsigned sb = SET::state_bits(m) & keep;
-            const ParamFetch P{a.pconst, a.pval, pvsrc, a.pconst_mask, q, NQ};
-            if (t > 0) {
-                SET::cnexp(m, vq, dtp, st, pick(ica, c), P);
-                SET::eca_after_cnexp(m, st);
+            if (q < N) {
+                const int h = QH[q];
+                const unsigned m = (unsigned)QM[q] & 0xffffffu;
+                const double cmq = a.q_cm[q];
+                const double fh = RF[h];
+                const unsigned sb = SET::state_bits(m) & keep;
+                const ParamFetch P{a.pconst, a.pval, pvsrc, a.pconst_mask, q, NQ};
+                if (t > 0) {
+                    SET::cnexp(m, vq, dtp, st, icaq, P);
+                    SET::eca_after_cnexp(m, st);
 #pragma unroll
-                for (int k = 0; k < SET::NSTATE; k++)
-                    if (sb & (1u << k)) S[k * NQ + q] = st[k];
+                    for (int k = 0; k < SET::NSTATE; k++)
+                        if (sb & (1u << k)) S[k * NQ + q] = st[k];
+                }
+                double sumDv = 0.0, sum = 0.0, icaDv = 0.0, icaNow = 0.0;
+                SET::current(m, vq + 0.001, st, P, sumDv, icaDv, false);
+                SET::current(m, vq, st, P, sum, icaNow, true);
+                icaq = icaNow;
+                const double istim = (q == a.stim_q) ? 100 * amp / a.stim_area : 0.0;
+                const double gModel = (sumDv - sum) / NG_EPS_V;
+                double rhs = istim - sum;
+                double D = gModel + cmq / (dt * 1000);
+                D -= fh;
+                rhs -= fh * dvq;
+                X[h] = rhs;
+                U[h] = D;
             }
-            double sumDv = 0.0, sum = 0.0, icaDv = 0.0, icaNow = 0.0;
-            SET::current(m, vq + 0.001, st, P, sumDv, icaDv, false);
-            SET::current(m, vq, st, P, sum, icaNow, true);
-            place(ica, c, icaNow);
-            const double istim = (q == a.stim_q) ? 100 * amp / a.stim_area : 0.0;
-            const double gModel = (sumDv - sum) / NG_EPS_V;
-            double rhs = istim - sum;
-            double D = gModel + cmq / (dt * 1000);
-            D -= fh;
-            rhs -= fh * pick(dv, c);
-            X[h] = rhs;
-            U[h] = D;
+#pragma unroll
+            for (int i = 0; i + 1 < NSLOT; i++) { v[i] = v[i + 1]; dv[i] = dv[i + 1]; ica[i] = ica[i + 1]; }
+            v[NSLOT - 1] = vq; dv[NSLOT - 1] = dvq; ica[NSLOT - 1] = icaq;
         }
         group_sync<MULTI>();
         /* children scatter into parents, first children then second children
@@ -283,10 +270,11 @@ ngpu_sim_kernel(SimArgs a)
         for (int son = 1; son <= 2; son++) {
 #pragma unroll
             for (int c = 0; c < NSLOT; c++) {
-                if (qh[c] < 0 || (qm[c] >> 24) != son) continue;
-                const double e = RE[qh[c]];
-                X[qph[c]] += e * dv[c];
-                U[qph[c]] -= e;
+                const int q = c * LW + g, h = QH[q];
+                if (h < 0 || (QM[q] >> 24) != son) continue;
+                const double e = RE[h];
+                X[QP[q]] += e * dv[c];
+                U[QP[q]] -= e;
             }
             group_sync<MULTI>();
         }
@@ -370,7 +358,7 @@ ngpu_sim_kernel(SimArgs a)
         /* v update (CudaStuff.cu:312-315) */
 #pragma unroll
         for (int c = 0; c < NSLOT; c++)
-            if (qh[c] >= 0) v[c] += X[qh[c]];
+            { const int h = QH[c * LW + g]; if (h >= 0) v[c] += X[h]; }
         dtp = dt;
 
         /* flush the recorded chunk */
