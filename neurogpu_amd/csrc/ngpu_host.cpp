@@ -266,6 +266,44 @@ extern "C" int ngpu_set_tree(ngpu_ctx *c, const ngpu_tree *t)
     }
     for (int r = 0; r < N - 1; r++)
         if (elim[r] != 1) return fail(c, NGPU_E_ARG, "tree maps eliminate row %d %d times (expected once)", r + 1, elim[r]);
+    /* Chains become (i0, i1, c0, c1): a father J folded after level L whose
+     * row is the one the chain at level L+1 starts from (J = i0 - 1), and
+     * that no other chain of that level touches, is folded by that chain's
+     * lane just before the chain (fchild[c0..c1) are its related rows).  The
+     * same operations in the same order, one barrier fewer per level.  Other
+     * fathers keep their own phase. */
+    {
+        const int nlev = (int)lvl_ptr.size() - 1;
+        std::vector<int> ch4(chains.size() / 2 * 4);
+        for (size_t k = 0; k < chains.size() / 2; k++) {
+            ch4[4 * k] = chains[2 * k]; ch4[4 * k + 1] = chains[2 * k + 1];
+            ch4[4 * k + 2] = 0; ch4[4 * k + 3] = 0;
+        }
+        std::vector<int> rest, rest_ptr{0};
+        for (int L = 0; L + 1 < nlev; L++) {
+            const int k0 = lvl_ptr[L + 1], k1 = lvl_ptr[L + 2];
+            for (int f = flvl_ptr[L]; f < flvl_ptr[L + 1]; f++) {
+                const int J = fath[3 * f], c0 = fath[3 * f + 1], c1 = fath[3 * f + 2];
+                int owner = -1, touching = 0;
+                bool clash = false;
+                for (int k = k0; k < k1; k++) {
+                    const int i0 = chains[2 * k], i1 = chains[2 * k + 1];
+                    if (J >= i0 - 1 && J < i1) { touching++; if (J == i0 - 1 && ch4[4 * k + 3] == 0) owner = k; }
+                    for (int cc = c0; cc < c1; cc++)
+                        if (fchild[cc] >= i0 && fchild[cc] < i1) clash = true;
+                }
+                if (owner >= 0 && touching == 1 && !clash) {
+                    ch4[4 * owner + 2] = c0; ch4[4 * owner + 3] = c1;
+                } else {
+                    rest.push_back(J); rest.push_back(c0); rest.push_back(c1);
+                }
+            }
+            rest_ptr.push_back((int)rest.size() / 3);
+        }
+        chains.swap(ch4);
+        fath.swap(rest);
+        flvl_ptr.swap(rest_ptr);
+    }
     std::vector<int> fidx((size_t)t->logn_depth * N);
     for (size_t i = 0; i < fidx.size(); i++) {
         int a = t->fidxs[i] - 1;
@@ -514,7 +552,7 @@ static int enqueue(ngpu_ctx *c, int &launches)
     a.pconst = c->d_pconst.p;
     a.chains = c->d_chains.p; a.lvl_ptr = c->d_lvl.p; a.fath = c->d_fath.p; a.flvl_ptr = c->d_flvl.p;
     a.fchild = c->d_fchild.p; a.fidx = c->d_fidx.p;
-    a.nchains = (int)c->chains.size() / 2; a.nfath = (int)c->fath.size() / 3; a.nfchild = (int)c->fchild.size(); a.dts = c->d_dts.p; a.amps = c->d_amps.p; a.rec_q = c->d_rec.p;
+    a.nchains = (int)c->chains.size() / 4; a.nfath = (int)c->fath.size() / 3; a.nfchild = (int)c->fchild.size(); a.dts = c->d_dts.p; a.amps = c->d_amps.p; a.rec_q = c->d_rec.p;
     a.states = c->d_states.p; a.carry = c->d_carry.p; a.vhot = c->d_vhot.p;
     c->last_R = R;
     dim3 block(LW * R), grid((nruns + R - 1) / R);

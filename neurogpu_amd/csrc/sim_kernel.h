@@ -59,8 +59,8 @@ struct SimArgs {
     unsigned pconst_mask;                /* bit j: parameter j has one value wherever it is used */
     int pv_in_lds;                       /* stage the run's varying values in LDS */
     const double *pconst;                /* [NPARAM] those values */
-    const int *chains, *lvl_ptr;         /* chains: (i0, i1) pairs; lvl_ptr[depth+2] */
-    const int *fath, *flvl_ptr, *fchild; /* fath: (J, c0, c1) triples; flvl_ptr[depth+1] */
+    const int *chains, *lvl_ptr;         /* chains: (i0, i1, c0, c1); lvl_ptr[depth+2] */
+    const int *fath, *flvl_ptr, *fchild; /* fathers with their own phase: (J, c0, c1); flvl_ptr[depth+1] */
     const int *fidx;                     /* [logn*N] 0-based ancestor row, N = beyond root */
     int nchains, nfath, nfchild;         /* schedule sizes (staged into LDS as int16) */
     const double *dts, *amps;
@@ -127,7 +127,7 @@ ngpu_sim_kernel(SimArgs a)
     short *QP = QH + NQ;
     short *FIDX = QP + NQ;
     short *CH = FIDX + a.logn * N;
-    short *FA = CH + 2 * a.nchains;
+    short *FA = CH + 4 * a.nchains;
     short *FC = FA + 3 * a.nfath;
     short *LP = FC + a.nfchild;
     short *FLP = LP + (a.depth + 2);
@@ -139,7 +139,7 @@ ngpu_sim_kernel(SimArgs a)
         QP[i] = (short)(in ? a.q_ph[i] : 0);
     }
     for (int i = threadIdx.x; i < a.logn * N; i += blockDim.x) FIDX[i] = (short)a.fidx[i];
-    for (int i = threadIdx.x; i < 2 * a.nchains; i += blockDim.x) CH[i] = (short)a.chains[i];
+    for (int i = threadIdx.x; i < 4 * a.nchains; i += blockDim.x) CH[i] = (short)a.chains[i];
     for (int i = threadIdx.x; i < 3 * a.nfath; i += blockDim.x) FA[i] = (short)a.fath[i];
     for (int i = threadIdx.x; i < a.nfchild; i += blockDim.x) FC[i] = (short)a.fchild[i];
     for (int i = threadIdx.x; i < a.depth + 2; i += blockDim.x) LP[i] = (short)a.lvl_ptr[i];
@@ -279,28 +279,14 @@ ngpu_sim_kernel(SimArgs a)
             group_sync<MULTI>();
         }
 
-        /* triangularisation by levels (BeforeLU, CudaStuff.cu:31-69) */
+        /* triangularisation by levels (BeforeLU, CudaStuff.cu:31-69).  Level L
+         * first folds the fathers of level L-1 that have their own phase, then
+         * eliminates its chains; a chain (i0, i1, c0, c1) with c0 < c1 first
+         * folds the father row i0-1 over the related rows FC[c0..c1). */
         for (int L = 0; L <= a.depth; L++) {
-            const int k1 = LP[L + 1];
-            for (int k = LP[L] + g; k < k1; k += LW) {
-                const int i0 = CH[2 * k], i1 = CH[2 * k + 1];
-                if (i0 >= i1) continue;
-                double um1 = U[i0 - 1], bm1 = X[i0 - 1];
-                double e1 = RE[i0 - 1], f1 = RF[i0 - 1], ui = U[i0], xi = X[i0];
-                for (int i = i0; i < i1; i++) {
-                    double e1n = 0.0, f1n = 0.0, un = 0.0, xn = 0.0;
-                    if (i + 1 < i1) { e1n = RE[i]; f1n = RF[i]; un = U[i + 1]; xn = X[i + 1]; }
-                    const double unew = ui - f1 * (e1 / um1);
-                    const double bnew = xi - bm1 * e1 / um1;
-                    U[i] = unew; X[i] = bnew;
-                    um1 = unew; bm1 = bnew;
-                    e1 = e1n; f1 = f1n; ui = un; xi = xn;
-                }
-            }
-            group_sync<MULTI>();
-            if (L < a.depth) {
-                const int f1e = FLP[L + 1];
-                for (int k = FLP[L] + g; k < f1e; k += LW) {
+            if (L > 0 && FLP[L] > FLP[L - 1]) {
+                const int f1e = FLP[L];
+                for (int k = FLP[L - 1] + g; k < f1e; k += LW) {
                     const int J = FA[3 * k], c0 = FA[3 * k + 1], c1 = FA[3 * k + 2];
                     double uJ = U[J], bJ = X[J];
                     for (int cc = c0; cc < c1; cc++) {
@@ -313,6 +299,32 @@ ngpu_sim_kernel(SimArgs a)
                 }
                 group_sync<MULTI>();
             }
+            const int k1 = LP[L + 1];
+            for (int k = LP[L] + g; k < k1; k += LW) {
+                const int i0 = CH[4 * k], i1 = CH[4 * k + 1], c0 = CH[4 * k + 2], c1 = CH[4 * k + 3];
+                double um1 = U[i0 - 1], bm1 = X[i0 - 1];
+                if (c0 < c1) {
+                    for (int cc = c0; cc < c1; cc++) {
+                        const int tr = FC[cc];
+                        const double ut = U[tr], bt = X[tr], e1 = RE[tr], f1 = RF[tr];
+                        um1 = um1 - f1 * (e1 / ut);
+                        bm1 = bm1 - bt * e1 / ut;
+                    }
+                    U[i0 - 1] = um1; X[i0 - 1] = bm1;
+                }
+                if (i0 >= i1) continue;
+                double e1 = RE[i0 - 1], f1 = RF[i0 - 1], ui = U[i0], xi = X[i0];
+                for (int i = i0; i < i1; i++) {
+                    double e1n = 0.0, f1n = 0.0, un = 0.0, xn = 0.0;
+                    if (i + 1 < i1) { e1n = RE[i]; f1n = RF[i]; un = U[i + 1]; xn = X[i + 1]; }
+                    const double unew = ui - f1 * (e1 / um1);
+                    const double bnew = xi - bm1 * e1 / um1;
+                    U[i] = unew; X[i] = bnew;
+                    um1 = unew; bm1 = bnew;
+                    e1 = e1n; f1 = f1n; ui = un; xi = xn;
+                }
+            }
+            group_sync<MULTI>();
         }
 
         /* back-substitution by pointer jumping (BKSUB1, CudaStuff.cu:71-103);
