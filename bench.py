@@ -115,12 +115,84 @@ def spike_counts(tr, thr: float = -30.0):
     return np.sum((tr[..., 1:] >= thr) & (tr[..., :-1] < thr), axis=-1)
 
 
+def bench_mixed(args):
+    """The uneven batch: four morphologies of 200 / 700 / 1,500 / 3,000
+    segments, 12,000 / 4,000 / 1,000 / 256 instances, 10,000 / 5,000 / 2,000 /
+    40,000 steps, in one persistent launch through the device work queue; the
+    launch shape of each morphology comes from the autotuner (untimed)."""
+    from neurogpu_amd import autotune, models
+    from neurogpu_amd._lib import Batch, workload_simulator
+    ws = models.mixed_workloads()
+    say = lambda m: print(m, flush=True)
+    choices = {}
+    for w in ws:
+        say(f"tuning {w.name}: N={w.model.tree.N} runs={w.nruns} nt={w.dts.size}")
+        c, _ = autotune.tune(w, log=say)
+        choices[w.name] = c
+    sims = [workload_simulator(w) for w in ws]
+    b = Batch(0)
+    for i, (w, sim) in enumerate(zip(ws, sims)):
+        b.add(sim)
+        b.set_launch(i, **choices[w.name].as_dict())
+    for i in range(args.warmup):
+        st = b.run()
+        say(f"warmup {i}: kernel {st['ms_kernel']:.1f} ms")
+    t0 = time.perf_counter()
+    kms = []
+    for i in range(args.steps):
+        st = b.run()
+        kms.append(st["ms_kernel"])
+        say(f"step {i}: kernel {st['ms_kernel']:.1f} ms")
+    wall = time.perf_counter() - t0
+    infos = [b.member_info(i) for i in range(len(ws))]
+    total = sum(w.comp_steps for w in ws)
+    launch_ms = float(np.mean(kms))
+    alg = sum(w.comp_steps * algorithmic_bytes_per_comp_step(w.model) for w in ws)
+    achieved = alg / (launch_ms * 1e-3) / 1e9
+    per = []
+    for w, info in zip(ws, infos):
+        span = info["last_end_ms"] - info["first_start_ms"]
+        per.append({"morphology": w.name, "segments": w.model.tree.N, "instances": int(w.x.shape[0]),
+                    "steps": int(w.dts.size), "comp_steps": w.comp_steps,
+                    "first_start_ms": round(info["first_start_ms"], 2), "last_end_ms": round(info["last_end_ms"], 2),
+                    "comp_steps_per_s_in_span": w.comp_steps / (span * 1e-3) if span > 0 else None,
+                    "launch": {k: info[k] for k in ("waves_per_run", "runs_per_item", "block_sync", "fidx_in_lds")}})
+    result = {
+        "metric": METRIC, "value": total * args.steps / wall, "unit": "comp-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "mixed4: TTPC mechanisms on 200/700/1500/3000-segment cells, 12000/4000/1000/256 "
+                               "instances, 10000/5000/2000/40000 steps, one persistent launch", "items": st["nitems"],
+                   "workgroups": st["workgroups"], "lds_bytes": st["lds_bytes"]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel_ms_per_launch": round(launch_ms, 3)},
+        "per_morphology": per,
+        "tail_ms": round(max(i["last_end_ms"] for i in infos) - sorted(i["last_end_ms"] for i in infos)[-2], 2),
+    }
+    b.close()
+    if args.separate:
+        sep = {}
+        for w, sim in zip(ws, sims):
+            _, st1 = sim.run(fetch=False)
+            _, st1 = sim.run(fetch=False)
+            sep[w.name] = st1["ms_kernel"]
+            say(f"separate {w.name}: {st1['ms_kernel']:.1f} ms")
+        result["separate_launches_ms"] = sep
+        result["separate_total_ms"] = sum(sep.values())
+    for sim in sims:
+        sim.close()
+    print(json.dumps(result), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="ttpc1200", choices=("passive64", "active500", "ttpc1200"))
+    ap.add_argument("--workload", default="ttpc1200", choices=("passive64", "active500", "ttpc1200", "mixed4"))
+    ap.add_argument("--separate", action="store_true",
+                    help="mixed4: also time each morphology in its own launch, one after another")
     ap.add_argument("--nsets", type=int, default=None)
     ap.add_argument("--nt", type=int, default=None)
     ap.add_argument("--waves-per-run", type=int, default=0)
@@ -153,6 +225,8 @@ def main():
 
     from neurogpu_amd import models
     from neurogpu_amd._lib import Simulator
+    if args.workload == "mixed4":
+        return bench_mixed(args)
 
     nt = args.nt if args.nt is not None else (1000 if args.workload == "ttpc1200" else None)
     w = models.workload(args.workload, nsets=args.nsets, nt=nt)

@@ -126,6 +126,41 @@ const char *ngpu_last_error(const ngpu_ctx *ctx);
 
 void ngpu_close(ngpu_ctx *ctx);
 
+/* ---- Batches: several contexts (morphologies of uneven size and step count,
+ * one mechanism set) simulated by one persistent launch.  No reference
+ * counterpart: the reference runs one morphology per process and launch
+ * (CANON/CudaStuff.cu:487-507).  At most 8 contexts per batch.  Each
+ * workgroup of 8 waves takes items from a device work queue, longest first;
+ * an item is 8 / waves_per_run runs of one member.  Traces and features stay
+ * with each member context (ngpu_fetch_vhot, ngpu_features). */
+typedef struct ngpu_batch ngpu_batch;
+typedef struct {
+    int waves_per_run;        /* 1, 2, 4 or 8; 0 = the fewest that hold the tree      */
+    int block_sync;           /* levels synchronised per workgroup (1) or per wave (0,
+                                 waves_per_run 1 only); -1 = per wave when possible   */
+    int fidx_in_lds;          /* ancestor table staged in LDS (1) or read from global
+                                 memory (0); -1 = in LDS when it fits                 */
+} ngpu_batch_launch;
+typedef struct {
+    double ms_kernel;         /* HIP events around the persistent launch              */
+    double comp_steps;        /* all members                                          */
+    int nitems, workgroups, lds_bytes;
+} ngpu_batch_stats;
+typedef struct {
+    double first_start_ms, last_end_ms;   /* from the first item's start (device clock) */
+    double comp_steps;
+    int waves_per_run, runs_per_item, block_sync, fidx_in_lds;   /* what ran          */
+} ngpu_batch_member;
+int ngpu_batch_create(int device, ngpu_batch **out);
+/* Add a configured context (tree, parameters, stimuli); returns its member
+ * index.  The caller keeps ownership and must keep it alive. */
+int ngpu_batch_add(ngpu_batch *batch, ngpu_ctx *ctx);
+int ngpu_batch_set_launch(ngpu_batch *batch, int member, const ngpu_batch_launch *cfg);
+int ngpu_batch_run(ngpu_batch *batch, ngpu_batch_stats *stats);
+int ngpu_batch_member_info(const ngpu_batch *batch, int member, ngpu_batch_member *out);
+const char *ngpu_batch_last_error(const ngpu_batch *batch);
+void ngpu_batch_close(ngpu_batch *batch);
+
 /* ---- Surrogate network (no reference counterpart: the reference GA has no
  * surrogate).  An ensemble of `members` MLPs nin -> H -> H -> H -> nout with
  * ReLU, BF16 weights and activations on MFMA with FP32 accumulation, FP32

@@ -24,7 +24,9 @@ EXPORTS = ("ngpu_create", "ngpu_set_tree", "ngpu_set_sim", "ngpu_set_params", "n
            "ngpu_set_launch", "ngpu_run", "ngpu_fetch_vhot", "ngpu_open", "ngpu_write_vhot",
            "ngpu_vhot_size", "ngpu_mechset_nparams", "ngpu_last_error", "ngpu_close", "ngpu_features",
            "ngpu_mlp_create", "ngpu_mlp_nparams", "ngpu_mlp_set_weights", "ngpu_mlp_get_weights", "ngpu_mlp_train",
-           "ngpu_mlp_predict", "ngpu_mlp_gradients", "ngpu_mlp_last_error", "ngpu_mlp_close", "ngpu_gemm_bf16")
+           "ngpu_mlp_predict", "ngpu_mlp_gradients", "ngpu_mlp_last_error", "ngpu_mlp_close", "ngpu_gemm_bf16",
+           "ngpu_batch_create", "ngpu_batch_add", "ngpu_batch_set_launch", "ngpu_batch_run",
+           "ngpu_batch_member_info", "ngpu_batch_last_error", "ngpu_batch_close")
 NFEAT = 5
 FEATURES = ("spike_count", "first_latency", "ap_amplitude", "ap_width", "mean_isi")
 
@@ -62,6 +64,27 @@ class ngpu_stats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class ngpu_batch_launch(C.Structure):
+    _fields_ = [("waves_per_run", C.c_int), ("block_sync", C.c_int), ("fidx_in_lds", C.c_int)]
+
+
+class ngpu_batch_stats(C.Structure):
+    _fields_ = [("ms_kernel", C.c_double), ("comp_steps", C.c_double), ("nitems", C.c_int),
+                ("workgroups", C.c_int), ("lds_bytes", C.c_int)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ngpu_batch_member(C.Structure):
+    _fields_ = [("first_start_ms", C.c_double), ("last_end_ms", C.c_double), ("comp_steps", C.c_double),
+                ("waves_per_run", C.c_int), ("runs_per_item", C.c_int), ("block_sync", C.c_int),
+                ("fidx_in_lds", C.c_int)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 _lib = None
 
 
@@ -92,6 +115,18 @@ def lib():
         L.ngpu_features.restype = C.c_int
         L.ngpu_close.argtypes = [vp]
         L.ngpu_close.restype = None
+        L.ngpu_batch_create.argtypes = [C.c_int, C.POINTER(vp)]
+        L.ngpu_batch_add.argtypes = [vp, vp]
+        L.ngpu_batch_set_launch.argtypes = [vp, C.c_int, C.POINTER(ngpu_batch_launch)]
+        L.ngpu_batch_run.argtypes = [vp, C.POINTER(ngpu_batch_stats)]
+        L.ngpu_batch_member_info.argtypes = [vp, C.c_int, C.POINTER(ngpu_batch_member)]
+        L.ngpu_batch_last_error.argtypes = [vp]
+        L.ngpu_batch_last_error.restype = C.c_char_p
+        L.ngpu_batch_close.argtypes = [vp]
+        L.ngpu_batch_close.restype = None
+        for name in ("ngpu_batch_create", "ngpu_batch_add", "ngpu_batch_set_launch", "ngpu_batch_run",
+                     "ngpu_batch_member_info"):
+            getattr(L, name).restype = C.c_int
         for name in ("ngpu_create", "ngpu_set_tree", "ngpu_set_sim", "ngpu_set_params", "ngpu_set_stimuli",
                      "ngpu_set_launch", "ngpu_run", "ngpu_fetch_vhot", "ngpu_open", "ngpu_write_vhot",
                      "ngpu_mechset_nparams"):
@@ -242,3 +277,68 @@ def simulate_workload(w, device: int = 0, launch: Optional[dict] = None, fetch: 
     if out is not None:
         out = out.reshape(w.x.shape[0], w.amps.shape[0], len(m.rec_sites), w.dts.size)
     return out, st
+
+
+class Batch:
+    """Several :class:`Simulator` contexts of one mechanism set simulated by one
+    persistent launch through a device work queue (``ngpu_batch_*``)."""
+
+    def __init__(self, device: int = 0):
+        self.L = lib()
+        self.h = C.c_void_p()
+        self.members = []
+        rc = self.L.ngpu_batch_create(device, C.byref(self.h))
+        if rc != 0:
+            raise NgpuError(rc, (self.L.ngpu_batch_last_error(None) or b"").decode())
+
+    def _check(self, rc: int) -> int:
+        if rc < 0:
+            raise NgpuError(rc, (self.L.ngpu_batch_last_error(self.h) or b"").decode())
+        return rc
+
+    def add(self, sim: Simulator) -> int:
+        idx = self._check(self.L.ngpu_batch_add(self.h, sim._h))
+        self.members.append(sim)
+        return idx
+
+    def set_launch(self, member: int, waves_per_run: int = 0, block_sync: int = -1, fidx_in_lds: int = -1) -> None:
+        cfg = ngpu_batch_launch(waves_per_run, block_sync, fidx_in_lds)
+        self._check(self.L.ngpu_batch_set_launch(self.h, member, C.byref(cfg)))
+
+    def run(self) -> dict:
+        st = ngpu_batch_stats()
+        self._check(self.L.ngpu_batch_run(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def member_info(self, member: int) -> dict:
+        info = ngpu_batch_member()
+        self._check(self.L.ngpu_batch_member_info(self.h, member, C.byref(info)))
+        return info.as_dict()
+
+    def close(self) -> None:
+        if self.h and self.h.value:
+            self.L.ngpu_batch_close(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def workload_simulator(w, device: int = 0) -> Simulator:
+    """A configured :class:`Simulator` for a :class:`models.Workload`."""
+    m = w.model
+    sim = Simulator(device)
+    sim.set_tree(m.tree, m.mechset_id)
+    sim.set_sim(m.v_init, m.rec_sites)
+    sim.set_params(w.params, m.ncomps, m.nparams)
+    sim.set_stimuli(w.amps, w.dts, m.stim_seg, m.stim_area)
+    return sim

@@ -1,0 +1,118 @@
+"""Autotuner of the batch launch: per morphology, waves per run (which sets the
+runs per 8-wave workgroup), level-sync granularity (per wave or per workgroup)
+and LDS staging of the ancestor table.
+
+Each candidate runs a probe batch of the morphology alone: enough runs to
+give every workgroup of the persistent launch several items, for a few
+hundred steps, and the candidate with the highest compartment-steps/s wins.
+Choices are cached in a JSON file keyed by (tree hash, mechanism set,
+architecture), so a morphology is tuned once per machine.
+
+No reference counterpart: the reference launch shape is fixed
+(GUI/BBP_TTPC_EXAMPLE/pyNeuroGPU_unix/src/CudaStuff.cu:487-507).
+"""
+from __future__ import annotations
+
+import dataclasses
+import hashlib
+import json
+import os
+from typing import Dict, List, Optional, Tuple
+
+import numpy as np
+
+from . import _lib, models
+
+ARCH = "gfx950"
+BATCH_NSLOT = 6                 # positions per lane of the batch kernel
+BATCH_WAVES = 8                 # waves per workgroup of the batch kernel
+DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "neurogpu_amd", "batch_tuning.json")
+
+
+@dataclasses.dataclass(frozen=True)
+class Choice:
+    waves_per_run: int
+    block_sync: int
+    fidx_in_lds: int
+
+    def as_dict(self) -> dict:
+        return dataclasses.asdict(self)
+
+
+def candidates(n_segments: int) -> List[Choice]:
+    """Every launch shape the batch kernel supports for a tree of this size."""
+    out = []
+    for w in (1, 2, 4, 8):
+        if 64 * w * BATCH_NSLOT < n_segments:
+            continue
+        for sync in ((0, 1) if w == 1 else (1,)):
+            for fidx in (1, 0):
+                out.append(Choice(w, sync, fidx))
+    return out
+
+
+def tree_key(w: models.Workload) -> str:
+    t = w.model.tree
+    h = hashlib.sha1()
+    for a in (t.ks, t.e, t.f, t.cm, t.bool_model, t.seg_to_comp):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return f"{h.hexdigest()[:16]}:{w.model.mechset}:{ARCH}"
+
+
+def _probe_workload(w: models.Workload, runs: int, steps: int) -> models.Workload:
+    nst = w.amps.shape[0]
+    nsets = max(1, min(w.x.shape[0], -(-runs // nst)))
+    steps = min(steps, w.dts.size)
+    return models.Workload(w.name, w.model, w.x[:nsets], w.amps[:, :steps], w.dts[:steps])
+
+
+def measure(w: models.Workload, choice: Choice, device: int = 0) -> float:
+    """Compartment-steps/s of one batch launch of w alone with this choice."""
+    sim = _lib.workload_simulator(w, device)
+    try:
+        with _lib.Batch(device) as b:
+            i = b.add(sim)
+            b.set_launch(i, **choice.as_dict())
+            st = b.run()
+    finally:
+        sim.close()
+    return st["comp_steps"] / (st["ms_kernel"] * 1e-3)
+
+
+def tune(w: models.Workload, device: int = 0, probe_runs: int = 4096, probe_steps: int = 200,
+         cache: Optional[str] = DEFAULT_CACHE, log=None) -> Tuple[Choice, Dict[str, float]]:
+    """Best launch choice for w (from the cache when present)."""
+    key = tree_key(w)
+    db = {}
+    if cache and os.path.exists(cache):
+        try:
+            with open(cache) as f:
+                db = json.load(f)
+        except (OSError, ValueError):
+            db = {}
+    if key in db:
+        return Choice(**db[key]["choice"]), db[key]["rates"]
+    probe = _probe_workload(w, probe_runs, probe_steps)
+    rates: Dict[str, float] = {}
+    best, best_rate = None, -1.0
+    for c in candidates(w.model.tree.N):
+        try:
+            measure(probe, c, device)            # first launch builds the tables
+            r = measure(probe, c, device)
+        except _lib.NgpuError as e:              # e.g. LDS too small for this shape
+            if log:
+                log(f"  {w.name} {c}: not possible ({e})")
+            continue
+        rates[json.dumps(c.as_dict(), sort_keys=True)] = r
+        if log:
+            log(f"  {w.name} {c}: {r:.3e} comp-steps/s")
+        if r > best_rate:
+            best, best_rate = c, r
+    if best is None:
+        raise RuntimeError(f"no launch shape fits {w.name} (N={w.model.tree.N})")
+    if cache:
+        db[key] = {"choice": best.as_dict(), "rates": rates, "name": w.name}
+        os.makedirs(os.path.dirname(cache), exist_ok=True)
+        with open(cache, "w") as f:
+            json.dump(db, f, indent=1, sort_keys=True)
+    return best, rates

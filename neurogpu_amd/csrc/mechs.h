@@ -26,6 +26,9 @@
  *   S::current(mask, v, st, P, sum, ica)  BREAKPOINT currents; ica accumulated when asked
  * st[] holds the state slots (ion concentration cai and reversal eca included)
  * and P(j) returns parameter j (the set's parameter-matrix index).
+ *   S::ICA, S::ica_bits(mask)             slot carrying the calcium current of the last
+ *                                         BREAKPOINT to the next DERIVATIVE pass, present
+ *                                         where a mechanism reads it
  */
 #ifndef NGPU_MECHS_H
 #define NGPU_MECHS_H
@@ -46,7 +49,8 @@ NG_DEV double ng_cnexp(double y, double yinf, double tau, double dt)
 /* ===================================================================== pas */
 struct SetPassive {
     static constexpr int NMECH = 1, NPARAM = 2, NSTATE = 1;
-    static constexpr int ECA = 31;
+    static constexpr int ECA = 31, ICA = 0;
+    NG_DEV static unsigned ica_bits(unsigned) { return 0u; }
     NG_DEV static void eca_after_cnexp(unsigned, double *) {}
     NG_DEV static unsigned state_bits(unsigned) { return 0u; }
     template <class PF> NG_DEV static void init(unsigned, double, double *, const PF &) {}
@@ -63,7 +67,7 @@ struct SetPassive {
  * 2 CaDynamics_E2 3 Ih 4 Im 5 K_Pst 6 K_Tst 7 Nap_Et2 8 NaTa_t 9 NaTs2_t 10 pas
  * 11 SK_E2 12 SKv3_1.  State slots: 0,1 Ca_HVA m,h | 2,3 Ca_LVAst m,h | 4 Ih m |
  * 5 Im m | 6,7 K_Pst m,h | 8,9 K_Tst m,h | 10,11 Nap_Et2 m,h | 12,13 NaTa_t m,h |
- * 14,15 NaTs2_t m,h | 16 SKv3_1 m | 17 SK_E2 z | 18 cai | 19 eca. */
+ * 14,15 NaTs2_t m,h | 16 SKv3_1 m | 17 SK_E2 z | 18 cai | 19 eca | 20 ica. */
 namespace ttpc {
 #define TT_KTF (1000. * 8.3134 * (34.0 + 273.15) / NG_FARADAY)
 #define TT_ENA 50.0
@@ -179,8 +183,10 @@ NG_DEV void skv3_1(double v, double &mi, double &mt)
 } // namespace ttpc
 
 struct SetTTPC {
-    static constexpr int NMECH = 13, NPARAM = 19, NSTATE = 20;
-    static constexpr int CAI = 18, ECA = 19;
+    static constexpr int NMECH = 13, NPARAM = 19, NSTATE = 21;
+    static constexpr int CAI = 18, ECA = 19, ICA = 20;
+    /* CaDynamics_E2 reads ica */
+    NG_DEV static unsigned ica_bits(unsigned m) { return (m & (1u << 2)) ? 1u << ICA : 0u; }
     NG_DEV static void eca_after_cnexp(unsigned m, double *st)
     {
         if (m & ((1u << 0) | (1u << 1) | (1u << 2) | (1u << 11))) st[ECA] = ttpc::eca_of(st[CAI]);
@@ -268,7 +274,7 @@ struct SetTTPC {
  * (gbar tha qa Ra Rb) | na p16 gbar p17 tha p18 qa p19 Ra p20 Rb p21 thi1
  * p22 thi2 p23 qi p24 thinf p25 qinf p26 Rg p27 Rd | pas p28 g p29 e.
  * State slots: 0,1 ca m,h | 2 cad ca | 3 kca n | 4 km n | 5 kv n | 6,7 na m,h |
- * 8 cai | 9 eca.  Globals of the generated model: celsius 37, tadj
+ * 8 cai | 9 eca | 10 ica.  Globals of the generated model: celsius 37, tadj
  * 3.2093639532679714, vshift_na -5, depth_cad 0.09334562733124982,
  * cainf 1e-4, taur 200, ena 60, ek -90. */
 namespace mainen {
@@ -326,8 +332,10 @@ NG_DEV void na(double vm, const PF &P, double &mi, double &mt, double &hi, doubl
 } // namespace mainen
 
 struct SetMainen {
-    static constexpr int NMECH = 7, NPARAM = 30, NSTATE = 10;
-    static constexpr int CAI = 8, ECA = 9;
+    static constexpr int NMECH = 7, NPARAM = 30, NSTATE = 11;
+    static constexpr int CAI = 8, ECA = 9, ICA = 10;
+    /* cad reads ica */
+    NG_DEV static unsigned ica_bits(unsigned m) { return (m & 2u) ? 1u << ICA : 0u; }
     NG_DEV static void eca_after_cnexp(unsigned m, double *st)
     {
         if (m & 7u) st[ECA] = mainen::eca_of(st[CAI]);

@@ -32,6 +32,9 @@ hipError_t ngpu_launch_ttpc(int nslot, bool multi, dim3 grid, dim3 block, size_t
 hipError_t ngpu_launch_features(const double *vhot, int nrec, int nt, int nruns, int nstim, const double *times,
                                 const double *t_on, const double *t_off, double thr, double *out, hipStream_t s);
 hipError_t ngpu_launch_mainen(int nslot, bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
+hipError_t ngpu_launch_batch_passive(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
+hipError_t ngpu_launch_batch_ttpc(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
+hipError_t ngpu_launch_batch_mainen(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
 
 static const int kSlotSizes[] = {1, 2, 4, 6, 8};
 static std::string g_last_error;
@@ -87,10 +90,12 @@ struct ngpu_ctx {
     std::vector<double> amps, dts;
     /* launch configuration */
     int cfg_w = 0, cfg_r = 0, cfg_spl = 0, cfg_graph = 0;
+    int cfg_fidx = -1;            /* ancestor table: -1 auto, 0 global memory, 1 LDS */
     /* device state */
     int lay_W = 0, lay_slot = 0, lay_NQ = 0;
     bool tables_dirty = true, params_dirty = true, stim_dirty = true, graph_dirty = true;
-    DevArr<int> d_qh, d_qph, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_fidx, d_rec;
+    DevArr<int> d_qh, d_qph, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_rec;
+    DevArr<short> d_fidx16;
     DevArr<double> d_qcm, d_e, d_f, d_pval, d_pconst, d_pv, d_dts, d_amps, d_states, d_carry, d_vhot, d_times, d_fwin, d_feat;
     int stim_q = -1;
     std::vector<int> rec_q;
@@ -100,6 +105,7 @@ struct ngpu_ctx {
     bool ran = false;
     int last_launches = 0;
     int last_R = 1;
+    int last_fidx = 1;
 };
 
 static int fail(ngpu_ctx *c, int code, const char *fmt, ...)
@@ -180,7 +186,8 @@ extern "C" void ngpu_close(ngpu_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
-    for (auto *d : {&c->d_qh, &c->d_qph, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_fidx, &c->d_rec}) d->release();
+    for (auto *d : {&c->d_qh, &c->d_qph, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_rec}) d->release();
+    c->d_fidx16.release();
     for (auto *d : {&c->d_qcm, &c->d_e, &c->d_f, &c->d_pval, &c->d_pconst, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_carry, &c->d_vhot, &c->d_times, &c->d_fwin, &c->d_feat}) d->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -504,12 +511,13 @@ static int build_tables(ngpu_ctx *c, int W, int slot)
     for (size_t r = 0; r < c->rec.size(); r++) c->rec_q[r] = pos[c->rec[r]];
     c->stim_q = c->stim_seg >= 0 ? pos[c->stim_seg] : -1;
     std::vector<double> re(c->e), rf(c->f);
+    std::vector<short> fidx16(c->fidx.begin(), c->fidx.end());
     int rc;
     if ((rc = upload(c, c->d_pconst, c->pconst)) || (rc = upload(c, c->d_qh, qh)) || (rc = upload(c, c->d_qph, qph)) || (rc = upload(c, c->d_qmeta, qmeta)) ||
         (rc = upload(c, c->d_qcm, qcm)) || (rc = upload(c, c->d_e, re)) || (rc = upload(c, c->d_f, rf)) ||
         (rc = upload(c, c->d_pval, pval)) || (rc = upload(c, c->d_chains, c->chains)) ||
         (rc = upload(c, c->d_lvl, c->lvl_ptr)) || (rc = upload(c, c->d_fath, c->fath)) || (rc = upload(c, c->d_flvl, c->flvl_ptr)) ||
-        (rc = upload(c, c->d_fchild, c->fchild)) || (rc = upload(c, c->d_fidx, c->fidx)) || (rc = upload(c, c->d_rec, c->rec_q)))
+        (rc = upload(c, c->d_fchild, c->fchild)) || (rc = upload(c, c->d_fidx16, fidx16)) || (rc = upload(c, c->d_rec, c->rec_q)))
         return rc;
     c->lay_W = W; c->lay_slot = slot; c->lay_NQ = NQ;
     c->tables_dirty = false;
@@ -526,35 +534,54 @@ static hipError_t launch_one(ngpu_ctx *c, const SimArgs &a, dim3 grid, dim3 bloc
     }
 }
 
-static int enqueue(ngpu_ctx *c, int &launches)
+static const size_t kLdsLimit = 160 * 1024;
+
+static int npv_staged(const ngpu_ctx *c) { return c->nvar <= 512 ? std::max(c->nvar, 1) : 0; }
+
+static size_t lds_need(const ngpu_ctx *c, int R, int NQS, int fidx_lds)
 {
-    const int N = c->N, NQ = c->lay_NQ, LW = 64 * c->lay_W;
-    const int nruns = c->nsets * c->nstim;
-    int R = 1;
-    if (c->lay_W == 1) R = c->cfg_r > 0 ? c->cfg_r : 4;   /* 4 single-wave runs per workgroup */
-    const int npv = c->nvar <= 512 ? std::max(c->nvar, 1) : 0;
-    const size_t per_run = (size_t)(2 * (N + 1) + npv + (int)c->rec.size() * 64) * sizeof(double);
-    const size_t nshort = (size_t)c->logn * N + c->chains.size() + c->fath.size() + c->fchild.size() +
-                          (size_t)(c->depth + 2) + (size_t)(c->depth + 1);
-    const size_t shared = (size_t)2 * N * sizeof(double) +
-                          (((size_t)NQ * sizeof(int) + (nshort + 2 * (size_t)NQ) * sizeof(short) + 7) / 8) * 8;
-    while (R > 1 && shared + per_run * R > 160 * 1024) R--;
-    const size_t lds = shared + per_run * R;
-    if (lds > 160 * 1024) return fail(c, NGPU_E_ARG, "tree needs %zu bytes of LDS per block (limit 160 KiB)", lds);
-    SimArgs a{};
-    a.N = N; a.NQ = NQ; a.LW = LW; a.runs_per_block = R;
+    return ng_lds_bytes(c->N, NQS, R, npv_staged(c), (int)c->rec.size(), c->logn, fidx_lds, (int)c->chains.size() / 4,
+                        (int)c->fath.size() / 3, (int)c->fchild.size(), c->depth);
+}
+
+/* Kernel arguments of a prepared context (tables built for its layout). */
+static void fill_args(const ngpu_ctx *c, SimArgs &a, int R, int fidx_lds, int block_sync)
+{
+    a = SimArgs{};
+    const int npv = npv_staged(c);
+    a.N = c->N; a.NQ = c->lay_NQ; a.LW = 64 * c->lay_W; a.runs_per_block = R;
     a.nvar = std::max(c->nvar, 1); a.nrec = (int)c->rec.size(); a.nstim = c->nstim; a.nt = c->nt; a.nt_amps = c->nt_amps;
-    a.nruns = nruns; a.depth = c->depth; a.logn = c->logn; a.stim_q = c->stim_q;
+    a.nruns = c->nsets * c->nstim; a.depth = c->depth; a.logn = c->logn; a.stim_q = c->stim_q;
     a.stim_area = c->area; a.v_init = c->v_init;
     a.q_h = c->d_qh.p; a.q_ph = c->d_qph.p; a.q_meta = c->d_qmeta.p; a.q_cm = c->d_qcm.p;
     a.row_e = c->d_e.p; a.row_f = c->d_f.p; a.pval = c->d_pval.p; a.pv = c->d_pv.p;
     a.pconst_mask = c->pconst_mask; a.pv_in_lds = npv > 0 ? 1 : 0;
     a.pconst = c->d_pconst.p;
     a.chains = c->d_chains.p; a.lvl_ptr = c->d_lvl.p; a.fath = c->d_fath.p; a.flvl_ptr = c->d_flvl.p;
-    a.fchild = c->d_fchild.p; a.fidx = c->d_fidx.p;
-    a.nchains = (int)c->chains.size() / 4; a.nfath = (int)c->fath.size() / 3; a.nfchild = (int)c->fchild.size(); a.dts = c->d_dts.p; a.amps = c->d_amps.p; a.rec_q = c->d_rec.p;
+    a.fchild = c->d_fchild.p; a.fidx16 = c->d_fidx16.p; a.fidx_in_lds = fidx_lds; a.block_sync = block_sync;
+    a.nchains = (int)c->chains.size() / 4; a.nfath = (int)c->fath.size() / 3; a.nfchild = (int)c->fchild.size();
+    a.dts = c->d_dts.p; a.amps = c->d_amps.p; a.rec_q = c->d_rec.p;
     a.states = c->d_states.p; a.carry = c->d_carry.p; a.vhot = c->d_vhot.p;
+    a.t0 = 0; a.t1 = c->nt;
+}
+
+static int enqueue(ngpu_ctx *c, int &launches)
+{
+    const int NQ = c->lay_NQ, LW = 64 * c->lay_W;
+    const int nruns = c->nsets * c->nstim;
+    int R = 1;
+    if (c->lay_W == 1) R = c->cfg_r > 0 ? c->cfg_r : 4;   /* 4 single-wave runs per workgroup */
+    /* the ancestor table goes to LDS when it fits (or as configured) */
+    int fidx = c->cfg_fidx;
+    if (fidx < 0) fidx = lds_need(c, 1, NQ, 1) <= kLdsLimit ? 1 : 0;
+    while (R > 1 && lds_need(c, R, NQ, fidx) > kLdsLimit) R--;
+    const size_t lds = lds_need(c, R, NQ, fidx);
+    if (lds > kLdsLimit)
+        return fail(c, NGPU_E_ARG, "tree needs %zu bytes of LDS per workgroup (limit 160 KiB)", lds);
+    SimArgs a;
+    fill_args(c, a, R, fidx, c->lay_W > 1);
     c->last_R = R;
+    c->last_fidx = fidx;
     dim3 block(LW * R), grid((nruns + R - 1) / R);
     int spl = c->cfg_spl > 0 ? c->cfg_spl : c->nt;
     launches = 0;
@@ -568,14 +595,24 @@ static int enqueue(ngpu_ctx *c, int &launches)
     return NGPU_OK;
 }
 
+static int prepare_layout(ngpu_ctx *c, int W, int slot);
+
 static int prepare(ngpu_ctx *c)
 {
     if (!c->have_tree || !c->have_params || !c->have_stim)
         return fail(c, NGPU_E_STATE, "ngpu_run needs ngpu_set_tree, ngpu_set_params and ngpu_set_stimuli first");
-    if (!c->have_sim) { c->rec.assign(1, 0); c->have_sim = true; }
-    HIPCHK(c, hipSetDevice(c->device));
     int W, slot;
     choose_layout(c, W, slot);
+    return prepare_layout(c, W, slot);
+}
+
+/* Tables, parameters, stimuli and buffers on the device for layout (W, slot). */
+static int prepare_layout(ngpu_ctx *c, int W, int slot)
+{
+    if (!c->have_tree || !c->have_params || !c->have_stim)
+        return fail(c, NGPU_E_STATE, "a run needs ngpu_set_tree, ngpu_set_params and ngpu_set_stimuli first");
+    if (!c->have_sim) { c->rec.assign(1, 0); c->have_sim = true; }
+    HIPCHK(c, hipSetDevice(c->device));
     int rc;
     if (c->tables_dirty || W != c->lay_W || slot != c->lay_slot) {
         if ((rc = build_tables(c, W, slot))) return rc;
@@ -589,9 +626,10 @@ static int prepare(ngpu_ctx *c)
         if ((rc = upload(c, c->d_dts, c->dts)) || (rc = upload(c, c->d_amps, c->amps))) return rc;
         c->stim_dirty = false;
     }
-    const size_t nruns = (size_t)c->nsets * c->nstim, NQ = c->lay_NQ;
+    /* buffers for whole items of up to 8 runs (padding runs of the batch kernel) */
+    const size_t nruns = ((size_t)c->nsets * c->nstim + 7) / 8 * 8, NQ = c->lay_NQ;
     if ((rc = alloc(c, c->d_states, nruns * mech_nstate(c->mechset) * NQ)) ||
-        (rc = alloc(c, c->d_carry, nruns * (3 * NQ + c->N + 1))) ||
+        (rc = alloc(c, c->d_carry, nruns * (2 * NQ + c->N + 1))) ||
         (rc = alloc(c, c->d_vhot, nruns * c->rec.size() * c->nt)))
         return rc;
     return NGPU_OK;
@@ -654,6 +692,231 @@ extern "C" int ngpu_run(ngpu_ctx *c, double *vhot, ngpu_stats *st)
     }
     (void)hipEventDestroy(h0);
     (void)hipEventDestroy(h1);
+    return NGPU_OK;
+}
+
+/* ------------------------------------------------------------------ batches */
+
+struct ngpu_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::string err;
+    std::vector<ngpu_ctx *> mem;
+    std::vector<ngpu_batch_launch> cfg;
+    std::vector<int> lay_w, lay_sync, lay_fidx;
+    std::vector<double> t_first, t_last, cs;   /* per member, ms from the first item start */
+    int4 *d_items = nullptr;
+    int *d_head = nullptr;
+    unsigned long long *d_stamps = nullptr;
+    size_t cap_items = 0;
+};
+
+static int bfail(ngpu_batch *b, int code, const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (b) b->err = buf;
+    g_last_error = buf;
+    return code;
+}
+
+#define BHIPCHK(b, call)                                                                          \
+    do {                                                                                          \
+        hipError_t _e = (call);                                                                   \
+        if (_e != hipSuccess)                                                                     \
+            return bfail((b), NGPU_E_HIP, "%s failed: %s", #call, hipGetErrorString(_e));         \
+    } while (0)
+
+extern "C" int ngpu_batch_create(int device, ngpu_batch **out)
+{
+    if (!out) return bfail(nullptr, NGPU_E_ARG, "ngpu_batch_create: out is NULL");
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return bfail(nullptr, NGPU_E_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= ndev) return bfail(nullptr, NGPU_E_ARG, "device %d out of range (0..%d)", device, ndev - 1);
+    ngpu_batch *b = new ngpu_batch();
+    b->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess ||
+        hipMalloc((void **)&b->d_head, sizeof(int)) != hipSuccess) {
+        ngpu_batch_close(b);
+        return bfail(nullptr, NGPU_E_HIP, "could not create the batch's stream, events or counter on device %d", device);
+    }
+    *out = b;
+    return NGPU_OK;
+}
+
+extern "C" void ngpu_batch_close(ngpu_batch *b)
+{
+    if (!b) return;
+    (void)hipSetDevice(b->device);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (void *p : {(void *)b->d_items, (void *)b->d_head, (void *)b->d_stamps})
+        if (p) (void)hipFree(p);
+    if (b->ev0) (void)hipEventDestroy(b->ev0);
+    if (b->ev1) (void)hipEventDestroy(b->ev1);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" const char *ngpu_batch_last_error(const ngpu_batch *b) { return b ? b->err.c_str() : g_last_error.c_str(); }
+
+extern "C" int ngpu_batch_add(ngpu_batch *b, ngpu_ctx *c)
+{
+    if (!b || !c) return bfail(b, NGPU_E_ARG, "ngpu_batch_add: NULL argument");
+    if (c->device != b->device) return bfail(b, NGPU_E_ARG, "member on device %d, batch on device %d", c->device, b->device);
+    if (!c->have_tree) return bfail(b, NGPU_E_STATE, "member has no tree");
+    if ((int)b->mem.size() >= NG_BATCH_MAX_MORPH)
+        return bfail(b, NGPU_E_ARG, "a batch holds at most %d contexts", NG_BATCH_MAX_MORPH);
+    if (!b->mem.empty() && c->mechset != b->mem[0]->mechset)
+        return bfail(b, NGPU_E_ARG, "all members of a batch need the same mechanism set");
+    for (auto *m : b->mem)
+        if (m == c) return bfail(b, NGPU_E_ARG, "context already in the batch");
+    b->mem.push_back(c);
+    b->cfg.push_back(ngpu_batch_launch{0, -1, -1});
+    return (int)b->mem.size() - 1;
+}
+
+extern "C" int ngpu_batch_set_launch(ngpu_batch *b, int member, const ngpu_batch_launch *cfg)
+{
+    if (!b || !cfg) return bfail(b, NGPU_E_ARG, "ngpu_batch_set_launch: NULL argument");
+    if (member < 0 || member >= (int)b->mem.size()) return bfail(b, NGPU_E_ARG, "member %d out of range", member);
+    const int w = cfg->waves_per_run;
+    if (!(w == 0 || w == 1 || w == 2 || w == 4 || w == 8)) return bfail(b, NGPU_E_ARG, "waves_per_run %d not in {0,1,2,4,8}", w);
+    if (cfg->block_sync < -1 || cfg->block_sync > 1 || cfg->fidx_in_lds < -1 || cfg->fidx_in_lds > 1)
+        return bfail(b, NGPU_E_ARG, "block_sync and fidx_in_lds take -1, 0 or 1");
+    if (w > 1 && cfg->block_sync == 0) return bfail(b, NGPU_E_ARG, "runs of several waves synchronise at workgroup scope");
+    b->cfg[member] = *cfg;
+    return NGPU_OK;
+}
+
+static int batch_slot(int N, int W)
+{
+    const int need = (N + 64 * W - 1) / (64 * W);
+    for (int s : kSlotSizes)
+        if (s >= need && s <= NG_BATCH_NSLOT) return s;
+    return -1;
+}
+
+extern "C" int ngpu_batch_run(ngpu_batch *b, ngpu_batch_stats *st)
+{
+    if (!b) return bfail(b, NGPU_E_ARG, "NULL batch");
+    if (b->mem.empty()) return bfail(b, NGPU_E_STATE, "empty batch");
+    BHIPCHK(b, hipSetDevice(b->device));
+    const int nm = (int)b->mem.size();
+    std::vector<SimArgs> args(nm);
+    std::vector<int4> items;
+    std::vector<double> cost;
+    size_t lds = 0;
+    b->lay_w.assign(nm, 0); b->lay_sync.assign(nm, 0); b->lay_fidx.assign(nm, 0);
+    for (int i = 0; i < nm; i++) {
+        ngpu_ctx *c = b->mem[i];
+        if (!c->have_params || !c->have_stim) return bfail(b, NGPU_E_STATE, "member %d lacks parameters or stimuli", i);
+        int W = b->cfg[i].waves_per_run;
+        if (W == 0) { W = 1; while (W < 8 && 64 * W * NG_BATCH_NSLOT < c->N) W *= 2; }
+        const int slot = batch_slot(c->N, W);
+        if (slot < 0) return bfail(b, NGPU_E_ARG, "member %d: %d segments do not fit %d waves x %d positions per lane", i, c->N, W, NG_BATCH_NSLOT);
+        int rc = prepare_layout(c, W, slot);
+        if (rc) return bfail(b, rc, "member %d: %s", i, c->err.c_str());
+        const int R = NG_BATCH_THREADS / (64 * W), NQS = NG_BATCH_NSLOT * 64 * W;
+        int sync = b->cfg[i].block_sync;
+        if (sync < 0) sync = W > 1 ? 1 : 0;
+        int fidx = b->cfg[i].fidx_in_lds;
+        if (fidx < 0) fidx = lds_need(c, R, NQS, 1) <= kLdsLimit ? 1 : 0;
+        const size_t need = lds_need(c, R, NQS, fidx);
+        if (need > kLdsLimit) return bfail(b, NGPU_E_ARG, "member %d needs %zu bytes of LDS per workgroup (limit 160 KiB)", i, need);
+        lds = std::max(lds, need);
+        fill_args(c, args[i], R, fidx, sync);
+        args[i].nslot = slot;
+        b->lay_w[i] = W; b->lay_sync[i] = sync; b->lay_fidx[i] = fidx;
+        const int nruns = c->nsets * c->nstim;
+        for (int r0 = 0; r0 < nruns; r0 += R) {
+            items.push_back(make_int4(i, r0, std::min(R, nruns - r0), 0));
+            cost.push_back((double)c->N * c->nt);
+        }
+    }
+    /* longest items first, so that the long runs do not form the tail */
+    std::vector<int> order(items.size());
+    for (size_t k = 0; k < order.size(); k++) order[k] = (int)k;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    std::vector<int4> sorted(items.size());
+    for (size_t k = 0; k < order.size(); k++) sorted[k] = items[order[k]];
+    if (b->cap_items < sorted.size()) {
+        if (b->d_items) (void)hipFree(b->d_items);
+        if (b->d_stamps) (void)hipFree(b->d_stamps);
+        b->d_items = nullptr; b->d_stamps = nullptr; b->cap_items = 0;
+        BHIPCHK(b, hipMalloc((void **)&b->d_items, sorted.size() * sizeof(int4)));
+        BHIPCHK(b, hipMalloc((void **)&b->d_stamps, sorted.size() * 2 * sizeof(unsigned long long)));
+        b->cap_items = sorted.size();
+    }
+    /* the members' uploads were queued on their own streams */
+    for (auto *c : b->mem) BHIPCHK(b, hipStreamSynchronize(c->stream));
+    BHIPCHK(b, hipMemcpyAsync(b->d_items, sorted.data(), sorted.size() * sizeof(int4), hipMemcpyHostToDevice, b->stream));
+    BHIPCHK(b, hipMemsetAsync(b->d_head, 0, sizeof(int), b->stream));
+    int ncu = 0;
+    BHIPCHK(b, hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, b->device));
+    BatchArgs ba{};
+    for (int i = 0; i < nm; i++) ba.morph[i] = args[i];
+    ba.items = b->d_items; ba.nitems = (int)sorted.size(); ba.head = b->d_head; ba.stamps = b->d_stamps;
+    const int grid = std::max(1, std::min(ncu, (int)sorted.size()));
+    BHIPCHK(b, hipEventRecord(b->ev0, b->stream));
+    hipError_t le;
+    switch (b->mem[0]->mechset) {
+    case NGPU_MECH_TTPC: le = ngpu_launch_batch_ttpc(dim3(grid), lds, b->stream, ba); break;
+    case NGPU_MECH_MAINEN: le = ngpu_launch_batch_mainen(dim3(grid), lds, b->stream, ba); break;
+    default: le = ngpu_launch_batch_passive(dim3(grid), lds, b->stream, ba); break;
+    }
+    if (le != hipSuccess) return bfail(b, NGPU_E_HIP, "batch kernel launch failed: %s", hipGetErrorString(le));
+    BHIPCHK(b, hipEventRecord(b->ev1, b->stream));
+    hipError_t se = hipStreamSynchronize(b->stream);
+    if (se != hipSuccess) return bfail(b, NGPU_E_HIP, "batch simulation failed: %s", hipGetErrorString(se));
+    float ms = 0.f;
+    BHIPCHK(b, hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    std::vector<unsigned long long> stamps(sorted.size() * 2);
+    BHIPCHK(b, hipMemcpy(stamps.data(), b->d_stamps, stamps.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    int khz = 0;
+    BHIPCHK(b, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device));
+    unsigned long long t0 = ~0ull;
+    for (size_t k = 0; k < sorted.size(); k++) t0 = std::min(t0, stamps[2 * k]);
+    b->t_first.assign(nm, 1e300); b->t_last.assign(nm, 0.0); b->cs.assign(nm, 0.0);
+    double total = 0;
+    for (size_t k = 0; k < sorted.size(); k++) {
+        const int i = sorted[k].x;
+        const double s0 = (double)(stamps[2 * k] - t0) / khz, s1 = (double)(stamps[2 * k + 1] - t0) / khz;
+        b->t_first[i] = std::min(b->t_first[i], s0);
+        b->t_last[i] = std::max(b->t_last[i], s1);
+        const double cs = (double)b->mem[i]->N * b->mem[i]->nt * sorted[k].z;
+        b->cs[i] += cs;
+        total += cs;
+    }
+    for (auto *c : b->mem) c->ran = true;
+    if (st) {
+        st->ms_kernel = ms;
+        st->comp_steps = total;
+        st->nitems = (int)sorted.size();
+        st->workgroups = grid;
+        st->lds_bytes = (int)lds;
+    }
+    return NGPU_OK;
+}
+
+extern "C" int ngpu_batch_member_info(const ngpu_batch *b, int member, ngpu_batch_member *out)
+{
+    if (!b || !out) return bfail(nullptr, NGPU_E_ARG, "ngpu_batch_member_info: NULL argument");
+    if (member < 0 || member >= (int)b->mem.size()) return bfail(nullptr, NGPU_E_ARG, "member %d out of range", member);
+    if (b->t_first.size() != b->mem.size()) return bfail(nullptr, NGPU_E_STATE, "no completed batch run");
+    out->first_start_ms = b->t_first[member];
+    out->last_end_ms = b->t_last[member];
+    out->comp_steps = b->cs[member];
+    out->waves_per_run = b->lay_w[member];
+    out->runs_per_item = NG_BATCH_THREADS / (64 * b->lay_w[member]);
+    out->block_sync = b->lay_sync[member];
+    out->fidx_in_lds = b->lay_fidx[member];
     return NGPU_OK;
 }
 

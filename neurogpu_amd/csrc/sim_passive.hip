@@ -27,3 +27,11 @@ hipError_t ngpu_launch_passive(int nslot, bool multi, dim3 grid, dim3 block, siz
     default: return hipErrorInvalidValue;
     }
 }
+
+hipError_t ngpu_launch_batch_passive(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b)
+{
+    auto k = ngpu_batch_kernel<SetPassive, NG_BATCH_NSLOT, 4>;
+    if (lds > 65536) { hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); if (e != hipSuccess) return e; }
+    hipLaunchKernelGGL(k, grid, dim3(NG_BATCH_THREADS), lds, s, b);
+    return hipGetLastError();
+}

@@ -27,3 +27,11 @@ hipError_t ngpu_launch_ttpc(int nslot, bool multi, dim3 grid, dim3 block, size_t
     default: return hipErrorInvalidValue;
     }
 }
+
+hipError_t ngpu_launch_batch_ttpc(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b)
+{
+    auto k = ngpu_batch_kernel<SetTTPC, NG_BATCH_NSLOT, 4>;
+    if (lds > 65536) { hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds); if (e != hipSuccess) return e; }
+    hipLaunchKernelGGL(k, grid, dim3(NG_BATCH_THREADS), lds, s, b);
+    return hipGetLastError();
+}
