@@ -156,7 +156,7 @@ def bench_mixed(args):
                     "steps": int(w.dts.size), "comp_steps": w.comp_steps,
                     "first_start_ms": round(info["first_start_ms"], 2), "last_end_ms": round(info["last_end_ms"], 2),
                     "comp_steps_per_s_in_span": w.comp_steps / (span * 1e-3) if span > 0 else None,
-                    "launch": {k: info[k] for k in ("waves_per_run", "runs_per_item", "block_sync", "fidx_in_lds")}})
+                    "launch": {k: info[k] for k in ("waves_per_run", "runs_per_item", "block_sync", "lds_staging")}})
     result = {
         "metric": METRIC, "value": total * args.steps / wall, "unit": "comp-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,

@@ -138,8 +138,9 @@ typedef struct {
     int waves_per_run;        /* 1, 2, 4 or 8; 0 = the fewest that hold the tree      */
     int block_sync;           /* levels synchronised per workgroup (1) or per wave (0,
                                  waves_per_run 1 only); -1 = per wave when possible   */
-    int fidx_in_lds;          /* ancestor table staged in LDS (1) or read from global
-                                 memory (0); -1 = in LDS when it fits                 */
+    int lds_staging;          /* Hines row coefficients e, f and the runs' varying
+                                 parameter values staged in LDS (1) or read from
+                                 global memory (0); -1 = in LDS when they fit         */
 } ngpu_batch_launch;
 typedef struct {
     double ms_kernel;         /* HIP events around the persistent launch              */
@@ -149,7 +150,7 @@ typedef struct {
 typedef struct {
     double first_start_ms, last_end_ms;   /* from the first item's start (device clock) */
     double comp_steps;
-    int waves_per_run, runs_per_item, block_sync, fidx_in_lds;   /* what ran          */
+    int waves_per_run, runs_per_item, block_sync, lds_staging;   /* what ran          */
 } ngpu_batch_member;
 int ngpu_batch_create(int device, ngpu_batch **out);
 /* Add a configured context (tree, parameters, stimuli); returns its member

@@ -65,7 +65,7 @@ class ngpu_stats(C.Structure):
 
 
 class ngpu_batch_launch(C.Structure):
-    _fields_ = [("waves_per_run", C.c_int), ("block_sync", C.c_int), ("fidx_in_lds", C.c_int)]
+    _fields_ = [("waves_per_run", C.c_int), ("block_sync", C.c_int), ("lds_staging", C.c_int)]
 
 
 class ngpu_batch_stats(C.Structure):
@@ -79,7 +79,7 @@ class ngpu_batch_stats(C.Structure):
 class ngpu_batch_member(C.Structure):
     _fields_ = [("first_start_ms", C.c_double), ("last_end_ms", C.c_double), ("comp_steps", C.c_double),
                 ("waves_per_run", C.c_int), ("runs_per_item", C.c_int), ("block_sync", C.c_int),
-                ("fidx_in_lds", C.c_int)]
+                ("lds_staging", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -301,8 +301,8 @@ class Batch:
         self.members.append(sim)
         return idx
 
-    def set_launch(self, member: int, waves_per_run: int = 0, block_sync: int = -1, fidx_in_lds: int = -1) -> None:
-        cfg = ngpu_batch_launch(waves_per_run, block_sync, fidx_in_lds)
+    def set_launch(self, member: int, waves_per_run: int = 0, block_sync: int = -1, lds_staging: int = -1) -> None:
+        cfg = ngpu_batch_launch(waves_per_run, block_sync, lds_staging)
         self._check(self.L.ngpu_batch_set_launch(self.h, member, C.byref(cfg)))
 
     def run(self) -> dict:

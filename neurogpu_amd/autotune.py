@@ -1,6 +1,6 @@
 """Autotuner of the batch launch: per morphology, waves per run (which sets the
 runs per 8-wave workgroup), level-sync granularity (per wave or per workgroup)
-and LDS staging of the ancestor table.
+and LDS staging of the Hines row coefficients and the run's varying parameters.
 
 Each candidate runs a probe batch of the morphology alone: enough runs to
 give every workgroup of the persistent launch several items, for a few
@@ -26,6 +26,7 @@ from . import _lib, models
 ARCH = "gfx950"
 BATCH_NSLOT = 6                 # positions per lane of the batch kernel
 BATCH_WAVES = 8                 # waves per workgroup of the batch kernel
+KERNEL_REV = 2                  # bump when the kernel changes: cached choices are per revision
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "neurogpu_amd", "batch_tuning.json")
 
 
@@ -33,7 +34,7 @@ DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "neurogpu_amd", 
 class Choice:
     waves_per_run: int
     block_sync: int
-    fidx_in_lds: int
+    lds_staging: int
 
     def as_dict(self) -> dict:
         return dataclasses.asdict(self)
@@ -46,8 +47,8 @@ def candidates(n_segments: int) -> List[Choice]:
         if 64 * w * BATCH_NSLOT < n_segments:
             continue
         for sync in ((0, 1) if w == 1 else (1,)):
-            for fidx in (1, 0):
-                out.append(Choice(w, sync, fidx))
+            for stage in (1, 0):
+                out.append(Choice(w, sync, stage))
     return out
 
 
@@ -56,7 +57,7 @@ def tree_key(w: models.Workload) -> str:
     h = hashlib.sha1()
     for a in (t.ks, t.e, t.f, t.cm, t.bool_model, t.seg_to_comp):
         h.update(np.ascontiguousarray(a).tobytes())
-    return f"{h.hexdigest()[:16]}:{w.model.mechset}:{ARCH}"
+    return f"{h.hexdigest()[:16]}:{w.model.mechset}:{ARCH}:{KERNEL_REV}"
 
 
 def _probe_workload(w: models.Workload, runs: int, steps: int) -> models.Workload:

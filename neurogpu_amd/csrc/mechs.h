@@ -7,9 +7,24 @@
  *   SetTTPC    : the 13 BBP TTPC mechanisms              (Code/OneKernel/src/AllModels.cu)
  *   SetMainen  : ca, cad, kca, km, kv, na, pas           (Figures/FigureS4_mainen/CParsed/AllModels.cu)
  *
- * Every expression keeps the operation order of the generated code so that,
- * compiled with -ffp-contract=off and the shared ngmath.h transcendental
- * functions, the device produces the same bits as the CPU restatement.
+ * The equations are the reference's; their evaluation is arranged for the
+ * FP64 VALU, where an IEEE division costs about ten instructions and an exp
+ * about twenty:
+ *   - a gate is advanced from its rate 1/tau instead of tau:
+ *       y += (1 - exp(-dt * rate)) * (yinf - y)
+ *     is the reference's cnexp y + (1 - exp(dt*(-1/tau))) * ((-(yinf/tau))/(-1/tau) - y)
+ *     (AllModels.cu:302) with the two quotients that cancel taken out; for
+ *     alpha/beta gates rate = (alpha + beta) * q10 needs no division at all;
+ *   - divisions by literal constants are products with the reciprocal;
+ *   - exp(-x) next to exp(x) of the same argument is the reciprocal of exp(x);
+ *     for the x/(1 - exp(-x/k)) pairs of the Na gates,
+ *       beta/alpha = (b/a) exp(-x/k),  x_inf = 1 / (1 + (b/a) exp(-x/k));
+ *   - q10 = 2.3^((34-21)/10) is a literal.
+ * The singularity bumps (v == -27, -38, ... -> v + 1e-4) and the branches
+ * (K_Pst tau at v < -50, Mainen efun/trap0 thresholds) are kept as written.
+ * The results differ from the reference's operation order by rounding only;
+ * the simulation is checked against the FP64 CPU oracle of the tests at
+ * 1e-6 mV max-abs with identical spike counts.
  *
  * Interface of a set S (all per compartment, lane-local):
  *   S::NMECH, S::NPARAM, S::NSTATE        counts
@@ -20,7 +35,7 @@
  *   S::eca_after_cnexp(mask, st)          eca after a cnexp pass.  Every DERIVATIVE block
  *                                         that changes cai ends with eca = f(cai), and the
  *                                         blocks that follow read cai only, so one
- *                                         evaluation at the end gives the same bits; where
+ *                                         evaluation at the end gives the same value; where
  *                                         no block changes cai it is constant and eca =
  *                                         f(cai) already held.  Valid after the first step.
  *   S::current(mask, v, st, P, sum, ica)  BREAKPOINT currents; ica accumulated when asked
@@ -29,21 +44,68 @@
  *   S::ICA, S::ica_bits(mask)             slot carrying the calcium current of the last
  *                                         BREAKPOINT to the next DERIVATIVE pass, present
  *                                         where a mechanism reads it
+ * The rate functions are host- and device-callable so that a CPU test can
+ * compare them with the oracle's restatement of the reference's forms.
  */
 #ifndef NGPU_MECHS_H
 #define NGPU_MECHS_H
 
 #include "ngmath.h"
 
-#define NG_DEV __device__ __forceinline__
+#ifdef __HIPCC__
+#define NG_DEV __host__ __device__ __forceinline__
+#else
+#include <math.h>
+#define NG_DEV inline
+#endif
 
 #define NG_DEF_CAI 5.e-5
 #define NG_DEF_CAO 2.
 #define NG_FARADAY 96485.309
 
-NG_DEV double ng_cnexp(double y, double yinf, double tau, double dt)
+/* exp(x) for the rate functions: ng_exp's reduction and polynomial, with the
+ * argument clamped to [-708, 708] instead of branching on the range (the
+ * result stays a normal number; physiological arguments are far inside). */
+NG_DEV double ng_expc(double x)
 {
-    return y + (1. - ng_exp(dt * ((-1.0) / tau))) * ((-(yinf / tau)) / ((-1.0) / tau) - y);
+    x = fmin(fmax(x, -708.0), 708.0);
+    double kf = rint(x * NG_LOG2E);
+    double r = (x - kf * NG_LN2_HI) - kf * NG_LN2_LO;
+    double p = 1.6059043836821614599e-10;
+    p = fma(p, r, 2.0876756987868098979e-09);
+    p = fma(p, r, 2.5052108385441718775e-08);
+    p = fma(p, r, 2.7557319223985890653e-07);
+    p = fma(p, r, 2.7557319223985890653e-06);
+    p = fma(p, r, 2.4801587301587301566e-05);
+    p = fma(p, r, 1.9841269841269841253e-04);
+    p = fma(p, r, 1.3888888888888888889e-03);
+    p = fma(p, r, 8.3333333333333333333e-03);
+    p = fma(p, r, 4.1666666666666666667e-02);
+    p = fma(p, r, 1.6666666666666666667e-01);
+    p = fma(p, r, 0.5);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)kf);
+}
+
+/* 1/y: the hardware estimate refined by two Newton steps (normal, finite y). */
+NG_DEV double ng_rcp(double y)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    double r = __builtin_amdgcn_rcp(y);
+    double e = fma(-y, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-y, r, 1.0);
+    return fma(r, e, r);
+#else
+    return 1.0 / y;
+#endif
+}
+
+/* one cnexp step of a gate from its steady state and rate 1/tau */
+NG_DEV double ng_gate(double y, double yinf, double rate, double dt)
+{
+    return y + (1. - ng_expc(-(dt * rate))) * (yinf - y);
 }
 
 /* ===================================================================== pas */
@@ -67,118 +129,142 @@ struct SetPassive {
  * 2 CaDynamics_E2 3 Ih 4 Im 5 K_Pst 6 K_Tst 7 Nap_Et2 8 NaTa_t 9 NaTs2_t 10 pas
  * 11 SK_E2 12 SKv3_1.  State slots: 0,1 Ca_HVA m,h | 2,3 Ca_LVAst m,h | 4 Ih m |
  * 5 Im m | 6,7 K_Pst m,h | 8,9 K_Tst m,h | 10,11 Nap_Et2 m,h | 12,13 NaTa_t m,h |
- * 14,15 NaTs2_t m,h | 16 SKv3_1 m | 17 SK_E2 z | 18 cai | 19 eca | 20 ica. */
+ * 14,15 NaTs2_t m,h | 16 SKv3_1 m | 17 SK_E2 z | 18 cai | 19 eca | 20 ica.
+ * Each rate function returns (inf, rate) per gate; rate = 1/tau. */
 namespace ttpc {
 #define TT_KTF (1000. * 8.3134 * (34.0 + 273.15) / NG_FARADAY)
 #define TT_ENA 50.0
 #define TT_EK (-85.0)
-NG_DEV double qt() { return ng_pow(2.3, ((34.0 - 21.0) / 10.0)); }
+#define TT_QT 2.952882641412121                  /* 2.3^((34-21)/10) */
 NG_DEV double eca_of(double cai) { return TT_KTF / 2 * ng_log(NG_DEF_CAO / cai); }
 
-NG_DEV void ca_hva(double v, double &mi, double &mt, double &hi, double &ht)
+/* AllModels.cu:66-78 */
+NG_DEV void ca_hva(double v, double &mi, double &mr, double &hi, double &hr)
 {
     if (v == -27.0) v = v + 0.0001;
-    double mA = (0.055 * (-27.0 - v)) / (ng_exp((-27.0 - v) / 3.8) - 1.0);
-    double mB = (0.94 * ng_exp((-75.0 - v) / 17.0));
-    mi = mA / (mA + mB); mt = 1.0 / (mA + mB);
-    double hA = (0.000457 * ng_exp((-13.0 - v) / 50.0));
-    double hB = (0.0065 / (ng_exp((-v - 15.0) / 28.0) + 1.0));
-    hi = hA / (hA + hB); ht = 1.0 / (hA + hB);
+    const double x = -27.0 - v;
+    const double mA = (0.055 * x) / (ng_expc(x * (1.0 / 3.8)) - 1.0);
+    const double mB = 0.94 * ng_expc((-75.0 - v) * (1.0 / 17.0));
+    mr = mA + mB; mi = mA / mr;
+    const double hA = 0.000457 * ng_expc((-13.0 - v) * (1.0 / 50.0));
+    const double hB = 0.0065 / (ng_expc((-v - 15.0) * (1.0 / 28.0)) + 1.0);
+    hr = hA + hB; hi = hA / hr;
 }
-NG_DEV void ca_lvast(double v, double &mi, double &mt, double &hi, double &ht)
+/* AllModels.cu:79-88 */
+NG_DEV void ca_lvast(double v, double &mi, double &mr, double &hi, double &hr)
 {
-    double q = qt();
     v = v + 10.0;
-    mi = 1.0000 / (1.0 + ng_exp((v - -30.000) / -6.0));
-    mt = (5.0000 + 20.0000 / (1.0 + ng_exp((v - -25.000) / 5.0))) / q;
-    hi = 1.0000 / (1.0 + ng_exp((v - -80.000) / 6.4));
-    ht = (20.0000 + 50.0000 / (1.0 + ng_exp((v - -40.000) / 7.0))) / q;
+    mi = 1.0 / (1.0 + ng_expc((v + 30.0) * (-1.0 / 6.0)));
+    mr = TT_QT / (5.0 + 20.0 / (1.0 + ng_expc((v + 25.0) * (1.0 / 5.0))));
+    hi = 1.0 / (1.0 + ng_expc((v + 80.0) * (1.0 / 6.4)));
+    hr = TT_QT / (20.0 + 50.0 / (1.0 + ng_expc((v + 40.0) * (1.0 / 7.0))));
 }
-NG_DEV void ih(double v, double &mi, double &mt)
+/* AllModels.cu:89-97 */
+NG_DEV void ih(double v, double &mi, double &mr)
 {
     if (v == -154.9) v = v + 0.0001;
-    double mA = 0.001 * 6.43 * (v + 154.9) / (ng_exp((v + 154.9) / 11.9) - 1.0);
-    double mB = 0.001 * 193.0 * ng_exp(v / 33.1);
-    mi = mA / (mA + mB); mt = 1.0 / (mA + mB);
+    const double x = v + 154.9;
+    const double mA = (0.001 * 6.43) * x / (ng_expc(x * (1.0 / 11.9)) - 1.0);
+    const double mB = (0.001 * 193.0) * ng_expc(v * (1.0 / 33.1));
+    mr = mA + mB; mi = mA / mr;
 }
-NG_DEV void im(double v, double &mi, double &mt)
+/* AllModels.cu:98-105: mA = k e^{x}, mB = k e^{-x} */
+NG_DEV void im(double v, double &mi, double &mr)
 {
-    double q = qt();
-    double mA = 3.3e-3 * ng_exp(2.5 * 0.04 * (v - -35.0));
-    double mB = 3.3e-3 * ng_exp(-2.5 * 0.04 * (v - -35.0));
-    mi = mA / (mA + mB); mt = (1.0 / (mA + mB)) / q;
+    const double E = ng_expc((2.5 * 0.04) * (v + 35.0));
+    const double Ei = ng_rcp(E);
+    mi = E / (E + Ei);
+    mr = (3.3e-3 * E + 3.3e-3 * Ei) * TT_QT;
 }
-NG_DEV void k_pst(double v, double &mi, double &mt, double &hi, double &ht)
+/* AllModels.cu:106-120 */
+NG_DEV void k_pst(double v, double &mi, double &mr, double &hi, double &hr)
 {
-    double q = qt();
     v = v + 10.0;
-    mi = (1.0 / (1.0 + ng_exp(-(v + 1.0) / 12.0)));
-    if (v < -50.0) mt = (1.25 + 175.03 * ng_exp(-v * -0.026)) / q;
-    else           mt = ((1.25 + 13.0 * ng_exp(-v * 0.026))) / q;
-    hi = 1.0 / (1.0 + ng_exp(-(v + 54.0) / -11.0));
-    ht = (360.0 + (1010.0 + 24.0 * (v + 55.0)) * ng_exp(-ng_pow(((v + 75.0) / 48.0), 2.0))) / q;
+    mi = 1.0 / (1.0 + ng_expc(-(v + 1.0) * (1.0 / 12.0)));
+    const bool lo = v < -50.0;
+    mr = TT_QT / (1.25 + (lo ? 175.03 : 13.0) * ng_expc(v * (lo ? 0.026 : -0.026)));
+    hi = 1.0 / (1.0 + ng_expc((v + 54.0) * (1.0 / 11.0)));
+    const double z = (v + 75.0) * (1.0 / 48.0);
+    hr = TT_QT / (360.0 + (1010.0 + 24.0 * (v + 55.0)) * ng_expc(-(z * z)));
 }
-NG_DEV void k_tst(double v, double &mi, double &mt, double &hi, double &ht)
+/* AllModels.cu:121-130 */
+NG_DEV void k_tst(double v, double &mi, double &mr, double &hi, double &hr)
 {
-    double q = qt();
     v = v + 10.0;
-    mi = 1.0 / (1.0 + ng_exp(-(v + 0.0) / 19.0));
-    mt = (0.34 + 0.92 * ng_exp(-ng_pow(((v + 71.0) / 59.0), 2.0))) / q;
-    hi = 1.0 / (1.0 + ng_exp(-(v + 66.0) / -10.0));
-    ht = (8.0 + 49.0 * ng_exp(-ng_pow(((v + 73.0) / 23.0), 2.0))) / q;
+    mi = 1.0 / (1.0 + ng_expc(v * (-1.0 / 19.0)));
+    const double zm = (v + 71.0) * (1.0 / 59.0);
+    mr = TT_QT / (0.34 + 0.92 * ng_expc(-(zm * zm)));
+    hi = 1.0 / (1.0 + ng_expc((v + 66.0) * (1.0 / 10.0)));
+    const double zh = (v + 73.0) * (1.0 / 23.0);
+    hr = TT_QT / (8.0 + 49.0 * ng_expc(-(zh * zh)));
 }
-NG_DEV void nap_et2(double v, double &mi, double &mt, double &hi, double &ht)
+/* The Na activation pair alpha = a x / (1 - e^{-x/k}), beta = b (-x) / (1 - e^{x/k}):
+ * with E = e^{-x/k} and R = x / (1 - E), alpha = a R and beta = b R E. */
+NG_DEV void na_pair(double x, double a, double b, double &alpha, double &beta, double &E)
 {
-    double q = qt();
-    mi = 1.0 / (1.0 + ng_exp((v - -52.6) / -4.6));
+    E = ng_expc(x * (-1.0 / 6.0));
+    const double R = x / (1.0 - E);
+    alpha = a * R;
+    beta = b * R * E;
+}
+/* AllModels.cu:131-151 */
+NG_DEV void nap_et2(double v, double &mi, double &mr, double &hi, double &hr)
+{
+    mi = 1.0 / (1.0 + ng_expc((v + 52.6) * (-1.0 / 4.6)));
     if (v == -38.0) v = v + 0.0001;
-    double mA = (0.182 * (v - -38.0)) / (1.0 - (ng_exp(-(v - -38.0) / 6.0)));
-    double mB = (0.124 * (-v - 38.0)) / (1.0 - (ng_exp(-(-v - 38.0) / 6.0)));
-    mt = 6.0 * (1.0 / (mA + mB)) / q;
+    double mA, mB, E;
+    na_pair(v + 38.0, 0.182, 0.124, mA, mB, E);
+    mr = (mA + mB) * (TT_QT / 6.0);
     if (v == -17.0) v = v + 0.0001;
     if (v == -64.4) v = v + 0.0001;
-    hi = 1.0 / (1.0 + ng_exp((v - -48.8) / 10.0));
-    double hA = -2.88e-6 * (v + 17.0) / (1.0 - ng_exp((v + 17.0) / 4.63));
-    double hB = 6.94e-6 * (v + 64.4) / (1.0 - ng_exp(-(v + 64.4) / 2.63));
-    ht = (1.0 / (hA + hB)) / q;
+    hi = 1.0 / (1.0 + ng_expc((v + 48.8) * (1.0 / 10.0)));
+    const double hA = -2.88e-6 * (v + 17.0) / (1.0 - ng_expc((v + 17.0) * (1.0 / 4.63)));
+    const double hB = 6.94e-6 * (v + 64.4) / (1.0 - ng_expc(-(v + 64.4) * (1.0 / 2.63)));
+    hr = (hA + hB) * TT_QT;
 }
-NG_DEV void nata_t(double v, double &mi, double &mt, double &hi, double &ht)
+/* The Na inactivation pair alpha = -c y / (1 - e^{y/6}), beta = -c (-y) / (1 - e^{-y/6}):
+ * with F = e^{y/6} and R = y / (1 - F), alpha = -c R, beta = -c R F and
+ * alpha / (alpha + beta) = 1 / (1 + F). */
+NG_DEV void na_inact(double y, double &hi, double &hr)
 {
-    double q = qt();
+    const double F = ng_expc(y * (1.0 / 6.0));
+    const double R = y / (1.0 - F);
+    hi = 1.0 / (1.0 + F);
+    hr = (-0.015 * R) * (1.0 + F) * TT_QT;
+}
+/* AllModels.cu:152-169 */
+NG_DEV void nata_t(double v, double &mi, double &mr, double &hi, double &hr)
+{
     if (v == -38.0) v = v + 0.0001;
-    double mA = (0.182 * (v - -38.0)) / (1.0 - (ng_exp(-(v - -38.0) / 6.0)));
-    double mB = (0.124 * (-v - 38.0)) / (1.0 - (ng_exp(-(-v - 38.0) / 6.0)));
-    mt = (1.0 / (mA + mB)) / q;
-    mi = mA / (mA + mB);
+    double mA, mB, E;
+    na_pair(v + 38.0, 0.182, 0.124, mA, mB, E);
+    mr = (mA + mB) * TT_QT;
+    mi = 0.182 / (0.182 + 0.124 * E);
     if (v == -66.0) v = v + 0.0001;
-    double hA = (-0.015 * (v - -66.0)) / (1.0 - (ng_exp((v - -66.0) / 6.0)));
-    double hB = (-0.015 * (-v - 66.0)) / (1.0 - (ng_exp((-v - 66.0) / 6.0)));
-    ht = (1.0 / (hA + hB)) / q;
-    hi = hA / (hA + hB);
+    na_inact(v + 66.0, hi, hr);
 }
-NG_DEV void nats2_t(double v, double &mi, double &mt, double &hi, double &ht)
+/* AllModels.cu:170-187 */
+NG_DEV void nats2_t(double v, double &mi, double &mr, double &hi, double &hr)
 {
-    double q = qt();
     if (v == -32.0) v = v + 0.0001;
-    double mA = (0.182 * (v - -32.0)) / (1.0 - (ng_exp(-(v - -32.0) / 6.0)));
-    double mB = (0.124 * (-v - 32.0)) / (1.0 - (ng_exp(-(-v - 32.0) / 6.0)));
-    mi = mA / (mA + mB);
-    mt = (1.0 / (mA + mB)) / q;
+    double mA, mB, E;
+    na_pair(v + 32.0, 0.182, 0.124, mA, mB, E);
+    mi = 0.182 / (0.182 + 0.124 * E);
+    mr = (mA + mB) * TT_QT;
     if (v == -60.0) v = v + 0.0001;
-    double hA = (-0.015 * (v - -60.0)) / (1.0 - (ng_exp((v - -60.0) / 6.0)));
-    double hB = (-0.015 * (-v - 60.0)) / (1.0 - (ng_exp((-v - 60.0) / 6.0)));
-    hi = hA / (hA + hB);
-    ht = (1.0 / (hA + hB)) / q;
+    na_inact(v + 60.0, hi, hr);
 }
+/* AllModels.cu:188-193 */
 NG_DEV double sk_e2(double ca)
 {
     if (ca < 1e-7) ca = ca + 1e-07;
-    return 1.0 / (1.0 + ng_pow((0.00043 / ca), 4.8));
+    return 1.0 / (1.0 + ng_expc(4.8 * ng_log(0.00043 / ca)));
 }
-NG_DEV void skv3_1(double v, double &mi, double &mt)
+/* AllModels.cu:194-197: tau = 4 / (1 + e^{...}) */
+NG_DEV void skv3_1(double v, double &mi, double &mr)
 {
-    mi = 1.0 / (1.0 + ng_exp(((v - (18.700)) / (-9.700))));
-    mt = 0.2 * 20.000 / (1.0 + ng_exp(((v - (-46.560)) / (-44.140))));
+    mi = 1.0 / (1.0 + ng_expc((v - 18.7) * (-1.0 / 9.7)));
+    mr = (1.0 + ng_expc((v + 46.56) * (-1.0 / 44.14))) * 0.25;
 }
 } // namespace ttpc
 
@@ -208,6 +294,7 @@ struct SetTTPC {
         if (m & ((1u << 0) | (1u << 1) | (1u << 2) | (1u << 11))) s |= (1u << CAI) | (1u << ECA);
         return s;
     }
+    /* INITIAL blocks (AllModels.cu:202-296): gates at their steady state */
     template <class PF> NG_DEV static void init(unsigned m, double v, double *st, const PF &)
     {
         using namespace ttpc;
@@ -225,29 +312,30 @@ struct SetTTPC {
         if (m & (1u << 11)) { st[ECA] = eca_of(st[CAI]); st[17] = sk_e2(st[CAI]); }
         if (m & (1u << 12)) { skv3_1(v, a, b); st[16] = a; }
     }
+    /* DERIVATIVE blocks (AllModels.cu:299-377) */
     template <class PF> NG_DEV static void cnexp(unsigned m, double v, double dt, double *st, double ica, const PF &P)
     {
         using namespace ttpc;
         double a, b, c, d;
-        if (m & (1u << 0)) { ca_hva(v, a, b, c, d); st[0] = ng_cnexp(st[0], a, b, dt); st[1] = ng_cnexp(st[1], c, d, dt); }
-        if (m & (1u << 1)) { ca_lvast(v, a, b, c, d); st[2] = ng_cnexp(st[2], a, b, dt); st[3] = ng_cnexp(st[3], c, d, dt); }
+        if (m & (1u << 0)) { ca_hva(v, a, b, c, d); st[0] = ng_gate(st[0], a, b, dt); st[1] = ng_gate(st[1], c, d, dt); }
+        if (m & (1u << 1)) { ca_lvast(v, a, b, c, d); st[2] = ng_gate(st[2], a, b, dt); st[3] = ng_gate(st[3], c, d, dt); }
         if (m & (1u << 2)) {
-            double gamma = P(2), decay = P(3), depth = P(4), minCai = P(5);
-            double A = (-(10000.0)) * (((ica) * (gamma)) / (2.0 * NG_FARADAY * depth));
-            double B = ((-minCai)) / decay;
-            double C = (-(1.0)) / decay;
-            st[CAI] = st[CAI] + (1. - ng_exp(dt * ((-(1.0)) / decay))) * ((-(A - B)) / C - st[CAI]);
+            /* cai += (1 - e^{-dt/decay}) (decay A + minCai - cai), A = -1e4 ica gamma / (2 F depth) */
+            const double gamma = P(2), decay = P(3), depth = P(4), minCai = P(5);
+            const double A = (-(10000.0)) * ((ica * gamma) / ((2.0 * NG_FARADAY) * depth));
+            st[CAI] = st[CAI] + (1. - ng_expc(-(dt / decay))) * ((A * decay + minCai) - st[CAI]);
         }
-        if (m & (1u << 3)) { ih(v, a, b); st[4] = ng_cnexp(st[4], a, b, dt); }
-        if (m & (1u << 4)) { im(v, a, b); st[5] = ng_cnexp(st[5], a, b, dt); }
-        if (m & (1u << 5)) { k_pst(v, a, b, c, d); st[6] = ng_cnexp(st[6], a, b, dt); st[7] = ng_cnexp(st[7], c, d, dt); }
-        if (m & (1u << 6)) { k_tst(v, a, b, c, d); st[8] = ng_cnexp(st[8], a, b, dt); st[9] = ng_cnexp(st[9], c, d, dt); }
-        if (m & (1u << 7)) { nap_et2(v, a, b, c, d); st[10] = ng_cnexp(st[10], a, b, dt); st[11] = ng_cnexp(st[11], c, d, dt); }
-        if (m & (1u << 8)) { nata_t(v, a, b, c, d); st[12] = ng_cnexp(st[12], a, b, dt); st[13] = ng_cnexp(st[13], c, d, dt); }
-        if (m & (1u << 9)) { nats2_t(v, a, b, c, d); st[14] = ng_cnexp(st[14], a, b, dt); st[15] = ng_cnexp(st[15], c, d, dt); }
-        if (m & (1u << 11)) { double zi = sk_e2(st[CAI]); st[17] = ng_cnexp(st[17], zi, P(17), dt); }
-        if (m & (1u << 12)) { skv3_1(v, a, b); st[16] = ng_cnexp(st[16], a, b, dt); }
+        if (m & (1u << 3)) { ih(v, a, b); st[4] = ng_gate(st[4], a, b, dt); }
+        if (m & (1u << 4)) { im(v, a, b); st[5] = ng_gate(st[5], a, b, dt); }
+        if (m & (1u << 5)) { k_pst(v, a, b, c, d); st[6] = ng_gate(st[6], a, b, dt); st[7] = ng_gate(st[7], c, d, dt); }
+        if (m & (1u << 6)) { k_tst(v, a, b, c, d); st[8] = ng_gate(st[8], a, b, dt); st[9] = ng_gate(st[9], c, d, dt); }
+        if (m & (1u << 7)) { nap_et2(v, a, b, c, d); st[10] = ng_gate(st[10], a, b, dt); st[11] = ng_gate(st[11], c, d, dt); }
+        if (m & (1u << 8)) { nata_t(v, a, b, c, d); st[12] = ng_gate(st[12], a, b, dt); st[13] = ng_gate(st[13], c, d, dt); }
+        if (m & (1u << 9)) { nats2_t(v, a, b, c, d); st[14] = ng_gate(st[14], a, b, dt); st[15] = ng_gate(st[15], c, d, dt); }
+        if (m & (1u << 11)) { const double zi = sk_e2(st[CAI]); st[17] = ng_gate(st[17], zi, ng_rcp(P(17)), dt); }
+        if (m & (1u << 12)) { skv3_1(v, a, b); st[16] = ng_gate(st[16], a, b, dt); }
     }
+    /* BREAKPOINT blocks (AllModels.cu:382-516) */
     template <class PF>
     NG_DEV static void current(unsigned m, double v, const double *st, const PF &P, double &sum, double &ica, bool add_ica)
     {
@@ -257,7 +345,7 @@ struct SetTTPC {
         if (m & (1u << 3)) { g = P(6) * st[4]; i = g * (v - P(7)); s += i; }
         if (m & (1u << 4)) { g = P(8) * st[5]; i = g * (v - TT_EK); s += i; }
         if (m & (1u << 5)) { g = P(9) * st[6] * st[6] * st[7]; i = g * (v - TT_EK); s += i; }
-        if (m & (1u << 6)) { g = P(10) * ng_pow(st[8], 4.0) * st[9]; i = g * (v - TT_EK); s += i; }
+        if (m & (1u << 6)) { const double m2 = st[8] * st[8]; g = P(10) * (m2 * m2) * st[9]; i = g * (v - TT_EK); s += i; }
         if (m & (1u << 7)) { g = P(11) * st[10] * st[10] * st[10] * st[11]; i = g * (v - TT_ENA); s += i; }
         if (m & (1u << 8)) { g = P(12) * st[12] * st[12] * st[12] * st[13]; i = g * (v - TT_ENA); s += i; }
         if (m & (1u << 9)) { g = P(13) * st[14] * st[14] * st[14] * st[15]; i = g * (v - TT_ENA); s += i; }
@@ -276,7 +364,7 @@ struct SetTTPC {
  * State slots: 0,1 ca m,h | 2 cad ca | 3 kca n | 4 km n | 5 kv n | 6,7 na m,h |
  * 8 cai | 9 eca | 10 ica.  Globals of the generated model: celsius 37, tadj
  * 3.2093639532679714, vshift_na -5, depth_cad 0.09334562733124982,
- * cainf 1e-4, taur 200, ena 60, ek -90. */
+ * cainf 1e-4, taur 200, ena 60, ek -90.  Rate functions return (inf, 1/tau). */
 namespace mainen {
 #define MN_KTF (1000. * 8.3134 * (37.0 + 273.15) / NG_FARADAY)
 #define MN_TADJ 3.2093639532679714
@@ -290,44 +378,55 @@ namespace mainen {
 NG_DEV double eca_of(double cai) { return MN_KTF / 2 * ng_log(NG_DEF_CAO / cai); }
 NG_DEV double efun(double z)
 {
-    if (fabs(z) < 1e-4) return 1 - z / 2;
-    return z / (ng_exp(z) - 1);
+    if (fabs(z) < 1e-4) return 1 - z * 0.5;
+    return z / (ng_expc(z) - 1);
 }
-NG_DEV double trap0(double v, double th, double a, double q)
+NG_DEV void ca(double vm, double &mi, double &mr, double &hi, double &hr)
 {
-    if (fabs((v - th) / q) > 1e-6) return a * (v - th) / (1 - ng_exp(-(v - th) / q));
-    return a * q;
+    double a = 0.209 * efun(-(27.0 + vm) * (1.0 / 3.8));
+    double b = 0.94 * ng_expc((-75.0 - vm) * (1.0 / 17.0));
+    mr = (a + b) * MN_TADJ; mi = a / (a + b);
+    a = 0.000457 * ng_expc((-13.0 - vm) * (1.0 / 50.0));
+    b = 0.0065 / (ng_expc((-vm - 15.0) * (1.0 / 28.0)) + 1.0);
+    hr = (a + b) * MN_TADJ; hi = a / (a + b);
 }
-NG_DEV void ca(double vm, double &mi, double &mt, double &hi, double &ht)
+NG_DEV void kca(double cai, double Ra, double Rb, double caix, double &ni, double &nr)
 {
-    double a = 0.209 * efun(-(27.0 + vm) / 3.8);
-    double b = 0.94 * ng_exp((-75.0 - vm) / 17.0);
-    mt = 1.0 / MN_TADJ / (a + b); mi = a / (a + b);
-    a = 0.000457 * ng_exp((-13.0 - vm) / 50.0);
-    b = 0.0065 / (ng_exp((-vm - 15.0) / 28.0) + 1.0);
-    ht = 1.0 / MN_TADJ / (a + b); hi = a / (a + b);
+    const double a = Ra * ng_pow(cai, caix), b = Rb;
+    nr = (a + b) * MN_TADJ; ni = a / (a + b);
 }
-NG_DEV void kca(double cai, double Ra, double Rb, double caix, double &ni, double &nt)
+/* alpha = Ra qa efun(-z), beta = Rb qa efun(z), z = (v - tha)/qa; away from
+ * z = 0, efun(-z) = e^{z} efun(z) */
+NG_DEV void kx(double v, double tha, double qa, double Ra, double Rb, double &ni, double &nr)
 {
-    double a = Ra * ng_pow(cai, caix), b = Rb;
-    nt = 1.0 / MN_TADJ / (a + b); ni = a / (a + b);
+    const double z = (v - tha) / qa;
+    double ep, em;
+    if (fabs(z) < 1e-4) { ep = 1 - z * 0.5; em = 1 + z * 0.5; }
+    else { const double E = ng_expc(z); ep = z / (E - 1); em = E * ep; }
+    const double a = Ra * qa * em, b = Rb * qa * ep;
+    nr = (a + b) * MN_TADJ; ni = a / (a + b);
 }
-NG_DEV void kx(double v, double tha, double qa, double Ra, double Rb, double &ni, double &nt)
-{
-    double a = Ra * qa * efun(-(v - tha) / qa);
-    double b = Rb * qa * efun((v - tha) / qa);
-    nt = 1.0 / MN_TADJ / (a + b); ni = a / (a + b);
-}
+/* trap0(v, th, a, q) = a (v - th) / (1 - e^{-(v - th)/q}), or a q near v = th;
+ * the m pair trap0(vm, th, Ra, q), trap0(-vm, -th, Rb, q) shares the exp:
+ * with x = vm - th, E = e^{-x/q}, R = x / (1 - E): alpha = Ra R, beta = Rb R E */
 template <class PF>
-NG_DEV void na(double vm, const PF &P, double &mi, double &mt, double &hi, double &ht)
+NG_DEV void na(double vm, const PF &P, double &mi, double &mr, double &hi, double &hr)
 {
-    double a = trap0(vm, P(17), P(19), P(18));
-    double b = trap0(-vm, -P(17), P(20), P(18));
-    mt = 1.0 / MN_TADJ / (a + b); mi = a / (a + b);
-    a = trap0(vm, P(21), P(27), P(23));
-    b = trap0(-vm, -P(22), P(26), P(23));
-    ht = 1.0 / MN_TADJ / (a + b);
-    hi = 1.0 / (1.0 + ng_exp((vm - P(24)) / P(25)));
+    const double tha = P(17), qa = P(18);
+    const double x = vm - tha;
+    double a, b;
+    if (fabs(x / qa) > 1e-6) {
+        const double E = ng_expc(-x / qa);
+        const double R = x / (1 - E);
+        a = P(19) * R; b = P(20) * R * E;
+    } else { a = P(19) * qa; b = P(20) * qa; }
+    mr = (a + b) * MN_TADJ; mi = a / (a + b);
+    const double qi = P(23);
+    const double x1 = vm - P(21), x2 = -vm + P(22);
+    a = (fabs(x1 / qi) > 1e-6) ? P(27) * x1 / (1 - ng_expc(-x1 / qi)) : P(27) * qi;
+    b = (fabs(x2 / qi) > 1e-6) ? P(26) * x2 / (1 - ng_expc(-x2 / qi)) : P(26) * qi;
+    hr = (a + b) * MN_TADJ;
+    hi = 1.0 / (1.0 + ng_expc((vm - P(24)) / P(25)));
 }
 } // namespace mainen
 
@@ -356,7 +455,7 @@ struct SetMainen {
     {
         using namespace mainen;
         double a, b, c, d;
-        if (m & 1u) { st[ECA] = eca_of(st[CAI]); st[ECA] = eca_of(st[CAI]); ca(v + MN_VSHIFT_CA, a, b, c, d); st[0] = a; st[1] = c; }
+        if (m & 1u) { st[ECA] = eca_of(st[CAI]); ca(v + MN_VSHIFT_CA, a, b, c, d); st[0] = a; st[1] = c; }
         if (m & 2u) { st[CAI] = NG_DEF_CAI; st[ECA] = eca_of(st[CAI]); st[2] = MN_CAINF; st[CAI] = st[2]; }
         if (m & 4u) { st[ECA] = eca_of(st[CAI]); kca(st[CAI], P(4), P(5), P(3), a, b); st[3] = a; }
         if (m & 8u) { kx(v, P(7), P(8), P(9), P(10), a, b); st[4] = a; }
@@ -367,17 +466,18 @@ struct SetMainen {
     {
         using namespace mainen;
         double a, b, c, d;
-        if (m & 1u) { ca(v + MN_VSHIFT_CA, a, b, c, d); st[0] = ng_cnexp(st[0], a, b, dt); st[1] = ng_cnexp(st[1], c, d, dt); }
+        if (m & 1u) { ca(v + MN_VSHIFT_CA, a, b, c, d); st[0] = ng_gate(st[0], a, b, dt); st[1] = ng_gate(st[1], c, d, dt); }
         if (m & 2u) {
-            double drive = -(10000.0) * ica / (2.0 * NG_FARADAY * MN_DEPTH_CAD);
+            /* ca += (1 - e^{-dt/taur}) (drive taur + cainf - ca) */
+            double drive = ica * (-(10000.0) / (2.0 * NG_FARADAY * MN_DEPTH_CAD));
             if (drive <= 0.) drive = 0.;
-            st[2] = st[2] + (1. - ng_exp(dt * ((-1.0) / MN_TAUR))) * ((-(drive + (MN_CAINF) / MN_TAUR)) / ((-1.0) / MN_TAUR) - st[2]);
+            st[2] = st[2] + (1. - ng_expc(dt * (-1.0 / MN_TAUR))) * ((drive * MN_TAUR + MN_CAINF) - st[2]);
             st[CAI] = st[2];
         }
-        if (m & 4u) { kca(st[CAI], P(4), P(5), P(3), a, b); st[3] = ng_cnexp(st[3], a, b, dt); }
-        if (m & 8u) { kx(v, P(7), P(8), P(9), P(10), a, b); st[4] = ng_cnexp(st[4], a, b, dt); }
-        if (m & 16u) { kx(v, P(12), P(13), P(14), P(15), a, b); st[5] = ng_cnexp(st[5], a, b, dt); }
-        if (m & 32u) { na(v + MN_VSHIFT_NA, P, a, b, c, d); st[6] = ng_cnexp(st[6], a, b, dt); st[7] = ng_cnexp(st[7], c, d, dt); }
+        if (m & 4u) { kca(st[CAI], P(4), P(5), P(3), a, b); st[3] = ng_gate(st[3], a, b, dt); }
+        if (m & 8u) { kx(v, P(7), P(8), P(9), P(10), a, b); st[4] = ng_gate(st[4], a, b, dt); }
+        if (m & 16u) { kx(v, P(12), P(13), P(14), P(15), a, b); st[5] = ng_gate(st[5], a, b, dt); }
+        if (m & 32u) { na(v + MN_VSHIFT_NA, P, a, b, c, d); st[6] = ng_gate(st[6], a, b, dt); st[7] = ng_gate(st[7], c, d, dt); }
     }
     template <class PF>
     NG_DEV static void current(unsigned m, double v, const double *st, const PF &P, double &sum, double &ica, bool add_ica)

@@ -73,7 +73,7 @@ struct ngpu_ctx {
     std::vector<double> e, f, cm;
     std::vector<int> par, son, seg_to_comp;
     std::vector<unsigned> mask;
-    std::vector<int> chains, lvl_ptr, fath, flvl_ptr, fchild, fidx;
+    std::vector<int> chains, lvl_ptr, fath, flvl_ptr, fchild;
     /* sim */
     bool have_sim = false;
     double v_init = -65.0;
@@ -90,12 +90,11 @@ struct ngpu_ctx {
     std::vector<double> amps, dts;
     /* launch configuration */
     int cfg_w = 0, cfg_r = 0, cfg_spl = 0, cfg_graph = 0;
-    int cfg_fidx = -1;            /* ancestor table: -1 auto, 0 global memory, 1 LDS */
+    int cfg_stage = -1;            /* row coefficients: -1 auto, 0 global memory, 1 LDS */
     /* device state */
     int lay_W = 0, lay_slot = 0, lay_NQ = 0;
     bool tables_dirty = true, params_dirty = true, stim_dirty = true, graph_dirty = true;
     DevArr<int> d_qh, d_qph, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_rec;
-    DevArr<short> d_fidx16;
     DevArr<double> d_qcm, d_e, d_f, d_pval, d_pconst, d_pv, d_dts, d_amps, d_states, d_carry, d_vhot, d_times, d_fwin, d_feat;
     int stim_q = -1;
     std::vector<int> rec_q;
@@ -105,7 +104,7 @@ struct ngpu_ctx {
     bool ran = false;
     int last_launches = 0;
     int last_R = 1;
-    int last_fidx = 1;
+    int last_stage = 1;
 };
 
 static int fail(ngpu_ctx *c, int code, const char *fmt, ...)
@@ -187,7 +186,6 @@ extern "C" void ngpu_close(ngpu_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
     for (auto *d : {&c->d_qh, &c->d_qph, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_rec}) d->release();
-    c->d_fidx16.release();
     for (auto *d : {&c->d_qcm, &c->d_e, &c->d_f, &c->d_pval, &c->d_pconst, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_carry, &c->d_vhot, &c->d_times, &c->d_fwin, &c->d_feat}) d->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -281,10 +279,12 @@ extern "C" int ngpu_set_tree(ngpu_ctx *c, const ngpu_tree *t)
      * fathers keep their own phase. */
     {
         const int nlev = (int)lvl_ptr.size() - 1;
-        std::vector<int> ch4(chains.size() / 2 * 4);
+        std::vector<int> ch4(chains.size() / 2 * 5);
         for (size_t k = 0; k < chains.size() / 2; k++) {
-            ch4[4 * k] = chains[2 * k]; ch4[4 * k + 1] = chains[2 * k + 1];
-            ch4[4 * k + 2] = 0; ch4[4 * k + 3] = 0;
+            ch4[5 * k] = chains[2 * k]; ch4[5 * k + 1] = chains[2 * k + 1];
+            ch4[5 * k + 2] = 0; ch4[5 * k + 3] = 0;
+            const int end = chains[2 * k + 1] - 1;          /* row of the chain's last segment */
+            ch4[5 * k + 4] = end == N - 1 ? N : N - 1 - par[N - 1 - end];
         }
         std::vector<int> rest, rest_ptr{0};
         for (int L = 0; L + 1 < nlev; L++) {
@@ -295,12 +295,12 @@ extern "C" int ngpu_set_tree(ngpu_ctx *c, const ngpu_tree *t)
                 bool clash = false;
                 for (int k = k0; k < k1; k++) {
                     const int i0 = chains[2 * k], i1 = chains[2 * k + 1];
-                    if (J >= i0 - 1 && J < i1) { touching++; if (J == i0 - 1 && ch4[4 * k + 3] == 0) owner = k; }
+                    if (J >= i0 - 1 && J < i1) { touching++; if (J == i0 - 1 && ch4[5 * k + 3] == 0) owner = k; }
                     for (int cc = c0; cc < c1; cc++)
                         if (fchild[cc] >= i0 && fchild[cc] < i1) clash = true;
                 }
                 if (owner >= 0 && touching == 1 && !clash) {
-                    ch4[4 * owner + 2] = c0; ch4[4 * owner + 3] = c1;
+                    ch4[5 * owner + 2] = c0; ch4[5 * owner + 3] = c1;
                 } else {
                     rest.push_back(J); rest.push_back(c0); rest.push_back(c1);
                 }
@@ -311,17 +311,40 @@ extern "C" int ngpu_set_tree(ngpu_ctx *c, const ngpu_tree *t)
         fath.swap(rest);
         flvl_ptr.swap(rest_ptr);
     }
-    std::vector<int> fidx((size_t)t->logn_depth * N);
-    for (size_t i = 0; i < fidx.size(); i++) {
+    /* The back-substitution walks each chain from its last row down to its
+     * first: the chains' row ranges [i0-1, i1-1] must cover every row once,
+     * each row's parent inside a chain must be the next row, and the parent
+     * of a chain's last row must lie in a chain of a higher level. */
+    {
+        std::vector<int> lvl_of(N, -1);
+        const int nlev = (int)lvl_ptr.size() - 1;
+        for (int L = 0; L < nlev; L++)
+            for (int k = lvl_ptr[L]; k < lvl_ptr[L + 1]; k++)
+                for (int r = chains[5 * k] - 1; r < chains[5 * k + 1]; r++) {
+                    if (lvl_of[r] >= 0) return fail(c, NGPU_E_ARG, "row %d lies in two segment runs", r + 1);
+                    lvl_of[r] = L;
+                }
+        for (int r = 0; r < N; r++)
+            if (lvl_of[r] < 0) return fail(c, NGPU_E_ARG, "row %d lies in no segment run", r + 1);
+        for (int L = 0; L < nlev; L++)
+            for (int k = lvl_ptr[L]; k < lvl_ptr[L + 1]; k++) {
+                const int i0 = chains[5 * k], i1 = chains[5 * k + 1], pe = chains[5 * k + 4];
+                for (int r = i0 - 1; r < i1 - 1; r++)
+                    if (N - 1 - par[N - 1 - r] != r + 1)
+                        return fail(c, NGPU_E_ARG, "segment run %d..%d is not a path to the root", i0, i1);
+                if (pe < N && lvl_of[pe] <= L)
+                    return fail(c, NGPU_E_ARG, "the parent of row %d is not on a higher level", i1);
+            }
+    }
+    for (size_t i = 0; i < (size_t)t->logn_depth * N; i++) {
         int a = t->fidxs[i] - 1;
         if (a < 0 || a > N) return fail(c, NGPU_E_ARG, "FIdxs entry %d out of range", a + 1);
-        fidx[i] = a;
     }
     c->N = N; c->ncomps = t->ncomps; c->mechset = t->mechset; c->nmodels = t->nmodels;
     c->depth = t->depth; c->logn = t->logn_depth;
     c->e.assign(t->e, t->e + N); c->f.assign(t->f, t->f + N); c->cm.assign(t->cm, t->cm + N);
     c->par = par; c->son = son; c->seg_to_comp.assign(t->seg_to_comp, t->seg_to_comp + N); c->mask = mask;
-    c->chains = chains; c->lvl_ptr = lvl_ptr; c->fath = fath; c->flvl_ptr = flvl_ptr; c->fchild = fchild; c->fidx = fidx;
+    c->chains = chains; c->lvl_ptr = lvl_ptr; c->fath = fath; c->flvl_ptr = flvl_ptr; c->fchild = fchild;
     c->have_tree = true;
     c->have_params = false;          /* parameter layout depends on the tree */
     c->lay_W = 0; c->tables_dirty = true;
@@ -511,13 +534,12 @@ static int build_tables(ngpu_ctx *c, int W, int slot)
     for (size_t r = 0; r < c->rec.size(); r++) c->rec_q[r] = pos[c->rec[r]];
     c->stim_q = c->stim_seg >= 0 ? pos[c->stim_seg] : -1;
     std::vector<double> re(c->e), rf(c->f);
-    std::vector<short> fidx16(c->fidx.begin(), c->fidx.end());
     int rc;
     if ((rc = upload(c, c->d_pconst, c->pconst)) || (rc = upload(c, c->d_qh, qh)) || (rc = upload(c, c->d_qph, qph)) || (rc = upload(c, c->d_qmeta, qmeta)) ||
         (rc = upload(c, c->d_qcm, qcm)) || (rc = upload(c, c->d_e, re)) || (rc = upload(c, c->d_f, rf)) ||
         (rc = upload(c, c->d_pval, pval)) || (rc = upload(c, c->d_chains, c->chains)) ||
         (rc = upload(c, c->d_lvl, c->lvl_ptr)) || (rc = upload(c, c->d_fath, c->fath)) || (rc = upload(c, c->d_flvl, c->flvl_ptr)) ||
-        (rc = upload(c, c->d_fchild, c->fchild)) || (rc = upload(c, c->d_fidx16, fidx16)) || (rc = upload(c, c->d_rec, c->rec_q)))
+        (rc = upload(c, c->d_fchild, c->fchild)) || (rc = upload(c, c->d_rec, c->rec_q)))
         return rc;
     c->lay_W = W; c->lay_slot = slot; c->lay_NQ = NQ;
     c->tables_dirty = false;
@@ -536,30 +558,31 @@ static hipError_t launch_one(ngpu_ctx *c, const SimArgs &a, dim3 grid, dim3 bloc
 
 static const size_t kLdsLimit = 160 * 1024;
 
-static int npv_staged(const ngpu_ctx *c) { return c->nvar <= 512 ? std::max(c->nvar, 1) : 0; }
+/* LDS staging (row coefficients and the run's varying parameter values) is
+ * considered for up to this many varying values per run. */
+static const int kStageMaxVar = 512;
 
-static size_t lds_need(const ngpu_ctx *c, int R, int NQS, int fidx_lds)
+static size_t lds_need(const ngpu_ctx *c, int R, int NQS, int staged)
 {
-    return ng_lds_bytes(c->N, NQS, R, npv_staged(c), (int)c->rec.size(), c->logn, fidx_lds, (int)c->chains.size() / 4,
+    return ng_lds_bytes(c->N, NQS, R, std::max(c->nvar, 1), (int)c->rec.size(), staged, (int)c->chains.size() / 5,
                         (int)c->fath.size() / 3, (int)c->fchild.size(), c->depth);
 }
 
 /* Kernel arguments of a prepared context (tables built for its layout). */
-static void fill_args(const ngpu_ctx *c, SimArgs &a, int R, int fidx_lds, int block_sync)
+static void fill_args(const ngpu_ctx *c, SimArgs &a, int R, int staged, int block_sync)
 {
     a = SimArgs{};
-    const int npv = npv_staged(c);
     a.N = c->N; a.NQ = c->lay_NQ; a.LW = 64 * c->lay_W; a.runs_per_block = R;
     a.nvar = std::max(c->nvar, 1); a.nrec = (int)c->rec.size(); a.nstim = c->nstim; a.nt = c->nt; a.nt_amps = c->nt_amps;
-    a.nruns = c->nsets * c->nstim; a.depth = c->depth; a.logn = c->logn; a.stim_q = c->stim_q;
+    a.nruns = c->nsets * c->nstim; a.depth = c->depth; a.stim_q = c->stim_q;
     a.stim_area = c->area; a.v_init = c->v_init;
     a.q_h = c->d_qh.p; a.q_ph = c->d_qph.p; a.q_meta = c->d_qmeta.p; a.q_cm = c->d_qcm.p;
     a.row_e = c->d_e.p; a.row_f = c->d_f.p; a.pval = c->d_pval.p; a.pv = c->d_pv.p;
-    a.pconst_mask = c->pconst_mask; a.pv_in_lds = npv > 0 ? 1 : 0;
+    a.pconst_mask = c->pconst_mask;
     a.pconst = c->d_pconst.p;
     a.chains = c->d_chains.p; a.lvl_ptr = c->d_lvl.p; a.fath = c->d_fath.p; a.flvl_ptr = c->d_flvl.p;
-    a.fchild = c->d_fchild.p; a.fidx16 = c->d_fidx16.p; a.fidx_in_lds = fidx_lds; a.block_sync = block_sync;
-    a.nchains = (int)c->chains.size() / 4; a.nfath = (int)c->fath.size() / 3; a.nfchild = (int)c->fchild.size();
+    a.fchild = c->d_fchild.p; a.lds_staging = staged; a.block_sync = block_sync;
+    a.nchains = (int)c->chains.size() / 5; a.nfath = (int)c->fath.size() / 3; a.nfchild = (int)c->fchild.size();
     a.dts = c->d_dts.p; a.amps = c->d_amps.p; a.rec_q = c->d_rec.p;
     a.states = c->d_states.p; a.carry = c->d_carry.p; a.vhot = c->d_vhot.p;
     a.t0 = 0; a.t1 = c->nt;
@@ -571,17 +594,17 @@ static int enqueue(ngpu_ctx *c, int &launches)
     const int nruns = c->nsets * c->nstim;
     int R = 1;
     if (c->lay_W == 1) R = c->cfg_r > 0 ? c->cfg_r : 4;   /* 4 single-wave runs per workgroup */
-    /* the ancestor table goes to LDS when it fits (or as configured) */
-    int fidx = c->cfg_fidx;
-    if (fidx < 0) fidx = lds_need(c, 1, NQ, 1) <= kLdsLimit ? 1 : 0;
-    while (R > 1 && lds_need(c, R, NQ, fidx) > kLdsLimit) R--;
-    const size_t lds = lds_need(c, R, NQ, fidx);
+    /* the row coefficients go to LDS when they fit (or as configured) */
+    int stage = c->cfg_stage;
+    if (stage < 0) stage = c->nvar <= kStageMaxVar && lds_need(c, 1, NQ, 1) <= kLdsLimit ? 1 : 0;
+    while (R > 1 && lds_need(c, R, NQ, stage) > kLdsLimit) R--;
+    const size_t lds = lds_need(c, R, NQ, stage);
     if (lds > kLdsLimit)
         return fail(c, NGPU_E_ARG, "tree needs %zu bytes of LDS per workgroup (limit 160 KiB)", lds);
     SimArgs a;
-    fill_args(c, a, R, fidx, c->lay_W > 1);
+    fill_args(c, a, R, stage, c->lay_W > 1);
     c->last_R = R;
-    c->last_fidx = fidx;
+    c->last_stage = stage;
     dim3 block(LW * R), grid((nruns + R - 1) / R);
     int spl = c->cfg_spl > 0 ? c->cfg_spl : c->nt;
     launches = 0;
@@ -704,7 +727,7 @@ struct ngpu_batch {
     std::string err;
     std::vector<ngpu_ctx *> mem;
     std::vector<ngpu_batch_launch> cfg;
-    std::vector<int> lay_w, lay_sync, lay_fidx;
+    std::vector<int> lay_w, lay_sync, lay_stage;
     std::vector<double> t_first, t_last, cs;   /* per member, ms from the first item start */
     int4 *d_items = nullptr;
     int *d_head = nullptr;
@@ -788,8 +811,8 @@ extern "C" int ngpu_batch_set_launch(ngpu_batch *b, int member, const ngpu_batch
     if (member < 0 || member >= (int)b->mem.size()) return bfail(b, NGPU_E_ARG, "member %d out of range", member);
     const int w = cfg->waves_per_run;
     if (!(w == 0 || w == 1 || w == 2 || w == 4 || w == 8)) return bfail(b, NGPU_E_ARG, "waves_per_run %d not in {0,1,2,4,8}", w);
-    if (cfg->block_sync < -1 || cfg->block_sync > 1 || cfg->fidx_in_lds < -1 || cfg->fidx_in_lds > 1)
-        return bfail(b, NGPU_E_ARG, "block_sync and fidx_in_lds take -1, 0 or 1");
+    if (cfg->block_sync < -1 || cfg->block_sync > 1 || cfg->lds_staging < -1 || cfg->lds_staging > 1)
+        return bfail(b, NGPU_E_ARG, "block_sync and lds_staging take -1, 0 or 1");
     if (w > 1 && cfg->block_sync == 0) return bfail(b, NGPU_E_ARG, "runs of several waves synchronise at workgroup scope");
     b->cfg[member] = *cfg;
     return NGPU_OK;
@@ -813,7 +836,7 @@ extern "C" int ngpu_batch_run(ngpu_batch *b, ngpu_batch_stats *st)
     std::vector<int4> items;
     std::vector<double> cost;
     size_t lds = 0;
-    b->lay_w.assign(nm, 0); b->lay_sync.assign(nm, 0); b->lay_fidx.assign(nm, 0);
+    b->lay_w.assign(nm, 0); b->lay_sync.assign(nm, 0); b->lay_stage.assign(nm, 0);
     for (int i = 0; i < nm; i++) {
         ngpu_ctx *c = b->mem[i];
         if (!c->have_params || !c->have_stim) return bfail(b, NGPU_E_STATE, "member %d lacks parameters or stimuli", i);
@@ -826,14 +849,14 @@ extern "C" int ngpu_batch_run(ngpu_batch *b, ngpu_batch_stats *st)
         const int R = NG_BATCH_THREADS / (64 * W), NQS = NG_BATCH_NSLOT * 64 * W;
         int sync = b->cfg[i].block_sync;
         if (sync < 0) sync = W > 1 ? 1 : 0;
-        int fidx = b->cfg[i].fidx_in_lds;
-        if (fidx < 0) fidx = lds_need(c, R, NQS, 1) <= kLdsLimit ? 1 : 0;
-        const size_t need = lds_need(c, R, NQS, fidx);
+        int stage = b->cfg[i].lds_staging;
+        if (stage < 0) stage = c->nvar <= kStageMaxVar && lds_need(c, R, NQS, 1) <= kLdsLimit ? 1 : 0;
+        const size_t need = lds_need(c, R, NQS, stage);
         if (need > kLdsLimit) return bfail(b, NGPU_E_ARG, "member %d needs %zu bytes of LDS per workgroup (limit 160 KiB)", i, need);
         lds = std::max(lds, need);
-        fill_args(c, args[i], R, fidx, sync);
+        fill_args(c, args[i], R, stage, sync);
         args[i].nslot = slot;
-        b->lay_w[i] = W; b->lay_sync[i] = sync; b->lay_fidx[i] = fidx;
+        b->lay_w[i] = W; b->lay_sync[i] = sync; b->lay_stage[i] = stage;
         const int nruns = c->nsets * c->nstim;
         for (int r0 = 0; r0 < nruns; r0 += R) {
             items.push_back(make_int4(i, r0, std::min(R, nruns - r0), 0));
@@ -916,7 +939,7 @@ extern "C" int ngpu_batch_member_info(const ngpu_batch *b, int member, ngpu_batc
     out->waves_per_run = b->lay_w[member];
     out->runs_per_item = NG_BATCH_THREADS / (64 * b->lay_w[member]);
     out->block_sync = b->lay_sync[member];
-    out->fidx_in_lds = b->lay_fidx[member];
+    out->lds_staging = b->lay_stage[member];
     return NGPU_OK;
 }
 

@@ -11,9 +11,14 @@
  *      compartment once, apply the previous step's cnexp update at the current
  *      v, evaluate BREAKPOINT at v+0.001 and v, store the states once;
  *   2. assembly of the Hines rows into LDS (rhs, diagonal, child scatter);
- *   3. triangularisation by tree levels: each lane eliminates one run of
- *      consecutive rows, then each father folds in its related run ends;
- *   4. back-substitution by pointer jumping over the ancestor tables;
+ *   3. triangularisation by tree levels: each lane eliminates one chain of
+ *      consecutive rows (a section), folding the ends of finished child
+ *      chains into the chain's first row, with the reference's arithmetic;
+ *      a row's pivot quotients PX = b/u and PF = -f/u are formed with one
+ *      reciprocal as soon as the row has been eliminated and replace b and u;
+ *   4. back-substitution by the same chains in reverse level order,
+ *      x_i = PX_i + PF_i * x_parent(i), the parent of a chain's end row
+ *      coming from a chain of a higher level;
  *   5. v += x.
  *
  * Per-lane values of the lane's positions (v, dv) live in registers and are
@@ -36,16 +41,22 @@
  *                     an item is up to 8/W runs of one morphology, so runs of
  *                     uneven size and step count share one launch.
  *
- * The algorithm and arithmetic follow the reference kernel:
+ * The algorithm follows the reference kernel:
  *   assembly        Code/OneKernel/src/CudaStuff.cu:263-305
  *   BeforeLU        Code/OneKernel/src/CudaStuff.cu:31-69
- *   BkSub (BKSUB1)  Code/OneKernel/src/CudaStuff.cu:71-103
  *   v update        Code/OneKernel/src/CudaStuff.cu:312-315
  *   record/stimulus Code/OneKernel/src/CudaStuff.cu:241-275
- * Deliberate differences: states are per run (the reference shares them
- * between the stimuli of a block), sibling scatter is ordered (the reference
- * races), duplicate padded level entries are not replayed, and the cnexp of
- * step t-1 is executed at the start of step t on the same loaded state.
+ * The elimination visits the rows in the reference's level order; the
+ * back-substitution solves the same triangular system as its pointer-jumping
+ * BKSUB1 (CudaStuff.cu:71-103) row by row instead of in log2(depth) rounds
+ * over the whole tree.  Deliberate differences: states are per run (the
+ * reference shares them between the stimuli of a block), sibling scatter is
+ * ordered (the reference races), duplicate padded level entries are not
+ * replayed, the cnexp of step t-1 is executed at the start of step t on the
+ * same loaded state.  The elimination keeps the reference's operation order
+ * (u_i -= f (e / u_{i-1}), b_i -= b_{i-1} e / u_{i-1}): the reference trees
+ * couple zero-area nodes with coefficients near 1e22, and there the order of
+ * these operations, not the rounding of the membrane terms, sets the result.
  */
 #ifndef NGPU_SIM_KERNEL_H
 #define NGPU_SIM_KERNEL_H
@@ -57,7 +68,7 @@
 
 struct SimArgs {
     int N, NQ, LW, runs_per_block;
-    int nvar, nrec, nstim, nt, nt_amps, t0, t1, nruns, depth, logn;
+    int nvar, nrec, nstim, nt, nt_amps, t0, t1, nruns, depth;
     int stim_q;
     double stim_area, v_init;
     const int *q_h, *q_ph, *q_meta;      /* [NQ]: row, parent row, mask | son<<24 (-1 = padding) */
@@ -66,12 +77,13 @@ struct SimArgs {
     const double *pval;                  /* [NPARAM*NQ] value, or tagged NaN = varying column */
     const double *pv;                    /* [nsets*nvar] varying parameter values */
     unsigned pconst_mask;                /* bit j: parameter j has one value wherever it is used */
-    int pv_in_lds;                       /* stage the run's varying values in LDS */
     const double *pconst;                /* [NPARAM] those values */
-    const int *chains, *lvl_ptr;         /* chains: (i0, i1, c0, c1); lvl_ptr[depth+2] */
+    const int *chains, *lvl_ptr;         /* chains: (i0, i1, c0, c1, pe); lvl_ptr[depth+2] */
     const int *fath, *flvl_ptr, *fchild; /* fathers with their own phase: (J, c0, c1); flvl_ptr[depth+1] */
-    const short *fidx16;                 /* [logn*N] 0-based ancestor row, N = beyond root */
-    int fidx_in_lds;                     /* stage the ancestor table in LDS */
+    int lds_staging;                     /* row coefficients e, f and the run's varying
+                                            parameter values staged in LDS (else read
+                                            from global memory); selects the kernel's
+                                            STAGE instantiation */
     int block_sync;                      /* batch kernel: synchronise levels at workgroup scope */
     int nslot;                           /* batch kernel: positions per lane of this layout (NQ / LW) */
     int nchains, nfath, nfchild;         /* schedule sizes (staged into LDS as int16) */
@@ -97,21 +109,21 @@ struct SyncDyn {
 };
 
 /* LDS layout of a workgroup, in this order:
- *   RE[N], RF[N]                                   doubles (by Hines row)
+ *   RE[N], RF[N]                                   doubles (by Hines row), when staged
  *   R per-run regions: X[N+1], U[N+1], PV[npv], REC[64*nrec]   doubles
+ *                                                  (npv = nvar when staged, else 0)
  *   QM[NQS] int, QH[NQS] short, QP[NQS] short      NQS = NSLOT*LW positions
- *   FIDX[logn*N] short                             only when staged
- *   CH[4*nchains], FA[3*nfath], FC[nfchild], LP[depth+2], FLP[depth+1] short */
+ *   CH[5*nchains], FA[3*nfath], FC[nfchild], LP[depth+2], FLP[depth+1] short */
 __host__ __device__ inline size_t ng_lds_per_run(int N, int npv, int nrec)
 {
     return (size_t)(2 * (N + 1) + npv + 64 * nrec);
 }
-__host__ __device__ inline size_t ng_lds_bytes(int N, int NQS, int R, int npv, int nrec, int logn, int fidx_in_lds,
+__host__ __device__ inline size_t ng_lds_bytes(int N, int NQS, int R, int npv, int nrec, int staged,
                                                int nchains, int nfath, int nfchild, int depth)
 {
-    const size_t nshort = 2 * (size_t)NQS + (fidx_in_lds ? (size_t)logn * N : 0) + 4 * (size_t)nchains +
-                          3 * (size_t)nfath + (size_t)nfchild + (size_t)(depth + 2) + (size_t)(depth + 1);
-    const size_t dbl = 2 * (size_t)N + (size_t)R * ng_lds_per_run(N, npv, nrec);
+    const size_t nshort = 2 * (size_t)NQS + 5 * (size_t)nchains + 3 * (size_t)nfath + (size_t)nfchild +
+                          (size_t)(depth + 2) + (size_t)(depth + 1);
+    const size_t dbl = (staged ? 2 * (size_t)N : 0) + (size_t)R * ng_lds_per_run(N, staged ? npv : 0, nrec);
     return dbl * sizeof(double) + (((size_t)NQS * sizeof(int) + nshort * sizeof(short) + 7) / 8) * 8;
 }
 
@@ -137,43 +149,49 @@ struct ParamFetch {
     }
 };
 
-/* Pointers into the block-shared part of the LDS layout. */
+/* Pointers into the block-shared part of the LDS layout; RE/RF point to LDS
+ * or to global memory depending on the STAGE instantiation, so that every
+ * access compiles to an LDS or a global load (never a flat one). */
 struct Tables {
-    double *RE, *RF;
+    const double *RE, *RF;               /* by Hines row */
+    double *runs;                        /* first per-run region */
     int *QM;
     short *QH, *QP;
-    short *FIDX;                         /* NULL: read a.fidx16 from global memory */
     short *CH, *FA, *FC, *LP, *FLP;
 };
 
 /* Stage the block-shared tables of morphology a for R runs and NQS = NSLOT*LW
  * positions; positions beyond the morphology's own table are padding. */
-__device__ __forceinline__ Tables stage_tables(const SimArgs &a, double *lds, int R, int NQS, int npv)
+template <bool STAGE>
+__device__ __forceinline__ Tables stage_tables(const SimArgs &a, double *lds, int R, int NQS)
 {
     const int N = a.N;
+    const int npv = STAGE ? a.nvar : 0;
     Tables T;
-    T.RE = lds;
-    T.RF = lds + N;
-    T.QM = (int *)(lds + 2 * N + (size_t)R * ng_lds_per_run(N, npv, a.nrec));
+    if constexpr (STAGE) {
+        double *re = lds, *rf = lds + N;
+        for (int i = threadIdx.x; i < N; i += blockDim.x) { re[i] = a.row_e[i]; rf[i] = a.row_f[i]; }
+        T.RE = re; T.RF = rf;
+        T.runs = lds + 2 * N;
+    } else {
+        T.RE = a.row_e; T.RF = a.row_f;
+        T.runs = lds;
+    }
+    T.QM = (int *)(T.runs + (size_t)R * ng_lds_per_run(N, npv, a.nrec));
     T.QH = (short *)(T.QM + NQS);
     T.QP = T.QH + NQS;
-    short *next = T.QP + NQS;
-    T.FIDX = a.fidx_in_lds ? next : nullptr;
-    T.CH = next + (a.fidx_in_lds ? a.logn * N : 0);
-    T.FA = T.CH + 4 * a.nchains;
+    T.CH = T.QP + NQS;
+    T.FA = T.CH + 5 * a.nchains;
     T.FC = T.FA + 3 * a.nfath;
     T.LP = T.FC + a.nfchild;
     T.FLP = T.LP + (a.depth + 2);
-    for (int i = threadIdx.x; i < N; i += blockDim.x) { T.RE[i] = a.row_e[i]; T.RF[i] = a.row_f[i]; }
     for (int i = threadIdx.x; i < NQS; i += blockDim.x) {
         const bool in = i < N;
         T.QM[i] = in ? a.q_meta[i] : 0;
         T.QH[i] = (short)(in ? a.q_h[i] : -1);
         T.QP[i] = (short)(in ? a.q_ph[i] : 0);
     }
-    if (a.fidx_in_lds)
-        for (int i = threadIdx.x; i < a.logn * N; i += blockDim.x) T.FIDX[i] = a.fidx16[i];
-    for (int i = threadIdx.x; i < 4 * a.nchains; i += blockDim.x) T.CH[i] = (short)a.chains[i];
+    for (int i = threadIdx.x; i < 5 * a.nchains; i += blockDim.x) T.CH[i] = (short)a.chains[i];
     for (int i = threadIdx.x; i < 3 * a.nfath; i += blockDim.x) T.FA[i] = (short)a.fath[i];
     for (int i = threadIdx.x; i < a.nfchild; i += blockDim.x) T.FC[i] = (short)a.fchild[i];
     for (int i = threadIdx.x; i < a.depth + 2; i += blockDim.x) T.LP[i] = (short)a.lvl_ptr[i];
@@ -181,17 +199,30 @@ __device__ __forceinline__ Tables stage_tables(const SimArgs &a, double *lds, in
     return T;
 }
 
+/* Eliminate row tr (already final) into the running father values (uJ, bJ)
+ * with the reference's arithmetic (CudaStuff.cu:53-64), then replace its u
+ * and b by its pivot quotients PF = -f/u and PX = b/u. */
+__device__ __forceinline__ void ng_fold(const double *RE, const double *RF, double *X, double *U, int tr, double &uJ,
+                                        double &bJ)
+{
+    const double ut = U[tr], bt = X[tr], e = RE[tr], f = RF[tr];
+    uJ = uJ - f * (e / ut);
+    bJ = bJ - bt * e / ut;
+    const double ru = ng_rcp(ut);
+    U[tr] = -(f * ru); X[tr] = bt * ru;
+}
+
 /* Steps t0..t1 of one run by the LW lanes of its group (lane g).  X is the
  * run's LDS region.  Runs from nruns up to the next multiple of 8 are padding
  * (buffers are allocated for them, their parameters are the last set's): the
  * batch kernel gives them to the unused groups of a partial item. */
-template <class SET, int NSLOT, class SYNC>
+template <class SET, int NSLOT, bool STAGE, class SYNC>
 __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, double *X, int run, int g,
                                         const SYNC &sync)
 {
     const int LW = a.LW, N = a.N, NQ = a.NQ;
-    const int npv = a.pv_in_lds ? a.nvar : 0;
-    double *RE = T.RE, *RF = T.RF;
+    const int npv = STAGE ? a.nvar : 0;
+    const double *RE = T.RE, *RF = T.RF;
     int *QM = T.QM;
     short *QH = T.QH, *QP = T.QP, *CH = T.CH, *FA = T.FA, *FC = T.FC, *LP = T.LP, *FLP = T.FLP;
     double *U = X + (N + 1);                              /* diagonal, then PF          */
@@ -203,7 +234,7 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
     double *CY = a.carry + (size_t)run * (2 * NQ + N + 1);
     const double *pvg = a.pv + (size_t)set * a.nvar;
     const double *pvsrc = pvg;
-    if (a.pv_in_lds) {
+    if constexpr (STAGE) {
         for (int k = g; k < a.nvar; k += LW) PV[k] = pvg[k];
         pvsrc = PV;
     }
@@ -236,7 +267,6 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
             if (q < N) { v[c] = CY[q]; dv[c] = CY[NQ + q]; }
         }
     }
-    if (g == 0) { X[N] = 0.0; U[N] = 1.0; }
     sync();
 
     double dtp = a.t0 > 0 ? a.dts[a.t0 - 1] : 0.0;
@@ -329,97 +359,68 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
 
         /* triangularisation by levels (BeforeLU, CudaStuff.cu:31-69).  Level L
          * first folds the fathers of level L-1 that have their own phase, then
-         * eliminates its chains; a chain (i0, i1, c0, c1) with c0 < c1 first
-         * folds the father row i0-1 over the related rows FC[c0..c1). */
+         * eliminates its chains; a chain (i0, i1, c0, c1, pe) folds the
+         * related rows FC[c0..c1) into its first row i0-1, then eliminates
+         * rows i0-1 .. i1-2 each into the next; its last row i1-1 keeps u
+         * and b until the father above folds it. */
         for (int L = 0; L <= a.depth; L++) {
             if (L > 0 && FLP[L] > FLP[L - 1]) {
                 const int f1e = FLP[L];
                 for (int k = FLP[L - 1] + g; k < f1e; k += LW) {
                     const int J = FA[3 * k], c0 = FA[3 * k + 1], c1 = FA[3 * k + 2];
                     double uJ = U[J], bJ = X[J];
-                    for (int cc = c0; cc < c1; cc++) {
-                        const int tr = FC[cc];
-                        const double ut = U[tr], bt = X[tr], e1 = RE[tr], f1 = RF[tr];
-                        uJ = uJ - f1 * (e1 / ut);
-                        bJ = bJ - bt * e1 / ut;
-                    }
+                    for (int cc = c0; cc < c1; cc++) ng_fold(RE, RF, X, U, FC[cc], uJ, bJ);
                     U[J] = uJ; X[J] = bJ;
                 }
                 sync();
             }
             const int k1 = LP[L + 1];
             for (int k = LP[L] + g; k < k1; k += LW) {
-                const int i0 = CH[4 * k], i1 = CH[4 * k + 1], c0 = CH[4 * k + 2], c1 = CH[4 * k + 3];
-                double um1 = U[i0 - 1], bm1 = X[i0 - 1];
-                if (c0 < c1) {
-                    for (int cc = c0; cc < c1; cc++) {
-                        const int tr = FC[cc];
-                        const double ut = U[tr], bt = X[tr], e1 = RE[tr], f1 = RF[tr];
-                        um1 = um1 - f1 * (e1 / ut);
-                        bm1 = bm1 - bt * e1 / ut;
-                    }
-                    U[i0 - 1] = um1; X[i0 - 1] = bm1;
-                }
-                if (i0 >= i1) continue;
-                double e1 = RE[i0 - 1], f1 = RF[i0 - 1], ui = U[i0], xi = X[i0];
+                const short *ch = CH + 5 * k;
+                const int i0 = ch[0], i1 = ch[1], c0 = ch[2], c1 = ch[3];
+                double u = U[i0 - 1], b = X[i0 - 1];
+                for (int cc = c0; cc < c1; cc++) ng_fold(RE, RF, X, U, FC[cc], u, b);
+                /* row i-1 eliminated into row i with the reference's arithmetic
+                 * (CudaStuff.cu:44-50); the next row's values are loaded one
+                 * iteration ahead */
+                double e = 0.0, f = 0.0, un = 0.0, bn = 0.0;
+                if (i0 < i1) { e = RE[i0 - 1]; f = RF[i0 - 1]; un = U[i0]; bn = X[i0]; }
                 for (int i = i0; i < i1; i++) {
-                    double e1n = 0.0, f1n = 0.0, un = 0.0, xn = 0.0;
-                    if (i + 1 < i1) { e1n = RE[i]; f1n = RF[i]; un = U[i + 1]; xn = X[i + 1]; }
-                    const double unew = ui - f1 * (e1 / um1);
-                    const double bnew = xi - bm1 * e1 / um1;
-                    U[i] = unew; X[i] = bnew;
-                    um1 = unew; bm1 = bnew;
-                    e1 = e1n; f1 = f1n; ui = un; xi = xn;
+                    double e2 = 0.0, f2 = 0.0, un2 = 0.0, bn2 = 0.0;
+                    if (i + 1 < i1) { e2 = RE[i]; f2 = RF[i]; un2 = U[i + 1]; bn2 = X[i + 1]; }
+                    const double unew = un - f * (e / u);
+                    const double bnew = bn - b * e / u;
+                    const double ru = ng_rcp(u);
+                    U[i - 1] = -(f * ru); X[i - 1] = b * ru;
+                    u = unew; b = bnew;
+                    e = e2; f = f2; un = un2; bn = bn2;
                 }
+                U[i1 - 1] = u; X[i1 - 1] = b;
             }
             sync();
         }
 
-        /* back-substitution by pointer jumping (BKSUB1, CudaStuff.cu:71-103);
-         * lanes own rows h = c*LW + g */
-#pragma unroll
-        for (int c = 0; c < NSLOT; c++) {
-            const int h = c * LW + g;
-            if (h < N) {
-                const double u = U[h];
-                X[h] = X[h] / u;
-                U[h] = -RF[h] / u;
-            }
-        }
-        sync();
-        for (int r = 0; r < a.logn; r++) {
-            double nx[NSLOT], nf[NSLOT];
-            int an[NSLOT];
-            if (T.FIDX) {
-                const short *fr = T.FIDX + r * N;
-#pragma unroll
-                for (int c = 0; c < NSLOT; c++) {
-                    const int h = c * LW + g;
-                    an[c] = h < N ? fr[h] : N;
+        /* back-substitution by chains, highest level first: a chain's last
+         * row takes x of its parent row pe (a row of a higher level; pe = N
+         * marks the root, whose x is b/u), then x_i = PX_i + PF_i x_{i+1}
+         * down to the chain's first row */
+        for (int L = a.depth; L >= 0; L--) {
+            const int k1 = LP[L + 1];
+            for (int k = LP[L] + g; k < k1; k += LW) {
+                const short *ch = CH + 5 * k;
+                const int i0 = ch[0], i1 = ch[1], pe = ch[4];
+                const int h = i1 - 1;
+                double x = pe < N ? X[h] + U[h] * X[pe] : X[h] / U[h];
+                X[h] = x;
+                double px = 0.0, pf = 0.0;
+                if (h - 1 >= i0 - 1) { px = X[h - 1]; pf = U[h - 1]; }
+                for (int i = h - 1; i >= i0 - 1; i--) {
+                    double px2 = 0.0, pf2 = 0.0;
+                    if (i - 1 >= i0 - 1) { px2 = X[i - 1]; pf2 = U[i - 1]; }
+                    x = px + pf * x;
+                    X[i] = x;
+                    px = px2; pf = pf2;
                 }
-            } else {
-                const short *fr = a.fidx16 + r * N;
-#pragma unroll
-                for (int c = 0; c < NSLOT; c++) {
-                    const int h = c * LW + g;
-                    an[c] = h < N ? fr[h] : N;
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < NSLOT; c++) {
-                const int h = c * LW + g;
-                nx[c] = 0.0; nf[c] = 0.0;
-                if (h < N) {
-                    const double pxj = X[h], pfj = U[h];
-                    nx[c] = pxj + X[an[c]] * pfj;
-                    nf[c] = pfj * U[an[c]];
-                }
-            }
-            sync();
-#pragma unroll
-            for (int c = 0; c < NSLOT; c++) {
-                const int h = c * LW + g;
-                if (h < N) { X[h] = nx[c]; U[h] = nf[c]; }
             }
             sync();
         }
@@ -455,9 +456,59 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
 
 #ifndef NG_WAVES_PER_EU
 #define NG_WAVES_PER_EU 2
+/* Host-side launchers of one mechanism set's kernels (instantiated once per
+ * set, in sim_<set>.hip). */
+template <class SET, int NS, bool MULTI, bool STAGE>
+static hipError_t ng_launch_sim(dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a)
+{
+    auto k = ngpu_sim_kernel<SET, NS, MULTI, STAGE>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k, grid, block, lds, s, a);
+    return hipGetLastError();
+}
+
+template <class SET, int NS>
+static hipError_t ng_launch_sim_ns(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a)
+{
+    if (multi) return a.lds_staging ? ng_launch_sim<SET, NS, true, true>(grid, block, lds, s, a)
+                                    : ng_launch_sim<SET, NS, true, false>(grid, block, lds, s, a);
+    return a.lds_staging ? ng_launch_sim<SET, NS, false, true>(grid, block, lds, s, a)
+                         : ng_launch_sim<SET, NS, false, false>(grid, block, lds, s, a);
+}
+
+/* nslot positions per lane (1, 2, 4, 6 or 8); multi: more than one wave per run */
+template <class SET>
+static hipError_t ng_launch_set(int nslot, bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s,
+                                const SimArgs &a)
+{
+    switch (nslot) {
+    case 1: return ng_launch_sim_ns<SET, 1>(multi, grid, block, lds, s, a);
+    case 2: return ng_launch_sim_ns<SET, 2>(multi, grid, block, lds, s, a);
+    case 4: return ng_launch_sim_ns<SET, 4>(multi, grid, block, lds, s, a);
+    case 6: return ng_launch_sim_ns<SET, 6>(multi, grid, block, lds, s, a);
+    case 8: return ng_launch_sim_ns<SET, 8>(multi, grid, block, lds, s, a);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <class SET>
+static hipError_t ng_launch_batch_set(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b)
+{
+    auto k = ngpu_batch_kernel<SET, NG_BATCH_NSLOT, 4>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k, grid, dim3(NG_BATCH_THREADS), lds, s, b);
+    return hipGetLastError();
+}
+
 #endif
 
-template <class SET, int NSLOT, bool MULTI>
+template <class SET, int NSLOT, bool MULTI, bool STAGE>
 __global__ void __launch_bounds__(MULTI ? 512 : 256) __attribute__((amdgpu_waves_per_eu(NG_WAVES_PER_EU)))
 ngpu_sim_kernel(SimArgs a)
 {
@@ -465,14 +516,13 @@ ngpu_sim_kernel(SimArgs a)
     const int LW = a.LW;
     const int group = threadIdx.x / LW;
     const int g = threadIdx.x - group * LW;
-    const int npv = a.pv_in_lds ? a.nvar : 0;
-    const Tables T = stage_tables(a, lds, a.runs_per_block, NSLOT * LW, npv);
+    const Tables T = stage_tables<STAGE>(a, lds, a.runs_per_block, NSLOT * LW);
     __syncthreads();
     const int run = blockIdx.x * a.runs_per_block + group;
     if (run >= a.nruns) return;          /* W > 1 implies one run per workgroup */
-    double *X = lds + 2 * a.N + (size_t)group * ng_lds_per_run(a.N, npv, a.nrec);
-    if constexpr (MULTI) sim_run<SET, NSLOT>(a, T, X, run, g, SyncBlock{});
-    else sim_run<SET, NSLOT>(a, T, X, run, g, SyncWave{});
+    double *X = T.runs + (size_t)group * ng_lds_per_run(a.N, STAGE ? a.nvar : 0, a.nrec);
+    if constexpr (MULTI) sim_run<SET, NSLOT, STAGE>(a, T, X, run, g, SyncBlock{});
+    else sim_run<SET, NSLOT, STAGE>(a, T, X, run, g, SyncWave{});
 }
 
 /* Work queue of the batch kernel: item i runs runs [y, y + z) of morphology x.
@@ -513,16 +563,74 @@ ngpu_batch_kernel(BatchArgs b)
         const int LW = a.LW, R = NG_BATCH_THREADS / LW;
         const int group = threadIdx.x / LW;
         const int g = threadIdx.x - group * LW;
-        const int npv = a.pv_in_lds ? a.nvar : 0;
-        const Tables T = stage_tables(a, lds, R, NSLOT * LW, npv);
-        __syncthreads();
+        const SyncDyn sync{a.block_sync != 0};
         (void)nrun;                      /* groups beyond nrun simulate padding runs */
-        double *X = lds + 2 * a.N + (size_t)group * ng_lds_per_run(a.N, npv, a.nrec);
-        if (a.nslot <= NSLOT_SMALL) sim_run<SET, NSLOT_SMALL>(a, T, X, run0 + group, g, SyncDyn{a.block_sync != 0});
-        else sim_run<SET, NSLOT>(a, T, X, run0 + group, g, SyncDyn{a.block_sync != 0});
+        if (a.lds_staging) {
+            const Tables T = stage_tables<true>(a, lds, R, NSLOT * LW);
+            __syncthreads();
+            double *X = T.runs + (size_t)group * ng_lds_per_run(a.N, a.nvar, a.nrec);
+            if (a.nslot <= NSLOT_SMALL) sim_run<SET, NSLOT_SMALL, true>(a, T, X, run0 + group, g, sync);
+            else sim_run<SET, NSLOT, true>(a, T, X, run0 + group, g, sync);
+        } else {
+            const Tables T = stage_tables<false>(a, lds, R, NSLOT * LW);
+            __syncthreads();
+            double *X = T.runs + (size_t)group * ng_lds_per_run(a.N, 0, a.nrec);
+            if (a.nslot <= NSLOT_SMALL) sim_run<SET, NSLOT_SMALL, false>(a, T, X, run0 + group, g, sync);
+            else sim_run<SET, NSLOT, false>(a, T, X, run0 + group, g, sync);
+        }
         __syncthreads();
         if (threadIdx.x == 0) b.stamps[2 * it + 1] = wall_clock64();
     }
+}
+
+/* Host-side launchers of one mechanism set's kernels (instantiated once per
+ * set, in sim_<set>.hip). */
+template <class SET, int NS, bool MULTI, bool STAGE>
+static hipError_t ng_launch_sim(dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a)
+{
+    auto k = ngpu_sim_kernel<SET, NS, MULTI, STAGE>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k, grid, block, lds, s, a);
+    return hipGetLastError();
+}
+
+template <class SET, int NS>
+static hipError_t ng_launch_sim_ns(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a)
+{
+    if (multi) return a.lds_staging ? ng_launch_sim<SET, NS, true, true>(grid, block, lds, s, a)
+                                    : ng_launch_sim<SET, NS, true, false>(grid, block, lds, s, a);
+    return a.lds_staging ? ng_launch_sim<SET, NS, false, true>(grid, block, lds, s, a)
+                         : ng_launch_sim<SET, NS, false, false>(grid, block, lds, s, a);
+}
+
+/* nslot positions per lane (1, 2, 4, 6 or 8); multi: more than one wave per run */
+template <class SET>
+static hipError_t ng_launch_set(int nslot, bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s,
+                                const SimArgs &a)
+{
+    switch (nslot) {
+    case 1: return ng_launch_sim_ns<SET, 1>(multi, grid, block, lds, s, a);
+    case 2: return ng_launch_sim_ns<SET, 2>(multi, grid, block, lds, s, a);
+    case 4: return ng_launch_sim_ns<SET, 4>(multi, grid, block, lds, s, a);
+    case 6: return ng_launch_sim_ns<SET, 6>(multi, grid, block, lds, s, a);
+    case 8: return ng_launch_sim_ns<SET, 8>(multi, grid, block, lds, s, a);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+template <class SET>
+static hipError_t ng_launch_batch_set(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b)
+{
+    auto k = ngpu_batch_kernel<SET, NG_BATCH_NSLOT, 4>;
+    if (lds > 65536) {
+        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k, grid, dim3(NG_BATCH_THREADS), lds, s, b);
+    return hipGetLastError();
 }
 
 #endif

@@ -75,9 +75,9 @@ def test_batch_matches_single_runs_and_oracle(choice):
             for i, s in enumerate(sims):
                 b.add(s)
                 if choice == "wave_global" and ws[i].model.tree.N <= 384:
-                    b.set_launch(i, waves_per_run=1, block_sync=0, fidx_in_lds=0)
+                    b.set_launch(i, waves_per_run=1, block_sync=0, lds_staging=0)
                 elif choice == "block_lds" and ws[i].model.tree.N <= 384:
-                    b.set_launch(i, waves_per_run=1, block_sync=1, fidx_in_lds=1)
+                    b.set_launch(i, waves_per_run=1, block_sync=1, lds_staging=1)
                 elif choice == "two_waves" and ws[i].model.tree.N <= 768:
                     b.set_launch(i, waves_per_run=2)
             st = b.run()

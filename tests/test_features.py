@@ -5,6 +5,7 @@ import math
 import numpy as np
 import pytest
 
+from conftest import TOL_MV
 from oracle import features_ref
 
 
@@ -63,6 +64,12 @@ def test_device_features_match_oracle():
         gv = sim.fetch().reshape(6, 3, 1, -1)
     ref = cpu_ref.simulate(m.tree, m.mechset_id, w.params, w.amps, w.dts, m.stim_seg, m.stim_area, m.v_init,
                            m.rec_sites)
-    np.testing.assert_array_equal(gv, ref)
-    np.testing.assert_array_equal(gf, features_ref.features(ref, w.dts, 20.0))
+    assert float(np.max(np.abs(gv - ref))) <= TOL_MV
+    # the device features are a function of the trace: on the GPU's own traces
+    # they equal the CPU restatement bit for bit; on the oracle's traces they
+    # agree to the trace tolerance (spike counts exactly)
+    np.testing.assert_array_equal(gf, features_ref.features(gv, w.dts, 20.0))
+    fr = features_ref.features(ref, w.dts, 20.0)
+    np.testing.assert_array_equal(gf[..., 0], fr[..., 0])
+    np.testing.assert_allclose(gf, fr, rtol=0, atol=1e-5, equal_nan=True)
     assert np.nansum(gf[..., 0]) > 0
