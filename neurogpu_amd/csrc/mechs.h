@@ -51,6 +51,7 @@
 #define NGPU_MECHS_H
 
 #include "ngmath.h"
+#include <type_traits>
 
 #ifdef __HIPCC__
 #define NG_DEV __host__ __device__ __forceinline__
@@ -58,6 +59,20 @@
 #include <math.h>
 #define NG_DEV inline
 #endif
+
+/* Call f with the mechanism mask m: as a compile-time constant when m is one
+ * of the listed masks (the mechanism code then compiles to straight-line code
+ * in which the independent gates of different mechanisms interleave), else as
+ * the run-time value. */
+template <unsigned... MS> struct NgMasks {};
+template <class F>
+NG_DEV void ng_dispatch(unsigned m, F &&f, NgMasks<>) { f(m); }
+template <class F, unsigned M0, unsigned... MS>
+NG_DEV void ng_dispatch(unsigned m, F &&f, NgMasks<M0, MS...>)
+{
+    if (m == M0) f(std::integral_constant<unsigned, M0>{});
+    else ng_dispatch(m, f, NgMasks<MS...>{});
+}
 
 #define NG_DEF_CAI 5.e-5
 #define NG_DEF_CAO 2.
@@ -112,6 +127,7 @@ NG_DEV double ng_gate(double y, double yinf, double rate, double dt)
 struct SetPassive {
     static constexpr int NMECH = 1, NPARAM = 2, NSTATE = 1;
     static constexpr int ECA = 31, ICA = 0;
+    template <class F> NG_DEV static void dispatch(unsigned m, F &&f) { ng_dispatch(m, f, NgMasks<0u, 1u>{}); }
     NG_DEV static unsigned ica_bits(unsigned) { return 0u; }
     NG_DEV static void eca_after_cnexp(unsigned, double *) {}
     NG_DEV static unsigned state_bits(unsigned) { return 0u; }
@@ -271,6 +287,12 @@ NG_DEV void skv3_1(double v, double &mi, double &mr)
 struct SetTTPC {
     static constexpr int NMECH = 13, NPARAM = 19, NSTATE = 21;
     static constexpr int CAI = 18, ECA = 19, ICA = 20;
+    /* the BBP cADpyr placement (Figures/Figure2_BBP_TTPC): basal {Ih, pas},
+     * apical {Ih, Im, NaTs2_t, pas, SKv3_1}, soma, axon; 0 = zero-area node */
+    template <class F> NG_DEV static void dispatch(unsigned m, F &&f)
+    {
+        ng_dispatch(m, f, NgMasks<0x408u, 0x1618u, 0x0u, 0x1e0fu, 0x1de7u>{});
+    }
     /* CaDynamics_E2 reads ica */
     NG_DEV static unsigned ica_bits(unsigned m) { return (m & (1u << 2)) ? 1u << ICA : 0u; }
     NG_DEV static void eca_after_cnexp(unsigned m, double *st)
@@ -433,6 +455,11 @@ NG_DEV void na(double vm, const PF &P, double &mi, double &mr, double &hi, doubl
 struct SetMainen {
     static constexpr int NMECH = 7, NPARAM = 30, NSTATE = 11;
     static constexpr int CAI = 8, ECA = 9, ICA = 10;
+    /* dendrites {ca, cad, kca, km, na, pas}, soma (all), axon {kv, na, pas} */
+    template <class F> NG_DEV static void dispatch(unsigned m, F &&f)
+    {
+        ng_dispatch(m, f, NgMasks<0x6fu, 0x70u, 0x7fu>{});
+    }
     /* cad reads ica */
     NG_DEV static unsigned ica_bits(unsigned m) { return (m & 2u) ? 1u << ICA : 0u; }
     NG_DEV static void eca_after_cnexp(unsigned m, double *st)

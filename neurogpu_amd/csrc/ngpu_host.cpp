@@ -95,7 +95,7 @@ struct ngpu_ctx {
     int lay_W = 0, lay_slot = 0, lay_NQ = 0;
     bool tables_dirty = true, params_dirty = true, stim_dirty = true, graph_dirty = true;
     DevArr<int> d_qh, d_qph, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_rec;
-    DevArr<double> d_qcm, d_e, d_f, d_pval, d_pconst, d_pv, d_dts, d_amps, d_states, d_carry, d_vhot, d_times, d_fwin, d_feat;
+    DevArr<double> d_qcm, d_ef, d_pval, d_pconst, d_pv, d_dts, d_amps, d_states, d_carry, d_vhot, d_times, d_fwin, d_feat;
     int stim_q = -1;
     std::vector<int> rec_q;
     unsigned pconst_mask = 0;
@@ -186,7 +186,7 @@ extern "C" void ngpu_close(ngpu_ctx *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
     for (auto *d : {&c->d_qh, &c->d_qph, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_rec}) d->release();
-    for (auto *d : {&c->d_qcm, &c->d_e, &c->d_f, &c->d_pval, &c->d_pconst, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_carry, &c->d_vhot, &c->d_times, &c->d_fwin, &c->d_feat}) d->release();
+    for (auto *d : {&c->d_qcm, &c->d_ef, &c->d_pval, &c->d_pconst, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_carry, &c->d_vhot, &c->d_times, &c->d_fwin, &c->d_feat}) d->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -271,18 +271,20 @@ extern "C" int ngpu_set_tree(ngpu_ctx *c, const ngpu_tree *t)
     }
     for (int r = 0; r < N - 1; r++)
         if (elim[r] != 1) return fail(c, NGPU_E_ARG, "tree maps eliminate row %d %d times (expected once)", r + 1, elim[r]);
-    /* Chains become (i0, i1, c0, c1): a father J folded after level L whose
-     * row is the one the chain at level L+1 starts from (J = i0 - 1), and
-     * that no other chain of that level touches, is folded by that chain's
-     * lane just before the chain (fchild[c0..c1) are its related rows).  The
-     * same operations in the same order, one barrier fewer per level.  Other
-     * fathers keep their own phase. */
+    /* Chains become (i0, i1, tA, tB, pe): a father J folded after level L
+     * whose row is the one the chain at level L+1 starts from (J = i0 - 1),
+     * that no other chain of that level touches and that has at most two
+     * related rows, is folded by that chain's lane just before the chain
+     * (tA, tB: its related rows in order, -1 = none).  The same operations
+     * in the same order, one barrier fewer per level.  Other fathers keep
+     * their own phase.  pe is the parent row of the chain's last row (N for
+     * the root), where the back-substitution of the chain starts. */
     {
         const int nlev = (int)lvl_ptr.size() - 1;
         std::vector<int> ch4(chains.size() / 2 * 5);
         for (size_t k = 0; k < chains.size() / 2; k++) {
             ch4[5 * k] = chains[2 * k]; ch4[5 * k + 1] = chains[2 * k + 1];
-            ch4[5 * k + 2] = 0; ch4[5 * k + 3] = 0;
+            ch4[5 * k + 2] = -1; ch4[5 * k + 3] = -1;
             const int end = chains[2 * k + 1] - 1;          /* row of the chain's last segment */
             ch4[5 * k + 4] = end == N - 1 ? N : N - 1 - par[N - 1 - end];
         }
@@ -295,12 +297,14 @@ extern "C" int ngpu_set_tree(ngpu_ctx *c, const ngpu_tree *t)
                 bool clash = false;
                 for (int k = k0; k < k1; k++) {
                     const int i0 = chains[2 * k], i1 = chains[2 * k + 1];
-                    if (J >= i0 - 1 && J < i1) { touching++; if (J == i0 - 1 && ch4[5 * k + 3] == 0) owner = k; }
+                    if (J >= i0 - 1 && J < i1) { touching++; if (J == i0 - 1 && ch4[5 * k + 2] < 0) owner = k; }
                     for (int cc = c0; cc < c1; cc++)
                         if (fchild[cc] >= i0 && fchild[cc] < i1) clash = true;
                 }
-                if (owner >= 0 && touching == 1 && !clash) {
-                    ch4[5 * owner + 2] = c0; ch4[5 * owner + 3] = c1;
+                if (c1 == c0) continue;                 /* nothing to fold */
+                if (owner >= 0 && touching == 1 && !clash && c1 - c0 <= 2) {
+                    ch4[5 * owner + 2] = fchild[c0];
+                    ch4[5 * owner + 3] = c1 - c0 == 2 ? fchild[c0 + 1] : -1;
                 } else {
                     rest.push_back(J); rest.push_back(c0); rest.push_back(c1);
                 }
@@ -533,10 +537,11 @@ static int build_tables(ngpu_ctx *c, int W, int slot)
     c->rec_q.resize(c->rec.size());
     for (size_t r = 0; r < c->rec.size(); r++) c->rec_q[r] = pos[c->rec[r]];
     c->stim_q = c->stim_seg >= 0 ? pos[c->stim_seg] : -1;
-    std::vector<double> re(c->e), rf(c->f);
+    std::vector<double> ef(2 * (size_t)N);        /* e, f interleaved by Hines row */
+    for (int h = 0; h < N; h++) { ef[2 * h] = c->e[h]; ef[2 * h + 1] = c->f[h]; }
     int rc;
     if ((rc = upload(c, c->d_pconst, c->pconst)) || (rc = upload(c, c->d_qh, qh)) || (rc = upload(c, c->d_qph, qph)) || (rc = upload(c, c->d_qmeta, qmeta)) ||
-        (rc = upload(c, c->d_qcm, qcm)) || (rc = upload(c, c->d_e, re)) || (rc = upload(c, c->d_f, rf)) ||
+        (rc = upload(c, c->d_qcm, qcm)) || (rc = upload(c, c->d_ef, ef)) ||
         (rc = upload(c, c->d_pval, pval)) || (rc = upload(c, c->d_chains, c->chains)) ||
         (rc = upload(c, c->d_lvl, c->lvl_ptr)) || (rc = upload(c, c->d_fath, c->fath)) || (rc = upload(c, c->d_flvl, c->flvl_ptr)) ||
         (rc = upload(c, c->d_fchild, c->fchild)) || (rc = upload(c, c->d_rec, c->rec_q)))
@@ -577,7 +582,7 @@ static void fill_args(const ngpu_ctx *c, SimArgs &a, int R, int staged, int bloc
     a.nruns = c->nsets * c->nstim; a.depth = c->depth; a.stim_q = c->stim_q;
     a.stim_area = c->area; a.v_init = c->v_init;
     a.q_h = c->d_qh.p; a.q_ph = c->d_qph.p; a.q_meta = c->d_qmeta.p; a.q_cm = c->d_qcm.p;
-    a.row_e = c->d_e.p; a.row_f = c->d_f.p; a.pval = c->d_pval.p; a.pv = c->d_pv.p;
+    a.row_ef = c->d_ef.p; a.pval = c->d_pval.p; a.pv = c->d_pv.p;
     a.pconst_mask = c->pconst_mask;
     a.pconst = c->d_pconst.p;
     a.chains = c->d_chains.p; a.lvl_ptr = c->d_lvl.p; a.fath = c->d_fath.p; a.flvl_ptr = c->d_flvl.p;

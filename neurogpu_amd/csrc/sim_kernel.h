@@ -14,8 +14,7 @@
  *   3. triangularisation by tree levels: each lane eliminates one chain of
  *      consecutive rows (a section), folding the ends of finished child
  *      chains into the chain's first row, with the reference's arithmetic;
- *      a row's pivot quotients PX = b/u and PF = -f/u are formed with one
- *      reciprocal as soon as the row has been eliminated and replace b and u;
+ *      then every row's pivot quotients PX = b/u and PF = -f/u, in parallel;
  *   4. back-substitution by the same chains in reverse level order,
  *      x_i = PX_i + PF_i * x_parent(i), the parent of a chain's end row
  *      coming from a chain of a higher level;
@@ -73,12 +72,12 @@ struct SimArgs {
     double stim_area, v_init;
     const int *q_h, *q_ph, *q_meta;      /* [NQ]: row, parent row, mask | son<<24 (-1 = padding) */
     const double *q_cm;                  /* [NQ] */
-    const double *row_e, *row_f;         /* [N] by Hines row */
+    const double *row_ef;                /* [2N]: e, f interleaved by Hines row */
     const double *pval;                  /* [NPARAM*NQ] value, or tagged NaN = varying column */
     const double *pv;                    /* [nsets*nvar] varying parameter values */
     unsigned pconst_mask;                /* bit j: parameter j has one value wherever it is used */
     const double *pconst;                /* [NPARAM] those values */
-    const int *chains, *lvl_ptr;         /* chains: (i0, i1, c0, c1, pe); lvl_ptr[depth+2] */
+    const int *chains, *lvl_ptr;         /* chains: (i0, i1, tA, tB, pe); lvl_ptr[depth+2] */
     const int *fath, *flvl_ptr, *fchild; /* fathers with their own phase: (J, c0, c1); flvl_ptr[depth+1] */
     int lds_staging;                     /* row coefficients e, f and the run's varying
                                             parameter values staged in LDS (else read
@@ -109,14 +108,16 @@ struct SyncDyn {
 };
 
 /* LDS layout of a workgroup, in this order:
- *   RE[N], RF[N]                                   doubles (by Hines row), when staged
- *   R per-run regions: X[N+1], U[N+1], PV[npv], REC[64*nrec]   doubles
- *                                                  (npv = nvar when staged, else 0)
+ *   EF[2N]                                         e, f interleaved by Hines row, when staged
+ *   R per-run regions: XU[2(N+1)], PV[npv], REC[64*nrec]   doubles; XU holds
+ *                      (b, u), then (PX, PF), then (x, PF) interleaved by row, so
+ *                      that a row is one 16-byte access; npv = nvar rounded up to
+ *                      even when staged, else 0 (regions stay 16-byte aligned)
  *   QM[NQS] int, QH[NQS] short, QP[NQS] short      NQS = NSLOT*LW positions
  *   CH[5*nchains], FA[3*nfath], FC[nfchild], LP[depth+2], FLP[depth+1] short */
 __host__ __device__ inline size_t ng_lds_per_run(int N, int npv, int nrec)
 {
-    return (size_t)(2 * (N + 1) + npv + 64 * nrec);
+    return (size_t)(2 * (N + 1) + ((npv + 1) & ~1) + 64 * nrec);
 }
 __host__ __device__ inline size_t ng_lds_bytes(int N, int NQS, int R, int npv, int nrec, int staged,
                                                int nchains, int nfath, int nfchild, int depth)
@@ -149,11 +150,11 @@ struct ParamFetch {
     }
 };
 
-/* Pointers into the block-shared part of the LDS layout; RE/RF point to LDS
- * or to global memory depending on the STAGE instantiation, so that every
- * access compiles to an LDS or a global load (never a flat one). */
+/* Pointers into the block-shared part of the LDS layout; EF points to LDS or
+ * to global memory depending on the STAGE instantiation, so that every access
+ * compiles to an LDS or a global load (never a flat one). */
 struct Tables {
-    const double *RE, *RF;               /* by Hines row */
+    const double2 *EF;                   /* (e, f) by Hines row */
     double *runs;                        /* first per-run region */
     int *QM;
     short *QH, *QP;
@@ -169,12 +170,11 @@ __device__ __forceinline__ Tables stage_tables(const SimArgs &a, double *lds, in
     const int npv = STAGE ? a.nvar : 0;
     Tables T;
     if constexpr (STAGE) {
-        double *re = lds, *rf = lds + N;
-        for (int i = threadIdx.x; i < N; i += blockDim.x) { re[i] = a.row_e[i]; rf[i] = a.row_f[i]; }
-        T.RE = re; T.RF = rf;
+        for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) lds[i] = a.row_ef[i];
+        T.EF = (const double2 *)lds;
         T.runs = lds + 2 * N;
     } else {
-        T.RE = a.row_e; T.RF = a.row_f;
+        T.EF = (const double2 *)a.row_ef;
         T.runs = lds;
     }
     T.QM = (int *)(T.runs + (size_t)R * ng_lds_per_run(N, npv, a.nrec));
@@ -199,34 +199,66 @@ __device__ __forceinline__ Tables stage_tables(const SimArgs &a, double *lds, in
     return T;
 }
 
-/* Eliminate row tr (already final) into the running father values (uJ, bJ)
- * with the reference's arithmetic (CudaStuff.cu:53-64), then replace its u
- * and b by its pivot quotients PF = -f/u and PX = b/u. */
-__device__ __forceinline__ void ng_fold(const double *RE, const double *RF, double *X, double *U, int tr, double &uJ,
-                                        double &bJ)
+/* n1/d and n2/d, correctly rounded, sharing one reciprocal of d.  When d has
+ * a binary exponent in [-300, 300) and n1, n2 have exponents in [-400, 400),
+ * the IEEE division on gfx9 (v_div_scale, rcp, two Newton steps, product,
+ * residual correction, v_div_fmas, v_div_fixup) neither scales nor fixes up,
+ * and its remaining steps are exactly the ones below, so the quotients are
+ * bit for bit those of n1/d and n2/d.  The test is a vote of the wave, so the
+ * common case is straight-line code; a wave with any other operand (zero,
+ * huge, tiny, non-finite) takes the divisions themselves.  The reciprocal
+ * refinement is done once. */
+__device__ __forceinline__ int ng_exp_in(double x, unsigned lo, unsigned span)
 {
-    const double ut = U[tr], bt = X[tr], e = RE[tr], f = RF[tr];
-    uJ = uJ - f * (e / ut);
-    bJ = bJ - bt * e / ut;
-    const double ru = ng_rcp(ut);
-    U[tr] = -(f * ru); X[tr] = bt * ru;
+    const unsigned ex = ((unsigned)__double2hiint(x) >> 20) & 0x7ffu;
+    return ex - lo < span;
+}
+__device__ __forceinline__ void ng_div2(double n1, double n2, double d, double &q1, double &q2)
+{
+    const int plain = ng_exp_in(d, 1023u - 300u, 600u) & ng_exp_in(n1, 1023u - 400u, 800u) &
+                      ng_exp_in(n2, 1023u - 400u, 800u);
+    if (__all(plain)) {
+        double r = __builtin_amdgcn_rcp(d);
+        double e = fma(-d, r, 1.0);
+        r = fma(r, e, r);
+        e = fma(-d, r, 1.0);
+        r = fma(r, e, r);
+        double q = n1 * r;
+        q1 = fma(fma(-d, q, n1), r, q);
+        q = n2 * r;
+        q2 = fma(fma(-d, q, n2), r, q);
+    } else {
+        q1 = n1 / d;
+        q2 = n2 / d;
+    }
 }
 
-/* Steps t0..t1 of one run by the LW lanes of its group (lane g).  X is the
+/* Eliminate the final row (ut, bt) with coefficients e, f into the running
+ * father values (uJ, bJ) with the reference's arithmetic
+ * (CudaStuff.cu:53-64): uJ -= f (e / ut), bJ -= (bt e) / ut. */
+__device__ __forceinline__ void ng_fold(double ut, double bt, double e, double f, double &uJ, double &bJ)
+{
+    double q1, q2;
+    ng_div2(e, bt * e, ut, q1, q2);
+    uJ = uJ - f * q1;
+    bJ = bJ - q2;
+}
+
+/* Steps t0..t1 of one run by the LW lanes of its group (lane g).  XU is the
  * run's LDS region.  Runs from nruns up to the next multiple of 8 are padding
  * (buffers are allocated for them, their parameters are the last set's): the
  * batch kernel gives them to the unused groups of a partial item. */
 template <class SET, int NSLOT, bool STAGE, class SYNC>
-__device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, double *X, int run, int g,
+__device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, double *XU, int run, int g,
                                         const SYNC &sync)
 {
     const int LW = a.LW, N = a.N, NQ = a.NQ;
-    const int npv = STAGE ? a.nvar : 0;
-    const double *RE = T.RE, *RF = T.RF;
+    const int npv = STAGE ? ((a.nvar + 1) & ~1) : 0;
+    const double2 *EF = T.EF;
     int *QM = T.QM;
     short *QH = T.QH, *QP = T.QP, *CH = T.CH, *FA = T.FA, *FC = T.FC, *LP = T.LP, *FLP = T.FLP;
-    double *U = X + (N + 1);                              /* diagonal, then PF          */
-    double *PV = U + (N + 1);                             /* run-varying parameters     */
+    double2 *XU2 = (double2 *)XU;                         /* (b, u) by Hines row        */
+    double *PV = XU + 2 * (N + 1);                        /* run-varying parameters     */
     double *REC = PV + npv;
     const int set0 = run / a.nstim, stim = run - set0 * a.nstim;
     const int set = min(set0, a.nruns / a.nstim - 1);
@@ -243,7 +275,7 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
 #pragma unroll
     for (int c = 0; c < NSLOT; c++) { v[c] = (c * LW + g < N) ? a.v_init : 0.0; dv[c] = 0.0; }
     if (a.t0 == 0) {
-        for (int i = g; i <= N; i += LW) { X[i] = 0.0; U[i] = 0.0; }
+        for (int i = g; i < 2 * (N + 1); i += LW) XU[i] = 0.0;
 #pragma unroll 1
         for (int c = 0; c < NSLOT; c++) {
             const int q = c * LW + g;
@@ -260,7 +292,7 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
                 if (sb & (1u << k)) S[k * NQ + q] = st[k];
         }
     } else {
-        for (int i = g; i < N; i += LW) X[i] = CY[2 * NQ + i];
+        for (int i = g; i < N; i += LW) XU[2 * i] = CY[2 * NQ + i];
 #pragma unroll
         for (int c = 0; c < NSLOT; c++) {
             const int q = c * LW + g;
@@ -289,7 +321,7 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
 #pragma unroll
         for (int c = 0; c < NSLOT; c++) {
             const int q = c * LW + g, h = QH[q];
-            if (h >= 0) dv[c] += X[QP[q]] - X[h];
+            if (h >= 0) dv[c] += XU[2 * QP[q]] - XU[2 * h];
         }
         sync();
 
@@ -302,41 +334,41 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
             /* slot c's values are at index 0: the arrays rotate by one per slot */
             const int q = c * LW + g;
             const double vq = v[0], dvq = dv[0];
-            double st[SET::NSTATE];
-            {
-                const unsigned mq = (unsigned)QM[q] & 0xffffffu;
-                const unsigned sbn = (SET::state_bits(mq) & keep) | (t > 0 ? SET::ica_bits(mq) : 0u);
+            /* the mechanism code is specialised for the set's common mechanism
+             * masks (straight-line code, independent gates interleaved) */
+            SET::dispatch((unsigned)QM[q] & 0xffffffu, [&](auto mask) {
+                const unsigned m = mask;
+                double st[SET::NSTATE];
+                const unsigned sbn = (SET::state_bits(m) & keep) | (t > 0 ? SET::ica_bits(m) : 0u);
 #pragma unroll
                 for (int k = 0; k < SET::NSTATE; k++) st[k] = (q < N && (sbn & (1u << k))) ? S[k * NQ + q] : 0.0;
-            }
-            if (q < N) {
-                const int h = QH[q];
-                const unsigned m = (unsigned)QM[q] & 0xffffffu;
-                const double cmq = a.q_cm[q];
-                const double fh = RF[h];
-                const unsigned sb = SET::state_bits(m) & keep;
-                const unsigned ib = SET::ica_bits(m);
-                const ParamFetch P{a.pconst, a.pval, pvsrc, a.pconst_mask, q, NQ};
-                if (t > 0) {
-                    SET::cnexp(m, vq, dtp, st, ib ? st[SET::ICA] : 0.0, P);
-                    SET::eca_after_cnexp(m, st);
+                if (q < N) {
+                    const int h = QH[q];
+                    const double cmq = a.q_cm[q];
+                    const double fh = EF[h].y;
+                    const unsigned sb = SET::state_bits(m) & keep;
+                    const unsigned ib = SET::ica_bits(m);
+                    const ParamFetch P{a.pconst, a.pval, pvsrc, a.pconst_mask, q, NQ};
+                    if (t > 0) {
+                        SET::cnexp(m, vq, dtp, st, ib ? st[SET::ICA] : 0.0, P);
+                        SET::eca_after_cnexp(m, st);
 #pragma unroll
-                    for (int k = 0; k < SET::NSTATE; k++)
-                        if (sb & (1u << k)) S[k * NQ + q] = st[k];
+                        for (int k = 0; k < SET::NSTATE; k++)
+                            if (sb & (1u << k)) S[k * NQ + q] = st[k];
+                    }
+                    double sumDv = 0.0, sum = 0.0, icaDv = 0.0, icaNow = 0.0;
+                    SET::current(m, vq + 0.001, st, P, sumDv, icaDv, false);
+                    SET::current(m, vq, st, P, sum, icaNow, true);
+                    if (ib) S[SET::ICA * NQ + q] = icaNow;
+                    const double istim = (q == a.stim_q) ? 100 * amp / a.stim_area : 0.0;
+                    const double gModel = (sumDv - sum) / NG_EPS_V;
+                    double rhs = istim - sum;
+                    double D = gModel + cmq / (dt * 1000);
+                    D -= fh;
+                    rhs -= fh * dvq;
+                    XU2[h] = make_double2(rhs, D);
                 }
-                double sumDv = 0.0, sum = 0.0, icaDv = 0.0, icaNow = 0.0;
-                SET::current(m, vq + 0.001, st, P, sumDv, icaDv, false);
-                SET::current(m, vq, st, P, sum, icaNow, true);
-                if (ib) S[SET::ICA * NQ + q] = icaNow;
-                const double istim = (q == a.stim_q) ? 100 * amp / a.stim_area : 0.0;
-                const double gModel = (sumDv - sum) / NG_EPS_V;
-                double rhs = istim - sum;
-                double D = gModel + cmq / (dt * 1000);
-                D -= fh;
-                rhs -= fh * dvq;
-                X[h] = rhs;
-                U[h] = D;
-            }
+            });
 #pragma unroll
             for (int i = 0; i + 1 < NSLOT; i++) { v[i] = v[i + 1]; dv[i] = dv[i + 1]; }
             v[NSLOT - 1] = vq; dv[NSLOT - 1] = dvq;
@@ -350,76 +382,98 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
             for (int c = 0; c < NSLOT; c++) {
                 const int q = c * LW + g, h = QH[q];
                 if (h < 0 || (QM[q] >> 24) != son) continue;
-                const double e = RE[h];
-                X[QP[q]] += e * dv[c];
-                U[QP[q]] -= e;
+                const double e = EF[h].x;
+                double2 p = XU2[QP[q]];
+                p.x += e * dv[c];
+                p.y -= e;
+                XU2[QP[q]] = p;
             }
             sync();
         }
 
-        /* triangularisation by levels (BeforeLU, CudaStuff.cu:31-69).  Level L
-         * first folds the fathers of level L-1 that have their own phase, then
-         * eliminates its chains; a chain (i0, i1, c0, c1, pe) folds the
-         * related rows FC[c0..c1) into its first row i0-1, then eliminates
-         * rows i0-1 .. i1-2 each into the next; its last row i1-1 keeps u
-         * and b until the father above folds it. */
+        /* triangularisation by levels (BeforeLU, CudaStuff.cu:31-69), with the
+         * reference's arithmetic.  Level L first folds the fathers of level
+         * L-1 that have their own phase, then eliminates its chains; a chain
+         * (i0, i1, tA, tB, pe) folds up to two related rows tA, tB (-1 = none)
+         * into its first row i0-1, then eliminates rows i0-1 .. i1-2 each into
+         * the next.  Every load that does not depend on the chain's own
+         * results is issued up front, and the rows' values are loaded two
+         * rows ahead (indices clamped to the chain, so unconditionally). */
         for (int L = 0; L <= a.depth; L++) {
             if (L > 0 && FLP[L] > FLP[L - 1]) {
                 const int f1e = FLP[L];
                 for (int k = FLP[L - 1] + g; k < f1e; k += LW) {
                     const int J = FA[3 * k], c0 = FA[3 * k + 1], c1 = FA[3 * k + 2];
-                    double uJ = U[J], bJ = X[J];
-                    for (int cc = c0; cc < c1; cc++) ng_fold(RE, RF, X, U, FC[cc], uJ, bJ);
-                    U[J] = uJ; X[J] = bJ;
+                    double2 pJ = XU2[J];
+                    for (int cc = c0; cc < c1; cc++) {
+                        const int tr = FC[cc];
+                        const double2 pt = XU2[tr], et = EF[tr];
+                        ng_fold(pt.y, pt.x, et.x, et.y, pJ.y, pJ.x);
+                    }
+                    XU2[J] = pJ;
                 }
                 sync();
             }
             const int k1 = LP[L + 1];
             for (int k = LP[L] + g; k < k1; k += LW) {
                 const short *ch = CH + 5 * k;
-                const int i0 = ch[0], i1 = ch[1], c0 = ch[2], c1 = ch[3];
-                double u = U[i0 - 1], b = X[i0 - 1];
-                for (int cc = c0; cc < c1; cc++) ng_fold(RE, RF, X, U, FC[cc], u, b);
-                /* row i-1 eliminated into row i with the reference's arithmetic
-                 * (CudaStuff.cu:44-50); the next row's values are loaded one
-                 * iteration ahead */
-                double e = 0.0, f = 0.0, un = 0.0, bn = 0.0;
-                if (i0 < i1) { e = RE[i0 - 1]; f = RF[i0 - 1]; un = U[i0]; bn = X[i0]; }
-                for (int i = i0; i < i1; i++) {
-                    double e2 = 0.0, f2 = 0.0, un2 = 0.0, bn2 = 0.0;
-                    if (i + 1 < i1) { e2 = RE[i]; f2 = RF[i]; un2 = U[i + 1]; bn2 = X[i + 1]; }
-                    const double unew = un - f * (e / u);
-                    const double bnew = bn - b * e / u;
-                    const double ru = ng_rcp(u);
-                    U[i - 1] = -(f * ru); X[i - 1] = b * ru;
-                    u = unew; b = bnew;
-                    e = e2; f = f2; un = un2; bn = bn2;
+                const int i0 = ch[0], i1 = ch[1], tA = ch[2], tB = ch[3];
+                const int ta = max(tA, 0), tb = max(tB, 0), j0 = min(i0, i1 - 1);
+                const double2 p0 = XU2[i0 - 1], pa = XU2[ta], ea = EF[ta], pb = XU2[tb], eb = EF[tb];
+                double2 ef = EF[i0 - 1], pn = XU2[j0];
+                double u = p0.y, b = p0.x;
+                if (tA >= 0) {
+                    ng_fold(pa.y, pa.x, ea.x, ea.y, u, b);
+                    if (tB >= 0) ng_fold(pb.y, pb.x, eb.x, eb.y, u, b);
+                    XU2[i0 - 1] = make_double2(b, u);
                 }
-                U[i1 - 1] = u; X[i1 - 1] = b;
+                for (int i = i0; i < i1; i++) {
+                    const int j = min(i + 1, i1 - 1);
+                    const double2 ef2 = EF[j - 1], pn2 = XU2[j];
+                    double q1, q2;
+                    ng_div2(ef.x, b * ef.x, u, q1, q2);
+                    const double unew = pn.y - ef.y * q1;
+                    const double bnew = pn.x - q2;
+                    XU2[i] = make_double2(bnew, unew);
+                    u = unew; b = bnew;
+                    ef = ef2; pn = pn2;
+                }
             }
             sync();
         }
 
+        /* pivot quotients of every row, in parallel: PX = b/u and PF = -f/u
+         * (BKSUB1's first step, CudaStuff.cu:75-78) replace b and u */
+        for (int h = g; h < N; h += LW) {
+            const double2 p = XU2[h];
+            const double r = ng_rcp(p.y);
+            XU2[h] = make_double2(p.x * r, -(EF[h].y * r));
+        }
+        sync();
+
         /* back-substitution by chains, highest level first: a chain's last
          * row takes x of its parent row pe (a row of a higher level; pe = N
-         * marks the root, whose x is b/u), then x_i = PX_i + PF_i x_{i+1}
-         * down to the chain's first row */
+         * marks the root, x = PX), then x_i = PX_i + PF_i x_{i+1} down to the
+         * chain's first row, with the quotients loaded two rows ahead; x
+         * replaces PX */
         for (int L = a.depth; L >= 0; L--) {
             const int k1 = LP[L + 1];
             for (int k = LP[L] + g; k < k1; k += LW) {
                 const short *ch = CH + 5 * k;
                 const int i0 = ch[0], i1 = ch[1], pe = ch[4];
-                const int h = i1 - 1;
-                double x = pe < N ? X[h] + U[h] * X[pe] : X[h] / U[h];
-                X[h] = x;
-                double px = 0.0, pf = 0.0;
-                if (h - 1 >= i0 - 1) { px = X[h - 1]; pf = U[h - 1]; }
-                for (int i = h - 1; i >= i0 - 1; i--) {
-                    double px2 = 0.0, pf2 = 0.0;
-                    if (i - 1 >= i0 - 1) { px2 = X[i - 1]; pf2 = U[i - 1]; }
-                    x = px + pf * x;
-                    X[i] = x;
-                    px = px2; pf = pf2;
+                const int h = i1 - 1, lo = i0 - 1;
+                const int j1 = max(h - 1, lo), j2 = max(h - 2, lo);
+                const double2 ph = XU2[h];
+                const double xp = XU[2 * min(pe, N - 1)];
+                double2 p1 = XU2[j1], p2 = XU2[j2];
+                double x = ph.x;
+                if (pe < N) x = x + ph.y * xp;
+                XU[2 * h] = x;
+                for (int i = h - 1; i >= lo; i--) {
+                    const double2 p3 = XU2[max(i - 2, lo)];
+                    x = p1.x + p1.y * x;
+                    XU[2 * i] = x;
+                    p1 = p2; p2 = p3;
                 }
             }
             sync();
@@ -428,7 +482,7 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
         /* v update (CudaStuff.cu:312-315) */
 #pragma unroll
         for (int c = 0; c < NSLOT; c++)
-            { const int h = QH[c * LW + g]; if (h >= 0) v[c] += X[h]; }
+            { const int h = QH[c * LW + g]; if (h >= 0) v[c] += XU[2 * h]; }
         dtp = dt;
 
         /* flush the recorded chunk */
@@ -445,7 +499,7 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
 
     /* carry for the next launch */
     if (a.t1 < a.nt) {
-        for (int i = g; i < N; i += LW) CY[2 * NQ + i] = X[i];
+        for (int i = g; i < N; i += LW) CY[2 * NQ + i] = XU[2 * i];
 #pragma unroll
         for (int c = 0; c < NSLOT; c++) {
             const int q = c * LW + g;
@@ -456,56 +510,6 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
 
 #ifndef NG_WAVES_PER_EU
 #define NG_WAVES_PER_EU 2
-/* Host-side launchers of one mechanism set's kernels (instantiated once per
- * set, in sim_<set>.hip). */
-template <class SET, int NS, bool MULTI, bool STAGE>
-static hipError_t ng_launch_sim(dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a)
-{
-    auto k = ngpu_sim_kernel<SET, NS, MULTI, STAGE>;
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(k, grid, block, lds, s, a);
-    return hipGetLastError();
-}
-
-template <class SET, int NS>
-static hipError_t ng_launch_sim_ns(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a)
-{
-    if (multi) return a.lds_staging ? ng_launch_sim<SET, NS, true, true>(grid, block, lds, s, a)
-                                    : ng_launch_sim<SET, NS, true, false>(grid, block, lds, s, a);
-    return a.lds_staging ? ng_launch_sim<SET, NS, false, true>(grid, block, lds, s, a)
-                         : ng_launch_sim<SET, NS, false, false>(grid, block, lds, s, a);
-}
-
-/* nslot positions per lane (1, 2, 4, 6 or 8); multi: more than one wave per run */
-template <class SET>
-static hipError_t ng_launch_set(int nslot, bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s,
-                                const SimArgs &a)
-{
-    switch (nslot) {
-    case 1: return ng_launch_sim_ns<SET, 1>(multi, grid, block, lds, s, a);
-    case 2: return ng_launch_sim_ns<SET, 2>(multi, grid, block, lds, s, a);
-    case 4: return ng_launch_sim_ns<SET, 4>(multi, grid, block, lds, s, a);
-    case 6: return ng_launch_sim_ns<SET, 6>(multi, grid, block, lds, s, a);
-    case 8: return ng_launch_sim_ns<SET, 8>(multi, grid, block, lds, s, a);
-    default: return hipErrorInvalidValue;
-    }
-}
-
-template <class SET>
-static hipError_t ng_launch_batch_set(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b)
-{
-    auto k = ngpu_batch_kernel<SET, NG_BATCH_NSLOT, 4>;
-    if (lds > 65536) {
-        hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL(k, grid, dim3(NG_BATCH_THREADS), lds, s, b);
-    return hipGetLastError();
-}
-
 #endif
 
 template <class SET, int NSLOT, bool MULTI, bool STAGE>
@@ -520,9 +524,9 @@ ngpu_sim_kernel(SimArgs a)
     __syncthreads();
     const int run = blockIdx.x * a.runs_per_block + group;
     if (run >= a.nruns) return;          /* W > 1 implies one run per workgroup */
-    double *X = T.runs + (size_t)group * ng_lds_per_run(a.N, STAGE ? a.nvar : 0, a.nrec);
-    if constexpr (MULTI) sim_run<SET, NSLOT, STAGE>(a, T, X, run, g, SyncBlock{});
-    else sim_run<SET, NSLOT, STAGE>(a, T, X, run, g, SyncWave{});
+    double *XU = T.runs + (size_t)group * ng_lds_per_run(a.N, STAGE ? a.nvar : 0, a.nrec);
+    if constexpr (MULTI) sim_run<SET, NSLOT, STAGE>(a, T, XU, run, g, SyncBlock{});
+    else sim_run<SET, NSLOT, STAGE>(a, T, XU, run, g, SyncWave{});
 }
 
 /* Work queue of the batch kernel: item i runs runs [y, y + z) of morphology x.
@@ -568,15 +572,15 @@ ngpu_batch_kernel(BatchArgs b)
         if (a.lds_staging) {
             const Tables T = stage_tables<true>(a, lds, R, NSLOT * LW);
             __syncthreads();
-            double *X = T.runs + (size_t)group * ng_lds_per_run(a.N, a.nvar, a.nrec);
-            if (a.nslot <= NSLOT_SMALL) sim_run<SET, NSLOT_SMALL, true>(a, T, X, run0 + group, g, sync);
-            else sim_run<SET, NSLOT, true>(a, T, X, run0 + group, g, sync);
+            double *XU = T.runs + (size_t)group * ng_lds_per_run(a.N, a.nvar, a.nrec);
+            if (a.nslot <= NSLOT_SMALL) sim_run<SET, NSLOT_SMALL, true>(a, T, XU, run0 + group, g, sync);
+            else sim_run<SET, NSLOT, true>(a, T, XU, run0 + group, g, sync);
         } else {
             const Tables T = stage_tables<false>(a, lds, R, NSLOT * LW);
             __syncthreads();
-            double *X = T.runs + (size_t)group * ng_lds_per_run(a.N, 0, a.nrec);
-            if (a.nslot <= NSLOT_SMALL) sim_run<SET, NSLOT_SMALL, false>(a, T, X, run0 + group, g, sync);
-            else sim_run<SET, NSLOT, false>(a, T, X, run0 + group, g, sync);
+            double *XU = T.runs + (size_t)group * ng_lds_per_run(a.N, 0, a.nrec);
+            if (a.nslot <= NSLOT_SMALL) sim_run<SET, NSLOT_SMALL, false>(a, T, XU, run0 + group, g, sync);
+            else sim_run<SET, NSLOT, false>(a, T, XU, run0 + group, g, sync);
         }
         __syncthreads();
         if (threadIdx.x == 0) b.stamps[2 * it + 1] = wall_clock64();
