@@ -156,7 +156,8 @@ def bench_mixed(args):
                     "steps": int(w.dts.size), "comp_steps": w.comp_steps,
                     "first_start_ms": round(info["first_start_ms"], 2), "last_end_ms": round(info["last_end_ms"], 2),
                     "comp_steps_per_s_in_span": w.comp_steps / (span * 1e-3) if span > 0 else None,
-                    "launch": {k: info[k] for k in ("waves_per_run", "runs_per_item", "block_sync", "lds_staging")}})
+                    "launch": {k: info[k] for k in ("waves_per_run", "runs_per_group", "runs_per_item", "block_sync",
+                                                    "lds_staging")}})
     result = {
         "metric": METRIC, "value": total * args.steps / wall, "unit": "comp-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
@@ -196,7 +197,7 @@ def main():
     ap.add_argument("--nsets", type=int, default=None)
     ap.add_argument("--nt", type=int, default=None)
     ap.add_argument("--waves-per-run", type=int, default=0)
-    ap.add_argument("--runs-per-block", type=int, default=0)
+    ap.add_argument("--runs-per-group", type=int, default=0)
     ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--cpu-sets", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
@@ -237,7 +238,7 @@ def main():
     sim.set_sim(m.v_init, m.rec_sites)
     sim.set_params(w.params, m.ncomps, m.nparams)
     sim.set_stimuli(w.amps, w.dts, m.stim_seg, m.stim_area)
-    sim.set_launch(args.waves_per_run, args.runs_per_block, 0, bool(args.graph))
+    sim.set_launch(args.waves_per_run, args.runs_per_group, 0, bool(args.graph))
     for i in range(args.warmup):
         _, st = sim.run(fetch=False)
         print(f"warmup {i}: kernel {st['ms_kernel']:.1f} ms", flush=True)
@@ -279,7 +280,7 @@ def main():
         "config": {"workload": f"{w.name}: {m.tree.N} compartments, {m.mechset} mechanisms, "
                                f"{w.x.shape[0]} sets x {w.amps.shape[0]} stimuli, dt {w.dts[0]} ms, "
                                f"{w.dts.size} steps", "runs": w.nruns, "compartments": m.tree.N,
-                   "nt": int(w.dts.size), "launch": {k: st[k] for k in ("waves_per_run", "slots_per_lane",
+                   "nt": int(w.dts.size), "launch": {k: st[k] for k in ("waves_per_run", "runs_per_group", "slots_per_lane",
                                                                       "runs_per_block", "launches")}},
         "roofline": roofline,
     }

@@ -56,6 +56,7 @@ typedef struct {
     int waves_per_run, slots_per_lane, runs_per_block, launches;
     int nvar;                 /* distinct run-varying parameter columns               */
     int nsets, nstim, nrec, nt, N;
+    int runs_per_group;       /* runs sharing one group of waves_per_run waves        */
 } ngpu_stats;
 
 /* Create a context on HIP device `device` (replaces cudaSetDevice in CANON/kernel.cu:24-31). */
@@ -78,12 +79,14 @@ int ngpu_set_params(ngpu_ctx *ctx, const double *allparams, int nsets, int ncomp
 int ngpu_set_stimuli(ngpu_ctx *ctx, const double *amps, int nstim, int nt_amps,
                      const double *dts, int nt, int stim_seg, double area);
 
-/* Launch shape (the autotuner's knobs; 0 = choose automatically):
- * waves_per_run in 1..8, runs_per_block for waves_per_run == 1 (1..4),
- * steps_per_launch (0 = all steps in one launch), use_graph (capture the
- * launch sequence into a hipGraph and replay it). No reference counterpart:
- * the reference fixes the launch at NSets x (32, NStim) (CANON/CudaStuff.cu:487-507). */
-int ngpu_set_launch(ngpu_ctx *ctx, int waves_per_run, int runs_per_block,
+/* Launch shape (the autotuner's knobs; 0 = choose automatically): a group of
+ * waves_per_run waves (1, 2, 4 or 8) advances runs_per_group runs (1..8)
+ * whose positions and solve tasks share the group's lanes; a workgroup holds
+ * as many groups as fit in LDS and 8 waves.  steps_per_launch (0 = all steps
+ * in one launch), use_graph (capture the launch sequence into a hipGraph and
+ * replay it). No reference counterpart: the reference fixes the launch at
+ * NSets x (32, NStim) (CANON/CudaStuff.cu:487-507). */
+int ngpu_set_launch(ngpu_ctx *ctx, int waves_per_run, int runs_per_group,
                     int steps_per_launch, int use_graph);
 
 /* Run every (set, stimulus) for all Nt steps and copy the traces to vhot
@@ -131,16 +134,17 @@ void ngpu_close(ngpu_ctx *ctx);
  * counterpart: the reference runs one morphology per process and launch
  * (CANON/CudaStuff.cu:487-507).  At most 8 contexts per batch.  Each
  * workgroup of 8 waves takes items from a device work queue, longest first;
- * an item is 8 / waves_per_run runs of one member.  Traces and features stay
+ * an item is (8 / waves_per_run) * runs_per_group runs of one member.  Traces and features stay
  * with each member context (ngpu_fetch_vhot, ngpu_features). */
 typedef struct ngpu_batch ngpu_batch;
 typedef struct {
-    int waves_per_run;        /* 1, 2, 4 or 8; 0 = the fewest that hold the tree      */
+    int waves_per_run;        /* 1, 2, 4 or 8; 0 = 1 up to 512 segments, else 8        */
     int block_sync;           /* levels synchronised per workgroup (1) or per wave (0,
                                  waves_per_run 1 only); -1 = per wave when possible   */
     int lds_staging;          /* Hines row coefficients e, f and the runs' varying
                                  parameter values staged in LDS (1) or read from
                                  global memory (0); -1 = in LDS when they fit         */
+    int runs_per_group;       /* 1..8 runs per group of waves; 0 = as many as fit    */
 } ngpu_batch_launch;
 typedef struct {
     double ms_kernel;         /* HIP events around the persistent launch              */
@@ -151,6 +155,7 @@ typedef struct {
     double first_start_ms, last_end_ms;   /* from the first item's start (device clock) */
     double comp_steps;
     int waves_per_run, runs_per_item, block_sync, lds_staging;   /* what ran          */
+    int runs_per_group;
 } ngpu_batch_member;
 int ngpu_batch_create(int device, ngpu_batch **out);
 /* Add a configured context (tree, parameters, stimuli); returns its member

@@ -58,14 +58,16 @@ class ngpu_stats(C.Structure):
     _fields_ = [("ms_total", C.c_double), ("ms_kernel", C.c_double), ("comp_steps", C.c_double),
                 ("comp_steps_per_s", C.c_double), ("waves_per_run", C.c_int), ("slots_per_lane", C.c_int),
                 ("runs_per_block", C.c_int), ("launches", C.c_int), ("nvar", C.c_int),
-                ("nsets", C.c_int), ("nstim", C.c_int), ("nrec", C.c_int), ("nt", C.c_int), ("N", C.c_int)]
+                ("nsets", C.c_int), ("nstim", C.c_int), ("nrec", C.c_int), ("nt", C.c_int), ("N", C.c_int),
+                ("runs_per_group", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class ngpu_batch_launch(C.Structure):
-    _fields_ = [("waves_per_run", C.c_int), ("block_sync", C.c_int), ("lds_staging", C.c_int)]
+    _fields_ = [("waves_per_run", C.c_int), ("block_sync", C.c_int), ("lds_staging", C.c_int),
+                ("runs_per_group", C.c_int)]
 
 
 class ngpu_batch_stats(C.Structure):
@@ -79,7 +81,7 @@ class ngpu_batch_stats(C.Structure):
 class ngpu_batch_member(C.Structure):
     _fields_ = [("first_start_ms", C.c_double), ("last_end_ms", C.c_double), ("comp_steps", C.c_double),
                 ("waves_per_run", C.c_int), ("runs_per_item", C.c_int), ("block_sync", C.c_int),
-                ("lds_staging", C.c_int)]
+                ("lds_staging", C.c_int), ("runs_per_group", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -208,9 +210,9 @@ class Simulator:
                                              D.ctypes.data_as(_DP), D.size, int(stim_seg), float(area)))
         self.nstim, self.nt = A.shape[0], D.size
 
-    def set_launch(self, waves_per_run: int = 0, runs_per_block: int = 0, steps_per_launch: int = 0,
+    def set_launch(self, waves_per_run: int = 0, runs_per_group: int = 0, steps_per_launch: int = 0,
                    use_graph: bool = False) -> None:
-        self._check(self._L.ngpu_set_launch(self._h, int(waves_per_run), int(runs_per_block),
+        self._check(self._L.ngpu_set_launch(self._h, int(waves_per_run), int(runs_per_group),
                                             int(steps_per_launch), int(bool(use_graph))))
 
     def vhot_size(self) -> int:
@@ -301,8 +303,9 @@ class Batch:
         self.members.append(sim)
         return idx
 
-    def set_launch(self, member: int, waves_per_run: int = 0, block_sync: int = -1, lds_staging: int = -1) -> None:
-        cfg = ngpu_batch_launch(waves_per_run, block_sync, lds_staging)
+    def set_launch(self, member: int, waves_per_run: int = 0, block_sync: int = -1, lds_staging: int = -1,
+                   runs_per_group: int = 0) -> None:
+        cfg = ngpu_batch_launch(waves_per_run, block_sync, lds_staging, runs_per_group)
         self._check(self.L.ngpu_batch_set_launch(self.h, member, C.byref(cfg)))
 
     def run(self) -> dict:

@@ -27,16 +27,15 @@
 #include "sim_kernel.h"
 
 /* Per-set launchers, one translation unit each (sim_<set>.hip). */
-hipError_t ngpu_launch_passive(int nslot, bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
-hipError_t ngpu_launch_ttpc(int nslot, bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
+hipError_t ngpu_launch_passive(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
+hipError_t ngpu_launch_ttpc(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
 hipError_t ngpu_launch_features(const double *vhot, int nrec, int nt, int nruns, int nstim, const double *times,
                                 const double *t_on, const double *t_off, double thr, double *out, hipStream_t s);
-hipError_t ngpu_launch_mainen(int nslot, bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
+hipError_t ngpu_launch_mainen(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
 hipError_t ngpu_launch_batch_passive(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
 hipError_t ngpu_launch_batch_ttpc(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
 hipError_t ngpu_launch_batch_mainen(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
 
-static const int kSlotSizes[] = {1, 2, 4, 6, 8};
 static std::string g_last_error;
 
 static int mech_nmodels(int ms) { return ms == NGPU_MECH_TTPC ? SetTTPC::NMECH : ms == NGPU_MECH_MAINEN ? SetMainen::NMECH : ms == NGPU_MECH_PASSIVE ? SetPassive::NMECH : -1; }
@@ -89,13 +88,13 @@ struct ngpu_ctx {
     double area = 1.0;
     std::vector<double> amps, dts;
     /* launch configuration */
-    int cfg_w = 0, cfg_r = 0, cfg_spl = 0, cfg_graph = 0;
+    int cfg_w = 0, cfg_r = 0, cfg_spl = 0, cfg_graph = 0;   /* waves and runs per group, 0 = auto */
     int cfg_stage = -1;            /* row coefficients: -1 auto, 0 global memory, 1 LDS */
     /* device state */
-    int lay_W = 0, lay_slot = 0, lay_NQ = 0;
+    int lay_W = 0, lay_R = 0, lay_NQ = 0;
     bool tables_dirty = true, params_dirty = true, stim_dirty = true, graph_dirty = true;
-    DevArr<int> d_qh, d_qph, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_rec;
-    DevArr<double> d_qcm, d_ef, d_pval, d_pconst, d_pv, d_dts, d_amps, d_states, d_carry, d_vhot, d_times, d_fwin, d_feat;
+    DevArr<int> d_qlink, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_rec;
+    DevArr<double> d_qcm, d_ef, d_pval, d_pconst, d_pv, d_dts, d_amps, d_states, d_vhot, d_times, d_fwin, d_feat;
     int stim_q = -1;
     std::vector<int> rec_q;
     unsigned pconst_mask = 0;
@@ -103,7 +102,7 @@ struct ngpu_ctx {
     hipGraphExec_t gexec = nullptr;
     bool ran = false;
     int last_launches = 0;
-    int last_R = 1;
+    int last_G = 1;               /* groups per workgroup of the last launch */
     int last_stage = 1;
 };
 
@@ -185,8 +184,8 @@ extern "C" void ngpu_close(ngpu_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
-    for (auto *d : {&c->d_qh, &c->d_qph, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_rec}) d->release();
-    for (auto *d : {&c->d_qcm, &c->d_ef, &c->d_pval, &c->d_pconst, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_carry, &c->d_vhot, &c->d_times, &c->d_fwin, &c->d_feat}) d->release();
+    for (auto *d : {&c->d_qlink, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_rec}) d->release();
+    for (auto *d : {&c->d_qcm, &c->d_ef, &c->d_pval, &c->d_pconst, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_vhot, &c->d_times, &c->d_fwin, &c->d_feat}) d->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -446,7 +445,7 @@ extern "C" int ngpu_set_stimuli(ngpu_ctx *c, const double *amps, int nstim, int 
 extern "C" int ngpu_set_launch(ngpu_ctx *c, int w, int r, int spl, int graph)
 {
     if (!c) return fail(c, NGPU_E_ARG, "NULL context");
-    if (w < 0 || w > 8 || r < 0 || r > 4 || spl < 0 || (graph != 0 && graph != 1))
+    if (!(w == 0 || w == 1 || w == 2 || w == 4 || w == 8) || r < 0 || r > 8 || spl < 0 || (graph != 0 && graph != 1))
         return fail(c, NGPU_E_ARG, "invalid launch configuration w=%d r=%d spl=%d graph=%d", w, r, spl, graph);
     c->cfg_w = w; c->cfg_r = r; c->cfg_spl = spl; c->cfg_graph = graph;
     c->tables_dirty = true;
@@ -462,44 +461,48 @@ extern "C" long long ngpu_vhot_size(const ngpu_ctx *c)
 
 /* ------------------------------------------------------------------ run */
 
-static void choose_layout(const ngpu_ctx *c, int &W, int &slot)
-{
-    const int N = c->N;
-    W = c->cfg_w;
-    if (W <= 0) {
-        /* 1, 2, 4 or 8 waves: at 2 waves per SIMD (the register budget of the
-         * mechanism code) a CU holds 8 waves, which these divide evenly */
-        W = 1;
-        while (W < 8 && 64 * 8 * W < N) W *= 2;
-    }
-    W = std::min(W, 8);
-    int need = (N + 64 * W - 1) / (64 * W);
-    slot = 8;
-    for (int s : kSlotSizes) if (s >= need) { slot = s; break; }
-    while (slot * 64 * W < N && W < 8) {             /* only reached for very large trees */
-        W++;
-        need = (N + 64 * W - 1) / (64 * W);
-        for (int s : kSlotSizes) if (s >= need) { slot = s; break; }
-    }
-}
+/* Positions per run: segments rounded up to whole waves. */
+static int positions(int N) { return (N + 63) / 64 * 64; }
 
-static int build_tables(ngpu_ctx *c, int W, int slot)
+/* Waves per group.  Up to 512 segments a group is one wave (its levels
+ * synchronise at wavefront scope, without a barrier instruction); larger
+ * trees take 8 waves.  Runs per group: as many as fit in LDS, up to
+ * max_runs_per_group (measured on the 500- and 1,200-segment cells: two runs
+ * per one-wave group, four per eight-wave group). */
+static void choose_layout(const ngpu_ctx *c, int &W, int &R)
 {
-    const int N = c->N, LW = 64 * W, NQ = slot * LW;
-    if (NQ < N) return fail(c, NGPU_E_ARG, "tree of %d segments exceeds the largest layout (%d positions)", N, NQ);
+    W = c->cfg_w > 0 ? c->cfg_w : (c->N <= 512 ? 1 : 8);
+    R = c->cfg_r > 0 ? c->cfg_r : 0;
+}
+static int max_runs_per_group(int W) { return W == 1 ? 2 : 4; }
+
+static int build_tables(ngpu_ctx *c)
+{
+    const int N = c->N, NQ = positions(N);
     /* positions: segments sorted by mechanism mask, stable */
     std::vector<int> order(N);
     for (int s = 0; s < N; s++) order[s] = s;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return c->mask[a] < c->mask[b]; });
-    std::vector<int> pos(N), qh(NQ, -1), qph(NQ, 0), qmeta(NQ, 0);
+    std::vector<int> pos(N), qlink(4 * (size_t)NQ, -1), qmeta(NQ, 0);
     std::vector<double> qcm(NQ, 0.0);
     for (int q = 0; q < N; q++) {
-        int s = order[q];
+        const int s = order[q];
         pos[s] = q;
-        qh[q] = N - 1 - s;
-        qph[q] = s == 0 ? N - 1 : N - 1 - c->par[s];
-        qmeta[q] = (int)(c->mask[s] | ((unsigned)std::min(c->son[s], 255) << 24));
+        qlink[4 * q] = N - 1 - s;                             /* row */
+        qlink[4 * q + 1] = s == 0 ? N - 1 : N - 1 - c->par[s]; /* parent row (the root: itself) */
+        qmeta[q] = (int)c->mask[s];
         qcm[q] = c->cm[s];
+    }
+    for (int q = N; q < NQ; q++) qlink[4 * q + 1] = 0;
+    /* sons in son order: the reference adds the children's terms to the
+     * parent's row first for SonNoVec 1, then for SonNoVec 2
+     * (CANON/CudaStuff.cu:296-305); other values add nothing */
+    for (int s = 1; s < N; s++) {
+        const int sn = c->son[s];
+        if (sn != 1 && sn != 2) continue;
+        int &slot = qlink[4 * (size_t)pos[c->par[s]] + 1 + sn];
+        if (slot >= 0) return fail(c, NGPU_E_ARG, "segment %d has two children with SonNoVec %d", c->par[s], sn);
+        slot = pos[s];
     }
     const int np = c->nparams;
     /* A parameter is a kernel constant when every segment that uses it holds
@@ -540,13 +543,13 @@ static int build_tables(ngpu_ctx *c, int W, int slot)
     std::vector<double> ef(2 * (size_t)N);        /* e, f interleaved by Hines row */
     for (int h = 0; h < N; h++) { ef[2 * h] = c->e[h]; ef[2 * h + 1] = c->f[h]; }
     int rc;
-    if ((rc = upload(c, c->d_pconst, c->pconst)) || (rc = upload(c, c->d_qh, qh)) || (rc = upload(c, c->d_qph, qph)) || (rc = upload(c, c->d_qmeta, qmeta)) ||
+    if ((rc = upload(c, c->d_pconst, c->pconst)) || (rc = upload(c, c->d_qlink, qlink)) || (rc = upload(c, c->d_qmeta, qmeta)) ||
         (rc = upload(c, c->d_qcm, qcm)) || (rc = upload(c, c->d_ef, ef)) ||
         (rc = upload(c, c->d_pval, pval)) || (rc = upload(c, c->d_chains, c->chains)) ||
         (rc = upload(c, c->d_lvl, c->lvl_ptr)) || (rc = upload(c, c->d_fath, c->fath)) || (rc = upload(c, c->d_flvl, c->flvl_ptr)) ||
         (rc = upload(c, c->d_fchild, c->fchild)) || (rc = upload(c, c->d_rec, c->rec_q)))
         return rc;
-    c->lay_W = W; c->lay_slot = slot; c->lay_NQ = NQ;
+    c->lay_NQ = NQ;
     c->tables_dirty = false;
     return NGPU_OK;
 }
@@ -555,9 +558,9 @@ static hipError_t launch_one(ngpu_ctx *c, const SimArgs &a, dim3 grid, dim3 bloc
 {
     bool multi = c->lay_W > 1;
     switch (c->mechset) {
-    case NGPU_MECH_TTPC: return ngpu_launch_ttpc(c->lay_slot, multi, grid, block, lds, c->stream, a);
-    case NGPU_MECH_MAINEN: return ngpu_launch_mainen(c->lay_slot, multi, grid, block, lds, c->stream, a);
-    default: return ngpu_launch_passive(c->lay_slot, multi, grid, block, lds, c->stream, a);
+    case NGPU_MECH_TTPC: return ngpu_launch_ttpc(multi, grid, block, lds, c->stream, a);
+    case NGPU_MECH_MAINEN: return ngpu_launch_mainen(multi, grid, block, lds, c->stream, a);
+    default: return ngpu_launch_passive(multi, grid, block, lds, c->stream, a);
     }
 }
 
@@ -567,21 +570,39 @@ static const size_t kLdsLimit = 160 * 1024;
  * considered for up to this many varying values per run. */
 static const int kStageMaxVar = 512;
 
-static size_t lds_need(const ngpu_ctx *c, int R, int NQS, int staged)
+static size_t lds_need(const ngpu_ctx *c, int runs, int staged)
 {
-    return ng_lds_bytes(c->N, NQS, R, std::max(c->nvar, 1), (int)c->rec.size(), staged, (int)c->chains.size() / 5,
-                        (int)c->fath.size() / 3, (int)c->fchild.size(), c->depth);
+    return ng_lds_bytes(c->N, positions(c->N), runs, std::max(c->nvar, 1), (int)c->rec.size(), staged,
+                        (int)c->chains.size() / 5, (int)c->fath.size() / 3, (int)c->fchild.size(), c->depth);
 }
 
-/* Kernel arguments of a prepared context (tables built for its layout). */
-static void fill_args(const ngpu_ctx *c, SimArgs &a, int R, int staged, int block_sync)
+/* Runs per group R (0 = as many as fit, up to max_runs_per_group), groups
+ * per workgroup G (as many as fit in LDS, at most 8 waves) and LDS staging
+ * (-1 = when it fits) for groups of W waves. */
+static int plan(ngpu_ctx *c, int W, int &R, int &G, int &stage)
+{
+    if (stage < 0) stage = c->nvar <= kStageMaxVar && lds_need(c, std::max(R, 1), 1) <= kLdsLimit ? 1 : 0;
+    if (R <= 0) {
+        R = 1;
+        while (R < max_runs_per_group(W) && lds_need(c, R + 1, stage) <= kLdsLimit) R++;
+    }
+    if (lds_need(c, R, stage) > kLdsLimit)
+        return fail(c, NGPU_E_ARG, "%d runs of a %d-segment tree need %zu bytes of LDS per workgroup (limit 160 KiB)", R,
+                    c->N, lds_need(c, R, stage));
+    G = 1;
+    while ((G + 1) * W <= 8 && lds_need(c, (G + 1) * R, stage) <= kLdsLimit) G++;
+    return NGPU_OK;
+}
+
+/* Kernel arguments of a prepared context. */
+static void fill_args(const ngpu_ctx *c, SimArgs &a, int W, int R, int staged, int block_sync)
 {
     a = SimArgs{};
-    a.N = c->N; a.NQ = c->lay_NQ; a.LW = 64 * c->lay_W; a.runs_per_block = R;
+    a.N = c->N; a.NQ = c->lay_NQ; a.LW = 64 * W; a.runs_per_group = R;
     a.nvar = std::max(c->nvar, 1); a.nrec = (int)c->rec.size(); a.nstim = c->nstim; a.nt = c->nt; a.nt_amps = c->nt_amps;
     a.nruns = c->nsets * c->nstim; a.depth = c->depth; a.stim_q = c->stim_q;
     a.stim_area = c->area; a.v_init = c->v_init;
-    a.q_h = c->d_qh.p; a.q_ph = c->d_qph.p; a.q_meta = c->d_qmeta.p; a.q_cm = c->d_qcm.p;
+    a.q_meta = c->d_qmeta.p; a.q_link = c->d_qlink.p; a.q_cm = c->d_qcm.p;
     a.row_ef = c->d_ef.p; a.pval = c->d_pval.p; a.pv = c->d_pv.p;
     a.pconst_mask = c->pconst_mask;
     a.pconst = c->d_pconst.p;
@@ -589,28 +610,22 @@ static void fill_args(const ngpu_ctx *c, SimArgs &a, int R, int staged, int bloc
     a.fchild = c->d_fchild.p; a.lds_staging = staged; a.block_sync = block_sync;
     a.nchains = (int)c->chains.size() / 5; a.nfath = (int)c->fath.size() / 3; a.nfchild = (int)c->fchild.size();
     a.dts = c->d_dts.p; a.amps = c->d_amps.p; a.rec_q = c->d_rec.p;
-    a.states = c->d_states.p; a.carry = c->d_carry.p; a.vhot = c->d_vhot.p;
+    a.states = c->d_states.p; a.vhot = c->d_vhot.p;
     a.t0 = 0; a.t1 = c->nt;
 }
 
 static int enqueue(ngpu_ctx *c, int &launches)
 {
-    const int NQ = c->lay_NQ, LW = 64 * c->lay_W;
-    const int nruns = c->nsets * c->nstim;
-    int R = 1;
-    if (c->lay_W == 1) R = c->cfg_r > 0 ? c->cfg_r : 4;   /* 4 single-wave runs per workgroup */
-    /* the row coefficients go to LDS when they fit (or as configured) */
-    int stage = c->cfg_stage;
-    if (stage < 0) stage = c->nvar <= kStageMaxVar && lds_need(c, 1, NQ, 1) <= kLdsLimit ? 1 : 0;
-    while (R > 1 && lds_need(c, R, NQ, stage) > kLdsLimit) R--;
-    const size_t lds = lds_need(c, R, NQ, stage);
-    if (lds > kLdsLimit)
-        return fail(c, NGPU_E_ARG, "tree needs %zu bytes of LDS per workgroup (limit 160 KiB)", lds);
+    const int W = c->lay_W, nruns = c->nsets * c->nstim;
+    int R = c->lay_R, G = 1, stage = c->cfg_stage;
+    int rc = plan(c, W, R, G, stage);
+    if (rc) return rc;
+    const size_t lds = lds_need(c, G * R, stage);
     SimArgs a;
-    fill_args(c, a, R, stage, c->lay_W > 1);
-    c->last_R = R;
+    fill_args(c, a, W, R, stage, W > 1);
+    c->last_G = G;
     c->last_stage = stage;
-    dim3 block(LW * R), grid((nruns + R - 1) / R);
+    dim3 block(64 * W * G), grid((nruns + G * R - 1) / (G * R));
     int spl = c->cfg_spl > 0 ? c->cfg_spl : c->nt;
     launches = 0;
     for (int t0 = 0; t0 < c->nt; t0 += spl) {
@@ -623,29 +638,35 @@ static int enqueue(ngpu_ctx *c, int &launches)
     return NGPU_OK;
 }
 
-static int prepare_layout(ngpu_ctx *c, int W, int slot);
+static int prepare_layout(ngpu_ctx *c, int W, int R);
 
 static int prepare(ngpu_ctx *c)
 {
     if (!c->have_tree || !c->have_params || !c->have_stim)
         return fail(c, NGPU_E_STATE, "ngpu_run needs ngpu_set_tree, ngpu_set_params and ngpu_set_stimuli first");
-    int W, slot;
-    choose_layout(c, W, slot);
-    return prepare_layout(c, W, slot);
+    int W, R;
+    choose_layout(c, W, R);
+    return prepare_layout(c, W, R);
 }
 
-/* Tables, parameters, stimuli and buffers on the device for layout (W, slot). */
-static int prepare_layout(ngpu_ctx *c, int W, int slot)
+/* Padding runs beyond nruns: a workgroup always simulates whole groups. */
+static const size_t kPadRuns = 64;
+
+/* Tables, parameters, stimuli and buffers on the device for groups of W waves
+ * and R runs (0 = as many as fit). */
+static int prepare_layout(ngpu_ctx *c, int W, int R)
 {
     if (!c->have_tree || !c->have_params || !c->have_stim)
         return fail(c, NGPU_E_STATE, "a run needs ngpu_set_tree, ngpu_set_params and ngpu_set_stimuli first");
     if (!c->have_sim) { c->rec.assign(1, 0); c->have_sim = true; }
     HIPCHK(c, hipSetDevice(c->device));
     int rc;
-    if (c->tables_dirty || W != c->lay_W || slot != c->lay_slot) {
-        if ((rc = build_tables(c, W, slot))) return rc;
+    if (c->tables_dirty) {
+        if ((rc = build_tables(c))) return rc;
         drop_graph(c);
     }
+    if (W != c->lay_W || R != c->lay_R) drop_graph(c);
+    c->lay_W = W; c->lay_R = R;
     if (c->params_dirty) {
         if ((rc = upload(c, c->d_pv, c->pv))) return rc;
         c->params_dirty = false;
@@ -654,10 +675,8 @@ static int prepare_layout(ngpu_ctx *c, int W, int slot)
         if ((rc = upload(c, c->d_dts, c->dts)) || (rc = upload(c, c->d_amps, c->amps))) return rc;
         c->stim_dirty = false;
     }
-    /* buffers for whole items of up to 8 runs (padding runs of the batch kernel) */
-    const size_t nruns = ((size_t)c->nsets * c->nstim + 7) / 8 * 8, NQ = c->lay_NQ;
-    if ((rc = alloc(c, c->d_states, nruns * mech_nstate(c->mechset) * NQ)) ||
-        (rc = alloc(c, c->d_carry, nruns * (2 * NQ + c->N + 1))) ||
+    const size_t nruns = (size_t)c->nsets * c->nstim + kPadRuns, NQ = c->lay_NQ;
+    if ((rc = alloc(c, c->d_states, nruns * (mech_nstate(c->mechset) + 2) * NQ)) ||
         (rc = alloc(c, c->d_vhot, nruns * c->rec.size() * c->nt)))
         return rc;
     return NGPU_OK;
@@ -711,9 +730,12 @@ extern "C" int ngpu_run(ngpu_ctx *c, double *vhot, ngpu_stats *st)
         st->ms_total = mt;
         st->comp_steps = (double)c->nsets * c->nstim * (double)c->N * c->nt;
         st->comp_steps_per_s = mk > 0 ? st->comp_steps / (mk * 1e-3) : 0.0;
+        int R = c->lay_R, G = 1, stage = c->cfg_stage;
+        (void)plan(c, c->lay_W, R, G, stage);
         st->waves_per_run = c->lay_W;
-        st->slots_per_lane = c->lay_slot;
-        st->runs_per_block = c->last_R;
+        st->slots_per_lane = (R * c->lay_NQ + 64 * c->lay_W - 1) / (64 * c->lay_W);
+        st->runs_per_block = c->last_G * R;
+        st->runs_per_group = R;
         st->launches = launches;
         st->nvar = c->nvar;
         st->nsets = c->nsets; st->nstim = c->nstim; st->nrec = (int)c->rec.size(); st->nt = c->nt; st->N = c->N;
@@ -732,7 +754,7 @@ struct ngpu_batch {
     std::string err;
     std::vector<ngpu_ctx *> mem;
     std::vector<ngpu_batch_launch> cfg;
-    std::vector<int> lay_w, lay_sync, lay_stage;
+    std::vector<int> lay_w, lay_r, lay_sync, lay_stage;
     std::vector<double> t_first, t_last, cs;   /* per member, ms from the first item start */
     int4 *d_items = nullptr;
     int *d_head = nullptr;
@@ -806,7 +828,7 @@ extern "C" int ngpu_batch_add(ngpu_batch *b, ngpu_ctx *c)
     for (auto *m : b->mem)
         if (m == c) return bfail(b, NGPU_E_ARG, "context already in the batch");
     b->mem.push_back(c);
-    b->cfg.push_back(ngpu_batch_launch{0, -1, -1});
+    b->cfg.push_back(ngpu_batch_launch{0, -1, -1, 0});
     return (int)b->mem.size() - 1;
 }
 
@@ -819,16 +841,9 @@ extern "C" int ngpu_batch_set_launch(ngpu_batch *b, int member, const ngpu_batch
     if (cfg->block_sync < -1 || cfg->block_sync > 1 || cfg->lds_staging < -1 || cfg->lds_staging > 1)
         return bfail(b, NGPU_E_ARG, "block_sync and lds_staging take -1, 0 or 1");
     if (w > 1 && cfg->block_sync == 0) return bfail(b, NGPU_E_ARG, "runs of several waves synchronise at workgroup scope");
+    if (cfg->runs_per_group < 0 || cfg->runs_per_group > 8) return bfail(b, NGPU_E_ARG, "runs_per_group %d not in 0..8", cfg->runs_per_group);
     b->cfg[member] = *cfg;
     return NGPU_OK;
-}
-
-static int batch_slot(int N, int W)
-{
-    const int need = (N + 64 * W - 1) / (64 * W);
-    for (int s : kSlotSizes)
-        if (s >= need && s <= NG_BATCH_NSLOT) return s;
-    return -1;
 }
 
 extern "C" int ngpu_batch_run(ngpu_batch *b, ngpu_batch_stats *st)
@@ -841,30 +856,33 @@ extern "C" int ngpu_batch_run(ngpu_batch *b, ngpu_batch_stats *st)
     std::vector<int4> items;
     std::vector<double> cost;
     size_t lds = 0;
-    b->lay_w.assign(nm, 0); b->lay_sync.assign(nm, 0); b->lay_stage.assign(nm, 0);
+    b->lay_w.assign(nm, 0); b->lay_r.assign(nm, 0); b->lay_sync.assign(nm, 0); b->lay_stage.assign(nm, 0);
     for (int i = 0; i < nm; i++) {
         ngpu_ctx *c = b->mem[i];
         if (!c->have_params || !c->have_stim) return bfail(b, NGPU_E_STATE, "member %d lacks parameters or stimuli", i);
         int W = b->cfg[i].waves_per_run;
-        if (W == 0) { W = 1; while (W < 8 && 64 * W * NG_BATCH_NSLOT < c->N) W *= 2; }
-        const int slot = batch_slot(c->N, W);
-        if (slot < 0) return bfail(b, NGPU_E_ARG, "member %d: %d segments do not fit %d waves x %d positions per lane", i, c->N, W, NG_BATCH_NSLOT);
-        int rc = prepare_layout(c, W, slot);
+        if (W == 0) W = c->N <= 512 ? 1 : 8;
+        int R = b->cfg[i].runs_per_group;
+        int rc = prepare_layout(c, W, R);
         if (rc) return bfail(b, rc, "member %d: %s", i, c->err.c_str());
-        const int R = NG_BATCH_THREADS / (64 * W), NQS = NG_BATCH_NSLOT * 64 * W;
+        /* the workgroup is 8 waves: G = 8 / W groups of R runs */
+        const int G = 8 / W;
+        int stage = b->cfg[i].lds_staging;
+        if (stage < 0) stage = c->nvar <= kStageMaxVar && lds_need(c, G * std::max(R, 1), 1) <= kLdsLimit ? 1 : 0;
+        if (R <= 0) {
+            R = 1;
+            while (R < max_runs_per_group(W) && lds_need(c, G * (R + 1), stage) <= kLdsLimit) R++;
+        }
         int sync = b->cfg[i].block_sync;
         if (sync < 0) sync = W > 1 ? 1 : 0;
-        int stage = b->cfg[i].lds_staging;
-        if (stage < 0) stage = c->nvar <= kStageMaxVar && lds_need(c, R, NQS, 1) <= kLdsLimit ? 1 : 0;
-        const size_t need = lds_need(c, R, NQS, stage);
+        const size_t need = lds_need(c, G * R, stage);
         if (need > kLdsLimit) return bfail(b, NGPU_E_ARG, "member %d needs %zu bytes of LDS per workgroup (limit 160 KiB)", i, need);
         lds = std::max(lds, need);
-        fill_args(c, args[i], R, stage, sync);
-        args[i].nslot = slot;
-        b->lay_w[i] = W; b->lay_sync[i] = sync; b->lay_stage[i] = stage;
-        const int nruns = c->nsets * c->nstim;
-        for (int r0 = 0; r0 < nruns; r0 += R) {
-            items.push_back(make_int4(i, r0, std::min(R, nruns - r0), 0));
+        fill_args(c, args[i], W, R, stage, sync);
+        b->lay_w[i] = W; b->lay_r[i] = R; b->lay_sync[i] = sync; b->lay_stage[i] = stage;
+        const int nruns = c->nsets * c->nstim, per_item = G * R;
+        for (int r0 = 0; r0 < nruns; r0 += per_item) {
+            items.push_back(make_int4(i, r0, std::min(per_item, nruns - r0), 0));
             cost.push_back((double)c->N * c->nt);
         }
     }
@@ -942,7 +960,8 @@ extern "C" int ngpu_batch_member_info(const ngpu_batch *b, int member, ngpu_batc
     out->last_end_ms = b->t_last[member];
     out->comp_steps = b->cs[member];
     out->waves_per_run = b->lay_w[member];
-    out->runs_per_item = NG_BATCH_THREADS / (64 * b->lay_w[member]);
+    out->runs_per_item = NG_BATCH_THREADS / (64 * b->lay_w[member]) * b->lay_r[member];
+    out->runs_per_group = b->lay_r[member];
     out->block_sync = b->lay_sync[member];
     out->lds_staging = b->lay_stage[member];
     return NGPU_OK;

@@ -1,44 +1,39 @@
 /*
  * sim_kernel.h -- the persistent simulation kernel for gfx950.
  *
- * One run (= parameter set x stimulus) is advanced by a group of W wavefronts
- * (64*W lanes).  Its segments are spread over the lanes: position
- * q = slot*LW + lane, NSLOT slots per lane, with segments sorted by their
- * mechanism set so that the 64 lanes of a wave mostly execute the same
- * mechanism code.  Per step:
+ * A group of W wavefronts (LW = 64*W lanes) advances R runs (run = parameter
+ * set x stimulus) together through steps t0..t1; a workgroup holds one or
+ * more groups.  The positions of the R runs are spread over the group's
+ * lanes: position p = c*LW + lane is position q = p mod NQ of run p / NQ,
+ * where the NQ >= N positions of a run are its segments sorted by mechanism
+ * mask (so the 64 lanes of a wave share a run and mostly one mask).  In the
+ * solve phases the lanes take (run, chain) or (run, row) tasks, so the serial
+ * chain latency of the tree is paid once for R runs.  Per step:
  *
- *   1. fused state update + currents: load the mechanism states of the
- *      compartment once, apply the previous step's cnexp update at the current
- *      v, evaluate BREAKPOINT at v+0.001 and v, store the states once;
- *   2. assembly of the Hines rows into LDS (rhs, diagonal, child scatter);
- *   3. triangularisation by tree levels: each lane eliminates one chain of
+ *   1. mechanism pass: per position, load the mechanism states, v and dv
+ *      (state slots), apply the previous step's cnexp at v, evaluate
+ *      BREAKPOINT at v+0.001 and v, store the states, and assemble the Hines
+ *      row (rhs, diagonal) in LDS, folding in the children's terms in son
+ *      order (the reference's separate scatter pass);
+ *   2. triangularisation by tree levels: each task eliminates one chain of
  *      consecutive rows (a section), folding the ends of finished child
  *      chains into the chain's first row, with the reference's arithmetic;
  *      then every row's pivot quotients PX = b/u and PF = -f/u, in parallel;
- *   4. back-substitution by the same chains in reverse level order,
- *      x_i = PX_i + PF_i * x_parent(i), the parent of a chain's end row
+ *   3. back-substitution by the same chains in reverse level order,
+ *      x_i = PX_i + PF_i * x_parent(i), the parent of a chain's last row
  *      coming from a chain of a higher level;
- *   5. v += x.
+ *   4. per position: v += x, dv += x[parent] - x[self], stored.
  *
- * Per-lane values of the lane's positions (v, dv) live in registers and are
- * only ever indexed with compile-time constants: the slot loop of phase 1 is
- * not unrolled (the mechanism code is large), so it works on index 0 and
- * rotates the two arrays by one position per slot.  The calcium current that
- * one step's BREAKPOINT hands to the next step's DERIVATIVE pass is a state
- * slot (SET::ICA) of the compartments whose mechanisms read it.  A run-time index would
- * put the arrays in scratch memory.  Position tables (row, parent row,
- * mechanism mask) are block-shared in LDS.
+ * Registers hold one position's values at a time, so the number of positions
+ * per lane is a run-time loop count.  Groups of one wave synchronise at
+ * wavefront scope (no barrier instruction), larger groups at workgroup scope.
  *
- * Levels are synchronised at wavefront scope when W == 1 (no barrier
- * instruction: a wave's LDS operations complete in issue order, so only the
- * compiler is fenced) and at workgroup scope when W > 1.
- *
- * Two kernels share the per-run code (sim_run):
- *   ngpu_sim_kernel   one morphology; workgroup b runs R = runs_per_block runs;
+ * Two kernels share the per-group code (sim_runs):
+ *   ngpu_sim_kernel   one morphology; workgroup b runs runs_per_block runs;
  *   ngpu_batch_kernel persistent: each workgroup of 8 waves takes items from a
  *                     device work queue (an atomic counter over an item list);
- *                     an item is up to 8/W runs of one morphology, so runs of
- *                     uneven size and step count share one launch.
+ *                     an item is the runs of one workgroup of one morphology,
+ *                     so runs of uneven size and step count share one launch.
  *
  * The algorithm follows the reference kernel:
  *   assembly        Code/OneKernel/src/CudaStuff.cu:263-305
@@ -52,10 +47,12 @@
  * reference shares them between the stimuli of a block), sibling scatter is
  * ordered (the reference races), duplicate padded level entries are not
  * replayed, the cnexp of step t-1 is executed at the start of step t on the
- * same loaded state.  The elimination keeps the reference's operation order
- * (u_i -= f (e / u_{i-1}), b_i -= b_{i-1} e / u_{i-1}): the reference trees
- * couple zero-area nodes with coefficients near 1e22, and there the order of
- * these operations, not the rounding of the membrane terms, sets the result.
+ * same loaded state, and the dv update of step t is done at the end of step
+ * t-1 (the same values).  The elimination keeps the reference's operation
+ * order (u_i -= f (e / u_{i-1}), b_i -= b_{i-1} e / u_{i-1}): the reference
+ * trees couple zero-area nodes with coefficients near 1e22, and there the
+ * order of these operations, not the rounding of the membrane terms, sets
+ * the result.
  */
 #ifndef NGPU_SIM_KERNEL_H
 #define NGPU_SIM_KERNEL_H
@@ -66,11 +63,16 @@
 #define NG_EPS_V 0.001
 
 struct SimArgs {
-    int N, NQ, LW, runs_per_block;
+    int N;                               /* segments */
+    int NQ;                              /* positions per run: N rounded up to a multiple of 64 */
+    int LW;                              /* lanes per group (64 * waves) */
+    int runs_per_group;                  /* R: runs sharing a group's lanes */
     int nvar, nrec, nstim, nt, nt_amps, t0, t1, nruns, depth;
     int stim_q;
     double stim_area, v_init;
-    const int *q_h, *q_ph, *q_meta;      /* [NQ]: row, parent row, mask | son<<24 (-1 = padding) */
+    const int *q_meta;                   /* [NQ] mechanism mask (0 for padding) */
+    const int *q_link;                   /* [4*NQ] row, parent row, first- and second-son position
+                                            (-1 = none); row -1 marks padding */
     const double *q_cm;                  /* [NQ] */
     const double *row_ef;                /* [2N]: e, f interleaved by Hines row */
     const double *pval;                  /* [NPARAM*NQ] value, or tagged NaN = varying column */
@@ -79,53 +81,66 @@ struct SimArgs {
     const double *pconst;                /* [NPARAM] those values */
     const int *chains, *lvl_ptr;         /* chains: (i0, i1, tA, tB, pe); lvl_ptr[depth+2] */
     const int *fath, *flvl_ptr, *fchild; /* fathers with their own phase: (J, c0, c1); flvl_ptr[depth+1] */
-    int lds_staging;                     /* row coefficients e, f and the run's varying
+    int lds_staging;                     /* row coefficients e, f and the runs' varying
                                             parameter values staged in LDS (else read
                                             from global memory); selects the kernel's
                                             STAGE instantiation */
-    int block_sync;                      /* batch kernel: synchronise levels at workgroup scope */
-    int nslot;                           /* batch kernel: positions per lane of this layout (NQ / LW) */
+    int block_sync;                      /* batch kernel: synchronise at workgroup scope */
     int nchains, nfath, nfchild;         /* schedule sizes (staged into LDS as int16) */
     const double *dts, *amps;
     const int *rec_q;
-    double *states;                      /* [nruns][NSTATE][NQ] */
-    double *carry;                       /* [nruns][2*NQ + N + 1]: v, dv, x between launches */
+    double *states;                      /* [nruns][NSTATE + 2][NQ]: mechanism states, v, dv */
     double *vhot;                        /* [nruns][nrec][nt] */
 };
 
-/* Level synchronisation of one run's waves. */
-__device__ __forceinline__ void ng_wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-struct SyncWave  { __device__ __forceinline__ void operator()() const { ng_wave_sync(); } };
-struct SyncBlock { __device__ __forceinline__ void operator()() const { __syncthreads(); } };
+/* Synchronisation of a group.  sync() orders LDS accesses; mem() also makes
+ * the group's global stores visible to its other lanes (the dv that a
+ * parent reads from its children). */
+struct SyncWave {
+    __device__ __forceinline__ void operator()() const
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    __device__ __forceinline__ void mem() const
+    {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
+};
+struct SyncBlock {
+    __device__ __forceinline__ void operator()() const { __syncthreads(); }
+    __device__ __forceinline__ void mem() const { __syncthreads(); }
+};
 struct SyncDyn {
     bool block;
-    __device__ __forceinline__ void operator()() const { if (block) __syncthreads(); else ng_wave_sync(); }
+    __device__ __forceinline__ void operator()() const { if (block) __syncthreads(); else SyncWave{}(); }
+    __device__ __forceinline__ void mem() const { if (block) __syncthreads(); else SyncWave{}.mem(); }
 };
 
 /* LDS layout of a workgroup, in this order:
- *   EF[2N]                                         e, f interleaved by Hines row, when staged
- *   R per-run regions: XU[2(N+1)], PV[npv], REC[64*nrec]   doubles; XU holds
- *                      (b, u), then (PX, PF), then (x, PF) interleaved by row, so
- *                      that a row is one 16-byte access; npv = nvar rounded up to
- *                      even when staged, else 0 (regions stay 16-byte aligned)
- *   QM[NQS] int, QH[NQS] short, QP[NQS] short      NQS = NSLOT*LW positions
+ *   EF[2N]                 e, f interleaved by Hines row, when staged
+ *   per run (runs_per_block of them): XU[2(N+1)], PV[npv], REC[64*nrec]
+ *                          doubles; XU holds (b, u), then (PX, PF), then
+ *                          (x, PF) interleaved by row, so that a row is one
+ *                          16-byte access; npv = nvar rounded up to even when
+ *                          staged, else 0 (regions stay 16-byte aligned)
+ *   QL[NQ]                 short4: row, parent row, son positions
+ *   QM[NQ]                 unsigned short: mechanism mask
  *   CH[5*nchains], FA[3*nfath], FC[nfchild], LP[depth+2], FLP[depth+1] short */
 __host__ __device__ inline size_t ng_lds_per_run(int N, int npv, int nrec)
 {
     return (size_t)(2 * (N + 1) + ((npv + 1) & ~1) + 64 * nrec);
 }
-__host__ __device__ inline size_t ng_lds_bytes(int N, int NQS, int R, int npv, int nrec, int staged,
+__host__ __device__ inline size_t ng_lds_bytes(int N, int NQ, int runs, int npv, int nrec, int staged,
                                                int nchains, int nfath, int nfchild, int depth)
 {
-    const size_t nshort = 2 * (size_t)NQS + 5 * (size_t)nchains + 3 * (size_t)nfath + (size_t)nfchild +
+    const size_t nshort = 5 * (size_t)NQ + 5 * (size_t)nchains + 3 * (size_t)nfath + (size_t)nfchild +
                           (size_t)(depth + 2) + (size_t)(depth + 1);
-    const size_t dbl = (staged ? 2 * (size_t)N : 0) + (size_t)R * ng_lds_per_run(N, staged ? npv : 0, nrec);
-    return dbl * sizeof(double) + (((size_t)NQS * sizeof(int) + nshort * sizeof(short) + 7) / 8) * 8;
+    const size_t dbl = (staged ? 2 * (size_t)N : 0) + (size_t)runs * ng_lds_per_run(N, staged ? npv : 0, nrec);
+    return dbl * sizeof(double) + ((nshort * sizeof(short) + 7) / 8) * 8;
 }
 
 /* Parameter tables hold either the value or, for a run-varying column k, a
@@ -156,17 +171,16 @@ struct ParamFetch {
 struct Tables {
     const double2 *EF;                   /* (e, f) by Hines row */
     double *runs;                        /* first per-run region */
-    int *QM;
-    short *QH, *QP;
-    short *CH, *FA, *FC, *LP, *FLP;
+    const short4 *QL;
+    const unsigned short *QM;
+    const short *CH, *FA, *FC, *LP, *FLP;
 };
 
-/* Stage the block-shared tables of morphology a for R runs and NQS = NSLOT*LW
- * positions; positions beyond the morphology's own table are padding. */
+/* Stage the block-shared tables of morphology a for `runs` per-run regions. */
 template <bool STAGE>
-__device__ __forceinline__ Tables stage_tables(const SimArgs &a, double *lds, int R, int NQS)
+__device__ __forceinline__ Tables stage_tables(const SimArgs &a, double *lds, int runs)
 {
-    const int N = a.N;
+    const int N = a.N, NQ = a.NQ;
     const int npv = STAGE ? a.nvar : 0;
     Tables T;
     if constexpr (STAGE) {
@@ -177,25 +191,21 @@ __device__ __forceinline__ Tables stage_tables(const SimArgs &a, double *lds, in
         T.EF = (const double2 *)a.row_ef;
         T.runs = lds;
     }
-    T.QM = (int *)(T.runs + (size_t)R * ng_lds_per_run(N, npv, a.nrec));
-    T.QH = (short *)(T.QM + NQS);
-    T.QP = T.QH + NQS;
-    T.CH = T.QP + NQS;
-    T.FA = T.CH + 5 * a.nchains;
-    T.FC = T.FA + 3 * a.nfath;
-    T.LP = T.FC + a.nfchild;
-    T.FLP = T.LP + (a.depth + 2);
-    for (int i = threadIdx.x; i < NQS; i += blockDim.x) {
-        const bool in = i < N;
-        T.QM[i] = in ? a.q_meta[i] : 0;
-        T.QH[i] = (short)(in ? a.q_h[i] : -1);
-        T.QP[i] = (short)(in ? a.q_ph[i] : 0);
+    short4 *QL = (short4 *)(T.runs + (size_t)runs * ng_lds_per_run(N, npv, a.nrec));
+    unsigned short *QM = (unsigned short *)(QL + NQ);
+    short *CH = (short *)(QM + NQ);
+    short *FA = CH + 5 * a.nchains, *FC = FA + 3 * a.nfath, *LP = FC + a.nfchild, *FLP = LP + (a.depth + 2);
+    for (int i = threadIdx.x; i < NQ; i += blockDim.x) {
+        QL[i] = make_short4((short)a.q_link[4 * i], (short)a.q_link[4 * i + 1], (short)a.q_link[4 * i + 2],
+                            (short)a.q_link[4 * i + 3]);
+        QM[i] = (unsigned short)a.q_meta[i];
     }
-    for (int i = threadIdx.x; i < 5 * a.nchains; i += blockDim.x) T.CH[i] = (short)a.chains[i];
-    for (int i = threadIdx.x; i < 3 * a.nfath; i += blockDim.x) T.FA[i] = (short)a.fath[i];
-    for (int i = threadIdx.x; i < a.nfchild; i += blockDim.x) T.FC[i] = (short)a.fchild[i];
-    for (int i = threadIdx.x; i < a.depth + 2; i += blockDim.x) T.LP[i] = (short)a.lvl_ptr[i];
-    for (int i = threadIdx.x; i < a.depth + 1; i += blockDim.x) T.FLP[i] = (short)a.flvl_ptr[i];
+    for (int i = threadIdx.x; i < 5 * a.nchains; i += blockDim.x) CH[i] = (short)a.chains[i];
+    for (int i = threadIdx.x; i < 3 * a.nfath; i += blockDim.x) FA[i] = (short)a.fath[i];
+    for (int i = threadIdx.x; i < a.nfchild; i += blockDim.x) FC[i] = (short)a.fchild[i];
+    for (int i = threadIdx.x; i < a.depth + 2; i += blockDim.x) LP[i] = (short)a.lvl_ptr[i];
+    for (int i = threadIdx.x; i < a.depth + 1; i += blockDim.x) FLP[i] = (short)a.flvl_ptr[i];
+    T.QL = QL; T.QM = QM; T.CH = CH; T.FA = FA; T.FC = FC; T.LP = LP; T.FLP = FLP;
     return T;
 }
 
@@ -244,165 +254,140 @@ __device__ __forceinline__ void ng_fold(double ut, double bt, double e, double f
     bJ = bJ - q2;
 }
 
-/* Steps t0..t1 of one run by the LW lanes of its group (lane g).  XU is the
- * run's LDS region.  Runs from nruns up to the next multiple of 8 are padding
- * (buffers are allocated for them, their parameters are the last set's): the
- * batch kernel gives them to the unused groups of a partial item. */
-template <class SET, int NSLOT, bool STAGE, class SYNC>
-__device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, double *XU, int run, int g,
-                                        const SYNC &sync)
+/* Steps t0..t1 of runs run0 .. run0 + R - 1 by the LW lanes of a group
+ * (lane g).  RG is the first of the group's R per-run LDS regions.  Runs from
+ * nruns up to the allocated count are padding (their parameters are the last
+ * set's): partial workgroups simulate them instead of leaving lanes idle. */
+template <class SET, bool STAGE, class SYNC>
+__device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, double *RG, int run0, int g,
+                                         const SYNC &sync)
 {
-    const int LW = a.LW, N = a.N, NQ = a.NQ;
+    const int LW = a.LW, N = a.N, NQ = a.NQ, R = a.runs_per_group;
+    constexpr int NS = SET::NSTATE + 2, SV = SET::NSTATE, SDV = SET::NSTATE + 1;
     const int npv = STAGE ? ((a.nvar + 1) & ~1) : 0;
+    const size_t PR = ng_lds_per_run(N, STAGE ? a.nvar : 0, a.nrec);
+    const int P = R * NQ;                                 /* positions of the group */
     const double2 *EF = T.EF;
-    int *QM = T.QM;
-    short *QH = T.QH, *QP = T.QP, *CH = T.CH, *FA = T.FA, *FC = T.FC, *LP = T.LP, *FLP = T.FLP;
-    double2 *XU2 = (double2 *)XU;                         /* (b, u) by Hines row        */
-    double *PV = XU + 2 * (N + 1);                        /* run-varying parameters     */
-    double *REC = PV + npv;
-    const int set0 = run / a.nstim, stim = run - set0 * a.nstim;
-    const int set = min(set0, a.nruns / a.nstim - 1);
-    double *S = a.states + (size_t)run * SET::NSTATE * NQ;
-    double *CY = a.carry + (size_t)run * (2 * NQ + N + 1);
-    const double *pvg = a.pv + (size_t)set * a.nvar;
-    const double *pvsrc = pvg;
-    if constexpr (STAGE) {
-        for (int k = g; k < a.nvar; k += LW) PV[k] = pvg[k];
-        pvsrc = PV;
-    }
+    const short4 *QL = T.QL;
+    const unsigned short *QM = T.QM;
+    const short *CH = T.CH, *FA = T.FA, *FC = T.FC, *LP = T.LP, *FLP = T.FLP;
+    const int nsets = a.nruns / a.nstim;
+    auto region = [&](int r) { return RG + (size_t)r * PR; };
+    auto states = [&](int r) { return a.states + (size_t)(run0 + r) * NS * NQ; };
+    auto pvof = [&](int r) -> const double * {
+        if constexpr (STAGE) return region(r) + 2 * (N + 1);
+        else return a.pv + (size_t)min((run0 + r) / a.nstim, nsets - 1) * a.nvar;
+    };
 
-    double v[NSLOT], dv[NSLOT];
-#pragma unroll
-    for (int c = 0; c < NSLOT; c++) { v[c] = (c * LW + g < N) ? a.v_init : 0.0; dv[c] = 0.0; }
+    if constexpr (STAGE)
+        for (int i = g; i < R * a.nvar; i += LW) {
+            const int r = i / a.nvar, k = i - r * a.nvar;
+            region(r)[2 * (N + 1) + k] = a.pv[(size_t)min((run0 + r) / a.nstim, nsets - 1) * a.nvar + k];
+        }
+    sync();
     if (a.t0 == 0) {
-        for (int i = g; i < 2 * (N + 1); i += LW) XU[i] = 0.0;
+        /* INITIAL: states at steady state, v = v_init, dv = 0 */
 #pragma unroll 1
-        for (int c = 0; c < NSLOT; c++) {
-            const int q = c * LW + g;
+        for (int p = g; p < P; p += LW) {
+            const int r = p / NQ, q = p - r * NQ;
             if (q >= N) continue;
-            const unsigned m = (unsigned)QM[q] & 0xffffffu;
-            const ParamFetch P{a.pconst, a.pval, pvsrc, a.pconst_mask, q, NQ};
+            double *S = states(r);
+            const unsigned m = QM[q];
+            const ParamFetch Pf{a.pconst, a.pval, pvof(r), a.pconst_mask, q, NQ};
             double st[SET::NSTATE];
 #pragma unroll
             for (int k = 0; k < SET::NSTATE; k++) st[k] = 0.0;
-            SET::init(m, a.v_init, st, P);
+            SET::init(m, a.v_init, st, Pf);
             const unsigned sb = SET::state_bits(m);
 #pragma unroll
             for (int k = 0; k < SET::NSTATE; k++)
                 if (sb & (1u << k)) S[k * NQ + q] = st[k];
-        }
-    } else {
-        for (int i = g; i < N; i += LW) XU[2 * i] = CY[2 * NQ + i];
-#pragma unroll
-        for (int c = 0; c < NSLOT; c++) {
-            const int q = c * LW + g;
-            if (q < N) { v[c] = CY[q]; dv[c] = CY[NQ + q]; }
+            S[SV * NQ + q] = a.v_init;
+            S[SDV * NQ + q] = 0.0;
         }
     }
-    sync();
+    sync.mem();
 
     double dtp = a.t0 > 0 ? a.dts[a.t0 - 1] : 0.0;
-    const double *amp_row = a.amps + (size_t)stim * a.nt_amps;
-    double *out = a.vhot + (size_t)run * a.nrec * a.nt;
 
     for (int t = a.t0; t < a.t1; t++) {
         const double dt = a.dts[t];
-        const double amp = amp_row[t];
         const int slot_t = (t - a.t0) & 63;
-        /* record v before the step (CudaStuff.cu:257-261) */
-        for (int r = 0; r < a.nrec; r++) {
-            const int rq = a.rec_q[r];
-#pragma unroll
-            for (int c = 0; c < NSLOT; c++)
-                if (c * LW + g == rq) REC[r * 64 + slot_t] = v[c];
-        }
 
-        /* dv += x[parent] - x[self] with the previous solution (CudaStuff.cu:287) */
-#pragma unroll
-        for (int c = 0; c < NSLOT; c++) {
-            const int q = c * LW + g, h = QH[q];
-            if (h >= 0) dv[c] += XU[2 * QP[q]] - XU[2 * h];
-        }
-        sync();
-
-        /* fused cnexp(t-1) + BREAKPOINT(t) + Hines row (CudaStuff.cu:263-294).
-         * eca is stored by INITIAL only: at t = 0 it is loaded, afterwards it is
-         * recomputed from cai after the cnexp pass. */
+        /* mechanism pass: fused cnexp(t-1) + BREAKPOINT(t) + Hines row
+         * (CudaStuff.cu:263-305).  eca is stored by INITIAL only: at t = 0 it
+         * is loaded, afterwards it is recomputed from cai after the cnexp. */
         const unsigned keep = (t > 0) ? ~(1u << SET::ECA) : ~0u;
 #pragma unroll 1
-        for (int c = 0; c < NSLOT; c++) {
-            /* slot c's values are at index 0: the arrays rotate by one per slot */
-            const int q = c * LW + g;
-            const double vq = v[0], dvq = dv[0];
+        for (int p = g; p < P; p += LW) {
+            const int r = p / NQ, q = p - r * NQ;
+            if (q >= N) continue;
+            double *S = states(r);
+            double *XU = region(r);
             /* the mechanism code is specialised for the set's common mechanism
              * masks (straight-line code, independent gates interleaved) */
-            SET::dispatch((unsigned)QM[q] & 0xffffffu, [&](auto mask) {
+            SET::dispatch(QM[q], [&](auto mask) {
                 const unsigned m = mask;
+                const short4 l = QL[q];                       /* row, parent row, sons */
                 double st[SET::NSTATE];
                 const unsigned sbn = (SET::state_bits(m) & keep) | (t > 0 ? SET::ica_bits(m) : 0u);
 #pragma unroll
-                for (int k = 0; k < SET::NSTATE; k++) st[k] = (q < N && (sbn & (1u << k))) ? S[k * NQ + q] : 0.0;
-                if (q < N) {
-                    const int h = QH[q];
-                    const double cmq = a.q_cm[q];
-                    const double fh = EF[h].y;
-                    const unsigned sb = SET::state_bits(m) & keep;
-                    const unsigned ib = SET::ica_bits(m);
-                    const ParamFetch P{a.pconst, a.pval, pvsrc, a.pconst_mask, q, NQ};
-                    if (t > 0) {
-                        SET::cnexp(m, vq, dtp, st, ib ? st[SET::ICA] : 0.0, P);
-                        SET::eca_after_cnexp(m, st);
+                for (int k = 0; k < SET::NSTATE; k++) st[k] = (sbn & (1u << k)) ? S[k * NQ + q] : 0.0;
+                const double vq = S[SV * NQ + q], dvq = S[SDV * NQ + q];
+                const double dv1 = l.z >= 0 ? S[SDV * NQ + l.z] : 0.0, dv2 = l.w >= 0 ? S[SDV * NQ + l.w] : 0.0;
+                const int h = l.x;
+                const double cmq = a.q_cm[q];
+                const double fh = EF[h].y;
+                /* record v before the step (CudaStuff.cu:257-261) */
+                for (int j = 0; j < a.nrec; j++)
+                    if (q == a.rec_q[j]) XU[2 * (N + 1) + npv + j * 64 + slot_t] = vq;
+                const unsigned sb = SET::state_bits(m) & keep;
+                const unsigned ib = SET::ica_bits(m);
+                const ParamFetch Pf{a.pconst, a.pval, pvof(r), a.pconst_mask, q, NQ};
+                if (t > 0) {
+                    SET::cnexp(m, vq, dtp, st, ib ? st[SET::ICA] : 0.0, Pf);
+                    SET::eca_after_cnexp(m, st);
 #pragma unroll
-                        for (int k = 0; k < SET::NSTATE; k++)
-                            if (sb & (1u << k)) S[k * NQ + q] = st[k];
-                    }
-                    double sumDv = 0.0, sum = 0.0, icaDv = 0.0, icaNow = 0.0;
-                    SET::current(m, vq + 0.001, st, P, sumDv, icaDv, false);
-                    SET::current(m, vq, st, P, sum, icaNow, true);
-                    if (ib) S[SET::ICA * NQ + q] = icaNow;
-                    const double istim = (q == a.stim_q) ? 100 * amp / a.stim_area : 0.0;
-                    const double gModel = (sumDv - sum) / NG_EPS_V;
-                    double rhs = istim - sum;
-                    double D = gModel + cmq / (dt * 1000);
-                    D -= fh;
-                    rhs -= fh * dvq;
-                    XU2[h] = make_double2(rhs, D);
+                    for (int k = 0; k < SET::NSTATE; k++)
+                        if (sb & (1u << k)) S[k * NQ + q] = st[k];
                 }
+                double sumDv = 0.0, sum = 0.0, icaDv = 0.0, icaNow = 0.0;
+                SET::current(m, vq + 0.001, st, Pf, sumDv, icaDv, false);
+                SET::current(m, vq, st, Pf, sum, icaNow, true);
+                if (ib) S[SET::ICA * NQ + q] = icaNow;
+                double istim = 0.0;
+                if (q == a.stim_q) {
+                    const int stim = (run0 + r) % a.nstim;
+                    istim = 100 * a.amps[(size_t)stim * a.nt_amps + t] / a.stim_area;
+                }
+                const double gModel = (sumDv - sum) / NG_EPS_V;
+                double rhs = istim - sum;
+                double D = gModel + cmq / (dt * 1000);
+                D -= fh;
+                rhs -= fh * dvq;
+                /* the children's terms, first son then second son
+                 * (CudaStuff.cu:296-305, ordered) */
+                if (l.z >= 0) { const double e = EF[QL[l.z].x].x; rhs += e * dv1; D -= e; }
+                if (l.w >= 0) { const double e = EF[QL[l.w].x].x; rhs += e * dv2; D -= e; }
+                ((double2 *)XU)[h] = make_double2(rhs, D);
             });
-#pragma unroll
-            for (int i = 0; i + 1 < NSLOT; i++) { v[i] = v[i + 1]; dv[i] = dv[i + 1]; }
-            v[NSLOT - 1] = vq; dv[NSLOT - 1] = dvq;
         }
         sync();
-        /* children scatter into parents, first children then second children
-         * (CudaStuff.cu:296-305; ordered, not racing) */
-#pragma unroll
-        for (int son = 1; son <= 2; son++) {
-#pragma unroll
-            for (int c = 0; c < NSLOT; c++) {
-                const int q = c * LW + g, h = QH[q];
-                if (h < 0 || (QM[q] >> 24) != son) continue;
-                const double e = EF[h].x;
-                double2 p = XU2[QP[q]];
-                p.x += e * dv[c];
-                p.y -= e;
-                XU2[QP[q]] = p;
-            }
-            sync();
-        }
 
         /* triangularisation by levels (BeforeLU, CudaStuff.cu:31-69), with the
          * reference's arithmetic.  Level L first folds the fathers of level
          * L-1 that have their own phase, then eliminates its chains; a chain
          * (i0, i1, tA, tB, pe) folds up to two related rows tA, tB (-1 = none)
          * into its first row i0-1, then eliminates rows i0-1 .. i1-2 each into
-         * the next.  Every load that does not depend on the chain's own
-         * results is issued up front, and the rows' values are loaded two
-         * rows ahead (indices clamped to the chain, so unconditionally). */
+         * the next.  A task is (run, chain); every load that does not depend
+         * on the chain's own results is issued up front, and the next row's
+         * values are loaded one row ahead (indices clamped to the chain). */
         for (int L = 0; L <= a.depth; L++) {
             if (L > 0 && FLP[L] > FLP[L - 1]) {
-                const int f1e = FLP[L];
-                for (int k = FLP[L - 1] + g; k < f1e; k += LW) {
+                const int f0 = FLP[L - 1], nf = FLP[L] - f0;
+                for (int tk = g; tk < R * nf; tk += LW) {
+                    const int r = tk / nf, k = f0 + tk - r * nf;
+                    double2 *XU2 = (double2 *)region(r);
                     const int J = FA[3 * k], c0 = FA[3 * k + 1], c1 = FA[3 * k + 2];
                     double2 pJ = XU2[J];
                     for (int cc = c0; cc < c1; cc++) {
@@ -414,8 +399,10 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
                 }
                 sync();
             }
-            const int k1 = LP[L + 1];
-            for (int k = LP[L] + g; k < k1; k += LW) {
+            const int k0 = LP[L], nc = LP[L + 1] - k0;
+            for (int tk = g; tk < R * nc; tk += LW) {
+                const int r = tk / nc, k = k0 + tk - r * nc;
+                double2 *XU2 = (double2 *)region(r);
                 const short *ch = CH + 5 * k;
                 const int i0 = ch[0], i1 = ch[1], tA = ch[2], tB = ch[3];
                 const int ta = max(tA, 0), tb = max(tB, 0), j0 = min(i0, i1 - 1);
@@ -444,10 +431,12 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
 
         /* pivot quotients of every row, in parallel: PX = b/u and PF = -f/u
          * (BKSUB1's first step, CudaStuff.cu:75-78) replace b and u */
-        for (int h = g; h < N; h += LW) {
-            const double2 p = XU2[h];
-            const double r = ng_rcp(p.y);
-            XU2[h] = make_double2(p.x * r, -(EF[h].y * r));
+        for (int i = g; i < R * N; i += LW) {
+            const int r = i / N, h = i - r * N;
+            double2 *XU2 = (double2 *)region(r);
+            const double2 ph = XU2[h];
+            const double rc = ng_rcp(ph.y);
+            XU2[h] = make_double2(ph.x * rc, -(EF[h].y * rc));
         }
         sync();
 
@@ -457,8 +446,11 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
          * chain's first row, with the quotients loaded two rows ahead; x
          * replaces PX */
         for (int L = a.depth; L >= 0; L--) {
-            const int k1 = LP[L + 1];
-            for (int k = LP[L] + g; k < k1; k += LW) {
+            const int k0 = LP[L], nc = LP[L + 1] - k0;
+            for (int tk = g; tk < R * nc; tk += LW) {
+                const int r = tk / nc, k = k0 + tk - r * nc;
+                double *XU = region(r);
+                const double2 *XU2 = (const double2 *)XU;
                 const short *ch = CH + 5 * k;
                 const int i0 = ch[0], i1 = ch[1], pe = ch[4];
                 const int h = i1 - 1, lo = i0 - 1;
@@ -479,32 +471,31 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
             sync();
         }
 
-        /* v update (CudaStuff.cu:312-315) */
-#pragma unroll
-        for (int c = 0; c < NSLOT; c++)
-            { const int h = QH[c * LW + g]; if (h >= 0) v[c] += XU[2 * h]; }
+        /* v += x (CudaStuff.cu:312-315) and dv += x[parent] - x[self] for the
+         * next step (CudaStuff.cu:287) */
+#pragma unroll 2
+        for (int p = g; p < P; p += LW) {
+            const int r = p / NQ, q = p - r * NQ;
+            if (q >= N) continue;
+            const double *XU = region(r);
+            double *S = states(r);
+            const short4 l = QL[q];
+            const double xh = XU[2 * l.x], xpar = XU[2 * l.y];
+            S[SV * NQ + q] = S[SV * NQ + q] + xh;
+            S[SDV * NQ + q] = S[SDV * NQ + q] + (xpar - xh);
+        }
         dtp = dt;
 
         /* flush the recorded chunk */
         if (slot_t == 63 || t == a.t1 - 1) {
-            sync();
-            const int base = t - slot_t;
-            for (int i = g; i < a.nrec * 64; i += LW) {
-                const int r = i >> 6, j = i & 63;
-                if (j <= slot_t) out[(size_t)r * a.nt + base + j] = REC[i];
+            const int base = t - slot_t, nrec64 = a.nrec * 64;
+            for (int i = g; i < R * nrec64; i += LW) {
+                const int r = i / nrec64, k = i - r * nrec64, j = k & 63;
+                if (j <= slot_t)
+                    a.vhot[((size_t)(run0 + r) * a.nrec + (k >> 6)) * a.nt + base + j] = region(r)[2 * (N + 1) + npv + k];
             }
         }
-        sync();
-    }
-
-    /* carry for the next launch */
-    if (a.t1 < a.nt) {
-        for (int i = g; i < N; i += LW) CY[2 * NQ + i] = XU[2 * i];
-#pragma unroll
-        for (int c = 0; c < NSLOT; c++) {
-            const int q = c * LW + g;
-            if (q < N) { CY[q] = v[c]; CY[NQ + q] = dv[c]; }
-        }
+        sync.mem();
     }
 }
 
@@ -512,21 +503,23 @@ __device__ __forceinline__ void sim_run(const SimArgs &a, const Tables &T, doubl
 #define NG_WAVES_PER_EU 2
 #endif
 
-template <class SET, int NSLOT, bool MULTI, bool STAGE>
-__global__ void __launch_bounds__(MULTI ? 512 : 256) __attribute__((amdgpu_waves_per_eu(NG_WAVES_PER_EU)))
+/* One morphology.  Workgroup b holds G = blockDim.x / LW groups of R runs:
+ * runs b * G * R + group * R + (0 .. R-1). */
+template <class SET, bool MULTI, bool STAGE>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NG_WAVES_PER_EU)))
 ngpu_sim_kernel(SimArgs a)
 {
     extern __shared__ double lds[];
-    const int LW = a.LW;
+    const int LW = a.LW, R = a.runs_per_group;
+    const int G = blockDim.x / LW;
     const int group = threadIdx.x / LW;
     const int g = threadIdx.x - group * LW;
-    const Tables T = stage_tables<STAGE>(a, lds, a.runs_per_block, NSLOT * LW);
+    const Tables T = stage_tables<STAGE>(a, lds, G * R);
     __syncthreads();
-    const int run = blockIdx.x * a.runs_per_block + group;
-    if (run >= a.nruns) return;          /* W > 1 implies one run per workgroup */
-    double *XU = T.runs + (size_t)group * ng_lds_per_run(a.N, STAGE ? a.nvar : 0, a.nrec);
-    if constexpr (MULTI) sim_run<SET, NSLOT, STAGE>(a, T, XU, run, g, SyncBlock{});
-    else sim_run<SET, NSLOT, STAGE>(a, T, XU, run, g, SyncWave{});
+    const int run0 = (blockIdx.x * G + group) * R;
+    double *RG = T.runs + (size_t)group * R * ng_lds_per_run(a.N, STAGE ? a.nvar : 0, a.nrec);
+    if constexpr (MULTI) sim_runs<SET, STAGE>(a, T, RG, run0, g, SyncBlock{});
+    else sim_runs<SET, STAGE>(a, T, RG, run0, g, SyncWave{});
 }
 
 /* Work queue of the batch kernel: item i runs runs [y, y + z) of morphology x.
@@ -542,11 +535,8 @@ struct BatchArgs {
 };
 
 #define NG_BATCH_THREADS 512
-#define NG_BATCH_NSLOT 6                 /* positions per lane: up to 64*8*6 = 3072 segments */
 
-/* NSLOT positions per lane at most; layouts of up to NSLOT_SMALL positions
- * per lane run a copy of the step code specialised for that count. */
-template <class SET, int NSLOT, int NSLOT_SMALL>
+template <class SET>
 __global__ void __launch_bounds__(NG_BATCH_THREADS) __attribute__((amdgpu_waves_per_eu(NG_WAVES_PER_EU)))
 ngpu_batch_kernel(BatchArgs b)
 {
@@ -561,26 +551,23 @@ ngpu_batch_kernel(BatchArgs b)
         const int4 item = b.items[it];
         const int mi = __builtin_amdgcn_readfirstlane(item.x);
         const int run0 = __builtin_amdgcn_readfirstlane(item.y);
-        const int nrun = __builtin_amdgcn_readfirstlane(item.z);
         const SimArgs &a = b.morph[mi];
         if (threadIdx.x == 0) b.stamps[2 * it] = wall_clock64();
-        const int LW = a.LW, R = NG_BATCH_THREADS / LW;
+        const int LW = a.LW, R = a.runs_per_group, G = NG_BATCH_THREADS / LW;
         const int group = threadIdx.x / LW;
         const int g = threadIdx.x - group * LW;
         const SyncDyn sync{a.block_sync != 0};
-        (void)nrun;                      /* groups beyond nrun simulate padding runs */
+        /* groups beyond the item's runs simulate padding runs */
         if (a.lds_staging) {
-            const Tables T = stage_tables<true>(a, lds, R, NSLOT * LW);
+            const Tables T = stage_tables<true>(a, lds, G * R);
             __syncthreads();
-            double *XU = T.runs + (size_t)group * ng_lds_per_run(a.N, a.nvar, a.nrec);
-            if (a.nslot <= NSLOT_SMALL) sim_run<SET, NSLOT_SMALL, true>(a, T, XU, run0 + group, g, sync);
-            else sim_run<SET, NSLOT, true>(a, T, XU, run0 + group, g, sync);
+            double *RG = T.runs + (size_t)group * R * ng_lds_per_run(a.N, a.nvar, a.nrec);
+            sim_runs<SET, true>(a, T, RG, run0 + group * R, g, sync);
         } else {
-            const Tables T = stage_tables<false>(a, lds, R, NSLOT * LW);
+            const Tables T = stage_tables<false>(a, lds, G * R);
             __syncthreads();
-            double *XU = T.runs + (size_t)group * ng_lds_per_run(a.N, 0, a.nrec);
-            if (a.nslot <= NSLOT_SMALL) sim_run<SET, NSLOT_SMALL, false>(a, T, XU, run0 + group, g, sync);
-            else sim_run<SET, NSLOT, false>(a, T, XU, run0 + group, g, sync);
+            double *RG = T.runs + (size_t)group * R * ng_lds_per_run(a.N, 0, a.nrec);
+            sim_runs<SET, false>(a, T, RG, run0 + group * R, g, sync);
         }
         __syncthreads();
         if (threadIdx.x == 0) b.stamps[2 * it + 1] = wall_clock64();
@@ -588,11 +575,11 @@ ngpu_batch_kernel(BatchArgs b)
 }
 
 /* Host-side launchers of one mechanism set's kernels (instantiated once per
- * set, in sim_<set>.hip). */
-template <class SET, int NS, bool MULTI, bool STAGE>
+ * set, in sim_<set>.hip).  multi: groups of more than one wave. */
+template <class SET, bool MULTI, bool STAGE>
 static hipError_t ng_launch_sim(dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a)
 {
-    auto k = ngpu_sim_kernel<SET, NS, MULTI, STAGE>;
+    auto k = ngpu_sim_kernel<SET, MULTI, STAGE>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -601,34 +588,19 @@ static hipError_t ng_launch_sim(dim3 grid, dim3 block, size_t lds, hipStream_t s
     return hipGetLastError();
 }
 
-template <class SET, int NS>
-static hipError_t ng_launch_sim_ns(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a)
-{
-    if (multi) return a.lds_staging ? ng_launch_sim<SET, NS, true, true>(grid, block, lds, s, a)
-                                    : ng_launch_sim<SET, NS, true, false>(grid, block, lds, s, a);
-    return a.lds_staging ? ng_launch_sim<SET, NS, false, true>(grid, block, lds, s, a)
-                         : ng_launch_sim<SET, NS, false, false>(grid, block, lds, s, a);
-}
-
-/* nslot positions per lane (1, 2, 4, 6 or 8); multi: more than one wave per run */
 template <class SET>
-static hipError_t ng_launch_set(int nslot, bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s,
-                                const SimArgs &a)
+static hipError_t ng_launch_set(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a)
 {
-    switch (nslot) {
-    case 1: return ng_launch_sim_ns<SET, 1>(multi, grid, block, lds, s, a);
-    case 2: return ng_launch_sim_ns<SET, 2>(multi, grid, block, lds, s, a);
-    case 4: return ng_launch_sim_ns<SET, 4>(multi, grid, block, lds, s, a);
-    case 6: return ng_launch_sim_ns<SET, 6>(multi, grid, block, lds, s, a);
-    case 8: return ng_launch_sim_ns<SET, 8>(multi, grid, block, lds, s, a);
-    default: return hipErrorInvalidValue;
-    }
+    if (multi) return a.lds_staging ? ng_launch_sim<SET, true, true>(grid, block, lds, s, a)
+                                    : ng_launch_sim<SET, true, false>(grid, block, lds, s, a);
+    return a.lds_staging ? ng_launch_sim<SET, false, true>(grid, block, lds, s, a)
+                         : ng_launch_sim<SET, false, false>(grid, block, lds, s, a);
 }
 
 template <class SET>
 static hipError_t ng_launch_batch_set(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b)
 {
-    auto k = ngpu_batch_kernel<SET, NG_BATCH_NSLOT, 4>;
+    auto k = ngpu_batch_kernel<SET>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;

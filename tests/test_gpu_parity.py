@@ -1,7 +1,8 @@
 """GPU parity: libneurogpu's HIP kernels against the FP64 oracle on the same
 inputs.  The bar is max-abs trace difference <= 1e-6 mV and identical spike
-counts; with the shared transcendental functions and -ffp-contract=off on both
-sides the traces are expected to be bit-identical."""
+counts.  The solve keeps the reference's elimination arithmetic bit for bit;
+the mechanism code rearranges the rate equations, so traces differ from the
+oracle by rounding (about 1e-8 mV at most over 10,000 steps)."""
 import os
 import sys
 
@@ -44,7 +45,7 @@ def test_passive64_full_size():
     _check(gpu, _oracle(w), "passive64")
 
 
-@pytest.mark.parametrize("launch", [None, dict(waves_per_run=2), dict(runs_per_block=1),
+@pytest.mark.parametrize("launch", [None, dict(waves_per_run=2), dict(runs_per_group=2),
                                     dict(steps_per_launch=700), dict(steps_per_launch=500, use_graph=True)])
 def test_active500_small(launch):
     w = models.workload("active500", nsets=6, nt=1500)
@@ -53,7 +54,7 @@ def test_active500_small(launch):
     _check(gpu, _oracle(w), f"active500 {launch}")
 
 
-@pytest.mark.parametrize("launch", [None, dict(waves_per_run=4), dict(waves_per_run=1, runs_per_block=2)])
+@pytest.mark.parametrize("launch", [None, dict(waves_per_run=4), dict(waves_per_run=8, runs_per_group=3)])
 def test_ttpc1200_small(launch):
     w = models.workload("ttpc1200", nsets=3, nt=1200)
     _say(f"case ttpc1200 small launch={launch}")
