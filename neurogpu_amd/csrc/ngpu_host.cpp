@@ -468,13 +468,13 @@ static int positions(int N) { return (N + 63) / 64 * 64; }
  * synchronise at wavefront scope, without a barrier instruction); larger
  * trees take 8 waves.  Runs per group: as many as fit in LDS, up to
  * max_runs_per_group (measured on the 500- and 1,200-segment cells: two runs
- * per one-wave group, four per eight-wave group). */
+ * per one-wave group, six per eight-wave group). */
 static void choose_layout(const ngpu_ctx *c, int &W, int &R)
 {
     W = c->cfg_w > 0 ? c->cfg_w : (c->N <= 512 ? 1 : 8);
     R = c->cfg_r > 0 ? c->cfg_r : 0;
 }
-static int max_runs_per_group(int W) { return W == 1 ? 2 : 4; }
+static int max_runs_per_group(int W) { return W == 1 ? 2 : 6; }
 
 static int build_tables(ngpu_ctx *c)
 {

@@ -473,16 +473,17 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
 
         /* v += x (CudaStuff.cu:312-315) and dv += x[parent] - x[self] for the
          * next step (CudaStuff.cu:287) */
-#pragma unroll 2
+#pragma unroll 4
         for (int p = g; p < P; p += LW) {
-            const int r = p / NQ, q = p - r * NQ;
-            if (q >= N) continue;
+            /* padding positions read a real one and store nothing, so that
+             * the loads of several positions can be in flight together */
+            const int r = p / NQ, q = p - r * NQ, qr = min(q, N - 1);
             const double *XU = region(r);
             double *S = states(r);
-            const short4 l = QL[q];
+            const short4 l = QL[qr];
             const double xh = XU[2 * l.x], xpar = XU[2 * l.y];
-            S[SV * NQ + q] = S[SV * NQ + q] + xh;
-            S[SDV * NQ + q] = S[SDV * NQ + q] + (xpar - xh);
+            const double vn = S[SV * NQ + qr] + xh, dvn = S[SDV * NQ + qr] + (xpar - xh);
+            if (q < N) { S[SV * NQ + q] = vn; S[SDV * NQ + q] = dvn; }
         }
         dtp = dt;
 
