@@ -143,6 +143,54 @@ def poly_mutation(rng, a, lo, hi, eta=20.0, indpb=None):
     return y
 
 
+def sbx_rows(A, B, lo, hi, u_gene, u_beta, u_swap, eta=20.0):
+    """First children of bounded SBX for rows of parents A, B [n, g], from
+    given uniform numbers [n, g] (gene crossed when u_gene <= 0.5, spread
+    from u_beta, order from u_swap): the same arithmetic as :func:`sbx`, one
+    array operation per step instead of one Python step per gene."""
+    x1, x2 = np.minimum(A, B), np.maximum(A, B)
+    cross = (u_gene <= 0.5) & (np.abs(A - B) >= 1e-14)
+    span = np.where(cross, x2 - x1, 1.0)
+    e1 = eta + 1.0
+
+    def spread(beta):
+        alpha = 2.0 - beta ** -e1
+        inner = np.where(u_beta <= 1.0 / alpha, u_beta * alpha, 1.0 / (2.0 - u_beta * alpha))
+        return inner ** (1.0 / e1)
+    y1 = 0.5 * (x1 + x2 - spread(1.0 + 2.0 * (x1 - lo) / span) * span)
+    y2 = 0.5 * (x1 + x2 + spread(1.0 + 2.0 * (hi - x2) / span) * span)
+    y1, y2 = np.clip(y1, lo, hi), np.clip(y2, lo, hi)
+    return np.where(cross, np.where(u_swap <= 0.5, y2, y1), A)
+
+
+def mutate_rows(A, lo, hi, u_pick, u_r, eta=20.0, indpb=None):
+    """Bounded polynomial mutation of rows A [n, g] from given uniform numbers
+    (gene mutated when u_pick <= indpb, default 1/g): the arithmetic of
+    :func:`poly_mutation`, vectorised."""
+    indpb = 1.0 / A.shape[1] if indpb is None else indpb
+    d1, d2 = (A - lo) / (hi - lo), (hi - A) / (hi - lo)
+    mp = 1.0 / (eta + 1.0)
+    lower = u_r < 0.5
+    val = np.where(lower, 2.0 * u_r + (1.0 - 2.0 * u_r) * (1.0 - d1) ** (eta + 1),
+                   2.0 * (1.0 - u_r) + 2.0 * (u_r - 0.5) * (1.0 - d2) ** (eta + 1))
+    dq = np.where(lower, val ** mp - 1.0, 1.0 - val ** mp)
+    y = np.clip(A + dq * (hi - lo), lo, hi)
+    return np.where(u_pick <= indpb, y, A)
+
+
+def var_or_rows(rng, pop, lam, lo, hi, cxpb=0.7, mutpb=0.3, eta=20.0):
+    """DEAP's varOr (PYS/DEAP_helper.py:126-) for lam offspring at once: each
+    is a crossover child (probability cxpb), a mutant (mutpb) or a copy."""
+    n, g = pop.shape
+    r = rng.random(lam)
+    i = rng.integers(n, size=lam)
+    j = (i + 1 + rng.integers(n - 1, size=lam)) % n          # a second, distinct parent
+    u = rng.random((5, lam, g))
+    cx = sbx_rows(pop[i], pop[j], lo, hi, u[0], u[1], u[2], eta)
+    mu = mutate_rows(pop[i], lo, hi, u[3], u[4], eta)
+    return np.where((r < cxpb)[:, None], cx, np.where((r < cxpb + mutpb)[:, None], mu, pop[i]))
+
+
 def var_or(rng, pop, lam, lo, hi, cxpb=0.7, mutpb=0.3):
     """Offspring as DEAP's varOr (PYS/DEAP_helper.py:126-): crossover, mutation or reproduction."""
     out = np.empty((lam, pop.shape[1]))
@@ -167,25 +215,31 @@ class GAResult:
 
 def run_ga(evaluator: Evaluator, mu: int, lam: int, ngen: int, seed: int = 1178,
            screen: Optional[Callable] = None, oversample: int = 1, time_limit: Optional[float] = None,
-           stop_score: Optional[float] = None, log: Optional[Callable] = None) -> GAResult:
+           stop_score: Optional[float] = None, log: Optional[Callable] = None,
+           observe: Optional[Callable] = None) -> GAResult:
     """(mu + lambda) GA in log10 parameter space.  With ``screen`` the GA draws
     ``oversample * lam`` offspring, and ``screen(genomes)`` returns the indices
     of those to simulate (the surrogate pre-screen); every score that enters
-    selection comes from the simulator."""
+    selection comes from the simulator.  ``observe(genomes, features)`` sees
+    every simulated batch (the screened GA's surrogate learns from them)."""
     m = evaluator.m
     rng = np.random.default_rng(seed)
     lo = np.log10([fp.lo for fp in m.free])
     hi = np.log10([fp.hi for fp in m.free])
     pop = rng.uniform(lo, hi, size=(mu, lo.size))
     t0 = time.perf_counter()
-    obj, _ = evaluator.evaluate(10 ** pop)
+    obj, feat = evaluator.evaluate(10 ** pop)
+    if observe is not None:
+        observe(pop, feat)
     score = obj.sum(axis=1)
     hist = [(0, time.perf_counter() - t0, float(score.min()), evaluator.nevals)]
     for gen in range(1, ngen + 1):
-        kids = var_or(rng, pop, lam * oversample, lo, hi)
+        kids = var_or_rows(rng, pop, lam * oversample, lo, hi)
         if screen is not None:
             kids = kids[screen(kids)]
-        kobj, _ = evaluator.evaluate(10 ** kids)
+        kobj, kfeat = evaluator.evaluate(10 ** kids)
+        if observe is not None:
+            observe(kids, kfeat)
         allx = np.vstack([pop, kids])
         alls = np.concatenate([score, kobj.sum(axis=1)])
         keep = np.argsort(alls, kind="stable")[:mu]

@@ -1,14 +1,18 @@
 """Plain GA vs surrogate-screened GA on one GPU, both scored by simulation.
 
-    python -m neurogpu_amd.ga_study [--workload ttpc1200] [--pop 1024] [--gens 25] ...
+    python -m neurogpu_amd.ga_study [--workload ttpc1200] [--pop 1024] [--gens 20] ...
 
 The plain (mu + lambda) GA runs ``--gens`` generations; its best simulated
-score S* and wall time T are recorded.  The screened GA first simulates
-``--train`` log-uniform samples, trains the MLP ensemble on their features
-(both timed), then runs generations in which 10x oversampled offspring are
-ranked by the surrogate and only the top 10 % plus the most uncertain 2 % are
-simulated, until its best simulated score reaches S*.  Prints one JSON line
-with both histories, the time to reach S*, and evaluations per second.
+score S* and wall time T are recorded.  The screened GA starts from the same
+simulated random population, trains the MLP ensemble on it, then runs
+generations in which ``--oversample``-fold offspring are ranked by the
+surrogate and only the best ``--keep`` of a generation's worth plus the most
+uncertain ``--disagree`` (the disagreement gate) are simulated.  Every
+simulated batch is added to the training set and the ensemble continues
+training on it (``--refit-epochs`` per generation), so the surrogate learns
+where the search goes.  The run stops when its best simulated score reaches
+S*; all of its time (training included) counts.  Prints one JSON line with
+both histories, the time to reach S*, and evaluations per second.
 """
 from __future__ import annotations
 
@@ -22,7 +26,8 @@ from . import ga, models
 from .surrogate import MLPEnsemble, feature_targets, genome_to_input, make_screen
 
 
-def study(name="ttpc1200", pop=1024, gens=25, train=8192, nt=2000, members=5, epochs=60, seed=1178, device=0,
+def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, seed=1178, device=0,
+          oversample=10, keep=0.25, disagree=0.05, refit_epochs=10, explore=0.0, max_train=8192,
           max_screen_gens=None):
     w = models.workload(name, nsets=1, nt=nt)
     m = w.model
@@ -44,20 +49,33 @@ def study(name="ttpc1200", pop=1024, gens=25, train=8192, nt=2000, members=5, ep
     plain_evals = ev.nevals
 
     ev.nevals = 0
-    t0 = time.perf_counter()
-    rng = np.random.default_rng(seed + 1)
-    g = rng.uniform(lo, hi, size=(train, lo.size))
-    f = ev.features(10 ** g)
-    say(f"surrogate: {train} training simulations in {time.perf_counter() - t0:.1f} s")
     ens = MLPEnsemble(lo.size, nstim * len(ga.GA_FEATURES), hidden=256, members=members, seed=seed, device=device)
-    losses = ens.fit(genome_to_input(g, lo, hi), feature_targets(f), epochs=epochs, batch=512, lr=2e-3)
-    prep_t = time.perf_counter() - t0
-    say(f"surrogate: trained {members} members, final loss {losses[:, -1].mean():.4f}, prep {prep_t:.1f} s")
-    screen = make_screen(ens, ev.target, lo, hi, nstim, keep_frac=0.1, disagree_frac=0.02)
-    remaining = None if plain_t is None else max(1.0, 4 * plain_t - prep_t)
-    scr = ga.run_ga(ev, mu=pop, lam=pop, ngen=max_screen_gens or 10 * gens, seed=seed, screen=screen, oversample=10,
-                    stop_score=s_star, time_limit=remaining, log=say)
-    scr_hist = [(gen, t + prep_t, s, n) for gen, t, s, n in scr.history]
+    data_g, data_y = [], []
+    fit = {"s": 0.0, "n": 0, "loss": float("nan")}
+
+    def observe(genomes, feats):
+        """Every simulated batch joins the training set; the ensemble trains
+        from scratch on the first (the random population) and continues on
+        the newest ``max_train`` samples after each generation."""
+        t = time.perf_counter()
+        data_g.append(genome_to_input(genomes, lo, hi))
+        data_y.append(feature_targets(feats))
+        x, y = np.vstack(data_g)[-max_train:], np.vstack(data_y)[-max_train:]
+        if fit["n"] == 0:
+            losses = ens.fit(x, y, epochs=epochs, batch=512, lr=2e-3)
+        else:
+            losses = ens.refit(x, y, epochs=refit_epochs, batch=512, lr=1e-3)
+        fit["loss"] = float(losses[:, -1].mean())
+        fit["n"] += 1
+        fit["s"] += time.perf_counter() - t
+
+    screen = make_screen(ens, ev.target, lo, hi, nstim, keep_frac=keep / oversample,
+                         disagree_frac=disagree / oversample, explore=explore)
+    say(f"screened GA: oversample {oversample}, simulate the best {keep:.0%} + most uncertain {disagree:.0%} "
+        f"of a generation, surrogate retrained every generation")
+    scr = ga.run_ga(ev, mu=pop, lam=pop, ngen=max_screen_gens or 10 * gens, seed=seed, screen=screen,
+                    oversample=oversample, stop_score=s_star, time_limit=plain_t, log=say, observe=observe)
+    scr_hist = list(scr.history)
     reached = next((t for _, t, s, _ in scr_hist if s <= s_star), None)
     ens.close()
     ev.close()
@@ -65,10 +83,10 @@ def study(name="ttpc1200", pop=1024, gens=25, train=8192, nt=2000, members=5, ep
         "workload": f"{w.name} protocol {nt} steps x {nstim} stimuli, pop {pop}",
         "plain": {"best_score": s_star, "wall_s": plain_t, "evals": plain_evals,
                   "evals_per_s": plain_evals / plain_t, "history": plain.history},
-        "screened": {"best_score": scr.best_score, "prep_s": prep_t, "train_samples": train,
-                     "final_loss": float(losses[:, -1].mean()), "evals": ev.nevals,
-                     "history": scr_hist, "wall_s_to_reach_plain_best": reached,
-                     "evals_per_s": ev.nevals / scr_hist[-1][1]},
+        "screened": {"best_score": scr.best_score, "evals": ev.nevals, "surrogate_fit_s": fit["s"],
+                     "surrogate_fits": fit["n"], "final_loss": fit["loss"], "oversample": oversample,
+                     "keep": keep, "disagree": disagree, "explore": explore, "history": scr_hist,
+                     "wall_s_to_reach_plain_best": reached, "evals_per_s": ev.nevals / scr_hist[-1][1]},
         "time_ratio": (reached / plain_t) if reached else None,
     }
 
@@ -77,13 +95,21 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--workload", default="ttpc1200", choices=models.WORKLOADS)
     ap.add_argument("--pop", type=int, default=1024)
-    ap.add_argument("--gens", type=int, default=25)
-    ap.add_argument("--train", type=int, default=8192)
+    ap.add_argument("--gens", type=int, default=20)
     ap.add_argument("--nt", type=int, default=2000)
     ap.add_argument("--members", type=int, default=5)
     ap.add_argument("--epochs", type=int, default=60)
+    ap.add_argument("--oversample", type=int, default=10)
+    ap.add_argument("--keep", type=float, default=0.25)
+    ap.add_argument("--disagree", type=float, default=0.05)
+    ap.add_argument("--refit-epochs", type=int, default=10)
+    ap.add_argument("--seed", type=int, default=1178)
+    ap.add_argument("--explore", type=float, default=0.0)
+    ap.add_argument("--max-train", type=int, default=8192)
     args = ap.parse_args()
-    r = study(args.workload, args.pop, args.gens, args.train, args.nt, args.members, args.epochs)
+    r = study(args.workload, args.pop, args.gens, args.nt, args.members, args.epochs, seed=args.seed,
+              oversample=args.oversample, keep=args.keep, disagree=args.disagree, refit_epochs=args.refit_epochs,
+              explore=args.explore, max_train=args.max_train)
     print(json.dumps(r), flush=True)
 
 

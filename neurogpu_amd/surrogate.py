@@ -158,6 +158,13 @@ class MLPEnsemble:
         self.ysd = y.std(axis=0) + 1e-9
         return self.train_raw(x, (y - self.ymu) / self.ysd, **kw)
 
+    def refit(self, x: np.ndarray, y: np.ndarray, **kw):
+        """Continue training from the current weights with the scaling of the
+        first fit (the screened GA's surrogate learning from new simulations)."""
+        if self.ymu is None:
+            return self.fit(x, y, **kw)
+        return self.train_raw(x, (y - self.ymu) / self.ysd, **kw)
+
     def predict(self, x: np.ndarray):
         """Ensemble mean and standard deviation of the de-standardised outputs."""
         p = self.predict_raw(x).astype(np.float64) * self.ysd + self.ymu
@@ -197,15 +204,17 @@ def predicted_scores(ens: MLPEnsemble, x_in: np.ndarray, target: Target, nstim: 
 
 
 def make_screen(ens: MLPEnsemble, target: Target, lo, hi, nstim: int, keep_frac: float = 0.1,
-                disagree_frac: float = 0.02):
-    """Screen for run_ga: keep the best ``keep_frac`` by predicted score plus the
-    ``disagree_frac`` most uncertain candidates (the disagreement gate: where the
-    ensemble does not agree the candidate goes to full simulation)."""
+                disagree_frac: float = 0.02, explore: float = 0.0):
+    """Screen for run_ga: keep the best ``keep_frac`` by predicted score minus
+    ``explore`` times the ensemble disagreement (an optimistic bound; 0 ranks by
+    the ensemble mean) plus the ``disagree_frac`` most uncertain candidates
+    (the disagreement gate: where the ensemble does not agree the candidate
+    goes to full simulation)."""
     def screen(genomes: np.ndarray) -> np.ndarray:
         s, d = predicted_scores(ens, genome_to_input(genomes, lo, hi), target, nstim)
         n = genomes.shape[0]
         k = max(1, int(round(keep_frac * n)))
-        best = np.argsort(s, kind="stable")[:k]
+        best = np.argsort(s - explore * d, kind="stable")[:k]
         kd = int(round(disagree_frac * n))
         unsure = np.argsort(-d, kind="stable")[:kd] if kd > 0 else np.array([], dtype=int)
         return np.unique(np.concatenate([best, unsure]))

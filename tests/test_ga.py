@@ -26,6 +26,55 @@ def test_sbx_preserves_the_mean_of_each_gene():
         np.testing.assert_allclose(c1 + c2, a + b, atol=1e-9)
 
 
+def _scalar_sbx(a, b, lo, hi, ug, ub, us, eta=20.0):
+    """ga.sbx's per-gene arithmetic with given uniform numbers, first child."""
+    c1 = a.copy()
+    for i in range(a.size):
+        if ug[i] > 0.5 or abs(a[i] - b[i]) < 1e-14:
+            continue
+        x1, x2 = min(a[i], b[i]), max(a[i], b[i])
+        r = ub[i]
+        beta = 1.0 + 2.0 * (x1 - lo[i]) / (x2 - x1)
+        alpha = 2.0 - beta ** -(eta + 1)
+        bq = (r * alpha) ** (1 / (eta + 1)) if r <= 1 / alpha else (1 / (2 - r * alpha)) ** (1 / (eta + 1))
+        y1 = 0.5 * (x1 + x2 - bq * (x2 - x1))
+        beta = 1.0 + 2.0 * (hi[i] - x2) / (x2 - x1)
+        alpha = 2.0 - beta ** -(eta + 1)
+        bq = (r * alpha) ** (1 / (eta + 1)) if r <= 1 / alpha else (1 / (2 - r * alpha)) ** (1 / (eta + 1))
+        y2 = 0.5 * (x1 + x2 + bq * (x2 - x1))
+        y1, y2 = min(max(y1, lo[i]), hi[i]), min(max(y2, lo[i]), hi[i])
+        c1[i] = y2 if us[i] <= 0.5 else y1
+    return c1
+
+
+def test_vectorised_variation_equals_the_per_gene_operators():
+    """sbx_rows / mutate_rows compute what sbx / poly_mutation compute, gene by
+    gene, from the same uniform numbers."""
+    rng = np.random.default_rng(2)
+    lo, hi = np.full(12, -2.0), np.full(12, 1.5)
+    A, B = rng.uniform(lo, hi, (300, 12)), rng.uniform(lo, hi, (300, 12))
+    B[::7, 3] = A[::7, 3]                                   # equal genes are not crossed
+    u = rng.random((5, 300, 12))
+    cx = ga.sbx_rows(A, B, lo, hi, u[0], u[1], u[2])
+    mu = ga.mutate_rows(A, lo, hi, u[3], u[4], indpb=0.5)
+    for k in range(300):
+        np.testing.assert_allclose(cx[k], _scalar_sbx(A[k], B[k], lo, hi, u[0, k], u[1, k], u[2, k]), rtol=1e-12)
+        y = A[k].copy()
+        for i in range(12):
+            if u[3, k, i] > 0.5:
+                continue
+            d1, d2 = (y[i] - lo[i]) / (hi[i] - lo[i]), (hi[i] - y[i]) / (hi[i] - lo[i])
+            r = u[4, k, i]
+            if r < 0.5:
+                dq = (2.0 * r + (1.0 - 2.0 * r) * (1.0 - d1) ** 21) ** (1 / 21) - 1.0
+            else:
+                dq = 1.0 - (2.0 * (1.0 - r) + 2.0 * (r - 0.5) * (1.0 - d2) ** 21) ** (1 / 21)
+            y[i] = min(max(y[i] + dq * (hi[i] - lo[i]), lo[i]), hi[i])
+        np.testing.assert_allclose(mu[k], y, rtol=1e-12)
+    kids = ga.var_or_rows(rng, A, 5000, lo, hi)
+    assert kids.shape == (5000, 12) and np.all(kids >= lo) and np.all(kids <= hi)
+
+
 def test_fill_and_objectives():
     f = np.array([[[2.0, 5.0, 90.0, 1.0, 20.0], [0.0, np.nan, np.nan, np.nan, np.nan]]])
     filled = ga.fill_features(f, 100.0)
