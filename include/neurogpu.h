@@ -80,9 +80,10 @@ int ngpu_set_stimuli(ngpu_ctx *ctx, const double *amps, int nstim, int nt_amps,
                      const double *dts, int nt, int stim_seg, double area);
 
 /* Launch shape (the autotuner's knobs; 0 = choose automatically): a group of
- * waves_per_run waves (1, 2, 4 or 8) advances runs_per_group runs (1..8)
+ * waves_per_run waves (1, 2, 4, 8 or 16) advances runs_per_group runs (1..8)
  * whose positions and solve tasks share the group's lanes; a workgroup holds
- * as many groups as fit in LDS and 8 waves.  steps_per_launch (0 = all steps
+ * as many groups as fit in LDS and 8 waves (one group of 16 waves: a
+ * 1024-thread workgroup, four waves per SIMD).  steps_per_launch (0 = all steps
  * in one launch), use_graph (capture the launch sequence into a hipGraph and
  * replay it). No reference counterpart: the reference fixes the launch at
  * NSets x (32, NStim) (CANON/CudaStuff.cu:487-507). */

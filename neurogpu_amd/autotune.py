@@ -26,7 +26,7 @@ from . import _lib, models
 ARCH = "gfx950"
 BATCH_NSLOT = 6                 # positions per lane of the batch kernel
 BATCH_WAVES = 8                 # waves per workgroup of the batch kernel
-KERNEL_REV = 2                  # bump when the kernel changes: cached choices are per revision
+KERNEL_REV = 3                  # bump when the kernel changes: cached choices are per revision
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "neurogpu_amd", "batch_tuning.json")
 
 

@@ -15,7 +15,11 @@
  *     is the reference's cnexp y + (1 - exp(dt*(-1/tau))) * ((-(yinf/tau))/(-1/tau) - y)
  *     (AllModels.cu:302) with the two quotients that cancel taken out; for
  *     alpha/beta gates rate = (alpha + beta) * q10 needs no division at all;
- *   - divisions by literal constants are products with the reciprocal;
+ *   - divisions by literal constants are products with the reciprocal, and
+ *     the other quotients are products with the hardware reciprocal refined
+ *     by two Newton steps (ng_rcp, ng_div: within about one ulp);
+ *   - exp and log (ng_expc, ng_logc) drop the special-case branches and
+ *     evaluate their polynomials by Estrin's scheme (short dependent chains);
  *   - exp(-x) next to exp(x) of the same argument is the reciprocal of exp(x);
  *     for the x/(1 - exp(-x/k)) pairs of the Na gates,
  *       beta/alpha = (b/a) exp(-x/k),  x_inf = 1 / (1 + (b/a) exp(-x/k));
@@ -78,30 +82,38 @@ NG_DEV void ng_dispatch(unsigned m, F &&f, NgMasks<M0, MS...>)
 #define NG_DEF_CAO 2.
 #define NG_FARADAY 96485.309
 
-/* exp(x) for the rate functions: ng_exp's reduction and polynomial, with the
- * argument clamped to [-708, 708] instead of branching on the range (the
- * result stays a normal number; physiological arguments are far inside). */
+/* exp(x) for the rate functions: ng_exp's Cody-Waite reduction x = k ln2 + r
+ * and its degree-13 Taylor polynomial, with the argument clamped to
+ * [-708, 708] instead of branching on the range (the result stays a normal
+ * number; physiological arguments are far inside).  The polynomial is
+ *   1 + r (1 + r Q(r)),   Q(r) = sum_{n=2..13} r^(n-2) / n!,
+ * with Q by Estrin's scheme (coefficient pairs, then r^2, r^4, r^8), so the
+ * dependent chain is six multiply-adds instead of thirteen; the two leading
+ * terms are added last as in Horner's rule, so the rounding stays that of
+ * ng_exp's final steps (checked against ng_exp in the host tests). */
 NG_DEV double ng_expc(double x)
 {
     x = fmin(fmax(x, -708.0), 708.0);
-    double kf = rint(x * NG_LOG2E);
-    double r = (x - kf * NG_LN2_HI) - kf * NG_LN2_LO;
-    double p = 1.6059043836821614599e-10;
-    p = fma(p, r, 2.0876756987868098979e-09);
-    p = fma(p, r, 2.5052108385441718775e-08);
-    p = fma(p, r, 2.7557319223985890653e-07);
-    p = fma(p, r, 2.7557319223985890653e-06);
-    p = fma(p, r, 2.4801587301587301566e-05);
-    p = fma(p, r, 1.9841269841269841253e-04);
-    p = fma(p, r, 1.3888888888888888889e-03);
-    p = fma(p, r, 8.3333333333333333333e-03);
-    p = fma(p, r, 4.1666666666666666667e-02);
-    p = fma(p, r, 1.6666666666666666667e-01);
-    p = fma(p, r, 0.5);
-    p = fma(p, r, 1.0);
-    p = fma(p, r, 1.0);
-    return ldexp(p, (int)kf);
+    const double kf = rint(x * NG_LOG2E);
+    const double r = fma(-kf, NG_LN2_LO, fma(-kf, NG_LN2_HI, x));
+    const double r2 = r * r, r4 = r2 * r2, r8 = r4 * r4;
+    const double q2 = fma(1.6666666666666666667e-01, r, 0.5);                      /* 1/2! + r/3!   */
+    const double q4 = fma(8.3333333333333333333e-03, r, 4.1666666666666666667e-02); /* 1/4! + r/5!   */
+    const double q6 = fma(1.9841269841269841253e-04, r, 1.3888888888888888889e-03); /* 1/6! + r/7!   */
+    const double q8 = fma(2.7557319223985890653e-06, r, 2.4801587301587301566e-05); /* 1/8! + r/9!   */
+    const double q10 = fma(2.5052108385441718775e-08, r, 2.7557319223985890653e-07); /* 1/10! + r/11! */
+    const double q12 = fma(1.6059043836821614599e-10, r, 2.0876756987868098979e-09); /* 1/12! + r/13! */
+    const double s2 = fma(q4, r2, q2), s6 = fma(q8, r2, q6), s10 = fma(q12, r2, q10);
+    const double Q = fma(s10, r8, fma(s6, r4, s2));
+    return ldexp(fma(fma(Q, r, 1.0), r, 1.0), (int)kf);
 }
+
+/* log(x) for positive normal finite x (concentration ratios): ng_log's
+ * reduction x = 2^k m, m in [sqrt(1/2), sqrt(2)), f = m - 1, s = f / (2 + f)
+ * and its atanh series R(z), z = s^2, to the same degree, without the
+ * special-case branches, with s from the refined reciprocal and R by
+ * Estrin's scheme. */
+NG_DEV double ng_logc(double x);
 
 /* 1/y: the hardware estimate refined by two Newton steps (normal, finite y). */
 NG_DEV double ng_rcp(double y)
@@ -115,6 +127,31 @@ NG_DEV double ng_rcp(double y)
 #else
     return 1.0 / y;
 #endif
+}
+
+/* a / b as a times the refined reciprocal of b (within about one ulp of the
+ * quotient; normal, finite b). */
+NG_DEV double ng_div(double a, double b) { return a * ng_rcp(b); }
+
+NG_DEV double ng_logc(double x)
+{
+    int k;
+    double m = frexp(x, &k);
+    const bool lo = m < NG_SQRT_HALF;
+    m = lo ? m * 2.0 : m;
+    const double kd = (double)(lo ? k - 1 : k);
+    const double f = m - 1.0;
+    const double s = ng_div(f, 2.0 + f);
+    const double z = s * s, z2 = z * z, z4 = z2 * z2, z8 = z4 * z4;
+    const double a1 = fma(0.4, z, 0.6666666666666666666667);                          /* 2/3 + z 2/5   */
+    const double a3 = fma(0.2222222222222222222222, z, 0.2857142857142857142857);     /* 2/7 + z 2/9   */
+    const double a5 = fma(0.1538461538461538461538, z, 0.1818181818181818181818);     /* 2/11 + z 2/13 */
+    const double a7 = fma(0.1176470588235294117647, z, 0.1333333333333333333333);     /* 2/15 + z 2/17 */
+    const double a9 = fma(0.0952380952380952380952, z, 0.1052631578947368421053);     /* 2/19 + z 2/21 */
+    const double a11 = 0.0869565217391304347826;                                       /* 2/23          */
+    const double R = z * fma(fma(a11, z2, a9), z8, fma(fma(a7, z2, a5), z4, fma(a3, z2, a1)));
+    const double hfsq = 0.5 * f * f;
+    return kd * NG_LN2_HI + (f - (hfsq - (s * (hfsq + R) + kd * NG_LN2_LO)));
 }
 
 /* one cnexp step of a gate from its steady state and rate 1/tau */
@@ -152,90 +189,93 @@ namespace ttpc {
 #define TT_ENA 50.0
 #define TT_EK (-85.0)
 #define TT_QT 2.952882641412121                  /* 2.3^((34-21)/10) */
-NG_DEV double eca_of(double cai) { return TT_KTF / 2 * ng_log(NG_DEF_CAO / cai); }
+NG_DEV double eca_of(double cai) { return TT_KTF / 2 * ng_logc(ng_div(NG_DEF_CAO, cai)); }
 
 /* AllModels.cu:66-78 */
 NG_DEV void ca_hva(double v, double &mi, double &mr, double &hi, double &hr)
 {
     if (v == -27.0) v = v + 0.0001;
     const double x = -27.0 - v;
-    const double mA = (0.055 * x) / (ng_expc(x * (1.0 / 3.8)) - 1.0);
+    const double mA = ng_div(0.055 * x, ng_expc(x * (1.0 / 3.8)) - 1.0);
     const double mB = 0.94 * ng_expc((-75.0 - v) * (1.0 / 17.0));
-    mr = mA + mB; mi = mA / mr;
+    mr = mA + mB; mi = ng_div(mA, mr);
     const double hA = 0.000457 * ng_expc((-13.0 - v) * (1.0 / 50.0));
-    const double hB = 0.0065 / (ng_expc((-v - 15.0) * (1.0 / 28.0)) + 1.0);
-    hr = hA + hB; hi = hA / hr;
+    const double hB = 0.0065 * ng_rcp(ng_expc((-v - 15.0) * (1.0 / 28.0)) + 1.0);
+    hr = hA + hB; hi = ng_div(hA, hr);
 }
-/* AllModels.cu:79-88 */
+/* AllModels.cu:79-88; with E = e^{(v+25)/5}, q / (5 + 20 / (1 + E)) = q (1 + E) / (25 + 5 E),
+ * and likewise for h */
 NG_DEV void ca_lvast(double v, double &mi, double &mr, double &hi, double &hr)
 {
     v = v + 10.0;
-    mi = 1.0 / (1.0 + ng_expc((v + 30.0) * (-1.0 / 6.0)));
-    mr = TT_QT / (5.0 + 20.0 / (1.0 + ng_expc((v + 25.0) * (1.0 / 5.0))));
-    hi = 1.0 / (1.0 + ng_expc((v + 80.0) * (1.0 / 6.4)));
-    hr = TT_QT / (20.0 + 50.0 / (1.0 + ng_expc((v + 40.0) * (1.0 / 7.0))));
+    mi = ng_rcp(1.0 + ng_expc((v + 30.0) * (-1.0 / 6.0)));
+    const double Em = ng_expc((v + 25.0) * (1.0 / 5.0));
+    mr = ng_div(TT_QT * (1.0 + Em), 25.0 + 5.0 * Em);
+    hi = ng_rcp(1.0 + ng_expc((v + 80.0) * (1.0 / 6.4)));
+    const double Eh = ng_expc((v + 40.0) * (1.0 / 7.0));
+    hr = ng_div(TT_QT * (1.0 + Eh), 70.0 + 20.0 * Eh);
 }
 /* AllModels.cu:89-97 */
 NG_DEV void ih(double v, double &mi, double &mr)
 {
     if (v == -154.9) v = v + 0.0001;
     const double x = v + 154.9;
-    const double mA = (0.001 * 6.43) * x / (ng_expc(x * (1.0 / 11.9)) - 1.0);
+    const double mA = ng_div((0.001 * 6.43) * x, ng_expc(x * (1.0 / 11.9)) - 1.0);
     const double mB = (0.001 * 193.0) * ng_expc(v * (1.0 / 33.1));
-    mr = mA + mB; mi = mA / mr;
+    mr = mA + mB; mi = ng_div(mA, mr);
 }
 /* AllModels.cu:98-105: mA = k e^{x}, mB = k e^{-x} */
 NG_DEV void im(double v, double &mi, double &mr)
 {
     const double E = ng_expc((2.5 * 0.04) * (v + 35.0));
     const double Ei = ng_rcp(E);
-    mi = E / (E + Ei);
+    mi = ng_div(E, E + Ei);
     mr = (3.3e-3 * E + 3.3e-3 * Ei) * TT_QT;
 }
 /* AllModels.cu:106-120 */
 NG_DEV void k_pst(double v, double &mi, double &mr, double &hi, double &hr)
 {
     v = v + 10.0;
-    mi = 1.0 / (1.0 + ng_expc(-(v + 1.0) * (1.0 / 12.0)));
+    mi = ng_rcp(1.0 + ng_expc(-(v + 1.0) * (1.0 / 12.0)));
     const bool lo = v < -50.0;
-    mr = TT_QT / (1.25 + (lo ? 175.03 : 13.0) * ng_expc(v * (lo ? 0.026 : -0.026)));
-    hi = 1.0 / (1.0 + ng_expc((v + 54.0) * (1.0 / 11.0)));
+    mr = TT_QT * ng_rcp(1.25 + (lo ? 175.03 : 13.0) * ng_expc(v * (lo ? 0.026 : -0.026)));
+    hi = ng_rcp(1.0 + ng_expc((v + 54.0) * (1.0 / 11.0)));
     const double z = (v + 75.0) * (1.0 / 48.0);
-    hr = TT_QT / (360.0 + (1010.0 + 24.0 * (v + 55.0)) * ng_expc(-(z * z)));
+    hr = TT_QT * ng_rcp(360.0 + (1010.0 + 24.0 * (v + 55.0)) * ng_expc(-(z * z)));
 }
 /* AllModels.cu:121-130 */
 NG_DEV void k_tst(double v, double &mi, double &mr, double &hi, double &hr)
 {
     v = v + 10.0;
-    mi = 1.0 / (1.0 + ng_expc(v * (-1.0 / 19.0)));
+    mi = ng_rcp(1.0 + ng_expc(v * (-1.0 / 19.0)));
     const double zm = (v + 71.0) * (1.0 / 59.0);
-    mr = TT_QT / (0.34 + 0.92 * ng_expc(-(zm * zm)));
-    hi = 1.0 / (1.0 + ng_expc((v + 66.0) * (1.0 / 10.0)));
+    mr = TT_QT * ng_rcp(0.34 + 0.92 * ng_expc(-(zm * zm)));
+    hi = ng_rcp(1.0 + ng_expc((v + 66.0) * (1.0 / 10.0)));
     const double zh = (v + 73.0) * (1.0 / 23.0);
-    hr = TT_QT / (8.0 + 49.0 * ng_expc(-(zh * zh)));
+    hr = TT_QT * ng_rcp(8.0 + 49.0 * ng_expc(-(zh * zh)));
 }
 /* The Na activation pair alpha = a x / (1 - e^{-x/k}), beta = b (-x) / (1 - e^{x/k}):
  * with E = e^{-x/k} and R = x / (1 - E), alpha = a R and beta = b R E. */
 NG_DEV void na_pair(double x, double a, double b, double &alpha, double &beta, double &E)
 {
     E = ng_expc(x * (-1.0 / 6.0));
-    const double R = x / (1.0 - E);
+    const double R = ng_div(x, 1.0 - E);
     alpha = a * R;
     beta = b * R * E;
 }
 /* AllModels.cu:131-151 */
 NG_DEV void nap_et2(double v, double &mi, double &mr, double &hi, double &hr)
 {
-    mi = 1.0 / (1.0 + ng_expc((v + 52.6) * (-1.0 / 4.6)));
+    mi = ng_rcp(1.0 + ng_expc((v + 52.6) * (-1.0 / 4.6)));
     if (v == -38.0) v = v + 0.0001;
     double mA, mB, E;
     na_pair(v + 38.0, 0.182, 0.124, mA, mB, E);
     mr = (mA + mB) * (TT_QT / 6.0);
     if (v == -17.0) v = v + 0.0001;
     if (v == -64.4) v = v + 0.0001;
-    hi = 1.0 / (1.0 + ng_expc((v + 48.8) * (1.0 / 10.0)));
-    const double hA = -2.88e-6 * (v + 17.0) / (1.0 - ng_expc((v + 17.0) * (1.0 / 4.63)));
-    const double hB = 6.94e-6 * (v + 64.4) / (1.0 - ng_expc(-(v + 64.4) * (1.0 / 2.63)));
+    hi = ng_rcp(1.0 + ng_expc((v + 48.8) * (1.0 / 10.0)));
+    const double hA = ng_div(-2.88e-6 * (v + 17.0), 1.0 - ng_expc((v + 17.0) * (1.0 / 4.63)));
+    const double hB = ng_div(6.94e-6 * (v + 64.4), 1.0 - ng_expc(-(v + 64.4) * (1.0 / 2.63)));
     hr = (hA + hB) * TT_QT;
 }
 /* The Na inactivation pair alpha = -c y / (1 - e^{y/6}), beta = -c (-y) / (1 - e^{-y/6}):
@@ -244,8 +284,8 @@ NG_DEV void nap_et2(double v, double &mi, double &mr, double &hi, double &hr)
 NG_DEV void na_inact(double y, double &hi, double &hr)
 {
     const double F = ng_expc(y * (1.0 / 6.0));
-    const double R = y / (1.0 - F);
-    hi = 1.0 / (1.0 + F);
+    const double R = ng_div(y, 1.0 - F);
+    hi = ng_rcp(1.0 + F);
     hr = (-0.015 * R) * (1.0 + F) * TT_QT;
 }
 /* AllModels.cu:152-169 */
@@ -255,7 +295,7 @@ NG_DEV void nata_t(double v, double &mi, double &mr, double &hi, double &hr)
     double mA, mB, E;
     na_pair(v + 38.0, 0.182, 0.124, mA, mB, E);
     mr = (mA + mB) * TT_QT;
-    mi = 0.182 / (0.182 + 0.124 * E);
+    mi = 0.182 * ng_rcp(0.182 + 0.124 * E);
     if (v == -66.0) v = v + 0.0001;
     na_inact(v + 66.0, hi, hr);
 }
@@ -265,7 +305,7 @@ NG_DEV void nats2_t(double v, double &mi, double &mr, double &hi, double &hr)
     if (v == -32.0) v = v + 0.0001;
     double mA, mB, E;
     na_pair(v + 32.0, 0.182, 0.124, mA, mB, E);
-    mi = 0.182 / (0.182 + 0.124 * E);
+    mi = 0.182 * ng_rcp(0.182 + 0.124 * E);
     mr = (mA + mB) * TT_QT;
     if (v == -60.0) v = v + 0.0001;
     na_inact(v + 60.0, hi, hr);
@@ -274,12 +314,12 @@ NG_DEV void nats2_t(double v, double &mi, double &mr, double &hi, double &hr)
 NG_DEV double sk_e2(double ca)
 {
     if (ca < 1e-7) ca = ca + 1e-07;
-    return 1.0 / (1.0 + ng_expc(4.8 * ng_log(0.00043 / ca)));
+    return ng_rcp(1.0 + ng_expc(4.8 * ng_logc(ng_div(0.00043, ca))));
 }
 /* AllModels.cu:194-197: tau = 4 / (1 + e^{...}) */
 NG_DEV void skv3_1(double v, double &mi, double &mr)
 {
-    mi = 1.0 / (1.0 + ng_expc((v - 18.7) * (-1.0 / 9.7)));
+    mi = ng_rcp(1.0 + ng_expc((v - 18.7) * (-1.0 / 9.7)));
     mr = (1.0 + ng_expc((v + 46.56) * (-1.0 / 44.14))) * 0.25;
 }
 } // namespace ttpc
@@ -287,11 +327,13 @@ NG_DEV void skv3_1(double v, double &mi, double &mr)
 struct SetTTPC {
     static constexpr int NMECH = 13, NPARAM = 19, NSTATE = 21;
     static constexpr int CAI = 18, ECA = 19, ICA = 20;
-    /* the BBP cADpyr placement (Figures/Figure2_BBP_TTPC): basal {Ih, pas},
-     * apical {Ih, Im, NaTs2_t, pas, SKv3_1}, soma, axon; 0 = zero-area node */
+    /* the BBP cADpyr placement (Figures/Figure2_BBP_TTPC): basal {Ih, pas} and
+     * apical {Ih, Im, NaTs2_t, pas, SKv3_1} hold nearly every segment; the few
+     * soma and axon segments take the run-time mask code, whose register
+     * demand is far below a specialisation with all thirteen mechanisms */
     template <class F> NG_DEV static void dispatch(unsigned m, F &&f)
     {
-        ng_dispatch(m, f, NgMasks<0x408u, 0x1618u, 0x0u, 0x1e0fu, 0x1de7u>{});
+        ng_dispatch(m, f, NgMasks<0x408u, 0x1618u>{});
     }
     /* CaDynamics_E2 reads ica */
     NG_DEV static unsigned ica_bits(unsigned m) { return (m & (1u << 2)) ? 1u << ICA : 0u; }
@@ -344,8 +386,8 @@ struct SetTTPC {
         if (m & (1u << 2)) {
             /* cai += (1 - e^{-dt/decay}) (decay A + minCai - cai), A = -1e4 ica gamma / (2 F depth) */
             const double gamma = P(2), decay = P(3), depth = P(4), minCai = P(5);
-            const double A = (-(10000.0)) * ((ica * gamma) / ((2.0 * NG_FARADAY) * depth));
-            st[CAI] = st[CAI] + (1. - ng_expc(-(dt / decay))) * ((A * decay + minCai) - st[CAI]);
+            const double A = (-(10000.0)) * ng_div(ica * gamma, (2.0 * NG_FARADAY) * depth);
+            st[CAI] = st[CAI] + (1. - ng_expc(-(dt * ng_rcp(decay)))) * ((A * decay + minCai) - st[CAI]);
         }
         if (m & (1u << 3)) { ih(v, a, b); st[4] = ng_gate(st[4], a, b, dt); }
         if (m & (1u << 4)) { im(v, a, b); st[5] = ng_gate(st[5], a, b, dt); }
@@ -397,36 +439,36 @@ namespace mainen {
 #define MN_TAUR 200.0
 #define MN_ENA 60.0
 #define MN_EK (-90.0)
-NG_DEV double eca_of(double cai) { return MN_KTF / 2 * ng_log(NG_DEF_CAO / cai); }
+NG_DEV double eca_of(double cai) { return MN_KTF / 2 * ng_logc(ng_div(NG_DEF_CAO, cai)); }
 NG_DEV double efun(double z)
 {
     if (fabs(z) < 1e-4) return 1 - z * 0.5;
-    return z / (ng_expc(z) - 1);
+    return ng_div(z, ng_expc(z) - 1);
 }
 NG_DEV void ca(double vm, double &mi, double &mr, double &hi, double &hr)
 {
     double a = 0.209 * efun(-(27.0 + vm) * (1.0 / 3.8));
     double b = 0.94 * ng_expc((-75.0 - vm) * (1.0 / 17.0));
-    mr = (a + b) * MN_TADJ; mi = a / (a + b);
+    mr = (a + b) * MN_TADJ; mi = ng_div(a, a + b);
     a = 0.000457 * ng_expc((-13.0 - vm) * (1.0 / 50.0));
-    b = 0.0065 / (ng_expc((-vm - 15.0) * (1.0 / 28.0)) + 1.0);
-    hr = (a + b) * MN_TADJ; hi = a / (a + b);
+    b = 0.0065 * ng_rcp(ng_expc((-vm - 15.0) * (1.0 / 28.0)) + 1.0);
+    hr = (a + b) * MN_TADJ; hi = ng_div(a, a + b);
 }
 NG_DEV void kca(double cai, double Ra, double Rb, double caix, double &ni, double &nr)
 {
     const double a = Ra * ng_pow(cai, caix), b = Rb;
-    nr = (a + b) * MN_TADJ; ni = a / (a + b);
+    nr = (a + b) * MN_TADJ; ni = ng_div(a, a + b);
 }
 /* alpha = Ra qa efun(-z), beta = Rb qa efun(z), z = (v - tha)/qa; away from
  * z = 0, efun(-z) = e^{z} efun(z) */
 NG_DEV void kx(double v, double tha, double qa, double Ra, double Rb, double &ni, double &nr)
 {
-    const double z = (v - tha) / qa;
+    const double z = ng_div(v - tha, qa);
     double ep, em;
     if (fabs(z) < 1e-4) { ep = 1 - z * 0.5; em = 1 + z * 0.5; }
-    else { const double E = ng_expc(z); ep = z / (E - 1); em = E * ep; }
+    else { const double E = ng_expc(z); ep = ng_div(z, E - 1); em = E * ep; }
     const double a = Ra * qa * em, b = Rb * qa * ep;
-    nr = (a + b) * MN_TADJ; ni = a / (a + b);
+    nr = (a + b) * MN_TADJ; ni = ng_div(a, a + b);
 }
 /* trap0(v, th, a, q) = a (v - th) / (1 - e^{-(v - th)/q}), or a q near v = th;
  * the m pair trap0(vm, th, Ra, q), trap0(-vm, -th, Rb, q) shares the exp:
@@ -435,30 +477,31 @@ template <class PF>
 NG_DEV void na(double vm, const PF &P, double &mi, double &mr, double &hi, double &hr)
 {
     const double tha = P(17), qa = P(18);
-    const double x = vm - tha;
+    const double x = vm - tha, z = ng_div(x, qa);
     double a, b;
-    if (fabs(x / qa) > 1e-6) {
-        const double E = ng_expc(-x / qa);
-        const double R = x / (1 - E);
+    if (fabs(z) > 1e-6) {
+        const double E = ng_expc(-z);
+        const double R = ng_div(x, 1 - E);
         a = P(19) * R; b = P(20) * R * E;
     } else { a = P(19) * qa; b = P(20) * qa; }
-    mr = (a + b) * MN_TADJ; mi = a / (a + b);
-    const double qi = P(23);
-    const double x1 = vm - P(21), x2 = -vm + P(22);
-    a = (fabs(x1 / qi) > 1e-6) ? P(27) * x1 / (1 - ng_expc(-x1 / qi)) : P(27) * qi;
-    b = (fabs(x2 / qi) > 1e-6) ? P(26) * x2 / (1 - ng_expc(-x2 / qi)) : P(26) * qi;
+    mr = (a + b) * MN_TADJ; mi = ng_div(a, a + b);
+    const double qi = P(23), rqi = ng_rcp(qi);
+    const double x1 = vm - P(21), x2 = -vm + P(22), z1 = x1 * rqi, z2 = x2 * rqi;
+    a = (fabs(z1) > 1e-6) ? ng_div(P(27) * x1, 1 - ng_expc(-z1)) : P(27) * qi;
+    b = (fabs(z2) > 1e-6) ? ng_div(P(26) * x2, 1 - ng_expc(-z2)) : P(26) * qi;
     hr = (a + b) * MN_TADJ;
-    hi = 1.0 / (1.0 + ng_expc((vm - P(24)) / P(25)));
+    hi = ng_rcp(1.0 + ng_expc(ng_div(vm - P(24), P(25))));
 }
 } // namespace mainen
 
 struct SetMainen {
     static constexpr int NMECH = 7, NPARAM = 30, NSTATE = 11;
     static constexpr int CAI = 8, ECA = 9, ICA = 10;
-    /* dendrites {ca, cad, kca, km, na, pas}, soma (all), axon {kv, na, pas} */
+    /* dendrites {ca, cad, kca, km, na, pas} hold nearly every segment; soma
+     * (all) and axon {kv, na, pas} take the run-time mask code */
     template <class F> NG_DEV static void dispatch(unsigned m, F &&f)
     {
-        ng_dispatch(m, f, NgMasks<0x6fu, 0x70u, 0x7fu>{});
+        ng_dispatch(m, f, NgMasks<0x6fu>{});
     }
     /* cad reads ica */
     NG_DEV static unsigned ica_bits(unsigned m) { return (m & 2u) ? 1u << ICA : 0u; }

@@ -445,7 +445,8 @@ extern "C" int ngpu_set_stimuli(ngpu_ctx *c, const double *amps, int nstim, int 
 extern "C" int ngpu_set_launch(ngpu_ctx *c, int w, int r, int spl, int graph)
 {
     if (!c) return fail(c, NGPU_E_ARG, "NULL context");
-    if (!(w == 0 || w == 1 || w == 2 || w == 4 || w == 8) || r < 0 || r > 8 || spl < 0 || (graph != 0 && graph != 1))
+    if (!(w == 0 || w == 1 || w == 2 || w == 4 || w == 8 || w == 16) || r < 0 || r > 8 || spl < 0 ||
+        (graph != 0 && graph != 1))
         return fail(c, NGPU_E_ARG, "invalid launch configuration w=%d r=%d spl=%d graph=%d", w, r, spl, graph);
     c->cfg_w = w; c->cfg_r = r; c->cfg_spl = spl; c->cfg_graph = graph;
     c->tables_dirty = true;
@@ -471,7 +472,7 @@ static int positions(int N) { return (N + 63) / 64 * 64; }
  * per one-wave group, six per eight-wave group). */
 static void choose_layout(const ngpu_ctx *c, int &W, int &R)
 {
-    W = c->cfg_w > 0 ? c->cfg_w : (c->N <= 512 ? 1 : 8);
+    W = c->cfg_w > 0 ? c->cfg_w : (c->N <= 512 ? 1 : 16);
     R = c->cfg_r > 0 ? c->cfg_r : 0;
 }
 static int max_runs_per_group(int W) { return W == 1 ? 2 : 6; }

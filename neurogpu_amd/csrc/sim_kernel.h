@@ -150,7 +150,10 @@ __host__ __device__ inline size_t ng_lds_bytes(int N, int NQ, int runs, int npv,
 
 /* Parameter j at position q, read where a mechanism uses it: a value shared
  * by every segment comes from the constant table (a scalar load), otherwise
- * the per-position table gives the value or the tag of a run-varying column. */
+ * the per-position table gives the value or the tag of a run-varying column.
+ * The tag is resolved without a branch (the run's first column is read when
+ * there is no tag), so that the loads of several parameters are issued
+ * together instead of each waiting for its table entry. */
 struct ParamFetch {
     const double *pconst, *pval, *pv;
     unsigned mask;
@@ -158,10 +161,10 @@ struct ParamFetch {
     __device__ __forceinline__ double operator()(int j) const
     {
         if (mask & (1u << j)) return pconst[j];
-        double x = pval[j * NQ + q];
-        const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
-        if ((unsigned)(bits >> 32) == NG_PTAG) x = pv[(unsigned)bits];
-        return x;
+        const double x = pval[j * NQ + q];
+        const bool tag = (unsigned)__double2hiint(x) == NG_PTAG;
+        const double y = pv[tag ? (unsigned)__double2loint(x) : 0u];
+        return tag ? y : x;
     }
 };
 
@@ -258,6 +261,21 @@ __device__ __forceinline__ void ng_fold(double ut, double bt, double e, double f
  * (lane g).  RG is the first of the group's R per-run LDS regions.  Runs from
  * nruns up to the allocated count are padding (their parameters are the last
  * set's): partial workgroups simulate them instead of leaving lanes idle. */
+/* Position p = r * n + q of a strided pass (p = g, g + LW, ...), stepped
+ * without an integer division per iteration. */
+struct NgPos {
+    int r, q, dr, dq, n;
+    __device__ __forceinline__ NgPos(int p, int stride, int n_) : n(n_)
+    {
+        r = p / n; q = p - r * n; dr = stride / n; dq = stride - dr * n;
+    }
+    __device__ __forceinline__ void step()
+    {
+        q += dq; r += dr;
+        if (q >= n) { q -= n; r++; }
+    }
+};
+
 template <class SET, bool STAGE, class SYNC>
 __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, double *RG, int run0, int g,
                                          const SYNC &sync)
@@ -287,9 +305,10 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
     sync();
     if (a.t0 == 0) {
         /* INITIAL: states at steady state, v = v_init, dv = 0 */
+        NgPos ps(g, LW, NQ);
 #pragma unroll 1
-        for (int p = g; p < P; p += LW) {
-            const int r = p / NQ, q = p - r * NQ;
+        for (int p = g; p < P; p += LW, ps.step()) {
+            const int r = ps.r, q = ps.q;
             if (q >= N) continue;
             double *S = states(r);
             const unsigned m = QM[q];
@@ -312,15 +331,17 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
 
     for (int t = a.t0; t < a.t1; t++) {
         const double dt = a.dts[t];
+        const double cm_dt = ng_rcp(dt * 1000);           /* cm / (dt * 1000) as cm * cm_dt */
         const int slot_t = (t - a.t0) & 63;
 
         /* mechanism pass: fused cnexp(t-1) + BREAKPOINT(t) + Hines row
          * (CudaStuff.cu:263-305).  eca is stored by INITIAL only: at t = 0 it
          * is loaded, afterwards it is recomputed from cai after the cnexp. */
         const unsigned keep = (t > 0) ? ~(1u << SET::ECA) : ~0u;
+        NgPos ps(g, LW, NQ);
 #pragma unroll 1
-        for (int p = g; p < P; p += LW) {
-            const int r = p / NQ, q = p - r * NQ;
+        for (int p = g; p < P; p += LW, ps.step()) {
+            const int r = ps.r, q = ps.q;
             if (q >= N) continue;
             double *S = states(r);
             double *XU = region(r);
@@ -360,9 +381,9 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                     const int stim = (run0 + r) % a.nstim;
                     istim = 100 * a.amps[(size_t)stim * a.nt_amps + t] / a.stim_area;
                 }
-                const double gModel = (sumDv - sum) / NG_EPS_V;
+                const double gModel = (sumDv - sum) * (1.0 / NG_EPS_V);
                 double rhs = istim - sum;
-                double D = gModel + cmq / (dt * 1000);
+                double D = gModel + cmq * cm_dt;
                 D -= fh;
                 rhs -= fh * dvq;
                 /* the children's terms, first son then second son
@@ -431,8 +452,9 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
 
         /* pivot quotients of every row, in parallel: PX = b/u and PF = -f/u
          * (BKSUB1's first step, CudaStuff.cu:75-78) replace b and u */
-        for (int i = g; i < R * N; i += LW) {
-            const int r = i / N, h = i - r * N;
+        NgPos rw(g, LW, N);
+        for (int i = g; i < R * N; i += LW, rw.step()) {
+            const int r = rw.r, h = rw.q;
             double2 *XU2 = (double2 *)region(r);
             const double2 ph = XU2[h];
             const double rc = ng_rcp(ph.y);
@@ -473,11 +495,12 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
 
         /* v += x (CudaStuff.cu:312-315) and dv += x[parent] - x[self] for the
          * next step (CudaStuff.cu:287) */
+        NgPos pu(g, LW, NQ);
 #pragma unroll 4
-        for (int p = g; p < P; p += LW) {
+        for (int p = g; p < P; p += LW, pu.step()) {
             /* padding positions read a real one and store nothing, so that
              * the loads of several positions can be in flight together */
-            const int r = p / NQ, q = p - r * NQ, qr = min(q, N - 1);
+            const int r = pu.r, q = pu.q, qr = min(q, N - 1);
             const double *XU = region(r);
             double *S = states(r);
             const short4 l = QL[qr];
@@ -505,9 +528,12 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
 #endif
 
 /* One morphology.  Workgroup b holds G = blockDim.x / LW groups of R runs:
- * runs b * G * R + group * R + (0 .. R-1). */
-template <class SET, bool MULTI, bool STAGE>
-__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(NG_WAVES_PER_EU)))
+ * runs b * G * R + group * R + (0 .. R-1).  MAXT is the largest workgroup
+ * the instantiation accepts: 512 (up to two waves per SIMD, up to 256 VGPRs
+ * a lane) or 1024 (16 waves, four per SIMD, 128 VGPRs a lane; what does not
+ * fit is spilled, in practice in the code of the rare many-mechanism masks). */
+template <class SET, bool MULTI, bool STAGE, int MAXT>
+__global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(MAXT > 512 ? 4 : NG_WAVES_PER_EU)))
 ngpu_sim_kernel(SimArgs a)
 {
     extern __shared__ double lds[];
@@ -577,10 +603,10 @@ ngpu_batch_kernel(BatchArgs b)
 
 /* Host-side launchers of one mechanism set's kernels (instantiated once per
  * set, in sim_<set>.hip).  multi: groups of more than one wave. */
-template <class SET, bool MULTI, bool STAGE>
+template <class SET, bool MULTI, bool STAGE, int MAXT = 512>
 static hipError_t ng_launch_sim(dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a)
 {
-    auto k = ngpu_sim_kernel<SET, MULTI, STAGE>;
+    auto k = ngpu_sim_kernel<SET, MULTI, STAGE, MAXT>;
     if (lds > 65536) {
         hipError_t e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -592,6 +618,8 @@ static hipError_t ng_launch_sim(dim3 grid, dim3 block, size_t lds, hipStream_t s
 template <class SET>
 static hipError_t ng_launch_set(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a)
 {
+    if (block.x > 512) return a.lds_staging ? ng_launch_sim<SET, true, true, 1024>(grid, block, lds, s, a)
+                                            : ng_launch_sim<SET, true, false, 1024>(grid, block, lds, s, a);
     if (multi) return a.lds_staging ? ng_launch_sim<SET, true, true>(grid, block, lds, s, a)
                                     : ng_launch_sim<SET, true, false>(grid, block, lds, s, a);
     return a.lds_staging ? ng_launch_sim<SET, false, true>(grid, block, lds, s, a)

@@ -48,3 +48,4 @@ extern "C" int host_rates(int ms, int mech, double x, const double *P, double *o
 
 extern "C" double host_gate(double y, double yinf, double rate, double dt) { return ng_gate(y, yinf, rate, dt); }
 extern "C" double host_expc(double x) { return ng_expc(x); }
+extern "C" double host_logc(double x) { return ng_logc(x); }

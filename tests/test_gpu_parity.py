@@ -45,7 +45,7 @@ def test_passive64_full_size():
     _check(gpu, _oracle(w), "passive64")
 
 
-@pytest.mark.parametrize("launch", [None, dict(waves_per_run=2), dict(runs_per_group=2),
+@pytest.mark.parametrize("launch", [None, dict(waves_per_run=2), dict(runs_per_group=2), dict(waves_per_run=16),
                                     dict(steps_per_launch=700), dict(steps_per_launch=500, use_graph=True)])
 def test_active500_small(launch):
     w = models.workload("active500", nsets=6, nt=1500)
@@ -54,7 +54,8 @@ def test_active500_small(launch):
     _check(gpu, _oracle(w), f"active500 {launch}")
 
 
-@pytest.mark.parametrize("launch", [None, dict(waves_per_run=4), dict(waves_per_run=8, runs_per_group=3)])
+@pytest.mark.parametrize("launch", [None, dict(waves_per_run=4), dict(waves_per_run=8, runs_per_group=3),
+                                    dict(waves_per_run=16, runs_per_group=4)])
 def test_ttpc1200_small(launch):
     w = models.workload("ttpc1200", nsets=3, nt=1200)
     _say(f"case ttpc1200 small launch={launch}")

@@ -32,6 +32,8 @@ def host(tmp_path_factory):
     L.host_gate.argtypes = [C.c_double] * 4
     L.host_expc.restype = C.c_double
     L.host_expc.argtypes = [C.c_double]
+    L.host_logc.restype = C.c_double
+    L.host_logc.argtypes = [C.c_double]
     return L
 
 
@@ -118,12 +120,27 @@ def test_gate_update_equals_cnexp(host):
 
 
 def test_clamped_exp(host):
-    """ng_expc is ng_exp inside [-708, 708] and saturates to finite normal
-    numbers outside it."""
+    """ng_expc (Estrin-evaluated polynomial) is within one ulp of ng_exp and of
+    libm inside [-708, 708] and saturates to finite normal numbers outside it."""
     L = cpu_ref.lib()
     L.ref_math.restype = C.c_double
     L.ref_math.argtypes = [C.c_int, C.c_double, C.c_double]
-    for x in np.concatenate([np.linspace(-700, 700, 3001), [-708.0, 708.0, 0.0, 1e-300]]):
-        assert host.host_expc(x) == L.ref_math(0, x, 0.0)
+    xs = np.concatenate([np.linspace(-700, 700, 3001), np.random.default_rng(3).uniform(-60, 60, 5000),
+                         [-708.0, 708.0, 1e-300, -1e-300]])
+    for x in xs:
+        got = host.host_expc(x)
+        assert abs(got - L.ref_math(0, x, 0.0)) <= math.ulp(got) and abs(got - math.exp(x)) <= math.ulp(got)
+    assert host.host_expc(0.0) == 1.0
     assert host.host_expc(1e6) == host.host_expc(708.0) and math.isfinite(host.host_expc(1e6))
     assert host.host_expc(-1e6) == host.host_expc(-708.0) > 0.0
+
+
+def test_branch_free_log(host):
+    """ng_logc (no special-case branches, Estrin-evaluated atanh series) is
+    within one ulp of libm's log over the normal range and exact at 1."""
+    ys = np.concatenate([np.logspace(-300, 300, 4001), np.random.default_rng(4).uniform(0.5, 2.0, 5000),
+                         np.logspace(-8, 8, 4001), [2.0 ** -1022, 2.0, 0.5, math.sqrt(0.5), math.sqrt(2.0)]])
+    for y in ys:
+        ref = math.log(y)
+        assert abs(host.host_logc(y) - ref) <= math.ulp(ref), y
+    assert host.host_logc(1.0) == 0.0
