@@ -53,6 +53,14 @@ static int param_owner(int ms, int j)
         if (j >= b[m] && j < b[m + 1]) return m;
     return -1;
 }
+/* Masks whose mechanism code is specialised (the dispatch lists of mechs.h);
+ * used by the schedule's cost estimate only. */
+static bool mask_fixed(int ms, unsigned m)
+{
+    if (ms == NGPU_MECH_TTPC) return m == 0x408u || m == 0x1618u;
+    if (ms == NGPU_MECH_MAINEN) return m == 0x6fu;
+    return m <= 1u;
+}
 static int mech_nstate(int ms) { return ms == NGPU_MECH_TTPC ? SetTTPC::NSTATE : ms == NGPU_MECH_MAINEN ? SetMainen::NSTATE : SetPassive::NSTATE; }
 
 template <class T> struct DevArr {
@@ -93,7 +101,10 @@ struct ngpu_ctx {
     /* device state */
     int lay_W = 0, lay_R = 0, lay_NQ = 0;
     bool tables_dirty = true, params_dirty = true, stim_dirty = true, graph_dirty = true;
-    DevArr<int> d_qlink, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_rec;
+    DevArr<int> d_qlink, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_rec, d_msched;
+    std::vector<int> qmask;        /* mechanism mask by position */
+    std::vector<int> msched;       /* mechanism-pass schedule of the last (W, R) */
+    int msched_W = 0, msched_R = 0;
     DevArr<double> d_qcm, d_ef, d_pval, d_pconst, d_pv, d_dts, d_amps, d_states, d_vhot, d_times, d_fwin, d_feat;
     int stim_q = -1;
     std::vector<int> rec_q;
@@ -184,7 +195,7 @@ extern "C" void ngpu_close(ngpu_ctx *c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
-    for (auto *d : {&c->d_qlink, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_rec}) d->release();
+    for (auto *d : {&c->d_qlink, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_rec, &c->d_msched}) d->release();
     for (auto *d : {&c->d_qcm, &c->d_ef, &c->d_pval, &c->d_pconst, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_vhot, &c->d_times, &c->d_fwin, &c->d_feat}) d->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -495,6 +506,8 @@ static int build_tables(ngpu_ctx *c)
         qcm[q] = c->cm[s];
     }
     for (int q = N; q < NQ; q++) qlink[4 * q + 1] = 0;
+    c->qmask = qmeta;
+    c->msched_W = 0;
     /* sons in son order: the reference adds the children's terms to the
      * parent's row first for SonNoVec 1, then for SonNoVec 2
      * (CANON/CudaStuff.cu:296-305); other values add nothing */
@@ -607,6 +620,7 @@ static void fill_args(const ngpu_ctx *c, SimArgs &a, int W, int R, int staged, i
     a.row_ef = c->d_ef.p; a.pval = c->d_pval.p; a.pv = c->d_pv.p;
     a.pconst_mask = c->pconst_mask;
     a.pconst = c->d_pconst.p;
+    a.msched = c->d_msched.p;
     a.chains = c->d_chains.p; a.lvl_ptr = c->d_lvl.p; a.fath = c->d_fath.p; a.flvl_ptr = c->d_flvl.p;
     a.fchild = c->d_fchild.p; a.lds_staging = staged; a.block_sync = block_sync;
     a.nchains = (int)c->chains.size() / 5; a.nfath = (int)c->fath.size() / 3; a.nfchild = (int)c->fchild.size();
@@ -640,6 +654,53 @@ static int enqueue(ngpu_ctx *c, int &launches)
 }
 
 static int prepare_layout(ngpu_ctx *c, int W, int R);
+
+/* The mechanism pass's work of a group: segments (chunk of 64 positions,
+ * runs r0 .. r0+n-1 of the group) assigned to its W waves, longest first to
+ * the least loaded wave.  A chunk's cost is the sum over the distinct masks
+ * of its positions of (mechanisms + 1), since a wave runs each mask's code in
+ * turn; a chunk of specialised masks is cut into segments of up to three
+ * runs (the kernel loads a run's states while it computes the previous run),
+ * one with run-time-mask code into single runs.  Layout: offsets[W + 1], then
+ * the segments packed as chunk << 8 | r0 << 4 | n. */
+static int mech_schedule(ngpu_ctx *c, int W, int R)
+{
+    if (c->msched_W == W && c->msched_R == R) return NGPU_OK;
+    const int NC = c->lay_NQ / 64;
+    struct Seg { int packed; double cost; };
+    std::vector<Seg> segs;
+    for (int k = 0; k < NC; k++) {
+        std::vector<unsigned> ms;
+        bool generic = false;
+        for (int q = 64 * k; q < std::min(64 * k + 64, c->N); q++) {
+            const unsigned m = (unsigned)c->qmask[q];
+            if (std::find(ms.begin(), ms.end(), m) == ms.end()) ms.push_back(m);
+            generic = generic || !mask_fixed(c->mechset, m);
+        }
+        double cost = 0.0;
+        for (unsigned m : ms) cost += __builtin_popcount(m) + 1;
+        const int L = generic ? 1 : 3;
+        for (int r0 = 0; r0 < R; r0 += L) {
+            const int n = std::min(L, R - r0);
+            segs.push_back({k << 8 | r0 << 4 | n, cost * n});
+        }
+    }
+    std::stable_sort(segs.begin(), segs.end(), [](const Seg &x, const Seg &y) { return x.cost > y.cost; });
+    std::vector<double> load(W, 0.0);
+    std::vector<std::vector<int>> per(W);
+    for (const Seg &sg : segs) {
+        const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[w] += sg.cost;
+        per[w].push_back(sg.packed);
+    }
+    c->msched.assign(1, 0);
+    for (int w = 0; w < W; w++) c->msched.push_back(c->msched[w] + (int)per[w].size());
+    for (int w = 0; w < W; w++) c->msched.insert(c->msched.end(), per[w].begin(), per[w].end());
+    int rc = upload(c, c->d_msched, c->msched);
+    if (rc) return rc;
+    c->msched_W = W; c->msched_R = R;
+    return NGPU_OK;
+}
 
 static int prepare(ngpu_ctx *c)
 {
@@ -680,6 +741,10 @@ static int prepare_layout(ngpu_ctx *c, int W, int R)
     if ((rc = alloc(c, c->d_states, nruns * (mech_nstate(c->mechset) + 2) * NQ)) ||
         (rc = alloc(c, c->d_vhot, nruns * c->rec.size() * c->nt)))
         return rc;
+    /* the schedule of the runs per group a launch will use (a shape that does
+     * not fit is reported by the launch) */
+    int Rp = R, G = 1, stage = c->cfg_stage;
+    if (plan(c, W, Rp, G, stage) == NGPU_OK && (rc = mech_schedule(c, W, Rp))) return rc;
     return NGPU_OK;
 }
 
@@ -879,6 +944,7 @@ extern "C" int ngpu_batch_run(ngpu_batch *b, ngpu_batch_stats *st)
         const size_t need = lds_need(c, G * R, stage);
         if (need > kLdsLimit) return bfail(b, NGPU_E_ARG, "member %d needs %zu bytes of LDS per workgroup (limit 160 KiB)", i, need);
         lds = std::max(lds, need);
+        if ((rc = mech_schedule(c, W, R))) return bfail(b, rc, "member %d: %s", i, c->err.c_str());
         fill_args(c, args[i], W, R, stage, sync);
         b->lay_w[i] = W; b->lay_r[i] = R; b->lay_sync[i] = sync; b->lay_stage[i] = stage;
         const int nruns = c->nsets * c->nstim, per_item = G * R;

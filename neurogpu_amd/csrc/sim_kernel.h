@@ -79,6 +79,8 @@ struct SimArgs {
     const double *pv;                    /* [nsets*nvar] varying parameter values */
     unsigned pconst_mask;                /* bit j: parameter j has one value wherever it is used */
     const double *pconst;                /* [NPARAM] those values */
+    const int *msched;                   /* mechanism-pass segments by wave: offsets[W+1], then
+                                            chunk << 8 | r0 << 4 | n */
     const int *chains, *lvl_ptr;         /* chains: (i0, i1, tA, tB, pe); lvl_ptr[depth+2] */
     const int *fath, *flvl_ptr, *fchild; /* fathers with their own phase: (J, c0, c1); flvl_ptr[depth+1] */
     int lds_staging;                     /* row coefficients e, f and the runs' varying
@@ -137,10 +139,19 @@ __host__ __device__ inline size_t ng_lds_per_run(int N, int npv, int nrec)
 __host__ __device__ inline size_t ng_lds_bytes(int N, int NQ, int runs, int npv, int nrec, int staged,
                                                int nchains, int nfath, int nfchild, int depth)
 {
-    const size_t nshort = 5 * (size_t)NQ + 5 * (size_t)nchains + 3 * (size_t)nfath + (size_t)nfchild +
+    const size_t nshort = 5 * (size_t)NQ + 8 * (size_t)nchains + 3 * (size_t)nfath + (size_t)nfchild +
                           (size_t)(depth + 2) + (size_t)(depth + 1);
     const size_t dbl = (staged ? 2 * (size_t)N : 0) + (size_t)runs * ng_lds_per_run(N, staged ? npv : 0, nrec);
     return dbl * sizeof(double) + ((nshort * sizeof(short) + 7) / 8) * 8;
+}
+
+/* Element idx of a global array as base + a 32-bit byte offset (a scalar
+ * base and one offset register per lane; every array indexed so holds less
+ * than 4 GiB). */
+template <class T>
+__device__ __forceinline__ T &ng_at(T *base, int idx)
+{
+    return *(T *)((char *)base + (unsigned)idx * (unsigned)sizeof(T));
 }
 
 /* Parameter tables hold either the value or, for a run-varying column k, a
@@ -161,7 +172,36 @@ struct ParamFetch {
     __device__ __forceinline__ double operator()(int j) const
     {
         if (mask & (1u << j)) return pconst[j];
-        const double x = pval[j * NQ + q];
+        const double x = ng_at(pval, j * NQ + q);
+        const bool tag = (unsigned)__double2hiint(x) == NG_PTAG;
+        const double y = pv[tag ? (unsigned)__double2loint(x) : 0u];
+        return tag ? y : x;
+    }
+};
+
+/* The parameter entries of position q (value or tag of a run-varying
+ * column), read once for all runs of a position; with a fixed mechanism mask
+ * only the entries its mechanisms use are loaded. */
+template <int NP>
+struct ParamEntries {
+    double x[NP];
+    unsigned cmask;
+    __device__ __forceinline__ ParamEntries(const SimArgs &a, int q) : cmask(a.pconst_mask)
+    {
+#pragma unroll
+        for (int j = 0; j < NP; j++) x[j] = (cmask & (1u << j)) ? a.pconst[j] : ng_at(a.pval, j * a.NQ + q);
+    }
+};
+/* Parameter j of one run at that position: a tag takes the run's column
+ * value (the run's first value is read when there is no tag, no branch). */
+template <int NP>
+struct ParamRun {
+    const ParamEntries<NP> &e;
+    const double *pv;
+    __device__ __forceinline__ double operator()(int j) const
+    {
+        const double x = e.x[j];
+        if (e.cmask & (1u << j)) return x;
         const bool tag = (unsigned)__double2hiint(x) == NG_PTAG;
         const double y = pv[tag ? (unsigned)__double2loint(x) : 0u];
         return tag ? y : x;
@@ -197,13 +237,16 @@ __device__ __forceinline__ Tables stage_tables(const SimArgs &a, double *lds, in
     short4 *QL = (short4 *)(T.runs + (size_t)runs * ng_lds_per_run(N, npv, a.nrec));
     unsigned short *QM = (unsigned short *)(QL + NQ);
     short *CH = (short *)(QM + NQ);
-    short *FA = CH + 5 * a.nchains, *FC = FA + 3 * a.nfath, *LP = FC + a.nfchild, *FLP = LP + (a.depth + 2);
+    short *FA = CH + 8 * a.nchains, *FC = FA + 3 * a.nfath, *LP = FC + a.nfchild, *FLP = LP + (a.depth + 2);
     for (int i = threadIdx.x; i < NQ; i += blockDim.x) {
         QL[i] = make_short4((short)a.q_link[4 * i], (short)a.q_link[4 * i + 1], (short)a.q_link[4 * i + 2],
                             (short)a.q_link[4 * i + 3]);
         QM[i] = (unsigned short)a.q_meta[i];
     }
-    for (int i = threadIdx.x; i < 5 * a.nchains; i += blockDim.x) CH[i] = (short)a.chains[i];
+    for (int i = threadIdx.x; i < 8 * a.nchains; i += blockDim.x) {
+        const int k = i >> 3, m = i & 7;
+        CH[i] = m < 5 ? (short)a.chains[5 * k + m] : (short)0;
+    }
     for (int i = threadIdx.x; i < 3 * a.nfath; i += blockDim.x) FA[i] = (short)a.fath[i];
     for (int i = threadIdx.x; i < a.nfchild; i += blockDim.x) FC[i] = (short)a.fchild[i];
     for (int i = threadIdx.x; i < a.depth + 2; i += blockDim.x) LP[i] = (short)a.lvl_ptr[i];
@@ -217,10 +260,11 @@ __device__ __forceinline__ Tables stage_tables(const SimArgs &a, double *lds, in
  * the IEEE division on gfx9 (v_div_scale, rcp, two Newton steps, product,
  * residual correction, v_div_fmas, v_div_fixup) neither scales nor fixes up,
  * and its remaining steps are exactly the ones below, so the quotients are
- * bit for bit those of n1/d and n2/d.  The test is a vote of the wave, so the
- * common case is straight-line code; a wave with any other operand (zero,
- * huge, tiny, non-finite) takes the divisions themselves.  The reciprocal
- * refinement is done once. */
+ * bit for bit those of n1/d and n2/d.  The shared-reciprocal quotients are
+ * computed first; the operand test runs beside them and is a vote of the
+ * wave, so a wave with any other operand (zero, huge, tiny, non-finite)
+ * replaces them by the divisions themselves, off the common path's chain of
+ * dependent instructions. */
 __device__ __forceinline__ int ng_exp_in(double x, unsigned lo, unsigned span)
 {
     const unsigned ex = ((unsigned)__double2hiint(x) >> 20) & 0x7ffu;
@@ -228,19 +272,18 @@ __device__ __forceinline__ int ng_exp_in(double x, unsigned lo, unsigned span)
 }
 __device__ __forceinline__ void ng_div2(double n1, double n2, double d, double &q1, double &q2)
 {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    double q = n1 * r;
+    q1 = fma(fma(-d, q, n1), r, q);
+    q = n2 * r;
+    q2 = fma(fma(-d, q, n2), r, q);
     const int plain = ng_exp_in(d, 1023u - 300u, 600u) & ng_exp_in(n1, 1023u - 400u, 800u) &
                       ng_exp_in(n2, 1023u - 400u, 800u);
-    if (__all(plain)) {
-        double r = __builtin_amdgcn_rcp(d);
-        double e = fma(-d, r, 1.0);
-        r = fma(r, e, r);
-        e = fma(-d, r, 1.0);
-        r = fma(r, e, r);
-        double q = n1 * r;
-        q1 = fma(fma(-d, q, n1), r, q);
-        q = n2 * r;
-        q2 = fma(fma(-d, q, n2), r, q);
-    } else {
+    if (!__all(plain)) {
         q1 = n1 / d;
         q2 = n2 / d;
     }
@@ -261,6 +304,30 @@ __device__ __forceinline__ void ng_fold(double ut, double bt, double e, double f
  * (lane g).  RG is the first of the group's R per-run LDS regions.  Runs from
  * nruns up to the allocated count are padding (their parameters are the last
  * set's): partial workgroups simulate them instead of leaving lanes idle. */
+/* tk / n for 0 <= tk < 8 n and 1 <= n < 32768 (a task's run): (tk + 1/2) / n
+ * lies at least 1/(2n) > 1.5e-5 from an integer, and the single-precision
+ * reciprocal and product are within 2e-6 of it below 8, so truncation gives
+ * the quotient exactly. */
+__device__ __forceinline__ int ng_task_run(int tk, int n)
+{
+    return (int)(((float)tk + 0.5f) * __builtin_amdgcn_rcpf((float)n));
+}
+
+/* A chain task of a level: run r and the chain's (i0, i1, tA, tB, pe), read
+ * from the chain table (eight shorts a chain, two 8-byte loads). */
+struct NgChain {
+    int r, i0, i1, tA, tB, pe;
+};
+__device__ __forceinline__ NgChain ng_chain(const short *CH, int tk, int k0, int nc)
+{
+    NgChain c;
+    c.r = ng_task_run(tk, nc);
+    const int k = k0 + tk - c.r * nc;
+    const short4 a = ((const short4 *)CH)[2 * k], b = ((const short4 *)CH)[2 * k + 1];
+    c.i0 = a.x; c.i1 = a.y; c.tA = a.z; c.tB = a.w; c.pe = b.x;
+    return c;
+}
+
 /* Position p = r * n + q of a strided pass (p = g, g + LW, ...), stepped
  * without an integer division per iteration. */
 struct NgPos {
@@ -290,6 +357,12 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
     const unsigned short *QM = T.QM;
     const short *CH = T.CH, *FA = T.FA, *FC = T.FC, *LP = T.LP, *FLP = T.FLP;
     const int nsets = a.nruns / a.nstim;
+    /* The positions of a pass are p = g, g + LW, ... with LW and NQ multiples
+     * of 64, so the 64 positions of a wave always belong to one run: the
+     * run index of the position passes is wave-uniform (scalar registers, so
+     * that the run's state and LDS bases are scalar and the loads take a
+     * 32-bit offset). */
+    auto wave_run = [](int r) { return __builtin_amdgcn_readfirstlane(r); };
     auto region = [&](int r) { return RG + (size_t)r * PR; };
     auto states = [&](int r) { return a.states + (size_t)(run0 + r) * NS * NQ; };
     auto pvof = [&](int r) -> const double * {
@@ -308,7 +381,7 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
         NgPos ps(g, LW, NQ);
 #pragma unroll 1
         for (int p = g; p < P; p += LW, ps.step()) {
-            const int r = ps.r, q = ps.q;
+            const int r = wave_run(ps.r), q = ps.q;
             if (q >= N) continue;
             double *S = states(r);
             const unsigned m = QM[q];
@@ -320,9 +393,9 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
             const unsigned sb = SET::state_bits(m);
 #pragma unroll
             for (int k = 0; k < SET::NSTATE; k++)
-                if (sb & (1u << k)) S[k * NQ + q] = st[k];
-            S[SV * NQ + q] = a.v_init;
-            S[SDV * NQ + q] = 0.0;
+                if (sb & (1u << k)) ng_at(S, k * NQ + q) = st[k];
+            ng_at(S, SV * NQ + q) = a.v_init;
+            ng_at(S, SDV * NQ + q) = 0.0;
         }
     }
     sync.mem();
@@ -338,59 +411,102 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
          * (CudaStuff.cu:263-305).  eca is stored by INITIAL only: at t = 0 it
          * is loaded, afterwards it is recomputed from cai after the cnexp. */
         const unsigned keep = (t > 0) ? ~(1u << SET::ECA) : ~0u;
-        NgPos ps(g, LW, NQ);
+        /* Wave wv takes the segments (chunk c of 64 positions, runs r0 ..
+         * r0+n-1) that the host's schedule gives it: it walks the runs of a
+         * chunk in turn with the chunk's mechanism mask, loading the next
+         * run's state while it computes the current one.  Parameter entries
+         * are the same for every run and are read once per segment. */
+        const int W = LW >> 6, wv = __builtin_amdgcn_readfirstlane(g >> 6);
+        const int s1 = a.msched[wv + 1];
 #pragma unroll 1
-        for (int p = g; p < P; p += LW, ps.step()) {
-            const int r = ps.r, q = ps.q;
+        for (int si = a.msched[wv]; si < s1; si++) {
+            const int d = a.msched[W + 1 + si];
+            const int c = d >> 8, r0 = (d >> 4) & 15, n = d & 15;
+            const int q = (c << 6) + (g & 63);
             if (q >= N) continue;
-            double *S = states(r);
-            double *XU = region(r);
             /* the mechanism code is specialised for the set's common mechanism
              * masks (straight-line code, independent gates interleaved) */
             SET::dispatch(QM[q], [&](auto mask) {
+                constexpr bool fixed = !std::is_same<decltype(mask), unsigned>::value;
                 const unsigned m = mask;
                 const short4 l = QL[q];                       /* row, parent row, sons */
-                double st[SET::NSTATE];
-                const unsigned sbn = (SET::state_bits(m) & keep) | (t > 0 ? SET::ica_bits(m) : 0u);
-#pragma unroll
-                for (int k = 0; k < SET::NSTATE; k++) st[k] = (sbn & (1u << k)) ? S[k * NQ + q] : 0.0;
-                const double vq = S[SV * NQ + q], dvq = S[SDV * NQ + q];
-                const double dv1 = l.z >= 0 ? S[SDV * NQ + l.z] : 0.0, dv2 = l.w >= 0 ? S[SDV * NQ + l.w] : 0.0;
                 const int h = l.x;
-                const double cmq = a.q_cm[q];
+                const double cmq = ng_at(a.q_cm, q);
                 const double fh = EF[h].y;
-                /* record v before the step (CudaStuff.cu:257-261) */
-                for (int j = 0; j < a.nrec; j++)
-                    if (q == a.rec_q[j]) XU[2 * (N + 1) + npv + j * 64 + slot_t] = vq;
+                const double e1 = l.z >= 0 ? EF[QL[l.z].x].x : 0.0, e2 = l.w >= 0 ? EF[QL[l.w].x].x : 0.0;
+                const unsigned sbn = (SET::state_bits(m) & keep) | (t > 0 ? SET::ica_bits(m) : 0u);
                 const unsigned sb = SET::state_bits(m) & keep;
                 const unsigned ib = SET::ica_bits(m);
-                const ParamFetch Pf{a.pconst, a.pval, pvof(r), a.pconst_mask, q, NQ};
-                if (t > 0) {
-                    SET::cnexp(m, vq, dtp, st, ib ? st[SET::ICA] : 0.0, Pf);
-                    SET::eca_after_cnexp(m, st);
+                struct In { double st[SET::NSTATE], vq, dvq, dv1, dv2; };
+                auto load = [&](int r, In &x) {
+                    const double *S = states(r);
 #pragma unroll
-                    for (int k = 0; k < SET::NSTATE; k++)
-                        if (sb & (1u << k)) S[k * NQ + q] = st[k];
+                    for (int k = 0; k < SET::NSTATE; k++) x.st[k] = (sbn & (1u << k)) ? ng_at(S, k * NQ + q) : 0.0;
+                    x.vq = ng_at(S, SV * NQ + q);
+                    x.dvq = ng_at(S, SDV * NQ + q);
+                    x.dv1 = l.z >= 0 ? ng_at(S, SDV * NQ + l.z) : 0.0;
+                    x.dv2 = l.w >= 0 ? ng_at(S, SDV * NQ + l.w) : 0.0;
+                };
+                auto step = [&](int r, In &cur, const auto &Pf) {
+                    double *S = states(r);
+                    double *XU = region(r);
+                    double *st = cur.st;
+                    const double vq = cur.vq;
+                    /* record v before the step (CudaStuff.cu:257-261) */
+                    for (int k = 0; k < a.nrec; k++)
+                        if (q == a.rec_q[k]) XU[2 * (N + 1) + npv + k * 64 + slot_t] = vq;
+                    if (t > 0) {
+                        SET::cnexp(m, vq, dtp, st, ib ? st[SET::ICA] : 0.0, Pf);
+                        SET::eca_after_cnexp(m, st);
+#pragma unroll
+                        for (int k = 0; k < SET::NSTATE; k++)
+                            if (sb & (1u << k)) ng_at(S, k * NQ + q) = st[k];
+                    }
+                    double sumDv = 0.0, sum = 0.0, icaDv = 0.0, icaNow = 0.0;
+                    SET::current(m, vq + 0.001, st, Pf, sumDv, icaDv, false);
+                    SET::current(m, vq, st, Pf, sum, icaNow, true);
+                    if (ib) ng_at(S, SET::ICA * NQ + q) = icaNow;
+                    double istim = 0.0;
+                    if (q == a.stim_q) {
+                        const int stim = (run0 + r) % a.nstim;
+                        istim = 100 * a.amps[(size_t)stim * a.nt_amps + t] / a.stim_area;
+                    }
+                    const double gModel = (sumDv - sum) * (1.0 / NG_EPS_V);
+                    double rhs = istim - sum;
+                    double D = gModel + cmq * cm_dt;
+                    D -= fh;
+                    rhs -= fh * cur.dvq;
+                    /* the children's terms, first son then second son
+                     * (CudaStuff.cu:296-305, ordered) */
+                    if (l.z >= 0) { rhs += e1 * cur.dv1; D -= e1; }
+                    if (l.w >= 0) { rhs += e2 * cur.dv2; D -= e2; }
+                    ((double2 *)XU)[h] = make_double2(rhs, D);
+                };
+                if constexpr (fixed) {
+                    const ParamEntries<SET::NPARAM> pe(a, q);
+                    In cur;
+                    load(r0, cur);
+#pragma unroll 1
+                    for (int j = 0; j < n; j++) {
+                        In nxt = cur;
+                        if (j + 1 < n) load(r0 + j + 1, nxt);
+                        step(r0 + j, cur, ParamRun<SET::NPARAM>{pe, pvof(r0 + j)});
+                        cur = nxt;
+                    }
+                } else {
+                    /* run-time masks (rare segments): one run at a time, the
+                     * parameters read where used; q passes through an empty asm
+                     * so that their loads are not hoisted out of the run loop
+                     * (which would hold every parameter of the set in registers) */
+#pragma unroll 1
+                    for (int j = 0; j < n; j++) {
+                        int qp = q;
+                        __asm__ volatile("" : "+v"(qp));
+                        In cur;
+                        load(r0 + j, cur);
+                        step(r0 + j, cur, ParamFetch{a.pconst, a.pval, pvof(r0 + j), a.pconst_mask, qp, NQ});
+                    }
                 }
-                double sumDv = 0.0, sum = 0.0, icaDv = 0.0, icaNow = 0.0;
-                SET::current(m, vq + 0.001, st, Pf, sumDv, icaDv, false);
-                SET::current(m, vq, st, Pf, sum, icaNow, true);
-                if (ib) S[SET::ICA * NQ + q] = icaNow;
-                double istim = 0.0;
-                if (q == a.stim_q) {
-                    const int stim = (run0 + r) % a.nstim;
-                    istim = 100 * a.amps[(size_t)stim * a.nt_amps + t] / a.stim_area;
-                }
-                const double gModel = (sumDv - sum) * (1.0 / NG_EPS_V);
-                double rhs = istim - sum;
-                double D = gModel + cmq * cm_dt;
-                D -= fh;
-                rhs -= fh * dvq;
-                /* the children's terms, first son then second son
-                 * (CudaStuff.cu:296-305, ordered) */
-                if (l.z >= 0) { const double e = EF[QL[l.z].x].x; rhs += e * dv1; D -= e; }
-                if (l.w >= 0) { const double e = EF[QL[l.w].x].x; rhs += e * dv2; D -= e; }
-                ((double2 *)XU)[h] = make_double2(rhs, D);
             });
         }
         sync();
@@ -402,12 +518,16 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
          * into its first row i0-1, then eliminates rows i0-1 .. i1-2 each into
          * the next.  A task is (run, chain); every load that does not depend
          * on the chain's own results is issued up front, and the next row's
-         * values are loaded one row ahead (indices clamped to the chain). */
+         * values are loaded one row ahead (indices clamped to the chain).  A
+         * lane's first task of the next level is read from the (constant)
+         * chain table before the level's barrier. */
+        int k0 = LP[0], nc = LP[1] - k0;
+        NgChain first = ng_chain(CH, min(g, R * nc - 1), k0, nc);
         for (int L = 0; L <= a.depth; L++) {
             if (L > 0 && FLP[L] > FLP[L - 1]) {
                 const int f0 = FLP[L - 1], nf = FLP[L] - f0;
                 for (int tk = g; tk < R * nf; tk += LW) {
-                    const int r = tk / nf, k = f0 + tk - r * nf;
+                    const int r = ng_task_run(tk, nf), k = f0 + tk - r * nf;
                     double2 *XU2 = (double2 *)region(r);
                     const int J = FA[3 * k], c0 = FA[3 * k + 1], c1 = FA[3 * k + 2];
                     double2 pJ = XU2[J];
@@ -420,12 +540,13 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                 }
                 sync();
             }
-            const int k0 = LP[L], nc = LP[L + 1] - k0;
+            const int k0n = LP[min(L + 1, a.depth)], ncn = LP[min(L + 1, a.depth) + 1] - k0n;
+            const NgChain next = ng_chain(CH, min(g, R * ncn - 1), k0n, ncn);
             for (int tk = g; tk < R * nc; tk += LW) {
-                const int r = tk / nc, k = k0 + tk - r * nc;
-                double2 *XU2 = (double2 *)region(r);
-                const short *ch = CH + 5 * k;
-                const int i0 = ch[0], i1 = ch[1], tA = ch[2], tB = ch[3];
+                NgChain c = first;
+                if (tk != g) c = ng_chain(CH, tk, k0, nc);
+                double2 *XU2 = (double2 *)region(c.r);
+                const int i0 = c.i0, i1 = c.i1, tA = c.tA, tB = c.tB;
                 const int ta = max(tA, 0), tb = max(tB, 0), j0 = min(i0, i1 - 1);
                 const double2 p0 = XU2[i0 - 1], pa = XU2[ta], ea = EF[ta], pb = XU2[tb], eb = EF[tb];
                 double2 ef = EF[i0 - 1], pn = XU2[j0];
@@ -448,6 +569,7 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                 }
             }
             sync();
+            first = next; k0 = k0n; nc = ncn;
         }
 
         /* pivot quotients of every row, in parallel: PX = b/u and PF = -f/u
@@ -465,32 +587,45 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
         /* back-substitution by chains, highest level first: a chain's last
          * row takes x of its parent row pe (a row of a higher level; pe = N
          * marks the root, x = PX), then x_i = PX_i + PF_i x_{i+1} down to the
-         * chain's first row, with the quotients loaded two rows ahead; x
-         * replaces PX */
+         * chain's first row, four rows at a time with the next four rows'
+         * quotients loaded ahead; x replaces PX */
+        k0 = LP[a.depth]; nc = LP[a.depth + 1] - k0;
+        first = ng_chain(CH, min(g, R * nc - 1), k0, nc);
         for (int L = a.depth; L >= 0; L--) {
-            const int k0 = LP[L], nc = LP[L + 1] - k0;
+            const int k0n = LP[max(L - 1, 0)], ncn = LP[max(L - 1, 0) + 1] - k0n;
+            const NgChain next = ng_chain(CH, min(g, R * ncn - 1), k0n, ncn);
             for (int tk = g; tk < R * nc; tk += LW) {
-                const int r = tk / nc, k = k0 + tk - r * nc;
-                double *XU = region(r);
+                NgChain c = first;
+                if (tk != g) c = ng_chain(CH, tk, k0, nc);
+                double *XU = region(c.r);
                 const double2 *XU2 = (const double2 *)XU;
-                const short *ch = CH + 5 * k;
-                const int i0 = ch[0], i1 = ch[1], pe = ch[4];
-                const int h = i1 - 1, lo = i0 - 1;
-                const int j1 = max(h - 1, lo), j2 = max(h - 2, lo);
+                const int h = c.i1 - 1, lo = c.i0 - 1, pe = c.pe;
                 const double2 ph = XU2[h];
                 const double xp = XU[2 * min(pe, N - 1)];
-                double2 p1 = XU2[j1], p2 = XU2[j2];
+                double2 a0 = XU2[max(h - 1, lo)], a1 = XU2[max(h - 2, lo)], a2 = XU2[max(h - 3, lo)],
+                        a3 = XU2[max(h - 4, lo)];
                 double x = ph.x;
                 if (pe < N) x = x + ph.y * xp;
                 XU[2 * h] = x;
-                for (int i = h - 1; i >= lo; i--) {
-                    const double2 p3 = XU2[max(i - 2, lo)];
-                    x = p1.x + p1.y * x;
-                    XU[2 * i] = x;
-                    p1 = p2; p2 = p3;
+                int i = h - 1;
+                while (i - 3 >= lo) {
+                    /* rows i .. i-3 from a0..a3; rows i-4 .. i-7 loaded (clamped
+                     * to the chain; a clamped value is not used) */
+                    const double2 b0 = XU2[max(i - 4, lo)], b1 = XU2[max(i - 5, lo)], b2 = XU2[max(i - 6, lo)],
+                                  b3 = XU2[max(i - 7, lo)];
+                    x = a0.x + a0.y * x; XU[2 * i] = x;
+                    x = a1.x + a1.y * x; XU[2 * (i - 1)] = x;
+                    x = a2.x + a2.y * x; XU[2 * (i - 2)] = x;
+                    x = a3.x + a3.y * x; XU[2 * (i - 3)] = x;
+                    a0 = b0; a1 = b1; a2 = b2; a3 = b3;
+                    i -= 4;
                 }
+                if (i >= lo) { x = a0.x + a0.y * x; XU[2 * i] = x; }
+                if (i - 1 >= lo) { x = a1.x + a1.y * x; XU[2 * (i - 1)] = x; }
+                if (i - 2 >= lo) { x = a2.x + a2.y * x; XU[2 * (i - 2)] = x; }
             }
             sync();
+            first = next; k0 = k0n; nc = ncn;
         }
 
         /* v += x (CudaStuff.cu:312-315) and dv += x[parent] - x[self] for the
@@ -500,13 +635,13 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
         for (int p = g; p < P; p += LW, pu.step()) {
             /* padding positions read a real one and store nothing, so that
              * the loads of several positions can be in flight together */
-            const int r = pu.r, q = pu.q, qr = min(q, N - 1);
+            const int r = wave_run(pu.r), q = pu.q, qr = min(q, N - 1);
             const double *XU = region(r);
             double *S = states(r);
             const short4 l = QL[qr];
             const double xh = XU[2 * l.x], xpar = XU[2 * l.y];
-            const double vn = S[SV * NQ + qr] + xh, dvn = S[SDV * NQ + qr] + (xpar - xh);
-            if (q < N) { S[SV * NQ + q] = vn; S[SDV * NQ + q] = dvn; }
+            const double vn = ng_at(S, SV * NQ + qr) + xh, dvn = ng_at(S, SDV * NQ + qr) + (xpar - xh);
+            if (q < N) { ng_at(S, SV * NQ + q) = vn; ng_at(S, SDV * NQ + q) = dvn; }
         }
         dtp = dt;
 
@@ -539,7 +674,7 @@ ngpu_sim_kernel(SimArgs a)
     extern __shared__ double lds[];
     const int LW = a.LW, R = a.runs_per_group;
     const int G = blockDim.x / LW;
-    const int group = threadIdx.x / LW;
+    const int group = __builtin_amdgcn_readfirstlane(threadIdx.x / LW);  /* LW is a multiple of 64 */
     const int g = threadIdx.x - group * LW;
     const Tables T = stage_tables<STAGE>(a, lds, G * R);
     __syncthreads();
@@ -581,7 +716,7 @@ ngpu_batch_kernel(BatchArgs b)
         const SimArgs &a = b.morph[mi];
         if (threadIdx.x == 0) b.stamps[2 * it] = wall_clock64();
         const int LW = a.LW, R = a.runs_per_group, G = NG_BATCH_THREADS / LW;
-        const int group = threadIdx.x / LW;
+        const int group = __builtin_amdgcn_readfirstlane(threadIdx.x / LW);
         const int g = threadIdx.x - group * LW;
         const SyncDyn sync{a.block_sync != 0};
         /* groups beyond the item's runs simulate padding runs */
