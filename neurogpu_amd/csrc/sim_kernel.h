@@ -281,6 +281,8 @@ __device__ __forceinline__ void ng_div2(double n1, double n2, double d, double &
     q1 = fma(fma(-d, q, n1), r, q);
     q = n2 * r;
     q2 = fma(fma(-d, q, n2), r, q);
+    /* keep the quotients computed ahead of the test (not sunk into a branch) */
+    __asm__ volatile("" : "+v"(q1), "+v"(q2));
     const int plain = ng_exp_in(d, 1023u - 300u, 600u) & ng_exp_in(n1, 1023u - 400u, 800u) &
                       ng_exp_in(n2, 1023u - 400u, 800u);
     if (!__all(plain)) {
@@ -521,10 +523,15 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
          * values are loaded one row ahead (indices clamped to the chain).  A
          * lane's first task of the next level is read from the (constant)
          * chain table before the level's barrier. */
+        const int D = a.depth;
         int k0 = LP[0], nc = LP[1] - k0;
+        int k0n = LP[min(1, D)], ncn = LP[min(1, D) + 1] - k0n;
         NgChain first = ng_chain(CH, min(g, R * nc - 1), k0, nc);
-        for (int L = 0; L <= a.depth; L++) {
-            if (L > 0 && FLP[L] > FLP[L - 1]) {
+        for (int L = 0; L <= D; L++) {
+            /* level L+2's range and level L+1's first task, read ahead */
+            const int k0nn = LP[min(L + 2, D)], ncnn = LP[min(L + 2, D) + 1] - k0nn;
+            const NgChain next = ng_chain(CH, min(g, R * ncn - 1), k0n, ncn);
+            if (a.nfath > 0 && L > 0 && FLP[L] > FLP[L - 1]) {
                 const int f0 = FLP[L - 1], nf = FLP[L] - f0;
                 for (int tk = g; tk < R * nf; tk += LW) {
                     const int r = ng_task_run(tk, nf), k = f0 + tk - r * nf;
@@ -540,16 +547,17 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                 }
                 sync();
             }
-            const int k0n = LP[min(L + 1, a.depth)], ncn = LP[min(L + 1, a.depth) + 1] - k0n;
-            const NgChain next = ng_chain(CH, min(g, R * ncn - 1), k0n, ncn);
             for (int tk = g; tk < R * nc; tk += LW) {
                 NgChain c = first;
                 if (tk != g) c = ng_chain(CH, tk, k0, nc);
                 double2 *XU2 = (double2 *)region(c.r);
                 const int i0 = c.i0, i1 = c.i1, tA = c.tA, tB = c.tB;
                 const int ta = max(tA, 0), tb = max(tB, 0), j0 = min(i0, i1 - 1);
-                const double2 p0 = XU2[i0 - 1], pa = XU2[ta], ea = EF[ta], pb = XU2[tb], eb = EF[tb];
+                double2 p0 = XU2[i0 - 1], pa = XU2[ta], ea = EF[ta], pb = XU2[tb], eb = EF[tb];
                 double2 ef = EF[i0 - 1], pn = XU2[j0];
+                /* all of them in flight together (not sunk into the branches) */
+                __asm__ volatile("" : "+v"(p0.x), "+v"(p0.y), "+v"(pa.x), "+v"(pa.y), "+v"(ea.x), "+v"(ea.y),
+                                 "+v"(pb.x), "+v"(pb.y), "+v"(eb.x), "+v"(eb.y));
                 double u = p0.y, b = p0.x;
                 if (tA >= 0) {
                     ng_fold(pa.y, pa.x, ea.x, ea.y, u, b);
@@ -569,7 +577,7 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                 }
             }
             sync();
-            first = next; k0 = k0n; nc = ncn;
+            first = next; k0 = k0n; nc = ncn; k0n = k0nn; ncn = ncnn;
         }
 
         /* pivot quotients of every row, in parallel: PX = b/u and PF = -f/u
@@ -589,10 +597,11 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
          * marks the root, x = PX), then x_i = PX_i + PF_i x_{i+1} down to the
          * chain's first row, four rows at a time with the next four rows'
          * quotients loaded ahead; x replaces PX */
-        k0 = LP[a.depth]; nc = LP[a.depth + 1] - k0;
+        k0 = LP[D]; nc = LP[D + 1] - k0;
+        k0n = LP[max(D - 1, 0)]; ncn = LP[max(D - 1, 0) + 1] - k0n;
         first = ng_chain(CH, min(g, R * nc - 1), k0, nc);
-        for (int L = a.depth; L >= 0; L--) {
-            const int k0n = LP[max(L - 1, 0)], ncn = LP[max(L - 1, 0) + 1] - k0n;
+        for (int L = D; L >= 0; L--) {
+            const int k0nn = LP[max(L - 2, 0)], ncnn = LP[max(L - 2, 0) + 1] - k0nn;
             const NgChain next = ng_chain(CH, min(g, R * ncn - 1), k0n, ncn);
             for (int tk = g; tk < R * nc; tk += LW) {
                 NgChain c = first;
@@ -625,7 +634,7 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                 if (i - 2 >= lo) { x = a2.x + a2.y * x; XU[2 * (i - 2)] = x; }
             }
             sync();
-            first = next; k0 = k0n; nc = ncn;
+            first = next; k0 = k0n; nc = ncn; k0n = k0nn; ncn = ncnn;
         }
 
         /* v += x (CudaStuff.cu:312-315) and dv += x[parent] - x[self] for the
