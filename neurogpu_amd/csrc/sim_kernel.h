@@ -440,16 +440,16 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                 const unsigned sb = SET::state_bits(m) & keep;
                 const unsigned ib = SET::ica_bits(m);
                 struct In { double st[SET::NSTATE], vq, dvq, dv1, dv2; };
-                auto load = [&](int r, In &x) {
+                auto load = [&](int r, int qq, In &x) {
                     const double *S = states(r);
 #pragma unroll
-                    for (int k = 0; k < SET::NSTATE; k++) x.st[k] = (sbn & (1u << k)) ? ng_at(S, k * NQ + q) : 0.0;
-                    x.vq = ng_at(S, SV * NQ + q);
-                    x.dvq = ng_at(S, SDV * NQ + q);
+                    for (int k = 0; k < SET::NSTATE; k++) x.st[k] = (sbn & (1u << k)) ? ng_at(S, k * NQ + qq) : 0.0;
+                    x.vq = ng_at(S, SV * NQ + qq);
+                    x.dvq = ng_at(S, SDV * NQ + qq);
                     x.dv1 = l.z >= 0 ? ng_at(S, SDV * NQ + l.z) : 0.0;
                     x.dv2 = l.w >= 0 ? ng_at(S, SDV * NQ + l.w) : 0.0;
                 };
-                auto step = [&](int r, In &cur, const auto &Pf) {
+                auto step = [&](int r, int qq, In &cur, const auto &Pf) {
                     double *S = states(r);
                     double *XU = region(r);
                     double *st = cur.st;
@@ -462,12 +462,12 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                         SET::eca_after_cnexp(m, st);
 #pragma unroll
                         for (int k = 0; k < SET::NSTATE; k++)
-                            if (sb & (1u << k)) ng_at(S, k * NQ + q) = st[k];
+                            if (sb & (1u << k)) ng_at(S, k * NQ + qq) = st[k];
                     }
                     double sumDv = 0.0, sum = 0.0, icaDv = 0.0, icaNow = 0.0;
                     SET::current(m, vq + 0.001, st, Pf, sumDv, icaDv, false);
                     SET::current(m, vq, st, Pf, sum, icaNow, true);
-                    if (ib) ng_at(S, SET::ICA * NQ + q) = icaNow;
+                    if (ib) ng_at(S, SET::ICA * NQ + qq) = icaNow;
                     double istim = 0.0;
                     if (q == a.stim_q) {
                         const int stim = (run0 + r) % a.nstim;
@@ -487,26 +487,27 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                 if constexpr (fixed) {
                     const ParamEntries<SET::NPARAM> pe(a, q);
                     In cur;
-                    load(r0, cur);
+                    load(r0, q, cur);
 #pragma unroll 1
                     for (int j = 0; j < n; j++) {
                         In nxt = cur;
-                        if (j + 1 < n) load(r0 + j + 1, nxt);
-                        step(r0 + j, cur, ParamRun<SET::NPARAM>{pe, pvof(r0 + j)});
+                        if (j + 1 < n) load(r0 + j + 1, q, nxt);
+                        step(r0 + j, q, cur, ParamRun<SET::NPARAM>{pe, pvof(r0 + j)});
                         cur = nxt;
                     }
                 } else {
                     /* run-time masks (rare segments): one run at a time, the
                      * parameters read where used; q passes through an empty asm
-                     * so that their loads are not hoisted out of the run loop
-                     * (which would hold every parameter of the set in registers) */
+                     * so that the addresses of the state slots and parameter
+                     * entries are not hoisted out of the run loop (which would
+                     * hold one for every slot and parameter of the set) */
 #pragma unroll 1
                     for (int j = 0; j < n; j++) {
                         int qp = q;
                         __asm__ volatile("" : "+v"(qp));
                         In cur;
-                        load(r0 + j, cur);
-                        step(r0 + j, cur, ParamFetch{a.pconst, a.pval, pvof(r0 + j), a.pconst_mask, qp, NQ});
+                        load(r0 + j, qp, cur);
+                        step(r0 + j, qp, cur, ParamFetch{a.pconst, a.pval, pvof(r0 + j), a.pconst_mask, qp, NQ});
                     }
                 }
             });
