@@ -255,51 +255,13 @@ __device__ __forceinline__ Tables stage_tables(const SimArgs &a, double *lds, in
     return T;
 }
 
-/* n1/d and n2/d, correctly rounded, sharing one reciprocal of d.  When d has
- * a binary exponent in [-300, 300) and n1, n2 have exponents in [-400, 400),
- * the IEEE division on gfx9 (v_div_scale, rcp, two Newton steps, product,
- * residual correction, v_div_fmas, v_div_fixup) neither scales nor fixes up,
- * and its remaining steps are exactly the ones below, so the quotients are
- * bit for bit those of n1/d and n2/d.  The shared-reciprocal quotients are
- * computed first; the operand test runs beside them and is a vote of the
- * wave, so a wave with any other operand (zero, huge, tiny, non-finite)
- * replaces them by the divisions themselves, off the common path's chain of
- * dependent instructions. */
-__device__ __forceinline__ int ng_exp_in(double x, unsigned lo, unsigned span)
-{
-    const unsigned ex = ((unsigned)__double2hiint(x) >> 20) & 0x7ffu;
-    return ex - lo < span;
-}
-__device__ __forceinline__ void ng_div2(double n1, double n2, double d, double &q1, double &q2)
-{
-    double r = __builtin_amdgcn_rcp(d);
-    double e = fma(-d, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-d, r, 1.0);
-    r = fma(r, e, r);
-    double q = n1 * r;
-    q1 = fma(fma(-d, q, n1), r, q);
-    q = n2 * r;
-    q2 = fma(fma(-d, q, n2), r, q);
-    /* keep the quotients computed ahead of the test (not sunk into a branch) */
-    __asm__ volatile("" : "+v"(q1), "+v"(q2));
-    const int plain = ng_exp_in(d, 1023u - 300u, 600u) & ng_exp_in(n1, 1023u - 400u, 800u) &
-                      ng_exp_in(n2, 1023u - 400u, 800u);
-    if (!__all(plain)) {
-        q1 = n1 / d;
-        q2 = n2 / d;
-    }
-}
-
 /* Eliminate the final row (ut, bt) with coefficients e, f into the running
  * father values (uJ, bJ) with the reference's arithmetic
  * (CudaStuff.cu:53-64): uJ -= f (e / ut), bJ -= (bt e) / ut. */
 __device__ __forceinline__ void ng_fold(double ut, double bt, double e, double f, double &uJ, double &bJ)
 {
-    double q1, q2;
-    ng_div2(e, bt * e, ut, q1, q2);
-    uJ = uJ - f * q1;
-    bJ = bJ - q2;
+    uJ = uJ - f * (e / ut);
+    bJ = bJ - (bt * e) / ut;
 }
 
 /* Steps t0..t1 of runs run0 .. run0 + R - 1 by the LW lanes of a group
@@ -520,10 +482,13 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
          * (i0, i1, tA, tB, pe) folds up to two related rows tA, tB (-1 = none)
          * into its first row i0-1, then eliminates rows i0-1 .. i1-2 each into
          * the next.  A task is (run, chain); every load that does not depend
-         * on the chain's own results is issued up front, and the next row's
-         * values are loaded one row ahead (indices clamped to the chain).  A
-         * lane's first task of the next level is read from the (constant)
-         * chain table before the level's barrier. */
+         * on the chain's own results is issued up front, and the rows' values
+         * are loaded two rows ahead (indices clamped to the chain; a clamped
+         * value is not used).  The divisions are the IEEE ones: a shared
+         * refined reciprocal is as fast, but its operand-range test and branch
+         * cost more per row than the two divisions.  A lane's first task of
+         * the next level is read from the (constant) chain table before the
+         * level's barrier. */
         const int D = a.depth;
         int k0 = LP[0], nc = LP[1] - k0;
         int k0n = LP[min(1, D)], ncn = LP[min(1, D) + 1] - k0n;
@@ -555,7 +520,8 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                 const int i0 = c.i0, i1 = c.i1, tA = c.tA, tB = c.tB;
                 const int ta = max(tA, 0), tb = max(tB, 0), j0 = min(i0, i1 - 1);
                 double2 p0 = XU2[i0 - 1], pa = XU2[ta], ea = EF[ta], pb = XU2[tb], eb = EF[tb];
-                double2 ef = EF[i0 - 1], pn = XU2[j0];
+                const int j1 = min(i0 + 1, i1 - 1);
+                double2 ef = EF[i0 - 1], pn = XU2[j0], ef1 = EF[j1 - 1], pn1 = XU2[j1];
                 /* all of them in flight together (not sunk into the branches) */
                 __asm__ volatile("" : "+v"(p0.x), "+v"(p0.y), "+v"(pa.x), "+v"(pa.y), "+v"(ea.x), "+v"(ea.y),
                                  "+v"(pb.x), "+v"(pb.y), "+v"(eb.x), "+v"(eb.y));
@@ -566,15 +532,13 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                     XU2[i0 - 1] = make_double2(b, u);
                 }
                 for (int i = i0; i < i1; i++) {
-                    const int j = min(i + 1, i1 - 1);
+                    const int j = min(i + 2, i1 - 1);
                     const double2 ef2 = EF[j - 1], pn2 = XU2[j];
-                    double q1, q2;
-                    ng_div2(ef.x, b * ef.x, u, q1, q2);
-                    const double unew = pn.y - ef.y * q1;
-                    const double bnew = pn.x - q2;
+                    const double unew = pn.y - ef.y * (ef.x / u);
+                    const double bnew = pn.x - (b * ef.x) / u;
                     XU2[i] = make_double2(bnew, unew);
                     u = unew; b = bnew;
-                    ef = ef2; pn = pn2;
+                    ef = ef1; pn = pn1; ef1 = ef2; pn1 = pn2;
                 }
             }
             sync();

@@ -27,8 +27,8 @@ from .surrogate import MLPEnsemble, feature_targets, genome_to_input, make_scree
 
 
 def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, seed=1178, device=0,
-          oversample=10, keep=0.25, disagree=0.05, refit_epochs=10, explore=0.0, max_train=8192,
-          max_screen_gens=None):
+          oversample=20, keep=0.15, disagree=0.03, refit_epochs=10, explore=0.0, max_train=4096,
+          max_screen_gens=None, mu_screen=None):
     w = models.workload(name, nsets=1, nt=nt)
     m = w.model
     dt = float(w.dts[0])
@@ -73,8 +73,9 @@ def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, see
                          disagree_frac=disagree / oversample, explore=explore)
     say(f"screened GA: oversample {oversample}, simulate the best {keep:.0%} + most uncertain {disagree:.0%} "
         f"of a generation, surrogate retrained every generation")
-    scr = ga.run_ga(ev, mu=pop, lam=pop, ngen=max_screen_gens or 10 * gens, seed=seed, screen=screen,
-                    oversample=oversample, stop_score=s_star, time_limit=plain_t, log=say, observe=observe)
+    scr = ga.run_ga(ev, mu=mu_screen or pop, lam=pop, ngen=max_screen_gens or 10 * gens, seed=seed, screen=screen,
+                    oversample=oversample, stop_score=s_star, time_limit=plain_t, log=say, observe=observe,
+                    n_init=pop)
     scr_hist = list(scr.history)
     reached = next((t for _, t, s, _ in scr_hist if s <= s_star), None)
     ens.close()
@@ -85,7 +86,8 @@ def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, see
                   "evals_per_s": plain_evals / plain_t, "history": plain.history},
         "screened": {"best_score": scr.best_score, "evals": ev.nevals, "surrogate_fit_s": fit["s"],
                      "surrogate_fits": fit["n"], "final_loss": fit["loss"], "oversample": oversample,
-                     "keep": keep, "disagree": disagree, "explore": explore, "history": scr_hist,
+                     "keep": keep, "disagree": disagree, "explore": explore, "mu": mu_screen or pop,
+                     "history": scr_hist,
                      "wall_s_to_reach_plain_best": reached, "evals_per_s": ev.nevals / scr_hist[-1][1]},
         "time_ratio": (reached / plain_t) if reached else None,
     }
@@ -99,17 +101,18 @@ def main():
     ap.add_argument("--nt", type=int, default=2000)
     ap.add_argument("--members", type=int, default=5)
     ap.add_argument("--epochs", type=int, default=60)
-    ap.add_argument("--oversample", type=int, default=10)
-    ap.add_argument("--keep", type=float, default=0.25)
-    ap.add_argument("--disagree", type=float, default=0.05)
+    ap.add_argument("--oversample", type=int, default=20)
+    ap.add_argument("--keep", type=float, default=0.15)
+    ap.add_argument("--disagree", type=float, default=0.03)
     ap.add_argument("--refit-epochs", type=int, default=10)
     ap.add_argument("--seed", type=int, default=1178)
     ap.add_argument("--explore", type=float, default=0.0)
-    ap.add_argument("--max-train", type=int, default=8192)
+    ap.add_argument("--max-train", type=int, default=4096)
+    ap.add_argument("--mu-screen", type=int, default=None, help="parents of the screened GA (default --pop)")
     args = ap.parse_args()
     r = study(args.workload, args.pop, args.gens, args.nt, args.members, args.epochs, seed=args.seed,
               oversample=args.oversample, keep=args.keep, disagree=args.disagree, refit_epochs=args.refit_epochs,
-              explore=args.explore, max_train=args.max_train)
+              explore=args.explore, max_train=args.max_train, mu_screen=args.mu_screen)
     print(json.dumps(r), flush=True)
 
 
