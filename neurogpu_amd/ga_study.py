@@ -28,7 +28,7 @@ from .surrogate import MLPEnsemble, feature_targets, genome_to_input, make_scree
 
 def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, seed=1178, device=0,
           oversample=20, keep=0.15, disagree=0.03, refit_epochs=10, explore=0.0, max_train=4096,
-          max_screen_gens=None, mu_screen=None):
+          max_screen_gens=None, mu_screen=None, random=0.0):
     w = models.workload(name, nsets=1, nt=nt)
     m = w.model
     dt = float(w.dts[0])
@@ -70,7 +70,8 @@ def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, see
         fit["s"] += time.perf_counter() - t
 
     screen = make_screen(ens, ev.target, lo, hi, nstim, keep_frac=keep / oversample,
-                         disagree_frac=disagree / oversample, explore=explore)
+                         disagree_frac=disagree / oversample, explore=explore, random_frac=random / oversample,
+                         seed=seed)
     say(f"screened GA: oversample {oversample}, simulate the best {keep:.0%} + most uncertain {disagree:.0%} "
         f"of a generation, surrogate retrained every generation")
     scr = ga.run_ga(ev, mu=mu_screen or pop, lam=pop, ngen=max_screen_gens or 10 * gens, seed=seed, screen=screen,
@@ -86,7 +87,8 @@ def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, see
                   "evals_per_s": plain_evals / plain_t, "history": plain.history},
         "screened": {"best_score": scr.best_score, "evals": ev.nevals, "surrogate_fit_s": fit["s"],
                      "surrogate_fits": fit["n"], "final_loss": fit["loss"], "oversample": oversample,
-                     "keep": keep, "disagree": disagree, "explore": explore, "mu": mu_screen or pop,
+                     "keep": keep, "disagree": disagree, "explore": explore, "random": random,
+                     "mu": mu_screen or pop,
                      "history": scr_hist,
                      "wall_s_to_reach_plain_best": reached, "evals_per_s": ev.nevals / scr_hist[-1][1]},
         "time_ratio": (reached / plain_t) if reached else None,
@@ -109,10 +111,12 @@ def main():
     ap.add_argument("--explore", type=float, default=0.0)
     ap.add_argument("--max-train", type=int, default=4096)
     ap.add_argument("--mu-screen", type=int, default=None, help="parents of the screened GA (default --pop)")
+    ap.add_argument("--random", type=float, default=0.0,
+                    help="fraction of a generation simulated unscreened (uniform over the candidates)")
     args = ap.parse_args()
     r = study(args.workload, args.pop, args.gens, args.nt, args.members, args.epochs, seed=args.seed,
               oversample=args.oversample, keep=args.keep, disagree=args.disagree, refit_epochs=args.refit_epochs,
-              explore=args.explore, max_train=args.max_train, mu_screen=args.mu_screen)
+              explore=args.explore, max_train=args.max_train, mu_screen=args.mu_screen, random=args.random)
     print(json.dumps(r), flush=True)
 
 

@@ -204,12 +204,15 @@ def predicted_scores(ens: MLPEnsemble, x_in: np.ndarray, target: Target, nstim: 
 
 
 def make_screen(ens: MLPEnsemble, target: Target, lo, hi, nstim: int, keep_frac: float = 0.1,
-                disagree_frac: float = 0.02, explore: float = 0.0):
+                disagree_frac: float = 0.02, explore: float = 0.0, random_frac: float = 0.0, seed: int = 0):
     """Screen for run_ga: keep the best ``keep_frac`` by predicted score minus
     ``explore`` times the ensemble disagreement (an optimistic bound; 0 ranks by
     the ensemble mean) plus the ``disagree_frac`` most uncertain candidates
     (the disagreement gate: where the ensemble does not agree the candidate
-    goes to full simulation)."""
+    goes to full simulation) plus ``random_frac`` drawn uniformly from the rest
+    (unscreened exploration)."""
+    rng = np.random.default_rng(seed)
+
     def screen(genomes: np.ndarray) -> np.ndarray:
         s, d = predicted_scores(ens, genome_to_input(genomes, lo, hi), target, nstim)
         n = genomes.shape[0]
@@ -217,5 +220,10 @@ def make_screen(ens: MLPEnsemble, target: Target, lo, hi, nstim: int, keep_frac:
         best = np.argsort(s - explore * d, kind="stable")[:k]
         kd = int(round(disagree_frac * n))
         unsure = np.argsort(-d, kind="stable")[:kd] if kd > 0 else np.array([], dtype=int)
-        return np.unique(np.concatenate([best, unsure]))
+        picked = np.unique(np.concatenate([best, unsure]))
+        kr = int(round(random_frac * n))
+        if kr > 0:
+            rest = np.setdiff1d(np.arange(n), picked)
+            picked = np.concatenate([picked, rng.choice(rest, size=min(kr, rest.size), replace=False)])
+        return picked
     return screen
