@@ -11,8 +11,11 @@ uncertain ``--disagree`` (the disagreement gate) are simulated.  Every
 simulated batch is added to the training set and the ensemble continues
 training on it (``--refit-epochs`` per generation), so the surrogate learns
 where the search goes.  The run stops when its best simulated score reaches
-S*; all of its time (training included) counts.  Prints one JSON line with
-both histories, the time to reach S*, and evaluations per second.
+S*; all of its time (training included) counts.  ``--pretrain N`` first
+simulates N seeded log-uniform individuals and trains the ensemble on them
+(the offline training set of the study at population 16,384); that time
+counts as well.  Prints one JSON line with both histories, the time to reach
+S*, and evaluations per second.
 """
 from __future__ import annotations
 
@@ -28,7 +31,7 @@ from .surrogate import MLPEnsemble, feature_targets, genome_to_input, make_scree
 
 def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, seed=1178, device=0,
           oversample=20, keep=0.15, disagree=0.03, refit_epochs=10, explore=0.0, max_train=4096,
-          max_screen_gens=None, mu_screen=None, random=0.0):
+          max_screen_gens=None, mu_screen=None, random=0.0, pretrain=0):
     w = models.workload(name, nsets=1, nt=nt)
     m = w.model
     dt = float(w.dts[0])
@@ -52,6 +55,22 @@ def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, see
     ens = MLPEnsemble(lo.size, nstim * len(ga.GA_FEATURES), hidden=256, members=members, seed=seed, device=device)
     data_g, data_y = [], []
     fit = {"s": 0.0, "n": 0, "loss": float("nan")}
+    t_pre = time.perf_counter()
+    if pretrain > 0:
+        # offline training set: seeded log-uniform samples, simulated in batches
+        # of one population; its time counts towards the screened GA's
+        prng = np.random.default_rng(seed + 7919)
+        for b0 in range(0, pretrain, pop):
+            g = prng.uniform(lo, hi, size=(min(pop, pretrain - b0), lo.size))
+            _, feats = ev.evaluate(10 ** g)
+            data_g.append(genome_to_input(g, lo, hi))
+            data_y.append(feature_targets(feats))
+            say(f"pretraining set: {b0 + g.shape[0]} of {pretrain} simulated")
+        t = time.perf_counter()
+        losses = ens.fit(np.vstack(data_g), np.vstack(data_y), epochs=epochs, batch=512, lr=2e-3)
+        fit.update(s=time.perf_counter() - t, n=1, loss=float(losses[:, -1].mean()))
+        say(f"surrogate trained on {pretrain} samples in {fit['s']:.1f} s, loss {fit['loss']:.4f}")
+    pre_s = time.perf_counter() - t_pre
 
     def observe(genomes, feats):
         """Every simulated batch joins the training set; the ensemble trains
@@ -75,9 +94,10 @@ def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, see
     say(f"screened GA: oversample {oversample}, simulate the best {keep:.0%} + most uncertain {disagree:.0%} "
         f"of a generation, surrogate retrained every generation")
     scr = ga.run_ga(ev, mu=mu_screen or pop, lam=pop, ngen=max_screen_gens or 10 * gens, seed=seed, screen=screen,
-                    oversample=oversample, stop_score=s_star, time_limit=plain_t, log=say, observe=observe,
+                    oversample=oversample, stop_score=s_star, time_limit=max(plain_t - pre_s, 0.0), log=say,
+                    observe=observe,
                     n_init=pop)
-    scr_hist = list(scr.history)
+    scr_hist = [(g, t + pre_s, b, n) for g, t, b, n in scr.history]
     reached = next((t for _, t, s, _ in scr_hist if s <= s_star), None)
     ens.close()
     ev.close()
@@ -87,6 +107,7 @@ def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, see
                   "evals_per_s": plain_evals / plain_t, "history": plain.history},
         "screened": {"best_score": scr.best_score, "evals": ev.nevals, "surrogate_fit_s": fit["s"],
                      "surrogate_fits": fit["n"], "final_loss": fit["loss"], "oversample": oversample,
+                     "pretrain": pretrain, "pretrain_s": pre_s,
                      "keep": keep, "disagree": disagree, "explore": explore, "random": random,
                      "mu": mu_screen or pop,
                      "history": scr_hist,
@@ -111,12 +132,15 @@ def main():
     ap.add_argument("--explore", type=float, default=0.0)
     ap.add_argument("--max-train", type=int, default=4096)
     ap.add_argument("--mu-screen", type=int, default=None, help="parents of the screened GA (default --pop)")
+    ap.add_argument("--pretrain", type=int, default=0,
+                    help="random individuals simulated to train the surrogate before the screened GA (time counted)")
     ap.add_argument("--random", type=float, default=0.0,
                     help="fraction of a generation simulated unscreened (uniform over the candidates)")
     args = ap.parse_args()
     r = study(args.workload, args.pop, args.gens, args.nt, args.members, args.epochs, seed=args.seed,
               oversample=args.oversample, keep=args.keep, disagree=args.disagree, refit_epochs=args.refit_epochs,
-              explore=args.explore, max_train=args.max_train, mu_screen=args.mu_screen, random=args.random)
+              explore=args.explore, max_train=args.max_train, mu_screen=args.mu_screen, random=args.random,
+              pretrain=args.pretrain)
     print(json.dumps(r), flush=True)
 
 
