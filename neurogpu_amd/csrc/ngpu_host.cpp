@@ -25,6 +25,7 @@
 
 #include "../../include/neurogpu.h"
 #include "sim_kernel.h"
+#include "schedule.h"
 
 /* Per-set launchers, one translation unit each (sim_<set>.hip). */
 hipError_t ngpu_launch_passive(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
@@ -52,14 +53,6 @@ static int param_owner(int ms, int j)
     for (int m = 0; m < nm; m++)
         if (j >= b[m] && j < b[m + 1]) return m;
     return -1;
-}
-/* Masks whose mechanism code is specialised (the dispatch lists of mechs.h);
- * used by the schedule's cost estimate only. */
-static bool mask_fixed(int ms, unsigned m)
-{
-    if (ms == NGPU_MECH_TTPC) return m == 0x408u || m == 0x1618u;
-    if (ms == NGPU_MECH_MAINEN) return m == 0x6fu;
-    return m <= 1u;
 }
 static int mech_nstate(int ms) { return ms == NGPU_MECH_TTPC ? SetTTPC::NSTATE : ms == NGPU_MECH_MAINEN ? SetMainen::NSTATE : SetPassive::NSTATE; }
 
@@ -655,47 +648,12 @@ static int enqueue(ngpu_ctx *c, int &launches)
 
 static int prepare_layout(ngpu_ctx *c, int W, int R);
 
-/* The mechanism pass's work of a group: segments (chunk of 64 positions,
- * runs r0 .. r0+n-1 of the group) assigned to its W waves, longest first to
- * the least loaded wave.  A chunk's cost is the sum over the distinct masks
- * of its positions of (mechanisms + 1), since a wave runs each mask's code in
- * turn; a chunk of specialised masks is cut into segments of up to three
- * runs (the kernel loads a run's states while it computes the previous run),
- * one with run-time-mask code into single runs.  Layout: offsets[W + 1], then
- * the segments packed as chunk << 8 | r0 << 4 | n. */
+/* The mechanism pass's segment schedule for groups of W waves and R runs
+ * (ng_mech_schedule, schedule.h), uploaded once per shape. */
 static int mech_schedule(ngpu_ctx *c, int W, int R)
 {
     if (c->msched_W == W && c->msched_R == R) return NGPU_OK;
-    const int NC = c->lay_NQ / 64;
-    struct Seg { int packed; double cost; };
-    std::vector<Seg> segs;
-    for (int k = 0; k < NC; k++) {
-        std::vector<unsigned> ms;
-        bool generic = false;
-        for (int q = 64 * k; q < std::min(64 * k + 64, c->N); q++) {
-            const unsigned m = (unsigned)c->qmask[q];
-            if (std::find(ms.begin(), ms.end(), m) == ms.end()) ms.push_back(m);
-            generic = generic || !mask_fixed(c->mechset, m);
-        }
-        double cost = 0.0;
-        for (unsigned m : ms) cost += __builtin_popcount(m) + 1;
-        const int L = generic ? 1 : 3;
-        for (int r0 = 0; r0 < R; r0 += L) {
-            const int n = std::min(L, R - r0);
-            segs.push_back({k << 8 | r0 << 4 | n, cost * n});
-        }
-    }
-    std::stable_sort(segs.begin(), segs.end(), [](const Seg &x, const Seg &y) { return x.cost > y.cost; });
-    std::vector<double> load(W, 0.0);
-    std::vector<std::vector<int>> per(W);
-    for (const Seg &sg : segs) {
-        const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        load[w] += sg.cost;
-        per[w].push_back(sg.packed);
-    }
-    c->msched.assign(1, 0);
-    for (int w = 0; w < W; w++) c->msched.push_back(c->msched[w] + (int)per[w].size());
-    for (int w = 0; w < W; w++) c->msched.insert(c->msched.end(), per[w].begin(), per[w].end());
+    c->msched = ng_mech_schedule(c->qmask, c->N, c->lay_NQ, c->mechset, W, R);
     int rc = upload(c, c->d_msched, c->msched);
     if (rc) return rc;
     c->msched_W = W; c->msched_R = R;

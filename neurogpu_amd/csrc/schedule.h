@@ -1,0 +1,71 @@
+/*
+ * schedule.h -- host-side work schedule of the simulation kernel's mechanism
+ * pass (plain C++, no HIP), shared by the library and its CPU tests.
+ *
+ * The positions of a run form NQ/64 chunks of 64 (positions sorted by
+ * mechanism mask).  A segment is (chunk, runs r0 .. r0+n-1 of a group); the
+ * W waves of a group take the segments assigned to them.  A chunk's cost is
+ * the sum over the distinct masks of its positions of (mechanisms + 1),
+ * since a wave runs each mask's code in turn.  A chunk whose masks all have
+ * specialised code is cut into segments of up to three runs (the kernel
+ * loads a run's states while it computes the previous run); a chunk with a
+ * run-time-mask position into single runs.  Segments go longest first to the
+ * least loaded wave.
+ *
+ * Layout of the result: offsets[W + 1] into the segment list, then the
+ * segments packed as chunk << 8 | r0 << 4 | n (r0 < 16, n < 16).
+ */
+#ifndef NGPU_SCHEDULE_H
+#define NGPU_SCHEDULE_H
+
+#include <algorithm>
+#include <vector>
+
+/* Masks whose mechanism code is specialised in mechs.h (SET::dispatch), by
+ * mechanism set: 0 passive, 1 TTPC, 2 Mainen.  Used for the cost estimate and
+ * the segment length only. */
+inline bool ng_mask_fixed(int mechset, unsigned m)
+{
+    if (mechset == 1) return m == 0x408u || m == 0x1618u;
+    if (mechset == 2) return m == 0x6fu;
+    return m <= 1u;
+}
+
+/* qmask[q]: mechanism mask of position q (q < N); NQ a multiple of 64; W waves
+ * per group, R runs per group (1..15). */
+inline std::vector<int> ng_mech_schedule(const std::vector<int> &qmask, int N, int NQ, int mechset, int W, int R)
+{
+    const int NC = NQ / 64;
+    struct Seg { int packed; double cost; };
+    std::vector<Seg> segs;
+    for (int k = 0; k < NC; k++) {
+        std::vector<unsigned> ms;
+        bool generic = false;
+        for (int q = 64 * k; q < std::min(64 * k + 64, N); q++) {
+            const unsigned m = (unsigned)qmask[q];
+            if (std::find(ms.begin(), ms.end(), m) == ms.end()) ms.push_back(m);
+            generic = generic || !ng_mask_fixed(mechset, m);
+        }
+        double cost = 0.0;
+        for (unsigned m : ms) cost += __builtin_popcount(m) + 1;
+        const int L = generic ? 1 : 3;
+        for (int r0 = 0; r0 < R; r0 += L) {
+            const int n = std::min(L, R - r0);
+            segs.push_back({k << 8 | r0 << 4 | n, cost * n});
+        }
+    }
+    std::stable_sort(segs.begin(), segs.end(), [](const Seg &x, const Seg &y) { return x.cost > y.cost; });
+    std::vector<double> load(W, 0.0);
+    std::vector<std::vector<int>> per(W);
+    for (const Seg &sg : segs) {
+        const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        load[w] += sg.cost;
+        per[w].push_back(sg.packed);
+    }
+    std::vector<int> out(1, 0);
+    for (int w = 0; w < W; w++) out.push_back(out[w] + (int)per[w].size());
+    for (int w = 0; w < W; w++) out.insert(out.end(), per[w].begin(), per[w].end());
+    return out;
+}
+
+#endif

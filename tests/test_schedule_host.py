@@ -1,0 +1,84 @@
+"""The mechanism-pass schedule (neurogpu_amd/csrc/schedule.h), compiled for
+the host: for the workloads' trees and every launch shape, each (chunk of 64
+positions, run of the group) is covered by exactly one segment, segments of a
+chunk with run-time-mask positions cover one run, and the longest-first
+assignment keeps every wave's estimated cost within one segment of the
+mean."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from neurogpu_amd import models
+
+_IP = C.POINTER(C.c_int)
+FIXED = {1: {0x408, 0x1618}, 2: {0x6F}, 0: {0, 1}}
+
+
+@pytest.fixture(scope="module")
+def lib(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("sched") / "libschedule_host.so")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", os.path.join(ROOT, "neurogpu_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "schedule_host.cpp"), "-o", out], check=True)
+    L = C.CDLL(out)
+    L.host_mech_schedule.restype = C.c_int
+    L.host_mech_schedule.argtypes = [_IP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _IP, C.c_int]
+    return L
+
+
+def _positions_masks(name):
+    m = models.workload(name, nsets=1, nt=10).model
+    t = m.tree
+    bm = np.asarray(t.bool_model).reshape(-1, t.N)
+    masks = (bm.astype(np.int64) << np.arange(bm.shape[0])[:, None]).sum(0)
+    return np.sort(masks, kind="stable").astype(np.int32), t.N, m.mechset_id
+
+
+def _schedule(lib, qmask, N, ms, W, R):
+    NQ = (N + 63) // 64 * 64
+    out = np.zeros(4096, np.int32)
+    n = lib.host_mech_schedule(qmask.ctypes.data_as(_IP), N, NQ, ms, W, R, out.ctypes.data_as(_IP), out.size)
+    assert n > 0
+    return out[:n], NQ
+
+
+def _cost(qmask, N, k):
+    ms = set(int(x) for x in qmask[64 * k:min(64 * k + 64, N)])
+    return sum(bin(m).count("1") + 1 for m in ms)
+
+
+@pytest.mark.parametrize("name", ["passive64", "active500", "ttpc1200"])
+@pytest.mark.parametrize("W,R", [(1, 1), (1, 2), (2, 3), (8, 6), (16, 6), (16, 4), (4, 8)])
+def test_every_chunk_and_run_once(lib, name, W, R):
+    qmask, N, ms = _positions_masks(name)
+    s, NQ = _schedule(lib, qmask, N, ms, W, R)
+    off, seg = s[:W + 1], s[W + 1:]
+    assert off[0] == 0 and off[-1] == seg.size and np.all(np.diff(off) >= 0)
+    covered = np.zeros((NQ // 64, R), np.int32)
+    load = np.zeros(W)
+    for w in range(W):
+        for d in seg[off[w]:off[w + 1]]:
+            c, r0, n = d >> 8, (d >> 4) & 15, d & 15
+            assert 0 <= c < NQ // 64 and n >= 1 and r0 + n <= R
+            covered[c, r0:r0 + n] += 1
+            generic = any(int(m) not in FIXED[ms] for m in qmask[64 * c:min(64 * c + 64, N)])
+            assert n <= (1 if generic else 3)
+            load[w] += _cost(qmask, N, c) * n
+    np.testing.assert_array_equal(covered, 1)
+    biggest = max(_cost(qmask, N, k) for k in range(NQ // 64)) * 3
+    assert load.max() <= load.mean() + biggest
+
+
+def test_soma_chunk_runs_spread_over_waves(lib):
+    """The expensive soma/axon chunk of ttpc1200: its six runs land on six
+    different waves of a 16-wave group (one wave holding them all tripled the
+    mechanism pass)."""
+    qmask, N, ms = _positions_masks("ttpc1200")
+    s, NQ = _schedule(lib, qmask, N, 1, 16, 6)
+    off, seg = s[:17], s[17:]
+    last = NQ // 64 - 1
+    waves = [w for w in range(16) for d in seg[off[w]:off[w + 1]] if d >> 8 == last]
+    assert len(waves) == 6 and len(set(waves)) == 6
