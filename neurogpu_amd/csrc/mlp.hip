@@ -8,7 +8,8 @@
  * accumulation on v_mfma_f32_32x32x16_bf16 (gfx950): a 256-thread block holds
  * a 64x64 tile of C, each wave a 32x32 sub-tile; operands are read straight
  * from global memory (the matrices here are at most a few hundred wide) with
- * bounds checks, so no size needs padding.  The epilogue optionally adds a
+ * bounds checks, so no size needs padding, four k-steps ahead of the MFMA
+ * that uses them.  The epilogue optionally adds a
  * bias per column, applies ReLU, multiplies by a ReLU mask (backward), and
  * writes FP32 and/or BF16.
  */
@@ -63,8 +64,8 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
 #pragma unroll
     for (int i = 0; i < 16; i++) acc[i] = 0.0f;
     const int am = m0 + row, bn = n0 + row;
-    for (int k0 = 0; k0 < g.K; k0 += 16) {
-        bf16x8 a, b;
+    /* operands of k-steps k0 .. k0 + 16 (zero beyond K or outside the matrix) */
+    auto load = [&](int k0, bf16x8 &a, bf16x8 &b) {
 #pragma unroll
         for (int e = 0; e < 8; e++) {
             const int k = k0 + kh + e;
@@ -74,7 +75,19 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
             a[e] = (short)av;
             b[e] = (short)bv;
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    };
+    /* four k-steps in flight: a step's slot is refilled with the operands of
+     * the step four ahead as soon as its MFMA has issued */
+    constexpr int PF = 4;
+    bf16x8 ra[PF], rb[PF];
+#pragma unroll
+    for (int i = 0; i < PF; i++) load(16 * i, ra[i], rb[i]);
+    for (int k0 = 0; k0 < g.K; k0 += 16 * PF) {
+#pragma unroll
+        for (int i = 0; i < PF; i++) {
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ra[i], rb[i], acc, 0, 0, 0);
+            load(k0 + 16 * (i + PF), ra[i], rb[i]);
+        }
     }
     /* accumulator i of this lane: row 8*(i/4) + 4*(lane/32) + i%4, column lane%32 */
     const int col = n0 + (lane & 31);
@@ -129,14 +142,26 @@ __global__ void mse_grad_kernel(const float *Y, const float *T, int n, float sca
     if (threadIdx.x == 0) atomicAdd(loss, part[0] * scale);
 }
 
-/* column sums of an [rows, cols] matrix (fp32 or bf16) -> out[cols] */
-__global__ void colsum_kernel(const float *x32, const uint16_t *x16, int rows, int cols, float *out)
+/* column sums of an [rows, cols] matrix (fp32 or bf16) -> out[cols]: a
+ * block of 64 columns x 16 row groups (coalesced 64-wide row reads, each
+ * thread summing every 16th row), then the 16 partial sums in a fixed order */
+#define COLSUM_RG 16
+__global__ void __launch_bounds__(64 * COLSUM_RG) colsum_kernel(const float *x32, const uint16_t *x16, int rows, int cols,
+                                                                float *out)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= cols) return;
+    __shared__ float part[COLSUM_RG][64];
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int c = blockIdx.x * 64 + tx;
     float s = 0.0f;
-    for (int r = 0; r < rows; r++) s += x32 ? x32[(size_t)r * cols + c] : bf2f(x16[(size_t)r * cols + c]);
-    out[c] = s;
+    if (c < cols)
+        for (int r = ty; r < rows; r += COLSUM_RG) s += x32 ? x32[(size_t)r * cols + c] : bf2f(x16[(size_t)r * cols + c]);
+    part[ty][tx] = s;
+    __syncthreads();
+    if (ty == 0 && c < cols) {
+        float t = 0.0f;
+        for (int k = 0; k < COLSUM_RG; k++) t += part[k][tx];
+        out[c] = t;
+    }
 }
 
 __global__ void adam_kernel(float *w, const float *gr, float *m, float *v, uint16_t *w16, int n, float lr, float b1,
@@ -154,6 +179,27 @@ __global__ void adam_kernel(float *w, const float *gr, float *m, float *v, uint1
     }
 }
 
+/* rows idx[0..batch) of the device training set -> the batch buffers */
+__global__ void gather_batch_kernel(const uint16_t *xall, const float *tall, const int *idx, int nin, int nout, int batch,
+                                    uint16_t *x16, float *t32)
+{
+    const int nx = batch * nin, nt = batch * nout;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nx + nt; i += gridDim.x * blockDim.x) {
+        if (i < nx) {
+            const int r = i / nin, c = i - r * nin;
+            x16[i] = xall[(size_t)idx[r] * nin + c];
+        } else {
+            const int j = i - nx, r = j / nout, c = j - r * nout;
+            t32[j] = tall[(size_t)idx[r] * nout + c];
+        }
+    }
+}
+
+__global__ void add_scalar_kernel(const float *x, float *acc)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *acc += *x;
+}
+
 /* ------------------------------------------------------------------ host */
 
 struct ngpu_mlp {
@@ -169,6 +215,12 @@ struct ngpu_mlp {
     int cap = 0;
     uint16_t *x16 = nullptr, *a16[3] = {nullptr, nullptr, nullptr}, *d16[3] = {nullptr, nullptr, nullptr}, *dy16 = nullptr;
     float *y32 = nullptr, *t32 = nullptr, *dy32 = nullptr, *loss = nullptr;
+    /* training set on the device (bf16 inputs, fp32 targets), the epoch's row
+     * order and the per-epoch loss sums */
+    size_t cap_rows = 0, cap_lacc = 0;
+    uint16_t *xall16 = nullptr;
+    float *tall = nullptr, *lacc = nullptr;
+    int *didx = nullptr;
 };
 
 static int mfail(ngpu_mlp *p, int code, const std::string &msg)
@@ -234,7 +286,8 @@ extern "C" void ngpu_mlp_close(ngpu_mlp *p)
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     for (void *q : {(void *)p->w, (void *)p->g, (void *)p->m, (void *)p->v, (void *)p->w16, (void *)p->loss, (void *)p->x16,
-                    (void *)p->dy16, (void *)p->y32, (void *)p->t32, (void *)p->dy32})
+                    (void *)p->dy16, (void *)p->y32, (void *)p->t32, (void *)p->dy32, (void *)p->xall16, (void *)p->tall,
+                    (void *)p->lacc, (void *)p->didx})
         if (q) (void)hipFree(q);
     for (int i = 0; i < 3; i++) { (void)hipFree(p->a16[i]); (void)hipFree(p->d16[i]); }
     if (p->stream) (void)hipStreamDestroy(p->stream);
@@ -300,7 +353,7 @@ static int backward(ngpu_mlp *p, int mem, int n)
     /* layer 4: dW4 = dY^T A3, db4 = colsum(dY), dZ3 = (dY W4) * (A3 > 0) */
     GemmArgs g{O, H, n, O, H, H, p->dy16, p->a16[2], nullptr, nullptr, G + p->off[6], nullptr, 0};
     MCHK(p, gemm(p->stream, true, false, g));
-    hipLaunchKernelGGL(colsum_kernel, dim3((O + 63) / 64), dim3(64), 0, p->stream, p->dy32, (const uint16_t *)nullptr, n, O,
+    hipLaunchKernelGGL(colsum_kernel, dim3((O + 63) / 64), dim3(64 * COLSUM_RG), 0, p->stream, p->dy32, (const uint16_t *)nullptr, n, O,
                        G + p->off[7]);
     g = GemmArgs{n, H, O, O, H, H, p->dy16, W16 + p->off[6], nullptr, p->a16[2], nullptr, p->d16[2], 0};
     MCHK(p, gemm(p->stream, false, false, g));
@@ -309,7 +362,7 @@ static int backward(ngpu_mlp *p, int mem, int n)
         const int K = l > 0 ? H : p->nin;
         g = GemmArgs{H, K, n, H, K, K, p->d16[l], prev, nullptr, nullptr, G + p->off[2 * l], nullptr, 0};
         MCHK(p, gemm(p->stream, true, false, g));
-        hipLaunchKernelGGL(colsum_kernel, dim3((H + 63) / 64), dim3(64), 0, p->stream, (const float *)nullptr, p->d16[l], n,
+        hipLaunchKernelGGL(colsum_kernel, dim3((H + 63) / 64), dim3(64 * COLSUM_RG), 0, p->stream, (const float *)nullptr, p->d16[l], n,
                            H, G + p->off[2 * l + 1]);
         if (l > 0) {
             g = GemmArgs{n, H, H, H, H, H, p->d16[l], W16 + p->off[2 * l], nullptr, p->a16[l - 1], nullptr, p->d16[l - 1], 0};
@@ -367,42 +420,62 @@ extern "C" int ngpu_mlp_train(ngpu_mlp *p, const float *x, const float *t, int n
     batch = std::min(batch, n);
     int rc;
     if ((rc = ensure_cap(p, batch))) return rc;
+    /* the training set stays on the device; each epoch uploads its row order
+     * and each step gathers its rows there, so the steps of an epoch run
+     * without a host round trip (the loss sums are read once at the end) */
+    if ((size_t)n > p->cap_rows) {
+        (void)hipFree(p->xall16); (void)hipFree(p->tall); (void)hipFree(p->didx);
+        p->xall16 = nullptr; p->tall = nullptr; p->didx = nullptr; p->cap_rows = 0;
+        MCHK(p, hipMalloc((void **)&p->xall16, (size_t)n * p->nin * 2));
+        MCHK(p, hipMalloc((void **)&p->tall, (size_t)n * p->nout * 4));
+        MCHK(p, hipMalloc((void **)&p->didx, (size_t)n * 4));
+        p->cap_rows = n;
+    }
+    const size_t nl = (size_t)p->members * std::max(epochs, 1);
+    if (nl > p->cap_lacc) {
+        (void)hipFree(p->lacc);
+        p->lacc = nullptr; p->cap_lacc = 0;
+        MCHK(p, hipMalloc((void **)&p->lacc, nl * 4));
+        p->cap_lacc = nl;
+    }
     std::vector<uint16_t> xb((size_t)n * p->nin);
     for (size_t i = 0; i < xb.size(); i++) xb[i] = f2bf(x[i]);
-    std::vector<uint16_t> bx((size_t)batch * p->nin);
-    std::vector<float> bt((size_t)batch * p->nout);
+    MCHK(p, hipMemcpyAsync(p->xall16, xb.data(), xb.size() * 2, hipMemcpyHostToDevice, p->stream));
+    MCHK(p, hipMemcpyAsync(p->tall, t, (size_t)n * p->nout * 4, hipMemcpyHostToDevice, p->stream));
+    MCHK(p, hipMemsetAsync(p->lacc, 0, nl * 4, p->stream));
     std::vector<int> idx(n);
     for (int i = 0; i < n; i++) idx[i] = i;
+    const int nb = n / batch;
+    const int gb = std::min(1024, (batch * (p->nin + p->nout) + 255) / 256);
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
     for (int mem = 0; mem < p->members; mem++) {
         std::mt19937 rng(seed * 1009u + (unsigned)mem);
         long long step = 0;
         for (int ep = 0; ep < epochs; ep++) {
             std::shuffle(idx.begin(), idx.end(), rng);
-            double esum = 0.0;
-            int nb = 0;
+            MCHK(p, hipMemcpyAsync(p->didx, idx.data(), (size_t)n * 4, hipMemcpyHostToDevice, p->stream));
             for (int s0 = 0; s0 + batch <= n; s0 += batch) {
-                for (int r = 0; r < batch; r++) {
-                    const int i = idx[s0 + r];
-                    std::memcpy(&bx[(size_t)r * p->nin], &xb[(size_t)i * p->nin], p->nin * 2);
-                    std::memcpy(&bt[(size_t)r * p->nout], &t[(size_t)i * p->nout], p->nout * 4);
-                }
-                MCHK(p, hipMemcpyAsync(p->x16, bx.data(), bx.size() * 2, hipMemcpyHostToDevice, p->stream));
-                MCHK(p, hipMemcpyAsync(p->t32, bt.data(), bt.size() * 4, hipMemcpyHostToDevice, p->stream));
+                hipLaunchKernelGGL(gather_batch_kernel, dim3(gb), dim3(256), 0, p->stream, p->xall16, p->tall, p->didx + s0,
+                                   p->nin, p->nout, batch, p->x16, p->t32);
                 if ((rc = forward(p, mem, batch)) || (rc = backward(p, mem, batch))) return rc;
                 step++;
                 const float c1 = 1.0f - std::pow(b1, (float)step), c2 = 1.0f - std::pow(b2, (float)step);
                 const size_t o = (size_t)mem * p->nparam;
                 hipLaunchKernelGGL(adam_kernel, dim3(256), dim3(256), 0, p->stream, p->w + o, p->g + o, p->m + o, p->v + o,
                                    p->w16 + o, (int)p->nparam, lr, b1, b2, eps, c1, c2);
-                float l = 0.0f;
-                MCHK(p, hipMemcpyAsync(&l, p->loss + mem, 4, hipMemcpyDeviceToHost, p->stream));
-                MCHK(p, hipStreamSynchronize(p->stream));
-                esum += l;
-                nb++;
+                hipLaunchKernelGGL(add_scalar_kernel, dim3(1), dim3(64), 0, p->stream, p->loss + mem,
+                                   p->lacc + (size_t)mem * epochs + ep);
             }
-            if (losses_out) losses_out[(size_t)mem * epochs + ep] = nb ? (float)(esum / nb) : 0.0f;
+            /* the next epoch rewrites the host row order */
+            MCHK(p, hipStreamSynchronize(p->stream));
         }
+    }
+    MCHK(p, hipGetLastError());
+    MCHK(p, hipStreamSynchronize(p->stream));    /* xb and t are read by the uploads */
+    if (losses_out && epochs > 0) {
+        std::vector<float> l(nl);
+        MCHK(p, hipMemcpy(l.data(), p->lacc, nl * 4, hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < nl; i++) losses_out[i] = nb ? l[i] / (float)nb : 0.0f;
     }
     return NGPU_OK;
 }
