@@ -216,24 +216,30 @@ class GAResult:
 def run_ga(evaluator: Evaluator, mu: int, lam: int, ngen: int, seed: int = 1178,
            screen: Optional[Callable] = None, oversample: int = 1, time_limit: Optional[float] = None,
            stop_score: Optional[float] = None, log: Optional[Callable] = None,
-           observe: Optional[Callable] = None, n_init: Optional[int] = None) -> GAResult:
+           observe: Optional[Callable] = None, n_init: Optional[int] = None,
+           init: Optional[tuple] = None) -> GAResult:
     """(mu + lambda) GA in log10 parameter space.  With ``screen`` the GA draws
     ``oversample * lam`` offspring, and ``screen(genomes)`` returns the indices
     of those to simulate (the surrogate pre-screen); every score that enters
     selection comes from the simulator.  ``observe(genomes, features)`` sees
     every simulated batch (the screened GA's surrogate learns from them).
     ``n_init`` random individuals (default ``mu``) are simulated first and
-    the best ``mu`` of them form the first population."""
+    the best ``mu`` of them form the first population; ``init`` = (genomes,
+    simulated scores) gives the first population instead (individuals already
+    simulated, e.g. the surrogate's training set)."""
     m = evaluator.m
     rng = np.random.default_rng(seed)
     lo = np.log10([fp.lo for fp in m.free])
     hi = np.log10([fp.hi for fp in m.free])
-    pop = rng.uniform(lo, hi, size=(n_init or mu, lo.size))
     t0 = time.perf_counter()
-    obj, feat = evaluator.evaluate(10 ** pop)
-    if observe is not None:
-        observe(pop, feat)
-    score = obj.sum(axis=1)
+    if init is not None:
+        pop, score = np.asarray(init[0]), np.asarray(init[1])
+    else:
+        pop = rng.uniform(lo, hi, size=(n_init or mu, lo.size))
+        obj, feat = evaluator.evaluate(10 ** pop)
+        if observe is not None:
+            observe(pop, feat)
+        score = obj.sum(axis=1)
     first = np.argsort(score, kind="stable")[:mu]
     pop, score = pop[first], score[first]
     hist = [(0, time.perf_counter() - t0, float(score.min()), evaluator.nevals)]

@@ -14,7 +14,8 @@ where the search goes.  The run stops when its best simulated score reaches
 S*; all of its time (training included) counts.  ``--pretrain N`` first
 simulates N seeded log-uniform individuals and trains the ensemble on them
 (the offline training set of the study at population 16,384); that time
-counts as well.  Prints one JSON line with both histories, the time to reach
+counts as well, and the screened GA then starts from the best ``--pop`` of
+those simulated individuals instead of simulating a population of its own.  Prints one JSON line with both histories, the time to reach
 S*, and evaluations per second.
 """
 from __future__ import annotations
@@ -56,21 +57,27 @@ def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, see
     data_g, data_y = [], []
     fit = {"s": 0.0, "n": 0, "loss": float("nan")}
     t_pre = time.perf_counter()
+    init = None
     if pretrain > 0:
         # offline training set: seeded log-uniform samples, simulated in batches
         # of one population; its time counts towards the screened GA's
         prng = np.random.default_rng(seed + 7919)
+        pre_g, pre_s = [], []
         for b0 in range(0, pretrain, pop):
             g = prng.uniform(lo, hi, size=(min(pop, pretrain - b0), lo.size))
-            _, feats = ev.evaluate(10 ** g)
+            obj, feats = ev.evaluate(10 ** g)
             data_g.append(genome_to_input(g, lo, hi))
             data_y.append(feature_targets(feats))
+            pre_g.append(g)
+            pre_s.append(obj.sum(axis=1))
             say(f"pretraining set: {b0 + g.shape[0]} of {pretrain} simulated")
+        # the screened GA starts from the best of these simulated individuals
+        init = (np.vstack(pre_g), np.concatenate(pre_s))
         t = time.perf_counter()
         losses = ens.fit(np.vstack(data_g), np.vstack(data_y), epochs=epochs, batch=512, lr=2e-3)
         fit.update(s=time.perf_counter() - t, n=1, loss=float(losses[:, -1].mean()))
         say(f"surrogate trained on {pretrain} samples in {fit['s']:.1f} s, loss {fit['loss']:.4f}")
-    pre_s = time.perf_counter() - t_pre
+    pre_t = time.perf_counter() - t_pre
 
     def observe(genomes, feats):
         """Every simulated batch joins the training set; the ensemble trains
@@ -94,10 +101,10 @@ def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, see
     say(f"screened GA: oversample {oversample}, simulate the best {keep:.0%} + most uncertain {disagree:.0%} "
         f"of a generation, surrogate retrained every generation")
     scr = ga.run_ga(ev, mu=mu_screen or pop, lam=pop, ngen=max_screen_gens or 10 * gens, seed=seed, screen=screen,
-                    oversample=oversample, stop_score=s_star, time_limit=max(plain_t - pre_s, 0.0), log=say,
-                    observe=observe,
+                    oversample=oversample, stop_score=s_star, time_limit=max(plain_t - pre_t, 0.0), log=say,
+                    observe=observe, init=init,
                     n_init=pop)
-    scr_hist = [(g, t + pre_s, b, n) for g, t, b, n in scr.history]
+    scr_hist = [(g, t + pre_t, b, n) for g, t, b, n in scr.history]
     reached = next((t for _, t, s, _ in scr_hist if s <= s_star), None)
     ens.close()
     ev.close()
@@ -107,7 +114,7 @@ def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, see
                   "evals_per_s": plain_evals / plain_t, "history": plain.history},
         "screened": {"best_score": scr.best_score, "evals": ev.nevals, "surrogate_fit_s": fit["s"],
                      "surrogate_fits": fit["n"], "final_loss": fit["loss"], "oversample": oversample,
-                     "pretrain": pretrain, "pretrain_s": pre_s,
+                     "pretrain": pretrain, "pretrain_s": pre_t,
                      "keep": keep, "disagree": disagree, "explore": explore, "random": random,
                      "mu": mu_screen or pop,
                      "history": scr_hist,
