@@ -616,6 +616,15 @@ static void fill_args(const ngpu_ctx *c, SimArgs &a, int W, int R, int staged, i
     a.msched = c->d_msched.p;
     a.chains = c->d_chains.p; a.lvl_ptr = c->d_lvl.p; a.fath = c->d_fath.p; a.flvl_ptr = c->d_flvl.p;
     a.fchild = c->d_fchild.p; a.lds_staging = staged; a.block_sync = block_sync;
+    /* the tail of the level schedule: levels with one chain and no fathers
+     * of their own, from the top down */
+    a.tail_level = c->depth + 1;
+    for (int L = c->depth; L >= 0; L--) {
+        const bool one = c->lvl_ptr[L + 1] - c->lvl_ptr[L] == 1;
+        const bool nof = L == 0 || c->flvl_ptr[L] == c->flvl_ptr[L - 1];
+        if (!(one && nof)) break;
+        a.tail_level = L;
+    }
     a.nchains = (int)c->chains.size() / 5; a.nfath = (int)c->fath.size() / 3; a.nfchild = (int)c->fchild.size();
     a.dts = c->d_dts.p; a.amps = c->d_amps.p; a.rec_q = c->d_rec.p;
     a.states = c->d_states.p; a.vhot = c->d_vhot.p;

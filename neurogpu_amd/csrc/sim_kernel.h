@@ -89,6 +89,8 @@ struct SimArgs {
                                             STAGE instantiation */
     int block_sync;                      /* batch kernel: synchronise at workgroup scope */
     int nchains, nfath, nfchild;         /* schedule sizes (staged into LDS as int16) */
+    int tail_level;                      /* levels tail_level..depth hold one chain each and no
+                                            fathers of their own (depth + 1: none) */
     const double *dts, *amps;
     const int *rec_q;
     double *states;                      /* [nruns][NSTATE + 2][NQ]: mechanism states, v, dv */
@@ -131,7 +133,7 @@ struct SyncDyn {
  *                          staged, else 0 (regions stay 16-byte aligned)
  *   QL[NQ]                 short4: row, parent row, son positions
  *   QM[NQ]                 unsigned short: mechanism mask
- *   CH[5*nchains], FA[3*nfath], FC[nfchild], LP[depth+2], FLP[depth+1] short */
+ *   CH[8*nchains], FA[3*nfath], FC[nfchild], LP[depth+2], FLP[depth+1] short */
 __host__ __device__ inline size_t ng_lds_per_run(int N, int npv, int nrec)
 {
     return (size_t)(2 * (N + 1) + ((npv + 1) & ~1) + 64 * nrec);
@@ -489,11 +491,39 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
          * cost more per row than the two divisions.  A lane's first task of
          * the next level is read from the (constant) chain table before the
          * level's barrier. */
-        const int D = a.depth;
+        const int D = a.depth, Lt = min(a.tail_level, D + 1);
+        /* a chain task: fold the related rows into the first row, then
+         * eliminate the rows in order */
+        auto elim = [&](const NgChain &c) {
+            double2 *XU2 = (double2 *)region(c.r);
+            const int i0 = c.i0, i1 = c.i1, tA = c.tA, tB = c.tB;
+            const int ta = max(tA, 0), tb = max(tB, 0), j0 = min(i0, i1 - 1);
+            double2 p0 = XU2[i0 - 1], pa = XU2[ta], ea = EF[ta], pb = XU2[tb], eb = EF[tb];
+            const int j1 = min(i0 + 1, i1 - 1);
+            double2 ef = EF[i0 - 1], pn = XU2[j0], ef1 = EF[j1 - 1], pn1 = XU2[j1];
+            /* all of them in flight together (not sunk into the branches) */
+            __asm__ volatile("" : "+v"(p0.x), "+v"(p0.y), "+v"(pa.x), "+v"(pa.y), "+v"(ea.x), "+v"(ea.y),
+                             "+v"(pb.x), "+v"(pb.y), "+v"(eb.x), "+v"(eb.y));
+            double u = p0.y, b = p0.x;
+            if (tA >= 0) {
+                ng_fold(pa.y, pa.x, ea.x, ea.y, u, b);
+                if (tB >= 0) ng_fold(pb.y, pb.x, eb.x, eb.y, u, b);
+                XU2[i0 - 1] = make_double2(b, u);
+            }
+            for (int i = i0; i < i1; i++) {
+                const int j = min(i + 2, i1 - 1);
+                const double2 ef2 = EF[j - 1], pn2 = XU2[j];
+                const double unew = pn.y - ef.y * (ef.x / u);
+                const double bnew = pn.x - (b * ef.x) / u;
+                XU2[i] = make_double2(bnew, unew);
+                u = unew; b = bnew;
+                ef = ef1; pn = pn1; ef1 = ef2; pn1 = pn2;
+            }
+        };
         int k0 = LP[0], nc = LP[1] - k0;
         int k0n = LP[min(1, D)], ncn = LP[min(1, D) + 1] - k0n;
         NgChain first = ng_chain(CH, min(g, R * nc - 1), k0, nc);
-        for (int L = 0; L <= D; L++) {
+        for (int L = 0; L < Lt; L++) {
             /* level L+2's range and level L+1's first task, read ahead */
             const int k0nn = LP[min(L + 2, D)], ncnn = LP[min(L + 2, D) + 1] - k0nn;
             const NgChain next = ng_chain(CH, min(g, R * ncn - 1), k0n, ncn);
@@ -516,33 +546,18 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
             for (int tk = g; tk < R * nc; tk += LW) {
                 NgChain c = first;
                 if (tk != g) c = ng_chain(CH, tk, k0, nc);
-                double2 *XU2 = (double2 *)region(c.r);
-                const int i0 = c.i0, i1 = c.i1, tA = c.tA, tB = c.tB;
-                const int ta = max(tA, 0), tb = max(tB, 0), j0 = min(i0, i1 - 1);
-                double2 p0 = XU2[i0 - 1], pa = XU2[ta], ea = EF[ta], pb = XU2[tb], eb = EF[tb];
-                const int j1 = min(i0 + 1, i1 - 1);
-                double2 ef = EF[i0 - 1], pn = XU2[j0], ef1 = EF[j1 - 1], pn1 = XU2[j1];
-                /* all of them in flight together (not sunk into the branches) */
-                __asm__ volatile("" : "+v"(p0.x), "+v"(p0.y), "+v"(pa.x), "+v"(pa.y), "+v"(ea.x), "+v"(ea.y),
-                                 "+v"(pb.x), "+v"(pb.y), "+v"(eb.x), "+v"(eb.y));
-                double u = p0.y, b = p0.x;
-                if (tA >= 0) {
-                    ng_fold(pa.y, pa.x, ea.x, ea.y, u, b);
-                    if (tB >= 0) ng_fold(pb.y, pb.x, eb.x, eb.y, u, b);
-                    XU2[i0 - 1] = make_double2(b, u);
-                }
-                for (int i = i0; i < i1; i++) {
-                    const int j = min(i + 2, i1 - 1);
-                    const double2 ef2 = EF[j - 1], pn2 = XU2[j];
-                    const double unew = pn.y - ef.y * (ef.x / u);
-                    const double bnew = pn.x - (b * ef.x) / u;
-                    XU2[i] = make_double2(bnew, unew);
-                    u = unew; b = bnew;
-                    ef = ef1; pn = pn1; ef1 = ef2; pn1 = pn2;
-                }
+                elim(c);
             }
             sync();
             first = next; k0 = k0n; nc = ncn; k0n = k0nn; ncn = ncnn;
+        }
+        /* the tail: one chain per level, each depending only on the chain
+         * below it of the same run, so lane r takes run r's chains of all
+         * tail levels in level order without a barrier between them */
+        if (Lt <= D) {
+            if (g < R)
+                for (int L = Lt; L <= D; L++) elim(ng_chain(CH, g, LP[L], 1));
+            sync();
         }
 
         /* pivot quotients of every row, in parallel: PX = b/u and PF = -f/u
@@ -562,41 +577,54 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
          * marks the root, x = PX), then x_i = PX_i + PF_i x_{i+1} down to the
          * chain's first row, four rows at a time with the next four rows'
          * quotients loaded ahead; x replaces PX */
-        k0 = LP[D]; nc = LP[D + 1] - k0;
-        k0n = LP[max(D - 1, 0)]; ncn = LP[max(D - 1, 0) + 1] - k0n;
-        first = ng_chain(CH, min(g, R * nc - 1), k0, nc);
-        for (int L = D; L >= 0; L--) {
+        auto bsub = [&](const NgChain &c) {
+            double *XU = region(c.r);
+            const double2 *XU2 = (const double2 *)XU;
+            const int h = c.i1 - 1, lo = c.i0 - 1, pe = c.pe;
+            const double2 ph = XU2[h];
+            const double xp = XU[2 * min(pe, N - 1)];
+            double2 a0 = XU2[max(h - 1, lo)], a1 = XU2[max(h - 2, lo)], a2 = XU2[max(h - 3, lo)],
+                    a3 = XU2[max(h - 4, lo)];
+            double x = ph.x;
+            if (pe < N) x = x + ph.y * xp;
+            XU[2 * h] = x;
+            int i = h - 1;
+            while (i - 3 >= lo) {
+                /* rows i .. i-3 from a0..a3; rows i-4 .. i-7 loaded (clamped
+                 * to the chain; a clamped value is not used) */
+                const double2 b0 = XU2[max(i - 4, lo)], b1 = XU2[max(i - 5, lo)], b2 = XU2[max(i - 6, lo)],
+                              b3 = XU2[max(i - 7, lo)];
+                x = a0.x + a0.y * x; XU[2 * i] = x;
+                x = a1.x + a1.y * x; XU[2 * (i - 1)] = x;
+                x = a2.x + a2.y * x; XU[2 * (i - 2)] = x;
+                x = a3.x + a3.y * x; XU[2 * (i - 3)] = x;
+                a0 = b0; a1 = b1; a2 = b2; a3 = b3;
+                i -= 4;
+            }
+            if (i >= lo) { x = a0.x + a0.y * x; XU[2 * i] = x; }
+            if (i - 1 >= lo) { x = a1.x + a1.y * x; XU[2 * (i - 1)] = x; }
+            if (i - 2 >= lo) { x = a2.x + a2.y * x; XU[2 * (i - 2)] = x; }
+        };
+        /* the tail first (top of the tree), lane r for run r, then the
+         * levels below it */
+        if (Lt <= D) {
+            if (g < R)
+                for (int L = D; L >= Lt; L--) bsub(ng_chain(CH, g, LP[L], 1));
+            sync();
+        }
+        const int Ls = min(Lt - 1, D);
+        if (Ls >= 0) {
+            k0 = LP[Ls]; nc = LP[Ls + 1] - k0;
+            k0n = LP[max(Ls - 1, 0)]; ncn = LP[max(Ls - 1, 0) + 1] - k0n;
+            first = ng_chain(CH, min(g, R * nc - 1), k0, nc);
+        }
+        for (int L = Ls; L >= 0; L--) {
             const int k0nn = LP[max(L - 2, 0)], ncnn = LP[max(L - 2, 0) + 1] - k0nn;
             const NgChain next = ng_chain(CH, min(g, R * ncn - 1), k0n, ncn);
             for (int tk = g; tk < R * nc; tk += LW) {
                 NgChain c = first;
                 if (tk != g) c = ng_chain(CH, tk, k0, nc);
-                double *XU = region(c.r);
-                const double2 *XU2 = (const double2 *)XU;
-                const int h = c.i1 - 1, lo = c.i0 - 1, pe = c.pe;
-                const double2 ph = XU2[h];
-                const double xp = XU[2 * min(pe, N - 1)];
-                double2 a0 = XU2[max(h - 1, lo)], a1 = XU2[max(h - 2, lo)], a2 = XU2[max(h - 3, lo)],
-                        a3 = XU2[max(h - 4, lo)];
-                double x = ph.x;
-                if (pe < N) x = x + ph.y * xp;
-                XU[2 * h] = x;
-                int i = h - 1;
-                while (i - 3 >= lo) {
-                    /* rows i .. i-3 from a0..a3; rows i-4 .. i-7 loaded (clamped
-                     * to the chain; a clamped value is not used) */
-                    const double2 b0 = XU2[max(i - 4, lo)], b1 = XU2[max(i - 5, lo)], b2 = XU2[max(i - 6, lo)],
-                                  b3 = XU2[max(i - 7, lo)];
-                    x = a0.x + a0.y * x; XU[2 * i] = x;
-                    x = a1.x + a1.y * x; XU[2 * (i - 1)] = x;
-                    x = a2.x + a2.y * x; XU[2 * (i - 2)] = x;
-                    x = a3.x + a3.y * x; XU[2 * (i - 3)] = x;
-                    a0 = b0; a1 = b1; a2 = b2; a3 = b3;
-                    i -= 4;
-                }
-                if (i >= lo) { x = a0.x + a0.y * x; XU[2 * i] = x; }
-                if (i - 1 >= lo) { x = a1.x + a1.y * x; XU[2 * (i - 1)] = x; }
-                if (i - 2 >= lo) { x = a2.x + a2.y * x; XU[2 * (i - 2)] = x; }
+                bsub(c);
             }
             sync();
             first = next; k0 = k0n; nc = ncn; k0n = k0nn; ncn = ncnn;
