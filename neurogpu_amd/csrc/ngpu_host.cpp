@@ -93,6 +93,7 @@ struct ngpu_ctx {
     int cfg_stage = -1;            /* row coefficients: -1 auto, 0 global memory, 1 LDS */
     /* device state */
     int lay_W = 0, lay_R = 0, lay_NQ = 0;
+    int ncu = 0;                   /* compute units of the device, once queried */
     bool tables_dirty = true, params_dirty = true, stim_dirty = true, graph_dirty = true;
     DevArr<int> d_qlink, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_rec, d_msched;
     std::vector<int> qmask;        /* mechanism mask by position */
@@ -469,16 +470,8 @@ extern "C" long long ngpu_vhot_size(const ngpu_ctx *c)
 /* Positions per run: segments rounded up to whole waves. */
 static int positions(int N) { return (N + 63) / 64 * 64; }
 
-/* Waves per group.  Up to 512 segments a group is one wave (its levels
- * synchronise at wavefront scope, without a barrier instruction); larger
- * trees take 8 waves.  Runs per group: as many as fit in LDS, up to
- * max_runs_per_group (measured on the 500- and 1,200-segment cells: two runs
- * per one-wave group, six per eight-wave group). */
-static void choose_layout(const ngpu_ctx *c, int &W, int &R)
-{
-    W = c->cfg_w > 0 ? c->cfg_w : (c->N <= 512 ? 1 : 16);
-    R = c->cfg_r > 0 ? c->cfg_r : 0;
-}
+/* Most runs per group (measured on the 500- and 1,200-segment cells: two
+ * runs per one-wave group, six per larger group). */
 static int max_runs_per_group(int W) { return W == 1 ? 2 : 6; }
 
 static int build_tables(ngpu_ctx *c)
@@ -657,6 +650,19 @@ static int enqueue(ngpu_ctx *c, int &launches)
 
 static int prepare_layout(ngpu_ctx *c, int W, int R);
 
+/* Device, default record site and the morphology's tables, once. */
+static int ensure_tables(ngpu_ctx *c)
+{
+    if (!c->have_sim) { c->rec.assign(1, 0); c->have_sim = true; }
+    HIPCHK(c, hipSetDevice(c->device));
+    if (c->tables_dirty) {
+        int rc = build_tables(c);
+        if (rc) return rc;
+        drop_graph(c);
+    }
+    return NGPU_OK;
+}
+
 /* The mechanism pass's segment schedule for groups of W waves and R runs
  * (ng_mech_schedule, schedule.h), uploaded once per shape. */
 static int mech_schedule(ngpu_ctx *c, int W, int R)
@@ -669,10 +675,42 @@ static int mech_schedule(ngpu_ctx *c, int W, int R)
     return NGPU_OK;
 }
 
+/* Groups per workgroup for groups of W waves and R runs (as plan() chooses
+ * them), 0 if R runs do not fit in LDS; stage as plan() resolves it. */
+static int fit_groups(const ngpu_ctx *c, int W, int R, int stage)
+{
+    if (stage < 0) stage = c->nvar <= kStageMaxVar && lds_need(c, R, 1) <= kLdsLimit ? 1 : 0;
+    if (lds_need(c, R, stage) > kLdsLimit) return 0;
+    int G = 1;
+    while ((G + 1) * W <= 8 && lds_need(c, (G + 1) * R, stage) <= kLdsLimit) G++;
+    return G;
+}
+
+/* Waves W and runs R per group where the caller left them at 0
+ * (ng_choose_shape, schedule.h): candidate waves 1, 2 and 4 up to 512
+ * segments, 16 above, or the caller's W; runs 1 up to max_runs_per_group, or
+ * the caller's R. */
+static void choose_layout(ngpu_ctx *c, int &W, int &R)
+{
+    if (c->cfg_w > 0 && c->cfg_r > 0) { W = c->cfg_w; R = c->cfg_r; return; }
+    if (c->ncu <= 0 && hipDeviceGetAttribute(&c->ncu, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess)
+        c->ncu = 0;
+    std::vector<int> ws;
+    if (c->cfg_w > 0) ws.push_back(c->cfg_w);
+    else if (c->N <= 512) ws = {1, 2, 4};
+    else ws.push_back(16);
+    ng_choose_shape((long)c->nsets * c->nstim, c->ncu, ws, c->cfg_r,
+                    [&](int w, int r) { return fit_groups(c, w, r, c->cfg_stage); }, max_runs_per_group, W, R);
+    if (R == 0) R = c->cfg_r;            /* nothing fits: plan() reports it */
+}
+
 static int prepare(ngpu_ctx *c)
 {
     if (!c->have_tree || !c->have_params || !c->have_stim)
         return fail(c, NGPU_E_STATE, "ngpu_run needs ngpu_set_tree, ngpu_set_params and ngpu_set_stimuli first");
+    /* the layout choice reads the tables' sizes */
+    int rc = ensure_tables(c);
+    if (rc) return rc;
     int W, R;
     choose_layout(c, W, R);
     return prepare_layout(c, W, R);
@@ -687,13 +725,8 @@ static int prepare_layout(ngpu_ctx *c, int W, int R)
 {
     if (!c->have_tree || !c->have_params || !c->have_stim)
         return fail(c, NGPU_E_STATE, "a run needs ngpu_set_tree, ngpu_set_params and ngpu_set_stimuli first");
-    if (!c->have_sim) { c->rec.assign(1, 0); c->have_sim = true; }
-    HIPCHK(c, hipSetDevice(c->device));
-    int rc;
-    if (c->tables_dirty) {
-        if ((rc = build_tables(c))) return rc;
-        drop_graph(c);
-    }
+    int rc = ensure_tables(c);
+    if (rc) return rc;
     if (W != c->lay_W || R != c->lay_R) drop_graph(c);
     c->lay_W = W; c->lay_R = R;
     if (c->params_dirty) {

@@ -68,4 +68,36 @@ inline std::vector<int> ng_mech_schedule(const std::vector<int> &qmask, int N, i
     return out;
 }
 
+/* Launch shape (waves W and runs R per group) for nruns runs on ncu compute
+ * units.  ws: candidate waves per group, in order of preference; rmax(w):
+ * most runs per group; r_only > 0 fixes R; fit(w, r): groups per workgroup
+ * for that shape, 0 if it does not fit.  A workgroup holds its CU alone, so
+ * n workgroups take ceil(n / ncu) rounds, and a round lasts about
+ * (6 + runs per workgroup) times one run's share of a step: the solve's
+ * dependent chains and the barriers cost about six runs' mechanism passes
+ * whatever the runs per workgroup (measured on the 500-segment cell with
+ * groups of 1, 2 and 4 waves and on the 1,200-segment cell with 16).  The
+ * shape with the least rounds x (6 + runs per workgroup) wins, among shapes
+ * with at least 8 waves per workgroup if any fits; ties keep the earlier
+ * candidate (ws order, then more runs).  R = 0 when no shape fits. */
+template <class Fit, class RMax>
+inline void ng_choose_shape(long nruns, int ncu, const std::vector<int> &ws, int r_only, Fit fit, RMax rmax,
+                            int &W, int &R)
+{
+    const long cus = std::max(ncu, 1), n = std::max(nruns, 1L);
+    W = ws.empty() ? 1 : ws[0];
+    R = 0;
+    for (int full = 1; full >= 0 && R == 0; full--) {
+        long best = 0;
+        for (int w : ws)
+            for (int r = r_only > 0 ? r_only : rmax(w); r >= (r_only > 0 ? r_only : 1); r--) {
+                const int G = fit(w, r);
+                if (G <= 0 || (full && G * w < 8)) continue;
+                const long per = (long)G * r, wgs = (n + per - 1) / per;
+                const long cost = (wgs + cus - 1) / cus * (6 + per);
+                if (R == 0 || cost < best) { best = cost; W = w; R = r; }
+            }
+    }
+}
+
 #endif

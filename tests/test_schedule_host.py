@@ -26,6 +26,8 @@ def lib(tmp_path_factory):
     L = C.CDLL(out)
     L.host_mech_schedule.restype = C.c_int
     L.host_mech_schedule.argtypes = [_IP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _IP, C.c_int]
+    L.host_choose_shape.restype = None
+    L.host_choose_shape.argtypes = [C.c_long, C.c_int, _IP, C.c_int, C.c_int, C.c_double, C.c_double, _IP]
     return L
 
 
@@ -82,3 +84,66 @@ def test_soma_chunk_runs_spread_over_waves(lib):
     last = NQ // 64 - 1
     waves = [w for w in range(16) for d in seg[off[w]:off[w + 1]] if d >> 8 == last]
     assert len(waves) == 6 and len(set(waves)) == 6
+
+
+def _shape(lib, nruns, ws, run_kib, table_kib, r_only=0, ncu=256):
+    w = np.asarray(ws, dtype=np.int32)
+    out = np.zeros(2, dtype=np.int32)
+    lib.host_choose_shape(nruns, ncu, w.ctypes.data_as(_IP), w.size, r_only, run_kib, table_kib,
+                          out.ctypes.data_as(_IP))
+    return int(out[0]), int(out[1])
+
+
+# LDS per run and per workgroup (KiB) of the 500- and 1,200-segment cells
+A500 = dict(run_kib=8.4, table_kib=13.0)
+T1200 = dict(run_kib=19.4, table_kib=29.0)
+
+
+@pytest.mark.parametrize("nruns,expect", [
+    (3072, (2, 3)),     # 1,024 sets x 3 stimuli: 256 workgroups of 12 runs, one per CU
+    (24576, (1, 2)),    # 8,192 sets x 3: whole rounds of 16-run workgroups
+    (16, (4, 1)),       # a handful of runs: spread over CUs
+])
+def test_shape_small_cell(lib, nruns, expect):
+    """Shapes chosen for the 500-segment cell.  On one MI355X, 3,072 runs
+    (2,000 steps) took 253 ms with groups of 2 waves x 3 runs and 314 ms with
+    1 wave x 2 runs (192 workgroups, 64 CUs idle)."""
+    assert _shape(lib, nruns, [1, 2, 4], **A500) == expect
+
+
+@pytest.mark.parametrize("nruns,expect", [(40960, 6), (81920, 6), (5120, 5), (5, 1)])
+def test_shape_large_cell(lib, nruns, expect):
+    """16-wave groups of the 1,200-segment cell: six runs at full size; 5,120
+    runs (a population of 1,024 x 5 stimuli) fill four rounds of 1,024
+    workgroups with five runs instead of 854 workgroups of six."""
+    assert _shape(lib, nruns, [16], **T1200) == (16, expect)
+
+
+def test_shape_fixed_runs_and_no_fit(lib):
+    assert _shape(lib, 40960, [16], r_only=4, **T1200) == (16, 4)
+    assert _shape(lib, 100, [16], run_kib=200.0, table_kib=0.0)[1] == 0
+    # with fewer CUs the same runs fill more rounds: the largest workgroups win
+    assert _shape(lib, 3072, [1, 2, 4], ncu=8, **A500) == (1, 2)
+
+
+def test_shape_minimises_rounds_times_size(lib):
+    """Against a brute-force evaluation on random sizes: no other fitting
+    shape with full workgroups has fewer rounds x (6 + runs)."""
+    rng = np.random.default_rng(5)
+    for _ in range(200):
+        nruns = int(rng.integers(1, 100000))
+        ncu = int(rng.integers(1, 300))
+        W, R = _shape(lib, nruns, [1, 2, 4], ncu=ncu, **A500)
+        def groups(w, r):
+            if r * A500["run_kib"] + A500["table_kib"] > 160:
+                return 0
+            g = 1
+            while (g + 1) * w <= 8 and (g + 1) * r * A500["run_kib"] + A500["table_kib"] <= 160:
+                g += 1
+            return g
+        def cost(w, r):
+            per = groups(w, r) * r
+            return -(-(-(-nruns // per)) // ncu) * (6 + per)
+        cands = [(w, r) for w in (1, 2, 4) for r in range(1, (2 if w == 1 else 6) + 1) if groups(w, r) * w >= 8]
+        assert (W, R) in cands
+        assert cost(W, R) == min(cost(w, r) for w, r in cands)
