@@ -26,7 +26,7 @@ from . import _lib, models
 ARCH = "gfx950"
 BATCH_NSLOT = 6                 # positions per lane of the batch kernel
 BATCH_WAVES = 8                 # waves per workgroup of the batch kernel
-KERNEL_REV = 3                  # bump when the kernel changes: cached choices are per revision
+KERNEL_REV = 4                  # bump when the kernel changes: cached choices are per revision
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "neurogpu_amd", "batch_tuning.json")
 
 
@@ -61,10 +61,15 @@ def tree_key(w: models.Workload) -> str:
 
 
 def _probe_workload(w: models.Workload, runs: int, steps: int) -> models.Workload:
+    """runs runs of w's parameter sets (repeated when w has fewer), so that a
+    probe fills every CU several times whatever the member's own size: the
+    batch shares the CUs among its members, so a shape is judged by its
+    throughput per CU, not by how fast the member alone would finish."""
     nst = w.amps.shape[0]
-    nsets = max(1, min(w.x.shape[0], -(-runs // nst)))
+    nsets = max(1, -(-runs // nst))
     steps = min(steps, w.dts.size)
-    return models.Workload(w.name, w.model, w.x[:nsets], w.amps[:, :steps], w.dts[:steps])
+    x = w.x[np.arange(nsets) % w.x.shape[0]]
+    return models.Workload(w.name, w.model, x, w.amps[:, :steps], w.dts[:steps])
 
 
 def measure(w: models.Workload, choice: Choice, device: int = 0) -> float:

@@ -57,6 +57,18 @@ def test_candidates_cover_every_size():
     assert autotune.candidates(3073) == []
 
 
+def test_probe_repeats_sets_to_fill_the_device():
+    """A member with fewer runs than the probe asks for is probed with its
+    parameter sets repeated, in order, and its stimuli cut to the probe's
+    steps."""
+    w = models.workload("active500", nsets=5, nt=300)
+    p = autotune._probe_workload(w, 40, 100)
+    nst = w.amps.shape[0]
+    assert p.x.shape[0] == -(-40 // nst) and p.nruns >= 40
+    np.testing.assert_array_equal(p.x, w.x[np.arange(p.x.shape[0]) % 5])
+    assert p.dts.size == 100 and p.amps.shape == (nst, 100)
+
+
 def test_tree_key_is_stable_and_distinct():
     a, b = models.mixed_workloads(0.001, 0.001)[:2]
     assert autotune.tree_key(a) == autotune.tree_key(a)
