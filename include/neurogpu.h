@@ -134,7 +134,7 @@ void ngpu_close(ngpu_ctx *ctx);
  * one mechanism set) simulated by one persistent launch.  No reference
  * counterpart: the reference runs one morphology per process and launch
  * (CANON/CudaStuff.cu:487-507).  At most 8 contexts per batch.  Each
- * workgroup of 8 waves takes items from a device work queue, longest first;
+ * workgroup of 8 waves takes items from a device work queue, most work first;
  * an item is (8 / waves_per_run) * runs_per_group runs of one member.  Traces and features stay
  * with each member context (ngpu_fetch_vhot, ngpu_features). */
 typedef struct ngpu_batch ngpu_batch;

@@ -3,8 +3,9 @@ runs per 8-wave workgroup), level-sync granularity (per wave or per workgroup)
 and LDS staging of the Hines row coefficients and the run's varying parameters.
 
 Each candidate runs a probe batch of the morphology alone: enough runs to
-give every workgroup of the persistent launch several items, for a few
-hundred steps, and the candidate with the highest compartment-steps/s wins.
+give every CU four items of the largest shape (parameter sets repeated), for
+a few hundred steps, and the candidate with the highest compartment-steps/s
+wins.
 Choices are cached in a JSON file keyed by (tree hash, mechanism set,
 architecture), so a morphology is tuned once per machine.
 
@@ -26,7 +27,8 @@ from . import _lib, models
 ARCH = "gfx950"
 BATCH_NSLOT = 6                 # positions per lane of the batch kernel
 BATCH_WAVES = 8                 # waves per workgroup of the batch kernel
-KERNEL_REV = 4                  # bump when the kernel changes: cached choices are per revision
+CUS = 256                       # compute units of an MI355X
+KERNEL_REV = 6                  # bump when the kernel changes: cached choices are per revision
 DEFAULT_CACHE = os.path.join(os.path.expanduser("~"), ".cache", "neurogpu_amd", "batch_tuning.json")
 
 
@@ -85,7 +87,13 @@ def measure(w: models.Workload, choice: Choice, device: int = 0) -> float:
     return st["comp_steps"] / (st["ms_kernel"] * 1e-3)
 
 
-def tune(w: models.Workload, device: int = 0, probe_runs: int = 4096, probe_steps: int = 200,
+def item_runs(choice: Choice) -> int:
+    """Most runs in one item of this shape: 8 / waves groups of up to four
+    runs (one-wave groups) or six (larger groups), as the library allows."""
+    return BATCH_WAVES // choice.waves_per_run * (4 if choice.waves_per_run == 1 else 6)
+
+
+def tune(w: models.Workload, device: int = 0, probe_runs: Optional[int] = None, probe_steps: int = 200,
          cache: Optional[str] = DEFAULT_CACHE, log=None) -> Tuple[Choice, Dict[str, float]]:
     """Best launch choice for w (from the cache when present)."""
     key = tree_key(w)
@@ -98,10 +106,14 @@ def tune(w: models.Workload, device: int = 0, probe_runs: int = 4096, probe_step
             db = {}
     if key in db:
         return Choice(**db[key]["choice"]), db[key]["rates"]
+    cands = candidates(w.model.tree.N)
+    if probe_runs is None:
+        # four items of the largest shape per CU
+        probe_runs = 4 * CUS * max([item_runs(c) for c in cands] or [1])
     probe = _probe_workload(w, probe_runs, probe_steps)
     rates: Dict[str, float] = {}
     best, best_rate = None, -1.0
-    for c in candidates(w.model.tree.N):
+    for c in cands:
         try:
             measure(probe, c, device)            # first launch builds the tables
             r = measure(probe, c, device)

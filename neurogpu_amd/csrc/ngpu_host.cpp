@@ -470,9 +470,11 @@ extern "C" long long ngpu_vhot_size(const ngpu_ctx *c)
 /* Positions per run: segments rounded up to whole waves. */
 static int positions(int N) { return (N + 63) / 64 * 64; }
 
-/* Most runs per group (measured on the 500- and 1,200-segment cells: two
- * runs per one-wave group, six per larger group). */
-static int max_runs_per_group(int W) { return W == 1 ? 2 : 6; }
+/* Most runs per group: four per one-wave group (on the 200-segment cell,
+ * at equal occupancy, 9.2e9 compartment-steps/s against 8.7e9 with two and
+ * 7.4e9 with one; the 500-segment cell fits two in LDS), six per larger
+ * group (the 1,200-segment cell). */
+static int max_runs_per_group(int W) { return W == 1 ? 4 : 6; }
 
 static int build_tables(ngpu_ctx *c)
 {
@@ -950,10 +952,11 @@ extern "C" int ngpu_batch_run(ngpu_batch *b, ngpu_batch_stats *st)
         const int nruns = c->nsets * c->nstim, per_item = G * R;
         for (int r0 = 0; r0 < nruns; r0 += per_item) {
             items.push_back(make_int4(i, r0, std::min(per_item, nruns - r0), 0));
-            cost.push_back((double)c->N * c->nt);
+            cost.push_back((double)c->N * c->nt * std::min(per_item, nruns - r0));
         }
     }
-    /* longest items first, so that the long runs do not form the tail */
+    /* most work first (segments x steps x runs of the item), so that the
+     * last items to start are the shortest and no CU idles long at the end */
     std::vector<int> order(items.size());
     for (size_t k = 0; k < order.size(); k++) order[k] = (int)k;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost[x] > cost[y]; });

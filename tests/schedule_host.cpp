@@ -29,6 +29,6 @@ extern "C" void host_choose_shape(long nruns, int ncu, const int *ws, int nws, i
         while ((G + 1) * W <= 8 && (G + 1) * R * run_kib + table_kib <= 160.0) G++;
         return G;
     };
-    auto rmax = [](int W) { return W == 1 ? 2 : 6; };
+    auto rmax = [](int W) { return W == 1 ? 4 : 6; };
     ng_choose_shape(nruns, ncu, w, r_only, fit, rmax, out[0], out[1]);
 }

@@ -69,6 +69,14 @@ def test_probe_repeats_sets_to_fill_the_device():
     assert p.dts.size == 100 and p.amps.shape == (nst, 100)
 
 
+def test_probe_size_covers_the_largest_item():
+    """Items of one-wave groups hold up to 8 x 4 runs: a default probe of a
+    small cell gives every CU four of them."""
+    c1 = autotune.Choice(1, 0, 1)
+    assert autotune.item_runs(c1) == 32 and autotune.item_runs(autotune.Choice(8, 1, 1)) == 6
+    assert max(autotune.item_runs(c) for c in autotune.candidates(200)) == 32
+
+
 def test_tree_key_is_stable_and_distinct():
     a, b = models.mixed_workloads(0.001, 0.001)[:2]
     assert autotune.tree_key(a) == autotune.tree_key(a)

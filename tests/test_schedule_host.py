@@ -144,6 +144,6 @@ def test_shape_minimises_rounds_times_size(lib):
         def cost(w, r):
             per = groups(w, r) * r
             return -(-(-(-nruns // per)) // ncu) * (6 + per)
-        cands = [(w, r) for w in (1, 2, 4) for r in range(1, (2 if w == 1 else 6) + 1) if groups(w, r) * w >= 8]
+        cands = [(w, r) for w in (1, 2, 4) for r in range(1, (4 if w == 1 else 6) + 1) if groups(w, r) * w >= 8]
         assert (W, R) in cands
         assert cost(W, R) == min(cost(w, r) for w, r in cands)
