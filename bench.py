@@ -5,11 +5,10 @@ One bench step = one pass of the simulation over the whole batch: every
 libneurogpu's HIP kernel, inputs already resident in device memory.  The
 headline workload is the thick-tufted-pyramidal-like cell (1,200 compartments,
 the 13 BBP TTPC mechanisms), 8,192 parameter sets x 5 step currents,
-dt = 0.025 ms; a bench step covers its first 1,000 dt steps (the work per
-compartment-step does not depend on the window: the kernel has no
-data-dependent branches besides the rate functions' singularity bumps).
-``--workload active500`` measures the active 500-compartment cell over its full 10,000
-steps.
+dt = 0.025 ms; a bench step is its whole 300 ms protocol, 12,000 dt steps in
+one persistent launch.  ``--workload active500`` measures the active
+500-compartment cell over its full 10,000 steps; ``--nt`` shortens the
+protocol (profiling runs).
 
 Prints one JSON line (the last line of stdout).
 """
@@ -189,7 +188,7 @@ def bench_mixed(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="ttpc1200", choices=("passive64", "active500", "ttpc1200", "mixed4"))
     ap.add_argument("--separate", action="store_true",
@@ -229,8 +228,7 @@ def main():
     if args.workload == "mixed4":
         return bench_mixed(args)
 
-    nt = args.nt if args.nt is not None else (1000 if args.workload == "ttpc1200" else None)
-    w = models.workload(args.workload, nsets=args.nsets, nt=nt)
+    w = models.workload(args.workload, nsets=args.nsets, nt=args.nt)
     m = w.model
     print(f"bench: {w.name} N={m.tree.N} runs={w.nruns} nt={w.dts.size} S={m.state_count():.2f}", flush=True)
     sim = Simulator(0)
@@ -286,7 +284,7 @@ def main():
     }
     if not args.no_cpu:
         threads = max(1, min(16, os.cpu_count() or 1))
-        cpu_sets = args.cpu_sets if args.cpu_sets is not None else (48 if args.workload == "ttpc1200" else 8)
+        cpu_sets = args.cpu_sets if args.cpu_sets is not None else (16 if args.workload == "ttpc1200" else 8)
         nsets = min(cpu_sets, w.x.shape[0])
         ref, cps, el = cpu_baseline(w, nsets, threads)
         d = float(np.max(np.abs(gpu[:nsets] - ref)))
