@@ -46,6 +46,8 @@ def test_passive64_full_size():
 
 
 @pytest.mark.parametrize("launch", [None, dict(waves_per_run=2), dict(runs_per_group=2), dict(waves_per_run=16),
+                                    dict(waves_per_run=1, runs_per_group=4), dict(waves_per_run=2, runs_per_group=3),
+                                    dict(waves_per_run=4, runs_per_group=6),
                                     dict(steps_per_launch=700), dict(steps_per_launch=500, use_graph=True)])
 def test_active500_small(launch):
     w = models.workload("active500", nsets=6, nt=1500)
