@@ -173,7 +173,7 @@ void ngpu_batch_close(ngpu_batch *batch);
  * ReLU, BF16 weights and activations on MFMA with FP32 accumulation, FP32
  * master weights and Adam.  Parameters of one member, in order: W1[H][nin],
  * b1[H], W2[H][H], b2[H], W3[H][H], b3[H], W4[nout][H], b4[nout]; members are
- * concatenated. */
+ * concatenated (1..64 members; every launch covers all of them). */
 typedef struct ngpu_mlp ngpu_mlp;
 int ngpu_mlp_create(int device, int nin, int hidden, int nout, int members, ngpu_mlp **out);
 long long ngpu_mlp_nparams(const ngpu_mlp *mlp);                      /* per member */

@@ -4,6 +4,11 @@
  * nin -> H -> H -> H -> nout with ReLU.  No reference counterpart (the
  * reference GA has no surrogate); it pre-screens GA offspring.
  *
+ * The members of the ensemble run side by side: every launch covers all of
+ * them (grid z = member, each operand a fixed element stride apart), so a
+ * training step is one gather, four forward and ten backward GEMMs, the
+ * loss, column sums and one Adam update for the whole ensemble.
+ *
  * GEMM: C[M,N] = sum_k A(m,k) B(k,n), A and B bf16 (either orientation), FP32
  * accumulation on v_mfma_f32_32x32x16_bf16 (gfx950): a 256-thread block holds
  * a 64x64 tile of C, each wave a 32x32 sub-tile; operands are read straight
@@ -52,6 +57,8 @@ struct GemmArgs {
     float *C32;                 /* fp32 out or null */
     uint16_t *C16;              /* bf16 out or null */
     int relu;
+    /* element strides between the matrices of consecutive grid z (members) */
+    long long sA, sB, sBias, sMask, sC;
 };
 
 template <bool TA, bool TB>
@@ -59,6 +66,13 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int m0 = blockIdx.y * 64 + (wave >> 1) * 32, n0 = blockIdx.x * 64 + (wave & 1) * 32;
+    const long long z = blockIdx.z;
+    g.A += z * g.sA;
+    g.B += z * g.sB;
+    if (g.bias) g.bias += z * g.sBias;
+    if (g.mask) g.mask += z * g.sMask;
+    if (g.C32) g.C32 += z * g.sC;
+    if (g.C16) g.C16 += z * g.sC;
     const int row = lane & 31, kh = (lane >> 5) * 8;
     f32x16 acc;
 #pragma unroll
@@ -104,9 +118,9 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
     }
 }
 
-static hipError_t gemm(hipStream_t s, bool ta, bool tb, const GemmArgs &g)
+static hipError_t gemm(hipStream_t s, bool ta, bool tb, const GemmArgs &g, int nz)
 {
-    dim3 grid((g.N + 63) / 64, (g.M + 63) / 64), block(256);
+    dim3 grid((g.N + 63) / 64, (g.M + 63) / 64, nz), block(256);
     if (!ta && tb) hipLaunchKernelGGL((gemm_bf16_kernel<false, true>), grid, block, 0, s, g);
     else if (!ta && !tb) hipLaunchKernelGGL((gemm_bf16_kernel<false, false>), grid, block, 0, s, g);
     else if (ta && !tb) hipLaunchKernelGGL((gemm_bf16_kernel<true, false>), grid, block, 0, s, g);
@@ -120,11 +134,15 @@ __global__ void f32_to_bf16_kernel(const float *x, uint16_t *y, size_t n)
         y[i] = f2bf(x[i]);
 }
 
-/* dY = 2 (Y - T) / (rows * cols) in bf16 and fp32; loss partial sums per block */
+/* dY = 2 (Y - T) / (rows * cols) in bf16 and fp32; loss partial sums per
+ * block; member blockIdx.y: its n elements at offset blockIdx.y * n, its loss
+ * in loss[blockIdx.y] */
 __global__ void mse_grad_kernel(const float *Y, const float *T, int n, float scale, float *dY32, uint16_t *dY16,
                                 float *loss)
 {
     __shared__ float part[256];
+    const size_t o = (size_t)blockIdx.y * n;
+    Y += o; T += o; dY32 += o; dY16 += o; loss += blockIdx.y;
     float s = 0.0f;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float d = Y[i] - T[i];
@@ -144,12 +162,18 @@ __global__ void mse_grad_kernel(const float *Y, const float *T, int n, float sca
 
 /* column sums of an [rows, cols] matrix (fp32 or bf16) -> out[cols]: a
  * block of 64 columns x 16 row groups (coalesced 64-wide row reads, each
- * thread summing every 16th row), then the 16 partial sums in a fixed order */
+ * thread summing every 16th row), then the 16 partial sums in a fixed order;
+ * member blockIdx.y: its matrix at offset blockIdx.y * rows * cols, its sums
+ * at out + blockIdx.y * sout */
 #define COLSUM_RG 16
 __global__ void __launch_bounds__(64 * COLSUM_RG) colsum_kernel(const float *x32, const uint16_t *x16, int rows, int cols,
-                                                                float *out)
+                                                                float *out, long long sout)
 {
     __shared__ float part[COLSUM_RG][64];
+    const size_t o = (size_t)blockIdx.y * rows * cols;
+    if (x32) x32 += o;
+    if (x16) x16 += o;
+    out += blockIdx.y * sout;
     const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
     const int c = blockIdx.x * 64 + tx;
     float s = 0.0f;
@@ -179,11 +203,16 @@ __global__ void adam_kernel(float *w, const float *gr, float *m, float *v, uint1
     }
 }
 
-/* rows idx[0..batch) of the device training set -> the batch buffers */
-__global__ void gather_batch_kernel(const uint16_t *xall, const float *tall, const int *idx, int nin, int nout, int batch,
-                                    uint16_t *x16, float *t32)
+/* rows idx[0..batch) of the device training set -> the batch buffers; member
+ * blockIdx.y: its row order at idx + blockIdx.y * sidx, its batch at
+ * offset blockIdx.y * batch rows */
+__global__ void gather_batch_kernel(const uint16_t *xall, const float *tall, const int *idx, long long sidx, int nin,
+                                    int nout, int batch, uint16_t *x16, float *t32)
 {
     const int nx = batch * nin, nt = batch * nout;
+    idx += blockIdx.y * sidx;
+    x16 += (size_t)blockIdx.y * nx;
+    t32 += (size_t)blockIdx.y * nt;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nx + nt; i += gridDim.x * blockDim.x) {
         if (i < nx) {
             const int r = i / nin, c = i - r * nin;
@@ -195,9 +224,10 @@ __global__ void gather_batch_kernel(const uint16_t *xall, const float *tall, con
     }
 }
 
-__global__ void add_scalar_kernel(const float *x, float *acc)
+/* acc[i * stride] += x[i] for i < n */
+__global__ void add_scalar_kernel(const float *x, float *acc, int n, int stride)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) *acc += *x;
+    if (blockIdx.x == 0 && (int)threadIdx.x < n) acc[(size_t)threadIdx.x * stride] += x[threadIdx.x];
 }
 
 /* ------------------------------------------------------------------ host */
@@ -211,7 +241,7 @@ struct ngpu_mlp {
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;  /* [members * nparam] */
     uint16_t *w16 = nullptr;                                      /* bf16 copy (biases unused) */
     long long step = 0;
-    /* scratch for batches of up to cap rows */
+    /* scratch for batches of up to cap rows per member, members side by side */
     int cap = 0;
     uint16_t *x16 = nullptr, *a16[3] = {nullptr, nullptr, nullptr}, *d16[3] = {nullptr, nullptr, nullptr}, *dy16 = nullptr;
     float *y32 = nullptr, *t32 = nullptr, *dy32 = nullptr, *loss = nullptr;
@@ -239,23 +269,27 @@ static int ensure_cap(ngpu_mlp *p, int n)
     if (n <= p->cap) return 0;
     (void)hipFree(p->x16); (void)hipFree(p->dy16); (void)hipFree(p->y32); (void)hipFree(p->t32); (void)hipFree(p->dy32);
     for (int i = 0; i < 3; i++) { (void)hipFree(p->a16[i]); (void)hipFree(p->d16[i]); }
-    const size_t H = p->H;
-    MCHK(p, hipMalloc((void **)&p->x16, (size_t)n * p->nin * 2));
+    p->x16 = p->dy16 = nullptr; p->y32 = p->t32 = p->dy32 = nullptr;
+    for (int i = 0; i < 3; i++) p->a16[i] = p->d16[i] = nullptr;
+    p->cap = 0;
+    const size_t H = p->H, rows = (size_t)n * p->members;
+    MCHK(p, hipMalloc((void **)&p->x16, rows * p->nin * 2));
     for (int i = 0; i < 3; i++) {
-        MCHK(p, hipMalloc((void **)&p->a16[i], (size_t)n * H * 2));
-        MCHK(p, hipMalloc((void **)&p->d16[i], (size_t)n * H * 2));
+        MCHK(p, hipMalloc((void **)&p->a16[i], rows * H * 2));
+        MCHK(p, hipMalloc((void **)&p->d16[i], rows * H * 2));
     }
-    MCHK(p, hipMalloc((void **)&p->dy16, (size_t)n * p->nout * 2));
-    MCHK(p, hipMalloc((void **)&p->y32, (size_t)n * p->nout * 4));
-    MCHK(p, hipMalloc((void **)&p->t32, (size_t)n * p->nout * 4));
-    MCHK(p, hipMalloc((void **)&p->dy32, (size_t)n * p->nout * 4));
+    MCHK(p, hipMalloc((void **)&p->dy16, rows * p->nout * 2));
+    MCHK(p, hipMalloc((void **)&p->y32, rows * p->nout * 4));
+    MCHK(p, hipMalloc((void **)&p->t32, rows * p->nout * 4));
+    MCHK(p, hipMalloc((void **)&p->dy32, rows * p->nout * 4));
     p->cap = n;
     return 0;
 }
 
 extern "C" int ngpu_mlp_create(int device, int nin, int hidden, int nout, int members, ngpu_mlp **out)
 {
-    if (!out || nin < 1 || hidden < 1 || nout < 1 || members < 1 || nin > 4096 || hidden > 4096 || nout > 4096)
+    if (!out || nin < 1 || hidden < 1 || nout < 1 || members < 1 || members > 64 || nin > 4096 || hidden > 4096 ||
+        nout > 4096)
         return NGPU_E_ARG;
     *out = nullptr;
     if (hipSetDevice(device) != hipSuccess) return NGPU_E_HIP;
@@ -320,53 +354,62 @@ extern "C" int ngpu_mlp_get_weights(ngpu_mlp *p, float *w)
     return NGPU_OK;
 }
 
-/* forward of member mem on x16 [n, nin] -> activations a16[0..2], y32 [n, nout] */
-static int forward(ngpu_mlp *p, int mem, int n)
+/* forward of members m0 .. m0+nm-1, slot j = member m0+j: inputs x16 (slot
+ * j at j * xs elements; xs = 0 shares one input) -> activations a16[0..2]
+ * and outputs y32, slot j at j * n rows */
+static int forward(ngpu_mlp *p, int m0, int nm, int n, long long xs)
 {
-    const float *W = p->w + mem * p->nparam;
-    const uint16_t *W16 = p->w16 + mem * p->nparam;
-    const int H = p->H;
+    const float *W = p->w + (size_t)m0 * p->nparam;
+    const uint16_t *W16 = p->w16 + (size_t)m0 * p->nparam;
+    const long long P = (long long)p->nparam, H = p->H;
     GemmArgs g{};
     const uint16_t *in = p->x16;
     int K = p->nin;
+    long long sin = xs;
     for (int l = 0; l < 3; l++) {
-        g = GemmArgs{n, H, K, K, K, H, in, W16 + p->off[2 * l], W + p->off[2 * l + 1], nullptr, nullptr, p->a16[l], 1};
-        MCHK(p, gemm(p->stream, false, true, g));
+        g = GemmArgs{n, (int)H, K, K, K, (int)H, in, W16 + p->off[2 * l], W + p->off[2 * l + 1], nullptr, nullptr, p->a16[l], 1,
+                     sin, P, P, 0, n * H};
+        MCHK(p, gemm(p->stream, false, true, g, nm));
         in = p->a16[l];
-        K = H;
+        K = (int)H;
+        sin = n * H;
     }
-    g = GemmArgs{n, p->nout, H, H, H, p->nout, in, W16 + p->off[6], W + p->off[7], nullptr, p->y32, nullptr, 0};
-    MCHK(p, gemm(p->stream, false, true, g));
+    g = GemmArgs{n, p->nout, (int)H, (int)H, (int)H, p->nout, in, W16 + p->off[6], W + p->off[7], nullptr, p->y32, nullptr, 0,
+                 n * H, P, P, 0, (long long)n * p->nout};
+    MCHK(p, gemm(p->stream, false, true, g, nm));
     return 0;
 }
 
-/* gradients of member mem (loss = mean squared error over n x nout) into p->g */
-static int backward(ngpu_mlp *p, int mem, int n)
+/* gradients of members m0 .. m0+nm-1 (loss = mean squared error over n x
+ * nout against t32, slot j) into p->g; loss of slot j in p->loss[j] */
+static int backward(ngpu_mlp *p, int m0, int nm, int n)
 {
-    const uint16_t *W16 = p->w16 + mem * p->nparam;
-    float *G = p->g + mem * p->nparam;
+    const uint16_t *W16 = p->w16 + (size_t)m0 * p->nparam;
+    float *G = p->g + (size_t)m0 * p->nparam;
     const int H = p->H, O = p->nout;
+    const long long P = (long long)p->nparam, nH = (long long)n * H, nO = (long long)n * O;
     const int tot = n * O;
-    MCHK(p, hipMemsetAsync(p->loss + mem, 0, 4, p->stream));
-    hipLaunchKernelGGL(mse_grad_kernel, dim3(std::min(256, (tot + 255) / 256)), dim3(256), 0, p->stream, p->y32, p->t32,
-                       tot, 1.0f / (float)tot, p->dy32, p->dy16, p->loss + mem);
+    MCHK(p, hipMemsetAsync(p->loss, 0, (size_t)nm * 4, p->stream));
+    hipLaunchKernelGGL(mse_grad_kernel, dim3(std::min(256, (tot + 255) / 256), nm), dim3(256), 0, p->stream, p->y32, p->t32,
+                       tot, 1.0f / (float)tot, p->dy32, p->dy16, p->loss);
     /* layer 4: dW4 = dY^T A3, db4 = colsum(dY), dZ3 = (dY W4) * (A3 > 0) */
-    GemmArgs g{O, H, n, O, H, H, p->dy16, p->a16[2], nullptr, nullptr, G + p->off[6], nullptr, 0};
-    MCHK(p, gemm(p->stream, true, false, g));
-    hipLaunchKernelGGL(colsum_kernel, dim3((O + 63) / 64), dim3(64 * COLSUM_RG), 0, p->stream, p->dy32, (const uint16_t *)nullptr, n, O,
-                       G + p->off[7]);
-    g = GemmArgs{n, H, O, O, H, H, p->dy16, W16 + p->off[6], nullptr, p->a16[2], nullptr, p->d16[2], 0};
-    MCHK(p, gemm(p->stream, false, false, g));
+    GemmArgs g{O, H, n, O, H, H, p->dy16, p->a16[2], nullptr, nullptr, G + p->off[6], nullptr, 0, nO, nH, 0, 0, P};
+    MCHK(p, gemm(p->stream, true, false, g, nm));
+    hipLaunchKernelGGL(colsum_kernel, dim3((O + 63) / 64, nm), dim3(64 * COLSUM_RG), 0, p->stream, p->dy32,
+                       (const uint16_t *)nullptr, n, O, G + p->off[7], P);
+    g = GemmArgs{n, H, O, O, H, H, p->dy16, W16 + p->off[6], nullptr, p->a16[2], nullptr, p->d16[2], 0, nO, P, 0, nH, nH};
+    MCHK(p, gemm(p->stream, false, false, g, nm));
     for (int l = 2; l >= 0; l--) {
         const uint16_t *prev = l > 0 ? p->a16[l - 1] : p->x16;
         const int K = l > 0 ? H : p->nin;
-        g = GemmArgs{H, K, n, H, K, K, p->d16[l], prev, nullptr, nullptr, G + p->off[2 * l], nullptr, 0};
-        MCHK(p, gemm(p->stream, true, false, g));
-        hipLaunchKernelGGL(colsum_kernel, dim3((H + 63) / 64), dim3(64 * COLSUM_RG), 0, p->stream, (const float *)nullptr, p->d16[l], n,
-                           H, G + p->off[2 * l + 1]);
+        g = GemmArgs{H, K, n, H, K, K, p->d16[l], prev, nullptr, nullptr, G + p->off[2 * l], nullptr, 0, nH, (long long)n * K, 0, 0, P};
+        MCHK(p, gemm(p->stream, true, false, g, nm));
+        hipLaunchKernelGGL(colsum_kernel, dim3((H + 63) / 64, nm), dim3(64 * COLSUM_RG), 0, p->stream, (const float *)nullptr,
+                           p->d16[l], n, H, G + p->off[2 * l + 1], P);
         if (l > 0) {
-            g = GemmArgs{n, H, H, H, H, H, p->d16[l], W16 + p->off[2 * l], nullptr, p->a16[l - 1], nullptr, p->d16[l - 1], 0};
-            MCHK(p, gemm(p->stream, false, false, g));
+            g = GemmArgs{n, H, H, H, H, H, p->d16[l], W16 + p->off[2 * l], nullptr, p->a16[l - 1], nullptr, p->d16[l - 1], 0,
+                         nH, P, 0, nH, nH};
+            MCHK(p, gemm(p->stream, false, false, g, nm));
         }
     }
     return 0;
@@ -387,11 +430,9 @@ extern "C" int ngpu_mlp_predict(ngpu_mlp *p, const float *x, int n, float *out)
     MCHK(p, hipSetDevice(p->device));
     int rc;
     if ((rc = ensure_cap(p, n)) || (rc = upload_x(p, x, n))) return rc;
-    for (int mem = 0; mem < p->members; mem++) {
-        if ((rc = forward(p, mem, n))) return rc;
-        MCHK(p, hipMemcpyAsync(out + (size_t)mem * n * p->nout, p->y32, (size_t)n * p->nout * 4, hipMemcpyDeviceToHost,
-                               p->stream));
-    }
+    /* every member on the one input; y32 holds member j's outputs at j * n rows */
+    if ((rc = forward(p, 0, p->members, n, 0))) return rc;
+    MCHK(p, hipMemcpyAsync(out, p->y32, (size_t)p->members * n * p->nout * 4, hipMemcpyDeviceToHost, p->stream));
     MCHK(p, hipStreamSynchronize(p->stream));
     return NGPU_OK;
 }
@@ -403,9 +444,9 @@ extern "C" int ngpu_mlp_gradients(ngpu_mlp *p, int mem, const float *x, const fl
     int rc;
     if ((rc = ensure_cap(p, n)) || (rc = upload_x(p, x, n))) return rc;
     MCHK(p, hipMemcpyAsync(p->t32, t, (size_t)n * p->nout * 4, hipMemcpyHostToDevice, p->stream));
-    if ((rc = forward(p, mem, n)) || (rc = backward(p, mem, n))) return rc;
+    if ((rc = forward(p, mem, 1, n, 0)) || (rc = backward(p, mem, 1, n))) return rc;
     MCHK(p, hipMemcpyAsync(grad, p->g + mem * p->nparam, p->nparam * 4, hipMemcpyDeviceToHost, p->stream));
-    if (loss) MCHK(p, hipMemcpyAsync(loss, p->loss + mem, 4, hipMemcpyDeviceToHost, p->stream));
+    if (loss) MCHK(p, hipMemcpyAsync(loss, p->loss, 4, hipMemcpyDeviceToHost, p->stream));
     MCHK(p, hipStreamSynchronize(p->stream));
     return NGPU_OK;
 }
@@ -428,7 +469,7 @@ extern "C" int ngpu_mlp_train(ngpu_mlp *p, const float *x, const float *t, int n
         p->xall16 = nullptr; p->tall = nullptr; p->didx = nullptr; p->cap_rows = 0;
         MCHK(p, hipMalloc((void **)&p->xall16, (size_t)n * p->nin * 2));
         MCHK(p, hipMalloc((void **)&p->tall, (size_t)n * p->nout * 4));
-        MCHK(p, hipMalloc((void **)&p->didx, (size_t)n * 4));
+        MCHK(p, hipMalloc((void **)&p->didx, (size_t)n * p->members * 4));
         p->cap_rows = n;
     }
     const size_t nl = (size_t)p->members * std::max(epochs, 1);
@@ -443,32 +484,34 @@ extern "C" int ngpu_mlp_train(ngpu_mlp *p, const float *x, const float *t, int n
     MCHK(p, hipMemcpyAsync(p->xall16, xb.data(), xb.size() * 2, hipMemcpyHostToDevice, p->stream));
     MCHK(p, hipMemcpyAsync(p->tall, t, (size_t)n * p->nout * 4, hipMemcpyHostToDevice, p->stream));
     MCHK(p, hipMemsetAsync(p->lacc, 0, nl * 4, p->stream));
-    std::vector<int> idx(n);
-    for (int i = 0; i < n; i++) idx[i] = i;
+    /* each member draws its own row order per epoch (its own generator);
+     * the members then take their steps side by side */
+    const int M = p->members;
+    std::vector<std::mt19937> rng;
+    for (int mem = 0; mem < M; mem++) rng.emplace_back(seed * 1009u + (unsigned)mem);
+    std::vector<int> idx((size_t)n * M);
+    for (int mem = 0; mem < M; mem++)
+        for (int i = 0; i < n; i++) idx[(size_t)mem * n + i] = i;
     const int nb = n / batch;
     const int gb = std::min(1024, (batch * (p->nin + p->nout) + 255) / 256);
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    for (int mem = 0; mem < p->members; mem++) {
-        std::mt19937 rng(seed * 1009u + (unsigned)mem);
-        long long step = 0;
-        for (int ep = 0; ep < epochs; ep++) {
-            std::shuffle(idx.begin(), idx.end(), rng);
-            MCHK(p, hipMemcpyAsync(p->didx, idx.data(), (size_t)n * 4, hipMemcpyHostToDevice, p->stream));
-            for (int s0 = 0; s0 + batch <= n; s0 += batch) {
-                hipLaunchKernelGGL(gather_batch_kernel, dim3(gb), dim3(256), 0, p->stream, p->xall16, p->tall, p->didx + s0,
-                                   p->nin, p->nout, batch, p->x16, p->t32);
-                if ((rc = forward(p, mem, batch)) || (rc = backward(p, mem, batch))) return rc;
-                step++;
-                const float c1 = 1.0f - std::pow(b1, (float)step), c2 = 1.0f - std::pow(b2, (float)step);
-                const size_t o = (size_t)mem * p->nparam;
-                hipLaunchKernelGGL(adam_kernel, dim3(256), dim3(256), 0, p->stream, p->w + o, p->g + o, p->m + o, p->v + o,
-                                   p->w16 + o, (int)p->nparam, lr, b1, b2, eps, c1, c2);
-                hipLaunchKernelGGL(add_scalar_kernel, dim3(1), dim3(64), 0, p->stream, p->loss + mem,
-                                   p->lacc + (size_t)mem * epochs + ep);
-            }
-            /* the next epoch rewrites the host row order */
-            MCHK(p, hipStreamSynchronize(p->stream));
+    long long step = 0;
+    for (int ep = 0; ep < epochs; ep++) {
+        for (int mem = 0; mem < M; mem++)
+            std::shuffle(idx.begin() + (size_t)mem * n, idx.begin() + (size_t)(mem + 1) * n, rng[mem]);
+        MCHK(p, hipMemcpyAsync(p->didx, idx.data(), idx.size() * 4, hipMemcpyHostToDevice, p->stream));
+        for (int s0 = 0; s0 + batch <= n; s0 += batch) {
+            hipLaunchKernelGGL(gather_batch_kernel, dim3(gb, M), dim3(256), 0, p->stream, p->xall16, p->tall, p->didx + s0,
+                               (long long)n, p->nin, p->nout, batch, p->x16, p->t32);
+            if ((rc = forward(p, 0, M, batch, (long long)batch * p->nin)) || (rc = backward(p, 0, M, batch))) return rc;
+            step++;
+            const float c1 = 1.0f - std::pow(b1, (float)step), c2 = 1.0f - std::pow(b2, (float)step);
+            hipLaunchKernelGGL(adam_kernel, dim3(1024), dim3(256), 0, p->stream, p->w, p->g, p->m, p->v, p->w16,
+                               (int)(p->nparam * M), lr, b1, b2, eps, c1, c2);
+            hipLaunchKernelGGL(add_scalar_kernel, dim3(1), dim3(64), 0, p->stream, p->loss, p->lacc + ep, M, epochs);
         }
+        /* the next epoch rewrites the host row order */
+        MCHK(p, hipStreamSynchronize(p->stream));
     }
     MCHK(p, hipGetLastError());
     MCHK(p, hipStreamSynchronize(p->stream));    /* xb and t are read by the uploads */
@@ -496,7 +539,7 @@ extern "C" int ngpu_gemm_bf16(int device, int ta, int tb, int M, int N, int K, c
         (void)hipMemcpy(dA, A, (size_t)M * K * 2, hipMemcpyHostToDevice);
         (void)hipMemcpy(dB, B, (size_t)K * N * 2, hipMemcpyHostToDevice);
         GemmArgs g{M, N, K, ta ? M : K, tb ? K : N, N, dA, dB, nullptr, nullptr, dC, nullptr, 0};
-        if (gemm(nullptr, ta != 0, tb != 0, g) != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = NGPU_E_HIP;
+        if (gemm(nullptr, ta != 0, tb != 0, g, 1) != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = NGPU_E_HIP;
         else if (hipMemcpy(C, dC, (size_t)M * N * 4, hipMemcpyDeviceToHost) != hipSuccess) rc = NGPU_E_HIP;
     }
     (void)hipFree(dA); (void)hipFree(dB); (void)hipFree(dC);

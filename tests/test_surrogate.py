@@ -55,3 +55,34 @@ def test_training_fits_a_smooth_function():
     assert np.all(r2 > 0.8), r2
     assert mu.shape == (1024, 2) and np.all(sd >= 0)
     ens.close()
+
+
+def test_ensemble_members_train_as_if_alone():
+    """Every launch covers all members side by side (grid z = member) and
+    member j's arithmetic reads only its own slots: a member trained in an
+    ensemble ends with the weights it reaches trained alone from the same
+    start with the same row orders (generator seed * 1009 + j), and each
+    member's predictions equal those of a one-member network holding its
+    weights."""
+    from neurogpu_amd.surrogate import MLPEnsemble
+    rng = np.random.default_rng(4)
+    x = rng.uniform(-1, 1, size=(1000, 12)).astype(np.float32)
+    t = rng.normal(size=(1000, 3)).astype(np.float32)
+    ens = MLPEnsemble(12, 3, members=3, seed=5)
+    one = MLPEnsemble(12, 3, members=1, seed=6)
+    P = ens.nparam
+    w0 = ens.get_weights()
+    one.set_weights(w0[:P])
+    le = ens.train_raw(x, t, epochs=3, batch=128, lr=1e-3, seed=11)
+    lo = one.train_raw(x, t, epochs=3, batch=128, lr=1e-3, seed=11)
+    we, wo = ens.get_weights(), one.get_weights()
+    assert np.array_equal(we[:P], wo)
+    assert not np.array_equal(we[P:2 * P], w0[P:2 * P])
+    np.testing.assert_allclose(le[0], lo[0], rtol=1e-5)
+    xt = rng.uniform(-1, 1, size=(700, 12)).astype(np.float32)
+    pe = ens.predict_raw(xt)
+    for j in range(3):
+        one.set_weights(we[j * P:(j + 1) * P])
+        assert np.array_equal(pe[j], one.predict_raw(xt)[0])
+    ens.close()
+    one.close()
