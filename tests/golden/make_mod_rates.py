@@ -1,0 +1,102 @@
+"""Writes tests/golden/mod_rates_ttpc.npz: the gating rates of the BBP TTPC
+mechanisms evaluated from the NMODL sources' own PROCEDURE rates blocks
+(Figures/Figure2_BBP_TTPC_clean/*.mod in the reference tree), on a voltage
+grid (a calcium grid for SK_E2), so that the oracle's C restatement of the
+rates (oracle/cpu_ref.c, which follows the generated AllModels.cu) can be
+checked against the equations it was generated from.
+
+Run here, where the reference tree is present:
+    python tests/golden/make_mod_rates.py [/root/reference]
+
+Only the arithmetic of each block is taken: its statements (assignments and
+if/else on the gating variables, with NMODL's ^ as a power) are translated
+into a Python function of v (or ca) and evaluated in double precision with
+the math module.  The fixture holds inputs and outputs only.
+"""
+import math
+import os
+import re
+import sys
+
+import numpy as np
+
+MECHS = ["Ca_HVA", "Ca_LVAst", "Ih", "Im", "K_Pst", "K_Tst", "Nap_Et2", "NaTa_t", "NaTs2_t", "SK_E2", "SKv3_1"]
+OUTS = {"SK_E2": ["zInf"]}
+
+
+def rates_block(text):
+    """Body and argument name of PROCEDURE rates(...) { ... }."""
+    m = re.search(r"PROCEDURE\s+rates\s*\((.*?)\)\s*\{", text)
+    if not m:
+        raise ValueError("no PROCEDURE rates")
+    arg = m.group(1).split("(")[0].strip() or "v"
+    i, depth = m.end(), 1
+    j = i
+    while depth:
+        depth += {"{": 1, "}": -1}.get(text[j], 0)
+        j += 1
+    return text[i:j - 1], arg
+
+
+def to_python(body, arg):
+    """The block's statements as the source of a Python function of arg."""
+    body = re.sub(r":[^\n]*", "", body)                      # NMODL comments
+    body = body.replace("{", "{\n").replace("}", "\n}\n")
+    lines = ["def rates(%s):" % arg]
+    depth = 1
+    for raw in body.split("\n"):
+        s = raw.strip()
+        if not s or s in ("UNITSOFF", "UNITSON") or s.startswith("LOCAL"):
+            continue
+        if s == "}":
+            depth -= 1
+            continue
+        if s.startswith("}"):
+            s = s[1:].strip()
+            depth -= 1
+        m = re.match(r"(else\s*)?if\s*\((.*)\)\s*\{$", s)
+        if m:
+            kw = "elif" if m.group(1) else "if"
+            lines.append("    " * depth + "%s %s:" % (kw, expr(m.group(2))))
+            depth += 1
+            continue
+        if re.match(r"else\s*\{$", s):
+            lines.append("    " * depth + "else:")
+            depth += 1
+            continue
+        m = re.match(r"([A-Za-z_]\w*)\s*=\s*(.+)$", s)
+        if not m:
+            raise ValueError("statement not understood: %r" % s)
+        lines.append("    " * depth + "%s = %s" % (m.group(1), expr(m.group(2))))
+    lines.append("    return dict(%s)" % ", ".join("%s=locals().get(%r)" % (o, o) for o in ("mInf", "mTau", "hInf", "hTau", "zInf")))
+    return "\n".join(lines)
+
+
+def expr(e):
+    e = e.replace("^", "**")
+    return re.sub(r"\bexp\s*\(", "math.exp(", e)
+
+
+def main():
+    root = sys.argv[1] if len(sys.argv) > 1 else "/root/reference"
+    src = os.path.join(root, "Figures", "Figure2_BBP_TTPC_clean")
+    v = np.concatenate([np.arange(-120.0, 60.0, 0.37),
+                        [-154.9, -66.0, -64.4, -60.0, -50.0, -38.0, -32.0, -27.0, -17.0, -60.0 - 10, -50.0 - 10]])
+    ca = np.concatenate([10.0 ** np.arange(-8.0, -1.0, 0.05), [1e-7, 5e-5, 4.3e-4]])
+    out = {"v": v, "ca": ca}
+    for name in MECHS:
+        body, arg = rates_block(open(os.path.join(src, name + ".mod")).read())
+        code = to_python(body, arg)
+        ns = {"math": math}
+        exec(compile(code, name + ".mod:rates", "exec"), ns)
+        xs = ca if name == "SK_E2" else v
+        res = [ns["rates"](float(x)) for x in xs]
+        keys = OUTS.get(name, [k for k in ("mInf", "mTau", "hInf", "hTau") if res[0][k] is not None])
+        out[name] = np.array([[r[k] for k in keys] for r in res])
+        out[name + "_keys"] = np.array(keys)
+        print(name, arg, keys, out[name].shape)
+    np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "mod_rates_ttpc.npz"), **out)
+
+
+if __name__ == "__main__":
+    main()

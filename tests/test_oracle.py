@@ -175,6 +175,30 @@ def test_rates_are_gates(ms, mech):
             assert r[3] > 0
 
 
+# oracle rate functions of the TTPC set by mechanism name (cpu_ref.ref_rates indices)
+TTPC_RATES = {"Ca_HVA": 0, "Ca_LVAst": 1, "Ih": 3, "Im": 4, "K_Pst": 5, "K_Tst": 6, "Nap_Et2": 7, "NaTa_t": 8,
+              "NaTs2_t": 9, "SK_E2": 11, "SKv3_1": 12}
+
+
+@pytest.mark.parametrize("name", sorted(TTPC_RATES))
+def test_ttpc_rates_equal_the_nmodl_sources(name):
+    """The oracle's rate functions (restated from the generated
+    AllModels.cu) against the same rates evaluated from the mechanisms'
+    NMODL PROCEDURE rates blocks (tests/golden/make_mod_rates.py, on a grid
+    of 487 voltages plus every singular point, or of calcium values for
+    SK_E2): every steady state and time constant agrees to 1e-10 relative.
+    A changed constant, a dropped term or a flipped sign moves these by far
+    more."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "mod_rates_ttpc.npz"))
+    keys = list(g[name + "_keys"])
+    xs = g["ca"] if name == "SK_E2" else g["v"]
+    ref = g[name]
+    slot = {"mInf": 0, "mTau": 1, "hInf": 2, "hTau": 3, "zInf": 0}
+    got = np.array([[cpu_ref.rates(1, TTPC_RATES[name], float(x))[slot[k]] for k in keys] for x in xs])
+    np.testing.assert_allclose(got, ref, rtol=1e-10, atol=0)
+
+
 def test_cnexp_exact_contraction():
     """cnexp is the exact solution at fixed v: y1 - yinf = (y0 - yinf) exp(-dt/tau)."""
     for y0, yinf, tau, dt in [(0.1, 0.7, 3.0, 0.025), (0.9, 0.05, 0.3, 0.1), (0.5, 0.5, 10.0, 0.02)]:
