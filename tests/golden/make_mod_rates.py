@@ -12,6 +12,13 @@ Only the arithmetic of each block is taken: its statements (assignments and
 if/else on the gating variables, with NMODL's ^ as a power) are translated
 into a Python function of v (or ca) and evaluated in double precision with
 the math module.  The fixture holds inputs and outputs only.
+
+It also writes tests/golden/mod_rates_mainen.npz from the Mainen set's
+sources (Figures/FigureS4_mainen/{ca,kca,km,kv,na}.mod): their rates
+procedures call the FUNCTIONs efun and trap0 of the same file and read the
+PARAMETER block's defaults and celsius = 37 (the operating temperature the
+files' comments and the model's fit.hoc set); the defaults used are stored
+beside the outputs.
 """
 import math
 import os
@@ -24,12 +31,51 @@ MECHS = ["Ca_HVA", "Ca_LVAst", "Ih", "Im", "K_Pst", "K_Tst", "Nap_Et2", "NaTa_t"
 OUTS = {"SK_E2": ["zInf"]}
 
 
+MAINEN = ["ca", "kca", "km", "kv", "na"]
+CELSIUS_MAINEN = 37.0
+
+
+def strip_comments(text):
+    text = re.sub(r"COMMENT.*?ENDCOMMENT", "", text, flags=re.S)
+    return re.sub(r":[^\n]*", "", text)
+
+
+def strip_units(e):
+    """NMODL unit annotations such as (mV) or (degC) inside expressions."""
+    return re.sub(r"\(\s*(?:mV|degC|mM|ms|/ms|pS/um2|um)\s*\)", "", e)
+
+
+def parameters(text):
+    """Defaults of the PARAMETER block: name = number."""
+    m = re.search(r"PARAMETER\s*\{(.*?)\}", text, flags=re.S)
+    out = {}
+    for line in m.group(1).split("\n"):
+        mm = re.match(r"\s*([A-Za-z_]\w*)\s*=\s*([-+0-9.eE]+)", line)
+        if mm:
+            out[mm.group(1)] = float(mm.group(2))
+    return out
+
+
+def function_blocks(text):
+    """FUNCTION name(args) { body } -> Python source defining name."""
+    out = []
+    for m in re.finditer(r"FUNCTION\s+(\w+)\s*\(([^)]*)\)\s*\{", text):
+        name, args = m.group(1), [a.split("(")[0].strip() for a in m.group(2).split(",")]
+        i, depth, j = m.end(), 1, m.end()
+        while depth:
+            depth += {"{": 1, "}": -1}.get(text[j], 0)
+            j += 1
+        src = to_python(text[i:j - 1], ", ".join(args), ret=name)
+        out.append(src.replace("def rates(", "def %s(" % name, 1))
+    return "\n".join(out)
+
+
 def rates_block(text):
     """Body and argument name of PROCEDURE rates(...) { ... }."""
     m = re.search(r"PROCEDURE\s+rates\s*\((.*?)\)\s*\{", text)
     if not m:
         raise ValueError("no PROCEDURE rates")
-    arg = m.group(1).split("(")[0].strip() or "v"
+    arg = strip_units(m.group(1)).split("(")[0].strip() or "v"
     i, depth = m.end(), 1
     j = i
     while depth:
@@ -38,15 +84,16 @@ def rates_block(text):
     return text[i:j - 1], arg
 
 
-def to_python(body, arg):
-    """The block's statements as the source of a Python function of arg."""
-    body = re.sub(r":[^\n]*", "", body)                      # NMODL comments
+def to_python(body, arg, ret=None):
+    """The block's statements as the source of a Python function of arg
+    (returning the named outputs, or the value assigned to ret)."""
+    body = strip_units(re.sub(r":[^\n]*", "", body))        # NMODL comments, units
     body = body.replace("{", "{\n").replace("}", "\n}\n")
     lines = ["def rates(%s):" % arg]
     depth = 1
     for raw in body.split("\n"):
         s = raw.strip()
-        if not s or s in ("UNITSOFF", "UNITSON") or s.startswith("LOCAL"):
+        if not s or s in ("UNITSOFF", "UNITSON") or s.startswith(("LOCAL", "TABLE", "DEPEND", "FROM")):
             continue
         if s == "}":
             depth -= 1
@@ -68,12 +115,17 @@ def to_python(body, arg):
         if not m:
             raise ValueError("statement not understood: %r" % s)
         lines.append("    " * depth + "%s = %s" % (m.group(1), expr(m.group(2))))
-    lines.append("    return dict(%s)" % ", ".join("%s=locals().get(%r)" % (o, o) for o in ("mInf", "mTau", "hInf", "hTau", "zInf")))
+    if ret:
+        return "\n".join(lines).replace(" %s = " % ret, " __ret = ") + "\n    return __ret"
+    lines.append("    return dict(%s)" % ", ".join("%s=locals().get(%r)" % (o, o) for o in
+                                               ("mInf", "mTau", "hInf", "hTau", "zInf", "minf", "mtau", "hinf",
+                                                "htau", "ninf", "ntau")))
     return "\n".join(lines)
 
 
 def expr(e):
     e = e.replace("^", "**")
+    e = re.sub(r"\bfabs\s*\(", "abs(", e)
     return re.sub(r"\bexp\s*\(", "math.exp(", e)
 
 
@@ -95,7 +147,30 @@ def main():
         out[name] = np.array([[r[k] for k in keys] for r in res])
         out[name + "_keys"] = np.array(keys)
         print(name, arg, keys, out[name].shape)
-    np.savez_compressed(os.path.join(os.path.dirname(os.path.abspath(__file__)), "mod_rates_ttpc.npz"), **out)
+    here = os.path.dirname(os.path.abspath(__file__))
+    np.savez_compressed(os.path.join(here, "mod_rates_ttpc.npz"), **out)
+
+    src = os.path.join(root, "Figures", "FigureS4_mainen")
+    cai = np.concatenate([10.0 ** np.arange(-7.0, -1.0, 0.05), [5e-5, 1e-4]])
+    vm = np.concatenate([np.arange(-120.0, 60.0, 0.37), [-35.0, -30.0, 25.0, -27.0, -50.0, -75.0, -65.0]])
+    out = {"v": vm, "cai": cai}
+    for name in MAINEN:
+        text = strip_comments(open(os.path.join(src, name + ".mod")).read())
+        pars = parameters(text)
+        body, arg = rates_block(text)
+        code = function_blocks(text) + "\n" + to_python(body, arg)
+        ns = {"math": math, "celsius": CELSIUS_MAINEN}
+        ns.update(pars)
+        exec(compile(code, name + ".mod:rates", "exec"), ns)
+        xs = cai if name == "kca" else vm
+        res = [ns["rates"](float(x)) for x in xs]
+        keys = [k for k in ("minf", "mtau", "hinf", "htau", "ninf", "ntau") if res[0][k] is not None]
+        out[name] = np.array([[r[k] for k in keys] for r in res])
+        out[name + "_keys"] = np.array(keys)
+        out[name + "_pnames"] = np.array(sorted(pars))
+        out[name + "_pvalues"] = np.array([pars[k] for k in sorted(pars)])
+        print(name, arg, keys, out[name].shape)
+    np.savez_compressed(os.path.join(here, "mod_rates_mainen.npz"), **out)
 
 
 if __name__ == "__main__":

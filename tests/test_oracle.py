@@ -199,6 +199,36 @@ def test_ttpc_rates_equal_the_nmodl_sources(name):
     np.testing.assert_allclose(got, ref, rtol=1e-10, atol=0)
 
 
+# Mainen set: oracle rate-function index and the parameter slots (cpu_ref.c's
+# layout of a compartment's parameters) that hold each NMODL parameter
+MAINEN_RATES = {"ca": (0, {}), "kca": (2, {"caix": 3, "Ra": 4, "Rb": 5}),
+                "kv": (3, {"tha": 7, "qa": 8, "Ra": 9, "Rb": 10}),
+                "km": (4, {"tha": 12, "qa": 13, "Ra": 14, "Rb": 15}),
+                "na": (5, {"tha": 17, "qa": 18, "Ra": 19, "Rb": 20, "thi1": 21, "thi2": 22, "qi": 23, "thinf": 24,
+                           "qinf": 25, "Rg": 26, "Rd": 27})}
+
+
+@pytest.mark.parametrize("name", sorted(MAINEN_RATES))
+def test_mainen_rates_equal_the_nmodl_sources(name):
+    """The Mainen set's rate functions against its NMODL rates procedures
+    (with their efun/trap0 FUNCTIONs, the PARAMETER defaults and celsius =
+    37, tests/golden/make_mod_rates.py), the defaults placed in the
+    oracle's parameter slots: agreement to 1e-10 relative on 487 voltages
+    plus the half-activation points (calcium values for kca)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "mod_rates_mainen.npz"))
+    mech, slots = MAINEN_RATES[name]
+    pars = dict(zip(g[name + "_pnames"], g[name + "_pvalues"]))
+    P = np.zeros(40)
+    for k, i in slots.items():
+        P[i] = pars[k]
+    keys = list(g[name + "_keys"])
+    slot = {"minf": 0, "mtau": 1, "hinf": 2, "htau": 3, "ninf": 0, "ntau": 1}
+    xs = g["cai"] if name == "kca" else g["v"]
+    got = np.array([[cpu_ref.rates(2, mech, float(x), P)[slot[k]] for k in keys] for x in xs])
+    np.testing.assert_allclose(got, g[name], rtol=1e-10, atol=0)
+
+
 def test_cnexp_exact_contraction():
     """cnexp is the exact solution at fixed v: y1 - yinf = (y0 - yinf) exp(-dt/tau)."""
     for y0, yinf, tau, dt in [(0.1, 0.7, 3.0, 0.025), (0.9, 0.05, 0.3, 0.1), (0.5, 0.5, 10.0, 0.02)]:
