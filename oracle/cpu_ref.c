@@ -25,6 +25,16 @@
  *     decimal double value and powf is pow;
  *   - the trace is unrotated: vhot[t] is v at the record site before step t.
  *
+ * How each part is pinned (tests/test_oracle.py, tests/test_formats_tree.py):
+ *   - the solve: dense linear solves of generated trees, the one-compartment
+ *     backward-Euler closed form, rest and identical-instance invariants;
+ *   - the rate functions: the mechanisms' NMODL rates procedures evaluated
+ *     from the .mod sources (tests/golden/mod_rates_*.npz), to 1e-10;
+ *   - cnexp: the exact per-step contraction at fixed v; the Nernst constant;
+ *   - the tree maps: integer equality with four reference BasicConstSegP.csv;
+ *   - end to end: the reference's TTPC fixture within Code/test_output.py's
+ *     bounds.
+ *
  * Build: see oracle/Makefile (gcc -O2 -ffp-contract=off -fopenmp -shared).
  */
 #include <math.h>
