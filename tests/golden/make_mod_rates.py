@@ -112,7 +112,7 @@ def to_python(body, arg, ret=None):
             depth += 1
             continue
         m = re.match(r"([A-Za-z_]\w*)\s*=\s*(.+)$", s)
-        if not m:
+        if not m or "__" in m.group(1):
             raise ValueError("statement not understood: %r" % s)
         lines.append("    " * depth + "%s = %s" % (m.group(1), expr(m.group(2))))
     if ret:
@@ -123,7 +123,17 @@ def to_python(body, arg, ret=None):
     return "\n".join(lines)
 
 
+_BUILTINS = {"abs": abs, "dict": dict, "locals": locals}
+
+
 def expr(e):
+    """An NMODL arithmetic expression as Python.  The source files are not
+    trusted: only numbers, operators, parentheses, commas and plain names
+    pass, and a name may be called only if it is exp, fabs or a FUNCTION of
+    the same file (checked when the function is compiled, by the restricted
+    builtins and namespace of the exec)."""
+    if not re.fullmatch(r"[\w\s.+\-*/^()<>=!,]*", e) or "__" in e or re.search(r"[A-Za-z_]\w*\s*\.", e):
+        raise ValueError("expression outside the arithmetic subset: %r" % e)
     e = e.replace("^", "**")
     e = re.sub(r"\bfabs\s*\(", "abs(", e)
     return re.sub(r"\bexp\s*\(", "math.exp(", e)
@@ -139,7 +149,7 @@ def main():
     for name in MECHS:
         body, arg = rates_block(open(os.path.join(src, name + ".mod")).read())
         code = to_python(body, arg)
-        ns = {"math": math}
+        ns = {"math": math, "__builtins__": _BUILTINS}
         exec(compile(code, name + ".mod:rates", "exec"), ns)
         xs = ca if name == "SK_E2" else v
         res = [ns["rates"](float(x)) for x in xs]
@@ -159,7 +169,7 @@ def main():
         pars = parameters(text)
         body, arg = rates_block(text)
         code = function_blocks(text) + "\n" + to_python(body, arg)
-        ns = {"math": math, "celsius": CELSIUS_MAINEN}
+        ns = {"math": math, "celsius": CELSIUS_MAINEN, "__builtins__": _BUILTINS}
         ns.update(pars)
         exec(compile(code, name + ".mod:rates", "exec"), ns)
         xs = cai if name == "kca" else vm
