@@ -28,8 +28,9 @@
  * How each part is pinned (tests/test_oracle.py, tests/test_formats_tree.py):
  *   - the solve: dense linear solves of generated trees, the one-compartment
  *     backward-Euler closed form, rest and identical-instance invariants;
- *   - the rate functions: the mechanisms' NMODL rates procedures evaluated
- *     from the .mod sources (tests/golden/mod_rates_*.npz), to 1e-10;
+ *   - the rate functions and currents: the mechanisms' NMODL rates
+ *     procedures and BREAKPOINT blocks evaluated from the .mod sources
+ *     (tests/golden/mod_rates_*.npz, mod_currents.npz), to 1e-10 and 1e-12;
  *   - cnexp: the exact per-step contraction at fixed v; the Nernst constant;
  *   - the tree maps: integer equality with four reference BasicConstSegP.csv;
  *   - end to end: the reference's TTPC fixture within Code/test_output.py's
@@ -654,6 +655,26 @@ int ref_rates(int ms, int mech, double x, const double *P, double *out)
         }
     }
     return -1;
+}
+
+/* The membrane current of one mechanism alone (the BREAKPOINT terms of
+ * tt_cur / mn_cur with only that mechanism present) at v, for states st
+ * (state_macro slots) and compartment parameters P, with eca given:
+ * out[0] = the current, out[1] = its calcium part.  Returns -1 for a
+ * mechanism without a current term. */
+int ref_current(int ms, int mech, double v, const double *st, const double *P, double eca, double *out)
+{
+    int has[16] = {0};
+    if (mech < 0 || mech >= 16) return -1;
+    has[mech] = 1;
+    ionreg io = {5e-5, eca, 0.0};
+    double s = 0.0;
+    if (ms == MS_TTPC && mech != 2) tt_cur(has, P, v, st, &io, 1, &s);
+    else if (ms == MS_MAINEN && mech != 1) mn_cur(has, P, v, st, &io, 1, &s);
+    else return -1;
+    out[0] = s;
+    out[1] = io.ica;
+    return 0;
 }
 
 double ref_cnexp(double y, double yinf, double tau, double dt) { return cnexp1(y, yinf, tau, dt); }

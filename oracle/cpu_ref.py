@@ -59,6 +59,8 @@ def lib():
         L.ref_rates.restype = C.c_int
         L.ref_rates.argtypes = [C.c_int, C.c_int, C.c_double, _DP, _DP]
         L.ref_cnexp.restype = C.c_double
+        L.ref_current.argtypes = [C.c_int, C.c_int, C.c_double, _DP, _DP, C.c_double, _DP]
+        L.ref_current.restype = C.c_int
         L.ref_cnexp.argtypes = [C.c_double] * 4
         L.ref_ktf.restype = C.c_double
         L.ref_ktf.argtypes = [C.c_int]
@@ -143,6 +145,20 @@ def rates(mechset, mech, x, params=None):
     rc = L.ref_rates(mechset, mech, float(x), P.ctypes.data_as(_DP), out.ctypes.data_as(_DP))
     if rc != 0:
         raise ValueError("no rate function for that mechanism")
+    return out
+
+
+def current(mechset, mech, v, st, params, eca=120.0):
+    """(current, calcium part) of one mechanism alone: st by state slot,
+    params by parameter slot."""
+    L = lib()
+    out = np.zeros(2)
+    S = np.ascontiguousarray(st, dtype=np.float64)
+    P = np.ascontiguousarray(params, dtype=np.float64)
+    rc = L.ref_current(mechset, mech, float(v), S.ctypes.data_as(_DP), P.ctypes.data_as(_DP), float(eca),
+                       out.ctypes.data_as(_DP))
+    if rc != 0:
+        raise ValueError("no current term for that mechanism")
     return out
 
 

@@ -1,4 +1,7 @@
-"""Writes tests/golden/mod_rates_ttpc.npz: the gating rates of the BBP TTPC
+"""Writes tests/golden/mod_rates_ttpc.npz, mod_rates_mainen.npz and
+mod_currents.npz.
+
+mod_rates_ttpc.npz: the gating rates of the BBP TTPC
 mechanisms evaluated from the NMODL sources' own PROCEDURE rates blocks
 (Figures/Figure2_BBP_TTPC_clean/*.mod in the reference tree), on a voltage
 grid (a calcium grid for SK_E2), so that the oracle's C restatement of the
@@ -181,6 +184,59 @@ def main():
         out[name + "_pvalues"] = np.array([pars[k] for k in sorted(pars)])
         print(name, arg, keys, out[name].shape)
     np.savez_compressed(os.path.join(here, "mod_rates_mainen.npz"), **out)
+    currents(root, here)
+
+
+# reversal potentials the models set (not part of the .mod files), as the
+# oracle holds them: TTPC Ek -85, ENa 50; Mainen Ek -90, ENa 60 (mV)
+IONS = {"ttpc": {"ek": -85.0, "ena": 50.0}, "mainen": {"ek": -90.0, "ena": 60.0}}
+
+
+def currents(root, here, nsamples=200):
+    """mod_currents.npz: the BREAKPOINT current of each mechanism (the sum of
+    the ionic and nonspecific currents it writes) at random v, states,
+    conductances and eca, from the .mod BREAKPOINT block."""
+    rng = np.random.default_rng(77)
+    out = {}
+    sets = [("ttpc", os.path.join(root, "Figures", "Figure2_BBP_TTPC_clean"),
+             ["Ca_HVA", "Ca_LVAst", "Ih", "Im", "K_Pst", "K_Tst", "Nap_Et2", "NaTa_t", "NaTs2_t", "SK_E2", "SKv3_1"]),
+            ("mainen", os.path.join(root, "Figures", "FigureS4_mainen"), ["ca", "kca", "km", "kv", "na", "pasx"])]
+    for tag, src, names in sets:
+        for name in names:
+            text = strip_comments(open(os.path.join(src, name + ".mod")).read())
+            m = re.search(r"BREAKPOINT\s*\{(.*?)\n\}", text, flags=re.S)
+            body = "\n".join(l for l in m.group(1).split("\n") if "SOLVE" not in l)
+            neuron = re.search(r"NEURON\s*\{(.*?)\}", text, flags=re.S).group(1)
+            writes = re.findall(r"WRITE\s+(\w+)", neuron) + re.findall(r"NONSPECIFIC_CURRENT\s+(\w+)", neuron)
+            stm = re.search(r"STATE\s*\{(.*?)\}", text, flags=re.S)
+            states = [w for w in re.findall(r"[A-Za-z_]\w*", stm.group(1)) if w not in ("FROM", "TO")] if stm else []
+            pars = parameters(text)
+            code = to_python(body, "v").replace("return dict(", "return dict(__cur=%s, " % " + ".join(writes), 1)
+            code = code.replace("__cur=", "cur=")
+            used = set(re.findall(r"[A-Za-z_]\w*", body))
+            inputs = ["v"] + states + sorted(k for k in pars if k in used) + [k for k in ("eca",) if k in used]
+            X = np.empty((nsamples, len(inputs)))
+            y = np.empty(nsamples)
+            for j in range(nsamples):
+                vals = {"v": rng.uniform(-100.0, 50.0), "eca": rng.uniform(50.0, 150.0)}
+                for st in states:
+                    vals[st] = rng.uniform(0.0, 1.0)
+                for k in pars:
+                    if k in used:
+                        vals[k] = 10 ** rng.uniform(-4, 0) if ("bar" in k or k == "g") else rng.uniform(-90.0, -30.0)
+                ns = {"math": math, "__builtins__": _BUILTINS}
+                ns.update(IONS[tag])
+                if "q10" in pars:
+                    ns["tadj"] = pars["q10"] ** ((CELSIUS_MAINEN - pars["temp"]) / 10)
+                ns.update({k: vals[k] for k in inputs if k != "v"})
+                exec(compile(code, name + ".mod:BREAKPOINT", "exec"), ns)
+                X[j] = [vals[k] for k in inputs]
+                y[j] = ns["rates"](vals["v"])["cur"]
+            out[tag + ":" + name + "_inputs"] = np.array(inputs)
+            out[tag + ":" + name + "_x"] = X
+            out[tag + ":" + name + "_i"] = y
+            print("current", tag, name, inputs, writes)
+    np.savez_compressed(os.path.join(here, "mod_currents.npz"), **out)
 
 
 if __name__ == "__main__":

@@ -229,6 +229,55 @@ def test_mainen_rates_equal_the_nmodl_sources(name):
     np.testing.assert_allclose(got, g[name], rtol=1e-10, atol=0)
 
 
+# oracle slots of each mechanism's BREAKPOINT inputs: (mechanism set, index,
+# {NMODL name: ("st", state slot) | ("P", parameter slot)})
+CURRENT_SLOTS = {
+    "ttpc:Ca_HVA": (1, 0, {"m": ("st", 1), "h": ("st", 2), "gCa_HVAbar": ("P", 0)}),
+    "ttpc:Ca_LVAst": (1, 1, {"m": ("st", 3), "h": ("st", 4), "gCa_LVAstbar": ("P", 1)}),
+    "ttpc:Ih": (1, 3, {"m": ("st", 5), "gIhbar": ("P", 6), "ehcn": ("P", 7)}),
+    "ttpc:Im": (1, 4, {"m": ("st", 6), "gImbar": ("P", 8)}),
+    "ttpc:K_Pst": (1, 5, {"m": ("st", 7), "h": ("st", 8), "gK_Pstbar": ("P", 9)}),
+    "ttpc:K_Tst": (1, 6, {"m": ("st", 9), "h": ("st", 10), "gK_Tstbar": ("P", 10)}),
+    "ttpc:Nap_Et2": (1, 7, {"m": ("st", 11), "h": ("st", 12), "gNap_Et2bar": ("P", 11)}),
+    "ttpc:NaTa_t": (1, 8, {"m": ("st", 13), "h": ("st", 14), "gNaTa_tbar": ("P", 12)}),
+    "ttpc:NaTs2_t": (1, 9, {"m": ("st", 15), "h": ("st", 16), "gNaTs2_tbar": ("P", 13)}),
+    "ttpc:SK_E2": (1, 11, {"z": ("st", 18), "gSK_E2bar": ("P", 16)}),
+    "ttpc:SKv3_1": (1, 12, {"m": ("st", 17), "gSKv3_1bar": ("P", 18)}),
+    "mainen:ca": (2, 0, {"m": ("st", 1), "h": ("st", 2), "gbar": ("P", 0)}),
+    "mainen:kca": (2, 2, {"n": ("st", 4), "gbar": ("P", 2)}),
+    "mainen:kv": (2, 3, {"n": ("st", 5), "gbar": ("P", 6)}),
+    "mainen:km": (2, 4, {"n": ("st", 6), "gbar": ("P", 11)}),
+    "mainen:na": (2, 5, {"m": ("st", 7), "h": ("st", 8), "gbar": ("P", 16)}),
+    "mainen:pasx": (2, 6, {"g": ("P", 28), "e": ("P", 29)}),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CURRENT_SLOTS))
+def test_currents_equal_the_nmodl_breakpoints(name):
+    """Each mechanism's current in the oracle (its term of tt_cur / mn_cur,
+    the mechanism alone) against the mechanism's NMODL BREAKPOINT block
+    evaluated at 200 random (v, states, conductances, eca) samples
+    (tests/golden/make_mod_rates.py, with the models' reversal potentials and
+    the Mainen set's tadj at 37 degC): equal to 1e-12 relative, and the
+    calcium channels' current is all calcium current."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "mod_currents.npz"))
+    ms, mech, slots = CURRENT_SLOTS[name]
+    inputs = list(g[name + "_inputs"])
+    X, want = g[name + "_x"], g[name + "_i"]
+    got = np.empty_like(want)
+    for j, row in enumerate(X):
+        vals = dict(zip(inputs, row))
+        st, P = np.zeros(24), np.zeros(40)
+        for k, (kind, i) in slots.items():
+            (st if kind == "st" else P)[i] = vals[k]
+        cur, ica = cpu_ref.current(ms, mech, vals["v"], st, P, vals.get("eca", 120.0))
+        got[j] = cur
+        if "eca" in vals:
+            assert ica == cur
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-15 * np.abs(want).max())
+
+
 def test_cnexp_exact_contraction():
     """cnexp is the exact solution at fixed v: y1 - yinf = (y0 - yinf) exp(-dt/tau)."""
     for y0, yinf, tau, dt in [(0.1, 0.7, 3.0, 0.025), (0.9, 0.05, 0.3, 0.1), (0.5, 0.5, 10.0, 0.02)]:
