@@ -202,8 +202,8 @@ def test_ttpc_rates_equal_the_nmodl_sources(name):
 # Mainen set: oracle rate-function index and the parameter slots (cpu_ref.c's
 # layout of a compartment's parameters) that hold each NMODL parameter
 MAINEN_RATES = {"ca": (0, {}), "kca": (2, {"caix": 3, "Ra": 4, "Rb": 5}),
-                "kv": (3, {"tha": 7, "qa": 8, "Ra": 9, "Rb": 10}),
-                "km": (4, {"tha": 12, "qa": 13, "Ra": 14, "Rb": 15}),
+                "km": (3, {"tha": 7, "qa": 8, "Ra": 9, "Rb": 10}),
+                "kv": (4, {"tha": 12, "qa": 13, "Ra": 14, "Rb": 15}),
                 "na": (5, {"tha": 17, "qa": 18, "Ra": 19, "Rb": 20, "thi1": 21, "thi2": 22, "qi": 23, "thinf": 24,
                            "qinf": 25, "Rg": 26, "Rd": 27})}
 
@@ -245,8 +245,8 @@ CURRENT_SLOTS = {
     "ttpc:SKv3_1": (1, 12, {"m": ("st", 17), "gSKv3_1bar": ("P", 18)}),
     "mainen:ca": (2, 0, {"m": ("st", 1), "h": ("st", 2), "gbar": ("P", 0)}),
     "mainen:kca": (2, 2, {"n": ("st", 4), "gbar": ("P", 2)}),
-    "mainen:kv": (2, 3, {"n": ("st", 5), "gbar": ("P", 6)}),
-    "mainen:km": (2, 4, {"n": ("st", 6), "gbar": ("P", 11)}),
+    "mainen:km": (2, 3, {"n": ("st", 5), "gbar": ("P", 6)}),
+    "mainen:kv": (2, 4, {"n": ("st", 6), "gbar": ("P", 11)}),
     "mainen:na": (2, 5, {"m": ("st", 7), "h": ("st", 8), "gbar": ("P", 16)}),
     "mainen:pasx": (2, 6, {"g": ("P", 28), "e": ("P", 29)}),
 }
