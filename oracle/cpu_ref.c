@@ -31,6 +31,8 @@
  *   - the rate functions and currents: the mechanisms' NMODL rates
  *     procedures and BREAKPOINT blocks evaluated from the .mod sources
  *     (tests/golden/mod_rates_*.npz, mod_currents.npz), to 1e-10 and 1e-12;
+ *   - the TTPC state updates: the exact solution over dt of each
+ *     mechanism's NMODL DERIVATIVE block (tests/golden/mod_derivs_ttpc.npz);
  *   - cnexp: the exact per-step contraction at fixed v; the Nernst constant;
  *   - the tree maps: integer equality with four reference BasicConstSegP.csv;
  *   - end to end: the reference's TTPC fixture within Code/test_output.py's
@@ -674,6 +676,28 @@ int ref_current(int ms, int mech, double v, const double *st, const double *P, d
     else return -1;
     out[0] = s;
     out[1] = io.ica;
+    return 0;
+}
+
+/* One mechanism's state update alone (its DERIVATIVE terms of tt_deriv /
+ * mn_deriv) over dt at v: st updated in place, *cai in and out, ica the
+ * calcium current of the step.  Returns -1 for a mechanism without states. */
+int ref_deriv(int ms, int mech, double dt, double v, double *st, const double *P, double *cai, double ica)
+{
+    int has[16] = {0};
+    if (mech < 0 || mech >= 16) return -1;
+    has[mech] = 1;
+    if (ms == MS_TTPC && mech != 10) {
+        ionreg io = {*cai, tt_eca(*cai), ica};
+        tt_deriv(has, P, dt, v, st, &io);
+        *cai = io.cai;
+    } else if (ms == MS_MAINEN && mech != 6) {
+        ionreg io = {*cai, mn_eca(*cai), ica};
+        mn_deriv(has, P, dt, v, st, &io);
+        *cai = io.cai;
+    } else {
+        return -1;
+    }
     return 0;
 }
 

@@ -61,6 +61,8 @@ def lib():
         L.ref_cnexp.restype = C.c_double
         L.ref_current.argtypes = [C.c_int, C.c_int, C.c_double, _DP, _DP, C.c_double, _DP]
         L.ref_current.restype = C.c_int
+        L.ref_deriv.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, _DP, _DP, C.POINTER(C.c_double), C.c_double]
+        L.ref_deriv.restype = C.c_int
         L.ref_cnexp.argtypes = [C.c_double] * 4
         L.ref_ktf.restype = C.c_double
         L.ref_ktf.argtypes = [C.c_int]
@@ -160,6 +162,19 @@ def current(mechset, mech, v, st, params, eca=120.0):
     if rc != 0:
         raise ValueError("no current term for that mechanism")
     return out
+
+
+def deriv(mechset, mech, dt, v, st, params, cai=5e-5, ica=0.0):
+    """One mechanism's state update alone: (new states by slot, new cai)."""
+    L = lib()
+    S = np.array(st, dtype=np.float64)
+    P = np.ascontiguousarray(params, dtype=np.float64)
+    c = C.c_double(cai)
+    rc = L.ref_deriv(mechset, mech, float(dt), float(v), S.ctypes.data_as(_DP), P.ctypes.data_as(_DP), C.byref(c),
+                     float(ica))
+    if rc != 0:
+        raise ValueError("no state update for that mechanism")
+    return S, c.value
 
 
 def cnexp(y, yinf, tau, dt):

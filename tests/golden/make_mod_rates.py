@@ -1,5 +1,5 @@
-"""Writes tests/golden/mod_rates_ttpc.npz, mod_rates_mainen.npz and
-mod_currents.npz.
+"""Writes tests/golden/mod_rates_ttpc.npz, mod_rates_mainen.npz,
+mod_currents.npz and mod_derivs_ttpc.npz.
 
 mod_rates_ttpc.npz: the gating rates of the BBP TTPC
 mechanisms evaluated from the NMODL sources' own PROCEDURE rates blocks
@@ -185,6 +185,83 @@ def main():
         print(name, arg, keys, out[name].shape)
     np.savez_compressed(os.path.join(here, "mod_rates_mainen.npz"), **out)
     currents(root, here)
+    derivatives(root, here)
+
+
+# NEURON's faraday constant as the generated code holds it (FARADAY in the
+# .mod UNITS block)
+FARADAY = 96485.309
+
+
+def derivatives(root, here, nsamples=200):
+    """mod_derivs_ttpc.npz: each TTPC mechanism's states after one cnexp step
+    of its DERIVATIVE block.  Every primed equation is linear in its state
+    (checked), y' = a + b y with a and b from two evaluations of the
+    right-hand side (rates called first, as the block does), so the step's
+    exact result y(dt) = y_inf + (y0 - y_inf) exp(b dt), y_inf = -a / b, is
+    what cnexp computes."""
+    rng = np.random.default_rng(78)
+    src = os.path.join(root, "Figures", "Figure2_BBP_TTPC_clean")
+    out = {}
+    for name in MECHS + ["CaDynamics_E2"]:
+        text = strip_comments(open(os.path.join(src, name + ".mod")).read())
+        m = re.search(r"DERIVATIVE\s+\w+\s*\{(.*?)\}", text, flags=re.S)
+        call, eqs = None, []
+        for line in m.group(1).split("\n"):
+            line = line.strip()
+            if not line:
+                continue
+            mm = re.match(r"rates\s*\(\s*(\w*)\s*\)$", line)
+            if mm:
+                call = mm.group(1) or "v"
+                continue
+            mm = re.match(r"([A-Za-z_]\w*)'\s*=\s*(.+)$", line)
+            if not mm or "__" in mm.group(1):
+                raise ValueError("statement not understood: %r" % line)
+            eqs.append((mm.group(1), compile(expr(mm.group(2)), name + ".mod:DERIVATIVE", "eval")))
+        rfun = None
+        if call:
+            body, arg = rates_block(text)
+            ns0 = {"math": math, "__builtins__": _BUILTINS}
+            exec(compile(to_python(body, arg), name + ".mod:rates", "exec"), ns0)
+            rfun = ns0["rates"]
+        pars = parameters(text)
+        used = set(re.findall(r"[A-Za-z_]\w*", m.group(1)))
+        pnames = sorted(k for k in pars if k in used)
+        states = [y for y, _ in eqs]
+        inputs = ["v", "dt", "cai", "ica"] + [y for y in states if y != "cai"] + pnames
+        X = np.empty((nsamples, len(inputs)))
+        Y = np.empty((nsamples, len(states)))
+        for j in range(nsamples):
+            vals = {"v": rng.uniform(-100.0, 50.0), "dt": float(rng.choice([0.02, 0.025, 0.1])),
+                    "cai": 10 ** rng.uniform(-5, -2), "ica": -10 ** rng.uniform(-4, 0)}
+            for y in states:
+                if y != "cai":
+                    vals[y] = rng.uniform(0.0, 1.0)
+            for k in pnames:
+                vals[k] = {"gamma": rng.uniform(5e-4, 0.05), "decay": rng.uniform(20.0, 1000.0),
+                           "depth": rng.uniform(0.05, 1.0), "minCai": 1e-4}.get(k, rng.uniform(1.0, 10.0))
+            ns = {"math": math, "__builtins__": _BUILTINS, "FARADAY": FARADAY}
+            ns.update(vals)
+            if rfun:
+                ns.update({k: w for k, w in rfun(vals[call]).items() if w is not None})
+            for i, (y, code) in enumerate(eqs):
+                y0 = vals[y]
+                ns[y] = 0.0
+                a = eval(code, ns)
+                ns[y] = 1.0
+                b = eval(code, ns) - a
+                ns[y] = y0
+                assert abs(eval(code, ns) - (a + b * y0)) <= 1e-12 * max(abs(a), abs(b * y0), 1e-300)
+                yinf = -a / b
+                Y[j, i] = yinf + (y0 - yinf) * math.exp(b * vals["dt"])
+            X[j] = [vals[k] for k in inputs]
+        out[name + "_inputs"] = np.array(inputs)
+        out[name + "_states"] = np.array(states)
+        out[name + "_x"] = X
+        out[name + "_y"] = Y
+        print("derivative", name, inputs, states)
+    np.savez_compressed(os.path.join(here, "mod_derivs_ttpc.npz"), **out)
 
 
 # reversal potentials the models set (not part of the .mod files), as the

@@ -278,6 +278,43 @@ def test_currents_equal_the_nmodl_breakpoints(name):
     np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-15 * np.abs(want).max())
 
 
+# TTPC mechanisms with states: oracle index, state slots, parameter slots
+DERIV_SLOTS = {
+    "Ca_HVA": (0, {"m": 1, "h": 2}, {}), "Ca_LVAst": (1, {"m": 3, "h": 4}, {}),
+    "CaDynamics_E2": (2, {}, {"gamma": 2, "decay": 3, "depth": 4, "minCai": 5}),
+    "Ih": (3, {"m": 5}, {}), "Im": (4, {"m": 6}, {}), "K_Pst": (5, {"m": 7, "h": 8}, {}),
+    "K_Tst": (6, {"m": 9, "h": 10}, {}), "Nap_Et2": (7, {"m": 11, "h": 12}, {}),
+    "NaTa_t": (8, {"m": 13, "h": 14}, {}), "NaTs2_t": (9, {"m": 15, "h": 16}, {}),
+    "SK_E2": (11, {"z": 18}, {"zTau": 17}), "SKv3_1": (12, {"m": 17}, {}),
+}
+
+
+@pytest.mark.parametrize("name", sorted(DERIV_SLOTS))
+def test_ttpc_state_updates_equal_the_nmodl_derivatives(name):
+    """One step of each TTPC mechanism's states in the oracle (the mechanism
+    alone) against the exact solution over dt of its NMODL DERIVATIVE block,
+    whose equations are linear in their state (tests/golden/
+    make_mod_rates.py): 200 random (v, dt, states, cai, ica, parameters)
+    samples agree to 1e-11 relative (the two forms of the exponential step
+    round differently by up to about 1e-12)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "mod_derivs_ttpc.npz"))
+    mech, sslots, pslots = DERIV_SLOTS[name]
+    inputs, states = list(g[name + "_inputs"]), list(g[name + "_states"])
+    X, Y = g[name + "_x"], g[name + "_y"]
+    got = np.empty_like(Y)
+    for j, row in enumerate(X):
+        vals = dict(zip(inputs, row))
+        st, P = np.zeros(24), np.zeros(40)
+        for k, i in sslots.items():
+            st[i] = vals[k]
+        for k, i in pslots.items():
+            P[i] = vals[k]
+        new, cai = cpu_ref.deriv(1, mech, vals["dt"], vals["v"], st, P, vals["cai"], vals["ica"])
+        got[j] = [cai if y == "cai" else new[sslots[y]] for y in states]
+    np.testing.assert_allclose(got, Y, rtol=1e-11, atol=0)
+
+
 def test_cnexp_exact_contraction():
     """cnexp is the exact solution at fixed v: y1 - yinf = (y0 - yinf) exp(-dt/tau)."""
     for y0, yinf, tau, dt in [(0.1, 0.7, 3.0, 0.025), (0.9, 0.05, 0.3, 0.1), (0.5, 0.5, 10.0, 0.02)]:
