@@ -31,8 +31,8 @@
  *   - the rate functions and currents: the mechanisms' NMODL rates
  *     procedures and BREAKPOINT blocks evaluated from the .mod sources
  *     (tests/golden/mod_rates_*.npz, mod_currents.npz), to 1e-10 and 1e-12;
- *   - the TTPC state updates: the exact solution over dt of each
- *     mechanism's NMODL DERIVATIVE block (tests/golden/mod_derivs_ttpc.npz);
+ *   - the state updates: the exact solution over dt of each mechanism's
+ *     NMODL DERIVATIVE block (tests/golden/mod_derivs_{ttpc,mainen}.npz);
  *   - cnexp: the exact per-step contraction at fixed v; the Nernst constant;
  *   - the tree maps: integer equality with four reference BasicConstSegP.csv;
  *   - end to end: the reference's TTPC fixture within Code/test_output.py's

@@ -1,5 +1,5 @@
 """Writes tests/golden/mod_rates_ttpc.npz, mod_rates_mainen.npz,
-mod_currents.npz and mod_derivs_ttpc.npz.
+mod_currents.npz, mod_derivs_ttpc.npz and mod_derivs_mainen.npz.
 
 mod_rates_ttpc.npz: the gating rates of the BBP TTPC
 mechanisms evaluated from the NMODL sources' own PROCEDURE rates blocks
@@ -186,6 +186,7 @@ def main():
     np.savez_compressed(os.path.join(here, "mod_rates_mainen.npz"), **out)
     currents(root, here)
     derivatives(root, here)
+    derivatives_mainen(root, here)
 
 
 # NEURON's faraday constant as the generated code holds it (FARADAY in the
@@ -314,6 +315,94 @@ def currents(root, here, nsamples=200):
             out[tag + ":" + name + "_i"] = y
             print("current", tag, name, inputs, writes)
     np.savez_compressed(os.path.join(here, "mod_currents.npz"), **out)
+
+
+# model values the Mainen cell sets outside its .mod files: vshift of na
+# (fit.hoc: vshift_na = -5) and ca (0), and cad's fitted shell depth, as the
+# oracle holds them
+MAINEN_MODEL = {"na": {"vshift": -5.0}, "ca": {"vshift": 0.0}, "cad": {"depth": 0.09334562733124982}}
+
+
+def derivatives_mainen(root, here, nsamples=200):
+    """mod_derivs_mainen.npz: as derivatives(), for the Mainen set.  Its
+    blocks call trates(x) (the rates procedure at x; the TABLE is not used)
+    or rates(cai), and cad's block sets a clamped drive term first."""
+    rng = np.random.default_rng(79)
+    src = os.path.join(root, "Figures", "FigureS4_mainen")
+    out = {}
+    for name in ["ca", "cad", "kca", "km", "kv", "na"]:
+        text = strip_comments(open(os.path.join(src, name + ".mod")).read())
+        m = re.search(r"DERIVATIVE\s+\w+\s*\{(.*?)\n\}", text, flags=re.S)
+        pars = parameters(text)
+        pars.update(MAINEN_MODEL.get(name, {}))
+        call, pre, eqs = None, [], []
+        for line in m.group(1).split("\n"):
+            line = line.strip()
+            if not line:
+                continue
+            mm = re.match(r"t?rates\s*\(\s*([^)]*)\)$", line)
+            if mm:
+                call = compile(expr(mm.group(1) or "v"), name + ".mod:call", "eval")
+                continue
+            mm = re.match(r"([A-Za-z_]\w*)'\s*=\s*(.+)$", line)
+            if mm and "__" not in mm.group(1):
+                eqs.append((mm.group(1), compile(expr(mm.group(2)), name + ".mod:DERIVATIVE", "eval")))
+                continue
+            mm = re.match(r"if\s*\((.*)\)\s*\{\s*([A-Za-z_]\w*)\s*=\s*(.+?)\s*\}$", line)
+            if mm and "__" not in mm.group(2):
+                pre.append((mm.group(2), compile(expr(mm.group(3)), name, "eval"), compile(expr(mm.group(1)), name, "eval")))
+                continue
+            mm = re.match(r"([A-Za-z_]\w*)\s*=\s*(.+)$", line)
+            if mm and "__" not in mm.group(1):
+                if eqs:
+                    continue                     # assignments after the equations (cai = ca)
+                pre.append((mm.group(1), compile(expr(mm.group(2)), name, "eval"), None))
+                continue
+            raise ValueError("statement not understood: %r" % line)
+        rfun = None
+        if call:
+            body, arg = rates_block(text)
+            ns0 = {"math": math, "__builtins__": _BUILTINS, "celsius": CELSIUS_MAINEN}
+            ns0.update(pars)
+            code = function_blocks(text) + "\n" + to_python(body, arg)
+            exec(compile(code, name + ".mod:rates", "exec"), ns0)
+            rfun = ns0["rates"]
+        states = [y for y, _ in eqs]
+        inputs = ["v", "dt", "cai", "ica"] + [y for y in states if y != "cai"]
+        X = np.empty((nsamples, len(inputs)))
+        Y = np.empty((nsamples, len(states)))
+        for j in range(nsamples):
+            vals = {"v": rng.uniform(-100.0, 50.0), "dt": float(rng.choice([0.02, 0.025, 0.1])),
+                    "cai": 10 ** rng.uniform(-5, -2), "ica": -10 ** rng.uniform(-4, 0) * rng.choice([1.0, -0.1])}
+            for y in states:
+                vals[y] = rng.uniform(0.0, 1.0) if y != "ca" else 10 ** rng.uniform(-5, -2)
+            ns = {"math": math, "__builtins__": _BUILTINS, "FARADAY": FARADAY, "celsius": CELSIUS_MAINEN}
+            ns.update(pars)
+            ns.update(vals)
+            for k, code, cond in pre:
+                if cond is None or eval(cond, ns):
+                    ns[k] = eval(code, ns)
+            if rfun:
+                ns.update({k: w for k, w in rfun(eval(call, ns)).items() if w is not None})
+            for i, (y, code) in enumerate(eqs):
+                y0 = vals[y]
+                ns[y] = 0.0
+                a = eval(code, ns)
+                ns[y] = 1.0
+                b = eval(code, ns) - a
+                ns[y] = y0
+                assert abs(eval(code, ns) - (a + b * y0)) <= 1e-12 * max(abs(a), abs(b * y0), 1e-300)
+                yinf = -a / b
+                Y[j, i] = yinf + (y0 - yinf) * math.exp(b * vals["dt"])
+            X[j] = [vals[k] for k in inputs]
+        out[name + "_inputs"] = np.array(inputs)
+        out[name + "_states"] = np.array(states)
+        out[name + "_x"] = X
+        out[name + "_y"] = Y
+        out[name + "_pnames"] = np.array(sorted(pars))
+        out[name + "_pvalues"] = np.array([pars[k] for k in sorted(pars)])
+        print("derivative mainen", name, inputs, states)
+    np.savez_compressed(os.path.join(here, "mod_derivs_mainen.npz"), **out)
 
 
 if __name__ == "__main__":

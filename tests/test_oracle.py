@@ -315,6 +315,40 @@ def test_ttpc_state_updates_equal_the_nmodl_derivatives(name):
     np.testing.assert_allclose(got, Y, rtol=1e-11, atol=0)
 
 
+MAINEN_DERIV_STATES = {"ca": {"m": 1, "h": 2}, "cad": {"ca": 3}, "kca": {"n": 4}, "km": {"n": 5}, "kv": {"n": 6},
+                       "na": {"m": 7, "h": 8}}
+MAINEN_INDEX = {"ca": 0, "cad": 1, "kca": 2, "km": 3, "kv": 4, "na": 5}
+
+
+@pytest.mark.parametrize("name", sorted(MAINEN_DERIV_STATES))
+def test_mainen_state_updates_equal_the_nmodl_derivatives(name):
+    """As for the TTPC set: each Mainen mechanism's update alone against the
+    exact solution over dt of its DERIVATIVE block (trates at v + vshift
+    with the model's vshift, cad's clamped drive with the model's shell
+    depth, the PARAMETER defaults in the oracle's parameter slots), 200
+    random samples, 1e-11 relative."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "mod_derivs_mainen.npz"))
+    sslots = MAINEN_DERIV_STATES[name]
+    pslots = MAINEN_RATES[name][1] if name in MAINEN_RATES else {}
+    pars = dict(zip(g[name + "_pnames"], g[name + "_pvalues"]))
+    inputs, states = list(g[name + "_inputs"]), list(g[name + "_states"])
+    X, Y = g[name + "_x"], g[name + "_y"]
+    got = np.empty_like(Y)
+    for j, row in enumerate(X):
+        vals = dict(zip(inputs, row))
+        st, P = np.zeros(24), np.zeros(40)
+        for k, i in sslots.items():
+            st[i] = vals[k]
+        for k, i in pslots.items():
+            P[i] = pars[k]
+        new, cai = cpu_ref.deriv(2, MAINEN_INDEX[name], vals["dt"], vals["v"], st, P, vals["cai"], vals["ica"])
+        got[j] = [new[sslots[y]] for y in states]
+        if name == "cad":
+            assert cai == new[3]
+    np.testing.assert_allclose(got, Y, rtol=1e-11, atol=0)
+
+
 def test_cnexp_exact_contraction():
     """cnexp is the exact solution at fixed v: y1 - yinf = (y0 - yinf) exp(-dt/tau)."""
     for y0, yinf, tau, dt in [(0.1, 0.7, 3.0, 0.025), (0.9, 0.05, 0.3, 0.1), (0.5, 0.5, 10.0, 0.02)]:
