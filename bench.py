@@ -1,14 +1,18 @@
 """Benchmark of the simulation loop on one MI355X.
 
 One bench step = one pass of the simulation over the whole batch: every
-(parameter set, stimulus) run advanced through a window of dt steps by
-libneurogpu's HIP kernel, inputs already resident in device memory.  The
-headline workload is the thick-tufted-pyramidal-like cell (1,200 compartments,
-the 13 BBP TTPC mechanisms), 8,192 parameter sets x 5 step currents,
-dt = 0.025 ms; a bench step is its whole 300 ms protocol, 12,000 dt steps in
-one persistent launch.  ``--workload active500`` measures the active
-500-compartment cell over its full 10,000 steps; ``--nt`` shortens the
-protocol (profiling runs).
+(parameter set, stimulus) run advanced through a window of dt steps by one
+launch of libneurogpu's HIP kernel (ngpu_run_steps), inputs already resident
+in device memory, the state carried from window to window.  The headline
+workload is the thick-tufted-pyramidal-like cell (1,200 compartments, the 13
+BBP TTPC mechanisms), 8,192 parameter sets x 5 step currents, dt = 0.025 ms,
+300 ms = 12,000 dt steps: ``--steps K`` timed windows of 12,000 / K steps
+cover the protocol exactly once, after ``--warmup W`` untimed windows.
+``--workload active500`` measures the active 500-compartment cell over its
+10,000 steps; ``--nt`` shortens the protocol (profiling runs).
+
+HBM traffic per launch comes from two rocprofv3 PMC passes (FETCH_SIZE,
+WRITE_SIZE) over a child process, each calibrated on a device-to-device copy.
 
 Prints one JSON line (the last line of stdout).
 """
@@ -34,29 +38,77 @@ def algorithmic_bytes_per_comp_step(model) -> float:
     return 16.0 * (1.0 + model.state_count()) + 8.0 * len(model.rec_sites) / model.tree.N
 
 
-def traffic_from_profile(name: str):
-    """HBM bytes per launch recorded by the rocprofv3 PMC passes, if present
-    (profiles/<name>_traffic.txt, line 'traffic_bytes_per_launch: <value>')."""
-    path = os.path.join(_ROOT, "profiles", f"{name}_traffic.txt")
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
+SIM_KERNEL = "ngpu_sim_kernel"
+COPY_KERNEL = "copyBuffer"
 
 
-def fp64_from_profile(name: str):
-    """Issued FP64 flops per compartment-step from the rocprofv3 FP64
-    instruction-count pass (profiles/<name>_fp64.txt, line
-    'fp64_flop_per_comp_step: <value>'), if present."""
-    path = os.path.join(_ROOT, "profiles", f"{name}_fp64.txt")
-    if not os.path.exists(path):
+def _counter_rows(outdir: str):
+    """Rows of every rocprofv3 counter-collection CSV written under outdir."""
+    import csv
+    import glob
+    rows = []
+    for path in glob.glob(os.path.join(outdir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path, newline="") as fh:
+            rows.extend(csv.DictReader(fh))
+    return rows
+
+
+def _pmc_pass(counter: str, child_args, kernel: str, timeout: int = 180):
+    """One rocprofv3 PMC pass over a child bench process: the counter's value
+    (summed over the counter's instances) for each dispatch of kernels whose
+    name contains ``kernel``, in dispatch order.  None if rocprofv3 is absent
+    or the pass fails."""
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3")
+    if prof is None:
         return None
-    for line in open(path):
-        if line.startswith("fp64_flop_per_comp_step:"):
-            return float(line.split(":", 1)[1])
-    return None
+    with tempfile.TemporaryDirectory(prefix="ngpu_pmc_") as d:
+        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+               sys.executable, os.path.abspath(__file__)] + list(child_args)
+        try:
+            r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
+        except subprocess.TimeoutExpired:
+            return None
+        rows = _counter_rows(d)
+        if r.returncode != 0 and not rows:
+            return None
+        per = {}
+        for row in rows:
+            keys = {k.lower(): k for k in row}
+            name = row.get(keys.get("kernel_name", ""), "")
+            if kernel not in name or row.get(keys.get("counter_name", ""), "") != counter:
+                continue
+            did = int(row.get(keys.get("dispatch_id", ""), 0) or 0)
+            per[did] = per.get(did, 0.0) + float(row[keys["counter_value"]])
+        return [per[k] for k in sorted(per)] or None
+
+
+def measured_traffic(workload: str, nsets, nt, window: int):
+    """HBM bytes of one window launch of the simulation kernel, from two
+    rocprofv3 PMC passes (FETCH_SIZE, WRITE_SIZE; KiB per dispatch) over a
+    child that runs the first two windows, read on the second window's
+    dispatch; each counter is scaled by what it reads for a 1 GiB
+    device-to-device copy (bytes copied / counter bytes)."""
+    child = ["--pmc-child", "--workload", workload, "--window", str(window)]
+    if nsets is not None:
+        child += ["--nsets", str(nsets)]
+    if nt is not None:
+        child += ["--nt", str(nt)]
+    copy = ["--copy-probe", "1024"]
+    out = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        sim = _pmc_pass(ctr, child, SIM_KERNEL)
+        cal = _pmc_pass(ctr, copy, COPY_KERNEL)
+        if not sim or not cal:
+            return None
+        copied = 1024.0 * 1024.0                      # KiB moved each way by one copy dispatch
+        factor = copied / (sum(cal) / len(cal))
+        out[ctr] = {"kib_per_dispatch": sim[-1], "calibration_factor": round(factor, 4),
+                    "bytes": sim[-1] * 1024.0 * factor}
+    out["bytes_per_launch"] = out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]
+    return out
 
 
 def copy_peak_gbs(mib: int = 2048, reps: int = 10) -> float:
@@ -121,7 +173,7 @@ def bench_mixed(args):
     launch shape of each morphology comes from the autotuner (untimed)."""
     from neurogpu_amd import autotune, models
     from neurogpu_amd._lib import Batch, workload_simulator
-    ws = models.mixed_workloads()
+    ws = models.mixed_workloads(mechset=args.mixed_set)
     say = lambda m: print(m, flush=True)
     choices = {}
     for w in ws:
@@ -161,8 +213,8 @@ def bench_mixed(args):
         "metric": METRIC, "value": total * args.steps / wall, "unit": "comp-steps/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": "mixed4: TTPC mechanisms on 200/700/1500/3000-segment cells, 12000/4000/1000/256 "
-                               "instances, 10000/5000/2000/40000 steps, one persistent launch", "items": st["nitems"],
+        "config": {"workload": f"mixed4: {args.mixed_set} mechanisms on 200/700/1500/3000-segment cells, "
+                               "12000/4000/1000/256 instances, 10000/5000/2000/40000 steps, one persistent launch", "items": st["nitems"],
                    "workgroups": st["workgroups"], "lds_bytes": st["lds_bytes"]},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
@@ -185,23 +237,69 @@ def bench_mixed(args):
     print(json.dumps(result), flush=True)
 
 
+def cpu_threads() -> int:
+    """Host threads of this process's CPU share: the affinity mask, bounded by
+    OMP_NUM_THREADS where the environment sets it."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(env))) if env.isdigit() and int(env) > 0 else max(1, n)
+
+
+def windows(nt: int, k: int):
+    """k consecutive step windows [b_i, b_i+1) that cover 0..nt once."""
+    b = np.linspace(0, nt, k + 1).round().astype(int)
+    return [(int(b[i]), int(b[i + 1])) for i in range(k) if b[i + 1] > b[i]]
+
+
+def configured_simulator(w, args):
+    from neurogpu_amd._lib import Simulator
+    m = w.model
+    sim = Simulator(0)
+    sim.set_tree(m.tree, m.mechset_id)
+    sim.set_sim(m.v_init, m.rec_sites)
+    sim.set_params(w.params, m.ncomps, m.nparams)
+    sim.set_stimuli(w.amps, w.dts, m.stim_seg, m.stim_area)
+    sim.set_launch(args.waves_per_run, args.runs_per_group, 0, False)
+    return sim
+
+
+def pmc_child(args):
+    """The profiled child of a PMC pass: the first two windows of the workload."""
+    from neurogpu_amd import models
+    w = models.workload(args.workload, nsets=args.nsets, nt=args.nt)
+    sim = configured_simulator(w, args)
+    n = min(args.window, w.dts.size)
+    sim.run_steps(0, n)
+    if 2 * n <= w.dts.size:
+        sim.run_steps(n, 2 * n)
+    sim.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed bench steps: windows of nt/steps dt steps that cover the protocol once")
+    ap.add_argument("--warmup", type=int, default=5, help="untimed windows before the timed protocol")
     ap.add_argument("--workload", default="ttpc1200", choices=("passive64", "active500", "ttpc1200", "mixed4"))
     ap.add_argument("--separate", action="store_true",
                     help="mixed4: also time each morphology in its own launch, one after another")
+    ap.add_argument("--mixed-set", default="mainen", choices=("mainen", "ttpc"),
+                    help="mixed4: the channel set of the four cells (mainen = the active cell's)")
     ap.add_argument("--nsets", type=int, default=None)
     ap.add_argument("--nt", type=int, default=None)
     ap.add_argument("--waves-per-run", type=int, default=0)
     ap.add_argument("--runs-per-group", type=int, default=0)
-    ap.add_argument("--graph", type=int, default=1)
     ap.add_argument("--cpu-sets", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-copy-peak", action="store_true")
+    ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 HBM traffic passes")
     ap.add_argument("--copy-peak", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--window", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run a device-to-device copy of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -218,84 +316,100 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8}))
         return
+    if args.pmc_child:
+        pmc_child(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (runs are independent, "
               f"so the throughput of N GPUs is N times this figure).", file=sys.stderr)
         sys.exit(2)
+    if args.steps < 1 or args.warmup < 0:
+        print("--steps must be positive and --warmup non-negative", file=sys.stderr)
+        sys.exit(2)
 
     from neurogpu_amd import models
-    from neurogpu_amd._lib import Simulator
     if args.workload == "mixed4":
         return bench_mixed(args)
 
     w = models.workload(args.workload, nsets=args.nsets, nt=args.nt)
     m = w.model
-    print(f"bench: {w.name} N={m.tree.N} runs={w.nruns} nt={w.dts.size} S={m.state_count():.2f}", flush=True)
-    sim = Simulator(0)
-    sim.set_tree(m.tree, m.mechset_id)
-    sim.set_sim(m.v_init, m.rec_sites)
-    sim.set_params(w.params, m.ncomps, m.nparams)
-    sim.set_stimuli(w.amps, w.dts, m.stim_seg, m.stim_area)
-    sim.set_launch(args.waves_per_run, args.runs_per_group, 0, bool(args.graph))
+    nt = int(w.dts.size)
+    wins = windows(nt, args.steps)
+    wlen = max(b - a for a, b in wins)
+    say = lambda msg: print(msg, flush=True)
+    say(f"bench: {w.name} N={m.tree.N} runs={w.nruns} nt={nt} S={m.state_count():.2f}; "
+        f"{len(wins)} windows of {wlen} steps")
+    # measurements in child processes first (rocprofv3 passes, torch copy peak)
+    traffic = None
+    if not args.no_pmc:
+        say("rocprofv3 FETCH_SIZE / WRITE_SIZE passes (child processes)")
+        traffic = measured_traffic(args.workload, args.nsets, args.nt, wlen)
+        say(f"traffic per window launch: {traffic['bytes_per_launch']:.4e} B" if traffic else "traffic: unavailable")
+    copy_peak = None if args.no_copy_peak else copy_peak_gbs()
+
+    sim = configured_simulator(w, args)
     for i in range(args.warmup):
-        _, st = sim.run(fetch=False)
-        print(f"warmup {i}: kernel {st['ms_kernel']:.1f} ms", flush=True)
-    kernel_ms = []
+        a, b = wins[i % len(wins)]
+        st = sim.run_steps(a, b)
+        say(f"warmup {i}: steps {a}..{b} kernel {st['ms_kernel']:.1f} ms")
+    kernel_ms, launch_cs = [], []
     t0 = time.perf_counter()
-    for i in range(args.steps):
-        _, st = sim.run(fetch=False)
+    for i, (a, b) in enumerate(wins):
+        st = sim.run_steps(a, b)
         kernel_ms.append(st["ms_kernel"])
-        print(f"step {i}: kernel {st['ms_kernel']:.1f} ms", flush=True)
+        launch_cs.append(st["comp_steps"])
+        say(f"step {i}: steps {a}..{b} kernel {st['ms_kernel']:.1f} ms")
     wall = time.perf_counter() - t0
-    gpu = sim.fetch().reshape(w.x.shape[0], w.amps.shape[0], len(m.rec_sites), w.dts.size)
+    gpu = sim.fetch().reshape(w.x.shape[0], w.amps.shape[0], len(m.rec_sites), nt)
     sim.close()
 
     comp_steps = w.comp_steps
-    value = comp_steps * args.steps / wall
-    launch_ms = float(np.mean(kernel_ms)) / max(1, st["launches"])
+    value = comp_steps / wall
+    launch_ms = float(np.mean(kernel_ms))
+    cs_launch = float(np.mean(launch_cs))
     bcs = algorithmic_bytes_per_comp_step(m)
-    alg_bytes_per_launch = comp_steps / max(1, st["launches"]) * bcs
-    achieved = alg_bytes_per_launch / (launch_ms * 1e-3) / 1e9
-    traffic = traffic_from_profile(args.workload)
+    achieved = cs_launch * bcs / (launch_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "bytes_per_comp_step": round(bcs, 3), "kernel_ms_per_launch": round(launch_ms, 3),
-                "comp_steps_per_launch": comp_steps / max(1, st["launches"])}
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic["bytes_per_launch"] if traffic else None,
+                "kernel": f"{SIM_KERNEL}<{m.mechset}>", "bytes_per_comp_step": round(bcs, 3),
+                "kernel_ms_per_launch": round(launch_ms, 3), "comp_steps_per_launch": cs_launch,
+                "launches": len(wins), "steps_per_launch": wlen}
     if traffic:
-        roofline["actual_GBs"] = round(traffic / (launch_ms * 1e-3) / 1e9, 1)
-    if not args.no_copy_peak:
-        roofline["copy_peak_GBs"] = round(copy_peak_gbs(), 1)
-    flop_cs = fp64_from_profile(args.workload)
-    if flop_cs:
-        tf = flop_cs * comp_steps / max(1, st["launches"]) / (launch_ms * 1e-3) / 1e12
-        roofline["fp64"] = {"issued_flop_per_comp_step": flop_cs, "achieved": round(tf, 2),
-                            "peak": FP64_PEAK_TFS, "unit": "TFLOP/s", "frac": round(tf / FP64_PEAK_TFS, 4)}
+        roofline["traffic_over_algorithmic"] = round(traffic["bytes_per_launch"] / (cs_launch * bcs), 3)
+        roofline["actual_GBs"] = round(traffic["bytes_per_launch"] / (launch_ms * 1e-3) / 1e9, 1)
+        roofline["pmc"] = {k: traffic[k] for k in ("FETCH_SIZE", "WRITE_SIZE")}
+    if copy_peak:
+        roofline["copy_peak_GBs"] = round(copy_peak, 1)
 
     result = {
-        "metric": METRIC, "value": value, "unit": "comp-steps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True,
+        "metric": METRIC, "value": value, "unit": "comp-steps/s", "n_gpus": 1, "steps": len(wins),
+        "warmup": args.warmup, "ms_per_step": wall / len(wins) * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"{w.name}: {m.tree.N} compartments, {m.mechset} mechanisms, "
                                f"{w.x.shape[0]} sets x {w.amps.shape[0]} stimuli, dt {w.dts[0]} ms, "
-                               f"{w.dts.size} steps", "runs": w.nruns, "compartments": m.tree.N,
-                   "nt": int(w.dts.size), "launch": {k: st[k] for k in ("waves_per_run", "runs_per_group", "slots_per_lane",
-                                                                      "runs_per_block", "launches")}},
+                               f"{nt} steps; a bench step advances every run by one window of about "
+                               f"{wlen} dt steps, the timed steps cover the protocol once",
+                   "runs": w.nruns, "compartments": m.tree.N, "nt": nt,
+                   "launch": {k: st[k] for k in ("waves_per_run", "runs_per_group", "slots_per_lane",
+                                                  "runs_per_block")}},
         "roofline": roofline,
     }
     if not args.no_cpu:
-        threads = max(1, min(16, os.cpu_count() or 1))
-        cpu_sets = args.cpu_sets if args.cpu_sets is not None else (16 if args.workload == "ttpc1200" else 8)
+        threads = cpu_threads()
+        cpu_sets = args.cpu_sets if args.cpu_sets is not None else (32 if args.workload == "ttpc1200" else 16)
         nsets = min(cpu_sets, w.x.shape[0])
+        say(f"cpu baseline: {nsets} sets on {threads} threads")
         ref, cps, el = cpu_baseline(w, nsets, threads)
         d = float(np.max(np.abs(gpu[:nsets] - ref)))
         same_spikes = bool(np.array_equal(spike_counts(gpu[:nsets]), spike_counts(ref)))
-        one_sets = max(1, nsets // threads)
-        _, cps1, el1 = cpu_baseline(w, one_sets, 1)
+        _, cps1, el1 = cpu_baseline(w, 1, 1)
         result["cpu_baseline"] = {"value": cps, "unit": "comp-steps/s", "cores": threads, "kind": "port",
-                                  "sample": f"{nsets} sets x {w.amps.shape[0]} stimuli x {w.dts.size} steps "
+                                  "sample": f"{nsets} sets x {w.amps.shape[0]} stimuli x {nt} steps "
                                             f"of the same workload ({el:.1f} s, OpenMP over runs)",
                                   "cpu": cpu_model(), "nproc": os.cpu_count(),
-                                  "one_thread": {"value": cps1, "sample": f"{one_sets} sets ({el1:.1f} s)"}}
+                                  "one_thread": {"value": cps1, "sample": f"1 set x {w.amps.shape[0]} stimuli "
+                                                                          f"({el1:.1f} s)"}}
         result["parity_sample_max_abs_mV"] = d
         result["parity_sample_spike_counts_equal"] = same_spikes
         if d > 1e-6 or not same_spikes:

@@ -95,10 +95,28 @@ int ngpu_set_launch(ngpu_ctx *ctx, int waves_per_run, int runs_per_group,
  * device).  Replaces callKernel/stEfork2Main (CANON/CudaStuff.cu:443-587). */
 int ngpu_run(ngpu_ctx *ctx, double *vhot, ngpu_stats *stats);
 
+/* Steps [t_begin, t_end) of every (set, stimulus): t_begin = 0 starts from
+ * the initial state, any other t_begin continues from the state the previous
+ * call left there (NGPU_E_STATE otherwise; changing the tree, record sites,
+ * parameters, stimuli or launch shape discards that state).  The traces of
+ * the window's steps are written to the context's trace buffer, so after the
+ * window that ends at Nt ngpu_fetch_vhot / ngpu_features see the whole run.
+ * Replaces the time loop of the reference kernel, which runs all Nt steps in
+ * one launch (CANON/CudaStuff.cu:228-338); windows let a caller interleave
+ * other work or bound the length of one launch. */
+int ngpu_run_steps(ngpu_ctx *ctx, int t_begin, int t_end, ngpu_stats *stats);
+
+/* Sizes the context holds (0 where not yet set): parameter sets, stimuli,
+ * record sites, steps, segments -- e.g. after ngpu_open, to size the buffers
+ * of ngpu_run / ngpu_features.  The reference reads them from the files'
+ * headers (CANON/MainC.cu:335-551, CANON/Util.cu:337-432). */
+int ngpu_get_sizes(const ngpu_ctx *ctx, int *nsets, int *nstim, int *nrec, int *nt, int *N);
+
 /* Spike features of every trace of the last run (record site 0), computed on
  * the device: out [nsets][nstim][NGPU_NFEAT] = spike count, first-spike
  * latency (ms after t_on[stim]), mean AP amplitude above the pre-stimulus
  * baseline (mV), mean AP width at thr (ms), mean ISI (ms); NaN where undefined.
+ * Only spikes whose threshold crossing lies in [t_on[stim], t_off[stim]) count.
  * Replaces the per-individual eFEL pass of the GA evaluator (PYS/efel_ext.py:23-97,
  * threshold at :15). */
 #define NGPU_NFEAT 5

@@ -26,7 +26,8 @@ EXPORTS = ("ngpu_create", "ngpu_set_tree", "ngpu_set_sim", "ngpu_set_params", "n
            "ngpu_mlp_create", "ngpu_mlp_nparams", "ngpu_mlp_set_weights", "ngpu_mlp_get_weights", "ngpu_mlp_train",
            "ngpu_mlp_predict", "ngpu_mlp_gradients", "ngpu_mlp_last_error", "ngpu_mlp_close", "ngpu_gemm_bf16",
            "ngpu_batch_create", "ngpu_batch_add", "ngpu_batch_set_launch", "ngpu_batch_run",
-           "ngpu_batch_member_info", "ngpu_batch_last_error", "ngpu_batch_close")
+           "ngpu_batch_member_info", "ngpu_batch_last_error", "ngpu_batch_close", "ngpu_run_steps",
+           "ngpu_get_sizes")
 NFEAT = 5
 FEATURES = ("spike_count", "first_latency", "ap_amplitude", "ap_width", "mean_isi")
 
@@ -105,6 +106,8 @@ def lib():
         L.ngpu_set_stimuli.argtypes = [vp, _DP, C.c_int, C.c_int, _DP, C.c_int, C.c_int, C.c_double]
         L.ngpu_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.ngpu_run.argtypes = [vp, _DP, C.POINTER(ngpu_stats)]
+        L.ngpu_run_steps.argtypes = [vp, C.c_int, C.c_int, C.POINTER(ngpu_stats)]
+        L.ngpu_get_sizes.argtypes = [vp] + [_IP] * 5
         L.ngpu_fetch_vhot.argtypes = [vp, _DP]
         L.ngpu_open.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.POINTER(vp)]
         L.ngpu_write_vhot.argtypes = [vp, C.c_char_p]
@@ -131,7 +134,7 @@ def lib():
             getattr(L, name).restype = C.c_int
         for name in ("ngpu_create", "ngpu_set_tree", "ngpu_set_sim", "ngpu_set_params", "ngpu_set_stimuli",
                      "ngpu_set_launch", "ngpu_run", "ngpu_fetch_vhot", "ngpu_open", "ngpu_write_vhot",
-                     "ngpu_mechset_nparams"):
+                     "ngpu_mechset_nparams", "ngpu_run_steps", "ngpu_get_sizes"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -170,7 +173,15 @@ class Simulator:
         self._h = C.c_void_p(_handle)
         if _handle is None:
             self._check(self._L.ngpu_create(int(device), C.byref(self._h)), None)
-        self.nsets = self.nstim = self.nrec = self.nt = 0
+        self.nsets = self.nstim = self.nrec = self.nt = self.N = 0
+        if _handle is not None:
+            self._sync_sizes()
+
+    def _sync_sizes(self) -> None:
+        """Read the sizes the context holds (after ngpu_open the files set them)."""
+        v = [C.c_int() for _ in range(5)]
+        self._check(self._L.ngpu_get_sizes(self._h, *[C.byref(x) for x in v]))
+        self.nsets, self.nstim, self.nrec, self.nt, self.N = (x.value for x in v)
 
     def _check(self, rc: int, h=...):
         if rc != 0:
@@ -229,6 +240,14 @@ class Simulator:
             self._check(self._L.ngpu_run(self._h, None, C.byref(st)))
         return out, st.as_dict()
 
+    def run_steps(self, t_begin: int, t_end: int) -> dict:
+        """Advance every run through steps [t_begin, t_end) (ngpu_run_steps);
+        t_begin = 0 starts from the initial state, otherwise the state the
+        previous window left at t_begin is continued.  Returns the stats."""
+        st = ngpu_stats()
+        self._check(self._L.ngpu_run_steps(self._h, int(t_begin), int(t_end), C.byref(st)))
+        return st.as_dict()
+
     def fetch(self) -> np.ndarray:
         out = np.empty(self.vhot_size(), dtype=np.float64)
         self._check(self._L.ngpu_fetch_vhot(self._h, out.ctypes.data_as(_DP)))
@@ -236,7 +255,8 @@ class Simulator:
 
     def features(self, t_on, t_off, thr: float = -30.0) -> np.ndarray:
         """Spike features of the last run's traces, computed on the device:
-        [nsets, nstim, 5] (see FEATURES)."""
+        [nsets, nstim, 5] (see FEATURES); spikes count in [t_on, t_off)."""
+        self._sync_sizes()
         on = np.ascontiguousarray(np.broadcast_to(np.asarray(t_on, np.float64), (self.nstim,)))
         off = np.ascontiguousarray(np.broadcast_to(np.asarray(t_off, np.float64), (self.nstim,)))
         out = np.empty((self.nsets, self.nstim, NFEAT), dtype=np.float64)

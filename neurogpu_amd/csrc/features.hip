@@ -5,59 +5,100 @@
  * Stands in for the per-individual eFEL pass of the GA evaluators
  * (PYS/efel_ext.py:23-97, threshold -30 mV at :15).  The features are this
  * build's own definitions (eFEL is not available to pin them), per trace of
- * record site 0 with t_i the start time of step i:
- *   0 spike count      upward crossings of thr (v[i-1] < thr <= v[i])
- *   1 first latency    first crossing time (linear interpolation) - t_on; NaN if none
+ * record site 0 with t_i the start time of step i.  A spike is an upward
+ * crossing of thr (v[i-1] < thr <= v[i]) whose linearly interpolated time tc
+ * lies in the stimulus window t_on <= tc < t_off; crossings outside the
+ * window are ignored.
+ *   0 spike count      spikes in the window
+ *   1 first latency    first spike's tc - t_on; NaN if none
  *   2 AP amplitude     mean over spikes of (peak - v_base), v_base = mean v for t < t_on
  *   3 AP width         mean time above thr of spikes that fall back below it; NaN if none
- *   4 mean ISI         mean interval between consecutive crossings; NaN if < 2 spikes
+ *   4 mean ISI         mean interval between consecutive spikes; NaN if < 2 spikes
+ *
+ * Layout: one 64-lane workgroup takes 64 runs.  The traces are read in tiles
+ * of 64 runs x 64 steps: for each run of the tile the 64 lanes read 64
+ * consecutive samples (one 512-byte coalesced load), the tile is written to
+ * LDS, and each lane then scans its own run's 64 samples from LDS.
  */
 #include <hip/hip_runtime.h>
 
 #define NG_NFEAT 5
+#define NG_FT 64
 
-__global__ void ngpu_features_kernel(const double *__restrict__ vhot, int nrec, int nt, int nruns, int nstim,
-                                     const double *__restrict__ times, const double *__restrict__ t_on,
-                                     const double *__restrict__ t_off, double thr, double *__restrict__ out)
+__global__ void __launch_bounds__(64) ngpu_features_kernel(const double *__restrict__ vhot, int nrec, int nt,
+                                                           int nruns, int nstim, const double *__restrict__ times,
+                                                           const double *__restrict__ t_on,
+                                                           const double *__restrict__ t_off, double thr,
+                                                           double *__restrict__ out)
 {
-    const int run = blockIdx.x * blockDim.x + threadIdx.x;
-    if (run >= nruns) return;
-    const double *v = vhot + (size_t)run * nrec * nt;
-    const int stim = run % nstim;
-    const double on = t_on[stim];
-    (void)t_off;
-    double vb = 0.0;
-    int nb = 0;
-    for (int i = 0; i < nt && times[i] < on; i++) { vb += v[i]; nb++; }
-    vb = nb > 0 ? vb / nb : v[0];
+    __shared__ double tile[64][NG_FT + 1];
+    const int lane = threadIdx.x;
+    const int run0 = blockIdx.x * 64;
+    const int run = run0 + lane;
+    const bool live = run < nruns;
+    const int stim = (live ? run : run0) % nstim;
+    const double on = t_on[stim], off = t_off[stim];
     const double nan = __builtin_nan("");
+    /* baseline: mean of the samples before the first step with t >= on */
+    double vsum = 0.0, vb = 0.0;
+    int nb = 0;
+    bool base_open = true;
     int cnt = 0, namp = 0, nw = 0;
     double first = nan, last_up = 0.0, sum_isi = 0.0, sum_amp = 0.0, sum_w = 0.0, t_up = 0.0, peak = 0.0;
+    double prev = 0.0, v0 = 0.0;
     bool in_spike = false;
-    for (int i = 1; i < nt; i++) {
-        const double a = v[i - 1], b = v[i];
-        const double ta = times[i - 1], tb = times[i];
-        if (a < thr && b >= thr) {
-            const double tc = ta + (thr - a) / (b - a) * (tb - ta);
-            cnt++;
-            if (cnt == 1) first = tc - on;
-            else sum_isi += tc - last_up;
-            last_up = tc;
-            t_up = tc;
-            peak = b;
-            in_spike = true;
-        } else if (in_spike) {
-            if (b > peak) peak = b;
-            if (a >= thr && b < thr) {
-                const double td = ta + (a - thr) / (a - b) * (tb - ta);
-                sum_w += td - t_up;
-                nw++;
-                sum_amp += peak - vb;
-                namp++;
-                in_spike = false;
+    for (int c0 = 0; c0 < nt; c0 += NG_FT) {
+        const int n = min(NG_FT, nt - c0);
+        __syncthreads();
+        for (int j = 0; j < 64; j++) {
+            const int r = run0 + j;
+            if (r < nruns && lane < n) tile[j][lane] = vhot[(size_t)r * nrec * nt + c0 + lane];
+        }
+        __syncthreads();
+        if (!live) continue;
+        for (int k = 0; k < n; k++) {
+            const int i = c0 + k;
+            const double b = tile[lane][k];
+            const double tb = times[i];
+            if (i == 0) v0 = b;
+            if (base_open) {
+                if (tb < on) {
+                    vsum += b;
+                    nb++;
+                } else {
+                    base_open = false;
+                }
+                if (!base_open) vb = nb > 0 ? vsum / nb : v0;
             }
+            if (i > 0) {
+                const double a = prev, ta = times[i - 1];
+                if (a < thr && b >= thr) {
+                    const double tc = ta + (thr - a) / (b - a) * (tb - ta);
+                    if (tc >= on && tc < off) {
+                        cnt++;
+                        if (cnt == 1) first = tc - on;
+                        else sum_isi += tc - last_up;
+                        last_up = tc;
+                        t_up = tc;
+                        peak = b;
+                        in_spike = true;
+                    }
+                } else if (in_spike) {
+                    if (b > peak) peak = b;
+                    if (a >= thr && b < thr) {
+                        const double td = ta + (a - thr) / (a - b) * (tb - ta);
+                        sum_w += td - t_up;
+                        nw++;
+                        sum_amp += peak - vb;
+                        namp++;
+                        in_spike = false;
+                    }
+                }
+            }
+            prev = b;
         }
     }
+    if (!live) return;
     if (in_spike) { sum_amp += peak - vb; namp++; }
     double *o = out + (size_t)run * NG_NFEAT;
     o[0] = (double)cnt;
@@ -70,8 +111,7 @@ __global__ void ngpu_features_kernel(const double *__restrict__ vhot, int nrec, 
 hipError_t ngpu_launch_features(const double *vhot, int nrec, int nt, int nruns, int nstim, const double *times,
                                 const double *t_on, const double *t_off, double thr, double *out, hipStream_t s)
 {
-    const int bs = 64;
-    hipLaunchKernelGGL(ngpu_features_kernel, dim3((nruns + bs - 1) / bs), dim3(bs), 0, s, vhot, nrec, nt, nruns,
-                       nstim, times, t_on, t_off, thr, out);
+    hipLaunchKernelGGL(ngpu_features_kernel, dim3((nruns + 63) / 64), dim3(64), 0, s, vhot, nrec, nt, nruns, nstim,
+                       times, t_on, t_off, thr, out);
     return hipGetLastError();
 }

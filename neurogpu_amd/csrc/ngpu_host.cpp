@@ -19,6 +19,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <new>
 #include <sstream>
 #include <string>
 #include <vector>
@@ -106,6 +107,7 @@ struct ngpu_ctx {
     std::vector<double> pconst;
     hipGraphExec_t gexec = nullptr;
     bool ran = false;
+    int t_done = 0;               /* steps [0, t_done) simulated; the next window starts there */
     int last_launches = 0;
     int last_G = 1;               /* groups per workgroup of the last launch */
     int last_stage = 1;
@@ -122,6 +124,13 @@ static int fail(ngpu_ctx *c, int code, const char *fmt, ...)
     g_last_error = buf;
     return code;
 }
+
+/* Host-side C++ exceptions (allocation failures, malformed input caught by
+ * the standard library) must not cross the C ABI: the entry points that
+ * allocate are function-try-blocks ending in NGPU_CATCH. */
+#define NGPU_CATCH(ctx, code)                                                                     \
+    catch (const std::bad_alloc &) { return fail((ctx), NGPU_E_NOMEM, "host allocation failed"); } \
+    catch (const std::exception &ex_) { return fail((ctx), (code), "%s", ex_.what()); }
 
 #define HIPCHK(c, call)                                                                           \
     do {                                                                                          \
@@ -155,29 +164,43 @@ static int alloc(ngpu_ctx *c, DevArr<T> &d, size_t n)
     return 0;
 }
 
+/* Drops the captured launch sequence; every change that calls this (tree,
+ * record sites, parameters, stimuli, launch shape) also invalidates the state
+ * a windowed run carries between calls. */
 static void drop_graph(ngpu_ctx *c)
 {
     if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
     c->gexec = nullptr;
     c->graph_dirty = true;
+    c->t_done = 0;
 }
 
 extern "C" int ngpu_mechset_nparams(int ms) { return mech_nparams(ms) < 0 ? NGPU_E_ARG : mech_nparams(ms); }
+
+/* Device part of a context: device, stream and events. */
+static int init_device(ngpu_ctx *c, int device)
+{
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev == 0) return fail(c, NGPU_E_HIP, "no HIP device available (%s)", hipGetErrorString(e));
+    if (device < 0 || device >= ndev) return fail(c, NGPU_E_ARG, "device %d out of range (0..%d)", device, ndev - 1);
+    c->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess)
+        return fail(c, NGPU_E_HIP, "could not create stream/events on device %d", device);
+    return NGPU_OK;
+}
 
 extern "C" int ngpu_create(int device, ngpu_ctx **out)
 {
     if (!out) return fail(nullptr, NGPU_E_ARG, "ngpu_create: out is NULL");
     *out = nullptr;
-    int ndev = 0;
-    hipError_t e = hipGetDeviceCount(&ndev);
-    if (e != hipSuccess || ndev == 0) return fail(nullptr, NGPU_E_HIP, "no HIP device available (%s)", hipGetErrorString(e));
-    if (device < 0 || device >= ndev) return fail(nullptr, NGPU_E_ARG, "device %d out of range (0..%d)", device, ndev - 1);
     ngpu_ctx *c = new ngpu_ctx();
-    c->device = device;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
-        delete c;
-        return fail(nullptr, NGPU_E_HIP, "could not create stream/events on device %d", device);
+    int rc = init_device(c, device);
+    if (rc) {
+        g_last_error = c->err;
+        ngpu_close(c);
+        return rc;
     }
     *out = c;
     return NGPU_OK;
@@ -186,8 +209,10 @@ extern "C" int ngpu_create(int device, ngpu_ctx **out)
 extern "C" void ngpu_close(ngpu_ctx *c)
 {
     if (!c) return;
-    (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->stream) {
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+    }
     drop_graph(c);
     for (auto *d : {&c->d_qlink, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_rec, &c->d_msched}) d->release();
     for (auto *d : {&c->d_qcm, &c->d_ef, &c->d_pval, &c->d_pconst, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_vhot, &c->d_times, &c->d_fwin, &c->d_feat}) d->release();
@@ -202,7 +227,7 @@ extern "C" const char *ngpu_last_error(const ngpu_ctx *c) { return c ? c->err.c_
 /* ------------------------------------------------------------------ tree */
 
 extern "C" int ngpu_set_tree(ngpu_ctx *c, const ngpu_tree *t)
-{
+try {
     if (!c || !t) return fail(c, NGPU_E_ARG, "ngpu_set_tree: NULL argument");
     const int N = t->N;
     if (N < 1 || t->ncomps < 1) return fail(c, NGPU_E_ARG, "N=%d and ncomps=%d must be positive", N, t->ncomps);
@@ -359,9 +384,10 @@ extern "C" int ngpu_set_tree(ngpu_ctx *c, const ngpu_tree *t)
     drop_graph(c);
     return NGPU_OK;
 }
+NGPU_CATCH(c, NGPU_E_ARG)
 
 extern "C" int ngpu_set_sim(ngpu_ctx *c, double v_init, int nrec, const int *rec_sites)
-{
+try {
     if (!c) return fail(c, NGPU_E_ARG, "NULL context");
     if (!c->have_tree) return fail(c, NGPU_E_STATE, "ngpu_set_sim before ngpu_set_tree");
     if (nrec < 0 || (nrec > 0 && !rec_sites) || nrec > 64) return fail(c, NGPU_E_ARG, "nrec=%d invalid (0..64)", nrec);
@@ -375,9 +401,10 @@ extern "C" int ngpu_set_sim(ngpu_ctx *c, double v_init, int nrec, const int *rec
     drop_graph(c);
     return NGPU_OK;
 }
+NGPU_CATCH(c, NGPU_E_ARG)
 
 extern "C" int ngpu_set_params(ngpu_ctx *c, const double *P, int nsets, int ncomps, int nparams)
-{
+try {
     if (!c) return fail(c, NGPU_E_ARG, "NULL context");
     if (!c->have_tree) return fail(c, NGPU_E_STATE, "ngpu_set_params before ngpu_set_tree");
     if (nsets < 1 || !P) return fail(c, NGPU_E_ARG, "nsets=%d must be positive", nsets);
@@ -427,9 +454,10 @@ extern "C" int ngpu_set_params(ngpu_ctx *c, const double *P, int nsets, int ncom
     drop_graph(c);
     return NGPU_OK;
 }
+NGPU_CATCH(c, NGPU_E_ARG)
 
 extern "C" int ngpu_set_stimuli(ngpu_ctx *c, const double *amps, int nstim, int nt_amps, const double *dts, int nt, int stim_seg, double area)
-{
+try {
     if (!c) return fail(c, NGPU_E_ARG, "NULL context");
     if (!c->have_tree) return fail(c, NGPU_E_STATE, "ngpu_set_stimuli before ngpu_set_tree");
     if (nstim < 1 || nt < 1 || nt_amps < nt || !amps || !dts) return fail(c, NGPU_E_ARG, "invalid stimulus sizes nstim=%d nt=%d nt_amps=%d", nstim, nt, nt_amps);
@@ -446,6 +474,7 @@ extern "C" int ngpu_set_stimuli(ngpu_ctx *c, const double *amps, int nstim, int 
     drop_graph(c);
     return NGPU_OK;
 }
+NGPU_CATCH(c, NGPU_E_ARG)
 
 extern "C" int ngpu_set_launch(ngpu_ctx *c, int w, int r, int spl, int graph)
 {
@@ -626,7 +655,7 @@ static void fill_args(const ngpu_ctx *c, SimArgs &a, int W, int R, int staged, i
     a.t0 = 0; a.t1 = c->nt;
 }
 
-static int enqueue(ngpu_ctx *c, int &launches)
+static int enqueue(ngpu_ctx *c, int t_begin, int t_end, int &launches)
 {
     const int W = c->lay_W, nruns = c->nsets * c->nstim;
     int R = c->lay_R, G = 1, stage = c->cfg_stage;
@@ -640,9 +669,9 @@ static int enqueue(ngpu_ctx *c, int &launches)
     dim3 block(64 * W * G), grid((nruns + G * R - 1) / (G * R));
     int spl = c->cfg_spl > 0 ? c->cfg_spl : c->nt;
     launches = 0;
-    for (int t0 = 0; t0 < c->nt; t0 += spl) {
+    for (int t0 = t_begin; t0 < t_end; t0 += spl) {
         a.t0 = t0;
-        a.t1 = std::min(c->nt, t0 + spl);
+        a.t1 = std::min(t_end, t0 + spl);
         hipError_t e = launch_one(c, a, grid, block, lds);
         if (e != hipSuccess) return fail(c, NGPU_E_HIP, "kernel launch failed: %s", hipGetErrorString(e));
         launches++;
@@ -750,21 +779,29 @@ static int prepare_layout(ngpu_ctx *c, int W, int R)
     return NGPU_OK;
 }
 
-extern "C" int ngpu_run(ngpu_ctx *c, double *vhot, ngpu_stats *st)
+/* Steps [t_begin, t_end) of every run (t_begin 0: from the initial state;
+ * otherwise from the state the previous call left at t_begin).  Whole runs
+ * may replay a captured hipGraph; windows are launched directly. */
+static int run_range(ngpu_ctx *c, int t_begin, int t_end, double *vhot, ngpu_stats *st)
 {
-    if (!c) return fail(c, NGPU_E_ARG, "NULL context");
     hipEvent_t h0, h1;
     int rc = prepare(c);
     if (rc) return rc;
+    if (t_begin < 0 || t_end > c->nt || t_begin >= t_end)
+        return fail(c, NGPU_E_ARG, "step window [%d, %d) is not inside [0, %d)", t_begin, t_end, c->nt);
+    if (t_begin != 0 && t_begin != c->t_done)
+        return fail(c, NGPU_E_STATE, "a window starting at step %d needs the state at step %d (the last run ended at %d)",
+                    t_begin, t_begin, c->t_done);
+    const bool whole = t_begin == 0 && t_end == c->nt;
     HIPCHK(c, hipEventCreate(&h0));
     HIPCHK(c, hipEventCreate(&h1));
     HIPCHK(c, hipEventRecord(h0, c->stream));
     int launches = 0;
-    if (c->cfg_graph) {
+    if (c->cfg_graph && whole) {
         if (!c->gexec) {
             hipGraph_t g;
             HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-            rc = enqueue(c, launches);
+            rc = enqueue(c, 0, c->nt, launches);
             hipError_t ce = hipStreamEndCapture(c->stream, &g);
             if (rc) return rc;
             if (ce != hipSuccess) return fail(c, NGPU_E_HIP, "graph capture failed: %s", hipGetErrorString(ce));
@@ -779,12 +816,13 @@ extern "C" int ngpu_run(ngpu_ctx *c, double *vhot, ngpu_stats *st)
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     } else {
         HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-        if ((rc = enqueue(c, launches))) return rc;
+        if ((rc = enqueue(c, t_begin, t_end, launches))) return rc;
         HIPCHK(c, hipEventRecord(c->ev1, c->stream));
     }
     hipError_t se = hipStreamSynchronize(c->stream);
     if (se != hipSuccess) return fail(c, NGPU_E_HIP, "simulation failed: %s", hipGetErrorString(se));
-    c->ran = true;
+    c->t_done = t_end;
+    c->ran = t_end == c->nt;
     if (vhot) {
         HIPCHK(c, hipMemcpy(vhot, c->d_vhot.p, (size_t)ngpu_vhot_size(c) * sizeof(double), hipMemcpyDeviceToHost));
     }
@@ -796,7 +834,7 @@ extern "C" int ngpu_run(ngpu_ctx *c, double *vhot, ngpu_stats *st)
         HIPCHK(c, hipEventElapsedTime(&mt, h0, h1));
         st->ms_kernel = mk;
         st->ms_total = mt;
-        st->comp_steps = (double)c->nsets * c->nstim * (double)c->N * c->nt;
+        st->comp_steps = (double)c->nsets * c->nstim * (double)c->N * (t_end - t_begin);
         st->comp_steps_per_s = mk > 0 ? st->comp_steps / (mk * 1e-3) : 0.0;
         int R = c->lay_R, G = 1, stage = c->cfg_stage;
         (void)plan(c, c->lay_W, R, G, stage);
@@ -810,6 +848,33 @@ extern "C" int ngpu_run(ngpu_ctx *c, double *vhot, ngpu_stats *st)
     }
     (void)hipEventDestroy(h0);
     (void)hipEventDestroy(h1);
+    return NGPU_OK;
+}
+
+extern "C" int ngpu_run(ngpu_ctx *c, double *vhot, ngpu_stats *st)
+try {
+    if (!c) return fail(c, NGPU_E_ARG, "NULL context");
+    if (!c->have_stim) return fail(c, NGPU_E_STATE, "ngpu_run needs ngpu_set_tree, ngpu_set_params and ngpu_set_stimuli first");
+    return run_range(c, 0, c->nt, vhot, st);
+}
+NGPU_CATCH(c, NGPU_E_ARG)
+
+extern "C" int ngpu_run_steps(ngpu_ctx *c, int t_begin, int t_end, ngpu_stats *st)
+try {
+    if (!c) return fail(c, NGPU_E_ARG, "NULL context");
+    if (!c->have_stim) return fail(c, NGPU_E_STATE, "ngpu_run_steps needs ngpu_set_tree, ngpu_set_params and ngpu_set_stimuli first");
+    return run_range(c, t_begin, t_end, nullptr, st);
+}
+NGPU_CATCH(c, NGPU_E_ARG)
+
+extern "C" int ngpu_get_sizes(const ngpu_ctx *c, int *nsets, int *nstim, int *nrec, int *nt, int *N)
+{
+    if (!c || !nsets || !nstim || !nrec || !nt || !N) return fail(nullptr, NGPU_E_ARG, "ngpu_get_sizes: NULL argument");
+    *nsets = c->have_params ? c->nsets : 0;
+    *nstim = c->have_stim ? c->nstim : 0;
+    *nrec = c->have_sim ? (int)c->rec.size() : (c->have_tree ? 1 : 0);
+    *nt = c->have_stim ? c->nt : 0;
+    *N = c->have_tree ? c->N : 0;
     return NGPU_OK;
 }
 
@@ -915,7 +980,7 @@ extern "C" int ngpu_batch_set_launch(ngpu_batch *b, int member, const ngpu_batch
 }
 
 extern "C" int ngpu_batch_run(ngpu_batch *b, ngpu_batch_stats *st)
-{
+try {
     if (!b) return bfail(b, NGPU_E_ARG, "NULL batch");
     if (b->mem.empty()) return bfail(b, NGPU_E_STATE, "empty batch");
     BHIPCHK(b, hipSetDevice(b->device));
@@ -1020,6 +1085,8 @@ extern "C" int ngpu_batch_run(ngpu_batch *b, ngpu_batch_stats *st)
     }
     return NGPU_OK;
 }
+catch (const std::bad_alloc &) { return bfail(b, NGPU_E_NOMEM, "host allocation failed"); }
+catch (const std::exception &ex_) { return bfail(b, NGPU_E_ARG, "%s", ex_.what()); }
 
 extern "C" int ngpu_batch_member_info(const ngpu_batch *b, int member, ngpu_batch_member *out)
 {
@@ -1038,7 +1105,7 @@ extern "C" int ngpu_batch_member_info(const ngpu_batch *b, int member, ngpu_batc
 }
 
 extern "C" int ngpu_features(ngpu_ctx *c, const double *t_on, const double *t_off, double thr, double *out)
-{
+try {
     if (!c || !t_on || !t_off || !out) return fail(c, NGPU_E_ARG, "ngpu_features: NULL argument");
     if (!c->ran) return fail(c, NGPU_E_STATE, "ngpu_features needs a completed ngpu_run");
     if (c->rec.empty()) return fail(c, NGPU_E_STATE, "no record site");
@@ -1058,6 +1125,7 @@ extern "C" int ngpu_features(ngpu_ctx *c, const double *t_on, const double *t_of
     HIPCHK(c, hipStreamSynchronize(c->stream));
     return NGPU_OK;
 }
+NGPU_CATCH(c, NGPU_E_ARG)
 
 extern "C" int ngpu_fetch_vhot(ngpu_ctx *c, double *vhot)
 {
@@ -1126,9 +1194,12 @@ static int parse_tree(ngpu_ctx *c, const std::string &path, ParsedTree &p)
     };
     int N, NC, NM, depth, logn, nf, ncall, cd, cfd, nl, nfl, rc;
     if ((rc = ione(N, "N")) || (rc = ione(NC, "NComps"))) return rc;
-    if (N < 1 || NC < 1) return fail(c, NGPU_E_FILE, "%s: N and NComps must be positive", path.c_str());
+    if (N < 1 || N > 32000 || NC < 1 || NC > N) return fail(c, NGPU_E_FILE, "%s: N=%d and NComps=%d out of range", path.c_str(), N, NC);
     if ((rc = dvec(N, p.e, "e")) || (rc = dvec(N, p.f, "f")) || (rc = ivec(N, p.ks, "Ks")) || (rc = ivec(N, p.s2c, "SegToComp")) ||
-        (rc = dvec(N, p.cm, "Cms")) || (rc = ione(NM, "NModels")) || (rc = ivec((size_t)NM * N, p.bm, "boolModel")) ||
+        (rc = dvec(N, p.cm, "Cms")) || (rc = ione(NM, "NModels")))
+        return rc;
+    if (NM < 1 || NM > 32) return fail(c, NGPU_E_FILE, "%s: NModels=%d out of range", path.c_str(), NM);
+    if ((rc = ivec((size_t)NM * N, p.bm, "boolModel")) ||
         (rc = ivec(N, p.son, "SonNoVec")) || (rc = ione(depth, "Depth")) || (rc = ione(logn, "LognDepth")) ||
         (rc = ione(nf, "nFathers")) || (rc = ione(ncall, "nCallForFather")) || (rc = ivec(nf, p.rs, "RelStarts")) ||
         (rc = ivec(nf, p.re, "RelEnds")) || (rc = ivec(ncall, p.rv, "RelVec")) || (rc = ivec(ncall + 1, p.ss, "SegStartI")) ||
@@ -1149,76 +1220,120 @@ static int parse_tree(ngpu_ctx *c, const std::string &path, ParsedTree &p)
     return NGPU_OK;
 }
 
-extern "C" int ngpu_open(const char *dir, const char *suffix, int mechset, int device, ngpu_ctx **out)
+/* A value of a one-value-per-line file (Sim.csv, Stim_meta.csv): the first
+ * token of line k, or false when the line is missing or empty. */
+static bool first_token(const std::vector<std::string> &L, size_t k, double &x)
 {
-    if (!dir || !out) return fail(nullptr, NGPU_E_ARG, "ngpu_open: NULL argument");
-    *out = nullptr;
-    std::string d(dir), sfx(suffix ? suffix : "");
-    ngpu_ctx *c = nullptr;
-    int rc = ngpu_create(device, &c);
-    if (rc) return rc;
-    auto bail = [&](int code) { g_last_error = c->err; ngpu_close(c); return code; };
+    if (k >= L.size()) return false;
+    std::vector<double> v = tokens(L[k]);
+    if (v.empty()) return false;
+    x = v[0];
+    return true;
+}
+
+/* Counts read from data files become allocation sizes: bounded here. */
+static bool count_ok(double x, double lo, double hi) { return std::isfinite(x) && x >= lo && x <= hi && x == std::floor(x); }
+
+static int open_into(ngpu_ctx *c, const std::string &d, const std::string &sfx, int mechset)
+{
+    const char *dir = d.c_str();
+    int rc;
     ParsedTree pt;
-    if ((rc = parse_tree(c, d + "/BasicConstSegP.csv", pt))) return bail(rc);
+    if ((rc = parse_tree(c, d + "/BasicConstSegP.csv", pt))) return rc;
     if (mechset < 0) {
         int nm = pt.t.nmodels;
         mechset = nm == SetTTPC::NMECH ? NGPU_MECH_TTPC : nm == SetMainen::NMECH ? NGPU_MECH_MAINEN : nm == 1 ? NGPU_MECH_PASSIVE : -1;
-        if (mechset < 0) { fail(c, NGPU_E_FILE, "cannot infer a mechanism set from NModels=%d", nm); return bail(NGPU_E_FILE); }
+        if (mechset < 0) return fail(c, NGPU_E_FILE, "cannot infer a mechanism set from NModels=%d", nm);
     }
     pt.t.mechset = mechset;
-    if ((rc = ngpu_set_tree(c, &pt.t))) return bail(rc);
+    if ((rc = ngpu_set_tree(c, &pt.t))) return rc;
     /* Sim.csv */
     std::vector<std::string> L;
-    if (!read_lines(d + "/Sim.csv", L) || L.size() < 3) { fail(c, NGPU_E_FILE, "cannot read %s/Sim.csv", dir); return bail(NGPU_E_FILE); }
-    std::vector<double> v0 = tokens(L[0]), nr = tokens(L[2]);
-    if (v0.size() != 1 || nr.size() != 1) { fail(c, NGPU_E_FILE, "Sim.csv malformed"); return bail(NGPU_E_FILE); }
+    double v0, nrd;
+    if (!read_lines(d + "/Sim.csv", L) || !first_token(L, 0, v0) || !first_token(L, 2, nrd))
+        return fail(c, NGPU_E_FILE, "cannot read %s/Sim.csv (v_init on line 1, nrecsite on line 3)", dir);
+    if (!count_ok(nrd, 0, 64)) return fail(c, NGPU_E_FILE, "Sim.csv: nrecsite %g is not a count in 0..64", nrd);
     std::vector<int> rec;
-    if ((int)nr[0] > 0) {
+    if ((int)nrd > 0) {
         std::vector<double> rs = L.size() > 3 ? tokens(L[3]) : std::vector<double>();
-        if ((int)rs.size() != (int)nr[0]) { fail(c, NGPU_E_FILE, "Sim.csv: %d record sites declared, %zu given", (int)nr[0], rs.size()); return bail(NGPU_E_FILE); }
-        for (double r : rs) rec.push_back((int)r);
+        if ((int)rs.size() != (int)nrd) return fail(c, NGPU_E_FILE, "Sim.csv: %d record sites declared, %zu given", (int)nrd, rs.size());
+        for (double r : rs) {
+            if (!count_ok(r, 0, pt.t.N - 1)) return fail(c, NGPU_E_FILE, "Sim.csv: record site %g outside the tree", r);
+            rec.push_back((int)r);
+        }
     }
-    if ((rc = ngpu_set_sim(c, v0[0], (int)rec.size(), rec.data()))) return bail(rc);
+    if ((rc = ngpu_set_sim(c, v0, (int)rec.size(), rec.data()))) return rc;
     /* AllParams.csv */
     L.clear();
-    if (!read_lines(d + "/AllParams.csv", L) || L.empty()) { fail(c, NGPU_E_FILE, "cannot read %s/AllParams.csv", dir); return bail(NGPU_E_FILE); }
+    if (!read_lines(d + "/AllParams.csv", L) || L.empty()) return fail(c, NGPU_E_FILE, "cannot read %s/AllParams.csv", dir);
     std::vector<double> hdr = tokens(L[0]);
-    int nsets = hdr.size() == 1 ? (int)hdr[0] : -1;
-    int np = mech_nparams(mechset);
-    if (nsets < 1 || (int)L.size() - 1 < nsets) { fail(c, NGPU_E_FILE, "AllParams.csv: NSets=%d but %zu rows", nsets, L.size() - 1); return bail(NGPU_E_FILE); }
+    const int np = mech_nparams(mechset);
+    if (hdr.size() != 1 || !count_ok(hdr[0], 1, (double)L.size() - 1))
+        return fail(c, NGPU_E_FILE, "AllParams.csv: the NSets header is not a count matching the %zu rows", L.size() - 1);
+    const int nsets = (int)hdr[0];
     std::vector<double> P;
     P.reserve((size_t)nsets * np * pt.t.ncomps);
     for (int s = 0; s < nsets; s++) {
         std::vector<double> row = tokens(L[1 + s]);
-        if ((int)row.size() != np * pt.t.ncomps) { fail(c, NGPU_E_FILE, "AllParams.csv row %d has %zu values, expected %d", s, row.size(), np * pt.t.ncomps); return bail(NGPU_E_FILE); }
+        if ((int)row.size() != np * pt.t.ncomps) return fail(c, NGPU_E_FILE, "AllParams.csv row %d has %zu values, expected %d", s, row.size(), np * pt.t.ncomps);
         P.insert(P.end(), row.begin(), row.end());
     }
-    if ((rc = ngpu_set_params(c, P.data(), nsets, pt.t.ncomps, np))) return bail(rc);
-    /* Stim_meta.csv, Stim_raw<sfx>.csv, times<sfx>.csv */
+    if ((rc = ngpu_set_params(c, P.data(), nsets, pt.t.ncomps, np))) return rc;
+    /* Stim_meta.csv: NStimuli, Nt, comp, loc (1-based segment), area */
     L.clear();
-    if (!read_lines(d + "/Stim_meta.csv", L) || L.size() < 5) { fail(c, NGPU_E_FILE, "cannot read %s/Stim_meta.csv", dir); return bail(NGPU_E_FILE); }
-    int nstim = (int)tokens(L[0]).at(0), nt = (int)tokens(L[1]).at(0), loc = (int)tokens(L[3]).at(0);
-    double area = tokens(L[4]).at(0);
+    double fnstim, fnt, floc, area;
+    if (!read_lines(d + "/Stim_meta.csv", L) || !first_token(L, 0, fnstim) || !first_token(L, 1, fnt) ||
+        !first_token(L, 3, floc) || !first_token(L, 4, area))
+        return fail(c, NGPU_E_FILE, "cannot read %s/Stim_meta.csv (NStimuli, Nt, comp, loc, area on lines 1-5)", dir);
+    if (!count_ok(fnstim, 1, 1 << 20) || !count_ok(fnt, 1, 1 << 26) || !count_ok(floc, 0, pt.t.N))
+        return fail(c, NGPU_E_FILE, "Stim_meta.csv: NStimuli=%g Nt=%g loc=%g out of range", fnstim, fnt, floc);
+    if (!std::isfinite(area) || !(area > 0)) return fail(c, NGPU_E_FILE, "Stim_meta.csv: area %g must be positive and finite", area);
+    const int nstim = (int)fnstim, nt = (int)fnt, loc = (int)floc;
     std::vector<std::string> R;
-    if (!read_lines(d + "/Stim_raw" + sfx + ".csv", R) || (int)R.size() < nstim) { fail(c, NGPU_E_FILE, "cannot read Stim_raw%s.csv", sfx.c_str()); return bail(NGPU_E_FILE); }
+    if (!read_lines(d + "/Stim_raw" + sfx + ".csv", R) || (int)R.size() < nstim) return fail(c, NGPU_E_FILE, "cannot read Stim_raw%s.csv with %d rows", sfx.c_str(), nstim);
     std::vector<double> amps((size_t)nstim * nt);
     for (int i = 0; i < nstim; i++) {
         std::vector<double> row = tokens(R[i]);
-        if ((int)row.size() < nt) { fail(c, NGPU_E_FILE, "Stim_raw row %d has %zu values, expected >= %d", i, row.size(), nt); return bail(NGPU_E_FILE); }
+        if ((int)row.size() < nt) return fail(c, NGPU_E_FILE, "Stim_raw row %d has %zu values, expected >= %d", i, row.size(), nt);
         std::copy(row.begin(), row.begin() + nt, amps.begin() + (size_t)i * nt);
     }
     std::vector<std::string> T;
-    if (!read_lines(d + "/times" + sfx + ".csv", T) && !read_lines(d + "/times.csv", T)) { fail(c, NGPU_E_FILE, "cannot read times.csv"); return bail(NGPU_E_FILE); }
+    if (!read_lines(d + "/times" + sfx + ".csv", T) && !read_lines(d + "/times.csv", T)) return fail(c, NGPU_E_FILE, "cannot read times.csv");
     std::vector<double> dts = T.empty() ? std::vector<double>() : tokens(T[0]);
-    if ((int)dts.size() < nt) { fail(c, NGPU_E_FILE, "times.csv has %zu values, expected >= %d", dts.size(), nt); return bail(NGPU_E_FILE); }
+    if ((int)dts.size() < nt) return fail(c, NGPU_E_FILE, "times.csv has %zu values, expected >= %d", dts.size(), nt);
     dts.resize(nt);
-    if ((rc = ngpu_set_stimuli(c, amps.data(), nstim, nt, dts.data(), nt, loc - 1, area))) return bail(rc);
+    return ngpu_set_stimuli(c, amps.data(), nstim, nt, dts.data(), nt, loc - 1, area);
+}
+
+/* The files are parsed and checked into a host-only context first (no device
+ * is touched, so a malformed directory is reported the same way on any
+ * machine); the device part is set up afterwards. */
+extern "C" int ngpu_open(const char *dir, const char *suffix, int mechset, int device, ngpu_ctx **out)
+{
+    if (!dir || !out) return fail(nullptr, NGPU_E_ARG, "ngpu_open: NULL argument");
+    *out = nullptr;
+    ngpu_ctx *c = new (std::nothrow) ngpu_ctx();
+    if (!c) return fail(nullptr, NGPU_E_NOMEM, "host allocation failed");
+    int rc;
+    try {
+        rc = open_into(c, std::string(dir), std::string(suffix ? suffix : ""), mechset);
+    } catch (const std::bad_alloc &) {
+        rc = fail(c, NGPU_E_NOMEM, "host allocation failed while reading %s", dir);
+    } catch (const std::exception &ex) {
+        rc = fail(c, NGPU_E_FILE, "malformed data in %s: %s", dir, ex.what());
+    }
+    if (!rc) rc = init_device(c, device);
+    if (rc) {
+        g_last_error = c->err;
+        ngpu_close(c);
+        return rc;
+    }
     *out = c;
     return NGPU_OK;
 }
 
 extern "C" int ngpu_write_vhot(ngpu_ctx *c, const char *path)
-{
+try {
     if (!c || !path) return fail(c, NGPU_E_ARG, "NULL argument");
     if (!c->ran) return fail(c, NGPU_E_STATE, "no completed run");
     long long n = ngpu_vhot_size(c);
@@ -1234,3 +1349,4 @@ extern "C" int ngpu_write_vhot(ngpu_ctx *c, const char *path)
     ok = (std::fclose(fp) == 0) && ok;
     return ok ? NGPU_OK : fail(c, NGPU_E_FILE, "short write to %s", path);
 }
+NGPU_CATCH(c, NGPU_E_ARG)

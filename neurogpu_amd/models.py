@@ -444,22 +444,30 @@ class Workload:
         return float(self.nruns) * self.model.tree.N * self.dts.size
 
 
-# Uneven batch: (segments, instances, dt steps) of four TTPC-like cells.
+# Uneven batch: (segments, instances, dt steps) of four cells.
 MIXED = ((200, 12000, 10000), (700, 4000, 5000), (1500, 1000, 2000), (3000, 256, 40000))
 
 
-def mixed_workloads(nsets_scale: float = 1.0, nt_scale: float = 1.0, seed: int = 1003) -> List["Workload"]:
+def mixed_workloads(nsets_scale: float = 1.0, nt_scale: float = 1.0, seed: int = 1003,
+                    mechset: str = "mainen") -> List["Workload"]:
     """The uneven batch: pyramidal morphologies of 200, 700, 1,500 and 3,000
-    segments with the TTPC mechanisms, 12,000 / 4,000 / 1,000 / 256 instances
-    and 10,000 / 5,000 / 2,000 / 40,000 steps of dt = 0.025 ms, one 0.5 nA step
-    from 10 ms to 80 % of each run.  Seeds seed + 10 k per morphology.  The
-    scales shrink instances and steps for tests."""
+    segments with the active cell's channel set (Mainen & Sejnowski: na, kv,
+    km, ca, kca, cad, pas, as :func:`active_model`), 12,000 / 4,000 / 1,000 /
+    256 instances and 10,000 / 5,000 / 2,000 / 40,000 steps of dt = 0.02 ms,
+    one 0.5 nA step from 10 ms to 80 % of each run.  Seeds seed + 10 k per
+    morphology.  ``mechset="ttpc"`` builds the same sizes with the TTPC
+    mechanisms (:func:`ttpc_model`, dt = 0.025 ms) instead.  The scales
+    shrink instances and steps for tests."""
+    if mechset not in ("mainen", "ttpc"):
+        raise ValueError(f"mixed batches use the mainen or ttpc set, not {mechset!r}")
     out = []
     for k, (n, inst, nt) in enumerate(MIXED):
-        model = ttpc_model(seed + 10 * k, n_target=n)
+        if mechset == "mainen":
+            model, dt = active_model(seed + 10 * k, n_target=n), 0.02
+        else:
+            model, dt = ttpc_model(seed + 10 * k, n_target=n), 0.025
         ns = max(1, int(round(inst * nsets_scale)))
         T = max(2, int(round(nt * nt_scale)))
-        dt = 0.025
         x = model.sample(ns, seed + 10 * k)
         a, d = step_stimulus([0.5], T, dt, 10.0, 0.8 * T * dt)
         out.append(Workload(f"mixed:{model.name}", model, x, a, d))

@@ -113,7 +113,13 @@ def simulate(t, mechset, params, amps, dts, stim_seg, stim_area, v_init, rec_sit
     vhot = np.zeros((nsets, nstim, rec.size, nt), dtype=np.float64)
     vfin = np.zeros((nsets * nstim, t.N), dtype=np.float64) if want_vfinal else None
     if nthreads is None:
-        nthreads = os.cpu_count() or 1
+        try:
+            nthreads = len(os.sched_getaffinity(0))
+        except (AttributeError, OSError):
+            nthreads = os.cpu_count() or 1
+        env = os.environ.get("OMP_NUM_THREADS", "")
+        if env.isdigit() and int(env) > 0:
+            nthreads = min(nthreads, int(env))
     rc = L.ref_simulate(C.byref(rt.s), params.ctypes.data_as(_DP), nsets,
                         amps.ctypes.data_as(_DP), nstim, amps.shape[1],
                         dts.ctypes.data_as(_DP), nt, int(stim_seg), float(stim_area),

@@ -5,7 +5,12 @@ Features replace the GA evaluator's eFEL pass (PYS/efel_ext.py:23-97,
 threshold -30 mV at :15); eFEL itself is not available, so their definitions
 are this build's and they are pinned by known-answer traces in the tests.
 The arithmetic follows the same operation order as the device code, so the two
-agree bit for bit."""
+agree bit for bit.
+
+A spike is an upward crossing of thr whose linearly interpolated time tc lies
+in the stimulus window t_on <= tc < t_off; crossings outside it are ignored.
+Width and amplitude of a spike that starts in the window are measured in full,
+also when it falls back after t_off."""
 import math
 
 import numpy as np
@@ -13,7 +18,7 @@ import numpy as np
 NFEAT = 5
 
 
-def trace_features(v, times, t_on, thr=-30.0):
+def trace_features(v, times, t_on, t_off=math.inf, thr=-30.0):
     v = [float(x) for x in v]
     nt = len(v)
     vb, nb = 0.0, 0
@@ -32,15 +37,16 @@ def trace_features(v, times, t_on, thr=-30.0):
         ta, tb = float(times[i - 1]), float(times[i])
         if a < thr and b >= thr:
             tc = ta + (thr - a) / (b - a) * (tb - ta)
-            cnt += 1
-            if cnt == 1:
-                first = tc - t_on
-            else:
-                sum_isi += tc - last_up
-            last_up = tc
-            t_up = tc
-            peak = b
-            in_spike = True
+            if t_on <= tc < t_off:
+                cnt += 1
+                if cnt == 1:
+                    first = tc - t_on
+                else:
+                    sum_isi += tc - last_up
+                last_up = tc
+                t_up = tc
+                peak = b
+                in_spike = True
         elif in_spike:
             if b > peak:
                 peak = b
@@ -58,7 +64,7 @@ def trace_features(v, times, t_on, thr=-30.0):
                      sum_w / nw if nw else math.nan, sum_isi / (cnt - 1) if cnt > 1 else math.nan])
 
 
-def features(vhot, dts, t_on, thr=-30.0):
+def features(vhot, dts, t_on, t_off=math.inf, thr=-30.0):
     """vhot [nsets, nstim, nrec, nt] -> [nsets, nstim, 5] for record site 0."""
     vhot = np.asarray(vhot)
     # step start times as a running sum, as the device computes them
@@ -69,8 +75,9 @@ def features(vhot, dts, t_on, thr=-30.0):
         acc += float(d)
     times = tt
     ons = np.broadcast_to(np.asarray(t_on, np.float64), (vhot.shape[1],))
+    offs = np.broadcast_to(np.asarray(t_off, np.float64), (vhot.shape[1],))
     out = np.empty(vhot.shape[:2] + (NFEAT,))
     for s in range(vhot.shape[0]):
         for k in range(vhot.shape[1]):
-            out[s, k] = trace_features(vhot[s, k, 0], times, float(ons[k]), thr)
+            out[s, k] = trace_features(vhot[s, k, 0], times, float(ons[k]), float(offs[k]), thr)
     return out

@@ -19,8 +19,10 @@ def _say(msg):
     sys.stdout.flush()
 
 
-def _mixed(nsets=(9, 5, 3, 2), nts=(300, 170, 90, 420)):
-    ws = models.mixed_workloads(1.0, 1.0)
+def _mixed(nsets=(9, 5, 3, 2), nts=(300, 170, 90, 420), mechset="mainen"):
+    """The uneven batch's four morphologies (the active cell's channel set by
+    default), cut to a few sets and steps."""
+    ws = models.mixed_workloads(1.0, 1.0, mechset=mechset)
     out = []
     for w, ns, nt in zip(ws, nsets, nts):
         out.append(models.Workload(w.name, w.model, w.x[:ns], w.amps[:, :nt], w.dts[:nt]))
@@ -84,10 +86,11 @@ def test_tree_key_is_stable_and_distinct():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("choice", [None, "wave_global", "block_lds", "two_waves"])
-def test_batch_matches_single_runs_and_oracle(choice):
-    ws = _mixed()
-    _say(f"case batch of {[w.model.tree.N for w in ws]} segments, choice {choice}")
+@pytest.mark.parametrize("mechset,choice", [("mainen", None), ("mainen", "wave_global"), ("mainen", "block_lds"),
+                                            ("mainen", "two_waves"), ("ttpc", None), ("ttpc", "two_waves")])
+def test_batch_matches_single_runs_and_oracle(mechset, choice):
+    ws = _mixed(mechset=mechset)
+    _say(f"case batch of {[w.model.tree.N for w in ws]} {mechset} segments, choice {choice}")
     alone = [_alone(w) for w in ws]
     sims = [workload_simulator(w) for w in ws]
     try:
@@ -141,7 +144,7 @@ def test_batch_runs_twice_and_features_follow():
 
 @pytest.mark.gpu
 def test_batch_errors():
-    w_t = _mixed(nsets=(2, 1, 1, 1), nts=(20, 20, 20, 20))[0]
+    w_t = _mixed(nsets=(2, 1, 1, 1), nts=(20, 20, 20, 20), mechset="ttpc")[0]
     w_m = models.workload("active500", nsets=2, nt=20)
     st = workload_simulator(w_t)
     sm = workload_simulator(w_m)

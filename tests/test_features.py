@@ -40,6 +40,28 @@ def test_spike_cut_by_the_end_counts_amplitude_not_width():
     assert f[0] == 1 and math.isnan(f[3]) and f[2] > 0
 
 
+def test_spikes_outside_the_window_are_ignored():
+    # crossings of -30 at c - 0.6: 2.4 (before t_on = 5), 9.4 and 19.4 (in the
+    # window), 35.4 (after t_off = 30)
+    v, t = _triangles([3.0, 10.0, 20.0, 36.0])
+    f = features_ref.trace_features(v, t, 5.0, 30.0)
+    vbase = float(np.mean(v[t < 5.0]))          # the early spike raises the baseline
+    np.testing.assert_allclose(f, [2.0, 4.4, 30.0 - vbase, 1.2, 10.0], rtol=0, atol=1e-9)
+    # the whole trace as the window counts all four
+    f = features_ref.trace_features(v, t, 0.0, 40.0)
+    assert f[0] == 4 and abs(f[1] - 2.4) < 1e-9
+
+
+def test_spike_that_starts_in_the_window_is_measured_in_full():
+    # up at 19.4 < t_off = 20, down at 20.6: counted, width 1.2
+    v, t = _triangles([20.0])
+    f = features_ref.trace_features(v, t, 5.0, 20.0)
+    np.testing.assert_allclose(f[:4], [1.0, 14.4, 100.0, 1.2], rtol=0, atol=1e-9)
+    # up at 19.4 = t_off: outside
+    f = features_ref.trace_features(v, t, 5.0, 19.4 - 1e-12)
+    assert f[0] == 0 and math.isnan(f[2])
+
+
 def test_baseline_is_mean_before_onset():
     v, t = _triangles([20.0])
     v[:50] = -60.0                      # t < 5 ms
@@ -68,8 +90,8 @@ def test_device_features_match_oracle():
     # the device features are a function of the trace: on the GPU's own traces
     # they equal the CPU restatement bit for bit; on the oracle's traces they
     # agree to the trace tolerance (spike counts exactly)
-    np.testing.assert_array_equal(gf, features_ref.features(gv, w.dts, 20.0))
-    fr = features_ref.features(ref, w.dts, 20.0)
+    np.testing.assert_array_equal(gf, features_ref.features(gv, w.dts, 20.0, 180.0))
+    fr = features_ref.features(ref, w.dts, 20.0, 180.0)
     np.testing.assert_array_equal(gf[..., 0], fr[..., 0])
     np.testing.assert_allclose(gf, fr, rtol=0, atol=1e-5, equal_nan=True)
     assert np.nansum(gf[..., 0]) > 0

@@ -65,6 +65,54 @@ def test_ttpc1200_small(launch):
     _check(gpu, _oracle(w), f"ttpc1200 {launch}")
 
 
+def test_ttpc1200_full_protocol():
+    """Configuration 3 over its whole 300 ms protocol (12,000 steps), two
+    parameter sets x all five stimuli, every run checked."""
+    w = models.workload("ttpc1200", nsets=2)
+    _say(f"case ttpc1200 full protocol: {w.nruns} runs x {w.dts.size} steps")
+    gpu, st = simulate_workload(w)
+    _check(gpu, _oracle(w), "ttpc1200 full protocol")
+
+
+@pytest.mark.parametrize("name,cuts", [("active500", (0, 500, 1100, 1500)), ("ttpc1200", (0, 1, 64, 700, 1200))])
+def test_windows_equal_the_whole_run(name, cuts):
+    """ngpu_run_steps over consecutive windows carries the state: the traces
+    equal one ngpu_run of the same steps bit for bit, and a window that does
+    not start where the last one ended is refused."""
+    w = models.workload(name, nsets=4, nt=cuts[-1])
+    _say(f"case {name} windows {cuts}")
+    whole, _ = simulate_workload(w)
+    m = w.model
+    with Simulator(0) as sim:
+        sim.set_tree(m.tree, m.mechset_id)
+        sim.set_sim(m.v_init, m.rec_sites)
+        sim.set_params(w.params, m.ncomps, m.nparams)
+        sim.set_stimuli(w.amps, w.dts, m.stim_seg, m.stim_area)
+        with pytest.raises(NgpuError):
+            sim.run_steps(cuts[1], cuts[2])        # no state at cuts[1] yet
+        for a, b in zip(cuts[:-1], cuts[1:]):
+            st = sim.run_steps(a, b)
+            assert st["comp_steps"] == w.nruns * m.tree.N * (b - a)
+        with pytest.raises(NgpuError):
+            sim.run_steps(cuts[1], cuts[2])        # the state is at the end now
+        got = sim.fetch().reshape(whole.shape)
+        feats = sim.features(20.0, 180.0)
+        assert feats.shape == (4, w.amps.shape[0], 5)
+    np.testing.assert_array_equal(got, whole)
+
+
+def test_open_reads_sizes_from_the_files():
+    """A context from ngpu_open knows its sizes (NSets, NStimuli, Nt from the
+    files), so features() and fetch() size their buffers from the C side."""
+    _say("case ngpu_open sizes")
+    with Simulator.open(TTPC_DIR) as sim:
+        assert (sim.nsets, sim.nstim, sim.nrec, sim.nt) == (24, 1, 1, 3168)
+        sim.run(fetch=False)
+        f = sim.features(10.0, 70.0)
+        assert f.shape == (24, 1, 5)
+        assert sim.fetch().size == 24 * 3168
+
+
 def test_reference_ttpc_fixture_through_file_boundary(tmp_path):
     """The reference's own data directory, parsed by ngpu_open (C++), run, and
     written as VHotP.dat; compared with the oracle run on the Python parse."""

@@ -34,8 +34,9 @@ def test_gradients_match_cpu_restatement():
         ref, rloss = mlp_ref.gradients(params, x, t)
         assert abs(loss - rloss) < 1e-3 * abs(rloss)
         gd = ens.unpack(np.concatenate([np.zeros(mem * ens.nparam, np.float32), g]), mem)
-        for k, (a, b) in enumerate(zip(gd, ref)):
-            err = np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)
+        errs = [float(np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)) for a, b in zip(gd, ref)]
+        print(f"member {mem}: relative gradient errors {['%.2e' % e for e in errs]}", flush=True)
+        for k, err in enumerate(errs):
             assert err < 2e-2, (mem, k, err)
     ens.close()
 
