@@ -122,6 +122,17 @@ int ngpu_get_sizes(const ngpu_ctx *ctx, int *nsets, int *nstim, int *nrec, int *
 #define NGPU_NFEAT 5
 int ngpu_features(ngpu_ctx *ctx, const double *t_on, const double *t_off, double thr, double *out);
 
+/* ngpu_features without any allocation: no counterpart in the reference.  It
+ * is for callers that cannot allocate (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The stimulus windows and
+ * the features are staged in a fixed device array of 2^20 doubles (8 MiB),
+ * which each device holds from the first call until the process ends:
+ * 2 * nstim (rounded up to 32) + 5 * nsets * nstim must fit in it, else
+ * NGPU_E_ARG.  Calls on one device share that array, so calls that overlap
+ * on one device race; its contents after a call are unspecified.  Same
+ * arguments and results as ngpu_features. */
+int ngpu_features_noalloc(ngpu_ctx *ctx, const double *t_on, const double *t_off, double thr, double *out);
+
 /* Copy the traces of the last run to vhot (same layout as ngpu_run). */
 int ngpu_fetch_vhot(ngpu_ctx *ctx, double *vhot);
 

@@ -27,7 +27,7 @@ EXPORTS = ("ngpu_create", "ngpu_set_tree", "ngpu_set_sim", "ngpu_set_params", "n
            "ngpu_mlp_predict", "ngpu_mlp_gradients", "ngpu_mlp_last_error", "ngpu_mlp_close", "ngpu_gemm_bf16",
            "ngpu_batch_create", "ngpu_batch_add", "ngpu_batch_set_launch", "ngpu_batch_run",
            "ngpu_batch_member_info", "ngpu_batch_last_error", "ngpu_batch_close", "ngpu_run_steps",
-           "ngpu_get_sizes")
+           "ngpu_get_sizes", "ngpu_features_noalloc")
 NFEAT = 5
 FEATURES = ("spike_count", "first_latency", "ap_amplitude", "ap_width", "mean_isi")
 
@@ -118,6 +118,8 @@ def lib():
         L.ngpu_last_error.restype = C.c_char_p
         L.ngpu_features.argtypes = [vp, _DP, _DP, C.c_double, _DP]
         L.ngpu_features.restype = C.c_int
+        L.ngpu_features_noalloc.argtypes = [vp, _DP, _DP, C.c_double, _DP]
+        L.ngpu_features_noalloc.restype = C.c_int
         L.ngpu_close.argtypes = [vp]
         L.ngpu_close.restype = None
         L.ngpu_batch_create.argtypes = [C.c_int, C.POINTER(vp)]

@@ -22,11 +22,15 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #define NG_NFEAT 5
 #define NG_FT 64
 
+/* t_i, the start time of step i, is the running sum dt_0 + ... + dt_{i-1}
+ * of the protocol's steps, accumulated by each lane in step order. */
 __global__ void __launch_bounds__(64) ngpu_features_kernel(const double *__restrict__ vhot, int nrec, int nt,
-                                                           int nruns, int nstim, const double *__restrict__ times,
+                                                           int nruns, int nstim, const double *__restrict__ dts,
                                                            const double *__restrict__ t_on,
                                                            const double *__restrict__ t_off, double thr,
                                                            double *__restrict__ out)
@@ -45,7 +49,7 @@ __global__ void __launch_bounds__(64) ngpu_features_kernel(const double *__restr
     bool base_open = true;
     int cnt = 0, namp = 0, nw = 0;
     double first = nan, last_up = 0.0, sum_isi = 0.0, sum_amp = 0.0, sum_w = 0.0, t_up = 0.0, peak = 0.0;
-    double prev = 0.0, v0 = 0.0;
+    double prev = 0.0, v0 = 0.0, tnow = 0.0, ta = 0.0;
     bool in_spike = false;
     for (int c0 = 0; c0 < nt; c0 += NG_FT) {
         const int n = min(NG_FT, nt - c0);
@@ -59,7 +63,8 @@ __global__ void __launch_bounds__(64) ngpu_features_kernel(const double *__restr
         for (int k = 0; k < n; k++) {
             const int i = c0 + k;
             const double b = tile[lane][k];
-            const double tb = times[i];
+            const double tb = tnow;
+            tnow += dts[i];
             if (i == 0) v0 = b;
             if (base_open) {
                 if (tb < on) {
@@ -71,7 +76,7 @@ __global__ void __launch_bounds__(64) ngpu_features_kernel(const double *__restr
                 if (!base_open) vb = nb > 0 ? vsum / nb : v0;
             }
             if (i > 0) {
-                const double a = prev, ta = times[i - 1];
+                const double a = prev;
                 if (a < thr && b >= thr) {
                     const double tc = ta + (thr - a) / (b - a) * (tb - ta);
                     if (tc >= on && tc < off) {
@@ -96,6 +101,7 @@ __global__ void __launch_bounds__(64) ngpu_features_kernel(const double *__restr
                 }
             }
             prev = b;
+            ta = tb;
         }
     }
     if (!live) return;
@@ -108,10 +114,39 @@ __global__ void __launch_bounds__(64) ngpu_features_kernel(const double *__restr
     o[4] = cnt > 1 ? sum_isi / (cnt - 1) : nan;
 }
 
-hipError_t ngpu_launch_features(const double *vhot, int nrec, int nt, int nruns, int nstim, const double *times,
+hipError_t ngpu_launch_features(const double *vhot, int nrec, int nt, int nruns, int nstim, const double *dts,
                                 const double *t_on, const double *t_off, double thr, double *out, hipStream_t s)
 {
     hipLaunchKernelGGL(ngpu_features_kernel, dim3((nruns + 63) / 64), dim3(64), 0, s, vhot, nrec, nt, nruns, nstim,
-                       times, t_on, t_off, thr, out);
+                       dts, t_on, t_off, thr, out);
     return hipGetLastError();
+}
+
+/* Device scratch of the no-allocation feature entry point: the stimulus
+ * windows and the features, staged on their way in and out.  Each device
+ * holds its own copy for the life of the process; every call on a device
+ * shares it, so calls that overlap on one device race. */
+#define NG_FEAT_SCRATCH_DOUBLES (1 << 20)
+static __device__ __align__(256) double scratch[NG_FEAT_SCRATCH_DOUBLES];
+
+#define NG_MAX_DEVICES 64
+
+/* The device address of `scratch` on the current device (`device`), looked
+ * up once per device with hipGetSymbolAddress; cap = its size in doubles. */
+hipError_t ngpu_features_scratch(int device, double **ptr, size_t *cap)
+{
+    static std::mutex mu;
+    static double *addr[NG_MAX_DEVICES];
+    if (device < 0 || device >= NG_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        addr[device] = (double *)p;
+    }
+    *ptr = addr[device];
+    *cap = NG_FEAT_SCRATCH_DOUBLES;
+    return hipSuccess;
 }

@@ -31,8 +31,9 @@
 /* Per-set launchers, one translation unit each (sim_<set>.hip). */
 hipError_t ngpu_launch_passive(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
 hipError_t ngpu_launch_ttpc(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
-hipError_t ngpu_launch_features(const double *vhot, int nrec, int nt, int nruns, int nstim, const double *times,
+hipError_t ngpu_launch_features(const double *vhot, int nrec, int nt, int nruns, int nstim, const double *dts,
                                 const double *t_on, const double *t_off, double thr, double *out, hipStream_t s);
+hipError_t ngpu_features_scratch(int device, double **ptr, size_t *cap);
 hipError_t ngpu_launch_mainen(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
 hipError_t ngpu_launch_batch_passive(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
 hipError_t ngpu_launch_batch_ttpc(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
@@ -100,7 +101,7 @@ struct ngpu_ctx {
     std::vector<int> qmask;        /* mechanism mask by position */
     std::vector<int> msched;       /* mechanism-pass schedule of the last (W, R) */
     int msched_W = 0, msched_R = 0;
-    DevArr<double> d_qcm, d_ef, d_pval, d_pconst, d_pv, d_dts, d_amps, d_states, d_vhot, d_times, d_fwin, d_feat;
+    DevArr<double> d_qcm, d_ef, d_pval, d_pconst, d_pv, d_dts, d_amps, d_states, d_vhot, d_fwin, d_feat;
     int stim_q = -1;
     std::vector<int> rec_q;
     unsigned pconst_mask = 0;
@@ -215,7 +216,7 @@ extern "C" void ngpu_close(ngpu_ctx *c)
     }
     drop_graph(c);
     for (auto *d : {&c->d_qlink, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_rec, &c->d_msched}) d->release();
-    for (auto *d : {&c->d_qcm, &c->d_ef, &c->d_pval, &c->d_pconst, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_vhot, &c->d_times, &c->d_fwin, &c->d_feat}) d->release();
+    for (auto *d : {&c->d_qcm, &c->d_ef, &c->d_pval, &c->d_pconst, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_vhot, &c->d_fwin, &c->d_feat}) d->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1110,15 +1111,13 @@ try {
     if (!c->ran) return fail(c, NGPU_E_STATE, "ngpu_features needs a completed ngpu_run");
     if (c->rec.empty()) return fail(c, NGPU_E_STATE, "no record site");
     HIPCHK(c, hipSetDevice(c->device));
-    std::vector<double> times(c->nt), win(2 * c->nstim);
-    double t = 0.0;
-    for (int i = 0; i < c->nt; i++) { times[i] = t; t += c->dts[i]; }
+    std::vector<double> win(2 * c->nstim);
     for (int k = 0; k < c->nstim; k++) { win[k] = t_on[k]; win[c->nstim + k] = t_off[k]; }
     const int nruns = c->nsets * c->nstim;
     int rc;
-    if ((rc = upload(c, c->d_times, times)) || (rc = upload(c, c->d_fwin, win)) || (rc = alloc(c, c->d_feat, (size_t)nruns * NGPU_NFEAT)))
+    if ((rc = upload(c, c->d_fwin, win)) || (rc = alloc(c, c->d_feat, (size_t)nruns * NGPU_NFEAT)))
         return rc;
-    hipError_t e = ngpu_launch_features(c->d_vhot.p, (int)c->rec.size(), c->nt, nruns, c->nstim, c->d_times.p, c->d_fwin.p,
+    hipError_t e = ngpu_launch_features(c->d_vhot.p, (int)c->rec.size(), c->nt, nruns, c->nstim, c->d_dts.p, c->d_fwin.p,
                                         c->d_fwin.p + c->nstim, thr, c->d_feat.p, c->stream);
     if (e != hipSuccess) return fail(c, NGPU_E_HIP, "feature kernel launch failed: %s", hipGetErrorString(e));
     HIPCHK(c, hipMemcpyAsync(out, c->d_feat.p, (size_t)nruns * NGPU_NFEAT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
@@ -1126,6 +1125,35 @@ try {
     return NGPU_OK;
 }
 NGPU_CATCH(c, NGPU_E_ARG)
+
+/* The features through the device's fixed scratch array instead of buffers
+ * the context allocates: [0, nstim) t_on, [nstim, 2 nstim) t_off, then the
+ * features from the next 256-byte boundary.  No allocation of any kind. */
+extern "C" int ngpu_features_noalloc(ngpu_ctx *c, const double *t_on, const double *t_off, double thr, double *out)
+{
+    if (!c || !t_on || !t_off || !out) return fail(c, NGPU_E_ARG, "ngpu_features_noalloc: NULL argument");
+    HIPCHK(c, hipSetDevice(c->device));
+    double *scr = nullptr;
+    size_t cap = 0;
+    hipError_t e = ngpu_features_scratch(c->device, &scr, &cap);
+    if (e != hipSuccess) return fail(c, NGPU_E_HIP, "device address of the feature scratch: %s", hipGetErrorString(e));
+    const size_t nstim = (size_t)std::max(c->nstim, 0), nruns = (size_t)std::max(c->nsets, 0) * nstim;
+    const size_t feat0 = (2 * nstim + 31) / 32 * 32;
+    if (feat0 + nruns * NGPU_NFEAT > cap)
+        return fail(c, NGPU_E_ARG, "%zu runs x %d features and %zu stimulus windows need %zu doubles of scratch; "
+                    "ngpu_features_noalloc holds %zu (use ngpu_features)", nruns, NGPU_NFEAT, nstim,
+                    feat0 + nruns * NGPU_NFEAT, cap);
+    if (!c->ran) return fail(c, NGPU_E_STATE, "ngpu_features_noalloc needs a completed ngpu_run");
+    if (c->rec.empty()) return fail(c, NGPU_E_STATE, "no record site");
+    HIPCHK(c, hipMemcpyAsync(scr, t_on, nstim * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(scr + nstim, t_off, nstim * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    e = ngpu_launch_features(c->d_vhot.p, (int)c->rec.size(), c->nt, (int)nruns, (int)nstim, c->d_dts.p, scr, scr + nstim,
+                             thr, scr + feat0, c->stream);
+    if (e != hipSuccess) return fail(c, NGPU_E_HIP, "feature kernel launch failed: %s", hipGetErrorString(e));
+    HIPCHK(c, hipMemcpyAsync(out, scr + feat0, nruns * NGPU_NFEAT * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return NGPU_OK;
+}
 
 extern "C" int ngpu_fetch_vhot(ngpu_ctx *c, double *vhot)
 {
