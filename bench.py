@@ -85,13 +85,13 @@ def _pmc_pass(counter: str, child_args, kernel: str, timeout: int = 180):
         return [per[k] for k in sorted(per)] or None
 
 
-def measured_traffic(workload: str, nsets, nt, window: int):
+def measured_traffic(workload: str, nsets, nt, window: int, solve: str = "auto"):
     """HBM bytes of one window launch of the simulation kernel, from two
     rocprofv3 PMC passes (FETCH_SIZE, WRITE_SIZE; KiB per dispatch) over a
     child that runs the first two windows, read on the second window's
     dispatch; each counter is scaled by what it reads for a 1 GiB
     device-to-device copy (bytes copied / counter bytes)."""
-    child = ["--pmc-child", "--workload", workload, "--window", str(window)]
+    child = ["--pmc-child", "--workload", workload, "--window", str(window), "--solve", solve]
     if nsets is not None:
         child += ["--nsets", str(nsets)]
     if nt is not None:
@@ -263,6 +263,7 @@ def configured_simulator(w, args):
     sim.set_params(w.params, m.ncomps, m.nparams)
     sim.set_stimuli(w.amps, w.dts, m.stim_seg, m.stim_area)
     sim.set_launch(args.waves_per_run, args.runs_per_group, 0, False)
+    sim.set_solve({"auto": 0, "reference": 1, "fast": 2}[args.solve])
     return sim
 
 
@@ -293,6 +294,8 @@ def main():
     ap.add_argument("--nt", type=int, default=None)
     ap.add_argument("--waves-per-run", type=int, default=0)
     ap.add_argument("--runs-per-group", type=int, default=0)
+    ap.add_argument("--solve", default="auto", choices=("auto", "reference", "fast"),
+                    help="form of the Hines elimination (auto: division-free on well-scaled trees)")
     ap.add_argument("--cpu-sets", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-copy-peak", action="store_true")
@@ -343,7 +346,7 @@ def main():
     traffic = None
     if not args.no_pmc:
         say("rocprofv3 FETCH_SIZE / WRITE_SIZE passes (child processes)")
-        traffic = measured_traffic(args.workload, args.nsets, args.nt, wlen)
+        traffic = measured_traffic(args.workload, args.nsets, args.nt, wlen, args.solve)
         say(f"traffic per window launch: {traffic['bytes_per_launch']:.4e} B" if traffic else "traffic: unavailable")
     copy_peak = None if args.no_copy_peak else copy_peak_gbs()
 
@@ -392,7 +395,8 @@ def main():
                                f"{wlen} dt steps, the timed steps cover the protocol once",
                    "runs": w.nruns, "compartments": m.tree.N, "nt": nt,
                    "launch": {k: st[k] for k in ("waves_per_run", "runs_per_group", "slots_per_lane",
-                                                  "runs_per_block")}},
+                                                  "runs_per_block")},
+                   "solve": {1: "reference order", 2: "division-free"}[st["solve"]]},
         "roofline": roofline,
     }
     if not args.no_cpu:

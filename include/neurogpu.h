@@ -57,6 +57,7 @@ typedef struct {
     int nvar;                 /* distinct run-varying parameter columns               */
     int nsets, nstim, nrec, nt, N;
     int runs_per_group;       /* runs sharing one group of waves_per_run waves        */
+    int solve;                /* elimination used: NGPU_SOLVE_REFERENCE or NGPU_SOLVE_FAST */
 } ngpu_stats;
 
 /* Create a context on HIP device `device` (replaces cudaSetDevice in CANON/kernel.cu:24-31). */
@@ -89,6 +90,20 @@ int ngpu_set_stimuli(ngpu_ctx *ctx, const double *amps, int nstim, int nt_amps,
  * NSets x (32, NStim) (CANON/CudaStuff.cu:487-507). */
 int ngpu_set_launch(ngpu_ctx *ctx, int waves_per_run, int runs_per_group,
                     int steps_per_launch, int use_graph);
+
+/* The form of the Hines elimination (BeforeLU, CANON/CudaStuff.cu:31-69).
+ * NGPU_SOLVE_REFERENCE keeps the reference's operations in its order
+ * (u_i -= f (e / u), b_i -= (b e) / u, two IEEE divisions a row), which its
+ * exported trees need: they couple zero-area nodes with coefficients near
+ * 1e21.  NGPU_SOLVE_FAST carries each chain's pivot as a fraction, one
+ * dependent multiply-add a row instead of two divisions; results equal the
+ * reference order's up to rounding on well-scaled trees.  NGPU_SOLVE_AUTO
+ * (the default) takes the fast form when every cm > 0 and every |e|, |f| is
+ * at most 1e6.  No reference counterpart (the reference has one form). */
+#define NGPU_SOLVE_AUTO 0
+#define NGPU_SOLVE_REFERENCE 1
+#define NGPU_SOLVE_FAST 2
+int ngpu_set_solve(ngpu_ctx *ctx, int mode);
 
 /* Run every (set, stimulus) for all Nt steps and copy the traces to vhot
  * [nsets][nstim][nrec][nt] (caller-owned; may be NULL to keep them on the

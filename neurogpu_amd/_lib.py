@@ -27,7 +27,7 @@ EXPORTS = ("ngpu_create", "ngpu_set_tree", "ngpu_set_sim", "ngpu_set_params", "n
            "ngpu_mlp_predict", "ngpu_mlp_gradients", "ngpu_mlp_last_error", "ngpu_mlp_close", "ngpu_gemm_bf16",
            "ngpu_batch_create", "ngpu_batch_add", "ngpu_batch_set_launch", "ngpu_batch_run",
            "ngpu_batch_member_info", "ngpu_batch_last_error", "ngpu_batch_close", "ngpu_run_steps",
-           "ngpu_get_sizes", "ngpu_features_noalloc")
+           "ngpu_get_sizes", "ngpu_features_noalloc", "ngpu_set_solve")
 NFEAT = 5
 FEATURES = ("spike_count", "first_latency", "ap_amplitude", "ap_width", "mean_isi")
 
@@ -60,10 +60,13 @@ class ngpu_stats(C.Structure):
                 ("comp_steps_per_s", C.c_double), ("waves_per_run", C.c_int), ("slots_per_lane", C.c_int),
                 ("runs_per_block", C.c_int), ("launches", C.c_int), ("nvar", C.c_int),
                 ("nsets", C.c_int), ("nstim", C.c_int), ("nrec", C.c_int), ("nt", C.c_int), ("N", C.c_int),
-                ("runs_per_group", C.c_int)]
+                ("runs_per_group", C.c_int), ("solve", C.c_int)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+SOLVE_AUTO, SOLVE_REFERENCE, SOLVE_FAST = 0, 1, 2
 
 
 class ngpu_batch_launch(C.Structure):
@@ -105,6 +108,7 @@ def lib():
         L.ngpu_set_params.argtypes = [vp, _DP, C.c_int, C.c_int, C.c_int]
         L.ngpu_set_stimuli.argtypes = [vp, _DP, C.c_int, C.c_int, _DP, C.c_int, C.c_int, C.c_double]
         L.ngpu_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.ngpu_set_solve.argtypes = [vp, C.c_int]
         L.ngpu_run.argtypes = [vp, _DP, C.POINTER(ngpu_stats)]
         L.ngpu_run_steps.argtypes = [vp, C.c_int, C.c_int, C.POINTER(ngpu_stats)]
         L.ngpu_get_sizes.argtypes = [vp] + [_IP] * 5
@@ -136,7 +140,7 @@ def lib():
             getattr(L, name).restype = C.c_int
         for name in ("ngpu_create", "ngpu_set_tree", "ngpu_set_sim", "ngpu_set_params", "ngpu_set_stimuli",
                      "ngpu_set_launch", "ngpu_run", "ngpu_fetch_vhot", "ngpu_open", "ngpu_write_vhot",
-                     "ngpu_mechset_nparams", "ngpu_run_steps", "ngpu_get_sizes"):
+                     "ngpu_mechset_nparams", "ngpu_run_steps", "ngpu_get_sizes", "ngpu_set_solve"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -228,6 +232,10 @@ class Simulator:
         self._check(self._L.ngpu_set_launch(self._h, int(waves_per_run), int(runs_per_group),
                                             int(steps_per_launch), int(bool(use_graph))))
 
+    def set_solve(self, mode: int) -> None:
+        """Form of the Hines elimination: SOLVE_AUTO, SOLVE_REFERENCE or SOLVE_FAST."""
+        self._check(self._L.ngpu_set_solve(self._h, int(mode)))
+
     def vhot_size(self) -> int:
         return int(self._L.ngpu_vhot_size(self._h))
 
@@ -287,7 +295,8 @@ class Simulator:
             pass
 
 
-def simulate_workload(w, device: int = 0, launch: Optional[dict] = None, fetch: bool = True):
+def simulate_workload(w, device: int = 0, launch: Optional[dict] = None, fetch: bool = True,
+                      solve: int = SOLVE_AUTO):
     """Run a :class:`models.Workload` on the GPU; returns (vhot [nsets,nstim,nrec,nt], stats)."""
     m = w.model
     with Simulator(device) as sim:
@@ -295,6 +304,7 @@ def simulate_workload(w, device: int = 0, launch: Optional[dict] = None, fetch: 
         sim.set_sim(m.v_init, m.rec_sites)
         sim.set_params(w.params, m.ncomps, m.nparams)
         sim.set_stimuli(w.amps, w.dts, m.stim_seg, m.stim_area)
+        sim.set_solve(solve)
         if launch:
             sim.set_launch(**launch)
         out, st = sim.run(fetch=fetch)

@@ -41,6 +41,12 @@ hipError_t ngpu_launch_batch_mainen(dim3 grid, size_t lds, hipStream_t s, const 
 
 static std::string g_last_error;
 
+/* Largest |e| or |f| of a tree that takes the division-free elimination by
+ * default.  Generated morphologies stay below 100; the reference's exported
+ * trees hold zero-area nodes (cm = 0) coupled with coefficients near 1e21,
+ * where only the reference's own operation order reproduces its results. */
+static const double kScaledCoef = 1e6;
+
 static int mech_nmodels(int ms) { return ms == NGPU_MECH_TTPC ? SetTTPC::NMECH : ms == NGPU_MECH_MAINEN ? SetMainen::NMECH : ms == NGPU_MECH_PASSIVE ? SetPassive::NMECH : -1; }
 static int mech_nparams(int ms) { return ms == NGPU_MECH_TTPC ? SetTTPC::NPARAM : ms == NGPU_MECH_MAINEN ? SetMainen::NPARAM : ms == NGPU_MECH_PASSIVE ? SetPassive::NPARAM : -1; }
 /* First parameter index of each mechanism of a set, plus the total (the
@@ -93,6 +99,9 @@ struct ngpu_ctx {
     /* launch configuration */
     int cfg_w = 0, cfg_r = 0, cfg_spl = 0, cfg_graph = 0;   /* waves and runs per group, 0 = auto */
     int cfg_stage = -1;            /* row coefficients: -1 auto, 0 global memory, 1 LDS */
+    int cfg_solve = NGPU_SOLVE_AUTO;
+    bool well_scaled = false;      /* every cm > 0 and |e|, |f| <= kScaledCoef: the division-free
+                                      elimination is as accurate as the reference's order */
     /* device state */
     int lay_W = 0, lay_R = 0, lay_NQ = 0;
     int ncu = 0;                   /* compute units of the device, once queried */
@@ -379,6 +388,10 @@ try {
     c->e.assign(t->e, t->e + N); c->f.assign(t->f, t->f + N); c->cm.assign(t->cm, t->cm + N);
     c->par = par; c->son = son; c->seg_to_comp.assign(t->seg_to_comp, t->seg_to_comp + N); c->mask = mask;
     c->chains = chains; c->lvl_ptr = lvl_ptr; c->fath = fath; c->flvl_ptr = flvl_ptr; c->fchild = fchild;
+    c->well_scaled = true;
+    for (int i = 0; i < N; i++)
+        if (!(t->cm[i] > 0) || !(std::fabs(t->e[i]) <= kScaledCoef) || !(std::fabs(t->f[i]) <= kScaledCoef))
+            c->well_scaled = false;
     c->have_tree = true;
     c->have_params = false;          /* parameter layout depends on the tree */
     c->lay_W = 0; c->tables_dirty = true;
@@ -386,6 +399,23 @@ try {
     return NGPU_OK;
 }
 NGPU_CATCH(c, NGPU_E_ARG)
+
+extern "C" int ngpu_set_solve(ngpu_ctx *c, int mode)
+{
+    if (!c) return fail(c, NGPU_E_ARG, "NULL context");
+    if (mode != NGPU_SOLVE_AUTO && mode != NGPU_SOLVE_REFERENCE && mode != NGPU_SOLVE_FAST)
+        return fail(c, NGPU_E_ARG, "solve mode %d is not NGPU_SOLVE_AUTO, _REFERENCE or _FAST", mode);
+    c->cfg_solve = mode;
+    drop_graph(c);
+    return NGPU_OK;
+}
+
+/* The elimination a run of c uses: the division-free form where chosen, or
+ * by default where the tree is well scaled. */
+static bool fast_solve(const ngpu_ctx *c)
+{
+    return c->cfg_solve == NGPU_SOLVE_FAST || (c->cfg_solve == NGPU_SOLVE_AUTO && c->well_scaled);
+}
 
 extern "C" int ngpu_set_sim(ngpu_ctx *c, double v_init, int nrec, const int *rec_sites)
 try {
@@ -641,6 +671,7 @@ static void fill_args(const ngpu_ctx *c, SimArgs &a, int W, int R, int staged, i
     a.msched = c->d_msched.p;
     a.chains = c->d_chains.p; a.lvl_ptr = c->d_lvl.p; a.fath = c->d_fath.p; a.flvl_ptr = c->d_flvl.p;
     a.fchild = c->d_fchild.p; a.lds_staging = staged; a.block_sync = block_sync;
+    a.fast_solve = fast_solve(c) ? 1 : 0;
     /* the tail of the level schedule: levels with one chain and no fathers
      * of their own, from the top down */
     a.tail_level = c->depth + 1;
@@ -846,6 +877,7 @@ static int run_range(ngpu_ctx *c, int t_begin, int t_end, double *vhot, ngpu_sta
         st->launches = launches;
         st->nvar = c->nvar;
         st->nsets = c->nsets; st->nstim = c->nstim; st->nrec = (int)c->rec.size(); st->nt = c->nt; st->N = c->N;
+        st->solve = fast_solve(c) ? NGPU_SOLVE_FAST : NGPU_SOLVE_REFERENCE;
     }
     (void)hipEventDestroy(h0);
     (void)hipEventDestroy(h1);

@@ -91,6 +91,8 @@ struct SimArgs {
     int nchains, nfath, nfchild;         /* schedule sizes (staged into LDS as int16) */
     int tail_level;                      /* levels tail_level..depth hold one chain each and no
                                             fathers of their own (depth + 1: none) */
+    int fast_solve;                      /* division-free elimination (well-scaled trees);
+                                            0: the reference's operation order */
     const double *dts, *amps;
     const int *rec_q;
     double *states;                      /* [nruns][NSTATE + 2][NQ]: mechanism states, v, dv */
@@ -500,7 +502,7 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
             const int ta = max(tA, 0), tb = max(tB, 0), j0 = min(i0, i1 - 1);
             double2 p0 = XU2[i0 - 1], pa = XU2[ta], ea = EF[ta], pb = XU2[tb], eb = EF[tb];
             const int j1 = min(i0 + 1, i1 - 1);
-            double2 ef = EF[i0 - 1], pn = XU2[j0], ef1 = EF[j1 - 1], pn1 = XU2[j1];
+            double2 ef = EF[i0 - 1], pn = XU2[j0], ef1 = EF[max(j1 - 1, 0)], pn1 = XU2[j1];
             /* all of them in flight together (not sunk into the branches) */
             __asm__ volatile("" : "+v"(p0.x), "+v"(p0.y), "+v"(pa.x), "+v"(pa.y), "+v"(ea.x), "+v"(ea.y),
                              "+v"(pb.x), "+v"(pb.y), "+v"(eb.x), "+v"(eb.y));
@@ -520,6 +522,80 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                 ef = ef1; pn = pn1; ef1 = ef2; pn1 = pn2;
             }
         };
+        /* The division-free form of the same elimination (fast_solve, for
+         * trees whose coefficients are well scaled): the chain keeps its
+         * running pivot as a fraction u = p / q and its right-hand side as
+         * b = B / q, so that the next row is
+         *     p' = d p - e f q,   B' = rhs p - e B,   q' = p
+         * (one dependent FMA per row instead of two divisions), and the
+         * finished row's pivot quotients PX = b / u = B / p and
+         * PF = -f / u = -f q / p are computed off that dependency and
+         * stored in place of (b, u).  Related rows are folded through their
+         * quotients: u_J -= f e / u_t is u_J += e PF_t, and b_J -= e b_t / u_t
+         * is b_J -= e PX_t.  (p, q, B) are scaled by a power of two every 4
+         * rows (exact), so that p stays in range on long chains.  Row N of
+         * a run's region is not a row of the tree; it takes the stores of
+         * the rows a block computes past the end of its chain.  The
+         * results equal the reference order's up to rounding. */
+        auto elim_fast = [&](const NgChain &c) {
+            double2 *XU2 = (double2 *)region(c.r);
+            const int i0 = c.i0, last = c.i1 - 1, tA = c.tA, tB = c.tB;
+            const int ta = max(tA, 0), tb = max(tB, 0);
+            double2 p0 = XU2[i0 - 1], pa = XU2[ta], ea = EF[ta], pb = XU2[tb], eb = EF[tb];
+            const double fh = EF[i0 - 1].y;
+            __asm__ volatile("" : "+v"(p0.x), "+v"(p0.y), "+v"(pa.x), "+v"(pa.y), "+v"(ea.x), "+v"(ea.y),
+                             "+v"(pb.x), "+v"(pb.y), "+v"(eb.x), "+v"(eb.y));
+            double p = p0.y, B = p0.x, q = 1.0;
+            if (tA >= 0) {
+                p = fma(ea.x, pa.y, p);
+                B = fma(-ea.x, pa.x, B);
+                if (tB >= 0) {
+                    p = fma(eb.x, pb.y, p);
+                    B = fma(-eb.x, pb.x, B);
+                }
+            }
+            {
+                const double r = ng_rcp(p);
+                XU2[i0 - 1] = make_double2(B * r, -(fh * r));
+            }
+            /* Rows i .. i+3 per block, without branches: first the four
+             * recurrences (one dependent multiply-add each), then the four
+             * rows' quotients, which are independent of one another and of
+             * the recurrence, so that their reciprocal chains overlap
+             * instead of each one holding up the next row.  Row j reads the
+             * (e, f) of row j - 1, its own (rhs, d) and its own f; indices
+             * past the chain's last row are clamped to it, and those rows'
+             * results go to the unused row N of the run's region. */
+            for (int i = i0; i <= last; i += 4) {
+                double2 ef[4], xr[4];
+                double fr[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const int j = min(i + k, last);
+                    ef[k] = EF[j - 1];
+                    xr[k] = XU2[j];
+                    fr[k] = EF[j].y;
+                }
+                double P[4], Q[4], Bv[4];
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const double fq = ef[k].y * q;
+                    const double pn = fma(xr[k].y, p, -(ef[k].x * fq));
+                    const double bn = fma(-ef[k].x, B, xr[k].x * p);
+                    q = p; p = pn; B = bn;
+                    P[k] = p; Q[k] = q; Bv[k] = B;
+                }
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const double r = ng_rcp(P[k]);
+                    XU2[i + k <= last ? i + k : N] = make_double2(Bv[k] * r, -((fr[k] * Q[k]) * r));
+                }
+                int e2;
+                (void)frexp(p, &e2);
+                p = ldexp(p, -e2); q = ldexp(q, -e2); B = ldexp(B, -e2);
+            }
+        };
+        const bool fast = a.fast_solve != 0;
         int k0 = LP[0], nc = LP[1] - k0;
         int k0n = LP[min(1, D)], ncn = LP[min(1, D) + 1] - k0n;
         NgChain first = ng_chain(CH, min(g, R * nc - 1), k0, nc);
@@ -537,7 +613,13 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                     for (int cc = c0; cc < c1; cc++) {
                         const int tr = FC[cc];
                         const double2 pt = XU2[tr], et = EF[tr];
-                        ng_fold(pt.y, pt.x, et.x, et.y, pJ.y, pJ.x);
+                        if (fast) {
+                            /* (b, u) of J from the quotients (PX, PF) of row tr */
+                            pJ.y = fma(et.x, pt.y, pJ.y);
+                            pJ.x = fma(-et.x, pt.x, pJ.x);
+                        } else {
+                            ng_fold(pt.y, pt.x, et.x, et.y, pJ.y, pJ.x);
+                        }
                     }
                     XU2[J] = pJ;
                 }
@@ -546,7 +628,8 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
             for (int tk = g; tk < R * nc; tk += LW) {
                 NgChain c = first;
                 if (tk != g) c = ng_chain(CH, tk, k0, nc);
-                elim(c);
+                if (fast) elim_fast(c);
+                else elim(c);
             }
             sync();
             first = next; k0 = k0n; nc = ncn; k0n = k0nn; ncn = ncnn;
@@ -556,21 +639,27 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
          * tail levels in level order without a barrier between them */
         if (Lt <= D) {
             if (g < R)
-                for (int L = Lt; L <= D; L++) elim(ng_chain(CH, g, LP[L], 1));
+                for (int L = Lt; L <= D; L++) {
+                    if (fast) elim_fast(ng_chain(CH, g, LP[L], 1));
+                    else elim(ng_chain(CH, g, LP[L], 1));
+                }
             sync();
         }
 
         /* pivot quotients of every row, in parallel: PX = b/u and PF = -f/u
-         * (BKSUB1's first step, CudaStuff.cu:75-78) replace b and u */
-        NgPos rw(g, LW, N);
-        for (int i = g; i < R * N; i += LW, rw.step()) {
-            const int r = rw.r, h = rw.q;
-            double2 *XU2 = (double2 *)region(r);
-            const double2 ph = XU2[h];
-            const double rc = ng_rcp(ph.y);
-            XU2[h] = make_double2(ph.x * rc, -(EF[h].y * rc));
+         * (BKSUB1's first step, CudaStuff.cu:75-78) replace b and u; the
+         * division-free elimination has stored them already */
+        if (!fast) {
+            NgPos rw(g, LW, N);
+            for (int i = g; i < R * N; i += LW, rw.step()) {
+                const int r = rw.r, h = rw.q;
+                double2 *XU2 = (double2 *)region(r);
+                const double2 ph = XU2[h];
+                const double rc = ng_rcp(ph.y);
+                XU2[h] = make_double2(ph.x * rc, -(EF[h].y * rc));
+            }
+            sync();
         }
-        sync();
 
         /* back-substitution by chains, highest level first: a chain's last
          * row takes x of its parent row pe (a row of a higher level; pe = N
@@ -584,26 +673,29 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
             const double2 ph = XU2[h];
             const double xp = XU[2 * min(pe, N - 1)];
             double2 a0 = XU2[max(h - 1, lo)], a1 = XU2[max(h - 2, lo)], a2 = XU2[max(h - 3, lo)],
-                    a3 = XU2[max(h - 4, lo)];
+                    a3 = XU2[max(h - 4, lo)], b0, b1, b2, b3;
             double x = ph.x;
-            if (pe < N) x = x + ph.y * xp;
+            if (pe < N) x = fma(ph.y, xp, x);
             XU[2 * h] = x;
+            /* rows i .. i-3 from one register set while rows i-4 .. i-7 are
+             * loaded into the other (clamped to the chain; a clamped value
+             * is not used) */
             int i = h - 1;
-            while (i - 3 >= lo) {
-                /* rows i .. i-3 from a0..a3; rows i-4 .. i-7 loaded (clamped
-                 * to the chain; a clamped value is not used) */
-                const double2 b0 = XU2[max(i - 4, lo)], b1 = XU2[max(i - 5, lo)], b2 = XU2[max(i - 6, lo)],
-                              b3 = XU2[max(i - 7, lo)];
-                x = a0.x + a0.y * x; XU[2 * i] = x;
-                x = a1.x + a1.y * x; XU[2 * (i - 1)] = x;
-                x = a2.x + a2.y * x; XU[2 * (i - 2)] = x;
-                x = a3.x + a3.y * x; XU[2 * (i - 3)] = x;
-                a0 = b0; a1 = b1; a2 = b2; a3 = b3;
+            while (i >= lo) {
+                b0 = XU2[max(i - 4, lo)]; b1 = XU2[max(i - 5, lo)]; b2 = XU2[max(i - 6, lo)]; b3 = XU2[max(i - 7, lo)];
+                x = fma(a0.y, x, a0.x); XU[2 * i] = x;
+                if (i - 1 >= lo) { x = fma(a1.y, x, a1.x); XU[2 * (i - 1)] = x; }
+                if (i - 2 >= lo) { x = fma(a2.y, x, a2.x); XU[2 * (i - 2)] = x; }
+                if (i - 3 >= lo) { x = fma(a3.y, x, a3.x); XU[2 * (i - 3)] = x; }
+                i -= 4;
+                if (i < lo) break;
+                a0 = XU2[max(i - 4, lo)]; a1 = XU2[max(i - 5, lo)]; a2 = XU2[max(i - 6, lo)]; a3 = XU2[max(i - 7, lo)];
+                x = fma(b0.y, x, b0.x); XU[2 * i] = x;
+                if (i - 1 >= lo) { x = fma(b1.y, x, b1.x); XU[2 * (i - 1)] = x; }
+                if (i - 2 >= lo) { x = fma(b2.y, x, b2.x); XU[2 * (i - 2)] = x; }
+                if (i - 3 >= lo) { x = fma(b3.y, x, b3.x); XU[2 * (i - 3)] = x; }
                 i -= 4;
             }
-            if (i >= lo) { x = a0.x + a0.y * x; XU[2 * i] = x; }
-            if (i - 1 >= lo) { x = a1.x + a1.y * x; XU[2 * (i - 1)] = x; }
-            if (i - 2 >= lo) { x = a2.x + a2.y * x; XU[2 * (i - 2)] = x; }
         };
         /* the tail first (top of the tree), lane r for run r, then the
          * levels below it */

@@ -11,7 +11,7 @@ import pytest
 
 from conftest import TOL_MV, TTPC_DIR, spike_counts
 from neurogpu_amd import formats, models
-from neurogpu_amd._lib import NgpuError, Simulator, simulate_workload
+from neurogpu_amd._lib import SOLVE_FAST, SOLVE_REFERENCE, NgpuError, Simulator, simulate_workload
 from oracle import cpu_ref
 
 pytestmark = pytest.mark.gpu
@@ -62,7 +62,26 @@ def test_ttpc1200_small(launch):
     w = models.workload("ttpc1200", nsets=3, nt=1200)
     _say(f"case ttpc1200 small launch={launch}")
     gpu, st = simulate_workload(w, launch=launch)
+    assert st["solve"] == SOLVE_FAST                 # a generated tree: the division-free form by default
     _check(gpu, _oracle(w), f"ttpc1200 {launch}")
+
+
+@pytest.mark.parametrize("name", ["active500", "ttpc1200", "passive64"])
+def test_both_eliminations_agree(name):
+    """The reference's operation order and the division-free form on the
+    same generated tree: both within the bar of the oracle, and within
+    rounding of each other."""
+    w = models.workload(name, nsets=3, nt=1500)
+    _say(f"case {name}: reference-order vs division-free elimination")
+    ref_order, st1 = simulate_workload(w, solve=SOLVE_REFERENCE)
+    fast, st2 = simulate_workload(w, solve=SOLVE_FAST)
+    assert st1["solve"] == SOLVE_REFERENCE and st2["solve"] == SOLVE_FAST
+    oracle = _oracle(w)
+    _check(ref_order, oracle, f"{name} reference order")
+    _check(fast, oracle, f"{name} division-free")
+    d = float(np.max(np.abs(fast - ref_order)))
+    _say(f"{name}: max|fast - reference order| = {d:.3e} mV")
+    assert d <= 1e-8
 
 
 def test_ttpc1200_full_protocol():
@@ -119,7 +138,8 @@ def test_reference_ttpc_fixture_through_file_boundary(tmp_path):
     _say("case ttpc512 fixture via ngpu_open")
     sim = Simulator.open(TTPC_DIR)
     out = tmp_path / "VHotP0.dat"
-    sim.run(fetch=False)
+    _, st = sim.run(fetch=False)
+    assert st["solve"] == SOLVE_REFERENCE            # zero-area nodes: the reference's order
     sim.write_vhot(str(out))
     sim.close()
     t = formats.read_tree(os.path.join(TTPC_DIR, "BasicConstSegP.csv"))
