@@ -74,6 +74,20 @@ int ngpu_set_sim(ngpu_ctx *ctx, double v_init, int nrec, const int *rec_sites);
  * (replaces ReadAllParams, CANON/Util.cu:210-234). */
 int ngpu_set_params(ngpu_ctx *ctx, const double *allparams, int nsets, int ncomps, int nparams);
 
+/* Parameters assembled on the device, for callers that change only a few
+ * free parameters per set (a GA).  tpl is one parameter-matrix row
+ * [nparams*ncomps]; free parameter i sits at every entry that holds the
+ * sentinel 12300 + i (the convention of the GA evaluators'
+ * create_params_to_all_params / my_assemble_allparams,
+ * Figures/Figure4_DEAP/extractModel_mappings.py:158-171, :258-270); every
+ * other entry is the value all sets share.  Each free parameter needs at
+ * least one entry.  Then ngpu_set_genomes uploads the free parameters of
+ * nsets sets, x [nsets][nfree], and a device kernel scatters them into the
+ * per-set parameter table (the host never builds the full matrix).  Equal to
+ * ngpu_set_params with the template filled in on the host, bit for bit. */
+int ngpu_set_param_template(ngpu_ctx *ctx, const double *tpl, int ncomps, int nparams, int nfree);
+int ngpu_set_genomes(ngpu_ctx *ctx, const double *x, int nsets);
+
 /* Stimuli: amps [nstim][nt_amps] in nA (nt_amps >= nt), dts [nt] in ms,
  * stim_seg is 0-based (Stim_meta loc - 1), area in um^2
  * (replaces ReadCSVStim, CANON/Util.cu:337-390). */

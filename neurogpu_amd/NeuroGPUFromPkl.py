@@ -16,13 +16,12 @@ import os
 import numpy as np
 
 from . import formats
-
-SENTINEL = 12300.0
+from .models import SENTINEL
 
 
 def save_model_description(model, data_dir: str, path: str) -> None:
     """Template row with sentinels at each free parameter's positions."""
-    tpl = model.allparams(np.array([[SENTINEL + i for i in range(len(model.free))]]))[0]
+    tpl = model.param_template()
     with open(path, "w") as fh:
         json.dump({"template": tpl.tolist(), "ncomps": model.ncomps, "nparams": model.nparams,
                    "free": [fp.name for fp in model.free], "data_dir": data_dir}, fh)

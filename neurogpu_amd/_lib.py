@@ -27,7 +27,8 @@ EXPORTS = ("ngpu_create", "ngpu_set_tree", "ngpu_set_sim", "ngpu_set_params", "n
            "ngpu_mlp_predict", "ngpu_mlp_gradients", "ngpu_mlp_last_error", "ngpu_mlp_close", "ngpu_gemm_bf16",
            "ngpu_batch_create", "ngpu_batch_add", "ngpu_batch_set_launch", "ngpu_batch_run",
            "ngpu_batch_member_info", "ngpu_batch_last_error", "ngpu_batch_close", "ngpu_run_steps",
-           "ngpu_get_sizes", "ngpu_features_noalloc", "ngpu_set_solve")
+           "ngpu_get_sizes", "ngpu_features_noalloc", "ngpu_set_solve",
+           "ngpu_set_param_template", "ngpu_set_genomes")
 NFEAT = 5
 FEATURES = ("spike_count", "first_latency", "ap_amplitude", "ap_width", "mean_isi")
 
@@ -109,6 +110,8 @@ def lib():
         L.ngpu_set_stimuli.argtypes = [vp, _DP, C.c_int, C.c_int, _DP, C.c_int, C.c_int, C.c_double]
         L.ngpu_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.ngpu_set_solve.argtypes = [vp, C.c_int]
+        L.ngpu_set_param_template.argtypes = [vp, _DP, C.c_int, C.c_int, C.c_int]
+        L.ngpu_set_genomes.argtypes = [vp, _DP, C.c_int]
         L.ngpu_run.argtypes = [vp, _DP, C.POINTER(ngpu_stats)]
         L.ngpu_run_steps.argtypes = [vp, C.c_int, C.c_int, C.POINTER(ngpu_stats)]
         L.ngpu_get_sizes.argtypes = [vp] + [_IP] * 5
@@ -140,7 +143,8 @@ def lib():
             getattr(L, name).restype = C.c_int
         for name in ("ngpu_create", "ngpu_set_tree", "ngpu_set_sim", "ngpu_set_params", "ngpu_set_stimuli",
                      "ngpu_set_launch", "ngpu_run", "ngpu_fetch_vhot", "ngpu_open", "ngpu_write_vhot",
-                     "ngpu_mechset_nparams", "ngpu_run_steps", "ngpu_get_sizes", "ngpu_set_solve"):
+                     "ngpu_mechset_nparams", "ngpu_run_steps", "ngpu_get_sizes", "ngpu_set_solve",
+                     "ngpu_set_param_template", "ngpu_set_genomes"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -219,6 +223,20 @@ class Simulator:
         P = np.ascontiguousarray(np.atleast_2d(params), dtype=np.float64)
         self._check(self._L.ngpu_set_params(self._h, P.ctypes.data_as(_DP), P.shape[0], int(ncomps), int(nparams)))
         self.nsets = P.shape[0]
+
+    def set_param_template(self, template: np.ndarray, ncomps: int, nparams: int, nfree: int) -> None:
+        """Template parameter row with free parameter i at the entries holding
+        12300 + i (ngpu_set_param_template)."""
+        T = np.ascontiguousarray(np.ravel(template), dtype=np.float64)
+        self._check(self._L.ngpu_set_param_template(self._h, T.ctypes.data_as(_DP), int(ncomps), int(nparams),
+                                                    int(nfree)))
+
+    def set_genomes(self, x: np.ndarray) -> None:
+        """Free parameters of each set [nsets, nfree]; assembled into the
+        parameter table on the device (ngpu_set_genomes)."""
+        X = np.ascontiguousarray(np.atleast_2d(x), dtype=np.float64)
+        self._check(self._L.ngpu_set_genomes(self._h, X.ctypes.data_as(_DP), X.shape[0]))
+        self.nsets = X.shape[0]
 
     def set_stimuli(self, amps: np.ndarray, dts: np.ndarray, stim_seg: int, area: float) -> None:
         A = np.ascontiguousarray(np.atleast_2d(amps), dtype=np.float64)

@@ -34,6 +34,8 @@ hipError_t ngpu_launch_ttpc(bool multi, dim3 grid, dim3 block, size_t lds, hipSt
 hipError_t ngpu_launch_features(const double *vhot, int nrec, int nt, int nruns, int nstim, const double *dts,
                                 const double *t_on, const double *t_off, double thr, double *out, hipStream_t s);
 hipError_t ngpu_features_scratch(int device, double **ptr, size_t *cap);
+hipError_t ngpu_launch_scatter_genomes(const double *genome, int nsets, int nfree, const int *src, int nvar, double *pv,
+                                       hipStream_t stream);
 hipError_t ngpu_launch_mainen(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
 hipError_t ngpu_launch_batch_passive(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
 hipError_t ngpu_launch_batch_ttpc(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
@@ -91,6 +93,8 @@ struct ngpu_ctx {
     int nsets = 0, nparams = 0, nvar = 0;
     std::vector<int> pref;        /* [nparams*N]: >=0 varying column, else -(uniform index)-1 */
     std::vector<double> puni, pv;
+    int nfree = 0;                 /* > 0: parameters from a template and genome rows */
+    bool pv_on_device = false;     /* d_pv written on the device (ngpu_set_genomes) */
     /* stimuli */
     bool have_stim = false;
     int nstim = 0, nt_amps = 0, nt = 0, stim_seg = -1;
@@ -106,11 +110,11 @@ struct ngpu_ctx {
     int lay_W = 0, lay_R = 0, lay_NQ = 0;
     int ncu = 0;                   /* compute units of the device, once queried */
     bool tables_dirty = true, params_dirty = true, stim_dirty = true, graph_dirty = true;
-    DevArr<int> d_qlink, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_rec, d_msched;
+    DevArr<int> d_qlink, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_rec, d_msched, d_gsrc;
     std::vector<int> qmask;        /* mechanism mask by position */
     std::vector<int> msched;       /* mechanism-pass schedule of the last (W, R) */
     int msched_W = 0, msched_R = 0;
-    DevArr<double> d_qcm, d_ef, d_pval, d_pconst, d_pv, d_dts, d_amps, d_states, d_vhot, d_fwin, d_feat;
+    DevArr<double> d_qcm, d_ef, d_pval, d_pconst, d_pv, d_dts, d_amps, d_states, d_vhot, d_fwin, d_feat, d_genome;
     int stim_q = -1;
     std::vector<int> rec_q;
     unsigned pconst_mask = 0;
@@ -224,8 +228,8 @@ extern "C" void ngpu_close(ngpu_ctx *c)
         (void)hipStreamSynchronize(c->stream);
     }
     drop_graph(c);
-    for (auto *d : {&c->d_qlink, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_rec, &c->d_msched}) d->release();
-    for (auto *d : {&c->d_qcm, &c->d_ef, &c->d_pval, &c->d_pconst, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_vhot, &c->d_fwin, &c->d_feat}) d->release();
+    for (auto *d : {&c->d_qlink, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_rec, &c->d_msched, &c->d_gsrc}) d->release();
+    for (auto *d : {&c->d_qcm, &c->d_ef, &c->d_pval, &c->d_pconst, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_vhot, &c->d_fwin, &c->d_feat, &c->d_genome}) d->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -479,10 +483,81 @@ try {
     c->puni = puni;
     c->nsets = nsets;
     c->nparams = nparams;
+    c->nfree = 0;
+    c->pv_on_device = false;
     c->have_params = true;
     c->params_dirty = true;
     c->tables_dirty = true;
     drop_graph(c);
+    return NGPU_OK;
+}
+NGPU_CATCH(c, NGPU_E_ARG)
+
+/* Sentinel of free parameter i in a template row (extractModel_mappings.py:258-270). */
+static const double kSentinel = 12300.0;
+
+extern "C" int ngpu_set_param_template(ngpu_ctx *c, const double *tpl, int ncomps, int nparams, int nfree)
+try {
+    if (!c) return fail(c, NGPU_E_ARG, "NULL context");
+    if (!c->have_tree) return fail(c, NGPU_E_STATE, "ngpu_set_param_template before ngpu_set_tree");
+    if (!tpl) return fail(c, NGPU_E_ARG, "ngpu_set_param_template: NULL template");
+    if (ncomps != c->ncomps) return fail(c, NGPU_E_ARG, "ncomps=%d but the tree has NComps=%d", ncomps, c->ncomps);
+    if (nparams != mech_nparams(c->mechset)) return fail(c, NGPU_E_ARG, "nparams=%d but mechanism set %d has %d", nparams, c->mechset, mech_nparams(c->mechset));
+    if (nfree < 1 || nfree > 4096) return fail(c, NGPU_E_ARG, "nfree=%d not in 1..4096", nfree);
+    const int N = c->N;
+    const size_t row = (size_t)nparams * ncomps;
+    std::vector<int> colref(row), used(nfree, 0);
+    std::vector<double> puni;
+    for (size_t k = 0; k < row; k++) {
+        const double x = tpl[k] - kSentinel;
+        const int i = (int)x;
+        if (x >= 0 && x < nfree && x == (double)i) {
+            colref[k] = i;
+            used[i] = 1;
+        } else {
+            colref[k] = -(int)puni.size() - 1;
+            puni.push_back(tpl[k]);
+        }
+    }
+    for (int i = 0; i < nfree; i++)
+        if (!used[i]) return fail(c, NGPU_E_ARG, "free parameter %d has no position (no template entry holds %g)", i, kSentinel + i);
+    c->pref.assign((size_t)nparams * N, 0);
+    for (int j = 0; j < nparams; j++)
+        for (int s = 0; s < N; s++) c->pref[(size_t)j * N + s] = colref[(size_t)j * ncomps + c->seg_to_comp[s]];
+    c->puni = puni;
+    c->nparams = nparams;
+    c->nvar = nfree;
+    c->nfree = nfree;
+    c->have_params = false;        /* until genome rows are given */
+    c->pv_on_device = false;
+    c->tables_dirty = true;
+    drop_graph(c);
+    return NGPU_OK;
+}
+NGPU_CATCH(c, NGPU_E_ARG)
+
+extern "C" int ngpu_set_genomes(ngpu_ctx *c, const double *x, int nsets)
+try {
+    if (!c) return fail(c, NGPU_E_ARG, "NULL context");
+    if (c->nfree < 1) return fail(c, NGPU_E_STATE, "ngpu_set_genomes needs ngpu_set_param_template first");
+    if (!x || nsets < 1) return fail(c, NGPU_E_ARG, "ngpu_set_genomes: NULL rows or nsets=%d", nsets);
+    HIPCHK(c, hipSetDevice(c->device));
+    std::vector<int> src(c->nvar);
+    for (int k = 0; k < c->nvar; k++) src[k] = k;            /* varying column k is free parameter k */
+    int rc;
+    if ((rc = upload(c, c->d_gsrc, src)) || (rc = alloc(c, c->d_genome, (size_t)nsets * c->nfree)) ||
+        (rc = alloc(c, c->d_pv, (size_t)nsets * c->nvar)))
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(c->d_genome.p, x, (size_t)nsets * c->nfree * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    hipError_t e = ngpu_launch_scatter_genomes(c->d_genome.p, nsets, c->nfree, c->d_gsrc.p, c->nvar, c->d_pv.p, c->stream);
+    if (e != hipSuccess) return fail(c, NGPU_E_HIP, "genome scatter launch failed: %s", hipGetErrorString(e));
+    if (nsets != c->nsets) drop_graph(c);
+    c->nsets = nsets;
+    c->pv_on_device = true;
+    c->have_params = true;
+    c->params_dirty = false;
+    c->ran = false;
+    c->t_done = 0;
     return NGPU_OK;
 }
 NGPU_CATCH(c, NGPU_E_ARG)
@@ -792,7 +867,7 @@ static int prepare_layout(ngpu_ctx *c, int W, int R)
     if (rc) return rc;
     if (W != c->lay_W || R != c->lay_R) drop_graph(c);
     c->lay_W = W; c->lay_R = R;
-    if (c->params_dirty) {
+    if (c->params_dirty && !c->pv_on_device) {
         if ((rc = upload(c, c->d_pv, c->pv))) return rc;
         c->params_dirty = false;
     }

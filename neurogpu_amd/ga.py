@@ -67,6 +67,9 @@ class Evaluator:
         self.sim.set_tree(self.m.tree, self.m.mechset_id)
         self.sim.set_sim(self.m.v_init, self.m.rec_sites)
         self.sim.set_stimuli(workload.amps, workload.dts, self.m.stim_seg, self.m.stim_area)
+        # the parameter table is assembled on the device from each
+        # generation's free parameters (no host-side parameter matrix)
+        self.sim.set_param_template(self.m.param_template(), self.m.ncomps, self.m.nparams, len(self.m.free))
         self.nevals = 0
         self.sim_seconds = 0.0
         self.target = None
@@ -78,7 +81,7 @@ class Evaluator:
         """Filled features [n, nstim, NFEAT] of free-parameter rows x."""
         x = np.atleast_2d(x)
         t0 = time.perf_counter()
-        self.sim.set_params(self.m.allparams(x), self.m.ncomps, self.m.nparams)
+        self.sim.set_genomes(x)
         self.sim.run(fetch=False)
         f = self.sim.features(self.t_on, self.t_off)
         self.sim_seconds += time.perf_counter() - t0

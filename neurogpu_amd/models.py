@@ -41,6 +41,10 @@ MECHSETS: Dict[str, dict] = {
 
 REGIONS = ("soma", "axon", "basal", "apical")
 
+# Entry value marking free parameter i in a template parameter row
+# (create_params_to_all_params, Figures/Figure4_DEAP/extractModel_mappings.py:258-270).
+SENTINEL = 12300.0
+
 
 # ----------------------------------------------------------------- morphology
 @dataclasses.dataclass
@@ -268,6 +272,11 @@ class Model:
                     if REGIONS[sreg[c]] in fp.regions]
             out[:, cols] = x[:, j:j + 1]
         return out
+
+    def param_template(self) -> np.ndarray:
+        """One parameter row in which free parameter i sits at every entry it
+        sets as the sentinel 12300 + i (the GA evaluators' template)."""
+        return self.allparams(np.array([[SENTINEL + i for i in range(len(self.free))]]))[0]
 
     def nominal(self) -> np.ndarray:
         return np.array([fp.nominal for fp in self.free])

@@ -12,7 +12,7 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "libcpuref.so")
+LIB_PATH = os.environ.get("NGPU_CPUREF_LIB") or os.path.join(_HERE, "_build", "libcpuref.so")
 
 MS_PASSIVE, MS_TTPC, MS_MAINEN = 0, 1, 2
 _IP = C.POINTER(C.c_int)

@@ -36,8 +36,10 @@ def test_gradients_match_cpu_restatement():
         gd = ens.unpack(np.concatenate([np.zeros(mem * ens.nparam, np.float32), g]), mem)
         errs = [float(np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-30)) for a, b in zip(gd, ref)]
         print(f"member {mem}: relative gradient errors {['%.2e' % e for e in errs]}", flush=True)
+        # FP32 accumulation against FP64 at the same BF16 rounding points:
+        # measured at most 1e-4 relative
         for k, err in enumerate(errs):
-            assert err < 2e-2, (mem, k, err)
+            assert err < 1e-3, (mem, k, err)
     ens.close()
 
 
