@@ -58,6 +58,7 @@ class Morphology:
     diam: np.ndarray
     region: np.ndarray
     section: np.ndarray
+    ext: Optional[np.ndarray] = None      # extension compartments of a dispersed branching
 
     @property
     def N(self) -> int:
@@ -77,6 +78,33 @@ class Morphology:
             p = self.parent[s]
             d[s] = d[p] + 0.5 * self.length[p] + 0.5 * self.length[s]
         return d
+
+
+EXT_LENGTH = 1e-9      # um, the extension section of runModel.hoc:168
+EXT_DIAM = 1.0         # um, runModel.hoc:169
+
+
+def disperse(m: Morphology) -> Morphology:
+    """Branch points with more than two children (the root: more than one)
+    get a chain of extension compartments that carries the children at most
+    two per compartment (:func:`tree.disperse_branching`), each extension a
+    cylinder of 1e-9 um x 1 um (runModel.hoc:168-169) without mechanisms, in
+    the host's region and a section of its own.  The cable coefficients are
+    then computed from the geometry as for any compartment, so the axial
+    path from a host to each child is still half the host plus half the
+    child.  A morphology that needs no dispersal is returned unchanged."""
+    newpar, src = treemod.disperse_branching(m.parent)
+    if np.array_equal(src, np.arange(m.N)):
+        return m
+    orig = src >= 0
+    host = np.where(orig, src, -src - 1)
+    nsec = m.nsections
+    sec = np.where(orig, m.section[host], 0)
+    hosts = np.unique(host[~orig])
+    for k, h in enumerate(hosts):
+        sec[(~orig) & (host == h)] = nsec + k
+    return Morphology(newpar, np.where(orig, m.length[host], EXT_LENGTH), np.where(orig, m.diam[host], EXT_DIAM),
+                      m.region[host].copy(), sec.astype(np.int64), ~orig)
 
 
 def cable_coefficients(m: Morphology, Ra) -> Tuple[np.ndarray, np.ndarray]:
@@ -217,6 +245,28 @@ def pyramidal(seed: int, n_target: int, n_basal: int = 4, seg_len: float = 20.0,
     return b.build()
 
 
+def branchy(seed: int = 1004) -> Morphology:
+    """A cell whose branch points have more than two children: a one-compartment
+    soma carrying an axon and five dendrites (six children), and a dendrite
+    that splits three ways at its end, twice.  Such trees are dispersed
+    before scheduling (:func:`disperse`)."""
+    rng = np.random.default_rng(seed)
+    b = _Builder()
+    soma = b.add(-1, 0, 20.0, 16.0, 16.0, 1)
+    b.add(soma, 1, 400.0, 1.2, 1.0, 10)
+    for i in range(5):
+        L = rng.uniform(80, 160)
+        k = b.add(soma, 2 + i % 2, L, 1.8, 1.2, max(1, int(round(L / 20))))
+        if i in (1, 3):
+            for _ in range(3):
+                L2 = rng.uniform(40, 100)
+                k2 = b.add(k, 2 + i % 2, L2, 1.0, 0.7, max(1, int(round(L2 / 20))))
+                if i == 3:
+                    for _ in range(3):
+                        b.add(k2, 3, 30.0, 0.6, 0.5, 2)
+    return b.build()
+
+
 # ----------------------------------------------------------------- models
 @dataclasses.dataclass
 class FreeParam:
@@ -305,12 +355,14 @@ class Model:
 def _assemble(name, mechset, morph, placement, base_by_region, free, v_init, Ra, cm_by_region) -> Model:
     mechs = MECHSETS[mechset]["mechs"]
     params = MECHSETS[mechset]["params"]
+    morph = disperse(morph)
     N = morph.N
     regnames = [REGIONS[r] for r in morph.region]
+    ext = morph.ext if morph.ext is not None else np.zeros(N, bool)
     bool_model = np.zeros((len(mechs), N), np.int64)
     for m, mech in enumerate(mechs):
         for s in range(N):
-            if mech in placement[regnames[s]]:
+            if mech in placement[regnames[s]] and not ext[s]:
                 bool_model[m, s] = 1
     e, f = cable_coefficients(morph, Ra)
     cm = np.array([cm_by_region[r] for r in regnames])
@@ -374,6 +426,24 @@ def active_model(seed: int = 1001, n_target: int = 500) -> Model:
         FreeParam("g_pas", "g_pas", REGIONS, nom["gpas"], nom["gpas"] / 10, nom["gpas"] * 10),
     ]
     return _assemble(f"active{m.N}", "mainen", m, placement, base, free, -70.0, 150.0,
+                     {r: 0.75 for r in REGIONS})
+
+
+def branchy_active_model(seed: int = 1004) -> Model:
+    """:func:`active_model`'s channels and free conductances on the
+    :func:`branchy` morphology (branch points of up to six children)."""
+    ref = active_model()
+    m = branchy(seed)
+    dend = ("na", "km", "kca", "ca", "cad", "pas")
+    placement = {"soma": ("na", "kv", "km", "kca", "ca", "cad", "pas"), "axon": ("na", "kv", "pas"),
+                 "basal": dend, "apical": dend}
+    base = {}
+    sreg = ref.section_regions()
+    B = ref.base.reshape(ref.nparams, ref.ncomps)
+    for r, rn in enumerate(REGIONS):
+        c = int(np.nonzero(sreg == r)[0][0])
+        base[rn] = {pn: float(B[p, c]) for p, pn in enumerate(MECHSETS["mainen"]["params"])}
+    return _assemble(f"branchy{m.N}", "mainen", m, placement, base, ref.free, -70.0, 150.0,
                      {r: 0.75 for r in REGIONS})
 
 

@@ -164,6 +164,59 @@ def schedule(ks: np.ndarray) -> Dict[str, np.ndarray]:
     )
 
 
+def disperse_branching(par):
+    """Give every node with more than two children (the root: more than one,
+    since the root counts as its own first child) a chain of extension nodes
+    that carries its children at most two per node, as the reference's
+    model export does (``hinesDisperseBranching``, NeuroGPU_Base/runModel.hoc:158-215:
+    an extension section of nchild - 1 segments at the node's end, child i
+    reconnected to segment min(i, nchild - 2)), so that the solver's son
+    numbering holds (create_auxilliary_data_3.py:72-84).
+
+    Returns ``(new_par, src)`` in a new depth-first Hines order: ``src[i]``
+    is the original segment of new node i, or ``-(h + 1)`` for an extension
+    node of host segment h.  Trees that need no dispersal come back with the
+    same order (``src = arange(N)``)."""
+    par = np.asarray(par, dtype=np.int64)
+    N = par.size
+    if N < 1 or par[0] != -1 or np.any(par[1:] < 0) or np.any(par[1:] >= np.arange(1, N)):
+        raise TreeError("parent vector must satisfy 0 <= parent(s) < s (Hines order) with the root first")
+    kids = [[] for _ in range(N)]
+    for c in range(1, N):
+        kids[int(par[c])].append(c)
+    # node ids: 0..N-1 original, N.. extension nodes; children lists in order
+    ch = {i: list(kids[i]) for i in range(N)}
+    host = {}
+    nxt = N
+    for p in range(N):
+        k = len(kids[p])
+        limit = 1 if p == 0 else 2
+        if k <= limit:
+            continue
+        xs = list(range(nxt, nxt + k - 1))
+        nxt += k - 1
+        for x in xs:
+            host[x] = p
+        ch[p] = [xs[0]]
+        for j, x in enumerate(xs):
+            if j < len(xs) - 1:
+                ch[x] = [kids[p][j], xs[j + 1]]
+            else:
+                ch[x] = [kids[p][j], kids[p][j + 1]]
+    if not host:
+        return par.copy(), np.arange(N, dtype=np.int64)
+    order, newpar, stack = [], [], [(0, -1)]
+    while stack:
+        node, pnew = stack.pop()
+        me = len(order)
+        order.append(node)
+        newpar.append(pnew)
+        for c in reversed(ch[node]):
+            stack.append((c, me))
+    src = np.array([n if n < N else -(host[n] + 1) for n in order], dtype=np.int64)
+    return np.array(newpar, dtype=np.int64), src
+
+
 def build_tree(par, e, f, cm, seg_to_comp, bool_model, ncomps: int) -> TreeData:
     """Assemble a complete :class:`TreeData` for a morphology.
 
@@ -176,7 +229,7 @@ def build_tree(par, e, f, cm, seg_to_comp, bool_model, ncomps: int) -> TreeData:
     son = son_numbers(par)
     if son.max(initial=0) > 2:
         raise TreeError("a node has more than 2 children (or the root more than 1); "
-                        "disperse the branching first (create_auxilliary_data_3.py:83-84)")
+                        "disperse the branching first (disperse_branching; create_auxilliary_data_3.py:83-84)")
     fidxs, logn = ancestor_table(ks)
     sch = schedule(ks)
     bm = np.asarray(bool_model, dtype=np.int64).reshape(-1, N)
