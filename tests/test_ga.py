@@ -111,3 +111,32 @@ def test_ga_improves_and_surrogate_screen_beats_random():
     assert np.median(true[pick]) < np.median(true[rand])
     ens.close()
     ev.close()
+
+
+@pytest.mark.gpu
+def test_ibea_on_the_device_evaluator_resumes_bitwise(tmp_path):
+    """bluepyopt's loop (IBEA selector, Pareto-front hall of fame,
+    checkpoint every generation) on simulated scores: a run stopped after two
+    generations and resumed from its checkpoint ends where the uninterrupted
+    run ends (the simulation is deterministic)."""
+    from neurogpu_amd import models, selection
+    w = models.workload("active500", nsets=1, nt=1500)
+    m = w.model
+    a, d = models.step_stimulus([0.3, 1.0], 1500, 0.02, 5.0, 25.0)
+    w.amps, w.dts = a, d
+    ev = ga.Evaluator(w, 5.0, 25.0, target_x=m.nominal())
+    print("case IBEA GA on active500, 32 individuals x 4 generations", flush=True)
+    runs = []
+    for stop, resume in ((4, False), (2, False), (4, True)):
+        ev.nevals = 0
+        hof = selection.ParetoFront()
+        cp = str(tmp_path / ("whole.npz" if stop == 4 and not resume else "part.npz"))
+        runs.append(ga.run_ga(ev, mu=32, lam=32, ngen=stop, seed=5, selector="ibea", hof=hof,
+                              cp_filename=cp, continue_cp=resume))
+    whole, _, resumed = runs
+    np.testing.assert_array_equal(resumed.population, whole.population)
+    np.testing.assert_array_equal(resumed.objectives, whole.objectives)
+    np.testing.assert_array_equal(resumed.hof.f, whole.hof.f)
+    assert resumed.best_score == whole.best_score <= whole.history[0][2]
+    assert not selection.dominated(whole.hof.f).any() and len(whole.hof) > 0
+    ev.close()
