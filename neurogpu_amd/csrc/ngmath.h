@@ -18,7 +18,8 @@
  *    summed to j = 11; the k*ln2 term uses the same split constant.
  *  - pow: small integer exponents by repeated multiplication, otherwise
  *    exp(y * log(x)).
- * Accuracy against glibc is checked in tests/test_ngmath.py.
+ * Accuracy against glibc is checked in tests/test_oracle.py
+ * (test_shared_exp_log_within_one_ulp_of_glibc).
  */
 #ifndef NGMATH_H
 #define NGMATH_H
