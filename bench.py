@@ -39,6 +39,9 @@ def algorithmic_bytes_per_comp_step(model) -> float:
 
 
 SIM_KERNEL = "ngpu_sim_kernel"
+BATCH_KERNEL = "ngpu_batch_kernel"
+ISSUE_COUNTERS = ["SQ_WAVES", "SQ_WAVE_CYCLES", "SQ_BUSY_CYCLES", "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU",
+                  "SQ_WAIT_ANY", "SQ_INSTS_LDS", "SQ_INSTS_VALU_FMA_F64"]
 COPY_KERNEL = "copyBuffer"
 
 
@@ -53,19 +56,21 @@ def _counter_rows(outdir: str):
     return rows
 
 
-def _pmc_pass(counter: str, child_args, kernel: str, timeout: int = 180):
+def _pmc_pass(counter, child_args, kernel: str, timeout: int = 180):
     """One rocprofv3 PMC pass over a child bench process: the counter's value
     (summed over the counter's instances) for each dispatch of kernels whose
-    name contains ``kernel``, in dispatch order.  None if rocprofv3 is absent
-    or the pass fails."""
+    name contains ``kernel``, in dispatch order; for a list of counters (one
+    pass collecting them together), a dict of such lists.  None if rocprofv3
+    is absent or the pass fails."""
     import shutil
     import subprocess
     import tempfile
     prof = shutil.which("rocprofv3")
     if prof is None:
         return None
+    names = [counter] if isinstance(counter, str) else list(counter)
     with tempfile.TemporaryDirectory(prefix="ngpu_pmc_") as d:
-        cmd = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+        cmd = [prof, "--pmc"] + names + ["--output-format", "csv", "-d", d, "-o", "run", "--",
                sys.executable, os.path.abspath(__file__)] + list(child_args)
         try:
             r = subprocess.run(cmd, capture_output=True, text=True, timeout=timeout)
@@ -74,15 +79,39 @@ def _pmc_pass(counter: str, child_args, kernel: str, timeout: int = 180):
         rows = _counter_rows(d)
         if r.returncode != 0 and not rows:
             return None
-        per = {}
+        per = {n: {} for n in names}
         for row in rows:
             keys = {k.lower(): k for k in row}
             name = row.get(keys.get("kernel_name", ""), "")
-            if kernel not in name or row.get(keys.get("counter_name", ""), "") != counter:
+            ctr = row.get(keys.get("counter_name", ""), "")
+            if kernel not in name or ctr not in per:
                 continue
             did = int(row.get(keys.get("dispatch_id", ""), 0) or 0)
-            per[did] = per.get(did, 0.0) + float(row[keys["counter_value"]])
-        return [per[k] for k in sorted(per)] or None
+            per[ctr][did] = per[ctr].get(did, 0.0) + float(row[keys["counter_value"]])
+        out = {n: [v[k] for k in sorted(v)] for n, v in per.items()}
+        if isinstance(counter, str):
+            return out[counter] or None
+        return out if all(out.values()) else None
+
+
+def calibrated_traffic(child, kernel: str):
+    """HBM bytes of the last dispatch of ``kernel`` in a child bench process,
+    from FETCH_SIZE and WRITE_SIZE passes (KiB per dispatch), each scaled by
+    what it reads for a 1 GiB device-to-device copy (bytes copied / counter
+    bytes)."""
+    copy = ["--copy-probe", "1024"]
+    out = {}
+    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
+        sim = _pmc_pass(ctr, child, kernel)
+        cal = _pmc_pass(ctr, copy, COPY_KERNEL)
+        if not sim or not cal:
+            return None
+        copied = 1024.0 * 1024.0                      # KiB moved each way by one copy dispatch
+        factor = copied / (sum(cal) / len(cal))
+        out[ctr] = {"kib_per_dispatch": sim[-1], "calibration_factor": round(factor, 4),
+                    "bytes": sim[-1] * 1024.0 * factor}
+    out["bytes_per_launch"] = out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]
+    return out
 
 
 def measured_traffic(workload: str, nsets, nt, window: int, solve: str = "auto"):
@@ -96,19 +125,7 @@ def measured_traffic(workload: str, nsets, nt, window: int, solve: str = "auto")
         child += ["--nsets", str(nsets)]
     if nt is not None:
         child += ["--nt", str(nt)]
-    copy = ["--copy-probe", "1024"]
-    out = {}
-    for ctr in ("FETCH_SIZE", "WRITE_SIZE"):
-        sim = _pmc_pass(ctr, child, SIM_KERNEL)
-        cal = _pmc_pass(ctr, copy, COPY_KERNEL)
-        if not sim or not cal:
-            return None
-        copied = 1024.0 * 1024.0                      # KiB moved each way by one copy dispatch
-        factor = copied / (sum(cal) / len(cal))
-        out[ctr] = {"kib_per_dispatch": sim[-1], "calibration_factor": round(factor, 4),
-                    "bytes": sim[-1] * 1024.0 * factor}
-    out["bytes_per_launch"] = out["FETCH_SIZE"]["bytes"] + out["WRITE_SIZE"]["bytes"]
-    return out
+    return calibrated_traffic(child, SIM_KERNEL)
 
 
 def copy_peak_gbs(mib: int = 2048, reps: int = 10) -> float:
@@ -175,16 +192,25 @@ def bench_mixed(args):
     from neurogpu_amd._lib import Batch, workload_simulator
     ws = models.mixed_workloads(mechset=args.mixed_set)
     say = lambda m: print(m, flush=True)
-    choices = {}
-    for w in ws:
-        say(f"tuning {w.name}: N={w.model.tree.N} runs={w.nruns} nt={w.dts.size}")
-        c, _ = autotune.tune(w, log=say)
-        choices[w.name] = c
+    if args.mixed_choices:
+        choices = {k: autotune.Choice(**v) for k, v in json.loads(args.mixed_choices).items()}
+    else:
+        choices = {}
+        for w in ws:
+            say(f"tuning {w.name}: N={w.model.tree.N} runs={w.nruns} nt={w.dts.size}")
+            c, _ = autotune.tune(w, log=say)
+            choices[w.name] = c
     sims = [workload_simulator(w) for w in ws]
     b = Batch(0)
     for i, (w, sim) in enumerate(zip(ws, sims)):
         b.add(sim)
         b.set_launch(i, **choices[w.name].as_dict())
+    if args.pmc_child:
+        b.run()
+        b.close()
+        for sim in sims:
+            sim.close()
+        return
     for i in range(args.warmup):
         st = b.run()
         say(f"warmup {i}: kernel {st['ms_kernel']:.1f} ms")
@@ -223,6 +249,22 @@ def bench_mixed(args):
         "tail_ms": round(max(i["last_end_ms"] for i in infos) - sorted(i["last_end_ms"] for i in infos)[-2], 2),
     }
     b.close()
+    if not args.no_pmc:
+        say("rocprofv3 passes over one batch launch (child processes)")
+        child = ["--pmc-child", "--workload", "mixed4", "--mixed-set", args.mixed_set, "--mixed-choices",
+                 json.dumps({k: c.as_dict() for k, c in choices.items()})]
+        traffic = calibrated_traffic(child, BATCH_KERNEL)
+        if traffic:
+            result["roofline"]["traffic"] = traffic["bytes_per_launch"]
+            result["roofline"]["traffic_over_algorithmic"] = round(traffic["bytes_per_launch"] / alg, 3)
+            result["roofline"]["actual_GBs"] = round(traffic["bytes_per_launch"] / (launch_ms * 1e-3) / 1e9, 1)
+            result["roofline"]["pmc"] = {k: traffic[k] for k in ("FETCH_SIZE", "WRITE_SIZE")}
+        issue = _pmc_pass(ISSUE_COUNTERS, child, BATCH_KERNEL, timeout=300)
+        if issue:
+            c = {k: v[-1] for k, v in issue.items()}
+            result["issue"] = {"counters": c, "valu_lane_instructions_per_comp_step": c["SQ_INSTS_VALU"] * 64 / total,
+                               "fraction_of_wave_cycles_waiting": c["SQ_WAIT_ANY"] / c["SQ_WAVE_CYCLES"],
+                               "fraction_of_wave_cycles_valu_active": c["SQ_ACTIVE_INST_VALU"] / c["SQ_WAVE_CYCLES"]}
     if args.separate:
         sep = {}
         for w, sim in zip(ws, sims):
@@ -290,6 +332,7 @@ def main():
                     help="mixed4: also time each morphology in its own launch, one after another")
     ap.add_argument("--mixed-set", default="mainen", choices=("mainen", "ttpc"),
                     help="mixed4: the channel set of the four cells (mainen = the active cell's)")
+    ap.add_argument("--mixed-choices", default="", help=argparse.SUPPRESS)
     ap.add_argument("--nsets", type=int, default=None)
     ap.add_argument("--nt", type=int, default=None)
     ap.add_argument("--waves-per-run", type=int, default=0)
@@ -319,7 +362,7 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8}))
         return
-    if args.pmc_child:
+    if args.pmc_child and args.workload != "mixed4":
         pmc_child(args)
         return
     if args.gpus != 1:
