@@ -32,14 +32,23 @@ from .surrogate import MLPEnsemble, feature_targets, genome_to_input, make_scree
 
 def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, seed=1178, device=0,
           oversample=20, keep=0.15, disagree=0.03, refit_epochs=10, explore=0.0, max_train=4096,
-          max_screen_gens=None, mu_screen=None, random=0.0, pretrain=0):
-    w = models.workload(name, nsets=1, nt=nt)
+          max_screen_gens=None, mu_screen=None, random=0.0, pretrain=0, protocol="short"):
+    """``protocol="short"``: ``nt`` steps with the step current on from 5 to
+    45 ms; ``"full"``: the workload's own protocol (active500: 10,000 steps
+    of 0.02 ms, 20-180 ms), ``nt`` ignored.  Features are measured while the
+    current is on."""
+    if protocol == "full":
+        w = models.workload(name, nsets=1)
+        nt = int(w.dts.size)
+        t_on, t_off = models.STIM_WINDOW[name]
+    else:
+        w = models.workload(name, nsets=1, nt=nt)
+        dt = float(w.dts[0])
+        t_on, t_off = 5.0, 45.0 if nt * dt >= 45.0 else nt * dt
+        amps = {"active500": [0.2, 0.5, 1.0], "ttpc1200": [0.1, 0.3, 0.5, 0.7, 0.9]}.get(name, [0.5])
+        w.amps, w.dts = models.step_stimulus(amps, nt, dt, t_on, t_off)
     m = w.model
-    dt = float(w.dts[0])
-    t_on, t_off = 5.0, 45.0 if nt * dt >= 45.0 else nt * dt
-    amps = {"active500": [0.2, 0.5, 1.0], "ttpc1200": [0.1, 0.3, 0.5, 0.7, 0.9]}.get(name, [0.5])
-    w.amps, w.dts = models.step_stimulus(amps, nt, dt, t_on, t_off)
-    nstim = len(amps)
+    nstim = w.amps.shape[0]
     ev = ga.Evaluator(w, t_on, t_off, device=device, target_x=m.nominal())
     lo = np.log10([fp.lo for fp in m.free])
     hi = np.log10([fp.hi for fp in m.free])
@@ -109,7 +118,7 @@ def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, see
     ens.close()
     ev.close()
     return {
-        "workload": f"{w.name} protocol {nt} steps x {nstim} stimuli, pop {pop}",
+        "workload": f"{w.name} protocol {nt} steps x {nstim} stimuli, current {t_on}-{t_off} ms, pop {pop}",
         "plain": {"best_score": s_star, "wall_s": plain_t, "evals": plain_evals,
                   "evals_per_s": plain_evals / plain_t, "history": plain.history},
         "screened": {"best_score": scr.best_score, "evals": ev.nevals, "surrogate_fit_s": fit["s"],
@@ -129,6 +138,8 @@ def main():
     ap.add_argument("--pop", type=int, default=1024)
     ap.add_argument("--gens", type=int, default=20)
     ap.add_argument("--nt", type=int, default=2000)
+    ap.add_argument("--protocol", default="short", choices=("short", "full"),
+                    help="full: the workload's own protocol (active500: 10,000 steps, current 20-180 ms)")
     ap.add_argument("--members", type=int, default=5)
     ap.add_argument("--epochs", type=int, default=60)
     ap.add_argument("--oversample", type=int, default=20)
@@ -147,7 +158,7 @@ def main():
     r = study(args.workload, args.pop, args.gens, args.nt, args.members, args.epochs, seed=args.seed,
               oversample=args.oversample, keep=args.keep, disagree=args.disagree, refit_epochs=args.refit_epochs,
               explore=args.explore, max_train=args.max_train, mu_screen=args.mu_screen, random=args.random,
-              pretrain=args.pretrain)
+              pretrain=args.pretrain, protocol=args.protocol)
     print(json.dumps(r), flush=True)
 
 

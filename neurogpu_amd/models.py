@@ -554,6 +554,8 @@ def mixed_workloads(nsets_scale: float = 1.0, nt_scale: float = 1.0, seed: int =
 
 
 WORKLOADS = ("passive64", "active500", "ttpc1200")
+# each workload's step current: (on, off) in ms
+STIM_WINDOW = {"passive64": (10.0, 90.0), "active500": (20.0, 180.0), "ttpc1200": (20.0, 280.0)}
 _SEEDS = {"passive64": 1000, "active500": 1001, "ttpc1200": 1002}
 
 
@@ -569,13 +571,14 @@ def workload(name: str, nsets: Optional[int] = None, nt: Optional[int] = None, s
     seed = _SEEDS[name] if seed is None else seed
     if name == "passive64":
         model = passive_model(seed)
-        n, T, dt, amps, on, off = 32, 4000, 0.025, [0.5], 10.0, 90.0
+        n, T, dt, amps = 32, 4000, 0.025, [0.5]
     elif name == "active500":
         model = active_model(seed)
-        n, T, dt, amps, on, off = 1024, 10000, 0.02, [0.2, 0.5, 1.0], 20.0, 180.0
+        n, T, dt, amps = 1024, 10000, 0.02, [0.2, 0.5, 1.0]
     else:
         model = ttpc_model(seed)
-        n, T, dt, amps, on, off = 8192, 12000, 0.025, [0.1, 0.3, 0.5, 0.7, 0.9], 20.0, 280.0
+        n, T, dt, amps = 8192, 12000, 0.025, [0.1, 0.3, 0.5, 0.7, 0.9]
+    on, off = STIM_WINDOW[name]
     n = n if nsets is None else nsets
     T = T if nt is None else nt
     x = model.sample(n, seed)

@@ -1,0 +1,570 @@
+"""NMODL -> HIP/C++ generator for Hodgkin-Huxley-style mechanisms (cnexp).
+
+The reference turns a model's .mod files into CUDA device code with its own
+parser and emitter (PYS/extractModel.py:1905-2000 reads the mod files'
+blocks, :2003- writes the per-mechanism state and current functions into
+AllModels.cu).  This module does the same job for the subset of NMODL that
+HH-style membrane mechanisms use, and emits host- and device-callable C++:
+
+* blocks NEURON (SUFFIX, USEION ... READ ... WRITE ..., NONSPECIFIC_CURRENT,
+  RANGE, GLOBAL), UNITS (named physical constants such as FARADAY), PARAMETER
+  (defaults), CONSTANT, STATE, ASSIGNED, BREAKPOINT (SOLVE states METHOD
+  cnexp, assignments), DERIVATIVE states, INITIAL, PROCEDURE and FUNCTION;
+* statements: assignments, ``x' = expr``, LOCAL, if / else if / else,
+  procedure calls; UNITSOFF / UNITSON and TABLE lines are ignored;
+  VERBATIM, KINETIC, NET_RECEIVE and other solve methods are refused.
+
+Each mechanism becomes a struct whose fields are its variables (v, dt,
+PARAMETER defaults, STATE, ASSIGNED, ion variables) and whose member
+functions are its blocks, so that PROCEDUREs act on the instance as in
+NEURON.  ``x' = expr`` must be linear in x (A + B x); it is integrated with
+NEURON's cnexp formula x += (1 - exp(dt B)) (-A/B - x), which is the form of
+the reference's generated code (e.g. Code/OneKernel/src/AllModels.cu:302).
+exp, log and pow are ngmath.h's ng_exp, ng_log and ng_pow (the same on the
+host and the device).
+
+    python -m neurogpu_amd.nmodl -o gen_mechs.h a.mod b.mod ...
+
+also writes a C-ABI evaluation entry per mechanism (``--harness``) that the
+tests compile on the host.
+"""
+from __future__ import annotations
+
+import argparse
+import ast
+import dataclasses
+import json
+import re
+import sys
+from typing import Dict, List, Optional, Tuple
+
+# physical constants of the UNITS block, as NEURON's units database defines them
+UNIT_CONSTANTS = {"faraday": 96485.309, "k-mole": 8.313424, "R": 8.313424}
+MATH = {"exp": "ng_exp", "log": "ng_log", "pow": "ng_pow", "fabs": "fabs", "sqrt": "sqrt", "tanh": "tanh",
+        "sin": "sin", "cos": "cos"}
+
+
+class NmodlError(ValueError):
+    pass
+
+
+@dataclasses.dataclass
+class Block:
+    kind: str
+    name: str
+    args: List[str]
+    body: str
+
+
+@dataclasses.dataclass
+class Mechanism:
+    suffix: str
+    params: Dict[str, Optional[float]]          # PARAMETER (None: no default, e.g. celsius)
+    constants: Dict[str, float]                 # UNITS and CONSTANT
+    states: List[str]
+    assigned: List[str]
+    ions: Dict[str, Tuple[List[str], List[str]]]   # ion -> (READ, WRITE)
+    nonspecific: List[str]
+    ranges: List[str]
+    blocks: Dict[str, Block]                    # BREAKPOINT, INITIAL, DERIVATIVE <name>, PROCEDURE <name>, FUNCTION <name>
+    solve: Optional[str]                        # the DERIVATIVE block BREAKPOINT solves
+
+    @property
+    def currents(self) -> List[str]:
+        out = list(self.nonspecific)
+        for ion, (_, w) in self.ions.items():
+            out += [x for x in w if x == "i" + ion]
+        return out
+
+
+def strip_comments(text: str) -> str:
+    text = re.sub(r"COMMENT.*?ENDCOMMENT", "", text, flags=re.S)
+    return re.sub(r":[^\n]*", "", text)
+
+
+def _match_brace(text: str, i: int) -> int:
+    """Index just past the brace that closes the one before position i."""
+    depth = 1
+    while depth:
+        if i >= len(text):
+            raise NmodlError("unbalanced braces")
+        depth += {"{": 1, "}": -1}.get(text[i], 0)
+        i += 1
+    return i
+
+
+_HEAD = re.compile(r"\b(NEURON|UNITS|PARAMETER|CONSTANT|STATE|ASSIGNED|BREAKPOINT|INITIAL|INDEPENDENT|"
+                   r"DERIVATIVE|PROCEDURE|FUNCTION|KINETIC|NET_RECEIVE|LINEAR|NONLINEAR)\b\s*(\w*)\s*")
+
+
+def _balanced(text: str, i: int) -> int:
+    """Index just past the parenthesis that closes the one at position i."""
+    depth, j = 0, i
+    while True:
+        if j >= len(text):
+            raise NmodlError("unbalanced parentheses")
+        depth += {"(": 1, ")": -1}.get(text[j], 0)
+        j += 1
+        if depth == 0:
+            return j
+
+
+def blocks(text: str) -> List[Block]:
+    """The top-level blocks: keyword, name, argument names (units dropped)
+    and body."""
+    out, i = [], 0
+    while True:
+        m = _HEAD.search(text, i)
+        if not m:
+            return out
+        j, args = m.end(), []
+        if j < len(text) and text[j] == "(":
+            k = _balanced(text, j)
+            inner = text[j + 1:k - 1]
+            depth, cur, parts = 0, "", []
+            for ch in inner:
+                depth += {"(": 1, ")": -1}.get(ch, 0)
+                if ch == "," and depth == 0:
+                    parts.append(cur)
+                    cur = ""
+                else:
+                    cur += ch
+            parts.append(cur)
+            args = [p.split("(")[0].strip() for p in parts if p.strip()]
+            j = k
+        while True:                                   # units of a FUNCTION's value, e.g. (mV)
+            while j < len(text) and text[j] in " \t\r\n":
+                j += 1
+            if j < len(text) and text[j] == "(":
+                j = _balanced(text, j)
+            else:
+                break
+        if j >= len(text) or text[j] != "{":
+            i = m.end()
+            continue
+        end = _match_brace(text, j + 1)
+        out.append(Block(m.group(1), m.group(2), args, text[j + 1:end - 1]))
+        i = end
+
+
+def _names(body: str) -> List[str]:
+    """Declared names of a STATE / ASSIGNED / PARAMETER-like block."""
+    body = re.sub(r"\([^)]*\)", " ", body)           # units
+    body = re.sub(r"<[^>]*>", " ", body)             # limits
+    body = re.sub(r"=\s*[-+0-9.eE]+", " ", body)
+    return [t for t in re.findall(r"[A-Za-z_]\w*", body) if t not in ("FROM", "TO")]
+
+
+def parse(text: str) -> Mechanism:
+    """A Mechanism from the text of a .mod file."""
+    text = strip_comments(text)
+    if "VERBATIM" in text:
+        raise NmodlError("VERBATIM blocks are not supported")
+    bl = blocks(text)
+    suffix, ions, nonspec, ranges = None, {}, [], []
+    params, consts, states, assigned, named = {}, {}, [], [], {}
+    solve = None
+    for b in bl:
+        if b.kind == "NEURON":
+            for line in b.body.split("\n"):
+                line = line.strip()
+                if line.startswith("SUFFIX"):
+                    suffix = line.split()[1]
+                elif line.startswith("USEION"):
+                    m = re.match(r"USEION\s+(\w+)(?:\s+READ\s+([\w\s,]+?))?(?:\s+WRITE\s+([\w\s,]+?))?(?:\s+VALENCE.*)?$", line)
+                    if not m:
+                        raise NmodlError(f"cannot read {line!r}")
+                    rd = [x.strip() for x in (m.group(2) or "").split(",") if x.strip()]
+                    wr = [x.strip() for x in (m.group(3) or "").split(",") if x.strip()]
+                    ions[m.group(1)] = (rd, wr)
+                elif line.startswith("NONSPECIFIC_CURRENT"):
+                    nonspec += [x.strip() for x in line.split(None, 1)[1].split(",")]
+                elif line.startswith("RANGE") or line.startswith("GLOBAL"):
+                    ranges += [x.strip() for x in line.split(None, 1)[1].split(",")]
+        elif b.kind == "UNITS":
+            for m in re.finditer(r"(\w+)\s*=\s*\(\s*([\w-]+)\s*\)", b.body):
+                if m.group(2) in UNIT_CONSTANTS:
+                    consts[m.group(1)] = UNIT_CONSTANTS[m.group(2)]
+        elif b.kind in ("PARAMETER", "CONSTANT"):
+            for line in b.body.split("\n"):
+                m = re.match(r"\s*([A-Za-z_]\w*)\s*(?:=\s*([-+0-9.eE]+))?", line)
+                if not m or not line.strip():
+                    continue
+                val = float(m.group(2)) if m.group(2) is not None else None
+                (consts if b.kind == "CONSTANT" else params)[m.group(1)] = val
+        elif b.kind == "STATE":
+            states += _names(b.body)
+        elif b.kind == "ASSIGNED":
+            assigned += _names(b.body)
+        elif b.kind == "BREAKPOINT":
+            named["BREAKPOINT"] = b
+            m = re.search(r"SOLVE\s+(\w+)\s+METHOD\s+(\w+)", b.body)
+            if m:
+                if m.group(2) != "cnexp":
+                    raise NmodlError(f"METHOD {m.group(2)} is not supported (cnexp only)")
+                solve = m.group(1)
+        elif b.kind == "INITIAL":
+            named["INITIAL"] = b
+        elif b.kind in ("DERIVATIVE", "PROCEDURE", "FUNCTION"):
+            named[f"{b.kind} {b.name}"] = b
+        elif b.kind in ("KINETIC", "NET_RECEIVE", "LINEAR", "NONLINEAR"):
+            raise NmodlError(f"{b.kind} blocks are not supported")
+    if suffix is None:
+        raise NmodlError("no SUFFIX")
+    return Mechanism(suffix, params, consts, states, assigned, ions, nonspec, ranges, named, solve)
+
+
+# ------------------------------------------------------------------ statements
+@dataclasses.dataclass
+class Stmt:
+    kind: str                     # assign, deriv, call, if, local
+    target: str = ""
+    expr: str = ""
+    branches: Optional[list] = None   # if: [(cond or None, [Stmt])]
+
+
+def _expr(e: str) -> str:
+    return e.strip()
+
+
+def statements(body: str) -> List[Stmt]:
+    """The statements of a block body."""
+    out, i, n = [], 0, len(body)
+    while i < n:
+        while i < n and body[i] in " \t\r\n;":
+            i += 1
+        if i >= n:
+            break
+        rest = body[i:]
+        m = re.match(r"if\s*\(", rest)
+        if m:
+            branches = []
+            j = i + m.end()
+            while True:
+                depth, k = 1, j
+                while depth:
+                    depth += {"(": 1, ")": -1}.get(body[k], 0)
+                    k += 1
+                cond = body[j:k - 1]
+                b0 = body.index("{", k)
+                b1 = _match_brace(body, b0 + 1)
+                branches.append((cond, statements(body[b0 + 1:b1 - 1])))
+                i = b1
+                m2 = re.match(r"\s*else\s*if\s*\(", body[i:])
+                if m2:
+                    j = i + m2.end()
+                    continue
+                m3 = re.match(r"\s*else\s*\{", body[i:])
+                if m3:
+                    b0 = i + m3.end()
+                    b1 = _match_brace(body, b0)
+                    branches.append((None, statements(body[b0:b1 - 1])))
+                    i = b1
+                break
+            out.append(Stmt("if", branches=branches))
+            continue
+        eol = body.find("\n", i)
+        line = body[i:] if eol < 0 else body[i:eol]
+        i = n if eol < 0 else eol + 1
+        line = line.strip()
+        if not line or line in ("UNITSOFF", "UNITSON") or line.startswith("TABLE") or line.startswith("SOLVE"):
+            continue
+        if line.startswith("LOCAL"):
+            for v in line[5:].split(","):
+                out.append(Stmt("local", target=v.strip()))
+            continue
+        m = re.match(r"([A-Za-z_]\w*)\s*'\s*=(.*)$", line)
+        if m:
+            out.append(Stmt("deriv", m.group(1), _expr(m.group(2))))
+            continue
+        m = re.match(r"([A-Za-z_]\w*)\s*=(.*)$", line)
+        if m:
+            out.append(Stmt("assign", m.group(1), _expr(m.group(2))))
+            continue
+        m = re.match(r"([A-Za-z_]\w*)\s*\((.*)\)\s*$", line)
+        if m:
+            out.append(Stmt("call", m.group(1), m.group(2)))
+            continue
+        raise NmodlError(f"cannot read statement {line!r}")
+    return out
+
+
+# ------------------------------------------------------------------ expressions
+def _py(e: str) -> ast.expr:
+    e = e.replace("^", "**")
+    try:
+        return ast.parse(e, mode="eval").body
+    except SyntaxError as err:
+        raise NmodlError(f"cannot read expression {e!r}: {err}") from None
+
+
+_BIN = {ast.Add: "+", ast.Sub: "-", ast.Mult: "*", ast.Div: "/"}
+_CMP = {ast.Eq: "==", ast.NotEq: "!=", ast.Lt: "<", ast.LtE: "<=", ast.Gt: ">", ast.GtE: ">="}
+
+
+def cxx(node: ast.expr) -> str:
+    """C++ for an expression tree."""
+    if isinstance(node, ast.BinOp):
+        if isinstance(node.op, ast.Pow):
+            return f"ng_pow({cxx(node.left)}, {cxx(node.right)})"
+        return f"({cxx(node.left)} {_BIN[type(node.op)]} {cxx(node.right)})"
+    if isinstance(node, ast.UnaryOp):
+        if isinstance(node.op, ast.USub):
+            return f"(-{cxx(node.operand)})"
+        if isinstance(node.op, ast.UAdd):
+            return cxx(node.operand)
+        if isinstance(node.op, ast.Not):
+            return f"(!{cxx(node.operand)})"
+    if isinstance(node, ast.Constant):
+        return repr(float(node.value))
+    if isinstance(node, ast.Name):
+        return node.id
+    if isinstance(node, ast.Call) and isinstance(node.func, ast.Name):
+        fn = MATH.get(node.func.id, node.func.id)
+        return f"{fn}({', '.join(cxx(a) for a in node.args)})"
+    if isinstance(node, ast.Compare) and len(node.ops) == 1:
+        return f"({cxx(node.left)} {_CMP[type(node.ops[0])]} {cxx(node.comparators[0])})"
+    if isinstance(node, ast.BoolOp):
+        op = " && " if isinstance(node.op, ast.And) else " || "
+        return "(" + op.join(cxx(v) for v in node.values) + ")"
+    raise NmodlError(f"unsupported expression {ast.dump(node)}")
+
+
+def _cond(e: str) -> ast.expr:
+    e = re.sub(r"(?<![=!<>])=(?!=)", "==", e)
+    e = e.replace("&&", " and ").replace("||", " or ")
+    return _py(e)
+
+
+def _mentions(node: ast.expr, name: str) -> bool:
+    return any(isinstance(n, ast.Name) and n.id == name for n in ast.walk(node))
+
+
+def _is(node: ast.expr, c: float) -> bool:
+    return isinstance(node, ast.Constant) and float(node.value) == c
+
+
+def _add(a: ast.expr, op: ast.operator, b: ast.expr) -> ast.expr:
+    """a + b or a - b, dropping an exact zero operand."""
+    if _is(b, 0.0):
+        return a
+    if _is(a, 0.0):
+        return b if isinstance(op, ast.Add) else ast.UnaryOp(ast.USub(), b)
+    return ast.BinOp(a, op, b)
+
+
+def _mul(a: ast.expr, op: ast.operator, b: ast.expr) -> ast.expr:
+    """a * b or a / b, keeping an exact zero and dropping a unit factor."""
+    if _is(a, 0.0) or (isinstance(op, ast.Mult) and _is(b, 0.0)):
+        return ast.Constant(0.0)
+    if _is(b, 1.0):
+        return a
+    if isinstance(op, ast.Mult) and _is(a, 1.0):
+        return b
+    return ast.BinOp(a, op, b)
+
+
+def linear(node: ast.expr, x: str) -> Tuple[ast.expr, ast.expr]:
+    """(A, B) with node = A + B x, for node linear in x (exact zeros and unit
+    factors dropped, which changes no rounding)."""
+    zero = ast.Constant(0.0)
+    if not _mentions(node, x):
+        return node, zero
+    if isinstance(node, ast.Name):
+        return zero, ast.Constant(1.0)
+    if isinstance(node, ast.UnaryOp) and isinstance(node.op, (ast.USub, ast.UAdd)):
+        a, b = linear(node.operand, x)
+        if isinstance(node.op, ast.UAdd):
+            return a, b
+        return ast.UnaryOp(ast.USub(), a), ast.UnaryOp(ast.USub(), b)
+    if isinstance(node, ast.BinOp):
+        if isinstance(node.op, (ast.Add, ast.Sub)):
+            a1, b1 = linear(node.left, x)
+            a2, b2 = linear(node.right, x)
+            return _add(a1, node.op, a2), _add(b1, node.op, b2)
+        if isinstance(node.op, ast.Mult):
+            if not _mentions(node.left, x):
+                a, b = linear(node.right, x)
+                return _mul(node.left, ast.Mult(), a), _mul(node.left, ast.Mult(), b)
+            if not _mentions(node.right, x):
+                a, b = linear(node.left, x)
+                return _mul(a, ast.Mult(), node.right), _mul(b, ast.Mult(), node.right)
+        if isinstance(node.op, ast.Div) and not _mentions(node.right, x):
+            a, b = linear(node.left, x)
+            return _mul(a, ast.Div(), node.right), _mul(b, ast.Div(), node.right)
+    raise NmodlError(f"{ast.unparse(node)} is not linear in {x} (cnexp needs x' = A + B x)")
+
+
+# ------------------------------------------------------------------ emission
+def _emit(stmts: List[Stmt], mech: Mechanism, ind: str, locals_: List[str], ret: Optional[str] = None) -> List[str]:
+    out = []
+    for s in stmts:
+        if s.kind == "local":
+            if s.target not in locals_:
+                locals_.append(s.target)
+                out.append(f"{ind}double {s.target} = 0.0;")
+        elif s.kind == "assign":
+            tgt = "ret_" if ret is not None and s.target == ret else s.target
+            out.append(f"{ind}{tgt} = {cxx(_py(s.expr))};")
+        elif s.kind == "call":
+            args = [a.strip() for a in s.expr.split(",") if a.strip()]
+            out.append(f"{ind}{s.target}({', '.join(cxx(_py(a)) for a in args)});")
+        elif s.kind == "deriv":
+            a, b = linear(_py(s.expr), s.target)
+            A, B = cxx(a), cxx(b)
+            x = s.target
+            out.append(f"{ind}{x} = {x} + (1.0 - ng_exp(dt * ({B}))) * (-({A}) / ({B}) - {x});")
+        elif s.kind == "if":
+            for k, (cond, body) in enumerate(s.branches):
+                head = "if" if k == 0 else ("} else if" if cond is not None else "} else")
+                out.append(f"{ind}{head}" + (f" ({cxx(_cond(cond))}) {{" if cond is not None else " {"))
+                out += _emit(body, mech, ind + "    ", locals_, ret)
+            out.append(f"{ind}}}")
+    return out
+
+
+def emit(mech: Mechanism, source: str = "") -> str:
+    """C++ for one mechanism: a struct of its variables with its blocks as
+    member functions (initial(), states() -- the solved DERIVATIVE block --,
+    breakpoint(), and every PROCEDURE and FUNCTION)."""
+    ionvars = []
+    for ion, (rd, wr) in mech.ions.items():
+        for v in rd + wr:
+            if v not in ionvars:
+                ionvars.append(v)
+    fields = ["v", "dt"]
+    lines = [f"/* {mech.suffix}: generated by neurogpu_amd.nmodl" + (f" from {source}" if source else "") + " */",
+             f"struct Mech_{mech.suffix} {{", "    double v = 0.0, dt = 0.0;"]
+    for name, val in mech.params.items():
+        if name in ("v", "dt") or name in fields:
+            continue
+        fields.append(name)
+        lines.append(f"    double {name} = {repr(float(val)) if val is not None else '0.0'};")
+    for name, val in mech.constants.items():
+        lines.append(f"    static constexpr double {name} = {repr(float(val))};")
+    for name in mech.states + mech.assigned + ionvars:
+        if name in fields or name in mech.constants:
+            continue
+        fields.append(name)
+        lines.append(f"    double {name} = 0.0;")
+    for key, b in mech.blocks.items():
+        if b.kind == "FUNCTION":
+            lines.append(f"    NG_HD_M double {b.name}({', '.join('double ' + a for a in b.args)})")
+            lines.append("    {")
+            lines.append("        double ret_ = 0.0;")
+            lines += _emit(statements(b.body), mech, "        ", list(b.args), ret=b.name)
+            lines.append("        return ret_;")
+            lines.append("    }")
+    for key, b in mech.blocks.items():
+        if b.kind == "PROCEDURE":
+            lines.append(f"    NG_HD_M void {b.name}({', '.join('double ' + a for a in b.args)})")
+            lines.append("    {")
+            lines += _emit(statements(b.body), mech, "        ", list(b.args))
+            lines.append("    }")
+    ini = mech.blocks.get("INITIAL")
+    lines.append("    NG_HD_M void initial()")
+    lines.append("    {")
+    if ini is not None:
+        lines += _emit(statements(ini.body), mech, "        ", [])
+    lines.append("    }")
+    lines.append("    NG_HD_M void states()")
+    lines.append("    {")
+    if mech.solve:
+        der = mech.blocks.get(f"DERIVATIVE {mech.solve}")
+        if der is None:
+            raise NmodlError(f"BREAKPOINT solves {mech.solve}, which is not a DERIVATIVE block")
+        lines += _emit(statements(der.body), mech, "        ", [])
+    lines.append("    }")
+    bp = mech.blocks.get("BREAKPOINT")
+    lines.append("    NG_HD_M void breakpoint()")
+    lines.append("    {")
+    if bp is not None:
+        lines += _emit(statements(bp.body), mech, "        ", [])
+    lines.append("    }")
+    lines.append("};")
+    return "\n".join(lines)
+
+
+def interface(mech: Mechanism) -> dict:
+    """Names of a generated mechanism's parameters, states, assigned and ion
+    variables, in the order the harness entry passes them."""
+    ionvars = []
+    for ion, (rd, wr) in mech.ions.items():
+        for v in rd + wr:
+            if v not in ionvars:
+                ionvars.append(v)
+    params = [p for p in mech.params if p not in ("v", "dt")]
+    seen = set(params) | {"v", "dt"}
+    states = [s for s in mech.states if s not in seen]
+    seen |= set(states)
+    assigned = [a for a in mech.assigned if a not in seen and a not in mech.constants]
+    seen |= set(assigned)
+    ions = [x for x in ionvars if x not in seen]
+    return {"suffix": mech.suffix, "params": params, "states": states, "assigned": assigned, "ions": ions,
+            "currents": mech.currents}
+
+
+def harness(mech: Mechanism) -> str:
+    """extern "C" int ng_gen_<suffix>(block, v, dt, params, states, assigned, ions):
+    loads the arrays into an instance, runs block 0 (initial), 1 (states) or
+    2 (breakpoint), and stores every array back."""
+    f = interface(mech)
+    s = mech.suffix
+    out = [f"extern \"C\" int ng_gen_{s}(int block, double v, double dt, double *p, double *st, double *as, double *io)",
+           "{", f"    Mech_{s} m;", "    m.v = v; m.dt = dt;"]
+    for arr, names in (("p", f["params"]), ("st", f["states"]), ("as", f["assigned"]), ("io", f["ions"])):
+        for k, n in enumerate(names):
+            out.append(f"    m.{n} = {arr}[{k}];")
+    out.append("    if (block == 0) m.initial(); else if (block == 1) m.states(); else if (block == 2) m.breakpoint(); else return -1;")
+    for arr, names in (("p", f["params"]), ("st", f["states"]), ("as", f["assigned"]), ("io", f["ions"])):
+        for k, n in enumerate(names):
+            out.append(f"    {arr}[{k}] = m.{n};")
+    out.append("    return 0;")
+    out.append("}")
+    return "\n".join(out)
+
+
+PROLOGUE = """/* Generated by neurogpu_amd.nmodl -- do not edit. */
+#pragma once
+#include "ngmath.h"
+#include <math.h>
+#ifdef __HIPCC__
+#define NG_HD_M __host__ __device__ inline
+#else
+#define NG_HD_M inline
+#endif
+"""
+
+
+def generate(sources: Dict[str, str], with_harness: bool = False) -> Tuple[str, List[dict]]:
+    """C++ for several .mod texts (name -> text) and their interfaces."""
+    parts, faces = [PROLOGUE], []
+    for name, text in sources.items():
+        mech = parse(text)
+        parts.append(emit(mech, name))
+        faces.append(interface(mech))
+        if with_harness:
+            parts.append(harness(mech))
+    return "\n\n".join(parts) + "\n", faces
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("mods", nargs="+")
+    ap.add_argument("-o", "--output", required=True)
+    ap.add_argument("--harness", action="store_true", help="also emit the extern \"C\" evaluation entries")
+    ap.add_argument("--interface", default="", help="write the mechanisms' variable lists as JSON here")
+    args = ap.parse_args(argv)
+    import os
+    srcs = {os.path.basename(p): open(p).read() for p in args.mods}
+    code, faces = generate(srcs, args.harness)
+    with open(args.output, "w") as fh:
+        fh.write(code)
+    if args.interface:
+        with open(args.interface, "w") as fh:
+            json.dump(faces, fh, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

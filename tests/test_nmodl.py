@@ -1,0 +1,322 @@
+"""The NMODL -> C++ generator (neurogpu_amd/nmodl.py), on the CPU.
+
+* A small Hodgkin-Huxley-style mechanism written here (not from the
+  reference) is generated, compiled on the host and compared with the
+  closed forms of its own equations: rates through a FUNCTION with a branch,
+  a LOCAL, a power, a PROCEDURE argument, and cnexp against the exact
+  solution of the linear ODE.
+* The TTPC mechanisms (the reference's Figures/Figure2_BBP_TTPC_clean/*.mod,
+  read from /root/reference where it is present) are generated and checked
+  against the fixtures that tests/golden/make_mod_rates.py evaluated from the
+  same files' equations (rates, one exact step of each state, currents) and
+  against the hand-written device forms of neurogpu_amd/csrc/mechs.h.
+"""
+import ctypes as C
+import math
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, ROOT
+from neurogpu_amd import nmodl
+
+CSRC = os.path.join(ROOT, "neurogpu_amd", "csrc")
+TTPC_MODS = "/root/reference/Figures/Figure2_BBP_TTPC_clean"
+TTPC = ["CaDynamics_E2", "Ca_HVA", "Ca_LVAst", "Ih", "Im", "K_Pst", "K_Tst", "Nap_Et2", "NaTa_t", "NaTs2_t", "SK_E2",
+        "SKv3_1"]
+_DP = C.POINTER(C.c_double)
+
+HHTEST = """
+TITLE a test potassium channel
+NEURON {
+    SUFFIX hhtest
+    USEION k READ ek WRITE ik
+    RANGE gbar
+}
+UNITS { (mV) = (millivolt) (mA) = (milliamp) }
+PARAMETER {
+    gbar = 0.036 (S/cm2) <0, 1e9>
+    tq = 26 (degC)
+}
+STATE { n c }
+ASSIGNED { v (mV) ek (mV) ik (mA/cm2) ninf ntau an bn }
+BREAKPOINT {
+    SOLVE states METHOD cnexp
+    ik = gbar*n^4*(v - ek)
+}
+INITIAL {
+    rates(v)
+    n = ninf
+    c = 0.25
+}
+DERIVATIVE states {
+    rates(v)
+    n' = (ninf - n)/ntau
+    c' = 0.5 - 2*c
+}
+PROCEDURE rates(vm (mV)) {
+    LOCAL q
+    UNITSOFF
+    q = 3^((tq - 6.3)/10)
+    an = 0.01*vtrap(-(vm + 55), 10)
+    bn = 0.125*exp(-(vm + 65)/80)
+    ntau = 1/(q*(an + bn))
+    ninf = an/(an + bn)
+    UNITSON
+}
+FUNCTION vtrap(x, y) {
+    if (fabs(x/y) < 1e-6) {
+        vtrap = y*(1 - x/y/2)
+    } else {
+        vtrap = x/(exp(x/y) - 1)
+    }
+}
+"""
+
+
+def _build(tmp, sources):
+    code, faces = nmodl.generate(sources, with_harness=True)
+    hdr = tmp / "gen.h"
+    hdr.write_text(code)
+    src = tmp / "gen.cpp"
+    src.write_text('#include "gen.h"\n')
+    lib = tmp / "libgen.so"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-I", CSRC, "-I", str(tmp),
+                    str(src), "-o", str(lib)], check=True)
+    L = C.CDLL(str(lib))
+    out = {}
+    for f in faces:
+        fn = getattr(L, f"ng_gen_{f['suffix']}")
+        fn.restype = C.c_int
+        fn.argtypes = [C.c_int, C.c_double, C.c_double, _DP, _DP, _DP, _DP]
+        out[f["suffix"]] = (fn, f)
+    return out
+
+
+def run_block(gen, block, v, dt=0.0, **values):
+    """Run one block of a generated mechanism with the given variables set
+    (the rest at their PARAMETER defaults or 0); every variable afterwards."""
+    fn, f = gen
+    code, _ = None, None
+    arrays = {}
+    for key in ("params", "states", "assigned", "ions"):
+        arrays[key] = np.zeros(max(1, len(f[key])))
+    defaults = run_block.defaults.get(f["suffix"], {})
+    for k, name in enumerate(f["params"]):
+        arrays["params"][k] = values.get(name, defaults.get(name, 0.0))
+    for key in ("states", "assigned", "ions"):
+        for k, name in enumerate(f[key]):
+            arrays[key][k] = values.get(name, 0.0)
+    rc = fn(block, v, dt, *(arrays[k].ctypes.data_as(_DP) for k in ("params", "states", "assigned", "ions")))
+    assert rc == 0
+    out = {}
+    for key in ("params", "states", "assigned", "ions"):
+        for k, name in enumerate(f[key]):
+            out[name] = float(arrays[key][k])
+    return out
+
+
+run_block.defaults = {}
+
+
+def _register_defaults(sources):
+    for text in sources.values():
+        m = nmodl.parse(text)
+        run_block.defaults[m.suffix] = {k: (v if v is not None else 0.0) for k, v in m.params.items()}
+
+
+# ------------------------------------------------------------------ own mechanism
+@pytest.fixture(scope="module")
+def hh(tmp_path_factory):
+    src = {"hhtest.mod": HHTEST}
+    _register_defaults(src)
+    return _build(tmp_path_factory.mktemp("hh"), src)["hhtest"]
+
+
+def _hh_rates(v, tq=26.0):
+    def vtrap(x, y):
+        return y * (1 - x / y / 2) if abs(x / y) < 1e-6 else x / (math.exp(x / y) - 1)
+    q = 3.0 ** ((tq - 6.3) / 10)
+    an = 0.01 * vtrap(-(v + 55), 10)
+    bn = 0.125 * math.exp(-(v + 65) / 80)
+    return an / (an + bn), 1 / (q * (an + bn))
+
+
+@pytest.mark.parametrize("v", [-90.0, -65.0, -55.0, -55.0 + 1e-9, -20.0, 30.0])
+def test_own_mechanism_rates_and_initial(hh, v):
+    out = run_block(hh, 0, v)
+    ninf, ntau = _hh_rates(v)
+    assert out["ninf"] == pytest.approx(ninf, rel=1e-13)
+    assert out["ntau"] == pytest.approx(ntau, rel=1e-13)
+    assert out["n"] == out["ninf"] and out["c"] == 0.25
+
+
+def test_own_mechanism_cnexp_is_the_exact_step(hh):
+    """For x' = (xinf - x)/tau the step is x(dt) = xinf + (x - xinf) e^(-dt/tau);
+    for c' = 0.5 - 2c, c(dt) = 0.25 + (c - 0.25) e^(-2 dt)."""
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        v, dt, n0, c0 = rng.uniform(-90, 40), rng.choice([0.01, 0.025, 0.5]), rng.uniform(0, 1), rng.uniform(0, 2)
+        out = run_block(hh, 1, v, dt, n=n0, c=c0)
+        ninf, ntau = _hh_rates(v)
+        assert out["n"] == pytest.approx(ninf + (n0 - ninf) * math.exp(-dt / ntau), rel=1e-12, abs=1e-15)
+        assert out["c"] == pytest.approx(0.25 + (c0 - 0.25) * math.exp(-2 * dt), rel=1e-12, abs=1e-15)
+
+
+def test_own_mechanism_current(hh):
+    out = run_block(hh, 2, -40.0, n=0.3, ek=-77.0, gbar=0.02)
+    assert out["ik"] == pytest.approx(0.02 * 0.3 ** 4 * (-40.0 + 77.0), rel=1e-14)
+
+
+def test_linear_extraction():
+    import ast
+    a, b = nmodl.linear(ast.parse("(xinf - x)/tau", mode="eval").body, "x")
+    assert ast.unparse(a) == "xinf / tau" and ast.unparse(b) == "-1.0 / tau"
+    a, b = nmodl.linear(ast.parse("-k*(x - m) + s", mode="eval").body, "x")
+    env = {"k": 3.0, "m": 2.0, "s": 0.5}
+    A, B = (eval(compile(ast.fix_missing_locations(ast.Expression(e)), "e", "eval"), env) for e in (a, b))
+    assert (A, B) == (6.5, -3.0)
+    with pytest.raises(nmodl.NmodlError):
+        nmodl.linear(ast.parse("x*x", mode="eval").body, "x")
+    with pytest.raises(nmodl.NmodlError):
+        nmodl.linear(ast.parse("exp(x)", mode="eval").body, "x")
+
+
+@pytest.mark.parametrize("bad, why", [
+    (HHTEST.replace("METHOD cnexp", "METHOD derivimplicit"), "derivimplicit"),
+    (HHTEST.replace("c' = 0.5 - 2*c", "c' = 0.5 - 2*c*c"), "linear"),
+    (HHTEST + "\nVERBATIM\nreturn 0;\nENDVERBATIM\n", "VERBATIM"),
+    (HHTEST.replace("SUFFIX hhtest", ""), "SUFFIX"),
+    (HHTEST.replace("    ik = gbar*n^4*(v - ek)", "    ik ~ gbar"), "statement"),
+])
+def test_unsupported_input_is_refused(bad, why):
+    with pytest.raises(nmodl.NmodlError, match=why):
+        nmodl.generate({"bad.mod": bad})
+
+
+# ------------------------------------------------------------------ the TTPC set
+def _ttpc_sources():
+    if not os.path.isdir(TTPC_MODS):
+        pytest.skip("the reference's .mod files are not present here")
+    return {f"{n}.mod": open(os.path.join(TTPC_MODS, f"{n}.mod")).read() for n in TTPC}
+
+
+@pytest.fixture(scope="module")
+def ttpc(tmp_path_factory):
+    src = _ttpc_sources()
+    _register_defaults(src)
+    return _build(tmp_path_factory.mktemp("ttpc"), src)
+
+
+def _close(got, want, rel):
+    return abs(got - want) <= rel * max(1.0, abs(want))
+
+
+def test_ttpc_rates_match_the_mod_fixture(ttpc):
+    z = np.load(os.path.join(GOLDEN, "mod_rates_ttpc.npz"))
+    for mech in ("Ca_HVA", "Ca_LVAst", "Ih", "Im", "K_Pst", "K_Tst", "Nap_Et2", "NaTa_t", "NaTs2_t", "SKv3_1"):
+        keys, want = list(z[f"{mech}_keys"]), z[mech]
+        for i, v in enumerate(z["v"]):
+            out = run_block(ttpc[mech], 0, float(v))
+            for k, key in enumerate(keys):
+                assert _close(out[key], want[i, k], 1e-12), (mech, float(v), key, out[key], want[i, k])
+    for i, ca in enumerate(z["ca"]):
+        out = run_block(ttpc["SK_E2"], 0, -70.0, cai=float(ca))
+        assert _close(out["zInf"], z["SK_E2"][i, 0], 1e-12), (float(ca), out["zInf"])
+
+
+def test_ttpc_cnexp_matches_the_exact_steps(ttpc):
+    z = np.load(os.path.join(GOLDEN, "mod_derivs_ttpc.npz"))
+    mechs = sorted({k[:-len("_inputs")] for k in z.files if k.endswith("_inputs")})
+    assert "CaDynamics_E2" in mechs and "Ih" in mechs
+    for mech in mechs:
+        names, states, X, Y = list(z[f"{mech}_inputs"]), list(z[f"{mech}_states"]), z[f"{mech}_x"], z[f"{mech}_y"]
+        for j in range(X.shape[0]):
+            vals = dict(zip(names, map(float, X[j])))
+            out = run_block(ttpc[mech], 1, vals.pop("v"), vals.pop("dt"), **vals)
+            for i, s in enumerate(states):
+                assert _close(out[s], Y[j, i], 1e-11), (mech, j, s, out[s], Y[j, i])
+
+
+def test_ttpc_currents_match_the_fixture(ttpc):
+    z = np.load(os.path.join(GOLDEN, "mod_currents.npz"))
+    for key in z.files:
+        if not (key.startswith("ttpc:") and key.endswith("_inputs")):
+            continue
+        mech = key[len("ttpc:"):-len("_inputs")]
+        names, X, I = list(z[key]), z[f"ttpc:{mech}_x"], z[f"ttpc:{mech}_i"]
+        cur = nmodl.interface(nmodl.parse(open(os.path.join(TTPC_MODS, f"{mech}.mod")).read()))["currents"][0]
+        for j in range(X.shape[0]):
+            # the reversal potentials the fixture holds for the TTPC cell (ek, ena)
+            vals = dict({"ek": -85.0, "ena": 50.0}, **dict(zip(names, map(float, X[j]))))
+            out = run_block(ttpc[mech], 2, vals.pop("v"), 0.0, **vals)
+            assert _close(out[cur], I[j], 1e-13), (mech, j, out[cur], I[j])
+
+
+def test_ttpc_generated_equals_the_device_forms(ttpc, tmp_path):
+    """The generated (inf, tau) of every gate against mechs.h's (inf, 1/tau)
+    (the forms the simulation kernel evaluates), on a voltage sweep."""
+    lib = tmp_path / "libmechs_host.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-I", CSRC,
+                    os.path.join(ROOT, "tests", "mechs_host.cpp"), "-o", str(lib)], check=True)
+    H = C.CDLL(str(lib))
+    H.host_rates.restype = C.c_int
+    H.host_rates.argtypes = [C.c_int, C.c_int, C.c_double, _DP, _DP]
+    order = {"Ca_HVA": 0, "Ca_LVAst": 1, "Ih": 3, "Im": 4, "K_Pst": 5, "K_Tst": 6, "Nap_Et2": 7, "NaTa_t": 8,
+             "NaTs2_t": 9, "SKv3_1": 12}
+    P = np.zeros(40)
+    worst = 0.0
+    for mech, idx in order.items():
+        gates = ["m", "h"] if ttpc[mech][1]["states"] == ["m", "h"] else ["m"]
+        for v in np.linspace(-120.0, 60.0, 361) + 0.0037:
+            out = run_block(ttpc[mech], 0, float(v))
+            dev = np.zeros(4)
+            assert H.host_rates(1, idx, float(v), P.ctypes.data_as(_DP), dev.ctypes.data_as(_DP)) == 0
+            for k, g in enumerate(gates):
+                assert _close(out[f"{g}Inf"], dev[2 * k], 1e-12), (mech, v, g)
+                worst = max(worst, abs(out[f"{g}Tau"] * dev[2 * k + 1] - 1.0))
+    assert worst < 1e-12, worst
+
+
+def test_cli_writes_the_header_and_interface(tmp_path):
+    srcs = _ttpc_sources()
+    paths = []
+    for name in ("Ih.mod", "SK_E2.mod"):
+        p = tmp_path / name
+        p.write_text(srcs[name])
+        paths.append(str(p))
+    out, face = tmp_path / "gen.h", tmp_path / "gen.json"
+    assert nmodl.main(["-o", str(out), "--interface", str(face)] + paths) == 0
+    text = out.read_text()
+    assert "struct Mech_Ih" in text and "struct Mech_SK_E2" in text and "ng_gen_" not in text
+    import json
+    assert [f["suffix"] for f in json.loads(face.read_text())] == ["Ih", "SK_E2"]
+
+
+def test_generated_code_compiles_for_gfx950(tmp_path):
+    """The generated structs are device code: a kernel that runs every block
+    of the test mechanism compiles for gfx950."""
+    hipcc = "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        pytest.skip("hipcc not available")
+    code, _ = nmodl.generate({"hhtest.mod": HHTEST})
+    (tmp_path / "gen.h").write_text(code)
+    (tmp_path / "k.hip").write_text('''#include "gen.h"
+__global__ void k(double *v, double *out, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Mech_hhtest m;
+    m.v = v[i]; m.dt = 0.025;
+    m.initial();
+    m.states();
+    m.breakpoint();
+    out[i] = m.n + m.c + m.ik;
+}
+''')
+    r = subprocess.run([hipcc, "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-Xclang",
+                        "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", CSRC, "-I", str(tmp_path), "-c",
+                        str(tmp_path / "k.hip"), "-o", str(tmp_path / "k.o")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
