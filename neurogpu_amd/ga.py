@@ -295,7 +295,7 @@ def run_ga(evaluator: Evaluator, mu: int, lam: int, ngen: int, seed: int = 1178,
     selection comes from the simulator.  ``n_init`` random individuals
     (default ``mu``) are simulated first and the best ``mu`` of them form the
     first population; ``init`` = (genomes, simulated scores[, objective
-    rows]) gives the first population instead (individuals already simulated,
+    rows) gives the first population instead (individuals already simulated,
     e.g. the surrogate's training set).
 
     ``selector="ibea"``: the loop of bluepyopt's DEAPOptimisation with its
@@ -366,8 +366,9 @@ def run_ga(evaluator: Evaluator, mu: int, lam: int, ngen: int, seed: int = 1178,
         start = int(cp["gen"]) + 1
     else:
         if init is not None:
-            pop = np.asarray(init[0])
-            obj = np.asarray(init[2]) if len(init) > 2 else np.asarray(init[1])[:, None]
+            if len(init) < 3:
+                raise ValueError("init = (genomes, scores, objective rows): the objective rows enter selection")
+            pop, obj = np.asarray(init[0]), np.asarray(init[2])
             note_best(pop, obj)
         else:
             pop = rng.uniform(lo, hi, size=(n_init or mu, lo.size))

@@ -71,7 +71,7 @@ def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, see
         # offline training set: seeded log-uniform samples, simulated in batches
         # of one population; its time counts towards the screened GA's
         prng = np.random.default_rng(seed + 7919)
-        pre_g, pre_s = [], []
+        pre_g, pre_s, pre_o = [], [], []
         for b0 in range(0, pretrain, pop):
             g = prng.uniform(lo, hi, size=(min(pop, pretrain - b0), lo.size))
             obj, feats = ev.evaluate(10 ** g)
@@ -79,9 +79,10 @@ def study(name="ttpc1200", pop=1024, gens=20, nt=2000, members=5, epochs=60, see
             data_y.append(feature_targets(feats))
             pre_g.append(g)
             pre_s.append(obj.sum(axis=1))
+            pre_o.append(obj)
             say(f"pretraining set: {b0 + g.shape[0]} of {pretrain} simulated")
         # the screened GA starts from the best of these simulated individuals
-        init = (np.vstack(pre_g), np.concatenate(pre_s))
+        init = (np.vstack(pre_g), np.concatenate(pre_s), np.vstack(pre_o))
         t = time.perf_counter()
         losses = ens.fit(np.vstack(data_g), np.vstack(data_y), epochs=epochs, batch=512, lr=2e-3)
         fit.update(s=time.perf_counter() - t, n=1, loss=float(losses[:, -1].mean()))

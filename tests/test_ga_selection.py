@@ -309,3 +309,18 @@ def test_deap_optimisation_facade(tmp_path):
     assert [r["gen"] for r in log2] == [4, 5] and len(hist2) == 6
     with pytest.raises(ValueError):
         ga.DEAPOptimisation(_ZDT1(), selector_name="NSGA2")
+
+
+@pytest.mark.parametrize("selector", ["sum", "ibea"])
+def test_init_population_of_simulated_individuals(selector):
+    """The screened GA starts from individuals already simulated (genomes,
+    their summed scores and objective rows); the best of them is kept."""
+    ev = _ZDT1()
+    g = np.random.default_rng(4).uniform(0, 1, size=(50, 6))
+    obj, _ = ev.evaluate(10 ** g)
+    ev.nevals = 0
+    res = ga.run_ga(ev, mu=20, lam=20, ngen=3, seed=1, selector=selector, init=(g, obj.sum(axis=1), obj))
+    assert ev.nevals == 60
+    assert res.best_score <= obj.sum(axis=1).min()
+    with pytest.raises(ValueError):
+        ga.run_ga(_ZDT1(), mu=20, lam=20, ngen=1, init=(g, obj.sum(axis=1)))
