@@ -62,6 +62,21 @@
 
 #define NG_EPS_V 0.001
 
+/* Per-phase cycle counters, compiled in only with -DNG_PHASE_CYCLES=1 (the
+ * profiling build, `make phases`): thread 0 of every workgroup adds the
+ * cycles (s_memtime) of each phase of each step -- mechanism pass,
+ * elimination, pivot quotients, back-substitution, v/dv update and trace --
+ * and the number of steps to ng_phase_cycles[0..5]. */
+#ifndef NG_PHASE_CYCLES
+#define NG_PHASE_CYCLES 0
+#endif
+#if NG_PHASE_CYCLES
+static __device__ unsigned long long ng_phase_cycles[8];
+#define NG_STAMP(var) const unsigned long long var = clock64()
+#else
+#define NG_STAMP(var)
+#endif
+
 struct SimArgs {
     int N;                               /* segments */
     int NQ;                              /* positions per run: N rounded up to a multiple of 64 */
@@ -369,11 +384,15 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
     sync.mem();
 
     double dtp = a.t0 > 0 ? a.dts[a.t0 - 1] : 0.0;
+#if NG_PHASE_CYCLES
+    unsigned long long ph[6] = {0, 0, 0, 0, 0, 0};
+#endif
 
     for (int t = a.t0; t < a.t1; t++) {
         const double dt = a.dts[t];
         const double cm_dt = ng_rcp(dt * 1000);           /* cm / (dt * 1000) as cm * cm_dt */
         const int slot_t = (t - a.t0) & 63;
+        NG_STAMP(T0);
 
         /* mechanism pass: fused cnexp(t-1) + BREAKPOINT(t) + Hines row
          * (CudaStuff.cu:263-305).  eca is stored by INITIAL only: at t = 0 it
@@ -480,6 +499,7 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
         }
         sync();
 
+        NG_STAMP(T1);
         /* triangularisation by levels (BeforeLU, CudaStuff.cu:31-69), with the
          * reference's arithmetic.  Level L first folds the fathers of level
          * L-1 that have their own phase, then eliminates its chains; a chain
@@ -646,6 +666,7 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
             sync();
         }
 
+        NG_STAMP(T2);
         /* pivot quotients of every row, in parallel: PX = b/u and PF = -f/u
          * (BKSUB1's first step, CudaStuff.cu:75-78) replace b and u; the
          * division-free elimination has stored them already */
@@ -661,6 +682,7 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
             sync();
         }
 
+        NG_STAMP(T3);
         /* back-substitution by chains, highest level first: a chain's last
          * row takes x of its parent row pe (a row of a higher level; pe = N
          * marks the root, x = PX), then x_i = PX_i + PF_i x_{i+1} down to the
@@ -722,6 +744,7 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
             first = next; k0 = k0n; nc = ncn; k0n = k0nn; ncn = ncnn;
         }
 
+        NG_STAMP(T4);
         /* v += x (CudaStuff.cu:312-315) and dv += x[parent] - x[self] for the
          * next step (CudaStuff.cu:287) */
         NgPos pu(g, LW, NQ);
@@ -749,7 +772,15 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
             }
         }
         sync.mem();
+#if NG_PHASE_CYCLES
+        NG_STAMP(T5);
+        ph[0] += T1 - T0; ph[1] += T2 - T1; ph[2] += T3 - T2; ph[3] += T4 - T3; ph[4] += T5 - T4; ph[5] += 1;
+#endif
     }
+#if NG_PHASE_CYCLES
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 6; k++) atomicAdd(&ng_phase_cycles[k], ph[k]);
+#endif
 }
 
 #ifndef NG_WAVES_PER_EU
