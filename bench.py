@@ -338,7 +338,7 @@ def main():
     ap.add_argument("--waves-per-run", type=int, default=0)
     ap.add_argument("--runs-per-group", type=int, default=0)
     ap.add_argument("--solve", default="auto", choices=("auto", "reference", "fast"),
-                    help="form of the Hines elimination (auto: division-free on well-scaled trees)")
+                    help="form of the Hines elimination (auto: reciprocal form on well-scaled trees)")
     ap.add_argument("--cpu-sets", type=int, default=None)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-copy-peak", action="store_true")
@@ -439,7 +439,7 @@ def main():
                    "runs": w.nruns, "compartments": m.tree.N, "nt": nt,
                    "launch": {k: st[k] for k in ("waves_per_run", "runs_per_group", "slots_per_lane",
                                                   "runs_per_block")},
-                   "solve": {1: "reference order", 2: "division-free"}[st["solve"]]},
+                   "solve": {1: "reference order", 2: "reciprocal form"}[st["solve"]]},
         "roofline": roofline,
     }
     if not args.no_cpu:

@@ -109,9 +109,10 @@ int ngpu_set_launch(ngpu_ctx *ctx, int waves_per_run, int runs_per_group,
  * NGPU_SOLVE_REFERENCE keeps the reference's operations in its order
  * (u_i -= f (e / u), b_i -= (b e) / u, two IEEE divisions a row), which its
  * exported trees need: they couple zero-area nodes with coefficients near
- * 1e21.  NGPU_SOLVE_FAST carries each chain's pivot as a fraction, one
- * dependent multiply-add a row instead of two divisions; results equal the
- * reference order's up to rounding on well-scaled trees.  NGPU_SOLVE_AUTO
+ * 1e21.  NGPU_SOLVE_FAST takes each row's pivot quotients once, from one
+ * refined reciprocal of the pivot, and folds the next row through them;
+ * results equal the reference order's up to rounding on well-scaled trees.
+ * NGPU_SOLVE_AUTO
  * (the default) takes the fast form when every cm > 0 and every |e|, |f| is
  * at most 1e6.  No reference counterpart (the reference has one form). */
 #define NGPU_SOLVE_AUTO 0

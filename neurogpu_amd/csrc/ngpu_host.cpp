@@ -43,7 +43,7 @@ hipError_t ngpu_launch_batch_mainen(dim3 grid, size_t lds, hipStream_t s, const 
 
 static std::string g_last_error;
 
-/* Largest |e| or |f| of a tree that takes the division-free elimination by
+/* Largest |e| or |f| of a tree that takes the reciprocal-form elimination by
  * default.  Generated morphologies stay below 100; the reference's exported
  * trees hold zero-area nodes (cm = 0) coupled with coefficients near 1e21,
  * where only the reference's own operation order reproduces its results. */
@@ -104,7 +104,7 @@ struct ngpu_ctx {
     int cfg_w = 0, cfg_r = 0, cfg_spl = 0, cfg_graph = 0;   /* waves and runs per group, 0 = auto */
     int cfg_stage = -1;            /* row coefficients: -1 auto, 0 global memory, 1 LDS */
     int cfg_solve = NGPU_SOLVE_AUTO;
-    bool well_scaled = false;      /* every cm > 0 and |e|, |f| <= kScaledCoef: the division-free
+    bool well_scaled = false;      /* every cm > 0 and |e|, |f| <= kScaledCoef: the reciprocal-form
                                       elimination is as accurate as the reference's order */
     /* device state */
     int lay_W = 0, lay_R = 0, lay_NQ = 0;
@@ -113,7 +113,7 @@ struct ngpu_ctx {
     DevArr<int> d_qlink, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_rec, d_msched, d_gsrc;
     std::vector<int> qmask;        /* mechanism mask by position */
     std::vector<int> msched;       /* mechanism-pass schedule of the last (W, R) */
-    int msched_W = 0, msched_R = 0;
+    int msched_R = 0;              /* 0: no schedule uploaded */
     DevArr<double> d_qcm, d_ef, d_pval, d_pconst, d_pv, d_dts, d_amps, d_states, d_vhot, d_fwin, d_feat, d_genome;
     int stim_q = -1;
     std::vector<int> rec_q;
@@ -414,7 +414,7 @@ extern "C" int ngpu_set_solve(ngpu_ctx *c, int mode)
     return NGPU_OK;
 }
 
-/* The elimination a run of c uses: the division-free form where chosen, or
+/* The elimination a run of c uses: the reciprocal form where chosen, or
  * by default where the tree is well scaled. */
 static bool fast_solve(const ngpu_ctx *c)
 {
@@ -630,7 +630,7 @@ static int build_tables(ngpu_ctx *c)
     }
     for (int q = N; q < NQ; q++) qlink[4 * q + 1] = 0;
     c->qmask = qmeta;
-    c->msched_W = 0;
+    c->msched_R = 0;
     /* sons in son order: the reference adds the children's terms to the
      * parent's row first for SonNoVec 1, then for SonNoVec 2
      * (CANON/CudaStuff.cu:296-305); other values add nothing */
@@ -701,7 +701,9 @@ static hipError_t launch_one(ngpu_ctx *c, const SimArgs &a, dim3 grid, dim3 bloc
     }
 }
 
-static const size_t kLdsLimit = 160 * 1024;
+/* LDS a workgroup may request: the CU's 160 KiB less the kernels' static
+ * part (the groups' segment counters and the batch kernel's item slot). */
+static const size_t kLdsLimit = 160 * 1024 - 128;
 
 /* LDS staging (row coefficients and the run's varying parameter values) is
  * considered for up to this many varying values per run. */
@@ -801,15 +803,15 @@ static int ensure_tables(ngpu_ctx *c)
     return NGPU_OK;
 }
 
-/* The mechanism pass's segment schedule for groups of W waves and R runs
- * (ng_mech_schedule, schedule.h), uploaded once per shape. */
-static int mech_schedule(ngpu_ctx *c, int W, int R)
+/* The mechanism pass's segment queue for groups of R runs
+ * (ng_mech_schedule, schedule.h), uploaded once per R. */
+static int mech_schedule(ngpu_ctx *c, int R)
 {
-    if (c->msched_W == W && c->msched_R == R) return NGPU_OK;
-    c->msched = ng_mech_schedule(c->qmask, c->N, c->lay_NQ, c->mechset, W, R);
+    if (c->msched_R == R) return NGPU_OK;
+    c->msched = ng_mech_schedule(c->qmask, c->N, c->lay_NQ, c->mechset, R);
     int rc = upload(c, c->d_msched, c->msched);
     if (rc) return rc;
-    c->msched_W = W; c->msched_R = R;
+    c->msched_R = R;
     return NGPU_OK;
 }
 
@@ -882,7 +884,7 @@ static int prepare_layout(ngpu_ctx *c, int W, int R)
     /* the schedule of the runs per group a launch will use (a shape that does
      * not fit is reported by the launch) */
     int Rp = R, G = 1, stage = c->cfg_stage;
-    if (plan(c, W, Rp, G, stage) == NGPU_OK && (rc = mech_schedule(c, W, Rp))) return rc;
+    if (plan(c, W, Rp, G, stage) == NGPU_OK && (rc = mech_schedule(c, Rp))) return rc;
     return NGPU_OK;
 }
 
@@ -1119,7 +1121,7 @@ try {
         const size_t need = lds_need(c, G * R, stage);
         if (need > kLdsLimit) return bfail(b, NGPU_E_ARG, "member %d needs %zu bytes of LDS per workgroup (limit 160 KiB)", i, need);
         lds = std::max(lds, need);
-        if ((rc = mech_schedule(c, W, R))) return bfail(b, rc, "member %d: %s", i, c->err.c_str());
+        if ((rc = mech_schedule(c, R))) return bfail(b, rc, "member %d: %s", i, c->err.c_str());
         fill_args(c, args[i], W, R, stage, sync);
         b->lay_w[i] = W; b->lay_r[i] = R; b->lay_sync[i] = sync; b->lay_stage[i] = stage;
         const int nruns = c->nsets * c->nstim, per_item = G * R;

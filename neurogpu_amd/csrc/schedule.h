@@ -3,17 +3,19 @@
  * pass (plain C++, no HIP), shared by the library and its CPU tests.
  *
  * The positions of a run form NQ/64 chunks of 64 (positions sorted by
- * mechanism mask).  A segment is (chunk, runs r0 .. r0+n-1 of a group); the
- * W waves of a group take the segments assigned to them.  A chunk's cost is
- * the sum over the distinct masks of its positions of (mechanisms + 1),
- * since a wave runs each mask's code in turn.  A chunk whose masks all have
- * specialised code is cut into segments of up to three runs (the kernel
- * loads a run's states while it computes the previous run); a chunk with a
- * run-time-mask position into single runs.  Segments go longest first to the
- * least loaded wave.
+ * mechanism mask).  A segment is (chunk, runs r0 .. r0+n-1 of a group).  A
+ * chunk's cost is the sum over the distinct masks of its positions of
+ * (mechanisms + 1), since a wave runs each mask's code in turn.  A chunk
+ * whose masks all have specialised code is cut into segments of up to three
+ * runs (the kernel loads a run's states while it computes the previous run);
+ * a chunk with a run-time-mask position into single runs.  The segments form
+ * one queue per group, longest first: each wave of the group takes the next
+ * segment when it has finished its last one (an LDS counter in the kernel),
+ * so the waves end the pass together whatever the costs really are, and the
+ * last segments taken are the cheapest.
  *
- * Layout of the result: offsets[W + 1] into the segment list, then the
- * segments packed as chunk << 8 | r0 << 4 | n (r0 < 16, n < 16).
+ * Layout of the result: the number of segments, then the segments packed as
+ * chunk << 8 | r0 << 4 | n (r0 < 16, n < 16).
  */
 #ifndef NGPU_SCHEDULE_H
 #define NGPU_SCHEDULE_H
@@ -31,9 +33,9 @@ inline bool ng_mask_fixed(int mechset, unsigned m)
     return m <= 1u;
 }
 
-/* qmask[q]: mechanism mask of position q (q < N); NQ a multiple of 64; W waves
- * per group, R runs per group (1..15). */
-inline std::vector<int> ng_mech_schedule(const std::vector<int> &qmask, int N, int NQ, int mechset, int W, int R)
+/* qmask[q]: mechanism mask of position q (q < N); NQ a multiple of 64; R
+ * runs per group (1..15). */
+inline std::vector<int> ng_mech_schedule(const std::vector<int> &qmask, int N, int NQ, int mechset, int R)
 {
     const int NC = NQ / 64;
     struct Seg { int packed; double cost; };
@@ -55,16 +57,8 @@ inline std::vector<int> ng_mech_schedule(const std::vector<int> &qmask, int N, i
         }
     }
     std::stable_sort(segs.begin(), segs.end(), [](const Seg &x, const Seg &y) { return x.cost > y.cost; });
-    std::vector<double> load(W, 0.0);
-    std::vector<std::vector<int>> per(W);
-    for (const Seg &sg : segs) {
-        const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-        load[w] += sg.cost;
-        per[w].push_back(sg.packed);
-    }
-    std::vector<int> out(1, 0);
-    for (int w = 0; w < W; w++) out.push_back(out[w] + (int)per[w].size());
-    for (int w = 0; w < W; w++) out.insert(out.end(), per[w].begin(), per[w].end());
+    std::vector<int> out(1, (int)segs.size());
+    for (const Seg &sg : segs) out.push_back(sg.packed);
     return out;
 }
 

@@ -94,8 +94,8 @@ struct SimArgs {
     const double *pv;                    /* [nsets*nvar] varying parameter values */
     unsigned pconst_mask;                /* bit j: parameter j has one value wherever it is used */
     const double *pconst;                /* [NPARAM] those values */
-    const int *msched;                   /* mechanism-pass segments by wave: offsets[W+1], then
-                                            chunk << 8 | r0 << 4 | n */
+    const int *msched;                   /* mechanism-pass segment queue: the count, then the
+                                            segments chunk << 8 | r0 << 4 | n, longest first */
     const int *chains, *lvl_ptr;         /* chains: (i0, i1, tA, tB, pe); lvl_ptr[depth+2] */
     const int *fath, *flvl_ptr, *fchild; /* fathers with their own phase: (J, c0, c1); flvl_ptr[depth+1] */
     int lds_staging;                     /* row coefficients e, f and the runs' varying
@@ -106,7 +106,7 @@ struct SimArgs {
     int nchains, nfath, nfchild;         /* schedule sizes (staged into LDS as int16) */
     int tail_level;                      /* levels tail_level..depth hold one chain each and no
                                             fathers of their own (depth + 1: none) */
-    int fast_solve;                      /* division-free elimination (well-scaled trees);
+    int fast_solve;                      /* reciprocal-form elimination (well-scaled trees);
                                             0: the reference's operation order */
     const double *dts, *amps;
     const int *rec_q;
@@ -284,7 +284,8 @@ __device__ __forceinline__ void ng_fold(double ut, double bt, double e, double f
 }
 
 /* Steps t0..t1 of runs run0 .. run0 + R - 1 by the LW lanes of a group
- * (lane g).  RG is the first of the group's R per-run LDS regions.  Runs from
+ * (lane g).  RG is the first of the group's R per-run LDS regions; qctr is
+ * the group's segment counter in LDS (the mechanism pass's queue).  Runs from
  * nruns up to the allocated count are padding (their parameters are the last
  * set's): partial workgroups simulate them instead of leaving lanes idle. */
 /* tk / n for 0 <= tk < 8 n and 1 <= n < 32768 (a task's run): (tk + 1/2) / n
@@ -328,7 +329,7 @@ struct NgPos {
 
 template <class SET, bool STAGE, class SYNC>
 __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, double *RG, int run0, int g,
-                                         const SYNC &sync)
+                                         const SYNC &sync, unsigned *qctr)
 {
     const int LW = a.LW, N = a.N, NQ = a.NQ, R = a.runs_per_group;
     constexpr int NS = SET::NSTATE + 2, SV = SET::NSTATE, SDV = SET::NSTATE + 1;
@@ -353,6 +354,7 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
         else return a.pv + (size_t)min((run0 + r) / a.nstim, nsets - 1) * a.nvar;
     };
 
+    if (g == 0) *qctr = 0u;
     if constexpr (STAGE)
         for (int i = g; i < R * a.nvar; i += LW) {
             const int r = i / a.nvar, k = i - r * a.nvar;
@@ -398,16 +400,22 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
          * (CudaStuff.cu:263-305).  eca is stored by INITIAL only: at t = 0 it
          * is loaded, afterwards it is recomputed from cai after the cnexp. */
         const unsigned keep = (t > 0) ? ~(1u << SET::ECA) : ~0u;
-        /* Wave wv takes the segments (chunk c of 64 positions, runs r0 ..
-         * r0+n-1) that the host's schedule gives it: it walks the runs of a
-         * chunk in turn with the chunk's mechanism mask, loading the next
-         * run's state while it computes the current one.  Parameter entries
-         * are the same for every run and are read once per segment. */
-        const int W = LW >> 6, wv = __builtin_amdgcn_readfirstlane(g >> 6);
-        const int s1 = a.msched[wv + 1];
+        /* The waves of the group take the segments (chunk c of 64 positions,
+         * runs r0 .. r0+n-1) from the group's queue, longest first, each the
+         * next one when it has finished its last (an LDS counter), so that
+         * they end the pass together.  A wave walks the runs of a chunk in
+         * turn with the chunk's mechanism mask, loading the next run's state
+         * while it computes the current one.  Parameter entries are the same
+         * for every run and are read once per segment. */
+        const int nseg = a.msched[0];
 #pragma unroll 1
-        for (int si = a.msched[wv]; si < s1; si++) {
-            const int d = a.msched[W + 1 + si];
+        for (;;) {
+            const unsigned lane = __lane_id(), lead = __builtin_amdgcn_readfirstlane(lane);
+            unsigned sv = 0u;
+            if (lane == lead) sv = __hip_atomic_fetch_add(qctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            const int si = (int)__builtin_amdgcn_readfirstlane(sv);
+            if (si >= nseg) break;
+            const int d = a.msched[1 + si];
             const int c = d >> 8, r0 = (d >> 4) & 15, n = d & 15;
             const int q = (c << 6) + (g & 63);
             if (q >= N) continue;
@@ -498,6 +506,7 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
             });
         }
         sync();
+        if (g == 0) *qctr = 0u;                           /* every wave has taken its last segment */
 
         NG_STAMP(T1);
         /* triangularisation by levels (BeforeLU, CudaStuff.cu:31-69), with the
@@ -542,77 +551,59 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                 ef = ef1; pn = pn1; ef1 = ef2; pn1 = pn2;
             }
         };
-        /* The division-free form of the same elimination (fast_solve, for
-         * trees whose coefficients are well scaled): the chain keeps its
-         * running pivot as a fraction u = p / q and its right-hand side as
-         * b = B / q, so that the next row is
-         *     p' = d p - e f q,   B' = rhs p - e B,   q' = p
-         * (one dependent FMA per row instead of two divisions), and the
-         * finished row's pivot quotients PX = b / u = B / p and
-         * PF = -f / u = -f q / p are computed off that dependency and
-         * stored in place of (b, u).  Related rows are folded through their
-         * quotients: u_J -= f e / u_t is u_J += e PF_t, and b_J -= e b_t / u_t
-         * is b_J -= e PX_t.  (p, q, B) are scaled by a power of two every 4
-         * rows (exact), so that p stays in range on long chains.  Row N of
-         * a run's region is not a row of the tree; it takes the stores of
-         * the rows a block computes past the end of its chain.  The
-         * results equal the reference order's up to rounding. */
+        /* The reciprocal form of the same elimination (fast_solve, for trees
+         * whose coefficients are well scaled): row i's pivot quotients
+         * PX = b w and PF = -f w with w = 1/u (the hardware reciprocal refined
+         * by two Newton steps) replace (b, u), and the next row is
+         * u' = d + e PF, b' = rhs - e PX, i.e. u_i -= f (e / u) and
+         * b_i -= (b e) / u with the quotients taken once.  Related rows are
+         * folded through their quotients the same way.  The results equal the
+         * reference order's up to rounding. */
         auto elim_fast = [&](const NgChain &c) {
             double2 *XU2 = (double2 *)region(c.r);
             const int i0 = c.i0, last = c.i1 - 1, tA = c.tA, tB = c.tB;
             const int ta = max(tA, 0), tb = max(tB, 0);
-            double2 p0 = XU2[i0 - 1], pa = XU2[ta], ea = EF[ta], pb = XU2[tb], eb = EF[tb];
-            const double fh = EF[i0 - 1].y;
-            __asm__ volatile("" : "+v"(p0.x), "+v"(p0.y), "+v"(pa.x), "+v"(pa.y), "+v"(ea.x), "+v"(ea.y),
-                             "+v"(pb.x), "+v"(pb.y), "+v"(eb.x), "+v"(eb.y));
-            double p = p0.y, B = p0.x, q = 1.0;
+            const double2 x = XU2[i0 - 1], pa = XU2[ta], ea = EF[ta], pb = XU2[tb], eb = EF[tb];
+            /* rows i .. i+7 of the chain at a time: row i needs its (e, f)
+             * and the next row's (rhs, d); all eight loads are issued before
+             * the first row is computed (indices clamped to the chain's last
+             * row; a clamped value is not used) */
+            int i = i0 - 1;
+            double2 ef[8], xn[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int j = min(i + k, last);
+                ef[k] = EF[j];
+                xn[k] = XU2[j + 1];
+            }
+            double u = x.y, b = x.x;
             if (tA >= 0) {
-                p = fma(ea.x, pa.y, p);
-                B = fma(-ea.x, pa.x, B);
+                u = fma(ea.x, pa.y, u);
+                b = fma(-ea.x, pa.x, b);
                 if (tB >= 0) {
-                    p = fma(eb.x, pb.y, p);
-                    B = fma(-eb.x, pb.x, B);
+                    u = fma(eb.x, pb.y, u);
+                    b = fma(-eb.x, pb.x, b);
                 }
             }
-            {
-                const double r = ng_rcp(p);
-                XU2[i0 - 1] = make_double2(B * r, -(fh * r));
-            }
-            /* Rows i .. i+3 per block, without branches: first the four
-             * recurrences (one dependent multiply-add each), then the four
-             * rows' quotients, which are independent of one another and of
-             * the recurrence, so that their reciprocal chains overlap
-             * instead of each one holding up the next row.  Row j reads the
-             * (e, f) of row j - 1, its own (rhs, d) and its own f; indices
-             * past the chain's last row are clamped to it, and those rows'
-             * results go to the unused row N of the run's region. */
-            for (int i = i0; i <= last; i += 4) {
-                double2 ef[4], xr[4];
-                double fr[4];
+            /* row i: w = 1/u, PX = b w, PF = -f_i w replace (b, u); the next
+             * row is u' = d + e_i PF, b' = rhs - e_i PX */
+            for (;;) {
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
+                for (int k = 0; k < 8; k++) {
+                    const double w = ng_rcp(u);
+                    const double PX = b * w, PF = -(ef[k].y * w);
+                    XU2[i] = make_double2(PX, PF);
+                    if (i >= last) return;
+                    u = fma(ef[k].x, PF, xn[k].y);
+                    b = fma(-ef[k].x, PX, xn[k].x);
+                    i++;
+                }
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
                     const int j = min(i + k, last);
-                    ef[k] = EF[j - 1];
-                    xr[k] = XU2[j];
-                    fr[k] = EF[j].y;
+                    ef[k] = EF[j];
+                    xn[k] = XU2[j + 1];
                 }
-                double P[4], Q[4], Bv[4];
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const double fq = ef[k].y * q;
-                    const double pn = fma(xr[k].y, p, -(ef[k].x * fq));
-                    const double bn = fma(-ef[k].x, B, xr[k].x * p);
-                    q = p; p = pn; B = bn;
-                    P[k] = p; Q[k] = q; Bv[k] = B;
-                }
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const double r = ng_rcp(P[k]);
-                    XU2[i + k <= last ? i + k : N] = make_double2(Bv[k] * r, -((fr[k] * Q[k]) * r));
-                }
-                int e2;
-                (void)frexp(p, &e2);
-                p = ldexp(p, -e2); q = ldexp(q, -e2); B = ldexp(B, -e2);
             }
         };
         const bool fast = a.fast_solve != 0;
@@ -669,7 +660,7 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
         NG_STAMP(T2);
         /* pivot quotients of every row, in parallel: PX = b/u and PF = -f/u
          * (BKSUB1's first step, CudaStuff.cu:75-78) replace b and u; the
-         * division-free elimination has stored them already */
+         * reciprocal-form elimination has stored them already */
         if (!fast) {
             NgPos rw(g, LW, N);
             for (int i = g; i < R * N; i += LW, rw.step()) {
@@ -686,37 +677,28 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
         /* back-substitution by chains, highest level first: a chain's last
          * row takes x of its parent row pe (a row of a higher level; pe = N
          * marks the root, x = PX), then x_i = PX_i + PF_i x_{i+1} down to the
-         * chain's first row, four rows at a time with the next four rows'
-         * quotients loaded ahead; x replaces PX */
+         * chain's first row, the quotients of eight rows loaded at a time;
+         * x replaces PX */
         auto bsub = [&](const NgChain &c) {
             double *XU = region(c.r);
             const double2 *XU2 = (const double2 *)XU;
-            const int h = c.i1 - 1, lo = c.i0 - 1, pe = c.pe;
-            const double2 ph = XU2[h];
-            const double xp = XU[2 * min(pe, N - 1)];
-            double2 a0 = XU2[max(h - 1, lo)], a1 = XU2[max(h - 2, lo)], a2 = XU2[max(h - 3, lo)],
-                    a3 = XU2[max(h - 4, lo)], b0, b1, b2, b3;
-            double x = ph.x;
-            if (pe < N) x = fma(ph.y, xp, x);
-            XU[2 * h] = x;
-            /* rows i .. i-3 from one register set while rows i-4 .. i-7 are
-             * loaded into the other (clamped to the chain; a clamped value
-             * is not used) */
-            int i = h - 1;
-            while (i >= lo) {
-                b0 = XU2[max(i - 4, lo)]; b1 = XU2[max(i - 5, lo)]; b2 = XU2[max(i - 6, lo)]; b3 = XU2[max(i - 7, lo)];
-                x = fma(a0.y, x, a0.x); XU[2 * i] = x;
-                if (i - 1 >= lo) { x = fma(a1.y, x, a1.x); XU[2 * (i - 1)] = x; }
-                if (i - 2 >= lo) { x = fma(a2.y, x, a2.x); XU[2 * (i - 2)] = x; }
-                if (i - 3 >= lo) { x = fma(a3.y, x, a3.x); XU[2 * (i - 3)] = x; }
-                i -= 4;
-                if (i < lo) break;
-                a0 = XU2[max(i - 4, lo)]; a1 = XU2[max(i - 5, lo)]; a2 = XU2[max(i - 6, lo)]; a3 = XU2[max(i - 7, lo)];
-                x = fma(b0.y, x, b0.x); XU[2 * i] = x;
-                if (i - 1 >= lo) { x = fma(b1.y, x, b1.x); XU[2 * (i - 1)] = x; }
-                if (i - 2 >= lo) { x = fma(b2.y, x, b2.x); XU[2 * (i - 2)] = x; }
-                if (i - 3 >= lo) { x = fma(b3.y, x, b3.x); XU[2 * (i - 3)] = x; }
-                i -= 4;
+            const int lo = c.i0 - 1;
+            int i = c.i1 - 1;
+            /* rows i .. i-7 loaded at once (clamped to the chain's first row;
+             * a clamped value is not used); the root's parent value is 0 */
+            double2 v[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) v[k] = XU2[max(i - k, lo)];
+            double x = c.pe < N ? XU[2 * c.pe] : 0.0;
+            for (;;) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    x = fma(v[k].y, x, v[k].x);
+                    XU[2 * i] = x;
+                    if (--i < lo) return;
+                }
+#pragma unroll
+                for (int k = 0; k < 8; k++) v[k] = XU2[max(i - k, lo)];
             }
         };
         /* the tail first (top of the tree), lane r for run r, then the
@@ -787,6 +769,9 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
 #define NG_WAVES_PER_EU 2
 #endif
 
+/* Most groups a workgroup holds (16 one-wave groups of 1,024 threads). */
+#define NG_MAX_GROUPS 16
+
 /* One morphology.  Workgroup b holds G = blockDim.x / LW groups of R runs:
  * runs b * G * R + group * R + (0 .. R-1).  MAXT is the largest workgroup
  * the instantiation accepts: 512 (up to two waves per SIMD, up to 256 VGPRs
@@ -797,6 +782,7 @@ __global__ void __launch_bounds__(MAXT) __attribute__((amdgpu_waves_per_eu(MAXT 
 ngpu_sim_kernel(SimArgs a)
 {
     extern __shared__ double lds[];
+    __shared__ unsigned qctr[NG_MAX_GROUPS];              /* the groups' segment counters */
     const int LW = a.LW, R = a.runs_per_group;
     const int G = blockDim.x / LW;
     const int group = __builtin_amdgcn_readfirstlane(threadIdx.x / LW);  /* LW is a multiple of 64 */
@@ -805,8 +791,8 @@ ngpu_sim_kernel(SimArgs a)
     __syncthreads();
     const int run0 = (blockIdx.x * G + group) * R;
     double *RG = T.runs + (size_t)group * R * ng_lds_per_run(a.N, STAGE ? a.nvar : 0, a.nrec);
-    if constexpr (MULTI) sim_runs<SET, STAGE>(a, T, RG, run0, g, SyncBlock{});
-    else sim_runs<SET, STAGE>(a, T, RG, run0, g, SyncWave{});
+    if constexpr (MULTI) sim_runs<SET, STAGE>(a, T, RG, run0, g, SyncBlock{}, qctr + group);
+    else sim_runs<SET, STAGE>(a, T, RG, run0, g, SyncWave{}, qctr + group);
 }
 
 /* Work queue of the batch kernel: item i runs runs [y, y + z) of morphology x.
@@ -829,6 +815,7 @@ ngpu_batch_kernel(BatchArgs b)
 {
     extern __shared__ double lds[];
     __shared__ int s_item;
+    __shared__ unsigned qctr[NG_MAX_GROUPS];
     for (;;) {
         __syncthreads();                 /* the previous item is done with LDS */
         if (threadIdx.x == 0) s_item = atomicAdd(b.head, 1);
@@ -849,12 +836,12 @@ ngpu_batch_kernel(BatchArgs b)
             const Tables T = stage_tables<true>(a, lds, G * R);
             __syncthreads();
             double *RG = T.runs + (size_t)group * R * ng_lds_per_run(a.N, a.nvar, a.nrec);
-            sim_runs<SET, true>(a, T, RG, run0 + group * R, g, sync);
+            sim_runs<SET, true>(a, T, RG, run0 + group * R, g, sync, qctr + group);
         } else {
             const Tables T = stage_tables<false>(a, lds, G * R);
             __syncthreads();
             double *RG = T.runs + (size_t)group * R * ng_lds_per_run(a.N, 0, a.nrec);
-            sim_runs<SET, false>(a, T, RG, run0 + group * R, g, sync);
+            sim_runs<SET, false>(a, T, RG, run0 + group * R, g, sync, qctr + group);
         }
         __syncthreads();
         if (threadIdx.x == 0) b.stamps[2 * it + 1] = wall_clock64();

@@ -1,6 +1,6 @@
 /*
  * sanitize_schedule.cpp -- the host-side schedule builders of
- * neurogpu_amd/csrc/schedule.h (mechanism-pass segments by wave, launch
+ * neurogpu_amd/csrc/schedule.h (mechanism-pass segment queue, launch
  * shape) over many synthetic position masks and shapes, for an
  * AddressSanitizer / UndefinedBehaviorSanitizer build.  Exits non-zero if a
  * schedule does not cover every (chunk, run) exactly once.  Test
@@ -22,18 +22,17 @@ int main()
             std::vector<int> mask(N);
             for (int q = 0; q < N; q++) mask[q] = (int)(g() % 5 == 0 ? g() % 8191 : (ms == 1 ? 0x1618 : ms == 2 ? 0x6f : 1));
             std::sort(mask.begin(), mask.end());
-            for (int W : {1, 2, 4, 8, 16})
-                for (int R = 1; R <= 8; R++) {
-                    const std::vector<int> s = ng_mech_schedule(mask, N, NQ, ms, W, R);
-                    std::vector<int> seen((size_t)(NQ / 64) * R, 0);
-                    for (int w = 0; w < W; w++)
-                        for (int i = s[w]; i < s[w + 1]; i++) {
-                            const int d = s[W + 1 + i], c = d >> 8, r0 = (d >> 4) & 15, n = d & 15;
-                            for (int j = r0; j < r0 + n; j++) seen[(size_t)c * R + j]++;
-                        }
-                    for (int v : seen) bad += v != 1;
-                    cases++;
+            for (int R = 1; R <= 15; R++) {
+                const std::vector<int> s = ng_mech_schedule(mask, N, NQ, ms, R);
+                std::vector<int> seen((size_t)(NQ / 64) * R, 0);
+                bad += s[0] != (int)s.size() - 1;
+                for (int i = 1; i < (int)s.size(); i++) {
+                    const int d = s[i], c = d >> 8, r0 = (d >> 4) & 15, n = d & 15;
+                    for (int j = r0; j < r0 + n; j++) seen[(size_t)c * R + j]++;
                 }
+                for (int v : seen) bad += v != 1;
+                cases++;
+            }
         }
     }
     for (long nruns : {1L, 100L, 3072L, 40960L})

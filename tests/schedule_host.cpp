@@ -6,12 +6,12 @@
  */
 #include "schedule.h"
 
-/* out receives the schedule (offsets[W+1] then packed segments); returns its
- * length, or -1 when cap is too small. */
-extern "C" int host_mech_schedule(const int *qmask, int N, int NQ, int mechset, int W, int R, int *out, int cap)
+/* out receives the schedule (the segment count, then the packed segments);
+ * returns its length, or -1 when cap is too small. */
+extern "C" int host_mech_schedule(const int *qmask, int N, int NQ, int mechset, int R, int *out, int cap)
 {
     std::vector<int> m(qmask, qmask + N);
-    const std::vector<int> s = ng_mech_schedule(m, N, NQ, mechset, W, R);
+    const std::vector<int> s = ng_mech_schedule(m, N, NQ, mechset, R);
     if ((int)s.size() > cap) return -1;
     for (size_t i = 0; i < s.size(); i++) out[i] = s[i];
     return (int)s.size();

@@ -62,23 +62,23 @@ def test_ttpc1200_small(launch):
     w = models.workload("ttpc1200", nsets=3, nt=1200)
     _say(f"case ttpc1200 small launch={launch}")
     gpu, st = simulate_workload(w, launch=launch)
-    assert st["solve"] == SOLVE_FAST                 # a generated tree: the division-free form by default
+    assert st["solve"] == SOLVE_FAST                 # a generated tree: the reciprocal form by default
     _check(gpu, _oracle(w), f"ttpc1200 {launch}")
 
 
 @pytest.mark.parametrize("name", ["active500", "ttpc1200", "passive64"])
 def test_both_eliminations_agree(name):
-    """The reference's operation order and the division-free form on the
+    """The reference's operation order and the reciprocal form on the
     same generated tree: both within the bar of the oracle, and within
     rounding of each other."""
     w = models.workload(name, nsets=3, nt=1500)
-    _say(f"case {name}: reference-order vs division-free elimination")
+    _say(f"case {name}: reference-order vs reciprocal-form elimination")
     ref_order, st1 = simulate_workload(w, solve=SOLVE_REFERENCE)
     fast, st2 = simulate_workload(w, solve=SOLVE_FAST)
     assert st1["solve"] == SOLVE_REFERENCE and st2["solve"] == SOLVE_FAST
     oracle = _oracle(w)
     _check(ref_order, oracle, f"{name} reference order")
-    _check(fast, oracle, f"{name} division-free")
+    _check(fast, oracle, f"{name} reciprocal form")
     d = float(np.max(np.abs(fast - ref_order)))
     _say(f"{name}: max|fast - reference order| = {d:.3e} mV")
     assert d <= 1e-8

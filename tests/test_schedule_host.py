@@ -1,9 +1,9 @@
 """The mechanism-pass schedule (neurogpu_amd/csrc/schedule.h), compiled for
-the host: for the workloads' trees and every launch shape, each (chunk of 64
-positions, run of the group) is covered by exactly one segment, segments of a
-chunk with run-time-mask positions cover one run, and the longest-first
-assignment keeps every wave's estimated cost within one segment of the
-mean."""
+the host: for the workloads' trees and every group size, each (chunk of 64
+positions, run of the group) is covered by exactly one segment of the queue,
+segments of a chunk with run-time-mask positions cover one run, and the queue
+is ordered by estimated cost, longest first (the waves take segments from it
+in that order)."""
 import ctypes as C
 import os
 import subprocess
@@ -25,7 +25,7 @@ def lib(tmp_path_factory):
                     os.path.join(ROOT, "tests", "schedule_host.cpp"), "-o", out], check=True)
     L = C.CDLL(out)
     L.host_mech_schedule.restype = C.c_int
-    L.host_mech_schedule.argtypes = [_IP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _IP, C.c_int]
+    L.host_mech_schedule.argtypes = [_IP, C.c_int, C.c_int, C.c_int, C.c_int, _IP, C.c_int]
     L.host_choose_shape.restype = None
     L.host_choose_shape.argtypes = [C.c_long, C.c_int, _IP, C.c_int, C.c_int, C.c_double, C.c_double, _IP]
     return L
@@ -39,10 +39,10 @@ def _positions_masks(name):
     return np.sort(masks, kind="stable").astype(np.int32), t.N, m.mechset_id
 
 
-def _schedule(lib, qmask, N, ms, W, R):
+def _schedule(lib, qmask, N, ms, R):
     NQ = (N + 63) // 64 * 64
     out = np.zeros(4096, np.int32)
-    n = lib.host_mech_schedule(qmask.ctypes.data_as(_IP), N, NQ, ms, W, R, out.ctypes.data_as(_IP), out.size)
+    n = lib.host_mech_schedule(qmask.ctypes.data_as(_IP), N, NQ, ms, R, out.ctypes.data_as(_IP), out.size)
     assert n > 0
     return out[:n], NQ
 
@@ -53,37 +53,34 @@ def _cost(qmask, N, k):
 
 
 @pytest.mark.parametrize("name", ["passive64", "active500", "ttpc1200"])
-@pytest.mark.parametrize("W,R", [(1, 1), (1, 2), (2, 3), (8, 6), (16, 6), (16, 4), (4, 8)])
-def test_every_chunk_and_run_once(lib, name, W, R):
+@pytest.mark.parametrize("R", [1, 2, 3, 4, 6, 8])
+def test_every_chunk_and_run_once(lib, name, R):
     qmask, N, ms = _positions_masks(name)
-    s, NQ = _schedule(lib, qmask, N, ms, W, R)
-    off, seg = s[:W + 1], s[W + 1:]
-    assert off[0] == 0 and off[-1] == seg.size and np.all(np.diff(off) >= 0)
+    s, NQ = _schedule(lib, qmask, N, ms, R)
+    assert s[0] == s.size - 1
     covered = np.zeros((NQ // 64, R), np.int32)
-    load = np.zeros(W)
-    for w in range(W):
-        for d in seg[off[w]:off[w + 1]]:
-            c, r0, n = d >> 8, (d >> 4) & 15, d & 15
-            assert 0 <= c < NQ // 64 and n >= 1 and r0 + n <= R
-            covered[c, r0:r0 + n] += 1
-            generic = any(int(m) not in FIXED[ms] for m in qmask[64 * c:min(64 * c + 64, N)])
-            assert n <= (1 if generic else 3)
-            load[w] += _cost(qmask, N, c) * n
+    costs = []
+    for d in s[1:]:
+        c, r0, n = d >> 8, (d >> 4) & 15, d & 15
+        assert 0 <= c < NQ // 64 and n >= 1 and r0 + n <= R
+        covered[c, r0:r0 + n] += 1
+        generic = any(int(m) not in FIXED[ms] for m in qmask[64 * c:min(64 * c + 64, N)])
+        assert n <= (1 if generic else 3)
+        costs.append(_cost(qmask, N, c) * n)
     np.testing.assert_array_equal(covered, 1)
-    biggest = max(_cost(qmask, N, k) for k in range(NQ // 64)) * 3
-    assert load.max() <= load.mean() + biggest
+    assert all(a >= b for a, b in zip(costs, costs[1:]))      # longest first
 
 
-def test_soma_chunk_runs_spread_over_waves(lib):
-    """The expensive soma/axon chunk of ttpc1200: its six runs land on six
-    different waves of a 16-wave group (one wave holding them all tripled the
-    mechanism pass)."""
+def test_soma_chunk_runs_come_early(lib):
+    """The expensive soma/axon chunk of ttpc1200 (run-time-mask code): its six
+    runs are six single-run segments among the first eight of the queue, so
+    that six waves start on them at the beginning of the pass."""
     qmask, N, ms = _positions_masks("ttpc1200")
-    s, NQ = _schedule(lib, qmask, N, 1, 16, 6)
-    off, seg = s[:17], s[17:]
+    s, NQ = _schedule(lib, qmask, N, 1, 6)
     last = NQ // 64 - 1
-    waves = [w for w in range(16) for d in seg[off[w]:off[w + 1]] if d >> 8 == last]
-    assert len(waves) == 6 and len(set(waves)) == 6
+    soma = [k for k, d in enumerate(s[1:]) if d >> 8 == last]
+    assert len(soma) == 6 and max(soma) < 8
+    assert all(s[1 + k] & 15 == 1 for k in soma)
 
 
 def _shape(lib, nruns, ws, run_kib, table_kib, r_only=0, ncu=256):
