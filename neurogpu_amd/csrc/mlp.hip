@@ -59,6 +59,7 @@ struct GemmArgs {
     int relu;
     /* element strides between the matrices of consecutive grid z (members) */
     long long sA, sB, sBias, sMask, sC;
+    int vec;                    /* set by gemm(): bit 0 A, bit 1 B read as 16-byte vectors */
 };
 
 template <bool TA, bool TB>
@@ -78,16 +79,34 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
 #pragma unroll
     for (int i = 0; i < 16; i++) acc[i] = 0.0f;
     const int am = m0 + row, bn = n0 + row;
-    /* operands of k-steps k0 .. k0 + 16 (zero beyond K or outside the matrix) */
+    /* operands of k-steps k0 .. k0 + 16 (zero beyond K or outside the matrix):
+     * a lane's eight k values are contiguous in memory when the operand is
+     * k-minor (A not transposed, B transposed), and then, where the host has
+     * checked 16-byte alignment and K % 8 == 0 (g.vec bit 0 for A, bit 1 for
+     * B), read as one 16-byte load */
     auto load = [&](int k0, bf16x8 &a, bf16x8 &b) {
+        const int kb = k0 + kh;
+        if (!TA && (g.vec & 1)) {
+            a = (am < g.M && kb < g.K) ? *(const bf16x8 *)(g.A + (size_t)am * g.lda + kb) : bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+        } else {
 #pragma unroll
-        for (int e = 0; e < 8; e++) {
-            const int k = k0 + kh + e;
-            uint16_t av = 0, bv = 0;
-            if (am < g.M && k < g.K) av = TA ? g.A[(size_t)k * g.lda + am] : g.A[(size_t)am * g.lda + k];
-            if (bn < g.N && k < g.K) bv = TB ? g.B[(size_t)bn * g.ldb + k] : g.B[(size_t)k * g.ldb + bn];
-            a[e] = (short)av;
-            b[e] = (short)bv;
+            for (int e = 0; e < 8; e++) {
+                const int k = kb + e;
+                uint16_t av = 0;
+                if (am < g.M && k < g.K) av = TA ? g.A[(size_t)k * g.lda + am] : g.A[(size_t)am * g.lda + k];
+                a[e] = (short)av;
+            }
+        }
+        if (TB && (g.vec & 2)) {
+            b = (bn < g.N && kb < g.K) ? *(const bf16x8 *)(g.B + (size_t)bn * g.ldb + kb) : bf16x8{0, 0, 0, 0, 0, 0, 0, 0};
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                const int k = kb + e;
+                uint16_t bv = 0;
+                if (bn < g.N && k < g.K) bv = TB ? g.B[(size_t)bn * g.ldb + k] : g.B[(size_t)k * g.ldb + bn];
+                b[e] = (short)bv;
+            }
         }
     };
     /* four k-steps in flight: a step's slot is refilled with the operands of
@@ -118,8 +137,17 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g)
     }
 }
 
-static hipError_t gemm(hipStream_t s, bool ta, bool tb, const GemmArgs &g, int nz)
+/* A k-minor operand (rows of ld elements, matrices `stride` elements apart)
+ * can be read eight k values at a time as one 16-byte load. */
+static bool vec_ok(const uint16_t *p, int K, int ld, long long stride)
 {
+    return K % 8 == 0 && ld % 8 == 0 && stride % 8 == 0 && ((uintptr_t)p & 15) == 0;
+}
+
+static hipError_t gemm(hipStream_t s, bool ta, bool tb, const GemmArgs &g0, int nz)
+{
+    GemmArgs g = g0;
+    g.vec = (!ta && vec_ok(g.A, g.K, g.lda, g.sA) ? 1 : 0) | (tb && vec_ok(g.B, g.K, g.ldb, g.sB) ? 2 : 0);
     dim3 grid((g.N + 63) / 64, (g.M + 63) / 64, nz), block(256);
     if (!ta && tb) hipLaunchKernelGGL((gemm_bf16_kernel<false, true>), grid, block, 0, s, g);
     else if (!ta && !tb) hipLaunchKernelGGL((gemm_bf16_kernel<false, false>), grid, block, 0, s, g);
@@ -128,8 +156,12 @@ static hipError_t gemm(hipStream_t s, bool ta, bool tb, const GemmArgs &g, int n
     return hipGetLastError();
 }
 
-__global__ void f32_to_bf16_kernel(const float *x, uint16_t *y, size_t n)
+/* member blockIdx.y: its n values at x + blockIdx.y * sx -> bf16 at
+ * y + blockIdx.y * sy */
+__global__ void f32_to_bf16_kernel(const float *x, uint16_t *y, size_t n, size_t sx, size_t sy)
 {
+    x += blockIdx.y * sx;
+    y += blockIdx.y * sy;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         y[i] = f2bf(x[i]);
 }
@@ -188,9 +220,14 @@ __global__ void __launch_bounds__(64 * COLSUM_RG) colsum_kernel(const float *x32
     }
 }
 
-__global__ void adam_kernel(float *w, const float *gr, float *m, float *v, uint16_t *w16, int n, float lr, float b1,
-                            float b2, float eps, float c1, float c2)
+/* member blockIdx.y: its n parameters at offset blockIdx.y * n, its bf16
+ * copy at blockIdx.y * n16 */
+__global__ void adam_kernel(float *w, const float *gr, float *m, float *v, uint16_t *w16, int n, int n16, float lr,
+                            float b1, float b2, float eps, float c1, float c2)
 {
+    const size_t o = blockIdx.y * (size_t)n;
+    w += o; gr += o; m += o; v += o;
+    if (w16) w16 += blockIdx.y * (size_t)n16;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float gi = gr[i];
         const float mi = b1 * m[i] + (1.0f - b1) * gi;
@@ -239,7 +276,10 @@ struct ngpu_mlp {
     /* per member: parameters in one fp32 buffer; offsets of W1 b1 W2 b2 W3 b3 W4 b4 */
     size_t off[8], nparam = 0;
     float *w = nullptr, *g = nullptr, *m = nullptr, *v = nullptr;  /* [members * nparam] */
-    uint16_t *w16 = nullptr;                                      /* bf16 copy (biases unused) */
+    size_t nparam16 = 0;                                          /* nparam rounded up to a multiple of 8 */
+    uint16_t *w16 = nullptr;                                      /* [members * nparam16] bf16 copy (biases
+                                                                     unused): every weight matrix 16-byte
+                                                                     aligned, so rows load as vectors */
     long long step = 0;
     /* scratch for batches of up to cap rows per member, members side by side */
     int cap = 0;
@@ -300,11 +340,13 @@ extern "C" int ngpu_mlp_create(int device, int nin, int hidden, int nout, int me
     size_t o = 0;
     for (int i = 0; i < 8; i++) { p->off[i] = o; o += sz[i]; }
     p->nparam = o;
+    p->nparam16 = (o + 7) & ~(size_t)7;
     const size_t tot = o * members;
     if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc((void **)&p->w, tot * 4) != hipSuccess || hipMalloc((void **)&p->g, tot * 4) != hipSuccess ||
         hipMalloc((void **)&p->m, tot * 4) != hipSuccess || hipMalloc((void **)&p->v, tot * 4) != hipSuccess ||
-        hipMalloc((void **)&p->w16, tot * 2) != hipSuccess || hipMalloc((void **)&p->loss, members * 4) != hipSuccess) {
+        hipMalloc((void **)&p->w16, p->nparam16 * members * 2) != hipSuccess ||
+        hipMalloc((void **)&p->loss, members * 4) != hipSuccess) {
         delete p;
         return NGPU_E_NOMEM;
     }
@@ -338,7 +380,8 @@ extern "C" int ngpu_mlp_set_weights(ngpu_mlp *p, const float *w)
     MCHK(p, hipSetDevice(p->device));
     const size_t tot = p->nparam * p->members;
     MCHK(p, hipMemcpy(p->w, w, tot * 4, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(256), dim3(256), 0, p->stream, p->w, p->w16, tot);
+    hipLaunchKernelGGL(f32_to_bf16_kernel, dim3(64, p->members), dim3(256), 0, p->stream, p->w, p->w16, p->nparam, p->nparam,
+                       p->nparam16);
     MCHK(p, hipMemsetAsync(p->m, 0, tot * 4, p->stream));
     MCHK(p, hipMemsetAsync(p->v, 0, tot * 4, p->stream));
     MCHK(p, hipStreamSynchronize(p->stream));
@@ -360,22 +403,22 @@ extern "C" int ngpu_mlp_get_weights(ngpu_mlp *p, float *w)
 static int forward(ngpu_mlp *p, int m0, int nm, int n, long long xs)
 {
     const float *W = p->w + (size_t)m0 * p->nparam;
-    const uint16_t *W16 = p->w16 + (size_t)m0 * p->nparam;
-    const long long P = (long long)p->nparam, H = p->H;
+    const uint16_t *W16 = p->w16 + (size_t)m0 * p->nparam16;
+    const long long P = (long long)p->nparam, P16 = (long long)p->nparam16, H = p->H;
     GemmArgs g{};
     const uint16_t *in = p->x16;
     int K = p->nin;
     long long sin = xs;
     for (int l = 0; l < 3; l++) {
         g = GemmArgs{n, (int)H, K, K, K, (int)H, in, W16 + p->off[2 * l], W + p->off[2 * l + 1], nullptr, nullptr, p->a16[l], 1,
-                     sin, P, P, 0, n * H};
+                     sin, P16, P, 0, n * H};
         MCHK(p, gemm(p->stream, false, true, g, nm));
         in = p->a16[l];
         K = (int)H;
         sin = n * H;
     }
     g = GemmArgs{n, p->nout, (int)H, (int)H, (int)H, p->nout, in, W16 + p->off[6], W + p->off[7], nullptr, p->y32, nullptr, 0,
-                 n * H, P, P, 0, (long long)n * p->nout};
+                 n * H, P16, P, 0, (long long)n * p->nout};
     MCHK(p, gemm(p->stream, false, true, g, nm));
     return 0;
 }
@@ -384,8 +427,9 @@ static int forward(ngpu_mlp *p, int m0, int nm, int n, long long xs)
  * nout against t32, slot j) into p->g; loss of slot j in p->loss[j] */
 static int backward(ngpu_mlp *p, int m0, int nm, int n)
 {
-    const uint16_t *W16 = p->w16 + (size_t)m0 * p->nparam;
+    const uint16_t *W16 = p->w16 + (size_t)m0 * p->nparam16;
     float *G = p->g + (size_t)m0 * p->nparam;
+    const long long P16 = (long long)p->nparam16;
     const int H = p->H, O = p->nout;
     const long long P = (long long)p->nparam, nH = (long long)n * H, nO = (long long)n * O;
     const int tot = n * O;
@@ -397,7 +441,7 @@ static int backward(ngpu_mlp *p, int m0, int nm, int n)
     MCHK(p, gemm(p->stream, true, false, g, nm));
     hipLaunchKernelGGL(colsum_kernel, dim3((O + 63) / 64, nm), dim3(64 * COLSUM_RG), 0, p->stream, p->dy32,
                        (const uint16_t *)nullptr, n, O, G + p->off[7], P);
-    g = GemmArgs{n, H, O, O, H, H, p->dy16, W16 + p->off[6], nullptr, p->a16[2], nullptr, p->d16[2], 0, nO, P, 0, nH, nH};
+    g = GemmArgs{n, H, O, O, H, H, p->dy16, W16 + p->off[6], nullptr, p->a16[2], nullptr, p->d16[2], 0, nO, P16, 0, nH, nH};
     MCHK(p, gemm(p->stream, false, false, g, nm));
     for (int l = 2; l >= 0; l--) {
         const uint16_t *prev = l > 0 ? p->a16[l - 1] : p->x16;
@@ -408,7 +452,7 @@ static int backward(ngpu_mlp *p, int m0, int nm, int n)
                            p->d16[l], n, H, G + p->off[2 * l + 1], P);
         if (l > 0) {
             g = GemmArgs{n, H, H, H, H, H, p->d16[l], W16 + p->off[2 * l], nullptr, p->a16[l - 1], nullptr, p->d16[l - 1], 0,
-                         nH, P, 0, nH, nH};
+                         nH, P16, 0, nH, nH};
             MCHK(p, gemm(p->stream, false, false, g, nm));
         }
     }
@@ -506,8 +550,8 @@ extern "C" int ngpu_mlp_train(ngpu_mlp *p, const float *x, const float *t, int n
             if ((rc = forward(p, 0, M, batch, (long long)batch * p->nin)) || (rc = backward(p, 0, M, batch))) return rc;
             step++;
             const float c1 = 1.0f - std::pow(b1, (float)step), c2 = 1.0f - std::pow(b2, (float)step);
-            hipLaunchKernelGGL(adam_kernel, dim3(1024), dim3(256), 0, p->stream, p->w, p->g, p->m, p->v, p->w16,
-                               (int)(p->nparam * M), lr, b1, b2, eps, c1, c2);
+            hipLaunchKernelGGL(adam_kernel, dim3(256, M), dim3(256), 0, p->stream, p->w, p->g, p->m, p->v, p->w16,
+                               (int)p->nparam, (int)p->nparam16, lr, b1, b2, eps, c1, c2);
             hipLaunchKernelGGL(add_scalar_kernel, dim3(1), dim3(64), 0, p->stream, p->loss, p->lacc + ep, M, epochs);
         }
         /* the next epoch rewrites the host row order */

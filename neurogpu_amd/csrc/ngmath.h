@@ -9,7 +9,9 @@
  * Methods (published; no vendor library code is reproduced):
  *  - exp: Cody & Waite argument reduction x = k*ln2 + r with ln2 split into a
  *    high part with trailing zero bits and a low part (W. J. Cody, W. Waite,
- *    "Software Manual for the Elementary Functions", 1980), then the Taylor
+ *    "Software Manual for the Elementary Functions", 1980; the split values
+ *    NG_LN2_HI / NG_LN2_LO and the log reduction below are the published
+ *    ones of Sun's freely distributable fdlibm, e_exp.c / e_log.c), then the Taylor
  *    series of e^r to degree 13 by Horner's rule (|r| <= ln2/2 makes the
  *    truncation error < 2^-57), then scaling by 2^k.
  *  - log: x = 2^k * m with m in [sqrt(1/2), sqrt(2)), f = m - 1, s = f/(2+f),

@@ -110,7 +110,8 @@ struct ngpu_ctx {
     int lay_W = 0, lay_R = 0, lay_NQ = 0;
     int ncu = 0;                   /* compute units of the device, once queried */
     bool tables_dirty = true, params_dirty = true, stim_dirty = true, graph_dirty = true;
-    DevArr<int> d_qlink, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_rec, d_msched, d_gsrc;
+    DevArr<int> d_qlink, d_qmeta, d_chains, d_lvl, d_fath, d_flvl, d_fchild, d_rec, d_msched, d_gsrc, d_genq;
+    int ngen = 0;                  /* run-time-mask positions (packed mechanism-pass segments) */
     std::vector<int> qmask;        /* mechanism mask by position */
     std::vector<int> msched;       /* mechanism-pass schedule of the last (W, R) */
     int msched_R = 0;              /* 0: no schedule uploaded */
@@ -228,7 +229,7 @@ extern "C" void ngpu_close(ngpu_ctx *c)
         (void)hipStreamSynchronize(c->stream);
     }
     drop_graph(c);
-    for (auto *d : {&c->d_qlink, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_rec, &c->d_msched, &c->d_gsrc}) d->release();
+    for (auto *d : {&c->d_qlink, &c->d_qmeta, &c->d_chains, &c->d_lvl, &c->d_fath, &c->d_flvl, &c->d_fchild, &c->d_rec, &c->d_msched, &c->d_gsrc, &c->d_genq}) d->release();
     for (auto *d : {&c->d_qcm, &c->d_ef, &c->d_pval, &c->d_pconst, &c->d_pv, &c->d_dts, &c->d_amps, &c->d_states, &c->d_vhot, &c->d_fwin, &c->d_feat, &c->d_genome}) d->release();
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
@@ -631,6 +632,10 @@ static int build_tables(ngpu_ctx *c)
     for (int q = N; q < NQ; q++) qlink[4 * q + 1] = 0;
     c->qmask = qmeta;
     c->msched_R = 0;
+    /* run-time-mask positions: run by the packed segments of the mechanism
+     * pass (ng_mech_schedule); bit 15 of the kernel's mask table marks them */
+    const std::vector<int> genq = ng_generic_positions(qmeta, N, c->mechset);
+    for (int q : genq) qmeta[q] |= 0x8000;
     /* sons in son order: the reference adds the children's terms to the
      * parent's row first for SonNoVec 1, then for SonNoVec 2
      * (CANON/CudaStuff.cu:296-305); other values add nothing */
@@ -684,8 +689,10 @@ static int build_tables(ngpu_ctx *c)
         (rc = upload(c, c->d_qcm, qcm)) || (rc = upload(c, c->d_ef, ef)) ||
         (rc = upload(c, c->d_pval, pval)) || (rc = upload(c, c->d_chains, c->chains)) ||
         (rc = upload(c, c->d_lvl, c->lvl_ptr)) || (rc = upload(c, c->d_fath, c->fath)) || (rc = upload(c, c->d_flvl, c->flvl_ptr)) ||
-        (rc = upload(c, c->d_fchild, c->fchild)) || (rc = upload(c, c->d_rec, c->rec_q)))
+        (rc = upload(c, c->d_fchild, c->fchild)) || (rc = upload(c, c->d_rec, c->rec_q)) ||
+        (rc = upload(c, c->d_genq, genq)))
         return rc;
+    c->ngen = (int)genq.size();
     c->lay_NQ = NQ;
     c->tables_dirty = false;
     return NGPU_OK;
@@ -746,6 +753,7 @@ static void fill_args(const ngpu_ctx *c, SimArgs &a, int W, int R, int staged, i
     a.pconst_mask = c->pconst_mask;
     a.pconst = c->d_pconst.p;
     a.msched = c->d_msched.p;
+    a.gen_q = c->d_genq.p; a.ngen = c->ngen;
     a.chains = c->d_chains.p; a.lvl_ptr = c->d_lvl.p; a.fath = c->d_fath.p; a.flvl_ptr = c->d_flvl.p;
     a.fchild = c->d_fchild.p; a.lds_staging = staged; a.block_sync = block_sync;
     a.fast_solve = fast_solve(c) ? 1 : 0;

@@ -3,19 +3,25 @@
  * pass (plain C++, no HIP), shared by the library and its CPU tests.
  *
  * The positions of a run form NQ/64 chunks of 64 (positions sorted by
- * mechanism mask).  A segment is (chunk, runs r0 .. r0+n-1 of a group).  A
- * chunk's cost is the sum over the distinct masks of its positions of
- * (mechanisms + 1), since a wave runs each mask's code in turn.  A chunk
- * whose masks all have specialised code is cut into segments of up to three
- * runs (the kernel loads a run's states while it computes the previous run);
- * a chunk with a run-time-mask position into single runs.  The segments form
- * one queue per group, longest first: each wave of the group takes the next
- * segment when it has finished its last one (an LDS counter in the kernel),
- * so the waves end the pass together whatever the costs really are, and the
- * last segments taken are the cheapest.
+ * mechanism mask).  A segment is (chunk, runs r0 .. r0+n-1 of a group): a
+ * wave runs the specialised code of the chunk's masks for those runs, in
+ * turn, loading a run's states while it computes the previous run.  A
+ * chunk's cost is the sum over its distinct masks of (mechanisms + 1), since
+ * a wave runs each mask's code in turn; chunks are cut into segments of up to
+ * three runs.  Positions whose mask has no specialised code (soma and axon
+ * segments; the run-time-mask code, which runs the union of its lanes'
+ * mechanisms whatever the lanes' masks) are not run by their chunk's
+ * segments: their (position, run) pairs over the group's R runs are packed
+ * 64 to a segment, lane i of packed segment k taking pair 64 k + i, so that
+ * the expensive code runs ceil(ngen R / 64) times a step instead of once per
+ * chunk and run.  The segments form one queue per group, longest first: each
+ * wave of the group takes the next segment when it has finished its last one
+ * (an LDS counter in the kernel), so the waves end the pass together whatever
+ * the costs really are, and the last segments taken are the cheapest.
  *
  * Layout of the result: the number of segments, then the segments packed as
- * chunk << 8 | r0 << 4 | n (r0 < 16, n < 16).
+ * chunk << 8 | r0 << 4 | n (r0 < 16, 1 <= n < 16), or k << 8 for packed
+ * segment k (n = 0).
  */
 #ifndef NGPU_SCHEDULE_H
 #define NGPU_SCHEDULE_H
@@ -33,6 +39,15 @@ inline bool ng_mask_fixed(int mechset, unsigned m)
     return m <= 1u;
 }
 
+/* The positions q < N whose mask has no specialised code, in order. */
+inline std::vector<int> ng_generic_positions(const std::vector<int> &qmask, int N, int mechset)
+{
+    std::vector<int> out;
+    for (int q = 0; q < N; q++)
+        if (!ng_mask_fixed(mechset, (unsigned)qmask[q])) out.push_back(q);
+    return out;
+}
+
 /* qmask[q]: mechanism mask of position q (q < N); NQ a multiple of 64; R
  * runs per group (1..15). */
 inline std::vector<int> ng_mech_schedule(const std::vector<int> &qmask, int N, int NQ, int mechset, int R)
@@ -40,22 +55,33 @@ inline std::vector<int> ng_mech_schedule(const std::vector<int> &qmask, int N, i
     const int NC = NQ / 64;
     struct Seg { int packed; double cost; };
     std::vector<Seg> segs;
+    std::vector<unsigned> gms;                   /* distinct masks of the run-time-mask positions */
+    int ngen = 0;
     for (int k = 0; k < NC; k++) {
         std::vector<unsigned> ms;
-        bool generic = false;
         for (int q = 64 * k; q < std::min(64 * k + 64, N); q++) {
             const unsigned m = (unsigned)qmask[q];
-            if (std::find(ms.begin(), ms.end(), m) == ms.end()) ms.push_back(m);
-            generic = generic || !ng_mask_fixed(mechset, m);
+            if (!ng_mask_fixed(mechset, m)) {
+                ngen++;
+                if (std::find(gms.begin(), gms.end(), m) == gms.end()) gms.push_back(m);
+            } else if (std::find(ms.begin(), ms.end(), m) == ms.end()) {
+                ms.push_back(m);
+            }
         }
+        if (ms.empty()) continue;
         double cost = 0.0;
         for (unsigned m : ms) cost += __builtin_popcount(m) + 1;
-        const int L = generic ? 1 : 3;
-        for (int r0 = 0; r0 < R; r0 += L) {
-            const int n = std::min(L, R - r0);
+        for (int r0 = 0; r0 < R; r0 += 3) {
+            const int n = std::min(3, R - r0);
             segs.push_back({k << 8 | r0 << 4 | n, cost * n});
         }
     }
+    /* the run-time-mask code runs about three times slower per mechanism than
+     * the straight-line code of a specialised mask, with every lane paying
+     * for the union of the lanes' masks */
+    double gcost = 0.0;
+    for (unsigned m : gms) gcost += 3.0 * (__builtin_popcount(m) + 1);
+    for (int k = 0; 64 * k < ngen * R; k++) segs.push_back({k << 8, gcost});
     std::stable_sort(segs.begin(), segs.end(), [](const Seg &x, const Seg &y) { return x.cost > y.cost; });
     std::vector<int> out(1, (int)segs.size());
     for (const Seg &sg : segs) out.push_back(sg.packed);

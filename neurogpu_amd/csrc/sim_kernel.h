@@ -85,7 +85,8 @@ struct SimArgs {
     int nvar, nrec, nstim, nt, nt_amps, t0, t1, nruns, depth;
     int stim_q;
     double stim_area, v_init;
-    const int *q_meta;                   /* [NQ] mechanism mask (0 for padding) */
+    const int *q_meta;                   /* [NQ] mechanism mask (0 for padding); bit 15: a
+                                            run-time-mask position */
     const int *q_link;                   /* [4*NQ] row, parent row, first- and second-son position
                                             (-1 = none); row -1 marks padding */
     const double *q_cm;                  /* [NQ] */
@@ -95,7 +96,10 @@ struct SimArgs {
     unsigned pconst_mask;                /* bit j: parameter j has one value wherever it is used */
     const double *pconst;                /* [NPARAM] those values */
     const int *msched;                   /* mechanism-pass segment queue: the count, then the
-                                            segments chunk << 8 | r0 << 4 | n, longest first */
+                                            segments chunk << 8 | r0 << 4 | n, or k << 8 for
+                                            packed segment k, longest first */
+    const int *gen_q;                    /* [ngen] the run-time-mask positions (packed segments) */
+    int ngen;
     const int *chains, *lvl_ptr;         /* chains: (i0, i1, tA, tB, pe); lvl_ptr[depth+2] */
     const int *fath, *flvl_ptr, *fchild; /* fathers with their own phase: (J, c0, c1); flvl_ptr[depth+1] */
     int lds_staging;                     /* row coefficients e, f and the runs' varying
@@ -149,7 +153,8 @@ struct SyncDyn {
  *                          16-byte access; npv = nvar rounded up to even when
  *                          staged, else 0 (regions stay 16-byte aligned)
  *   QL[NQ]                 short4: row, parent row, son positions
- *   QM[NQ]                 unsigned short: mechanism mask
+ *   QM[NQ]                 unsigned short: mechanism mask; bit 15: a run-time-mask
+ *                          position, run by the packed segments
  *   CH[8*nchains], FA[3*nfath], FC[nfchild], LP[depth+2], FLP[depth+1] short */
 __host__ __device__ inline size_t ng_lds_per_run(int N, int npv, int nrec)
 {
@@ -369,7 +374,7 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
             const int r = wave_run(ps.r), q = ps.q;
             if (q >= N) continue;
             double *S = states(r);
-            const unsigned m = QM[q];
+            const unsigned m = QM[q] & 0x7fffu;
             const ParamFetch Pf{a.pconst, a.pval, pvof(r), a.pconst_mask, q, NQ};
             double st[SET::NSTATE];
 #pragma unroll
@@ -406,7 +411,11 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
          * they end the pass together.  A wave walks the runs of a chunk in
          * turn with the chunk's mechanism mask, loading the next run's state
          * while it computes the current one.  Parameter entries are the same
-         * for every run and are read once per segment. */
+         * for every run and are read once per segment.  Positions whose mask
+         * has no specialised code skip their chunk's segments: packed
+         * segments run them, each lane one (position, run) pair, so the
+         * run-time-mask code (which costs the union of its lanes' masks)
+         * runs over 64 pairs at a time instead of once per chunk and run. */
         const int nseg = a.msched[0];
 #pragma unroll 1
         for (;;) {
@@ -416,12 +425,10 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
             const int si = (int)__builtin_amdgcn_readfirstlane(sv);
             if (si >= nseg) break;
             const int d = a.msched[1 + si];
-            const int c = d >> 8, r0 = (d >> 4) & 15, n = d & 15;
-            const int q = (c << 6) + (g & 63);
-            if (q >= N) continue;
-            /* the mechanism code is specialised for the set's common mechanism
-             * masks (straight-line code, independent gates interleaved) */
-            SET::dispatch(QM[q], [&](auto mask) {
+            /* segment code for position q, runs r0 .. r0+n-1: specialised for
+             * the set's common mechanism masks (straight-line code, independent
+             * gates interleaved), else the run-time-mask code */
+            auto seg = [&](auto mask, const int q, const int r0, const int n) {
                 constexpr bool fixed = !std::is_same<decltype(mask), unsigned>::value;
                 const unsigned m = mask;
                 const short4 l = QL[q];                       /* row, parent row, sons */
@@ -503,7 +510,21 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                         step(r0 + j, qp, cur, ParamFetch{a.pconst, a.pval, pvof(r0 + j), a.pconst_mask, qp, NQ});
                     }
                 }
-            });
+            };
+            if ((d & 15) == 0) {
+                /* packed segment: lane i takes (position, run) pair 64 k + i of
+                 * the group's run-time-mask positions */
+                const int p = (d >> 8) * 64 + (g & 63);
+                if (p >= a.ngen * R) continue;
+                const int r = p / a.ngen;
+                const int q = a.gen_q[p - r * a.ngen];
+                seg(QM[q] & 0x7fffu, q, r, 1);
+            } else {
+                const int q = ((d >> 8) << 6) + (g & 63);
+                if (q >= N || (QM[q] & 0x8000u)) continue;    /* run-time-mask positions: packed */
+                const int r0 = (d >> 4) & 15, n = d & 15;
+                SET::dispatch(QM[q] & 0x7fffu, [&](auto mask) { seg(mask, q, r0, n); });
+            }
         }
         sync();
         if (g == 0) *qctr = 0u;                           /* every wave has taken its last segment */

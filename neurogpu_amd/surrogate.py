@@ -227,3 +227,39 @@ def make_screen(ens: MLPEnsemble, target: Target, lo, hi, nstim: int, keep_frac:
             picked = np.concatenate([picked, rng.choice(rest, size=min(kr, rest.size), replace=False)])
         return picked
     return screen
+
+
+def bench(nin: int = 12, nout: int = 20, members: int = 5, ntrain: int = 65536, epochs: int = 10,
+          npredict: int = 163840, seed: int = 0) -> dict:
+    """Time the surrogate's two GPU workloads at the GA study's sizes: training
+    the ensemble on ``ntrain`` samples (batch 512) and ranking ``npredict``
+    candidates with every member.  Returns wall times and the GEMM FLOP
+    rate of each (2 M N K per product, forward and backward)."""
+    import time
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, size=(ntrain, nin)).astype(np.float32)
+    t = np.tanh(x @ rng.normal(size=(nin, nout))).astype(np.float32)
+    ens = MLPEnsemble(nin, nout, members=members, seed=seed)
+    H = ens.H
+    fwd = 2 * (nin * H + 2 * H * H + H * nout)             # flops per sample, forward
+    ens.train_raw(x[:4096], t[:4096], epochs=1, batch=512, lr=1e-3, seed=1)   # warm-up
+    t0 = time.perf_counter()
+    ens.train_raw(x, t, epochs=epochs, batch=512, lr=1e-3, seed=2)
+    train_s = time.perf_counter() - t0
+    steps = epochs * (ntrain // 512)
+    # a step: forward, then backward = two products per layer (the first layer's input gradient is skipped)
+    train_flops = steps * 512 * members * (fwd + 2 * fwd - 2 * nin * H)
+    xp = rng.uniform(-1, 1, size=(npredict, nin)).astype(np.float32)
+    ens.predict_raw(xp[:1024])
+    t0 = time.perf_counter()
+    ens.predict_raw(xp)
+    pred_s = time.perf_counter() - t0
+    ens.close()
+    return {"members": members, "train_samples": ntrain, "epochs": epochs, "train_s": train_s,
+            "train_gemm_tflops": train_flops / train_s / 1e12, "predict_rows": npredict, "predict_s": pred_s,
+            "predict_gemm_tflops": npredict * members * fwd / pred_s / 1e12}
+
+
+if __name__ == "__main__":
+    import json
+    print(json.dumps(bench()), flush=True)

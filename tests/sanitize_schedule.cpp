@@ -3,7 +3,9 @@
  * neurogpu_amd/csrc/schedule.h (mechanism-pass segment queue, launch
  * shape) over many synthetic position masks and shapes, for an
  * AddressSanitizer / UndefinedBehaviorSanitizer build.  Exits non-zero if a
- * schedule does not cover every (chunk, run) exactly once.  Test
+ * schedule does not cover every (chunk with specialised-mask positions, run)
+ * exactly once, or packs the run-time-mask positions into the wrong number of
+ * segments.  Test
  * infrastructure only.
  */
 #include <cstdio>
@@ -24,13 +26,21 @@ int main()
             std::sort(mask.begin(), mask.end());
             for (int R = 1; R <= 15; R++) {
                 const std::vector<int> s = ng_mech_schedule(mask, N, NQ, ms, R);
+                const int ngen = (int)ng_generic_positions(mask, N, ms).size();
                 std::vector<int> seen((size_t)(NQ / 64) * R, 0);
+                int npacked = 0;
                 bad += s[0] != (int)s.size() - 1;
                 for (int i = 1; i < (int)s.size(); i++) {
                     const int d = s[i], c = d >> 8, r0 = (d >> 4) & 15, n = d & 15;
+                    if (n == 0) { bad += c != npacked++; continue; }
                     for (int j = r0; j < r0 + n; j++) seen[(size_t)c * R + j]++;
                 }
-                for (int v : seen) bad += v != 1;
+                bad += npacked != (ngen * R + 63) / 64;
+                for (int k = 0; k < NQ / 64; k++) {
+                    bool fixed = false;
+                    for (int q = 64 * k; q < std::min(64 * k + 64, N); q++) fixed = fixed || ng_mask_fixed(ms, (unsigned)mask[q]);
+                    for (int j = 0; j < R; j++) bad += seen[(size_t)k * R + j] != (fixed ? 1 : 0);
+                }
                 cases++;
             }
         }

@@ -54,33 +54,46 @@ def _cost(qmask, N, k):
 
 @pytest.mark.parametrize("name", ["passive64", "active500", "ttpc1200"])
 @pytest.mark.parametrize("R", [1, 2, 3, 4, 6, 8])
-def test_every_chunk_and_run_once(lib, name, R):
+def test_every_position_and_run_once(lib, name, R):
+    """Chunk segments cover each (chunk, run) with specialised-mask positions
+    once, in segments of up to three runs; the packed segments cover the
+    (run-time-mask position, run) pairs, 64 a segment; the queue is ordered
+    by estimated cost, longest first."""
     qmask, N, ms = _positions_masks(name)
     s, NQ = _schedule(lib, qmask, N, ms, R)
     assert s[0] == s.size - 1
+    fixed = np.array([int(m) in FIXED[ms] for m in qmask])
+    ngen = int((~fixed).sum())
     covered = np.zeros((NQ // 64, R), np.int32)
+    packed = []
     costs = []
+    gcost = 3 * sum(bin(int(m)).count("1") + 1 for m in set(qmask[~fixed].tolist()))
     for d in s[1:]:
         c, r0, n = d >> 8, (d >> 4) & 15, d & 15
-        assert 0 <= c < NQ // 64 and n >= 1 and r0 + n <= R
+        if n == 0:
+            packed.append(c)
+            costs.append(gcost)
+            continue
+        assert 0 <= c < NQ // 64 and r0 + n <= R and n <= 3
+        assert fixed[64 * c:min(64 * c + 64, N)].any()
         covered[c, r0:r0 + n] += 1
-        generic = any(int(m) not in FIXED[ms] for m in qmask[64 * c:min(64 * c + 64, N)])
-        assert n <= (1 if generic else 3)
-        costs.append(_cost(qmask, N, c) * n)
-    np.testing.assert_array_equal(covered, 1)
+        costs.append(_cost(qmask[fixed], N, c) if False else
+                     sum(bin(int(m)).count("1") + 1 for m in set(qmask[64 * c:min(64 * c + 64, N)][fixed[64 * c:min(64 * c + 64, N)]].tolist())) * n)
+    has_fixed = np.array([fixed[64 * k:min(64 * k + 64, N)].any() for k in range(NQ // 64)])
+    np.testing.assert_array_equal(covered[has_fixed], 1)
+    np.testing.assert_array_equal(covered[~has_fixed], 0)
+    assert sorted(packed) == list(range((ngen * R + 63) // 64))
     assert all(a >= b for a, b in zip(costs, costs[1:]))      # longest first
 
 
-def test_soma_chunk_runs_come_early(lib):
-    """The expensive soma/axon chunk of ttpc1200 (run-time-mask code): its six
-    runs are six single-run segments among the first eight of the queue, so
-    that six waves start on them at the beginning of the pass."""
+def test_runtime_mask_positions_are_packed(lib):
+    """ttpc1200's 21 soma and axon segments (run-time-mask code) over a group's
+    six runs: two packed segments of 64 pairs, at the head of the queue, in
+    place of six single-run passes over their chunk."""
     qmask, N, ms = _positions_masks("ttpc1200")
     s, NQ = _schedule(lib, qmask, N, 1, 6)
-    last = NQ // 64 - 1
-    soma = [k for k, d in enumerate(s[1:]) if d >> 8 == last]
-    assert len(soma) == 6 and max(soma) < 8
-    assert all(s[1 + k] & 15 == 1 for k in soma)
+    assert int(sum(int(m) not in FIXED[1] for m in qmask)) == 21
+    assert [d & 15 for d in s[1:3]] == [0, 0] and all(d & 15 for d in s[3:])
 
 
 def _shape(lib, nruns, ws, run_kib, table_kib, r_only=0, ncu=256):
