@@ -682,7 +682,9 @@ static int build_tables(ngpu_ctx *c)
     c->rec_q.resize(c->rec.size());
     for (size_t r = 0; r < c->rec.size(); r++) c->rec_q[r] = pos[c->rec[r]];
     c->stim_q = c->stim_seg >= 0 ? pos[c->stim_seg] : -1;
-    std::vector<double> ef(2 * (size_t)N);        /* e, f interleaved by Hines row */
+    std::vector<double> ef(2 * (size_t)N + NG_ROW_PAD, 0.0); /* e, f interleaved by Hines row, then
+                                                               eight rows of zeros (read past a
+                                                               chain's end, not used) */
     for (int h = 0; h < N; h++) { ef[2 * h] = c->e[h]; ef[2 * h + 1] = c->f[h]; }
     int rc;
     if ((rc = upload(c, c->d_pconst, c->pconst)) || (rc = upload(c, c->d_qlink, qlink)) || (rc = upload(c, c->d_qmeta, qmeta)) ||

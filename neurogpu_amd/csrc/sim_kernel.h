@@ -90,7 +90,7 @@ struct SimArgs {
     const int *q_link;                   /* [4*NQ] row, parent row, first- and second-son position
                                             (-1 = none); row -1 marks padding */
     const double *q_cm;                  /* [NQ] */
-    const double *row_ef;                /* [2N]: e, f interleaved by Hines row */
+    const double *row_ef;                /* [2N + NG_ROW_PAD]: e, f interleaved by Hines row, then zeros */
     const double *pval;                  /* [NPARAM*NQ] value, or tagged NaN = varying column */
     const double *pv;                    /* [nsets*nvar] varying parameter values */
     unsigned pconst_mask;                /* bit j: parameter j has one value wherever it is used */
@@ -147,15 +147,22 @@ struct SyncDyn {
 
 /* LDS layout of a workgroup, in this order:
  *   EF[2N]                 e, f interleaved by Hines row, when staged
+ *   PAD[16]                eight rows of padding (read, never used: the solve's
+ *                          batches of eight rows are not clamped to the chain)
  *   per run (runs_per_block of them): XU[2(N+1)], PV[npv], REC[64*nrec]
  *                          doubles; XU holds (b, u), then (PX, PF), then
  *                          (x, PF) interleaved by row, so that a row is one
  *                          16-byte access; npv = nvar rounded up to even when
  *                          staged, else 0 (regions stay 16-byte aligned)
+ *   PAD[16]                eight rows of padding
  *   QL[NQ]                 short4: row, parent row, son positions
  *   QM[NQ]                 unsigned short: mechanism mask; bit 15: a run-time-mask
  *                          position, run by the packed segments
  *   CH[8*nchains], FA[3*nfath], FC[nfchild], LP[depth+2], FLP[depth+1] short */
+/* Doubles of padding before and after the runs' regions in LDS (and rows of
+ * zeros after the row coefficients in global memory): eight rows, the most a
+ * solve batch reads past either end of a chain. */
+#define NG_ROW_PAD 16
 __host__ __device__ inline size_t ng_lds_per_run(int N, int npv, int nrec)
 {
     return (size_t)(2 * (N + 1) + ((npv + 1) & ~1) + 64 * nrec);
@@ -165,7 +172,7 @@ __host__ __device__ inline size_t ng_lds_bytes(int N, int NQ, int runs, int npv,
 {
     const size_t nshort = 5 * (size_t)NQ + 8 * (size_t)nchains + 3 * (size_t)nfath + (size_t)nfchild +
                           (size_t)(depth + 2) + (size_t)(depth + 1);
-    const size_t dbl = (staged ? 2 * (size_t)N : 0) + (size_t)runs * ng_lds_per_run(N, staged ? npv : 0, nrec);
+    const size_t dbl = (staged ? 2 * (size_t)N : 0) + 2 * NG_ROW_PAD + (size_t)runs * ng_lds_per_run(N, staged ? npv : 0, nrec);
     return dbl * sizeof(double) + ((nshort * sizeof(short) + 7) / 8) * 8;
 }
 
@@ -253,12 +260,12 @@ __device__ __forceinline__ Tables stage_tables(const SimArgs &a, double *lds, in
     if constexpr (STAGE) {
         for (int i = threadIdx.x; i < 2 * N; i += blockDim.x) lds[i] = a.row_ef[i];
         T.EF = (const double2 *)lds;
-        T.runs = lds + 2 * N;
+        T.runs = lds + 2 * N + NG_ROW_PAD;
     } else {
         T.EF = (const double2 *)a.row_ef;
-        T.runs = lds;
+        T.runs = lds + NG_ROW_PAD;
     }
-    short4 *QL = (short4 *)(T.runs + (size_t)runs * ng_lds_per_run(N, npv, a.nrec));
+    short4 *QL = (short4 *)(T.runs + (size_t)runs * ng_lds_per_run(N, npv, a.nrec) + NG_ROW_PAD);
     unsigned short *QM = (unsigned short *)(QL + NQ);
     short *CH = (short *)(QM + NQ);
     short *FA = CH + 8 * a.nchains, *FC = FA + 3 * a.nfath, *LP = FC + a.nfchild, *FLP = LP + (a.depth + 2);
@@ -587,15 +594,16 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
             const double2 x = XU2[i0 - 1], pa = XU2[ta], ea = EF[ta], pb = XU2[tb], eb = EF[tb];
             /* rows i .. i+7 of the chain at a time: row i needs its (e, f)
              * and the next row's (rhs, d); all eight loads are issued before
-             * the first row is computed (indices clamped to the chain's last
-             * row; a clamped value is not used) */
+             * the first row is computed, at immediate offsets from one
+             * address (past the chain's last row they read other rows or the
+             * padding after the regions and the coefficients; those values
+             * are not used) */
             int i = i0 - 1;
             double2 ef[8], xn[8];
 #pragma unroll
             for (int k = 0; k < 8; k++) {
-                const int j = min(i + k, last);
-                ef[k] = EF[j];
-                xn[k] = XU2[j + 1];
+                ef[k] = EF[i + k];
+                xn[k] = XU2[i + k + 1];
             }
             double u = x.y, b = x.x;
             if (tA >= 0) {
@@ -621,9 +629,8 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                 }
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
-                    const int j = min(i + k, last);
-                    ef[k] = EF[j];
-                    xn[k] = XU2[j + 1];
+                    ef[k] = EF[i + k];
+                    xn[k] = XU2[i + k + 1];
                 }
             }
         };
@@ -705,11 +712,13 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
             const double2 *XU2 = (const double2 *)XU;
             const int lo = c.i0 - 1;
             int i = c.i1 - 1;
-            /* rows i .. i-7 loaded at once (clamped to the chain's first row;
-             * a clamped value is not used); the root's parent value is 0 */
+            /* rows i .. i-7 loaded at once, at immediate offsets from one
+             * address (below the chain's first row they read other rows or
+             * the padding before the regions; those values are not used);
+             * the root's parent value is 0 */
             double2 v[8];
 #pragma unroll
-            for (int k = 0; k < 8; k++) v[k] = XU2[max(i - k, lo)];
+            for (int k = 0; k < 8; k++) v[k] = XU2[i - k];
             double x = c.pe < N ? XU[2 * c.pe] : 0.0;
             for (;;) {
 #pragma unroll
@@ -719,7 +728,7 @@ __device__ __forceinline__ void sim_runs(const SimArgs &a, const Tables &T, doub
                     if (--i < lo) return;
                 }
 #pragma unroll
-                for (int k = 0; k < 8; k++) v[k] = XU2[max(i - k, lo)];
+                for (int k = 0; k < 8; k++) v[k] = XU2[i - k];
             }
         };
         /* the tail first (top of the tree), lane r for run r, then the
