@@ -30,6 +30,10 @@ extern "C" {
 #define NGPU_MECH_PASSIVE 0  /* pas                                               */
 #define NGPU_MECH_TTPC 1     /* BBP TTPC, 13 mechanisms (Code/OneKernel/src/AllModels.cu) */
 #define NGPU_MECH_MAINEN 2   /* ca cad kca km kv na pas (Figures/FigureS4_mainen/CParsed) */
+#define NGPU_MECH_TTPC_GEN 3 /* the TTPC set generated by neurogpu_amd.nmodl from the model's
+                                .mod files in Figures/Figure2_BBP_TTPC_clean (as
+                                PYS/extractModel.py generates AllModels.cu): the mechanisms,
+                                parameters and tree files of NGPU_MECH_TTPC */
 
 typedef struct ngpu_ctx ngpu_ctx;
 

@@ -68,6 +68,7 @@ class ngpu_stats(C.Structure):
 
 
 SOLVE_AUTO, SOLVE_REFERENCE, SOLVE_FAST = 0, 1, 2
+MECH_PASSIVE, MECH_TTPC, MECH_MAINEN, MECH_TTPC_GEN = 0, 1, 2, 3
 
 
 class ngpu_batch_launch(C.Structure):
@@ -314,11 +315,13 @@ class Simulator:
 
 
 def simulate_workload(w, device: int = 0, launch: Optional[dict] = None, fetch: bool = True,
-                      solve: int = SOLVE_AUTO):
-    """Run a :class:`models.Workload` on the GPU; returns (vhot [nsets,nstim,nrec,nt], stats)."""
+                      solve: int = SOLVE_AUTO, mechset: Optional[int] = None):
+    """Run a :class:`models.Workload` on the GPU; returns (vhot [nsets,nstim,nrec,nt], stats).
+    ``mechset`` overrides the model's mechanism set id with one of the same
+    mechanisms and parameter layout (MECH_TTPC_GEN for a TTPC model)."""
     m = w.model
     with Simulator(device) as sim:
-        sim.set_tree(m.tree, m.mechset_id)
+        sim.set_tree(m.tree, m.mechset_id if mechset is None else int(mechset))
         sim.set_sim(m.v_init, m.rec_sites)
         sim.set_params(w.params, m.ncomps, m.nparams)
         sim.set_stimuli(w.amps, w.dts, m.stim_seg, m.stim_area)
@@ -386,11 +389,12 @@ class Batch:
             pass
 
 
-def workload_simulator(w, device: int = 0) -> Simulator:
-    """A configured :class:`Simulator` for a :class:`models.Workload`."""
+def workload_simulator(w, device: int = 0, mechset: Optional[int] = None) -> Simulator:
+    """A configured :class:`Simulator` for a :class:`models.Workload` (``mechset``
+    as in :func:`simulate_workload`)."""
     m = w.model
     sim = Simulator(device)
-    sim.set_tree(m.tree, m.mechset_id)
+    sim.set_tree(m.tree, m.mechset_id if mechset is None else int(mechset))
     sim.set_sim(m.v_init, m.rec_sites)
     sim.set_params(w.params, m.ncomps, m.nparams)
     sim.set_stimuli(w.amps, w.dts, m.stim_seg, m.stim_area)

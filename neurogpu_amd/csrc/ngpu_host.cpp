@@ -26,6 +26,7 @@
 
 #include "../../include/neurogpu.h"
 #include "sim_kernel.h"
+#include "gen_ttpc.h"
 #include "schedule.h"
 
 /* Per-set launchers, one translation unit each (sim_<set>.hip). */
@@ -40,6 +41,8 @@ hipError_t ngpu_launch_mainen(bool multi, dim3 grid, dim3 block, size_t lds, hip
 hipError_t ngpu_launch_batch_passive(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
 hipError_t ngpu_launch_batch_ttpc(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
 hipError_t ngpu_launch_batch_mainen(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
+hipError_t ngpu_launch_ttpcgen(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
+hipError_t ngpu_launch_batch_ttpcgen(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
 
 static std::string g_last_error;
 
@@ -49,8 +52,16 @@ static std::string g_last_error;
  * where only the reference's own operation order reproduces its results. */
 static const double kScaledCoef = 1e6;
 
-static int mech_nmodels(int ms) { return ms == NGPU_MECH_TTPC ? SetTTPC::NMECH : ms == NGPU_MECH_MAINEN ? SetMainen::NMECH : ms == NGPU_MECH_PASSIVE ? SetPassive::NMECH : -1; }
-static int mech_nparams(int ms) { return ms == NGPU_MECH_TTPC ? SetTTPC::NPARAM : ms == NGPU_MECH_MAINEN ? SetMainen::NPARAM : ms == NGPU_MECH_PASSIVE ? SetPassive::NPARAM : -1; }
+static int mech_nmodels(int ms)
+{
+    return ms == NGPU_MECH_TTPC ? SetTTPC::NMECH : ms == NGPU_MECH_MAINEN ? SetMainen::NMECH
+         : ms == NGPU_MECH_PASSIVE ? SetPassive::NMECH : ms == NGPU_MECH_TTPC_GEN ? SetGen_ttpc::NMECH : -1;
+}
+static int mech_nparams(int ms)
+{
+    return ms == NGPU_MECH_TTPC ? SetTTPC::NPARAM : ms == NGPU_MECH_MAINEN ? SetMainen::NPARAM
+         : ms == NGPU_MECH_PASSIVE ? SetPassive::NPARAM : ms == NGPU_MECH_TTPC_GEN ? SetGen_ttpc::NPARAM : -1;
+}
 /* First parameter index of each mechanism of a set, plus the total (the
  * parameter layout of the generated AllModels.cuh). */
 static const int kPBaseTTPC[14] = {0, 1, 2, 6, 8, 9, 10, 11, 12, 13, 14, 16, 18, 19};
@@ -58,13 +69,18 @@ static const int kPBaseMainen[8] = {0, 2, 2, 6, 11, 16, 28, 30};
 static const int kPBasePassive[2] = {0, 2};
 static int param_owner(int ms, int j)
 {
-    const int *b = ms == NGPU_MECH_TTPC ? kPBaseTTPC : ms == NGPU_MECH_MAINEN ? kPBaseMainen : kPBasePassive;
+    const int *b = ms == NGPU_MECH_TTPC ? kPBaseTTPC : ms == NGPU_MECH_MAINEN ? kPBaseMainen
+                 : ms == NGPU_MECH_TTPC_GEN ? SetGen_ttpc::PBASE : kPBasePassive;
     int nm = mech_nmodels(ms);
     for (int m = 0; m < nm; m++)
         if (j >= b[m] && j < b[m + 1]) return m;
     return -1;
 }
-static int mech_nstate(int ms) { return ms == NGPU_MECH_TTPC ? SetTTPC::NSTATE : ms == NGPU_MECH_MAINEN ? SetMainen::NSTATE : SetPassive::NSTATE; }
+static int mech_nstate(int ms)
+{
+    return ms == NGPU_MECH_TTPC ? SetTTPC::NSTATE : ms == NGPU_MECH_MAINEN ? SetMainen::NSTATE
+         : ms == NGPU_MECH_TTPC_GEN ? SetGen_ttpc::NSTATE : SetPassive::NSTATE;
+}
 
 template <class T> struct DevArr {
     T *p = nullptr;
@@ -706,6 +722,7 @@ static hipError_t launch_one(ngpu_ctx *c, const SimArgs &a, dim3 grid, dim3 bloc
     switch (c->mechset) {
     case NGPU_MECH_TTPC: return ngpu_launch_ttpc(multi, grid, block, lds, c->stream, a);
     case NGPU_MECH_MAINEN: return ngpu_launch_mainen(multi, grid, block, lds, c->stream, a);
+    case NGPU_MECH_TTPC_GEN: return ngpu_launch_ttpcgen(multi, grid, block, lds, c->stream, a);
     default: return ngpu_launch_passive(multi, grid, block, lds, c->stream, a);
     }
 }
@@ -1170,6 +1187,7 @@ try {
     switch (b->mem[0]->mechset) {
     case NGPU_MECH_TTPC: le = ngpu_launch_batch_ttpc(dim3(grid), lds, b->stream, ba); break;
     case NGPU_MECH_MAINEN: le = ngpu_launch_batch_mainen(dim3(grid), lds, b->stream, ba); break;
+    case NGPU_MECH_TTPC_GEN: le = ngpu_launch_batch_ttpcgen(dim3(grid), lds, b->stream, ba); break;
     default: le = ngpu_launch_batch_passive(dim3(grid), lds, b->stream, ba); break;
     }
     if (le != hipSuccess) return bfail(b, NGPU_E_HIP, "batch kernel launch failed: %s", hipGetErrorString(le));

@@ -29,12 +29,12 @@
 #include <algorithm>
 #include <vector>
 
-/* Masks whose mechanism code is specialised in mechs.h (SET::dispatch), by
- * mechanism set: 0 passive, 1 TTPC, 2 Mainen.  Used for the cost estimate and
- * the segment length only. */
+/* Masks whose mechanism code is specialised (SET::dispatch), by mechanism
+ * set: 0 passive, 1 TTPC, 2 Mainen, 3 TTPC generated from NMODL (the same
+ * masks).  Positions with other masks run in packed segments. */
 inline bool ng_mask_fixed(int mechset, unsigned m)
 {
-    if (mechset == 1) return m == 0x408u || m == 0x1618u;
+    if (mechset == 1 || mechset == 3) return m == 0x408u || m == 0x1618u;
     if (mechset == 2) return m == 0x6fu;
     return m <= 1u;
 }

@@ -524,6 +524,178 @@ def harness(mech: Mechanism) -> str:
     return "\n".join(out)
 
 
+# NEURON's built-in passive mechanism, as .mod text (it has no .mod file)
+PAS_MOD = """NEURON {
+    SUFFIX pas
+    NONSPECIFIC_CURRENT i
+    RANGE g, e
+}
+PARAMETER {
+    g = .001 (S/cm2)
+    e = -70 (mV)
+}
+ASSIGNED {
+    v (mV)
+    i (mA/cm2)
+}
+BREAKPOINT {
+    i = g*(v - e)
+}
+"""
+
+
+def set_parameters(mech: Mechanism) -> List[str]:
+    """The parameter-matrix columns a mechanism owns: its PARAMETERs in
+    declaration order, without v, dt, celsius and its ion variables (the
+    layout of the reference's generated parameter list, e.g. Ih's gIhbar,
+    ehcn)."""
+    ionvars = {x for rd, wr in mech.ions.values() for x in rd + wr}
+    return [p for p in mech.params if p not in ("v", "dt", "celsius") and p not in ionvars]
+
+
+def emit_set(name: str, mechs: List[Mechanism], celsius: float, ion_values: Dict[str, float],
+             masks: List[int], nernst_R: float = 8.3134) -> str:
+    """A mechanism set for the simulation kernel (the interface of mechs.h's
+    SetTTPC) over generated mechanisms, in the given order (bit k = mechanism
+    k): state slots in mechanism order, then the calcium concentration, its
+    reversal and the calcium current handed from BREAKPOINT to the next
+    DERIVATIVE pass; parameters in mechanism order (set_parameters); each
+    block call fills an instance of the mechanism's struct from the slots,
+    the parameters and the model's ion values, runs the block and stores
+    the states back.  ``ion_values``: constant ion variables (ek, ena, cao,
+    cai0 = the initial inside calcium); eca follows cai by Nernst
+    (1000 R (celsius + 273.15) / (2 F) ln(cao / cai)) where a mechanism
+    writes cai.  ``masks``: mechanism masks that get straight-line code."""
+    slots, nslot = [], 0
+    for m in mechs:
+        own = [st for st in interface(m)["states"] if not (st.endswith("i") and st[:-1] in m.ions)]
+        slots.append({st: nslot + k for k, st in enumerate(own)})
+        nslot += len(own)
+    ca = any("ca" in m.ions for m in mechs)
+    cai_writer = any("cai" in m.ions.get("ca", ([], []))[1] for m in mechs)
+    CAI, ECA, ICA = (nslot, nslot + 1, nslot + 2) if ca else (31, 31, 0)
+    nstate = nslot + 3 if ca else max(nslot, 1)
+    params, pbase = [], []
+    for m in mechs:
+        pbase.append(len(params))
+        params += [(m.suffix, p) for p in set_parameters(m)]
+    pbase.append(len(params))
+    ca_users = sum(1 << k for k, m in enumerate(mechs) if "ca" in m.ions)
+    ica_readers = sum(1 << k for k, m in enumerate(mechs) if "ica" in m.ions.get("ca", ([], []))[0])
+    cao = ion_values.get("cao", 2.0)
+    cai0 = ion_values.get("cai0", 5e-5)
+    ktf = 1000.0 * nernst_R * (celsius + 273.15) / UNIT_CONSTANTS["faraday"]
+    S = f"SetGen_{name}"
+
+    def fill(k: int, m: Mechanism, ica_expr: str, ind: str) -> List[str]:
+        out = [f"{ind}Mech_{m.suffix} M;", f"{ind}M.v = v;"]
+        if "celsius" in m.params:
+            out.append(f"{ind}M.celsius = {celsius!r};")
+        for j, p in enumerate(set_parameters(m)):
+            out.append(f"{ind}M.{p} = P({pbase[k] + j});")
+        for st, sl in slots[k].items():
+            out.append(f"{ind}M.{st} = st[{sl}];")
+        for ion, (rd, wr) in m.ions.items():
+            for x in dict.fromkeys(rd + wr):
+                if x == "e" + ion:
+                    out.append(f"{ind}M.{x} = " + ("st[ECA];" if ion == "ca" and cai_writer else f"{ion_values[x]!r};"))
+                elif x == ion + "i":
+                    out.append(f"{ind}M.{x} = st[CAI];" if ion == "ca" else f"{ind}M.{x} = {ion_values[x]!r};")
+                elif x == ion + "o":
+                    out.append(f"{ind}M.{x} = {ion_values[x]!r};")
+                elif x == "i" + ion and x in rd:
+                    out.append(f"{ind}M.{x} = {ica_expr};")
+        return out
+
+    def store(k: int, m: Mechanism, ind: str) -> List[str]:
+        out = [f"{ind}st[{sl}] = M.{st};" for st, sl in slots[k].items()]
+        if "cai" in m.ions.get("ca", ([], []))[1]:
+            out.append(f"{ind}st[CAI] = M.cai;")
+        return out
+
+    L = [f"/* {S}: the simulation kernel's mechanism set over the generated mechanisms",
+         f" * {', '.join(m.suffix for m in mechs)} (bit k = mechanism k); generated by neurogpu_amd.nmodl. */",
+         f"struct {S} {{",
+         f"    static constexpr int NMECH = {len(mechs)}, NPARAM = {len(params)}, NSTATE = {nstate};",
+         f"    static constexpr int CAI = {CAI}, ECA = {ECA}, ICA = {ICA};",
+         f"    /* first parameter-matrix column of each mechanism, then the total */",
+         f"    static constexpr int PBASE[{len(pbase)}] = {{{', '.join(map(str, pbase))}}};",
+         "    template <class F> NG_DEV static void dispatch(unsigned m, F &&f)",
+         "    {",
+         f"        ng_dispatch(m, f, NgMasks<{', '.join(hex(x) + 'u' for x in masks)}>{{}});",
+         "    }",
+         f"    NG_DEV static unsigned ica_bits(unsigned m) {{ return (m & {hex(ica_readers)}u) ? 1u << ICA : 0u; }}",
+         f"    NG_DEV static double eca_of(double cai) {{ return {ktf / 2!r} * ng_log({cao!r} / cai); }}",
+         "    NG_DEV static void eca_after_cnexp(unsigned m, double *st)",
+         "    {",
+         (f"        if (m & {hex(ca_users)}u) st[ECA] = eca_of(st[CAI]);" if cai_writer else "        (void)m; (void)st;"),
+         "    }",
+         "    NG_DEV static unsigned state_bits(unsigned m)",
+         "    {",
+         "        unsigned s = 0;"]
+    for k, m in enumerate(mechs):
+        if slots[k]:
+            bits = sum(1 << sl for sl in slots[k].values())
+            L.append(f"        if (m & (1u << {k})) s |= {hex(bits)}u;")
+    if ca:
+        L.append(f"        if (m & {hex(ca_users)}u) s |= (1u << CAI) | (1u << ECA);")
+    L += ["        return s;", "    }",
+          "    /* INITIAL blocks in mechanism order.  A calcium mechanism that does not",
+          "     * read cai first sets it to its resting value; every calcium mechanism",
+          "     * sets eca from cai (CuInitModel_*, Code/OneKernel/src/AllModels.cu:202-296) */",
+          "    template <class PF> NG_DEV static void init(unsigned m, double v, double *st, const PF &P)",
+          "    {"]
+    for k, m in enumerate(mechs):
+        L.append(f"        if (m & (1u << {k})) {{")
+        if "ca" in m.ions:
+            if "cai" not in m.ions["ca"][0]:
+                L.append(f"            st[CAI] = {cai0!r};")
+            if cai_writer:
+                L.append("            st[ECA] = eca_of(st[CAI]);")
+        L += fill(k, m, "0.0", "            ")
+        L.append("            M.initial();")
+        L += store(k, m, "            ")
+        L.append("        }")
+    L += ["    }",
+          "    /* DERIVATIVE (cnexp) blocks at the new v, in mechanism order; ica is the",
+          "     * calcium current of the last BREAKPOINT */",
+          "    template <class PF> NG_DEV static void cnexp(unsigned m, double v, double dt, double *st, double ica, const PF &P)",
+          "    {",
+          "        (void)ica;"]
+    for k, m in enumerate(mechs):
+        if m.solve is None:
+            continue
+        L.append(f"        if (m & (1u << {k})) {{")
+        L += fill(k, m, "ica", "            ")
+        L.append("            M.dt = dt;")
+        L.append("            M.states();")
+        L += store(k, m, "            ")
+        L.append("        }")
+    L += ["    }",
+          "    /* BREAKPOINT blocks: the sum of the currents the mechanisms write, and",
+          "     * the calcium currents into ica when asked */",
+          "    template <class PF>",
+          "    NG_DEV static void current(unsigned m, double v, const double *stc, const PF &P, double &sum, double &ica, bool add_ica)",
+          "    {",
+          "        double *st = const_cast<double *>(stc);   /* read only: no block below stores a slot */",
+          "        double s = sum;",
+          "        (void)ica; (void)add_ica;"]
+    for k, m in enumerate(mechs):
+        cur = m.currents
+        if not cur:
+            continue
+        L.append(f"        if (m & (1u << {k})) {{")
+        L += fill(k, m, "0.0", "            ")
+        L.append("            M.breakpoint();")
+        for c in cur:
+            L.append(f"            s += M.{c};")
+            if c == "ica":
+                L.append("            if (add_ica) ica += M.ica;")
+        L.append("        }")
+    L += ["        sum = s;", "    }", "};"]
+    return "\n".join(L)
+
+
 PROLOGUE = """/* Generated by neurogpu_amd.nmodl -- do not edit. */
 #pragma once
 #include "ngmath.h"
@@ -554,10 +726,28 @@ def main(argv=None):
     ap.add_argument("-o", "--output", required=True)
     ap.add_argument("--harness", action="store_true", help="also emit the extern \"C\" evaluation entries")
     ap.add_argument("--interface", default="", help="write the mechanisms' variable lists as JSON here")
+    ap.add_argument("--set", default="", help="also emit SetGen_<NAME>, a mechanism set for the simulation kernel")
+    ap.add_argument("--pas", action="store_true", help="add NEURON's built-in pas mechanism")
+    ap.add_argument("--celsius", type=float, default=6.3)
+    ap.add_argument("--ion", action="append", default=[], help="constant ion value, e.g. ek=-85 (repeatable)")
+    ap.add_argument("--masks", default="", help="comma-separated mechanism masks with straight-line code")
     args = ap.parse_args(argv)
     import os
     srcs = {os.path.basename(p): open(p).read() for p in args.mods}
+    if args.pas:
+        srcs["pas (built in)"] = PAS_MOD
+    if args.set:
+        # mechanism order: suffixes sorted case-insensitively (the reference's
+        # model order, PYS/extractModel.py:327)
+        order = sorted(srcs, key=lambda k: parse(srcs[k]).suffix.lower())
+        srcs = {k: srcs[k] for k in order}
     code, faces = generate(srcs, args.harness)
+    if args.set:
+        mechs = [parse(t) for t in srcs.values()]
+        ions = {kv.split("=")[0]: float(kv.split("=")[1]) for kv in args.ion}
+        masks = [int(x, 0) for x in args.masks.split(",") if x.strip()]
+        code = code.replace('#include "ngmath.h"', '#include "mechs.h"', 1)
+        code += "\n" + emit_set(args.set, mechs, args.celsius, ions, masks) + "\n"
     with open(args.output, "w") as fh:
         fh.write(code)
     if args.interface:

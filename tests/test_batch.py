@@ -10,7 +10,7 @@ import pytest
 
 from conftest import TOL_MV, spike_counts
 from neurogpu_amd import autotune, models
-from neurogpu_amd._lib import Batch, NgpuError, workload_simulator
+from neurogpu_amd._lib import MECH_TTPC_GEN, Batch, NgpuError, workload_simulator
 from oracle import cpu_ref
 
 
@@ -33,8 +33,8 @@ def _shape(w):
     return (w.x.shape[0], w.amps.shape[0], len(w.model.rec_sites), w.dts.size)
 
 
-def _alone(w):
-    sim = workload_simulator(w)
+def _alone(w, mechset=None):
+    sim = workload_simulator(w, mechset=mechset)
     try:
         out, _ = sim.run(fetch=True)
     finally:
@@ -87,12 +87,16 @@ def test_tree_key_is_stable_and_distinct():
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("mechset,choice", [("mainen", None), ("mainen", "wave_global"), ("mainen", "block_lds"),
-                                            ("mainen", "two_waves"), ("ttpc", None), ("ttpc", "two_waves")])
+                                            ("mainen", "two_waves"), ("ttpc", None), ("ttpc", "two_waves"),
+                                            ("ttpc_gen", None)])
 def test_batch_matches_single_runs_and_oracle(mechset, choice):
-    ws = _mixed(mechset=mechset)
+    """ttpc_gen: the TTPC cells through the mechanism set generated from
+    their NMODL files (MECH_TTPC_GEN), checked against the same oracle."""
+    gen = MECH_TTPC_GEN if mechset == "ttpc_gen" else None
+    ws = _mixed(mechset="ttpc" if gen else mechset)
     _say(f"case batch of {[w.model.tree.N for w in ws]} {mechset} segments, choice {choice}")
-    alone = [_alone(w) for w in ws]
-    sims = [workload_simulator(w) for w in ws]
+    alone = [_alone(w, gen) for w in ws]
+    sims = [workload_simulator(w, mechset=gen) for w in ws]
     try:
         with Batch(0) as b:
             for i, s in enumerate(sims):

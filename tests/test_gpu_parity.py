@@ -11,7 +11,7 @@ import pytest
 
 from conftest import TOL_MV, TTPC_DIR, spike_counts
 from neurogpu_amd import formats, models
-from neurogpu_amd._lib import SOLVE_FAST, SOLVE_REFERENCE, NgpuError, Simulator, simulate_workload
+from neurogpu_amd._lib import MECH_TTPC_GEN, SOLVE_FAST, SOLVE_REFERENCE, NgpuError, Simulator, simulate_workload
 from oracle import cpu_ref
 
 pytestmark = pytest.mark.gpu
@@ -64,6 +64,30 @@ def test_ttpc1200_small(launch):
     gpu, st = simulate_workload(w, launch=launch)
     assert st["solve"] == SOLVE_FAST                 # a generated tree: the reciprocal form by default
     _check(gpu, _oracle(w), f"ttpc1200 {launch}")
+
+
+@pytest.mark.parametrize("launch", [None, dict(waves_per_run=4), dict(waves_per_run=16, runs_per_group=4)])
+def test_ttpc1200_generated_mechanisms_small(launch):
+    """The same cell through the mechanism set generated from its NMODL files
+    (MECH_TTPC_GEN, gen_ttpc.h) instead of the hand-written one: the oracle's
+    bar."""
+    w = models.workload("ttpc1200", nsets=3, nt=1200)
+    _say(f"case ttpc1200 generated mechanisms launch={launch}")
+    gpu, st = simulate_workload(w, launch=launch, mechset=MECH_TTPC_GEN)
+    _check(gpu, _oracle(w), f"ttpc1200 generated {launch}")
+
+
+def test_ttpc1200_generated_mechanisms_full_protocol():
+    """Configuration 3's whole protocol with the generated mechanism set: the
+    oracle's bar, and within rounding of the hand-written set's traces."""
+    w = models.workload("ttpc1200", nsets=2)
+    _say(f"case ttpc1200 generated mechanisms, full protocol: {w.nruns} runs x {w.dts.size} steps")
+    gen, _ = simulate_workload(w, mechset=MECH_TTPC_GEN)
+    hand, _ = simulate_workload(w)
+    _check(gen, _oracle(w), "ttpc1200 generated full protocol")
+    d = float(np.max(np.abs(gen - hand)))
+    _say(f"ttpc1200: max|generated - hand-written| = {d:.3e} mV")
+    assert d <= TOL_MV
 
 
 @pytest.mark.parametrize("name", ["active500", "ttpc1200", "passive64"])

@@ -320,3 +320,123 @@ __global__ void k(double *v, double *out, int n)
                         "-target-feature", "-Xclang", "-packed-fp32-ops", "-I", CSRC, "-I", str(tmp_path), "-c",
                         str(tmp_path / "k.hip"), "-o", str(tmp_path / "k.o")], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-3000:]
+
+
+# ------------------------------------------- the generated set in the simulator
+def test_committed_ttpc_set_is_the_generators_output(tmp_path):
+    """neurogpu_amd/csrc/gen_ttpc.h (mechanism set NGPU_MECH_TTPC_GEN) is what
+    the Makefile's gen recipe writes from the model's .mod files today."""
+    _ttpc_sources()
+    out = tmp_path / "gen_ttpc.h"
+    subprocess.run(["make", "-s", "-C", CSRC, "gen", f"GEN_OUT={out}"], check=True)
+    assert out.read_text() == open(os.path.join(CSRC, "gen_ttpc.h")).read()
+
+
+@pytest.fixture(scope="module")
+def sets(tmp_path_factory):
+    lib = tmp_path_factory.mktemp("sets") / "libsets_host.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-I", CSRC,
+                    os.path.join(ROOT, "tests", "sets_host.cpp"), "-o", str(lib)], check=True)
+    H = C.CDLL(str(lib))
+    H.set_run.restype = C.c_int
+    H.set_run.argtypes = [C.c_int, C.c_uint, C.c_int, C.c_double, C.c_double, _DP, _DP, _DP]
+    H.set_state_bits.restype = C.c_uint
+    H.set_state_bits.argtypes = [C.c_int, C.c_uint]
+    H.set_ica_bits.restype = C.c_uint
+    H.set_ica_bits.argtypes = [C.c_int, C.c_uint]
+    H.set_shape.argtypes = [C.c_int, C.POINTER(C.c_int)]
+    return H
+
+
+def _shape(H, which):
+    out = (C.c_int * 32)()
+    H.set_shape(which, out)
+    return list(out)
+
+
+def test_generated_set_keeps_the_parameter_layout(sets):
+    """The generated set reads the parameter matrix of the hand-written TTPC
+    set: its columns per mechanism are the parameters named for that mechanism,
+    in mechanism order (the layout of the reference's AllParams rows)."""
+    from neurogpu_amd import models
+    g, h = _shape(sets, 1), _shape(sets, 0)
+    assert g[:6] == h[:6] == [13, 19, 21, 18, 19, 20]
+    ms = models.MECHSETS["ttpc"]
+    base, names = [0], ms["params"]
+    for mech in ms["mechs"]:
+        base.append(base[-1] + sum(1 for p in names if p.endswith("_" + mech)))
+    assert base[-1] == len(names) == 19
+    assert g[6:6 + 14] == base
+
+
+def _slot_map(H, mask):
+    """(hand slot, generated slot) pairs of the states of the mechanisms in
+    mask, matched by mechanism and by order within a mechanism."""
+    h, g = _shape(H, 0), _shape(H, 1)
+    pairs = []
+    for k in range(13):
+        if not mask >> k & 1:
+            continue
+        ion = (1 << h[3]) | (1 << h[4])
+        hs = [b for b in range(32) if (H.set_state_bits(0, 1 << k) & ~ion) >> b & 1]
+        gs = [b for b in range(32) if (H.set_state_bits(1, 1 << k) & ~((1 << g[3]) | (1 << g[4]))) >> b & 1]
+        assert len(hs) == len(gs), k
+        pairs += list(zip(hs, gs))
+    if H.set_state_bits(0, mask) >> h[3] & 1:
+        pairs += [(h[3], g[3]), (h[4], g[4])]
+    return pairs
+
+
+def test_generated_set_steps_like_the_hand_written_one(sets):
+    """Every mechanism mask of the TTPC cell, and random ones, for 200 steps of
+    a swept voltage: init, current (with the calcium current) and cnexp of
+    the generated set against the hand-written set, state by state (relative
+    1e-11; they differ in how the rate equations are arranged)."""
+    from neurogpu_amd import models
+    m = models.workload("ttpc1200", nsets=1, nt=8).model
+    P = m.allparams(m.nominal()[None])[0].reshape(19, m.ncomps)
+    masks = {}
+    for seg in range(m.tree.N):
+        mk = int(sum(1 << k for k in range(13) if m.tree.bool_model[k, seg]))
+        masks.setdefault(mk, int(m.tree.seg_to_comp[seg]))
+    assert {0x408, 0x1618} <= set(masks)
+    cols = {mk: np.ascontiguousarray(P[:, c]) for mk, c in masks.items()}
+    # random masks take each mechanism's parameters from a segment that holds it
+    g = _shape(sets, 1)
+    pall = np.zeros(19)
+    for k in range(13):
+        seg = int(np.flatnonzero(m.tree.bool_model[k])[0])
+        lo, hi = g[6 + k], g[7 + k]
+        pall[lo:hi] = P[lo:hi, int(m.tree.seg_to_comp[seg])]
+    rng = np.random.default_rng(5)
+    for _ in range(12):
+        mk = int(rng.integers(1, 1 << 13))
+        if mk & 0x800 and not mk & 0x7:
+            # SK_E2 reads cai: without a calcium mechanism cai stays 0 and eca
+            # is log(cao / 0), left undefined by both (no cell places SK_E2 so)
+            mk |= 0x4
+        cols.setdefault(mk, pall)
+    H = sets
+    worst = 0.0
+    for mk, p in cols.items():
+        pairs = _slot_map(H, mk)
+        sh, sg = np.zeros(21), np.zeros(21)
+        io_h, io_g = np.zeros(2), np.zeros(2)
+        v = -75.0
+        assert H.set_run(0, mk, 0, v, 0.0, sh.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_h.ctypes.data_as(_DP)) == 0
+        assert H.set_run(1, mk, 0, v, 0.0, sg.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_g.ctypes.data_as(_DP)) == 0
+        for step in range(200):
+            for a, b in pairs:
+                assert _close(sg[b], sh[a], 1e-11), (hex(mk), step, a, b, sg[b], sh[a])
+                worst = max(worst, abs(sg[b] - sh[a]) / max(1e-300, abs(sh[a])))
+            v = -70.0 + 60.0 * math.sin(0.05 * step)
+            io_h[:] = 0.0
+            io_g[:] = 0.0
+            H.set_run(0, mk, 2, v, 0.0, sh.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_h.ctypes.data_as(_DP))
+            H.set_run(1, mk, 2, v, 0.0, sg.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_g.ctypes.data_as(_DP))
+            assert _close(io_g[0], io_h[0], 1e-11) and _close(io_g[1], io_h[1], 1e-11), (hex(mk), step, io_g, io_h)
+            io_g[1] = io_h[1]
+            H.set_run(0, mk, 1, v, 0.025, sh.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_h.ctypes.data_as(_DP))
+            H.set_run(1, mk, 1, v, 0.025, sg.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_g.ctypes.data_as(_DP))
+    assert H.set_ica_bits(0, 0x1fff) == H.set_ica_bits(1, 0x1fff) != 0
+    print(f"worst relative state difference {worst:.2e} over {len(cols)} masks")
