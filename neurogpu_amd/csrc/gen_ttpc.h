@@ -28,17 +28,21 @@ struct Mech_Ca_HVA {
     double hBeta = 0.0;
     NG_HD_M void rates()
     {
-        if ((v == (-27.0))) {
+        if ((v == -27.0)) {
             v = (v + 0.0001);
         }
-        mAlpha = ((0.055 * ((-27.0) - v)) / (ng_exp((((-27.0) - v) / 3.8)) - 1.0));
-        mBeta = (0.94 * ng_exp((((-75.0) - v) / 17.0)));
-        mInf = (mAlpha / (mAlpha + mBeta));
-        mTau = (1.0 / (mAlpha + mBeta));
-        hAlpha = (0.000457 * ng_exp((((-13.0) - v) / 50.0)));
-        hBeta = (0.0065 / (ng_exp((((-v) - 15.0) / 28.0)) + 1.0));
-        hInf = (hAlpha / (hAlpha + hBeta));
-        hTau = (1.0 / (hAlpha + hBeta));
+        const double exp_0 = ng_expc(((-27.0 - v) * 0.2631578947368421));
+        mAlpha = ng_div((0.055 * (-27.0 - v)), (exp_0 - 1.0));
+        const double exp_1 = ng_expc(((-75.0 - v) * 0.058823529411764705));
+        mBeta = (0.94 * exp_1);
+        mInf = ng_div(mAlpha, (mAlpha + mBeta));
+        mTau = ng_rcp((mAlpha + mBeta));
+        const double exp_2 = ng_expc(((-13.0 - v) * 0.02));
+        hAlpha = (0.000457 * exp_2);
+        const double exp_3 = ng_expc((((-v) - 15.0) * 0.03571428571428571));
+        hBeta = ng_div(0.0065, (exp_3 + 1.0));
+        hInf = ng_div(hAlpha, (hAlpha + hBeta));
+        hTau = ng_rcp((hAlpha + hBeta));
     }
     NG_HD_M void initial()
     {
@@ -49,8 +53,8 @@ struct Mech_Ca_HVA {
     NG_HD_M void states()
     {
         rates();
-        m = m + (1.0 - ng_exp(dt * (((-1.0) / mTau)))) * (-((mInf / mTau)) / (((-1.0) / mTau)) - m);
-        h = h + (1.0 - ng_exp(dt * (((-1.0) / hTau)))) * (-((hInf / hTau)) / (((-1.0) / hTau)) - h);
+        m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
+        h = h + (1.0 - ng_expc(-(dt * ng_rcp(hTau)))) * (hInf - h);
     }
     NG_HD_M void breakpoint()
     {
@@ -75,12 +79,16 @@ struct Mech_Ca_LVAst {
     NG_HD_M void rates()
     {
         double qt = 0.0;
-        qt = ng_pow(2.3, ((34.0 - 21.0) / 10.0));
+        qt = 2.952882641412121;
         v = (v + 10.0);
-        mInf = (1.0 / (1.0 + ng_exp(((v - (-30.0)) / (-6.0)))));
-        mTau = ((5.0 + (20.0 / (1.0 + ng_exp(((v - (-25.0)) / 5.0))))) / qt);
-        hInf = (1.0 / (1.0 + ng_exp(((v - (-80.0)) / 6.4))));
-        hTau = ((20.0 + (50.0 / (1.0 + ng_exp(((v - (-40.0)) / 7.0))))) / qt);
+        const double exp_0 = ng_expc(((v - -30.0) * -0.16666666666666666));
+        mInf = ng_rcp((1.0 + exp_0));
+        const double exp_1 = ng_expc(((v - -25.0) * 0.2));
+        mTau = ((5.0 + ng_div(20.0, (1.0 + exp_1))) * 0.3386521313023745);
+        const double exp_2 = ng_expc(((v - -80.0) * 0.15625));
+        hInf = ng_rcp((1.0 + exp_2));
+        const double exp_3 = ng_expc(((v - -40.0) * 0.14285714285714285));
+        hTau = ((20.0 + ng_div(50.0, (1.0 + exp_3))) * 0.3386521313023745);
         v = (v - 10.0);
     }
     NG_HD_M void initial()
@@ -92,8 +100,8 @@ struct Mech_Ca_LVAst {
     NG_HD_M void states()
     {
         rates();
-        m = m + (1.0 - ng_exp(dt * (((-1.0) / mTau)))) * (-((mInf / mTau)) / (((-1.0) / mTau)) - m);
-        h = h + (1.0 - ng_exp(dt * (((-1.0) / hTau)))) * (-((hInf / hTau)) / (((-1.0) / hTau)) - h);
+        m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
+        h = h + (1.0 - ng_expc(-(dt * ng_rcp(hTau)))) * (hInf - h);
     }
     NG_HD_M void breakpoint()
     {
@@ -117,7 +125,7 @@ struct Mech_CaDynamics_E2 {
     }
     NG_HD_M void states()
     {
-        cai = cai + (1.0 - ng_exp(dt * ((-(1.0 / decay))))) * (-((((-10000.0) * ((ica * gamma) / ((2.0 * FARADAY) * depth))) - ((-minCai) / decay))) / ((-(1.0 / decay))) - cai);
+        cai = cai + (1.0 - ng_expc(-(dt * ng_rcp(decay)))) * ((((-10000.0 * ng_div((ica * gamma), (192970.618 * depth))) - ng_div((-minCai), decay)) * decay) - cai);
     }
     NG_HD_M void breakpoint()
     {
@@ -138,13 +146,15 @@ struct Mech_Ih {
     double mBeta = 0.0;
     NG_HD_M void rates()
     {
-        if ((v == (-154.9))) {
+        if ((v == -154.9)) {
             v = (v + 0.0001);
         }
-        mAlpha = (((0.001 * 6.43) * (v + 154.9)) / (ng_exp(((v + 154.9) / 11.9)) - 1.0));
-        mBeta = ((0.001 * 193.0) * ng_exp((v / 33.1)));
-        mInf = (mAlpha / (mAlpha + mBeta));
-        mTau = (1.0 / (mAlpha + mBeta));
+        const double exp_0 = ng_expc(((v + 154.9) * 0.08403361344537814));
+        mAlpha = ng_div((0.00643 * (v + 154.9)), (exp_0 - 1.0));
+        const double exp_1 = ng_expc((v * 0.030211480362537763));
+        mBeta = (0.193 * exp_1);
+        mInf = ng_div(mAlpha, (mAlpha + mBeta));
+        mTau = ng_rcp((mAlpha + mBeta));
     }
     NG_HD_M void initial()
     {
@@ -154,7 +164,7 @@ struct Mech_Ih {
     NG_HD_M void states()
     {
         rates();
-        m = m + (1.0 - ng_exp(dt * (((-1.0) / mTau)))) * (-((mInf / mTau)) / (((-1.0) / mTau)) - m);
+        m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
     }
     NG_HD_M void breakpoint()
     {
@@ -178,11 +188,12 @@ struct Mech_Im {
     NG_HD_M void rates()
     {
         double qt = 0.0;
-        qt = ng_pow(2.3, ((34.0 - 21.0) / 10.0));
-        mAlpha = (0.0033 * ng_exp(((2.5 * 0.04) * (v - (-35.0)))));
-        mBeta = (0.0033 * ng_exp((((-2.5) * 0.04) * (v - (-35.0)))));
-        mInf = (mAlpha / (mAlpha + mBeta));
-        mTau = ((1.0 / (mAlpha + mBeta)) / qt);
+        qt = 2.952882641412121;
+        const double exp_0 = ng_expc((0.1 * (v - -35.0)));
+        mAlpha = (0.0033 * exp_0);
+        mBeta = (0.0033 * ng_rcp(exp_0));
+        mInf = ng_div(mAlpha, (mAlpha + mBeta));
+        mTau = (ng_rcp((mAlpha + mBeta)) * 0.3386521313023745);
     }
     NG_HD_M void initial()
     {
@@ -192,7 +203,7 @@ struct Mech_Im {
     NG_HD_M void states()
     {
         rates();
-        m = m + (1.0 - ng_exp(dt * (((-1.0) / mTau)))) * (-((mInf / mTau)) / (((-1.0) / mTau)) - m);
+        m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
     }
     NG_HD_M void breakpoint()
     {
@@ -217,16 +228,20 @@ struct Mech_K_Pst {
     NG_HD_M void rates()
     {
         double qt = 0.0;
-        qt = ng_pow(2.3, ((34.0 - 21.0) / 10.0));
+        qt = 2.952882641412121;
         v = (v + 10.0);
-        mInf = (1.0 / (1.0 + ng_exp(((-(v + 1.0)) / 12.0))));
-        if ((v < (-50.0))) {
-            mTau = ((1.25 + (175.03 * ng_exp(((-v) * (-0.026))))) / qt);
+        const double exp_0 = ng_expc(((-(v + 1.0)) * 0.08333333333333333));
+        mInf = ng_rcp((1.0 + exp_0));
+        if ((v < -50.0)) {
+            const double exp_1 = ng_expc(((-v) * -0.026));
+            mTau = ((1.25 + (175.03 * exp_1)) * 0.3386521313023745);
         } else {
-            mTau = ((1.25 + (13.0 * ng_exp(((-v) * 0.026)))) / qt);
+            const double exp_2 = ng_expc(((-v) * 0.026));
+            mTau = ((1.25 + (13.0 * exp_2)) * 0.3386521313023745);
         }
-        hInf = (1.0 / (1.0 + ng_exp(((-(v + 54.0)) / (-11.0)))));
-        hTau = ((360.0 + ((1010.0 + (24.0 * (v + 55.0))) * ng_exp((-ng_pow(((v + 75.0) / 48.0), 2.0))))) / qt);
+        const double exp_3 = ng_expc(((-(v + 54.0)) * -0.09090909090909091));
+        hInf = ng_rcp((1.0 + exp_3));
+        hTau = ((360.0 + ((1010.0 + (24.0 * (v + 55.0))) * ng_expc((-ng_pow(((v + 75.0) * 0.020833333333333332), 2.0))))) * 0.3386521313023745);
         v = (v - 10.0);
     }
     NG_HD_M void initial()
@@ -238,8 +253,8 @@ struct Mech_K_Pst {
     NG_HD_M void states()
     {
         rates();
-        m = m + (1.0 - ng_exp(dt * (((-1.0) / mTau)))) * (-((mInf / mTau)) / (((-1.0) / mTau)) - m);
-        h = h + (1.0 - ng_exp(dt * (((-1.0) / hTau)))) * (-((hInf / hTau)) / (((-1.0) / hTau)) - h);
+        m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
+        h = h + (1.0 - ng_expc(-(dt * ng_rcp(hTau)))) * (hInf - h);
     }
     NG_HD_M void breakpoint()
     {
@@ -264,12 +279,14 @@ struct Mech_K_Tst {
     NG_HD_M void rates()
     {
         double qt = 0.0;
-        qt = ng_pow(2.3, ((34.0 - 21.0) / 10.0));
+        qt = 2.952882641412121;
         v = (v + 10.0);
-        mInf = (1.0 / (1.0 + ng_exp(((-(v + 0.0)) / 19.0))));
-        mTau = ((0.34 + (0.92 * ng_exp((-ng_pow(((v + 71.0) / 59.0), 2.0))))) / qt);
-        hInf = (1.0 / (1.0 + ng_exp(((-(v + 66.0)) / (-10.0)))));
-        hTau = ((8.0 + (49.0 * ng_exp((-ng_pow(((v + 73.0) / 23.0), 2.0))))) / qt);
+        const double exp_0 = ng_expc(((-(v + 0.0)) * 0.05263157894736842));
+        mInf = ng_rcp((1.0 + exp_0));
+        mTau = ((0.34 + (0.92 * ng_expc((-ng_pow(((v + 71.0) * 0.01694915254237288), 2.0))))) * 0.3386521313023745);
+        const double exp_1 = ng_expc(((-(v + 66.0)) * -0.1));
+        hInf = ng_rcp((1.0 + exp_1));
+        hTau = ((8.0 + (49.0 * ng_expc((-ng_pow(((v + 73.0) * 0.043478260869565216), 2.0))))) * 0.3386521313023745);
         v = (v - 10.0);
     }
     NG_HD_M void initial()
@@ -281,8 +298,8 @@ struct Mech_K_Tst {
     NG_HD_M void states()
     {
         rates();
-        m = m + (1.0 - ng_exp(dt * (((-1.0) / mTau)))) * (-((mInf / mTau)) / (((-1.0) / mTau)) - m);
-        h = h + (1.0 - ng_exp(dt * (((-1.0) / hTau)))) * (-((hInf / hTau)) / (((-1.0) / hTau)) - h);
+        m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
+        h = h + (1.0 - ng_expc(-(dt * ng_rcp(hTau)))) * (hInf - h);
     }
     NG_HD_M void breakpoint()
     {
@@ -311,24 +328,29 @@ struct Mech_Nap_Et2 {
     NG_HD_M void rates()
     {
         double qt = 0.0;
-        qt = ng_pow(2.3, ((34.0 - 21.0) / 10.0));
-        mInf = (1.0 / (1.0 + ng_exp(((v - (-52.6)) / (-4.6)))));
-        if ((v == (-38.0))) {
+        qt = 2.952882641412121;
+        const double exp_0 = ng_expc(((v - -52.6) * -0.2173913043478261));
+        mInf = ng_rcp((1.0 + exp_0));
+        if ((v == -38.0)) {
             v = (v + 0.0001);
         }
-        mAlpha = ((0.182 * (v - (-38.0))) / (1.0 - ng_exp(((-(v - (-38.0))) / 6.0))));
-        mBeta = ((0.124 * ((-v) - 38.0)) / (1.0 - ng_exp(((-((-v) - 38.0)) / 6.0))));
-        mTau = ((6.0 * (1.0 / (mAlpha + mBeta))) / qt);
-        if ((v == (-17.0))) {
+        const double exp_1 = ng_expc(((-(v - -38.0)) * 0.16666666666666666));
+        mAlpha = ng_div((0.182 * (v - -38.0)), (1.0 - exp_1));
+        mBeta = ng_div((0.124 * ((-v) - 38.0)), (1.0 - ng_rcp(exp_1)));
+        mTau = ((6.0 * ng_rcp((mAlpha + mBeta))) * 0.3386521313023745);
+        if ((v == -17.0)) {
             v = (v + 0.0001);
         }
-        if ((v == (-64.4))) {
+        if ((v == -64.4)) {
             v = (v + 0.0001);
         }
-        hInf = (1.0 / (1.0 + ng_exp(((v - (-48.8)) / 10.0))));
-        hAlpha = (((-2.88e-06) * (v + 17.0)) / (1.0 - ng_exp(((v + 17.0) / 4.63))));
-        hBeta = ((6.94e-06 * (v + 64.4)) / (1.0 - ng_exp(((-(v + 64.4)) / 2.63))));
-        hTau = ((1.0 / (hAlpha + hBeta)) / qt);
+        const double exp_2 = ng_expc(((v - -48.8) * 0.1));
+        hInf = ng_rcp((1.0 + exp_2));
+        const double exp_3 = ng_expc(((v + 17.0) * 0.2159827213822894));
+        hAlpha = ng_div((-2.88e-06 * (v + 17.0)), (1.0 - exp_3));
+        const double exp_4 = ng_expc(((-(v + 64.4)) * 0.3802281368821293));
+        hBeta = ng_div((6.94e-06 * (v + 64.4)), (1.0 - exp_4));
+        hTau = (ng_rcp((hAlpha + hBeta)) * 0.3386521313023745);
     }
     NG_HD_M void initial()
     {
@@ -339,8 +361,8 @@ struct Mech_Nap_Et2 {
     NG_HD_M void states()
     {
         rates();
-        m = m + (1.0 - ng_exp(dt * (((-1.0) / mTau)))) * (-((mInf / mTau)) / (((-1.0) / mTau)) - m);
-        h = h + (1.0 - ng_exp(dt * (((-1.0) / hTau)))) * (-((hInf / hTau)) / (((-1.0) / hTau)) - h);
+        m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
+        h = h + (1.0 - ng_expc(-(dt * ng_rcp(hTau)))) * (hInf - h);
     }
     NG_HD_M void breakpoint()
     {
@@ -369,21 +391,23 @@ struct Mech_NaTa_t {
     NG_HD_M void rates()
     {
         double qt = 0.0;
-        qt = ng_pow(2.3, ((34.0 - 21.0) / 10.0));
-        if ((v == (-38.0))) {
+        qt = 2.952882641412121;
+        if ((v == -38.0)) {
             v = (v + 0.0001);
         }
-        mAlpha = ((0.182 * (v - (-38.0))) / (1.0 - ng_exp(((-(v - (-38.0))) / 6.0))));
-        mBeta = ((0.124 * ((-v) - 38.0)) / (1.0 - ng_exp(((-((-v) - 38.0)) / 6.0))));
-        mTau = ((1.0 / (mAlpha + mBeta)) / qt);
-        mInf = (mAlpha / (mAlpha + mBeta));
-        if ((v == (-66.0))) {
+        const double exp_0 = ng_expc(((-(v - -38.0)) * 0.16666666666666666));
+        mAlpha = ng_div((0.182 * (v - -38.0)), (1.0 - exp_0));
+        mBeta = ng_div((0.124 * ((-v) - 38.0)), (1.0 - ng_rcp(exp_0)));
+        mTau = (ng_rcp((mAlpha + mBeta)) * 0.3386521313023745);
+        mInf = ng_div(mAlpha, (mAlpha + mBeta));
+        if ((v == -66.0)) {
             v = (v + 0.0001);
         }
-        hAlpha = (((-0.015) * (v - (-66.0))) / (1.0 - ng_exp(((v - (-66.0)) / 6.0))));
-        hBeta = (((-0.015) * ((-v) - 66.0)) / (1.0 - ng_exp((((-v) - 66.0) / 6.0))));
-        hTau = ((1.0 / (hAlpha + hBeta)) / qt);
-        hInf = (hAlpha / (hAlpha + hBeta));
+        const double exp_1 = ng_expc(((v - -66.0) * 0.16666666666666666));
+        hAlpha = ng_div((-0.015 * (v - -66.0)), (1.0 - exp_1));
+        hBeta = ng_div((-0.015 * ((-v) - 66.0)), (1.0 - ng_rcp(exp_1)));
+        hTau = (ng_rcp((hAlpha + hBeta)) * 0.3386521313023745);
+        hInf = ng_div(hAlpha, (hAlpha + hBeta));
     }
     NG_HD_M void initial()
     {
@@ -394,8 +418,8 @@ struct Mech_NaTa_t {
     NG_HD_M void states()
     {
         rates();
-        m = m + (1.0 - ng_exp(dt * (((-1.0) / mTau)))) * (-((mInf / mTau)) / (((-1.0) / mTau)) - m);
-        h = h + (1.0 - ng_exp(dt * (((-1.0) / hTau)))) * (-((hInf / hTau)) / (((-1.0) / hTau)) - h);
+        m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
+        h = h + (1.0 - ng_expc(-(dt * ng_rcp(hTau)))) * (hInf - h);
     }
     NG_HD_M void breakpoint()
     {
@@ -424,21 +448,23 @@ struct Mech_NaTs2_t {
     NG_HD_M void rates()
     {
         double qt = 0.0;
-        qt = ng_pow(2.3, ((34.0 - 21.0) / 10.0));
-        if ((v == (-32.0))) {
+        qt = 2.952882641412121;
+        if ((v == -32.0)) {
             v = (v + 0.0001);
         }
-        mAlpha = ((0.182 * (v - (-32.0))) / (1.0 - ng_exp(((-(v - (-32.0))) / 6.0))));
-        mBeta = ((0.124 * ((-v) - 32.0)) / (1.0 - ng_exp(((-((-v) - 32.0)) / 6.0))));
-        mInf = (mAlpha / (mAlpha + mBeta));
-        mTau = ((1.0 / (mAlpha + mBeta)) / qt);
-        if ((v == (-60.0))) {
+        const double exp_0 = ng_expc(((-(v - -32.0)) * 0.16666666666666666));
+        mAlpha = ng_div((0.182 * (v - -32.0)), (1.0 - exp_0));
+        mBeta = ng_div((0.124 * ((-v) - 32.0)), (1.0 - ng_rcp(exp_0)));
+        mInf = ng_div(mAlpha, (mAlpha + mBeta));
+        mTau = (ng_rcp((mAlpha + mBeta)) * 0.3386521313023745);
+        if ((v == -60.0)) {
             v = (v + 0.0001);
         }
-        hAlpha = (((-0.015) * (v - (-60.0))) / (1.0 - ng_exp(((v - (-60.0)) / 6.0))));
-        hBeta = (((-0.015) * ((-v) - 60.0)) / (1.0 - ng_exp((((-v) - 60.0) / 6.0))));
-        hInf = (hAlpha / (hAlpha + hBeta));
-        hTau = ((1.0 / (hAlpha + hBeta)) / qt);
+        const double exp_1 = ng_expc(((v - -60.0) * 0.16666666666666666));
+        hAlpha = ng_div((-0.015 * (v - -60.0)), (1.0 - exp_1));
+        hBeta = ng_div((-0.015 * ((-v) - 60.0)), (1.0 - ng_rcp(exp_1)));
+        hInf = ng_div(hAlpha, (hAlpha + hBeta));
+        hTau = (ng_rcp((hAlpha + hBeta)) * 0.3386521313023745);
     }
     NG_HD_M void initial()
     {
@@ -449,8 +475,8 @@ struct Mech_NaTs2_t {
     NG_HD_M void states()
     {
         rates();
-        m = m + (1.0 - ng_exp(dt * (((-1.0) / mTau)))) * (-((mInf / mTau)) / (((-1.0) / mTau)) - m);
-        h = h + (1.0 - ng_exp(dt * (((-1.0) / hTau)))) * (-((hInf / hTau)) / (((-1.0) / hTau)) - h);
+        m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
+        h = h + (1.0 - ng_expc(-(dt * ng_rcp(hTau)))) * (hInf - h);
     }
     NG_HD_M void breakpoint()
     {
@@ -493,7 +519,7 @@ struct Mech_SK_E2 {
         if ((ca < 1e-07)) {
             ca = (ca + 1e-07);
         }
-        zInf = (1.0 / (1.0 + ng_pow((0.00043 / ca), 4.8)));
+        zInf = ng_rcp((1.0 + ng_expc((4.8 * ng_logc(ng_div(0.00043, ca))))));
     }
     NG_HD_M void initial()
     {
@@ -503,7 +529,7 @@ struct Mech_SK_E2 {
     NG_HD_M void states()
     {
         rates(cai);
-        z = z + (1.0 - ng_exp(dt * (((-1.0) / zTau)))) * (-((zInf / zTau)) / (((-1.0) / zTau)) - z);
+        z = z + (1.0 - ng_expc(-(dt * ng_rcp(zTau)))) * (zInf - z);
     }
     NG_HD_M void breakpoint()
     {
@@ -524,8 +550,10 @@ struct Mech_SKv3_1 {
     double mTau = 0.0;
     NG_HD_M void rates()
     {
-        mInf = (1.0 / (1.0 + ng_exp(((v - 18.7) / (-9.7)))));
-        mTau = ((0.2 * 20.0) / (1.0 + ng_exp(((v - (-46.56)) / (-44.14)))));
+        const double exp_0 = ng_expc(((v - 18.7) * -0.10309278350515465));
+        mInf = ng_rcp((1.0 + exp_0));
+        const double exp_1 = ng_expc(((v - -46.56) * -0.022655188038060714));
+        mTau = ng_div(4.0, (1.0 + exp_1));
     }
     NG_HD_M void initial()
     {
@@ -535,7 +563,7 @@ struct Mech_SKv3_1 {
     NG_HD_M void states()
     {
         rates();
-        m = m + (1.0 - ng_exp(dt * (((-1.0) / mTau)))) * (-((mInf / mTau)) / (((-1.0) / mTau)) - m);
+        m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
     }
     NG_HD_M void breakpoint()
     {

@@ -34,6 +34,7 @@ import argparse
 import ast
 import dataclasses
 import json
+import math
 import re
 import sys
 from typing import Dict, List, Optional, Tuple
@@ -395,9 +396,203 @@ def linear(node: ast.expr, x: str) -> Tuple[ast.expr, ast.expr]:
     raise NmodlError(f"{ast.unparse(node)} is not linear in {x} (cnexp needs x' = A + B x)")
 
 
+# ------------------------------------------------------------------ fast forms
+# The fast emission (emit(..., fast=constants)) evaluates the same equations
+# with the arrangements mechs.h's hand-written sets use on the FP64 VALU,
+# where an IEEE division costs about ten instructions and an exp about twenty;
+# results differ from the reference's operation order by rounding only:
+#   - subexpressions of literals and known constants (celsius, and LOCALs
+#     assigned such a value) are evaluated here;
+#   - a division by a constant is a product with its reciprocal, any other
+#     division a product with the refined reciprocal (ng_div);
+#   - exp is ng_expc (no special-case branches), c^y for a constant c > 0 is
+#     ng_expc(y log c);
+#   - cnexp of x' = (X - x) / T is x += (1 - exp(-dt / T)) (X - x), with the
+#     quotients that cancel in -A/B taken out.
+_FOLD = {"exp": math.exp, "log": math.log, "fabs": abs, "sqrt": math.sqrt, "tanh": math.tanh, "sin": math.sin,
+         "cos": math.cos}
+
+
+def _c(x: float) -> ast.expr:
+    return ast.Constant(float(x))
+
+
+def _num(node: ast.expr) -> Optional[float]:
+    return float(node.value) if isinstance(node, ast.Constant) and isinstance(node.value, (int, float)) else None
+
+
+def _finite(x) -> bool:
+    return isinstance(x, float) and math.isfinite(x)
+
+
+def _affine(node: ast.expr) -> Optional[Tuple[Dict[str, float], float]]:
+    """(coefficients, constant) when node is an affine function of names with
+    literal coefficients, else None."""
+    if _num(node) is not None:
+        return {}, _num(node)
+    if isinstance(node, ast.Name):
+        return {node.id: 1.0}, 0.0
+    if isinstance(node, ast.UnaryOp) and isinstance(node.op, (ast.USub, ast.UAdd)):
+        f = _affine(node.operand)
+        if f is None or isinstance(node.op, ast.UAdd):
+            return f
+        return {k: -c for k, c in f[0].items()}, -f[1]
+    if isinstance(node, ast.BinOp):
+        if isinstance(node.op, (ast.Add, ast.Sub)):
+            a, b = _affine(node.left), _affine(node.right)
+            if a is None or b is None:
+                return None
+            sg = 1.0 if isinstance(node.op, ast.Add) else -1.0
+            co = dict(a[0])
+            for k, c in b[0].items():
+                co[k] = co.get(k, 0.0) + sg * c
+            return co, a[1] + sg * b[1]
+        if isinstance(node.op, ast.Mult):
+            for x, y in ((node.left, node.right), (node.right, node.left)):
+                if _num(y) is not None:
+                    f = _affine(x)
+                    if f is None:
+                        return None
+                    return {k: c * _num(y) for k, c in f[0].items()}, f[1] * _num(y)
+    return None
+
+
+class _ExpShare:
+    """exp(x) and exp(-x) of one statement list share one ng_expc: the second
+    is the refined reciprocal of the first.  Arguments are matched as affine
+    functions of names with literal coefficients; an entry ends when one of
+    its names is assigned.  ``pre`` collects the declarations of the shared
+    values to emit before the statement being converted."""
+
+    def __init__(self, parent: Optional["_ExpShare"] = None):
+        self.table = dict(parent.table) if parent else {}
+        self.count = parent.count if parent else [0]
+        self.pre: List[str] = []
+
+    def exp(self, arg: ast.expr) -> ast.expr:
+        f = _affine(arg)
+        if f is None or not f[0]:
+            return ast.Call(ast.Name("ng_expc"), [arg], [])
+        key = (tuple(sorted(f[0].items())), f[1])
+        neg = (tuple(sorted((k, -c) for k, c in f[0].items())), -f[1])
+        if key in self.table:
+            return ast.Name(self.table[key][0])
+        if neg in self.table:
+            return ast.Call(ast.Name("ng_rcp"), [ast.Name(self.table[neg][0])], [])
+        name = f"exp_{self.count[0]}"
+        self.count[0] += 1
+        self.pre.append(f"const double {name} = ng_expc({cxx(arg)});")
+        self.table[key] = (name, set(f[0]))
+        return ast.Name(name)
+
+    def kill(self, target: str) -> None:
+        self.table = {k: e for k, e in self.table.items() if target not in e[1]}
+
+
+def fast_expr(node: ast.expr, consts: Dict[str, float], share: Optional[_ExpShare] = None) -> ast.expr:
+    """The fast form of an expression tree (see above)."""
+    def fe(n):
+        return fast_expr(n, consts, share)
+
+    if isinstance(node, ast.Name):
+        return _c(consts[node.id]) if node.id in consts else node
+    if isinstance(node, ast.Constant):
+        return node
+    if isinstance(node, ast.UnaryOp):
+        o = fe(node.operand)
+        if _num(o) is not None and isinstance(node.op, (ast.USub, ast.UAdd)):
+            return _c(-_num(o) if isinstance(node.op, ast.USub) else _num(o))
+        return ast.UnaryOp(node.op, o)
+    if isinstance(node, ast.BinOp):
+        l, r = fe(node.left), fe(node.right)
+        lv, rv = _num(l), _num(r)
+        if lv is not None and rv is not None:
+            try:
+                val = {ast.Add: lambda: lv + rv, ast.Sub: lambda: lv - rv, ast.Mult: lambda: lv * rv,
+                       ast.Div: lambda: lv / rv, ast.Pow: lambda: math.pow(lv, rv)}[type(node.op)]()
+            except (ZeroDivisionError, ValueError, OverflowError):
+                val = None
+            if _finite(val):
+                return _c(val)
+        if isinstance(node.op, ast.Div):
+            if rv is not None and rv != 0.0 and _finite(1.0 / rv):
+                return ast.BinOp(l, ast.Mult(), _c(1.0 / rv))
+            if lv == 1.0:
+                return ast.Call(ast.Name("ng_rcp"), [r], [])
+            return ast.Call(ast.Name("ng_div"), [l, r], [])
+        if isinstance(node.op, ast.Pow) and lv is not None and lv > 0.0 and rv is None:
+            return ast.Call(ast.Name("ng_expc"), [ast.BinOp(r, ast.Mult(), _c(math.log(lv)))], [])
+        if isinstance(node.op, ast.Pow) and lv is None and rv is not None and rv not in (0.0, 1.0, 2.0, 3.0, 4.0):
+            # x^c = exp(c log x) for x > 0 (a concentration ratio, a rate)
+            return ast.Call(ast.Name("ng_expc"), [ast.BinOp(_c(rv), ast.Mult(), ast.Call(ast.Name("ng_logc"), [l], []))], [])
+        return ast.BinOp(l, node.op, r)
+    if isinstance(node, ast.Call) and isinstance(node.func, ast.Name):
+        args = [fe(a) for a in node.args]
+        vals = [_num(a) for a in args]
+        if node.func.id in _FOLD and len(args) == 1 and vals[0] is not None:
+            try:
+                val = _FOLD[node.func.id](vals[0])
+            except (ValueError, OverflowError):
+                val = None
+            if _finite(val):
+                return _c(val)
+        if node.func.id == "exp":
+            return share.exp(args[0]) if share is not None and len(args) == 1 else ast.Call(ast.Name("ng_expc"), args, [])
+        return ast.Call(node.func, args, [])
+    if isinstance(node, ast.Compare):
+        return ast.Compare(fe(node.left), node.ops, [fe(c) for c in node.comparators])
+    if isinstance(node, ast.BoolOp):
+        return ast.BoolOp(node.op, [fe(v) for v in node.values])
+    return node
+
+
+def _neg_unit_over(node: ast.expr) -> Optional[ast.expr]:
+    """T when node is -1/T (as (-1)/T or -(1/T)), else None."""
+    if isinstance(node, ast.BinOp) and isinstance(node.op, ast.Div):
+        n = node.left
+        if _num(n) == -1.0 or (isinstance(n, ast.UnaryOp) and isinstance(n.op, ast.USub) and _num(n.operand) == 1.0):
+            return node.right
+    if isinstance(node, ast.UnaryOp) and isinstance(node.op, ast.USub):
+        n = node.operand
+        if isinstance(n, ast.BinOp) and isinstance(n.op, ast.Div) and _num(n.left) == 1.0:
+            return n.right
+    return None
+
+
+def fast_cnexp(x: str, a: ast.expr, b: ast.expr, consts: Dict[str, float]) -> str:
+    """C++ of one cnexp step of x' = A + B x in the fast form."""
+    T = _neg_unit_over(b)
+    if T is not None:
+        rate = cxx(fast_expr(ast.Call(ast.Name("ng_rcp"), [T], []), consts))
+        if isinstance(a, ast.BinOp) and isinstance(a.op, ast.Div) and ast.dump(a.right) == ast.dump(T):
+            inf = fast_expr(a.left, consts)                      # A = X / T: -A/B = X
+        else:
+            inf = fast_expr(ast.BinOp(a, ast.Mult(), T), consts)  # -A/B = A T
+        return f"{x} = {x} + (1.0 - ng_expc(-(dt * {rate}))) * ({cxx(inf)} - {x});"
+    A, B = fast_expr(a, consts), fast_expr(b, consts)
+    q = fast_expr(ast.BinOp(ast.UnaryOp(ast.USub(), a), ast.Div(), b), consts)
+    return f"{x} = {x} + (1.0 - ng_expc(dt * ({cxx(B)}))) * ({cxx(q)} - {x});"
+
+
 # ------------------------------------------------------------------ emission
-def _emit(stmts: List[Stmt], mech: Mechanism, ind: str, locals_: List[str], ret: Optional[str] = None) -> List[str]:
+def _emit(stmts: List[Stmt], mech: Mechanism, ind: str, locals_: List[str], ret: Optional[str] = None,
+          fast: Optional[Dict[str, float]] = None, share: Optional[_ExpShare] = None) -> List[str]:
+    """C++ of a statement list; ``fast``: None for the reference's operation
+    order, else the known constants of the fast forms (LOCALs assigned a
+    constant at the top level of the list join them); ``share``: the shared
+    exps of the enclosing list."""
     out = []
+    env = None if fast is None else dict(fast)
+    sh = None if fast is None else _ExpShare(share)
+
+    def ex(e: ast.expr) -> str:
+        if env is None:
+            return cxx(e)
+        sh.pre = []
+        code = cxx(fast_expr(e, env, sh))
+        out.extend(ind + line for line in sh.pre)
+        return code
+
     for s in stmts:
         if s.kind == "local":
             if s.target not in locals_:
@@ -405,28 +600,71 @@ def _emit(stmts: List[Stmt], mech: Mechanism, ind: str, locals_: List[str], ret:
                 out.append(f"{ind}double {s.target} = 0.0;")
         elif s.kind == "assign":
             tgt = "ret_" if ret is not None and s.target == ret else s.target
-            out.append(f"{ind}{tgt} = {cxx(_py(s.expr))};")
+            e = _py(s.expr)
+            code = ex(e)
+            if env is not None:
+                env.pop(s.target, None)
+                sh.kill(s.target)
+                val = _num(fast_expr(e, env))
+                if val is not None and s.target in locals_ and tgt != "ret_":
+                    env[s.target] = val
+            out.append(f"{ind}{tgt} = {code};")
         elif s.kind == "call":
             args = [a.strip() for a in s.expr.split(",") if a.strip()]
-            out.append(f"{ind}{s.target}({', '.join(cxx(_py(a)) for a in args)});")
+            code = ", ".join([ex(_py(a)) for a in args])
+            out.append(f"{ind}{s.target}({code});")
+            if env is not None:
+                # a procedure may assign any field
+                sh.table = {}
         elif s.kind == "deriv":
             a, b = linear(_py(s.expr), s.target)
-            A, B = cxx(a), cxx(b)
             x = s.target
-            out.append(f"{ind}{x} = {x} + (1.0 - ng_exp(dt * ({B}))) * (-({A}) / ({B}) - {x});")
+            if env is not None:
+                out.append(ind + fast_cnexp(x, a, b, env))
+                sh.kill(x)
+            else:
+                A, B = cxx(a), cxx(b)
+                out.append(f"{ind}{x} = {x} + (1.0 - ng_exp(dt * ({B}))) * (-({A}) / ({B}) - {x});")
         elif s.kind == "if":
             for k, (cond, body) in enumerate(s.branches):
                 head = "if" if k == 0 else ("} else if" if cond is not None else "} else")
-                out.append(f"{ind}{head}" + (f" ({cxx(_cond(cond))}) {{" if cond is not None else " {"))
-                out += _emit(body, mech, ind + "    ", locals_, ret)
+                if cond is not None and env is not None and k > 0:
+                    # an else-if condition is evaluated inside the chain: no shared exps there
+                    c = cxx(fast_expr(_cond(cond), env))
+                else:
+                    c = None if cond is None else ex(_cond(cond))
+                out.append(f"{ind}{head}" + (f" ({c}) {{" if cond is not None else " {"))
+                sub = None if env is None else {n: v for n, v in env.items()}
+                out += _emit(body, mech, ind + "    ", locals_, ret, sub, sh)
             out.append(f"{ind}}}")
+            if env is not None:
+                # a name assigned in a branch is no longer a known constant
+                for _, body in s.branches:
+                    for t in _assigned_names(body):
+                        env.pop(t, None)
+                        sh.kill(t)
     return out
 
 
-def emit(mech: Mechanism, source: str = "") -> str:
+def _assigned_names(stmts: List[Stmt]) -> List[str]:
+    out = []
+    for s in stmts:
+        if s.kind in ("assign", "deriv"):
+            out.append(s.target)
+        elif s.kind == "if":
+            for _, body in s.branches:
+                out += _assigned_names(body)
+    return out
+
+
+def emit(mech: Mechanism, source: str = "", fast: Optional[Dict[str, float]] = None) -> str:
     """C++ for one mechanism: a struct of its variables with its blocks as
     member functions (initial(), states() -- the solved DERIVATIVE block --,
-    breakpoint(), and every PROCEDURE and FUNCTION)."""
+    breakpoint(), and every PROCEDURE and FUNCTION).  ``fast``: None for the
+    reference's operation order, else the constants of the fast forms (they
+    need mechs.h's ng_expc, ng_rcp and ng_div)."""
+    if fast is not None:
+        fast = dict(fast, **{k: float(c) for k, c in mech.constants.items()})
     ionvars = []
     for ion, (rd, wr) in mech.ions.items():
         for v in rd + wr:
@@ -452,20 +690,20 @@ def emit(mech: Mechanism, source: str = "") -> str:
             lines.append(f"    NG_HD_M double {b.name}({', '.join('double ' + a for a in b.args)})")
             lines.append("    {")
             lines.append("        double ret_ = 0.0;")
-            lines += _emit(statements(b.body), mech, "        ", list(b.args), ret=b.name)
+            lines += _emit(statements(b.body), mech, "        ", list(b.args), ret=b.name, fast=fast)
             lines.append("        return ret_;")
             lines.append("    }")
     for key, b in mech.blocks.items():
         if b.kind == "PROCEDURE":
             lines.append(f"    NG_HD_M void {b.name}({', '.join('double ' + a for a in b.args)})")
             lines.append("    {")
-            lines += _emit(statements(b.body), mech, "        ", list(b.args))
+            lines += _emit(statements(b.body), mech, "        ", list(b.args), fast=fast)
             lines.append("    }")
     ini = mech.blocks.get("INITIAL")
     lines.append("    NG_HD_M void initial()")
     lines.append("    {")
     if ini is not None:
-        lines += _emit(statements(ini.body), mech, "        ", [])
+        lines += _emit(statements(ini.body), mech, "        ", [], fast=fast)
     lines.append("    }")
     lines.append("    NG_HD_M void states()")
     lines.append("    {")
@@ -473,13 +711,13 @@ def emit(mech: Mechanism, source: str = "") -> str:
         der = mech.blocks.get(f"DERIVATIVE {mech.solve}")
         if der is None:
             raise NmodlError(f"BREAKPOINT solves {mech.solve}, which is not a DERIVATIVE block")
-        lines += _emit(statements(der.body), mech, "        ", [])
+        lines += _emit(statements(der.body), mech, "        ", [], fast=fast)
     lines.append("    }")
     bp = mech.blocks.get("BREAKPOINT")
     lines.append("    NG_HD_M void breakpoint()")
     lines.append("    {")
     if bp is not None:
-        lines += _emit(statements(bp.body), mech, "        ", [])
+        lines += _emit(statements(bp.body), mech, "        ", [], fast=fast)
     lines.append("    }")
     lines.append("};")
     return "\n".join(lines)
@@ -708,12 +946,14 @@ PROLOGUE = """/* Generated by neurogpu_amd.nmodl -- do not edit. */
 """
 
 
-def generate(sources: Dict[str, str], with_harness: bool = False) -> Tuple[str, List[dict]]:
-    """C++ for several .mod texts (name -> text) and their interfaces."""
-    parts, faces = [PROLOGUE], []
+def generate(sources: Dict[str, str], with_harness: bool = False,
+             fast: Optional[Dict[str, float]] = None) -> Tuple[str, List[dict]]:
+    """C++ for several .mod texts (name -> text) and their interfaces
+    (``fast`` as in :func:`emit`; the header then includes mechs.h)."""
+    parts, faces = [PROLOGUE if fast is None else PROLOGUE.replace('"ngmath.h"', '"mechs.h"')], []
     for name, text in sources.items():
         mech = parse(text)
-        parts.append(emit(mech, name))
+        parts.append(emit(mech, name, fast))
         faces.append(interface(mech))
         if with_harness:
             parts.append(harness(mech))
@@ -731,6 +971,10 @@ def main(argv=None):
     ap.add_argument("--celsius", type=float, default=6.3)
     ap.add_argument("--ion", action="append", default=[], help="constant ion value, e.g. ek=-85 (repeatable)")
     ap.add_argument("--masks", default="", help="comma-separated mechanism masks with straight-line code")
+    ap.add_argument("--fast", action="store_true",
+                    help="emit the fast forms (constants folded, refined reciprocals, branch-free exp, "
+                         "cnexp of (X - x) / T without the cancelling quotients) instead of the reference's "
+                         "operation order")
     args = ap.parse_args(argv)
     import os
     srcs = {os.path.basename(p): open(p).read() for p in args.mods}
@@ -741,7 +985,7 @@ def main(argv=None):
         # model order, PYS/extractModel.py:327)
         order = sorted(srcs, key=lambda k: parse(srcs[k]).suffix.lower())
         srcs = {k: srcs[k] for k in order}
-    code, faces = generate(srcs, args.harness)
+    code, faces = generate(srcs, args.harness, {"celsius": args.celsius} if args.fast else None)
     if args.set:
         mechs = [parse(t) for t in srcs.values()]
         ions = {kv.split("=")[0]: float(kv.split("=")[1]) for kv in args.ion}
