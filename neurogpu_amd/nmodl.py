@@ -187,10 +187,9 @@ def parse(text: str) -> Mechanism:
                 if m.group(2) in UNIT_CONSTANTS:
                     consts[m.group(1)] = UNIT_CONSTANTS[m.group(2)]
         elif b.kind in ("PARAMETER", "CONSTANT"):
-            for line in b.body.split("\n"):
-                m = re.match(r"\s*([A-Za-z_]\w*)\s*(?:=\s*([-+0-9.eE]+))?", line)
-                if not m or not line.strip():
-                    continue
+            # name [= value] [(units)] [<lo, hi>], any number to a line
+            body = re.sub(r"<[^>]*>", " ", re.sub(r"\([^)]*\)", " ", b.body))
+            for m in re.finditer(r"([A-Za-z_]\w*)\s*(?:=\s*([-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?))?", body):
                 val = float(m.group(2)) if m.group(2) is not None else None
                 (consts if b.kind == "CONSTANT" else params)[m.group(1)] = val
         elif b.kind == "STATE":

@@ -189,6 +189,78 @@ def test_linear_extraction():
         nmodl.linear(ast.parse("exp(x)", mode="eval").body, "x")
 
 
+def test_declarations_several_to_a_line():
+    m = nmodl.parse("NEURON { SUFFIX t }\nPARAMETER { k = 10 q = 3 gbar = 0.036 (S/cm2) <0, 1e9>\n"
+                    "  e = -70 (mV) x = 1e-5 y = .5 celsius (degC) z=2.5e+3 }\nASSIGNED { v a b }\nSTATE { s u }\n")
+    assert m.params == {"k": 10.0, "q": 3.0, "gbar": 0.036, "e": -70.0, "x": 1e-5, "y": 0.5, "celsius": None,
+                        "z": 2500.0}
+    assert m.assigned == ["v", "a", "b"] and m.states == ["s", "u"]
+
+
+SHARETEST = """
+NEURON {
+    SUFFIX sharetest
+    RANGE a, b, c, d, e
+}
+PARAMETER {
+    k = 10
+    q = 3
+}
+ASSIGNED {
+    v
+    a
+    b
+    c
+    d
+    e
+}
+STATE { s }
+BREAKPOINT { SOLVE st METHOD cnexp }
+INITIAL {
+    LOCAL w
+    a = exp(v/k)
+    b = exp(-v/k)
+    c = exp(-(v - 0)*0.1)
+    w = 2
+    v = v + 1
+    d = exp(-v/10) + w^q
+    if (v > 0) {
+        e = exp(v/10)
+        v = v - 2
+    } else {
+        e = 2^(v/4)
+    }
+    e = e + exp(-v/10)
+    s = 0.5
+}
+DERIVATIVE st { s' = (c - s)/k }
+"""
+
+
+def test_fast_forms_share_and_fold_correctly(tmp_path):
+    """The fast emission's shared exps end where a name in their argument is
+    assigned (here v, in the block and in a branch), constants fold through
+    LOCALs, c^y becomes exp(y ln c), and the results equal the reference-order
+    emission's to rounding on both sides of the branch."""
+    code, _ = nmodl.generate({"sharetest.mod": SHARETEST}, fast={"celsius": 34.0})
+    ini = code[code.index("void initial()"):code.index("void states()")]
+    # c: a shared exp; d: a new one after v = v + 1; e (v > 0): its reciprocal;
+    # after the branch (v assigned in it) a third; a and b divide by the
+    # PARAMETER k, so they are not affine in v and stay separate
+    assert ini.count("const double exp_") == 3 and "e = ng_rcp(exp_1);" in ini
+    assert "ng_expc(ng_div(v, k))" in ini and "ng_expc(ng_div((-v), k))" in ini
+    assert "ng_expc((q * 0.6931471805599453))" in ini        # w^q with the LOCAL w = 2 folded
+    _register_defaults({"sharetest.mod": SHARETEST})
+    out = {}
+    for form in FORMS:
+        (tmp_path / form).mkdir()
+        gen = _build(tmp_path / form, {"sharetest.mod": SHARETEST}, FORMS[form])
+        out[form] = [run_block(gen["sharetest"], 0, v) for v in (-35.0, -0.5, 0.25, 12.0)]
+    for ref, fast in zip(out["reference-order"], out["fast"]):
+        for key in ("a", "b", "c", "d", "e", "s"):
+            assert _close(fast[key], ref[key], 1e-14), (key, fast[key], ref[key])
+
+
 @pytest.mark.parametrize("bad, why", [
     (HHTEST.replace("METHOD cnexp", "METHOD derivimplicit"), "derivimplicit"),
     (HHTEST.replace("c' = 0.5 - 2*c", "c' = 0.5 - 2*c*c"), "linear"),
