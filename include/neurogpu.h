@@ -34,6 +34,9 @@ extern "C" {
                                 .mod files in Figures/Figure2_BBP_TTPC_clean (as
                                 PYS/extractModel.py generates AllModels.cu): the mechanisms,
                                 parameters and tree files of NGPU_MECH_TTPC */
+#define NGPU_MECH_MAINEN_GEN 4 /* the Mainen set generated the same way from
+                                  Figures/FigureS4_mainen's .mod files: the mechanisms,
+                                  parameters and tree files of NGPU_MECH_MAINEN */
 
 typedef struct ngpu_ctx ngpu_ctx;
 

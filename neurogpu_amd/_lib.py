@@ -68,7 +68,7 @@ class ngpu_stats(C.Structure):
 
 
 SOLVE_AUTO, SOLVE_REFERENCE, SOLVE_FAST = 0, 1, 2
-MECH_PASSIVE, MECH_TTPC, MECH_MAINEN, MECH_TTPC_GEN = 0, 1, 2, 3
+MECH_PASSIVE, MECH_TTPC, MECH_MAINEN, MECH_TTPC_GEN, MECH_MAINEN_GEN = 0, 1, 2, 3, 4
 
 
 class ngpu_batch_launch(C.Structure):

@@ -9,6 +9,8 @@
 #endif
 
 
+namespace ng_gen_ttpc {
+
 /* Ca_HVA: generated by neurogpu_amd.nmodl from Ca_HVA.mod */
 struct Mech_Ca_HVA {
     double v = 0.0, dt = 0.0;
@@ -584,7 +586,7 @@ struct SetGen_ttpc {
         ng_dispatch(m, f, NgMasks<0x408u, 0x1618u>{});
     }
     NG_DEV static unsigned ica_bits(unsigned m) { return (m & 0x4u) ? 1u << ICA : 0u; }
-    NG_DEV static double eca_of(double cai) { return 13.23238136699132 * ng_log(2.0 / cai); }
+    NG_DEV static double eca_of(double cai) { return 13.23238136699132 * ng_logc(ng_div(2.0, cai)); }
     NG_DEV static void eca_after_cnexp(unsigned m, double *st)
     {
         if (m & 0x807u) st[ECA] = eca_of(st[CAI]);
@@ -606,9 +608,10 @@ struct SetGen_ttpc {
         if (m & 0x807u) s |= (1u << CAI) | (1u << ECA);
         return s;
     }
-    /* INITIAL blocks in mechanism order.  A calcium mechanism that does not
-     * read cai first sets it to its resting value; every calcium mechanism
-     * sets eca from cai (CuInitModel_*, Code/OneKernel/src/AllModels.cu:202-296) */
+    /* INITIAL blocks in mechanism order.  A calcium mechanism that writes cai
+     * or does not read it first sets it to its resting value; every calcium
+     * mechanism sets eca from cai (CuInitModel_*, Code/OneKernel/src/AllModels.cu:202-296,
+     * Figures/FigureS4_mainen/CParsed/AllModels.cu) */
     template <class PF> NG_DEV static void init(unsigned m, double v, double *st, const PF &P)
     {
         if (m & (1u << 0)) {
@@ -1027,3 +1030,7 @@ struct SetGen_ttpc {
         sum = s;
     }
 };
+
+} // namespace ng_gen_ttpc
+
+using ng_gen_ttpc::SetGen_ttpc;

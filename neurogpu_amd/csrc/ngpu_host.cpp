@@ -27,6 +27,7 @@
 #include "../../include/neurogpu.h"
 #include "sim_kernel.h"
 #include "gen_ttpc.h"
+#include "gen_mainen.h"
 #include "schedule.h"
 
 /* Per-set launchers, one translation unit each (sim_<set>.hip). */
@@ -43,6 +44,8 @@ hipError_t ngpu_launch_batch_ttpc(dim3 grid, size_t lds, hipStream_t s, const Ba
 hipError_t ngpu_launch_batch_mainen(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
 hipError_t ngpu_launch_ttpcgen(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
 hipError_t ngpu_launch_batch_ttpcgen(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
+hipError_t ngpu_launch_mainengen(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
+hipError_t ngpu_launch_batch_mainengen(dim3 grid, size_t lds, hipStream_t s, const BatchArgs &b);
 
 static std::string g_last_error;
 
@@ -52,34 +55,45 @@ static std::string g_last_error;
  * where only the reference's own operation order reproduces its results. */
 static const double kScaledCoef = 1e6;
 
-static int mech_nmodels(int ms)
-{
-    return ms == NGPU_MECH_TTPC ? SetTTPC::NMECH : ms == NGPU_MECH_MAINEN ? SetMainen::NMECH
-         : ms == NGPU_MECH_PASSIVE ? SetPassive::NMECH : ms == NGPU_MECH_TTPC_GEN ? SetGen_ttpc::NMECH : -1;
-}
-static int mech_nparams(int ms)
-{
-    return ms == NGPU_MECH_TTPC ? SetTTPC::NPARAM : ms == NGPU_MECH_MAINEN ? SetMainen::NPARAM
-         : ms == NGPU_MECH_PASSIVE ? SetPassive::NPARAM : ms == NGPU_MECH_TTPC_GEN ? SetGen_ttpc::NPARAM : -1;
-}
 /* First parameter index of each mechanism of a set, plus the total (the
  * parameter layout of the generated AllModels.cuh). */
 static const int kPBaseTTPC[14] = {0, 1, 2, 6, 8, 9, 10, 11, 12, 13, 14, 16, 18, 19};
 static const int kPBaseMainen[8] = {0, 2, 2, 6, 11, 16, 28, 30};
 static const int kPBasePassive[2] = {0, 2};
+
+/* The mechanism sets: counts and parameter layout (a generated set carries
+ * its own PBASE, equal to the hand-written set's for the same model). */
+struct SetInfo {
+    int nmech, nparam, nstate;
+    const int *pbase;
+};
+static const SetInfo *set_info(int ms)
+{
+    static const SetInfo passive{SetPassive::NMECH, SetPassive::NPARAM, SetPassive::NSTATE, kPBasePassive};
+    static const SetInfo ttpc{SetTTPC::NMECH, SetTTPC::NPARAM, SetTTPC::NSTATE, kPBaseTTPC};
+    static const SetInfo mainen{SetMainen::NMECH, SetMainen::NPARAM, SetMainen::NSTATE, kPBaseMainen};
+    static const SetInfo ttpc_gen{SetGen_ttpc::NMECH, SetGen_ttpc::NPARAM, SetGen_ttpc::NSTATE, SetGen_ttpc::PBASE};
+    static const SetInfo mainen_gen{SetGen_mainen::NMECH, SetGen_mainen::NPARAM, SetGen_mainen::NSTATE,
+                                    SetGen_mainen::PBASE};
+    switch (ms) {
+    case NGPU_MECH_PASSIVE: return &passive;
+    case NGPU_MECH_TTPC: return &ttpc;
+    case NGPU_MECH_MAINEN: return &mainen;
+    case NGPU_MECH_TTPC_GEN: return &ttpc_gen;
+    case NGPU_MECH_MAINEN_GEN: return &mainen_gen;
+    default: return nullptr;
+    }
+}
+static int mech_nmodels(int ms) { return set_info(ms) ? set_info(ms)->nmech : -1; }
+static int mech_nparams(int ms) { return set_info(ms) ? set_info(ms)->nparam : -1; }
+static int mech_nstate(int ms) { return set_info(ms) ? set_info(ms)->nstate : SetPassive::NSTATE; }
 static int param_owner(int ms, int j)
 {
-    const int *b = ms == NGPU_MECH_TTPC ? kPBaseTTPC : ms == NGPU_MECH_MAINEN ? kPBaseMainen
-                 : ms == NGPU_MECH_TTPC_GEN ? SetGen_ttpc::PBASE : kPBasePassive;
-    int nm = mech_nmodels(ms);
-    for (int m = 0; m < nm; m++)
-        if (j >= b[m] && j < b[m + 1]) return m;
+    const SetInfo *si = set_info(ms);
+    if (!si) return -1;
+    for (int m = 0; m < si->nmech; m++)
+        if (j >= si->pbase[m] && j < si->pbase[m + 1]) return m;
     return -1;
-}
-static int mech_nstate(int ms)
-{
-    return ms == NGPU_MECH_TTPC ? SetTTPC::NSTATE : ms == NGPU_MECH_MAINEN ? SetMainen::NSTATE
-         : ms == NGPU_MECH_TTPC_GEN ? SetGen_ttpc::NSTATE : SetPassive::NSTATE;
 }
 
 template <class T> struct DevArr {
@@ -723,6 +737,7 @@ static hipError_t launch_one(ngpu_ctx *c, const SimArgs &a, dim3 grid, dim3 bloc
     case NGPU_MECH_TTPC: return ngpu_launch_ttpc(multi, grid, block, lds, c->stream, a);
     case NGPU_MECH_MAINEN: return ngpu_launch_mainen(multi, grid, block, lds, c->stream, a);
     case NGPU_MECH_TTPC_GEN: return ngpu_launch_ttpcgen(multi, grid, block, lds, c->stream, a);
+    case NGPU_MECH_MAINEN_GEN: return ngpu_launch_mainengen(multi, grid, block, lds, c->stream, a);
     default: return ngpu_launch_passive(multi, grid, block, lds, c->stream, a);
     }
 }
@@ -1188,6 +1203,7 @@ try {
     case NGPU_MECH_TTPC: le = ngpu_launch_batch_ttpc(dim3(grid), lds, b->stream, ba); break;
     case NGPU_MECH_MAINEN: le = ngpu_launch_batch_mainen(dim3(grid), lds, b->stream, ba); break;
     case NGPU_MECH_TTPC_GEN: le = ngpu_launch_batch_ttpcgen(dim3(grid), lds, b->stream, ba); break;
+    case NGPU_MECH_MAINEN_GEN: le = ngpu_launch_batch_mainengen(dim3(grid), lds, b->stream, ba); break;
     default: le = ngpu_launch_batch_passive(dim3(grid), lds, b->stream, ba); break;
     }
     if (le != hipSuccess) return bfail(b, NGPU_E_HIP, "batch kernel launch failed: %s", hipGetErrorString(le));

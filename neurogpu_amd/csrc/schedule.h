@@ -30,12 +30,13 @@
 #include <vector>
 
 /* Masks whose mechanism code is specialised (SET::dispatch), by mechanism
- * set: 0 passive, 1 TTPC, 2 Mainen, 3 TTPC generated from NMODL (the same
- * masks).  Positions with other masks run in packed segments. */
+ * set: 0 passive, 1 TTPC, 2 Mainen, 3 and 4 TTPC and Mainen generated from
+ * NMODL (the same masks).  Positions with other masks run in packed
+ * segments. */
 inline bool ng_mask_fixed(int mechset, unsigned m)
 {
     if (mechset == 1 || mechset == 3) return m == 0x408u || m == 0x1618u;
-    if (mechset == 2) return m == 0x6fu;
+    if (mechset == 2 || mechset == 4) return m == 0x6fu;
     return m <= 1u;
 }
 

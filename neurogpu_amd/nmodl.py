@@ -267,7 +267,10 @@ def statements(body: str) -> List[Stmt]:
         line = body[i:] if eol < 0 else body[i:eol]
         i = n if eol < 0 else eol + 1
         line = line.strip()
-        if not line or line in ("UNITSOFF", "UNITSON") or line.startswith("TABLE") or line.startswith("SOLVE"):
+        if (not line or line in ("UNITSOFF", "UNITSON") or line.startswith("TABLE") or line.startswith("SOLVE")
+                or line.startswith("DEPEND") or re.match(r"FROM\b.*\bTO\b.*\bWITH\b", line)):
+            # TABLE (with its DEPEND and FROM ... TO ... WITH lines) asks NEURON
+            # to interpolate; the equations are evaluated as written instead
             continue
         if line.startswith("LOCAL"):
             for v in line[5:].split(","):
@@ -290,7 +293,11 @@ def statements(body: str) -> List[Stmt]:
 
 
 # ------------------------------------------------------------------ expressions
+_NUM_UNITS = re.compile(r"(?<![A-Za-z_\d.])(\d+\.?\d*(?:[eE][-+]?\d+)?|\.\d+(?:[eE][-+]?\d+)?)\s*\(\s*[A-Za-z][\w/ ]*\)")
+
+
 def _py(e: str) -> ast.expr:
+    e = _NUM_UNITS.sub(r"\1", e)                     # a unit after a number: 10 (degC)
     e = e.replace("^", "**")
     try:
         return ast.parse(e, mode="eval").body
@@ -409,7 +416,7 @@ def linear(node: ast.expr, x: str) -> Tuple[ast.expr, ast.expr]:
 #   - cnexp of x' = (X - x) / T is x += (1 - exp(-dt / T)) (X - x), with the
 #     quotients that cancel in -A/B taken out.
 _FOLD = {"exp": math.exp, "log": math.log, "fabs": abs, "sqrt": math.sqrt, "tanh": math.tanh, "sin": math.sin,
-         "cos": math.cos}
+         "cos": math.cos, "ng_rcp": lambda x: 1.0 / x}
 
 
 def _c(x: float) -> ast.expr:
@@ -531,7 +538,7 @@ def fast_expr(node: ast.expr, consts: Dict[str, float], share: Optional[_ExpShar
         if node.func.id in _FOLD and len(args) == 1 and vals[0] is not None:
             try:
                 val = _FOLD[node.func.id](vals[0])
-            except (ValueError, OverflowError):
+            except (ValueError, OverflowError, ZeroDivisionError):
                 val = None
             if _finite(val):
                 return _c(val)
@@ -602,10 +609,11 @@ def _emit(stmts: List[Stmt], mech: Mechanism, ind: str, locals_: List[str], ret:
             e = _py(s.expr)
             code = ex(e)
             if env is not None:
+                val = _num(fast_expr(e, env))
                 env.pop(s.target, None)
                 sh.kill(s.target)
-                val = _num(fast_expr(e, env))
-                if val is not None and s.target in locals_ and tgt != "ret_":
+                # a LOCAL assigned a constant, or a derived constant assigned its value
+                if val is not None and tgt != "ret_" and (s.target in locals_ or fast.get(s.target) == val):
                     env[s.target] = val
             out.append(f"{ind}{tgt} = {code};")
         elif s.kind == "call":
@@ -656,14 +664,55 @@ def _assigned_names(stmts: List[Stmt]) -> List[str]:
     return out
 
 
-def emit(mech: Mechanism, source: str = "", fast: Optional[Dict[str, float]] = None) -> str:
+def derived_constants(mech: Mechanism, known: Dict[str, float]) -> Dict[str, float]:
+    """ASSIGNED variables that every assignment in the mechanism's blocks sets
+    to one and the same constant, given the known constants (celsius, the
+    model's constant parameters, CONSTANTs and UNITS constants): for example
+    Mainen's tadj = q10^((celsius - temp)/10), assigned in INITIAL and in the
+    rates procedure and read in BREAKPOINT.  NEURON keeps such a variable
+    between blocks; here it is a constant of the mechanism."""
+    assigns: Dict[str, list] = {}
+
+    def walk(stmts):
+        for st in stmts:
+            if st.kind in ("assign", "deriv"):
+                assigns.setdefault(st.target, []).append(st.expr if st.kind == "assign" else None)
+            elif st.kind == "if":
+                for _, body in st.branches:
+                    walk(body)
+
+    for b in mech.blocks.values():
+        if b.kind in ("INITIAL", "BREAKPOINT", "DERIVATIVE", "PROCEDURE", "FUNCTION"):
+            walk(statements(b.body))
+    env = dict(known, **{k: float(c) for k, c in mech.constants.items()})
+    out: Dict[str, float] = {}
+    changed = True
+    while changed:
+        changed = False
+        for name, exprs in assigns.items():
+            if name in out or name not in mech.assigned or name in mech.states or any(e is None for e in exprs):
+                continue
+            vals = {_num(fast_expr(_py(e), env)) for e in exprs}
+            if len(vals) == 1 and None not in vals:
+                out[name] = env[name] = vals.pop()
+                changed = True
+    return out
+
+
+def emit(mech: Mechanism, source: str = "", fast: Optional[Dict[str, float]] = None,
+         known: Optional[Dict[str, float]] = None) -> str:
     """C++ for one mechanism: a struct of its variables with its blocks as
     member functions (initial(), states() -- the solved DERIVATIVE block --,
     breakpoint(), and every PROCEDURE and FUNCTION).  ``fast``: None for the
     reference's operation order, else the constants of the fast forms (they
-    need mechs.h's ng_expc, ng_rcp and ng_div)."""
+    need mechs.h's ng_expc, ng_rcp and ng_div).  ``known``: constants known
+    at generation (celsius, the model's constant parameters): the fields of
+    derived constants (derived_constants) start at their value, and the fast
+    forms fold all of them."""
+    known = dict(known or {})
+    derived = derived_constants(mech, dict(known, **(fast or {})))
     if fast is not None:
-        fast = dict(fast, **{k: float(c) for k, c in mech.constants.items()})
+        fast = dict(fast, **known, **derived, **{k: float(c) for k, c in mech.constants.items()})
     ionvars = []
     for ion, (rd, wr) in mech.ions.items():
         for v in rd + wr:
@@ -683,7 +732,7 @@ def emit(mech: Mechanism, source: str = "", fast: Optional[Dict[str, float]] = N
         if name in fields or name in mech.constants:
             continue
         fields.append(name)
-        lines.append(f"    double {name} = 0.0;")
+        lines.append(f"    double {name} = {repr(derived[name]) if name in derived else '0.0'};")
     for key, b in mech.blocks.items():
         if b.kind == "FUNCTION":
             lines.append(f"    NG_HD_M double {b.name}({', '.join('double ' + a for a in b.args)})")
@@ -781,17 +830,36 @@ BREAKPOINT {
 """
 
 
-def set_parameters(mech: Mechanism) -> List[str]:
+def set_parameters(mech: Mechanism, exclude=()) -> List[str]:
     """The parameter-matrix columns a mechanism owns: its PARAMETERs in
-    declaration order, without v, dt, celsius and its ion variables (the
-    layout of the reference's generated parameter list, e.g. Ih's gIhbar,
-    ehcn)."""
+    declaration order, without v, dt, celsius, its ion variables and the
+    names in ``exclude`` (the model's constant parameters) -- the layout of
+    the reference's generated parameter list, e.g. Ih's gIhbar, ehcn, or
+    Mainen's na: gbar, tha, qa, Ra, Rb, thi1, thi2, qi, thinf, qinf, Rg, Rd
+    with vshift, temp, q10, vmin and vmax constant."""
     ionvars = {x for rd, wr in mech.ions.values() for x in rd + wr}
-    return [p for p in mech.params if p not in ("v", "dt", "celsius") and p not in ionvars]
+    return [p for p in mech.params if p not in ("v", "dt", "celsius") and p not in ionvars and p not in exclude]
+
+
+def model_constants(mech: Mechanism, exclude=(), overrides: Optional[Dict[str, float]] = None) -> Dict[str, float]:
+    """Values of a mechanism's constant parameters (``exclude``): the
+    PARAMETER default, or overrides["<name>_<suffix>"] (NEURON's name of a
+    GLOBAL, e.g. vshift_na)."""
+    overrides = overrides or {}
+    out = {}
+    for p in mech.params:
+        if p in exclude:
+            key = f"{p}_{mech.suffix}"
+            val = overrides.get(key, mech.params[p])
+            if val is None:
+                raise NmodlError(f"constant parameter {key} has no value")
+            out[p] = float(val)
+    return out
 
 
 def emit_set(name: str, mechs: List[Mechanism], celsius: float, ion_values: Dict[str, float],
-             masks: List[int], nernst_R: float = 8.3134) -> str:
+             masks: List[int], nernst_R: float = 8.3134, exclude=(),
+             overrides: Optional[Dict[str, float]] = None, fast: bool = False) -> str:
     """A mechanism set for the simulation kernel (the interface of mechs.h's
     SetTTPC) over generated mechanisms, in the given order (bit k = mechanism
     k): state slots in mechanism order, then the calcium concentration, its
@@ -802,7 +870,10 @@ def emit_set(name: str, mechs: List[Mechanism], celsius: float, ion_values: Dict
     the states back.  ``ion_values``: constant ion variables (ek, ena, cao,
     cai0 = the initial inside calcium); eca follows cai by Nernst
     (1000 R (celsius + 273.15) / (2 F) ln(cao / cai)) where a mechanism
-    writes cai.  ``masks``: mechanism masks that get straight-line code."""
+    writes cai.  ``masks``: mechanism masks that get straight-line code.
+    ``exclude``, ``overrides``: the model's constant parameters and their
+    values (model_constants).  ``fast``: eca by the branch-free log and the
+    refined reciprocal, as the fast forms of the mechanisms."""
     slots, nslot = [], 0
     for m in mechs:
         own = [st for st in interface(m)["states"] if not (st.endswith("i") and st[:-1] in m.ions)]
@@ -815,7 +886,7 @@ def emit_set(name: str, mechs: List[Mechanism], celsius: float, ion_values: Dict
     params, pbase = [], []
     for m in mechs:
         pbase.append(len(params))
-        params += [(m.suffix, p) for p in set_parameters(m)]
+        params += [(m.suffix, p) for p in set_parameters(m, exclude)]
     pbase.append(len(params))
     ca_users = sum(1 << k for k, m in enumerate(mechs) if "ca" in m.ions)
     ica_readers = sum(1 << k for k, m in enumerate(mechs) if "ica" in m.ions.get("ca", ([], []))[0])
@@ -826,9 +897,11 @@ def emit_set(name: str, mechs: List[Mechanism], celsius: float, ion_values: Dict
 
     def fill(k: int, m: Mechanism, ica_expr: str, ind: str) -> List[str]:
         out = [f"{ind}Mech_{m.suffix} M;", f"{ind}M.v = v;"]
-        if "celsius" in m.params:
+        if "celsius" in m.params or "celsius" in m.assigned:
             out.append(f"{ind}M.celsius = {celsius!r};")
-        for j, p in enumerate(set_parameters(m)):
+        for p, val in model_constants(m, exclude, overrides).items():
+            out.append(f"{ind}M.{p} = {val!r};")
+        for j, p in enumerate(set_parameters(m, exclude)):
             out.append(f"{ind}M.{p} = P({pbase[k] + j});")
         for st, sl in slots[k].items():
             out.append(f"{ind}M.{st} = st[{sl}];")
@@ -862,7 +935,9 @@ def emit_set(name: str, mechs: List[Mechanism], celsius: float, ion_values: Dict
          f"        ng_dispatch(m, f, NgMasks<{', '.join(hex(x) + 'u' for x in masks)}>{{}});",
          "    }",
          f"    NG_DEV static unsigned ica_bits(unsigned m) {{ return (m & {hex(ica_readers)}u) ? 1u << ICA : 0u; }}",
-         f"    NG_DEV static double eca_of(double cai) {{ return {ktf / 2!r} * ng_log({cao!r} / cai); }}",
+         (f"    NG_DEV static double eca_of(double cai) {{ return {ktf / 2!r} * ng_logc(ng_div({cao!r}, cai)); }}"
+          if fast else
+          f"    NG_DEV static double eca_of(double cai) {{ return {ktf / 2!r} * ng_log({cao!r} / cai); }}"),
          "    NG_DEV static void eca_after_cnexp(unsigned m, double *st)",
          "    {",
          (f"        if (m & {hex(ca_users)}u) st[ECA] = eca_of(st[CAI]);" if cai_writer else "        (void)m; (void)st;"),
@@ -877,15 +952,16 @@ def emit_set(name: str, mechs: List[Mechanism], celsius: float, ion_values: Dict
     if ca:
         L.append(f"        if (m & {hex(ca_users)}u) s |= (1u << CAI) | (1u << ECA);")
     L += ["        return s;", "    }",
-          "    /* INITIAL blocks in mechanism order.  A calcium mechanism that does not",
-          "     * read cai first sets it to its resting value; every calcium mechanism",
-          "     * sets eca from cai (CuInitModel_*, Code/OneKernel/src/AllModels.cu:202-296) */",
+          "    /* INITIAL blocks in mechanism order.  A calcium mechanism that writes cai",
+          "     * or does not read it first sets it to its resting value; every calcium",
+          "     * mechanism sets eca from cai (CuInitModel_*, Code/OneKernel/src/AllModels.cu:202-296,",
+          "     * Figures/FigureS4_mainen/CParsed/AllModels.cu) */",
           "    template <class PF> NG_DEV static void init(unsigned m, double v, double *st, const PF &P)",
           "    {"]
     for k, m in enumerate(mechs):
         L.append(f"        if (m & (1u << {k})) {{")
         if "ca" in m.ions:
-            if "cai" not in m.ions["ca"][0]:
+            if "cai" not in m.ions["ca"][0] or "cai" in m.ions["ca"][1]:
                 L.append(f"            st[CAI] = {cai0!r};")
             if cai_writer:
                 L.append("            st[ECA] = eca_of(st[CAI]);")
@@ -946,16 +1022,32 @@ PROLOGUE = """/* Generated by neurogpu_amd.nmodl -- do not edit. */
 
 
 def generate(sources: Dict[str, str], with_harness: bool = False,
-             fast: Optional[Dict[str, float]] = None) -> Tuple[str, List[dict]]:
+             fast: Optional[Dict[str, float]] = None,
+             known: Optional[Dict[str, Dict[str, float]]] = None,
+             namespace: str = "", extra: str = "", export: str = "") -> Tuple[str, List[dict]]:
     """C++ for several .mod texts (name -> text) and their interfaces
-    (``fast`` as in :func:`emit`; the header then includes mechs.h)."""
+    (``fast`` as in :func:`emit`; the header then includes mechs.h;
+    ``known``: per mechanism suffix, constants known at generation, with the
+    entry "" applying to every mechanism).  With ``namespace`` the structs,
+    and ``extra`` after them, are in that namespace (so that several
+    generated sets can share a translation unit), and ``export`` is made
+    visible outside it."""
+    known = known or {}
     parts, faces = [PROLOGUE if fast is None else PROLOGUE.replace('"ngmath.h"', '"mechs.h"')], []
+    if namespace:
+        parts.append(f"namespace {namespace} {{")
     for name, text in sources.items():
         mech = parse(text)
-        parts.append(emit(mech, name, fast))
+        parts.append(emit(mech, name, fast, dict(known.get("", {}), **known.get(mech.suffix, {}))))
         faces.append(interface(mech))
         if with_harness:
             parts.append(harness(mech))
+    if extra:
+        parts.append(extra)
+    if namespace:
+        parts.append(f"}} // namespace {namespace}")
+        if export:
+            parts.append(f"using {namespace}::{export};")
     return "\n\n".join(parts) + "\n", faces
 
 
@@ -974,6 +1066,11 @@ def main(argv=None):
                     help="emit the fast forms (constants folded, refined reciprocals, branch-free exp, "
                          "cnexp of (X - x) / T without the cancelling quotients) instead of the reference's "
                          "operation order")
+    ap.add_argument("--constant", default="",
+                    help="comma-separated PARAMETER names that are constants of the model, not parameter-matrix "
+                         "columns (e.g. temp,q10,vmin,vmax,vshift)")
+    ap.add_argument("--global", dest="globals_", action="append", default=[],
+                    help="value of a constant parameter, by NEURON's GLOBAL name, e.g. vshift_na=-5 (repeatable)")
     args = ap.parse_args(argv)
     import os
     srcs = {os.path.basename(p): open(p).read() for p in args.mods}
@@ -984,13 +1081,24 @@ def main(argv=None):
         # model order, PYS/extractModel.py:327)
         order = sorted(srcs, key=lambda k: parse(srcs[k]).suffix.lower())
         srcs = {k: srcs[k] for k in order}
-    code, faces = generate(srcs, args.harness, {"celsius": args.celsius} if args.fast else None)
+    exclude = tuple(x.strip() for x in args.constant.split(",") if x.strip())
+    overrides = {kv.split("=")[0]: float(kv.split("=")[1]) for kv in args.globals_}
+    known = {"": {"celsius": args.celsius}}
+    for text in srcs.values():
+        m = parse(text)
+        known[m.suffix] = model_constants(m, exclude, overrides)
+    extra, ns = "", ""
     if args.set:
         mechs = [parse(t) for t in srcs.values()]
         ions = {kv.split("=")[0]: float(kv.split("=")[1]) for kv in args.ion}
         masks = [int(x, 0) for x in args.masks.split(",") if x.strip()]
+        extra = emit_set(args.set, mechs, args.celsius, ions, masks, exclude=exclude, overrides=overrides,
+                         fast=args.fast)
+        ns = f"ng_gen_{args.set}"
+    code, faces = generate(srcs, args.harness, {"celsius": args.celsius} if args.fast else None, known,
+                           namespace=ns, extra=extra, export=f"SetGen_{args.set}" if args.set else "")
+    if args.set:
         code = code.replace('#include "ngmath.h"', '#include "mechs.h"', 1)
-        code += "\n" + emit_set(args.set, mechs, args.celsius, ions, masks) + "\n"
     with open(args.output, "w") as fh:
         fh.write(code)
     if args.interface:

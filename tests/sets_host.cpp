@@ -1,11 +1,13 @@
 /*
- * sets_host.cpp -- the hand-written TTPC mechanism set (neurogpu_amd/csrc/
- * mechs.h, SetTTPC) and the one generated from the model's NMODL files
- * (neurogpu_amd/csrc/gen_ttpc.h, SetGen_ttpc), compiled for the host by
- * tests/test_nmodl.py so that the two can be compared compartment by
+ * sets_host.cpp -- the hand-written mechanism sets (neurogpu_amd/csrc/mechs.h:
+ * SetTTPC, SetMainen) and the ones generated from the models' NMODL files
+ * (neurogpu_amd/csrc/gen_ttpc.h, gen_mainen.h: SetGen_ttpc, SetGen_mainen),
+ * compiled for the host by tests/test_nmodl.py so that each generated set
+ * can be compared with its hand-written counterpart compartment by
  * compartment on the CPU.  Test infrastructure only.
  */
 #include "gen_ttpc.h"
+#include "gen_mainen.h"
 
 struct HostParams {
     const double *p;
@@ -23,32 +25,60 @@ template <class S> static int run(unsigned m, int op, double v, double dt, doubl
     return -1;
 }
 
-/* which: 0 hand-written, 1 generated.  op 0 init, 1 cnexp (ica = io[1]) then
- * the reversal update, 2 current (io[0] += i, io[1] += ica). */
+template <class S> static void shape(int *out)
+{
+    out[0] = S::NMECH; out[1] = S::NPARAM; out[2] = S::NSTATE;
+    out[3] = S::CAI; out[4] = S::ECA; out[5] = S::ICA;
+}
+
+/* which: 0 SetTTPC, 1 SetGen_ttpc, 2 SetMainen, 3 SetGen_mainen.  op 0 init,
+ * 1 cnexp (ica = io[1]) then the reversal update, 2 current (io[0] += i,
+ * io[1] += ica). */
 extern "C" int set_run(int which, unsigned m, int op, double v, double dt, double *st, const double *P, double *io)
 {
-    return which ? run<SetGen_ttpc>(m, op, v, dt, st, P, io) : run<SetTTPC>(m, op, v, dt, st, P, io);
+    switch (which) {
+    case 0: return run<SetTTPC>(m, op, v, dt, st, P, io);
+    case 1: return run<SetGen_ttpc>(m, op, v, dt, st, P, io);
+    case 2: return run<SetMainen>(m, op, v, dt, st, P, io);
+    case 3: return run<SetGen_mainen>(m, op, v, dt, st, P, io);
+    }
+    return -1;
 }
 
 extern "C" unsigned set_state_bits(int which, unsigned m)
 {
-    return which ? SetGen_ttpc::state_bits(m) : SetTTPC::state_bits(m);
+    switch (which) {
+    case 0: return SetTTPC::state_bits(m);
+    case 1: return SetGen_ttpc::state_bits(m);
+    case 2: return SetMainen::state_bits(m);
+    default: return SetGen_mainen::state_bits(m);
+    }
 }
 
 extern "C" unsigned set_ica_bits(int which, unsigned m)
 {
-    return which ? SetGen_ttpc::ica_bits(m) : SetTTPC::ica_bits(m);
+    switch (which) {
+    case 0: return SetTTPC::ica_bits(m);
+    case 1: return SetGen_ttpc::ica_bits(m);
+    case 2: return SetMainen::ica_bits(m);
+    default: return SetGen_mainen::ica_bits(m);
+    }
 }
 
+/* counts and ion slots; the generated sets also their parameter bases */
 extern "C" int set_shape(int which, int *out)
 {
-    if (which) {
-        out[0] = SetGen_ttpc::NMECH; out[1] = SetGen_ttpc::NPARAM; out[2] = SetGen_ttpc::NSTATE;
-        out[3] = SetGen_ttpc::CAI; out[4] = SetGen_ttpc::ECA; out[5] = SetGen_ttpc::ICA;
+    switch (which) {
+    case 0: shape<SetTTPC>(out); return 0;
+    case 2: shape<SetMainen>(out); return 0;
+    case 1:
+        shape<SetGen_ttpc>(out);
         for (int k = 0; k <= SetGen_ttpc::NMECH; k++) out[6 + k] = SetGen_ttpc::PBASE[k];
-    } else {
-        out[0] = SetTTPC::NMECH; out[1] = SetTTPC::NPARAM; out[2] = SetTTPC::NSTATE;
-        out[3] = SetTTPC::CAI; out[4] = SetTTPC::ECA; out[5] = SetTTPC::ICA;
+        return 0;
+    case 3:
+        shape<SetGen_mainen>(out);
+        for (int k = 0; k <= SetGen_mainen::NMECH; k++) out[6 + k] = SetGen_mainen::PBASE[k];
+        return 0;
     }
-    return 0;
+    return -1;
 }

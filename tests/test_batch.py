@@ -10,7 +10,7 @@ import pytest
 
 from conftest import TOL_MV, spike_counts
 from neurogpu_amd import autotune, models
-from neurogpu_amd._lib import MECH_TTPC_GEN, Batch, NgpuError, workload_simulator
+from neurogpu_amd._lib import MECH_MAINEN_GEN, MECH_TTPC_GEN, Batch, NgpuError, workload_simulator
 from oracle import cpu_ref
 
 
@@ -88,12 +88,13 @@ def test_tree_key_is_stable_and_distinct():
 @pytest.mark.gpu
 @pytest.mark.parametrize("mechset,choice", [("mainen", None), ("mainen", "wave_global"), ("mainen", "block_lds"),
                                             ("mainen", "two_waves"), ("ttpc", None), ("ttpc", "two_waves"),
-                                            ("ttpc_gen", None)])
+                                            ("ttpc_gen", None), ("mainen_gen", None)])
 def test_batch_matches_single_runs_and_oracle(mechset, choice):
-    """ttpc_gen: the TTPC cells through the mechanism set generated from
-    their NMODL files (MECH_TTPC_GEN), checked against the same oracle."""
-    gen = MECH_TTPC_GEN if mechset == "ttpc_gen" else None
-    ws = _mixed(mechset="ttpc" if gen else mechset)
+    """ttpc_gen, mainen_gen: the cells through the mechanism sets generated
+    from their NMODL files (MECH_TTPC_GEN, MECH_MAINEN_GEN), checked against
+    the same oracle."""
+    gen = {"ttpc_gen": MECH_TTPC_GEN, "mainen_gen": MECH_MAINEN_GEN}.get(mechset)
+    ws = _mixed(mechset=mechset[:-len("_gen")] if gen else mechset)
     _say(f"case batch of {[w.model.tree.N for w in ws]} {mechset} segments, choice {choice}")
     alone = [_alone(w, gen) for w in ws]
     sims = [workload_simulator(w, mechset=gen) for w in ws]

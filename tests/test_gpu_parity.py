@@ -11,7 +11,7 @@ import pytest
 
 from conftest import TOL_MV, TTPC_DIR, spike_counts
 from neurogpu_amd import formats, models
-from neurogpu_amd._lib import MECH_TTPC_GEN, SOLVE_FAST, SOLVE_REFERENCE, NgpuError, Simulator, simulate_workload
+from neurogpu_amd._lib import MECH_MAINEN_GEN, MECH_TTPC_GEN, SOLVE_FAST, SOLVE_REFERENCE, NgpuError, Simulator, simulate_workload
 from oracle import cpu_ref
 
 pytestmark = pytest.mark.gpu
@@ -87,6 +87,29 @@ def test_ttpc1200_generated_mechanisms_full_protocol():
     _check(gen, _oracle(w), "ttpc1200 generated full protocol")
     d = float(np.max(np.abs(gen - hand)))
     _say(f"ttpc1200: max|generated - hand-written| = {d:.3e} mV")
+    assert d <= TOL_MV
+
+
+@pytest.mark.parametrize("launch", [None, dict(waves_per_run=1, runs_per_group=4), dict(waves_per_run=4, runs_per_group=3)])
+def test_active500_generated_mechanisms_small(launch):
+    """The Mainen cell through the mechanism set generated from its NMODL
+    files (MECH_MAINEN_GEN, gen_mainen.h): the oracle's bar."""
+    w = models.workload("active500", nsets=6, nt=1500)
+    _say(f"case active500 generated mechanisms launch={launch}")
+    gpu, st = simulate_workload(w, launch=launch, mechset=MECH_MAINEN_GEN)
+    _check(gpu, _oracle(w), f"active500 generated {launch}")
+
+
+def test_active500_generated_mechanisms_full_protocol():
+    """Configuration 2's whole 10,000-step protocol with the generated Mainen
+    set: the oracle's bar, and within rounding of the hand-written set."""
+    w = models.workload("active500", nsets=4)
+    _say(f"case active500 generated mechanisms, full protocol: {w.nruns} runs x {w.dts.size} steps")
+    gen, _ = simulate_workload(w, mechset=MECH_MAINEN_GEN)
+    hand, _ = simulate_workload(w)
+    _check(gen, _oracle(w), "active500 generated full protocol")
+    d = float(np.max(np.abs(gen - hand)))
+    _say(f"active500: max|generated - hand-written| = {d:.3e} mV")
     assert d <= TOL_MV
 
 

@@ -405,13 +405,18 @@ __global__ void k(double *v, double *out, int n)
 
 
 # ------------------------------------------- the generated set in the simulator
-def test_committed_ttpc_set_is_the_generators_output(tmp_path):
-    """neurogpu_amd/csrc/gen_ttpc.h (mechanism set NGPU_MECH_TTPC_GEN) is what
-    the Makefile's gen recipe writes from the model's .mod files today."""
-    _ttpc_sources()
-    out = tmp_path / "gen_ttpc.h"
-    subprocess.run(["make", "-s", "-C", CSRC, "gen", f"GEN_OUT={out}"], check=True)
-    assert out.read_text() == open(os.path.join(CSRC, "gen_ttpc.h")).read()
+@pytest.mark.parametrize("header,target,var,moddir", [
+    ("gen_ttpc.h", "gen", "GEN_OUT", TTPC_MODS),
+    ("gen_mainen.h", "gen-mainen", "GEN_MAINEN_OUT", "/root/reference/Figures/FigureS4_mainen")])
+def test_committed_sets_are_the_generators_output(tmp_path, header, target, var, moddir):
+    """neurogpu_amd/csrc/gen_ttpc.h and gen_mainen.h (mechanism sets
+    NGPU_MECH_TTPC_GEN and NGPU_MECH_MAINEN_GEN) are what the Makefile's gen
+    recipes write from the models' .mod files today."""
+    if not os.path.isdir(moddir):
+        pytest.skip("the reference's .mod files are not present here")
+    out = tmp_path / header
+    subprocess.run(["make", "-s", "-C", CSRC, target, f"{var}={out}"], check=True)
+    assert out.read_text() == open(os.path.join(CSRC, header)).read()
 
 
 @pytest.fixture(scope="module")
@@ -436,77 +441,86 @@ def _shape(H, which):
     return list(out)
 
 
-def test_generated_set_keeps_the_parameter_layout(sets):
-    """The generated set reads the parameter matrix of the hand-written TTPC
-    set: its columns per mechanism are the parameters named for that mechanism,
-    in mechanism order (the layout of the reference's AllParams rows)."""
+# (hand-written set, generated set, mechanism set name, workload, a fix-up for
+# random masks): a calcium reader without a mechanism that sets cai leaves
+# cai at 0 and eca = log(cao / 0), undefined in both sets (no cell places a
+# mechanism so): SK_E2 alone gets CaDynamics_E2, ca or kca alone get cad
+FAMILIES = {
+    "ttpc": (0, 1, "ttpc", "ttpc1200", lambda mk: mk | 0x4 if mk & 0x800 and not mk & 0x7 else mk),
+    "mainen": (2, 3, "mainen", "active500", lambda mk: mk | 0x2 if mk & 0x5 and not mk & 0x2 else mk),
+}
+
+
+@pytest.mark.parametrize("family", list(FAMILIES))
+def test_generated_set_keeps_the_parameter_layout(sets, family):
+    """A generated set reads the parameter matrix of the hand-written set of
+    the same model: its columns per mechanism are the parameters named for
+    that mechanism, in mechanism order (the layout of the reference's
+    AllParams rows)."""
     from neurogpu_amd import models
-    g, h = _shape(sets, 1), _shape(sets, 0)
-    assert g[:6] == h[:6] == [13, 19, 21, 18, 19, 20]
-    ms = models.MECHSETS["ttpc"]
+    hw, gw, ms_name, _, _ = FAMILIES[family]
+    g, h = _shape(sets, gw), _shape(sets, hw)
+    assert g[:6] == h[:6]
+    ms = models.MECHSETS[ms_name]
     base, names = [0], ms["params"]
     for mech in ms["mechs"]:
         base.append(base[-1] + sum(1 for p in names if p.endswith("_" + mech)))
-    assert base[-1] == len(names) == 19
-    assert g[6:6 + 14] == base
+    assert base[-1] == len(names) == g[1]
+    assert g[6:6 + len(ms["mechs"]) + 1] == base
 
 
-def _slot_map(H, mask):
+def _slot_map(H, mask, hw, gw):
     """(hand slot, generated slot) pairs of the states of the mechanisms in
     mask, matched by mechanism and by order within a mechanism."""
-    h, g = _shape(H, 0), _shape(H, 1)
+    h, g = _shape(H, hw), _shape(H, gw)
     pairs = []
-    for k in range(13):
+    for k in range(h[0]):
         if not mask >> k & 1:
             continue
-        ion = (1 << h[3]) | (1 << h[4])
-        hs = [b for b in range(32) if (H.set_state_bits(0, 1 << k) & ~ion) >> b & 1]
-        gs = [b for b in range(32) if (H.set_state_bits(1, 1 << k) & ~((1 << g[3]) | (1 << g[4]))) >> b & 1]
+        hs = [b for b in range(32) if (H.set_state_bits(hw, 1 << k) & ~((1 << h[3]) | (1 << h[4]))) >> b & 1]
+        gs = [b for b in range(32) if (H.set_state_bits(gw, 1 << k) & ~((1 << g[3]) | (1 << g[4]))) >> b & 1]
         assert len(hs) == len(gs), k
         pairs += list(zip(hs, gs))
-    if H.set_state_bits(0, mask) >> h[3] & 1:
+    if H.set_state_bits(hw, mask) >> h[3] & 1:
         pairs += [(h[3], g[3]), (h[4], g[4])]
     return pairs
 
 
-def test_generated_set_steps_like_the_hand_written_one(sets):
-    """Every mechanism mask of the TTPC cell, and random ones, for 200 steps of
-    a swept voltage: init, current (with the calcium current) and cnexp of
-    the generated set against the hand-written set, state by state (relative
-    1e-11; they differ in how the rate equations are arranged)."""
+@pytest.mark.parametrize("family", list(FAMILIES))
+def test_generated_set_steps_like_the_hand_written_one(sets, family):
+    """Every mechanism mask of the family's cell, and random ones, for 200
+    steps of a swept voltage: init, current (with the calcium current) and
+    cnexp of the generated set against the hand-written set, state by state
+    (relative 1e-11; they differ in how the rate equations are arranged)."""
     from neurogpu_amd import models
-    m = models.workload("ttpc1200", nsets=1, nt=8).model
-    P = m.allparams(m.nominal()[None])[0].reshape(19, m.ncomps)
+    hw, gw, _, wname, fix = FAMILIES[family]
+    m = models.workload(wname, nsets=1, nt=8).model
+    nmech, npar, nst = _shape(sets, hw)[:3]
+    P = m.allparams(m.nominal()[None])[0].reshape(npar, m.ncomps)
     masks = {}
     for seg in range(m.tree.N):
-        mk = int(sum(1 << k for k in range(13) if m.tree.bool_model[k, seg]))
+        mk = int(sum(1 << k for k in range(nmech) if m.tree.bool_model[k, seg]))
         masks.setdefault(mk, int(m.tree.seg_to_comp[seg]))
-    assert {0x408, 0x1618} <= set(masks)
     cols = {mk: np.ascontiguousarray(P[:, c]) for mk, c in masks.items()}
     # random masks take each mechanism's parameters from a segment that holds it
-    g = _shape(sets, 1)
-    pall = np.zeros(19)
-    for k in range(13):
+    g = _shape(sets, gw)
+    pall = np.zeros(npar)
+    for k in range(nmech):
         seg = int(np.flatnonzero(m.tree.bool_model[k])[0])
         lo, hi = g[6 + k], g[7 + k]
         pall[lo:hi] = P[lo:hi, int(m.tree.seg_to_comp[seg])]
     rng = np.random.default_rng(5)
     for _ in range(12):
-        mk = int(rng.integers(1, 1 << 13))
-        if mk & 0x800 and not mk & 0x7:
-            # SK_E2 reads cai: without a calcium mechanism cai stays 0 and eca
-            # is log(cao / 0), left undefined by both (no cell places SK_E2 so)
-            mk |= 0x4
-        cols.setdefault(mk, pall)
+        cols.setdefault(fix(int(rng.integers(1, 1 << nmech))), pall)
     H = sets
     worst = 0.0
     for mk, p in cols.items():
-        pairs = _slot_map(H, mk)
-        sh, sg = np.zeros(21), np.zeros(21)
+        pairs = _slot_map(H, mk, hw, gw)
+        sh, sg = np.zeros(nst), np.zeros(nst)
         io_h, io_g = np.zeros(2), np.zeros(2)
         v = -75.0
-        assert H.set_run(0, mk, 0, v, 0.0, sh.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_h.ctypes.data_as(_DP)) == 0
-        assert H.set_run(1, mk, 0, v, 0.0, sg.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_g.ctypes.data_as(_DP)) == 0
+        assert H.set_run(hw, mk, 0, v, 0.0, sh.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_h.ctypes.data_as(_DP)) == 0
+        assert H.set_run(gw, mk, 0, v, 0.0, sg.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_g.ctypes.data_as(_DP)) == 0
         for step in range(200):
             for a, b in pairs:
                 assert _close(sg[b], sh[a], 1e-11), (hex(mk), step, a, b, sg[b], sh[a])
@@ -514,11 +528,12 @@ def test_generated_set_steps_like_the_hand_written_one(sets):
             v = -70.0 + 60.0 * math.sin(0.05 * step)
             io_h[:] = 0.0
             io_g[:] = 0.0
-            H.set_run(0, mk, 2, v, 0.0, sh.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_h.ctypes.data_as(_DP))
-            H.set_run(1, mk, 2, v, 0.0, sg.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_g.ctypes.data_as(_DP))
+            H.set_run(hw, mk, 2, v, 0.0, sh.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_h.ctypes.data_as(_DP))
+            H.set_run(gw, mk, 2, v, 0.0, sg.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_g.ctypes.data_as(_DP))
             assert _close(io_g[0], io_h[0], 1e-11) and _close(io_g[1], io_h[1], 1e-11), (hex(mk), step, io_g, io_h)
             io_g[1] = io_h[1]
-            H.set_run(0, mk, 1, v, 0.025, sh.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_h.ctypes.data_as(_DP))
-            H.set_run(1, mk, 1, v, 0.025, sg.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_g.ctypes.data_as(_DP))
-    assert H.set_ica_bits(0, 0x1fff) == H.set_ica_bits(1, 0x1fff) != 0
-    print(f"worst relative state difference {worst:.2e} over {len(cols)} masks")
+            H.set_run(hw, mk, 1, v, 0.025, sh.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_h.ctypes.data_as(_DP))
+            H.set_run(gw, mk, 1, v, 0.025, sg.ctypes.data_as(_DP), p.ctypes.data_as(_DP), io_g.ctypes.data_as(_DP))
+    full = (1 << nmech) - 1
+    assert H.set_ica_bits(hw, full) == H.set_ica_bits(gw, full) != 0
+    print(f"{family}: worst relative state difference {worst:.2e} over {len(cols)} masks")

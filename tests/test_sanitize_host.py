@@ -38,7 +38,7 @@ def open_exe(tmp_path_factory):
     drv = str(d / "drv.o")
     subprocess.run([HIPCC] + flags + host + ["-c", os.path.join(ROOT, "tests", "sanitize_open.cpp"), "-o", drv],
                    check=True, capture_output=True)
-    others = [os.path.join(BUILD, f"{n}.o") for n in ("sim_passive", "sim_ttpc", "sim_mainen", "sim_ttpcgen", "features", "mlp",
+    others = [os.path.join(BUILD, f"{n}.o") for n in ("sim_passive", "sim_ttpc", "sim_mainen", "sim_ttpcgen", "sim_mainengen", "features", "mlp",
                                                         "params")]
     exe = str(d / "sanitize_open")
     subprocess.run([HIPCC, "--offload-arch=gfx950", "-fsanitize=address", "-fsanitize=undefined", "-fno-gpu-sanitize",
