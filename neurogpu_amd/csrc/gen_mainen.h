@@ -55,15 +55,22 @@ struct Mech_ca {
         double a = 0.0;
         double b = 0.0;
         tadj = 3.2093639532679714;
-        a = (0.209 * efun(((-(27.0 + vm)) * 0.2631578947368421)));
-        const double exp_0 = ng_expc(((-75.0 - vm) * 0.058823529411764705));
-        b = (0.94 * exp_0);
+        double efun_v0 = 0.0;
+        if ((fabs(((-(27.0 + vm)) * 0.2631578947368421)) < 0.0001)) {
+            efun_v0 = (1.0 - (((-(27.0 + vm)) * 0.2631578947368421) * 0.5));
+        } else {
+            const double exp_0 = ng_expc(((-(27.0 + vm)) * 0.2631578947368421));
+            efun_v0 = ng_div(((-(27.0 + vm)) * 0.2631578947368421), (exp_0 - 1.0));
+        }
+        a = (0.209 * efun_v0);
+        const double exp_1 = ng_expc(((-75.0 - vm) * 0.058823529411764705));
+        b = (0.94 * exp_1);
         mtau = ng_div(0.3115882195229801, (a + b));
         minf = ng_div(a, (a + b));
-        const double exp_1 = ng_expc(((-13.0 - vm) * 0.02));
-        a = (0.000457 * exp_1);
-        const double exp_2 = ng_expc((((-vm) - 15.0) * 0.03571428571428571));
-        b = ng_div(0.0065, (exp_2 + 1.0));
+        const double exp_2 = ng_expc(((-13.0 - vm) * 0.02));
+        a = (0.000457 * exp_2);
+        const double exp_3 = ng_expc((((-vm) - 15.0) * 0.03571428571428571));
+        b = ng_div(0.0065, (exp_3 + 1.0));
         htau = ng_div(0.3115882195229801, (a + b));
         hinf = ng_div(a, (a + b));
     }
@@ -203,8 +210,21 @@ struct Mech_km {
     }
     NG_HD_M void rates(double v)
     {
-        a = ((Ra * qa) * efun(ng_div((-(v - tha)), qa)));
-        b = ((Rb * qa) * efun(ng_div((v - tha), qa)));
+        double efun_v0 = 0.0;
+        const double exp_0 = ng_expc(ng_div((-(v - tha)), qa));
+        if ((fabs(ng_div((-(v - tha)), qa)) < 0.0001)) {
+            efun_v0 = (1.0 - (ng_div((-(v - tha)), qa) * 0.5));
+        } else {
+            efun_v0 = ng_div(ng_div((-(v - tha)), qa), (exp_0 - 1.0));
+        }
+        a = ((Ra * qa) * efun_v0);
+        double efun_v1 = 0.0;
+        if ((fabs(ng_div((v - tha), qa)) < 0.0001)) {
+            efun_v1 = (1.0 - (ng_div((v - tha), qa) * 0.5));
+        } else {
+            efun_v1 = ng_div(ng_div((v - tha), qa), (ng_rcp(exp_0) - 1.0));
+        }
+        b = ((Rb * qa) * efun_v1);
         tadj = 3.2093639532679714;
         ntau = ng_div(0.3115882195229801, (a + b));
         ninf = ng_div(a, (a + b));
@@ -266,8 +286,21 @@ struct Mech_kv {
     }
     NG_HD_M void rates(double v)
     {
-        a = ((Ra * qa) * efun(ng_div((-(v - tha)), qa)));
-        b = ((Rb * qa) * efun(ng_div((v - tha), qa)));
+        double efun_v0 = 0.0;
+        const double exp_0 = ng_expc(ng_div((-(v - tha)), qa));
+        if ((fabs(ng_div((-(v - tha)), qa)) < 0.0001)) {
+            efun_v0 = (1.0 - (ng_div((-(v - tha)), qa) * 0.5));
+        } else {
+            efun_v0 = ng_div(ng_div((-(v - tha)), qa), (exp_0 - 1.0));
+        }
+        a = ((Ra * qa) * efun_v0);
+        double efun_v1 = 0.0;
+        if ((fabs(ng_div((v - tha), qa)) < 0.0001)) {
+            efun_v1 = (1.0 - (ng_div((v - tha), qa) * 0.5));
+        } else {
+            efun_v1 = ng_div(ng_div((v - tha), qa), (ng_rcp(exp_0) - 1.0));
+        }
+        b = ((Rb * qa) * efun_v1);
         tadj = 3.2093639532679714;
         ntau = ng_div(0.3115882195229801, (a + b));
         ninf = ng_div(a, (a + b));
@@ -325,7 +358,8 @@ struct Mech_na {
     {
         double ret_ = 0.0;
         if ((fabs(ng_div((v - th), q)) > 1e-06)) {
-            ret_ = ng_div((a * (v - th)), (1.0 - ng_expc(ng_div((-(v - th)), q))));
+            const double exp_0 = ng_expc(ng_div((-(v - th)), q));
+            ret_ = ng_div((a * (v - th)), (1.0 - exp_0));
         } else {
             ret_ = (a * q);
         }
@@ -339,15 +373,43 @@ struct Mech_na {
     {
         double a = 0.0;
         double b = 0.0;
-        a = trap0(vm, tha, Ra, qa);
-        b = trap0((-vm), (-tha), Rb, qa);
+        double trap0_v0 = 0.0;
+        const double exp_0 = ng_expc(ng_div((-(vm - tha)), qa));
+        if ((fabs(ng_div((vm - tha), qa)) > 1e-06)) {
+            trap0_v0 = ng_div((Ra * (vm - tha)), (1.0 - exp_0));
+        } else {
+            trap0_v0 = (Ra * qa);
+        }
+        a = trap0_v0;
+        double trap0_v1 = 0.0;
+        if ((fabs(ng_div(((-vm) - (-tha)), qa)) > 1e-06)) {
+            trap0_v1 = ng_div((Rb * ((-vm) - (-tha))), (1.0 - ng_rcp(exp_0)));
+        } else {
+            trap0_v1 = (Rb * qa);
+        }
+        b = trap0_v1;
         tadj = 3.2093639532679714;
         mtau = ng_div(0.3115882195229801, (a + b));
         minf = ng_div(a, (a + b));
-        a = trap0(vm, thi1, Rd, qi);
-        b = trap0((-vm), (-thi2), Rg, qi);
+        double trap0_v2 = 0.0;
+        if ((fabs(ng_div((vm - thi1), qi)) > 1e-06)) {
+            const double exp_1 = ng_expc(ng_div((-(vm - thi1)), qi));
+            trap0_v2 = ng_div((Rd * (vm - thi1)), (1.0 - exp_1));
+        } else {
+            trap0_v2 = (Rd * qi);
+        }
+        a = trap0_v2;
+        double trap0_v3 = 0.0;
+        if ((fabs(ng_div(((-vm) - (-thi2)), qi)) > 1e-06)) {
+            const double exp_2 = ng_expc(ng_div((-((-vm) - (-thi2))), qi));
+            trap0_v3 = ng_div((Rg * ((-vm) - (-thi2))), (1.0 - exp_2));
+        } else {
+            trap0_v3 = (Rg * qi);
+        }
+        b = trap0_v3;
         htau = ng_div(0.3115882195229801, (a + b));
-        hinf = ng_rcp((1.0 + ng_expc(ng_div((vm - thinf), qinf))));
+        const double exp_3 = ng_expc(ng_div((vm - thinf), qinf));
+        hinf = ng_rcp((1.0 + exp_3));
     }
     NG_HD_M void initial()
     {

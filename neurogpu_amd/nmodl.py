@@ -32,6 +32,7 @@ from __future__ import annotations
 
 import argparse
 import ast
+import copy
 import dataclasses
 import json
 import math
@@ -431,56 +432,92 @@ def _finite(x) -> bool:
     return isinstance(x, float) and math.isfinite(x)
 
 
-def _affine(node: ast.expr) -> Optional[Tuple[Dict[str, float], float]]:
-    """(coefficients, constant) when node is an affine function of names with
-    literal coefficients, else None."""
-    if _num(node) is not None:
-        return {}, _num(node)
+def _names_in(node: ast.expr) -> frozenset:
+    return frozenset(n.id for n in ast.walk(node) if isinstance(n, ast.Name))
+
+
+def _affine(node: ast.expr) -> Optional[Tuple[Dict[str, float], float, str, frozenset]]:
+    """(coefficients, constant, divisor, the divisor's names) when node is an
+    affine function of names with literal coefficients, or such a function
+    divided by one expression (ng_div(N, D): the divisor is D, as its AST
+    dump; "" for none), else None."""
+    c = _num(node)
+    if c is not None:
+        return {}, c, "", frozenset()
     if isinstance(node, ast.Name):
-        return {node.id: 1.0}, 0.0
+        return {node.id: 1.0}, 0.0, "", frozenset()
     if isinstance(node, ast.UnaryOp) and isinstance(node.op, (ast.USub, ast.UAdd)):
         f = _affine(node.operand)
         if f is None or isinstance(node.op, ast.UAdd):
             return f
-        return {k: -c for k, c in f[0].items()}, -f[1]
+        return {k: -c for k, c in f[0].items()}, -f[1], f[2], f[3]
+    if (isinstance(node, ast.Call) and isinstance(node.func, ast.Name) and node.func.id == "ng_div"
+            and len(node.args) == 2):
+        f = _affine(node.args[0])
+        if f is None or f[2]:
+            return None
+        return f[0], f[1], ast.dump(node.args[1]), _names_in(node.args[1])
     if isinstance(node, ast.BinOp):
         if isinstance(node.op, (ast.Add, ast.Sub)):
             a, b = _affine(node.left), _affine(node.right)
             if a is None or b is None:
                 return None
+            if a[2] != b[2]:
+                # a divided and an undivided part only combine if one is zero
+                if not a[0] and a[1] == 0.0:
+                    a = ({}, 0.0, b[2], b[3])
+                elif not b[0] and b[1] == 0.0:
+                    b = ({}, 0.0, a[2], a[3])
+                else:
+                    return None
             sg = 1.0 if isinstance(node.op, ast.Add) else -1.0
             co = dict(a[0])
             for k, c in b[0].items():
                 co[k] = co.get(k, 0.0) + sg * c
-            return co, a[1] + sg * b[1]
+            return co, a[1] + sg * b[1], a[2], a[3]
         if isinstance(node.op, ast.Mult):
             for x, y in ((node.left, node.right), (node.right, node.left)):
                 if _num(y) is not None:
                     f = _affine(x)
                     if f is None:
                         return None
-                    return {k: c * _num(y) for k, c in f[0].items()}, f[1] * _num(y)
+                    return {k: c * _num(y) for k, c in f[0].items()}, f[1] * _num(y), f[2], f[3]
     return None
+
+
+def _exp_key(arg: ast.expr):
+    """(key, key of the negated argument, names) of an exp argument that
+    _affine reads and that depends on a name, else None."""
+    f = _affine(arg)
+    if f is None or not f[0]:
+        return None
+    key = (tuple(sorted(f[0].items())), f[1], f[2])
+    neg = (tuple(sorted((k, -c) for k, c in f[0].items())), -f[1], f[2])
+    return key, neg, set(f[0]) | set(f[3])
 
 
 class _ExpShare:
     """exp(x) and exp(-x) of one statement list share one ng_expc: the second
     is the refined reciprocal of the first.  Arguments are matched as affine
-    functions of names with literal coefficients; an entry ends when one of
-    its names is assigned.  ``pre`` collects the declarations of the shared
-    values to emit before the statement being converted."""
+    functions of names with literal coefficients, possibly over a common
+    divisor (_affine); an entry ends when one of its names is assigned.
+    ``pre`` collects the declarations of the shared values to emit before the
+    statement being converted."""
 
     def __init__(self, parent: Optional["_ExpShare"] = None):
         self.table = dict(parent.table) if parent else {}
         self.count = parent.count if parent else [0]
         self.pre: List[str] = []
 
+    def has(self, arg: ast.expr) -> bool:
+        k = _exp_key(arg)
+        return k is not None and (k[0] in self.table or k[1] in self.table)
+
     def exp(self, arg: ast.expr) -> ast.expr:
-        f = _affine(arg)
-        if f is None or not f[0]:
+        k = _exp_key(arg)
+        if k is None:
             return ast.Call(ast.Name("ng_expc"), [arg], [])
-        key = (tuple(sorted(f[0].items())), f[1])
-        neg = (tuple(sorted((k, -c) for k, c in f[0].items())), -f[1])
+        key, neg, names = k
         if key in self.table:
             return ast.Name(self.table[key][0])
         if neg in self.table:
@@ -488,7 +525,7 @@ class _ExpShare:
         name = f"exp_{self.count[0]}"
         self.count[0] += 1
         self.pre.append(f"const double {name} = ng_expc({cxx(arg)});")
-        self.table[key] = (name, set(f[0]))
+        self.table[key] = (name, names)
         return ast.Name(name)
 
     def kill(self, target: str) -> None:
@@ -580,16 +617,133 @@ def fast_cnexp(x: str, a: ast.expr, b: ast.expr, consts: Dict[str, float]) -> st
     return f"{x} = {x} + (1.0 - ng_expc(dt * ({cxx(B)}))) * ({cxx(q)} - {x});"
 
 
+# ------------------------------------------------------------------ inlining
+class _Subst(ast.NodeTransformer):
+    def __init__(self, mapping: Dict[str, ast.expr]):
+        self.mapping = mapping
+
+    def visit_Name(self, node):
+        return copy.deepcopy(self.mapping[node.id]) if node.id in self.mapping else node
+
+
+def _subst(text: str, mapping: Dict[str, ast.expr], cond: bool = False) -> str:
+    return ast.unparse(_Subst(mapping).visit(_cond(text) if cond else _py(text)))
+
+
+def _rename(stmts: List[Stmt], mapping: Dict[str, ast.expr]) -> List[Stmt]:
+    out = []
+    for s in stmts:
+        tgt = s.target
+        if tgt in mapping:
+            if not isinstance(mapping[tgt], ast.Name):
+                raise NmodlError(f"cannot inline an assignment to {tgt}")
+            tgt = mapping[tgt].id
+        if s.kind == "local":
+            out.append(Stmt("local", tgt))
+        elif s.kind == "assign":
+            out.append(Stmt("assign", tgt, _subst(s.expr, mapping)))
+        elif s.kind == "call":
+            args = [a.strip() for a in s.expr.split(",") if a.strip()]
+            out.append(Stmt("call", s.target, ", ".join(_subst(x, mapping) for x in args)))
+        elif s.kind == "if":
+            out.append(Stmt("if", branches=[(None if c is None else _subst(c, mapping, True), _rename(body, mapping))
+                                            for c, body in s.branches]))
+        else:
+            raise NmodlError(f"cannot inline a {s.kind} statement")
+    return out
+
+
+def _inlinable(fn: Block) -> bool:
+    body = statements(fn.body)
+    assigned = set(_assigned_names(body))
+
+    def plain(st):
+        return all(x.kind in ("assign", "local") or (x.kind == "if" and all(plain(b) for _, b in x.branches))
+                   for x in st)
+    return plain(body) and not (assigned & set(fn.args))
+
+
+def _inline_functions(stmts: List[Stmt], mech: Mechanism, counter: List[int]) -> List[Stmt]:
+    """The statement list with each call of one of the mechanism's FUNCTIONs
+    in an assignment replaced by the function's statements (its arguments
+    substituted as expressions, its LOCALs and its value renamed apart), so
+    that exps inside FUNCTIONs take part in the sharing.  Functions that
+    assign their arguments or call procedures stay calls."""
+    funcs = {b.name: b for b in mech.blocks.values() if b.kind == "FUNCTION" and _inlinable(b)}
+    if not funcs:
+        return stmts
+
+    def inline_expr(node: ast.expr, pre: List[Stmt]) -> ast.expr:
+        class T(ast.NodeTransformer):
+            def visit_Call(self, c):
+                self.generic_visit(c)
+                if not (isinstance(c.func, ast.Name) and c.func.id in funcs):
+                    return c
+                fn = funcs[c.func.id]
+                if len(c.args) != len(fn.args):
+                    raise NmodlError(f"{fn.name} takes {len(fn.args)} arguments")
+                k = counter[0]
+                counter[0] += 1
+                body = statements(fn.body)
+                mapping = dict(zip(fn.args, c.args))
+                val = f"{fn.name}_v{k}"
+                mapping[fn.name] = ast.Name(val)
+                for st in body:
+                    if st.kind == "local":
+                        mapping[st.target] = ast.Name(f"{st.target}_l{k}")
+                pre.append(Stmt("local", val))
+                pre.extend(_inline_functions(_rename(body, mapping), mech, counter))
+                return ast.Name(val)
+        return T().visit(node)
+
+    out = []
+    for s in stmts:
+        if s.kind == "assign":
+            pre: List[Stmt] = []
+            node = inline_expr(_py(s.expr), pre)
+            out += pre
+            out.append(Stmt("assign", s.target, ast.unparse(node)) if pre else s)
+        elif s.kind == "if":
+            out.append(Stmt("if", branches=[(c, _inline_functions(body, mech, counter)) for c, body in s.branches]))
+        else:
+            out.append(s)
+    return out
+
+
+def _exp_args(stmts: List[Stmt]) -> List[ast.expr]:
+    """The arguments of the exp calls in a statement list, at any depth."""
+    out = []
+    for s in stmts:
+        nodes = []
+        if s.kind == "assign":
+            nodes.append(_py(s.expr))
+        elif s.kind == "if":
+            for c, body in s.branches:
+                if c is not None:
+                    nodes.append(_cond(c))
+                out += _exp_args(body)
+        for n in nodes:
+            out += [c.args[0] for c in ast.walk(n) if isinstance(c, ast.Call) and isinstance(c.func, ast.Name)
+                    and c.func.id == "exp" and len(c.args) == 1]
+    return out
+
+
 # ------------------------------------------------------------------ emission
 def _emit(stmts: List[Stmt], mech: Mechanism, ind: str, locals_: List[str], ret: Optional[str] = None,
-          fast: Optional[Dict[str, float]] = None, share: Optional[_ExpShare] = None) -> List[str]:
+          fast: Optional[Dict[str, float]] = None, share: Optional[_ExpShare] = None,
+          counter: Optional[List[int]] = None) -> List[str]:
     """C++ of a statement list; ``fast``: None for the reference's operation
     order, else the known constants of the fast forms (LOCALs assigned a
-    constant at the top level of the list join them); ``share``: the shared
-    exps of the enclosing list."""
+    constant at the top level of the list join them; FUNCTION calls inlined;
+    an exp inside a branch whose argument the branches do not change, and
+    that shares with an exp after the branch, is taken before it);
+    ``share``: the shared exps of the enclosing list."""
     out = []
     env = None if fast is None else dict(fast)
     sh = None if fast is None else _ExpShare(share)
+    if fast is not None and counter is None:
+        counter = [0]
+        stmts = _inline_functions(stmts, mech, counter)
 
     def ex(e: ast.expr) -> str:
         if env is None:
@@ -599,7 +753,7 @@ def _emit(stmts: List[Stmt], mech: Mechanism, ind: str, locals_: List[str], ret:
         out.extend(ind + line for line in sh.pre)
         return code
 
-    for s in stmts:
+    for i, s in enumerate(stmts):
         if s.kind == "local":
             if s.target not in locals_:
                 locals_.append(s.target)
@@ -633,6 +787,20 @@ def _emit(stmts: List[Stmt], mech: Mechanism, ind: str, locals_: List[str], ret:
                 A, B = cxx(a), cxx(b)
                 out.append(f"{ind}{x} = {x} + (1.0 - ng_exp(dt * ({B}))) * (-({A}) / ({B}) - {x});")
         elif s.kind == "if":
+            if env is not None:
+                changed = set(_assigned_names([s]))
+                later = set()
+                for x in _exp_args(stmts[i + 1:]):
+                    k = _exp_key(fast_expr(x, env))
+                    if k is not None:
+                        later |= {k[0], k[1]}
+                sh.pre = []
+                for x in _exp_args([s]):
+                    fx = fast_expr(x, env)
+                    k = _exp_key(fx)
+                    if k is not None and not (k[2] & changed) and k[0] in later and not sh.has(fx):
+                        sh.exp(fx)
+                out.extend(ind + line for line in sh.pre)
             for k, (cond, body) in enumerate(s.branches):
                 head = "if" if k == 0 else ("} else if" if cond is not None else "} else")
                 if cond is not None and env is not None and k > 0:
@@ -642,7 +810,7 @@ def _emit(stmts: List[Stmt], mech: Mechanism, ind: str, locals_: List[str], ret:
                     c = None if cond is None else ex(_cond(cond))
                 out.append(f"{ind}{head}" + (f" ({c}) {{" if cond is not None else " {"))
                 sub = None if env is None else {n: v for n, v in env.items()}
-                out += _emit(body, mech, ind + "    ", locals_, ret, sub, sh)
+                out += _emit(body, mech, ind + "    ", locals_, ret, sub, sh, counter)
             out.append(f"{ind}}}")
             if env is not None:
                 # a name assigned in a branch is no longer a known constant

@@ -244,11 +244,11 @@ def test_fast_forms_share_and_fold_correctly(tmp_path):
     emission's to rounding on both sides of the branch."""
     code, _ = nmodl.generate({"sharetest.mod": SHARETEST}, fast={"celsius": 34.0})
     ini = code[code.index("void initial()"):code.index("void states()")]
-    # c: a shared exp; d: a new one after v = v + 1; e (v > 0): its reciprocal;
-    # after the branch (v assigned in it) a third; a and b divide by the
-    # PARAMETER k, so they are not affine in v and stay separate
-    assert ini.count("const double exp_") == 3 and "e = ng_rcp(exp_1);" in ini
-    assert "ng_expc(ng_div(v, k))" in ini and "ng_expc(ng_div((-v), k))" in ini
+    # a: a shared exp over the divisor k (a PARAMETER), b: its reciprocal;
+    # c: a second; d: a new one after v = v + 1; e (v > 0): its reciprocal;
+    # after the branch (v assigned in it) a fourth
+    assert ini.count("const double exp_") == 4
+    assert "b = ng_rcp(exp_0);" in ini and "e = ng_rcp(exp_2);" in ini
     assert "ng_expc((q * 0.6931471805599453))" in ini        # w^q with the LOCAL w = 2 folded
     _register_defaults({"sharetest.mod": SHARETEST})
     out = {}
