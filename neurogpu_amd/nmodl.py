@@ -11,8 +11,10 @@ HH-style membrane mechanisms use, and emits host- and device-callable C++:
   (defaults), CONSTANT, STATE, ASSIGNED, BREAKPOINT (SOLVE states METHOD
   cnexp, assignments), DERIVATIVE states, INITIAL, PROCEDURE and FUNCTION;
 * statements: assignments, ``x' = expr``, LOCAL, if / else if / else,
-  procedure calls; UNITSOFF / UNITSON and TABLE lines are ignored;
-  VERBATIM, KINETIC, NET_RECEIVE and other solve methods are refused.
+  procedure calls; UNITSOFF / UNITSON lines and TABLE statements (with their
+  DEPEND and FROM ... TO ... WITH lines) are ignored, units after numbers in
+  expressions (``10 (degC)``) dropped; VERBATIM, KINETIC, NET_RECEIVE and
+  other solve methods are refused.
 
 Each mechanism becomes a struct whose fields are its variables (v, dt,
 PARAMETER defaults, STATE, ASSIGNED, ion variables) and whose member
@@ -26,7 +28,14 @@ host and the device).
     python -m neurogpu_amd.nmodl -o gen_mechs.h a.mod b.mod ...
 
 also writes a C-ABI evaluation entry per mechanism (``--harness``) that the
-tests compile on the host.
+tests compile on the host.  ``--set NAME`` adds SetGen_NAME, a mechanism set
+the simulation kernel instantiates (emit_set: mask dispatch, state slots,
+the parameter-matrix layout, calcium bookkeeping), with the model's
+temperature (``--celsius``), constant ion values (``--ion``), constant
+parameters (``--constant``, ``--global name_suffix=value``) and specialised
+masks (``--masks``).  ``--fast`` emits the fast forms (fast_expr,
+fast_cnexp, _inline_functions, _ExpShare) instead of the reference's
+operation order; both agree to rounding.
 """
 from __future__ import annotations
 
