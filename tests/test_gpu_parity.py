@@ -179,11 +179,14 @@ def test_open_reads_sizes_from_the_files():
         assert sim.fetch().size == 24 * 3168
 
 
-def test_reference_ttpc_fixture_through_file_boundary(tmp_path):
+@pytest.mark.parametrize("mechset", [-1, MECH_TTPC_GEN])
+def test_reference_ttpc_fixture_through_file_boundary(tmp_path, mechset):
     """The reference's own data directory, parsed by ngpu_open (C++), run, and
-    written as VHotP.dat; compared with the oracle run on the Python parse."""
-    _say("case ttpc512 fixture via ngpu_open")
-    sim = Simulator.open(TTPC_DIR)
+    written as VHotP.dat; compared with the oracle run on the Python parse.
+    mechset -1 infers the hand-written TTPC set from the tree file; the
+    generated set reads the same files."""
+    _say(f"case ttpc512 fixture via ngpu_open, mechset {mechset}")
+    sim = Simulator.open(TTPC_DIR, mechset=mechset)
     out = tmp_path / "VHotP0.dat"
     _, st = sim.run(fetch=False)
     assert st["solve"] == SOLVE_REFERENCE            # zero-area nodes: the reference's order
