@@ -173,6 +173,31 @@ int ngpu_features_noalloc(ngpu_ctx *ctx, const double *t_on, const double *t_off
 /* Copy the traces of the last run to vhot (same layout as ngpu_run). */
 int ngpu_fetch_vhot(ngpu_ctx *ctx, double *vhot);
 
+/* ---- Tree maps (the reference builds them in Python at model export,
+ * NeuroGPU_Base/scripts/pyscripts/create_auxilliary_data_3.py:22-315 and
+ * make_tree_from_parent_vec.py:16-59, and writes BasicConstSegP.csv at
+ * :229-272).  parent[s] is the parent segment of segment s (parent[0] = -1,
+ * parent[s] < s), at most two children a segment and one on the root
+ * (ngpu_tree_disperse first otherwise); e, f in Hines-row order (row
+ * N-1-s for segment s); bool_model [nmodels][N].  *out gets every map of
+ * BasicConstSegP.csv, owned by the library until ngpu_tree_free; its
+ * mechset is 0 (set it before ngpu_set_tree).  NGPU_E_ARG with a message
+ * (ngpu_last_error(NULL)) for a parent vector the schedule cannot take. */
+int ngpu_tree_build(int N, const int *parent, const double *e, const double *f, const double *cm,
+                    const int *seg_to_comp, int ncomps, int nmodels, const int *bool_model, ngpu_tree **out);
+void ngpu_tree_free(ngpu_tree *tree);
+/* Write a tree as BasicConstSegP.csv, "%.9f," values, one record a line
+ * (create_auxilliary_data_3.py:229-272). */
+int ngpu_tree_write(const ngpu_tree *tree, const char *path);
+/* Wide branch points dispersed into extension segments, as the reference's
+ * model export does (hinesDisperseBranching, NeuroGPU_Base/runModel.hoc:158-215):
+ * a segment with k > 2 children (the root: k > 1) gets k - 1 extension
+ * segments carrying its children two at a time; the tree is renumbered depth
+ * first.  new_parent and src hold up to cap entries (2N always suffices);
+ * src[i] = the original segment of new segment i, or -(h + 1) for an
+ * extension of segment h; *new_n = the new size. */
+int ngpu_tree_disperse(int N, const int *parent, int cap, int *new_parent, int *src, int *new_n);
+
 /* Parse a reference data directory (BasicConstSegP.csv, Sim.csv,
  * AllParams.csv, Stim_meta.csv, Stim_raw<suffix>.csv, times<suffix>.csv) and
  * load it into a new context (replaces RunByModelP's Init,

@@ -28,7 +28,8 @@ EXPORTS = ("ngpu_create", "ngpu_set_tree", "ngpu_set_sim", "ngpu_set_params", "n
            "ngpu_batch_create", "ngpu_batch_add", "ngpu_batch_set_launch", "ngpu_batch_run",
            "ngpu_batch_member_info", "ngpu_batch_last_error", "ngpu_batch_close", "ngpu_run_steps",
            "ngpu_get_sizes", "ngpu_features_noalloc", "ngpu_set_solve",
-           "ngpu_set_param_template", "ngpu_set_genomes")
+           "ngpu_set_param_template", "ngpu_set_genomes", "ngpu_tree_build", "ngpu_tree_free",
+           "ngpu_tree_write", "ngpu_tree_disperse")
 NFEAT = 5
 FEATURES = ("spike_count", "first_latency", "ap_amplitude", "ap_width", "mean_isi")
 
@@ -112,6 +113,12 @@ def lib():
         L.ngpu_set_launch.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.ngpu_set_solve.argtypes = [vp, C.c_int]
         L.ngpu_set_param_template.argtypes = [vp, _DP, C.c_int, C.c_int, C.c_int]
+        L.ngpu_tree_build.argtypes = [C.c_int, _IP, _DP, _DP, _DP, _IP, C.c_int, C.c_int, _IP,
+                                      C.POINTER(C.POINTER(ngpu_tree))]
+        L.ngpu_tree_free.argtypes = [C.POINTER(ngpu_tree)]
+        L.ngpu_tree_free.restype = None
+        L.ngpu_tree_write.argtypes = [C.POINTER(ngpu_tree), C.c_char_p]
+        L.ngpu_tree_disperse.argtypes = [C.c_int, _IP, C.c_int, _IP, _IP, C.POINTER(C.c_int)]
         L.ngpu_set_genomes.argtypes = [vp, _DP, C.c_int]
         L.ngpu_run.argtypes = [vp, _DP, C.POINTER(ngpu_stats)]
         L.ngpu_run_steps.argtypes = [vp, C.c_int, C.c_int, C.POINTER(ngpu_stats)]
@@ -145,7 +152,8 @@ def lib():
         for name in ("ngpu_create", "ngpu_set_tree", "ngpu_set_sim", "ngpu_set_params", "ngpu_set_stimuli",
                      "ngpu_set_launch", "ngpu_run", "ngpu_fetch_vhot", "ngpu_open", "ngpu_write_vhot",
                      "ngpu_mechset_nparams", "ngpu_run_steps", "ngpu_get_sizes", "ngpu_set_solve",
-                     "ngpu_set_param_template", "ngpu_set_genomes"):
+                     "ngpu_set_param_template", "ngpu_set_genomes", "ngpu_tree_build", "ngpu_tree_free",
+           "ngpu_tree_write", "ngpu_tree_disperse"):
             getattr(L, name).restype = C.c_int
         _lib = L
     return _lib
@@ -171,6 +179,78 @@ def _tree_struct(t: TreeData, mechset: int):
                   ia(t.comp_by_level32), ia(t.comp_by_flevel32), t.lrel_starts.size, t.flrel_starts.size,
                   ia(t.lrel_starts), ia(t.lrel_ends), ia(t.flrel_starts), ia(t.flrel_ends))
     return s, keep
+
+
+def _i32(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x).ravel(), dtype=np.int32)
+
+
+def _f64(x) -> np.ndarray:
+    return np.ascontiguousarray(np.asarray(x).ravel(), dtype=np.float64)
+
+
+def tree_build(par, e, f, cm, seg_to_comp, bool_model, ncomps: int) -> TreeData:
+    """Every map of BasicConstSegP.csv for a morphology, computed by the
+    library's tree-map scheduler (ngpu_tree_build)."""
+    L = lib()
+    par, e, f, cm, s2c = _i32(par), _f64(e), _f64(f), _f64(cm), _i32(seg_to_comp)
+    N = par.size
+    bm = np.ascontiguousarray(np.asarray(bool_model).reshape(-1, N), dtype=np.int32)
+    out = C.POINTER(ngpu_tree)()
+    rc = L.ngpu_tree_build(N, par.ctypes.data_as(_IP), e.ctypes.data_as(_DP), f.ctypes.data_as(_DP),
+                           cm.ctypes.data_as(_DP), s2c.ctypes.data_as(_IP), int(ncomps), bm.shape[0],
+                           bm.ctypes.data_as(_IP), C.byref(out))
+    if rc != 0:
+        raise NgpuError(rc, (L.ngpu_last_error(None) or b"").decode())
+    try:
+        t = out.contents
+
+        def ints(ptr, n):
+            return np.ctypeslib.as_array(ptr, shape=(n,)).astype(np.int64) if n else np.zeros(0, np.int64)
+
+        nl, nfl = t.n_lrel, t.n_flrel
+        td = TreeData(
+            N=t.N, ncomps=t.ncomps, e=e.copy(), f=f.copy(), ks=ints(t.ks, N), seg_to_comp=ints(t.seg_to_comp, N),
+            cm=cm.copy(), nmodels=t.nmodels, bool_model=ints(t.bool_model, t.nmodels * N).reshape(t.nmodels, N),
+            son_no=ints(t.son_no, N), depth=t.depth, logn_depth=t.logn_depth,
+            rel_starts=ints(t.rel_starts, t.n_fathers), rel_ends=ints(t.rel_ends, t.n_fathers),
+            rel_vec=ints(t.rel_vec, t.n_call), seg_start=ints(t.seg_start, t.n_call + 1),
+            seg_end=ints(t.seg_end, t.n_call + 1), fathers=ints(t.fathers, t.n_fathers),
+            fidxs=ints(t.fidxs, t.logn_depth * N).reshape(t.logn_depth, N),
+            comp_by_level32=ints(t.comp_by_level32, t.comp_depth * 32).reshape(t.comp_depth, 32),
+            comp_by_flevel32=ints(t.comp_by_flevel32, t.comp_fdepth * 32).reshape(t.comp_fdepth, 32),
+            lrel_starts=ints(t.lrel_starts, nl), lrel_ends=ints(t.lrel_ends, nl),
+            flrel_starts=ints(t.flrel_starts, nfl), flrel_ends=ints(t.flrel_ends, nfl), ksb_m1=np.zeros(0, np.int64))
+    finally:
+        L.ngpu_tree_free(out)
+    # the file's last record (Ks - 1 as uint16, then the deepest level's first run end - 1)
+    last_run = int(td.comp_by_level32[td.lrel_starts[td.depth], 0]) - 1
+    ksb = np.concatenate([td.ks, [td.seg_end[last_run]]]) - 1
+    td.ksb_m1 = np.where(ksb < 0, 65535, ksb)
+    return td
+
+
+def tree_write(t: TreeData, path: str) -> None:
+    """Write BasicConstSegP.csv with the library's writer (ngpu_tree_write)."""
+    L = lib()
+    s, keep = _tree_struct(t, 0)
+    rc = L.ngpu_tree_write(C.byref(s), path.encode())
+    del keep
+    if rc != 0:
+        raise NgpuError(rc, (L.ngpu_last_error(None) or b"").decode())
+
+
+def tree_disperse(par):
+    """(new_par, src) of ngpu_tree_disperse."""
+    L = lib()
+    par = _i32(par)
+    cap = 2 * max(par.size, 1)
+    newp, src, n = np.zeros(cap, np.int32), np.zeros(cap, np.int32), C.c_int(0)
+    rc = L.ngpu_tree_disperse(par.size, par.ctypes.data_as(_IP), cap, newp.ctypes.data_as(_IP),
+                              src.ctypes.data_as(_IP), C.byref(n))
+    if rc != 0:
+        raise NgpuError(rc, (L.ngpu_last_error(None) or b"").decode())
+    return newp[:n.value].astype(np.int64), src[:n.value].astype(np.int64)
 
 
 class Simulator:

@@ -19,6 +19,8 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <new>
 #include <sstream>
 #include <string>
@@ -29,6 +31,7 @@
 #include "gen_ttpc.h"
 #include "gen_mainen.h"
 #include "schedule.h"
+#include "treemap.h"
 
 /* Per-set launchers, one translation unit each (sim_<set>.hip). */
 hipError_t ngpu_launch_passive(bool multi, dim3 grid, dim3 block, size_t lds, hipStream_t s, const SimArgs &a);
@@ -1489,6 +1492,149 @@ static int open_into(ngpu_ctx *c, const std::string &d, const std::string &sfx, 
 /* The files are parsed and checked into a host-only context first (no device
  * is touched, so a malformed directory is reported the same way on any
  * machine); the device part is set up afterwards. */
+/* ---- tree maps */
+namespace {
+struct TreeHolder {
+    ngpu_tree t{};
+    NgTreeMaps m;
+    std::vector<double> e, f, cm;
+    std::vector<int> seg_to_comp, bool_model;
+};
+std::mutex g_trees_mu;
+std::map<const ngpu_tree *, std::unique_ptr<TreeHolder>> g_trees;
+
+int tree_fail(int code, const std::string &msg)
+{
+    g_last_error = msg;
+    return code;
+}
+}
+
+extern "C" int ngpu_tree_build(int N, const int *parent, const double *e, const double *f, const double *cm,
+                               const int *seg_to_comp, int ncomps, int nmodels, const int *bool_model,
+                               ngpu_tree **out)
+{
+    if (!out) return tree_fail(NGPU_E_ARG, "ngpu_tree_build: out is NULL");
+    *out = nullptr;
+    if (N < 1 || ncomps < 1 || nmodels < 1 || !parent || !e || !f || !cm || !seg_to_comp || !bool_model)
+        return tree_fail(NGPU_E_ARG, "ngpu_tree_build: N, ncomps and nmodels must be positive and every array given");
+    try {
+        auto h = std::make_unique<TreeHolder>();
+        h->m = ng_tree_maps(std::vector<int>(parent, parent + N));
+        h->e.assign(e, e + N); h->f.assign(f, f + N); h->cm.assign(cm, cm + N);
+        h->seg_to_comp.assign(seg_to_comp, seg_to_comp + N);
+        h->bool_model.assign(bool_model, bool_model + (size_t)nmodels * N);
+        for (int s = 0; s < N; s++)
+            if (seg_to_comp[s] < 0 || seg_to_comp[s] >= ncomps)
+                return tree_fail(NGPU_E_ARG, "ngpu_tree_build: seg_to_comp[" + std::to_string(s) + "] out of range");
+        const NgTreeMaps &m = h->m;
+        ngpu_tree &t = h->t;
+        t.N = N; t.ncomps = ncomps; t.mechset = 0; t.nmodels = nmodels;
+        t.e = h->e.data(); t.f = h->f.data(); t.cm = h->cm.data();
+        t.ks = m.ks.data(); t.seg_to_comp = h->seg_to_comp.data(); t.bool_model = h->bool_model.data();
+        t.son_no = m.son_no.data();
+        t.depth = m.depth; t.logn_depth = m.logn_depth;
+        t.n_fathers = (int)m.fathers.size(); t.n_call = (int)m.rel_vec.size();
+        t.rel_starts = m.rel_starts.data(); t.rel_ends = m.rel_ends.data(); t.rel_vec = m.rel_vec.data();
+        t.seg_start = m.seg_start.data(); t.seg_end = m.seg_end.data(); t.fathers = m.fathers.data();
+        t.fidxs = m.fidxs.data();
+        t.comp_depth = m.comp_depth; t.comp_fdepth = m.comp_fdepth;
+        t.comp_by_level32 = m.comp_by_level32.data(); t.comp_by_flevel32 = m.comp_by_flevel32.data();
+        t.n_lrel = (int)m.lrel_starts.size(); t.n_flrel = (int)m.flrel_starts.size();
+        t.lrel_starts = m.lrel_starts.data(); t.lrel_ends = m.lrel_ends.data();
+        t.flrel_starts = m.flrel_starts.data(); t.flrel_ends = m.flrel_ends.data();
+        std::lock_guard<std::mutex> lk(g_trees_mu);
+        *out = &h->t;
+        g_trees[&h->t] = std::move(h);
+        return NGPU_OK;
+    } catch (const std::invalid_argument &ex) {
+        return tree_fail(NGPU_E_ARG, std::string("ngpu_tree_build: ") + ex.what());
+    } catch (const std::bad_alloc &) {
+        return tree_fail(NGPU_E_NOMEM, "ngpu_tree_build: out of host memory");
+    }
+}
+
+extern "C" void ngpu_tree_free(ngpu_tree *tree)
+{
+    std::lock_guard<std::mutex> lk(g_trees_mu);
+    g_trees.erase(tree);
+}
+
+extern "C" int ngpu_tree_write(const ngpu_tree *t, const char *path)
+{
+    if (!t || !path) return tree_fail(NGPU_E_ARG, "ngpu_tree_write: tree and path must be given");
+    if (t->N < 1 || t->depth < 0 || t->n_lrel != t->depth + 1 || t->comp_depth < 1)
+        return tree_fail(NGPU_E_ARG, "ngpu_tree_write: inconsistent tree sizes");
+    /* the file's last record: Ks - 1 (uint16: -1 as 65535) and the end of
+     * the first run of the deepest level, less one */
+    const int last_run = t->comp_by_level32[(size_t)t->lrel_starts[t->depth] * 32] - 1;
+    if (last_run < 0 || last_run > t->n_call) return tree_fail(NGPU_E_ARG, "ngpu_tree_write: bad level table");
+    std::vector<int> ksb(t->N + 1);
+    for (int i = 0; i <= t->N; i++) {
+        const int v = (i < t->N ? t->ks[i] : t->seg_end[last_run]) - 1;
+        ksb[i] = v < 0 ? 65535 : v;
+    }
+    std::FILE *fp = std::fopen(path, "w");
+    if (!fp) return tree_fail(NGPU_E_FILE, std::string("ngpu_tree_write: cannot open ") + path);
+    const size_t N = t->N;
+    ng_write_scalar(fp, t->N);
+    ng_write_scalar(fp, t->ncomps);
+    ng_write_record(fp, t->e, N);
+    ng_write_record(fp, t->f, N);
+    ng_write_record(fp, t->ks, N);
+    ng_write_record(fp, t->seg_to_comp, N);
+    ng_write_record(fp, t->cm, N);
+    ng_write_scalar(fp, t->nmodels);
+    ng_write_record(fp, t->bool_model, (size_t)t->nmodels * N);
+    ng_write_record(fp, t->son_no, N);
+    ng_write_scalar(fp, t->depth);
+    ng_write_scalar(fp, t->logn_depth);
+    ng_write_scalar(fp, t->n_fathers);
+    ng_write_scalar(fp, t->n_call);
+    ng_write_record(fp, t->rel_starts, t->n_fathers);
+    ng_write_record(fp, t->rel_ends, t->n_fathers);
+    ng_write_record(fp, t->rel_vec, t->n_call);
+    ng_write_record(fp, t->seg_start, t->n_call + 1);
+    ng_write_record(fp, t->seg_end, t->n_call + 1);
+    ng_write_record(fp, t->fathers, t->n_fathers);
+    ng_write_record(fp, t->fidxs, (size_t)t->logn_depth * N);
+    ng_write_scalar(fp, t->comp_depth);
+    ng_write_scalar(fp, t->comp_fdepth);
+    ng_write_record(fp, t->comp_by_level32, (size_t)t->comp_depth * 32);
+    ng_write_record(fp, t->comp_by_flevel32, (size_t)t->comp_fdepth * 32);
+    ng_write_scalar(fp, t->n_lrel);
+    ng_write_record(fp, t->lrel_starts, t->n_lrel);
+    ng_write_record(fp, t->lrel_ends, t->n_lrel);
+    ng_write_scalar(fp, t->n_flrel);
+    ng_write_record(fp, t->flrel_starts, t->n_flrel);
+    ng_write_record(fp, t->flrel_ends, t->n_flrel);
+    ng_write_record(fp, ksb);
+    const bool bad = std::ferror(fp) != 0;
+    if (std::fclose(fp) != 0 || bad) return tree_fail(NGPU_E_FILE, std::string("ngpu_tree_write: error writing ") + path);
+    return NGPU_OK;
+}
+
+extern "C" int ngpu_tree_disperse(int N, const int *parent, int cap, int *new_parent, int *src, int *new_n)
+{
+    if (N < 1 || !parent || !new_parent || !src || !new_n || cap < 0)
+        return tree_fail(NGPU_E_ARG, "ngpu_tree_disperse: N must be positive and every array given");
+    try {
+        std::vector<int> np, sv;
+        ng_disperse(std::vector<int>(parent, parent + N), np, sv);
+        *new_n = (int)np.size();
+        if ((int)np.size() > cap)
+            return tree_fail(NGPU_E_ARG, "ngpu_tree_disperse: the dispersed tree has " + std::to_string(np.size()) +
+                                             " segments, more than cap = " + std::to_string(cap));
+        std::copy(np.begin(), np.end(), new_parent);
+        std::copy(sv.begin(), sv.end(), src);
+        return NGPU_OK;
+    } catch (const std::invalid_argument &ex) {
+        return tree_fail(NGPU_E_ARG, std::string("ngpu_tree_disperse: ") + ex.what());
+    } catch (const std::bad_alloc &) {
+        return tree_fail(NGPU_E_NOMEM, "ngpu_tree_disperse: out of host memory");
+    }
+}
+
 extern "C" int ngpu_open(const char *dir, const char *suffix, int mechset, int device, ngpu_ctx **out)
 {
     if (!dir || !out) return fail(nullptr, NGPU_E_ARG, "ngpu_open: NULL argument");

@@ -13,7 +13,7 @@ from typing import Optional
 
 import numpy as np
 
-from . import formats
+from . import formats, tree
 from ._lib import Simulator
 
 
@@ -62,7 +62,7 @@ def write_data_dir(workload, data_dir: str, suffix: str = "", rec_sites: Optiona
     """All input files of the reference's Data/ directory for a workload."""
     os.makedirs(data_dir, exist_ok=True)
     m = workload.model
-    formats.write_tree(os.path.join(data_dir, "BasicConstSegP.csv"), m.tree)
+    tree.write_tree(os.path.join(data_dir, "BasicConstSegP.csv"), m.tree)
     formats.write_allparams(os.path.join(data_dir, "AllParams.csv"), workload.params)
     rec = m.rec_sites if rec_sites is None else rec_sites
     formats.write_sim(os.path.join(data_dir, "Sim.csv"), formats.SimData(m.v_init, float(workload.dts[0]), rec))

@@ -9,6 +9,7 @@ import pytest
 
 from conftest import TOL_MV, spike_counts
 from neurogpu_amd import formats, models, tree
+from oracle.tree_ref import son_numbers
 from oracle import cpu_ref
 
 
@@ -21,7 +22,7 @@ def test_three_children_are_dispersed():
     #      0 -- 1 -- {2, 3, 4}      node 1 has three children
     par = np.array([-1, 0, 1, 1, 1])
     newpar, src = tree.disperse_branching(par)
-    son = tree.son_numbers(newpar)
+    son = son_numbers(newpar)
     assert son.max() <= 2
     ext = src < 0
     assert ext.sum() == 2                                   # nchild - 1 extension nodes
@@ -40,7 +41,7 @@ def test_three_children_are_dispersed():
 def test_root_with_two_children_is_dispersed():
     par = np.array([-1, 0, 0, 1, 2])
     newpar, src = tree.disperse_branching(par)
-    assert tree.son_numbers(newpar).max() <= 2 and (src < 0).sum() == 1
+    assert son_numbers(newpar).max() <= 2 and (src < 0).sum() == 1
 
 
 def test_trees_without_wide_branch_points_are_unchanged():

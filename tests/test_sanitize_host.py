@@ -1,7 +1,8 @@
 """Host code under AddressSanitizer and UndefinedBehaviorSanitizer, on the CPU:
 libneurogpu's file parsers, tree checks and level-schedule builder (through
 ngpu_open, on the reference fixtures and on malformed copies), the
-mechanism-pass schedule builders of schedule.h, and the FP64 oracle
+mechanism-pass schedule builders of schedule.h, the tree-map scheduler of
+treemap.h, and the FP64 oracle
 (oracle/cpu_ref.c).  Sanitizers instrument host code only; no GPU code is
 built with them."""
 import os
@@ -93,6 +94,16 @@ def test_schedule_builders_under_sanitizers(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300, env=SAN_ENV)
     _clean(r.stdout + r.stderr)
     assert r.returncode == 0, r.stdout + r.stderr[-3000:]
+
+
+def test_tree_map_scheduler_under_sanitizers(tmp_path):
+    """treemap.h (dispersal, maps, records) over 400 random morphologies."""
+    exe = str(tmp_path / "sanitize_treemap")
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
+                    "-I", CSRC, os.path.join(ROOT, "tests", "sanitize_treemap.cpp"), "-o", exe], check=True)
+    r = subprocess.run([exe, str(tmp_path / "maps.csv")], capture_output=True, text=True, timeout=300, env=SAN_ENV)
+    _clean(r.stdout + r.stderr)
+    assert r.returncode == 0 and "0 bad" in r.stdout, r.stdout + r.stderr[-3000:]
 
 
 def test_oracle_under_sanitizers(tmp_path):
