@@ -154,6 +154,8 @@ int ngpu_get_sizes(const ngpu_ctx *ctx, int *nsets, int *nstim, int *nrec, int *
  * latency (ms after t_on[stim]), mean AP amplitude above the pre-stimulus
  * baseline (mV), mean AP width at thr (ms), mean ISI (ms); NaN where undefined.
  * Only spikes whose threshold crossing lies in [t_on[stim], t_off[stim]) count.
+ * A trace with a NaN sample (a diverged run) has every feature NaN, the spike
+ * count included (the evaluator's NaN-trace case, PYS/efel_ext.py:47-49, :90-95).
  * Replaces the per-individual eFEL pass of the GA evaluator (PYS/efel_ext.py:23-97,
  * threshold at :15). */
 #define NGPU_NFEAT 5

@@ -14,6 +14,8 @@
  *   2 AP amplitude     mean over spikes of (peak - v_base), v_base = mean v for t < t_on
  *   3 AP width         mean time above thr of spikes that fall back below it; NaN if none
  *   4 mean ISI         mean interval between consecutive spikes; NaN if < 2 spikes
+ * A trace with a NaN sample (a diverged run) has all five NaN, the count
+ * included (the evaluator scores such traces apart, PYS/efel_ext.py:47-49).
  *
  * Layout: one 64-lane workgroup takes 64 runs.  The traces are read in tiles
  * of 64 runs x 64 steps: for each run of the tile the 64 lanes read 64
@@ -50,7 +52,7 @@ __global__ void __launch_bounds__(64) ngpu_features_kernel(const double *__restr
     int cnt = 0, namp = 0, nw = 0;
     double first = nan, last_up = 0.0, sum_isi = 0.0, sum_amp = 0.0, sum_w = 0.0, t_up = 0.0, peak = 0.0;
     double prev = 0.0, v0 = 0.0, tnow = 0.0, ta = 0.0;
-    bool in_spike = false;
+    bool in_spike = false, diverged = false;
     for (int c0 = 0; c0 < nt; c0 += NG_FT) {
         const int n = min(NG_FT, nt - c0);
         __syncthreads();
@@ -63,6 +65,7 @@ __global__ void __launch_bounds__(64) ngpu_features_kernel(const double *__restr
         for (int k = 0; k < n; k++) {
             const int i = c0 + k;
             const double b = tile[lane][k];
+            diverged |= b != b;
             const double tb = tnow;
             tnow += dts[i];
             if (i == 0) v0 = b;
@@ -107,6 +110,10 @@ __global__ void __launch_bounds__(64) ngpu_features_kernel(const double *__restr
     if (!live) return;
     if (in_spike) { sum_amp += peak - vb; namp++; }
     double *o = out + (size_t)run * NG_NFEAT;
+    if (diverged) {                       /* a NaN sample: every feature undefined */
+        for (int k = 0; k < NG_NFEAT; k++) o[k] = nan;
+        return;
+    }
     o[0] = (double)cnt;
     o[1] = first;
     o[2] = namp > 0 ? sum_amp / namp : nan;

@@ -29,15 +29,27 @@ import numpy as np
 from ._lib import FEATURES, Simulator
 from .selection import ParetoFront, eps_indicator, ibea_fitness, tournament
 
-PENALTY = 250.0          # objective value for a feature that is undefined (efel_ext.py:73-83 uses 10000 / 100000)
+# Objective of an undefined feature difference, and of every objective of a
+# diverged run (a trace with a NaN sample).  The reference's evaluator scores
+# them 10000 and 100000 on raw feature differences (PYS/efel_ext.py:73-83,
+# :90-95); on these scale-normalised objectives the same two levels, ten
+# times apart, are 250 (also the cap of a finite difference) and 2500.
+PENALTY = 250.0
+DIVERGED_PENALTY = 2500.0
 GA_FEATURES = (0, 1, 2, 4)   # spike count, first-spike latency, AP amplitude, mean ISI per stimulus
+
+
+def diverged(f: np.ndarray) -> np.ndarray:
+    """Runs whose trace had a NaN sample (raw features: spike count NaN)."""
+    return np.isnan(np.asarray(f)[..., 0])
 
 
 def fill_features(f: np.ndarray, window: float) -> np.ndarray:
     """Features with the undefined entries replaced by their natural limits
-    (no spike: latency and ISI = the stimulus window, amplitude = width = 0),
-    so that they can be regressed and compared."""
+    (no spike: latency and ISI = the stimulus window, amplitude = width = 0;
+    a diverged run: no spike), so that they can be regressed and compared."""
     out = np.array(f, dtype=np.float64, copy=True)
+    out[..., 0][np.isnan(out[..., 0])] = 0.0
     lat, amp, wid, isi = (out[..., 1], out[..., 2], out[..., 3], out[..., 4])
     lat[np.isnan(lat)] = window
     amp[np.isnan(amp)] = 0.0
@@ -52,11 +64,15 @@ class Target:
     scale: np.ndarray         # [NFEAT] per-feature scale of the objectives
 
 
-def objectives(filled: np.ndarray, target: Target) -> np.ndarray:
-    """|f - f*| / scale per stimulus and GA feature -> [n, nstim * len(GA_FEATURES)]."""
+def objectives(filled: np.ndarray, target: Target, bad: Optional[np.ndarray] = None) -> np.ndarray:
+    """|f - f*| / scale per stimulus and GA feature -> [n, nstim * len(GA_FEATURES)],
+    capped at PENALTY; ``bad`` [n, nstim]: diverged runs, whose objectives
+    are all DIVERGED_PENALTY."""
     d = np.abs(filled[..., GA_FEATURES] - target.features[None, :, GA_FEATURES]) / target.scale[list(GA_FEATURES)]
-    d = np.where(np.isfinite(d), d, PENALTY)
-    return np.minimum(d, PENALTY).reshape(filled.shape[0], -1)
+    d = np.minimum(np.where(np.isfinite(d), d, PENALTY), PENALTY)
+    if bad is not None:
+        d = np.where(np.asarray(bad)[..., None], DIVERGED_PENALTY, d)
+    return d.reshape(filled.shape[0], -1)
 
 
 class Evaluator:
@@ -90,11 +106,12 @@ class Evaluator:
         f = self.sim.features(self.t_on, self.t_off)
         self.sim_seconds += time.perf_counter() - t0
         self.nevals += x.shape[0]
+        self.last_diverged = diverged(f)
         return fill_features(f, self.window)
 
     def evaluate(self, x: np.ndarray):
         f = self.features(x)
-        return objectives(f, self.target), f
+        return objectives(f, self.target, self.last_diverged), f
 
     def close(self):
         self.sim.close()

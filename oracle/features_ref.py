@@ -10,7 +10,8 @@ agree bit for bit.
 A spike is an upward crossing of thr whose linearly interpolated time tc lies
 in the stimulus window t_on <= tc < t_off; crossings outside it are ignored.
 Width and amplitude of a spike that starts in the window are measured in full,
-also when it falls back after t_off."""
+also when it falls back after t_off.  A trace with a NaN sample (a diverged
+run) has every feature NaN, the spike count included."""
 import math
 
 import numpy as np
@@ -21,6 +22,9 @@ NFEAT = 5
 def trace_features(v, times, t_on, t_off=math.inf, thr=-30.0):
     v = [float(x) for x in v]
     nt = len(v)
+    if any(x != x for x in v):
+        # a trace with a NaN sample (a diverged run): every feature undefined
+        return np.full(NFEAT, math.nan)
     vb, nb = 0.0, 0
     for i in range(nt):
         if not times[i] < t_on:

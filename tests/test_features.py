@@ -69,6 +69,62 @@ def test_baseline_is_mean_before_onset():
     assert abs(f[2] - (30.0 - (-60.0))) < 1e-9
 
 
+def test_diverged_trace_has_every_feature_undefined():
+    """A NaN sample anywhere in the trace (a diverged run) makes every
+    feature NaN, the spike count included, whatever else the trace holds."""
+    v, t = _triangles([10.0, 20.0])
+    for i in (0, 150, 399):
+        w = v.copy()
+        w[i] = math.nan
+        assert np.all(np.isnan(features_ref.trace_features(w, t, 5.0)))
+    assert not np.any(np.isnan(features_ref.trace_features(v, t, 5.0)[:2]))
+
+
+def test_diverged_runs_score_apart():
+    """The evaluator's scores: an undefined feature difference costs PENALTY
+    (the cap of a finite one), a diverged run DIVERGED_PENALTY on every
+    objective of its stimulus, ten times more (PYS/efel_ext.py:73-83, :90-95
+    score them 10000 and 100000)."""
+    from neurogpu_amd import ga
+    raw = np.array([[[3.0, 4.0, 90.0, 1.0, 11.0], [0.0] + [math.nan] * 4],
+                    [[math.nan] * 5, [2.0, 5.0, 80.0, 1.1, math.nan]]])
+    bad = ga.diverged(raw)
+    assert bad.tolist() == [[False, False], [True, False]]
+    filled = ga.fill_features(raw, 100.0)
+    assert np.all(np.isfinite(filled)) and filled[1, 0, 0] == 0.0
+    target = ga.Target(np.array([[3.0, 4.0, 90.0, 1.0, 11.0], [1.0, 50.0, 90.0, 1.0, 100.0]]),
+                       np.array([1.0, 2.0, 5.0, 0.2, 2.0]))
+    obj = ga.objectives(filled, target, bad).reshape(2, 2, len(ga.GA_FEATURES))
+    assert np.all(obj[0, 0] == 0.0)
+    assert np.all(obj[1, 0] == ga.DIVERGED_PENALTY) and ga.DIVERGED_PENALTY == 10 * ga.PENALTY
+    assert np.all(obj[1, 1] < ga.PENALTY) and np.all(obj <= ga.DIVERGED_PENALTY)
+    huge = filled.copy()
+    huge[0, 1, 2] = 1e9
+    assert ga.objectives(huge, target, bad).reshape(2, 2, -1)[0, 1, 2] == ga.PENALTY
+
+
+@pytest.mark.gpu
+def test_device_features_of_a_diverged_run():
+    """A NaN parameter poisons its own run only: that run's device features
+    are all NaN, the other runs' equal the restatement on their traces."""
+    from neurogpu_amd import models
+    from neurogpu_amd._lib import Simulator
+    w = models.workload("active500", nsets=3, nt=1500)
+    m = w.model
+    P = w.params.copy()
+    P[1, :] = np.nan
+    with Simulator(0) as sim:
+        sim.set_tree(m.tree, m.mechset_id)
+        sim.set_sim(m.v_init, m.rec_sites)
+        sim.set_params(P, m.ncomps, m.nparams)
+        sim.set_stimuli(w.amps, w.dts, m.stim_seg, m.stim_area)
+        sim.run(fetch=False)
+        gf = sim.features(20.0, 180.0)
+        gv = sim.fetch().reshape(3, 3, 1, -1)
+    assert np.all(np.isnan(gf[1])) and not np.any(np.isnan(gf[[0, 2], :, 0]))
+    np.testing.assert_array_equal(gf, features_ref.features(gv, w.dts, 20.0, 180.0))
+
+
 @pytest.mark.gpu
 def test_device_features_match_oracle():
     from neurogpu_amd import models
