@@ -114,13 +114,15 @@ def calibrated_traffic(child, kernel: str):
     return out
 
 
-def measured_traffic(workload: str, nsets, nt, window: int, solve: str = "auto"):
+def measured_traffic(workload: str, nsets, nt, window: int, solve: str = "auto", generated: bool = False):
     """HBM bytes of one window launch of the simulation kernel, from two
     rocprofv3 PMC passes (FETCH_SIZE, WRITE_SIZE; KiB per dispatch) over a
     child that runs the first two windows, read on the second window's
     dispatch; each counter is scaled by what it reads for a 1 GiB
     device-to-device copy (bytes copied / counter bytes)."""
     child = ["--pmc-child", "--workload", workload, "--window", str(window), "--solve", solve]
+    if generated:
+        child.append("--generated")
     if nsets is not None:
         child += ["--nsets", str(nsets)]
     if nt is not None:
@@ -296,11 +298,22 @@ def windows(nt: int, k: int):
     return [(int(b[i]), int(b[i + 1])) for i in range(k) if b[i + 1] > b[i]]
 
 
+GENERATED_SET = {"ttpc": 3, "mainen": 4}        # NGPU_MECH_TTPC_GEN, NGPU_MECH_MAINEN_GEN
+
+
+def mechset_of(w, args) -> int:
+    if getattr(args, "generated", False):
+        if w.model.mechset not in GENERATED_SET:
+            raise SystemExit(f"no generated mechanism set for {w.model.mechset}")
+        return GENERATED_SET[w.model.mechset]
+    return w.model.mechset_id
+
+
 def configured_simulator(w, args):
     from neurogpu_amd._lib import Simulator
     m = w.model
     sim = Simulator(0)
-    sim.set_tree(m.tree, m.mechset_id)
+    sim.set_tree(m.tree, mechset_of(w, args))
     sim.set_sim(m.v_init, m.rec_sites)
     sim.set_params(w.params, m.ncomps, m.nparams)
     sim.set_stimuli(w.amps, w.dts, m.stim_seg, m.stim_area)
@@ -343,6 +356,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-copy-peak", action="store_true")
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 HBM traffic passes")
+    ap.add_argument("--generated", action="store_true",
+                    help="simulate with the mechanism set generated from the model's NMODL files "
+                         "(ttpc1200: set 3, active500: set 4) instead of the hand-written one")
     ap.add_argument("--copy-peak", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--pmc-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--window", type=int, default=0, help=argparse.SUPPRESS)
@@ -389,7 +405,7 @@ def main():
     traffic = None
     if not args.no_pmc:
         say("rocprofv3 FETCH_SIZE / WRITE_SIZE passes (child processes)")
-        traffic = measured_traffic(args.workload, args.nsets, args.nt, wlen, args.solve)
+        traffic = measured_traffic(args.workload, args.nsets, args.nt, wlen, args.solve, args.generated)
         say(f"traffic per window launch: {traffic['bytes_per_launch']:.4e} B" if traffic else "traffic: unavailable")
     copy_peak = None if args.no_copy_peak else copy_peak_gbs()
 
@@ -418,7 +434,8 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic["bytes_per_launch"] if traffic else None,
-                "kernel": f"{SIM_KERNEL}<{m.mechset}>", "bytes_per_comp_step": round(bcs, 3),
+                "kernel": f"{SIM_KERNEL}<{'SetGen_' if args.generated else ''}{m.mechset}>",
+                "bytes_per_comp_step": round(bcs, 3),
                 "kernel_ms_per_launch": round(launch_ms, 3), "comp_steps_per_launch": cs_launch,
                 "launches": len(wins), "steps_per_launch": wlen}
     if traffic:
@@ -432,7 +449,8 @@ def main():
         "metric": METRIC, "value": value, "unit": "comp-steps/s", "n_gpus": 1, "steps": len(wins),
         "warmup": args.warmup, "ms_per_step": wall / len(wins) * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"{w.name}: {m.tree.N} compartments, {m.mechset} mechanisms, "
+        "config": {"workload": f"{w.name}: {m.tree.N} compartments, "
+                               f"{m.mechset} mechanisms{' generated from their NMODL files' if args.generated else ''}, "
                                f"{w.x.shape[0]} sets x {w.amps.shape[0]} stimuli, dt {w.dts[0]} ms, "
                                f"{nt} steps; a bench step advances every run by one window of about "
                                f"{wlen} dt steps, the timed steps cover the protocol once",
