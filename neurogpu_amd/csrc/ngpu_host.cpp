@@ -1567,7 +1567,9 @@ extern "C" int ngpu_tree_write(const ngpu_tree *t, const char *path)
         return tree_fail(NGPU_E_ARG, "ngpu_tree_write: inconsistent tree sizes");
     /* the file's last record: Ks - 1 (uint16: -1 as 65535) and the end of
      * the first run of the deepest level, less one */
-    const int last_run = t->comp_by_level32[(size_t)t->lrel_starts[t->depth] * 32] - 1;
+    const int lrow = t->lrel_starts[t->depth];
+    if (lrow < 0 || lrow >= t->comp_depth) return tree_fail(NGPU_E_ARG, "ngpu_tree_write: bad level table");
+    const int last_run = t->comp_by_level32[(size_t)lrow * 32] - 1;
     if (last_run < 0 || last_run > t->n_call) return tree_fail(NGPU_E_ARG, "ngpu_tree_write: bad level table");
     std::vector<int> ksb(t->N + 1);
     for (int i = 0; i <= t->N; i++) {
