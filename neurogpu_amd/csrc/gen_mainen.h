@@ -48,7 +48,27 @@ struct Mech_ca {
     }
     NG_HD_M void trates(double v)
     {
-        rates(v);
+        double a_l0 = 0.0;
+        double b_l0 = 0.0;
+        tadj = 3.2093639532679714;
+        double efun_v1 = 0.0;
+        if ((fabs(((-(27.0 + v)) * 0.2631578947368421)) < 0.0001)) {
+            efun_v1 = (1.0 - (((-(27.0 + v)) * 0.2631578947368421) * 0.5));
+        } else {
+            const double exp_0 = ng_expc(((-(27.0 + v)) * 0.2631578947368421));
+            efun_v1 = ng_div(((-(27.0 + v)) * 0.2631578947368421), (exp_0 - 1.0));
+        }
+        a_l0 = (0.209 * efun_v1);
+        const double exp_1 = ng_expc(((-75.0 - v) * 0.058823529411764705));
+        b_l0 = (0.94 * exp_1);
+        mtau = ng_div(0.3115882195229801, (a_l0 + b_l0));
+        minf = ng_div(a_l0, (a_l0 + b_l0));
+        const double exp_2 = ng_expc(((-13.0 - v) * 0.02));
+        a_l0 = (0.000457 * exp_2);
+        const double exp_3 = ng_expc((((-v) - 15.0) * 0.03571428571428571));
+        b_l0 = ng_div(0.0065, (exp_3 + 1.0));
+        htau = ng_div(0.3115882195229801, (a_l0 + b_l0));
+        hinf = ng_div(a_l0, (a_l0 + b_l0));
     }
     NG_HD_M void rates(double vm)
     {
@@ -77,15 +97,55 @@ struct Mech_ca {
     NG_HD_M void initial()
     {
         tadj = 3.2093639532679714;
-        trates((v + 0.0));
+        double a_l1 = 0.0;
+        double b_l1 = 0.0;
+        tadj = 3.2093639532679714;
+        double efun_v2 = 0.0;
+        if ((fabs(((-(27.0 + (v + 0.0))) * 0.2631578947368421)) < 0.0001)) {
+            efun_v2 = (1.0 - (((-(27.0 + (v + 0.0))) * 0.2631578947368421) * 0.5));
+        } else {
+            const double exp_0 = ng_expc(((-(27.0 + (v + 0.0))) * 0.2631578947368421));
+            efun_v2 = ng_div(((-(27.0 + (v + 0.0))) * 0.2631578947368421), (exp_0 - 1.0));
+        }
+        a_l1 = (0.209 * efun_v2);
+        const double exp_1 = ng_expc(((-75.0 - (v + 0.0)) * 0.058823529411764705));
+        b_l1 = (0.94 * exp_1);
+        mtau = ng_div(0.3115882195229801, (a_l1 + b_l1));
+        minf = ng_div(a_l1, (a_l1 + b_l1));
+        const double exp_2 = ng_expc(((-13.0 - (v + 0.0)) * 0.02));
+        a_l1 = (0.000457 * exp_2);
+        const double exp_3 = ng_expc((((-(v + 0.0)) - 15.0) * 0.03571428571428571));
+        b_l1 = ng_div(0.0065, (exp_3 + 1.0));
+        htau = ng_div(0.3115882195229801, (a_l1 + b_l1));
+        hinf = ng_div(a_l1, (a_l1 + b_l1));
         m = minf;
         h = hinf;
     }
     NG_HD_M void states()
     {
-        trates((v + 0.0));
+        double a_l1 = 0.0;
+        double b_l1 = 0.0;
+        tadj = 3.2093639532679714;
+        double efun_v2 = 0.0;
+        if ((fabs(((-(27.0 + (v + 0.0))) * 0.2631578947368421)) < 0.0001)) {
+            efun_v2 = (1.0 - (((-(27.0 + (v + 0.0))) * 0.2631578947368421) * 0.5));
+        } else {
+            const double exp_0 = ng_expc(((-(27.0 + (v + 0.0))) * 0.2631578947368421));
+            efun_v2 = ng_div(((-(27.0 + (v + 0.0))) * 0.2631578947368421), (exp_0 - 1.0));
+        }
+        a_l1 = (0.209 * efun_v2);
+        const double exp_1 = ng_expc(((-75.0 - (v + 0.0)) * 0.058823529411764705));
+        b_l1 = (0.94 * exp_1);
+        mtau = ng_div(0.3115882195229801, (a_l1 + b_l1));
+        minf = ng_div(a_l1, (a_l1 + b_l1));
+        const double exp_2 = ng_expc(((-13.0 - (v + 0.0)) * 0.02));
+        a_l1 = (0.000457 * exp_2);
+        const double exp_3 = ng_expc((((-(v + 0.0)) - 15.0) * 0.03571428571428571));
+        b_l1 = ng_div(0.0065, (exp_3 + 1.0));
+        htau = ng_div(0.3115882195229801, (a_l1 + b_l1));
+        hinf = ng_div(a_l1, (a_l1 + b_l1));
         m = m + (1.0 - ng_expc(-(dt * ng_rcp(mtau)))) * (minf - m);
-        h = h + (1.0 - ng_expc(-(dt * ng_rcp(htau)))) * (hinf - h);
+        h = h + (1.0 - ng_expc(-(dt * ((a_l1 + b_l1) * 3.2093639532679714)))) * (hinf - h);
     }
     NG_HD_M void breakpoint()
     {
@@ -156,13 +216,21 @@ struct Mech_kca {
     }
     NG_HD_M void initial()
     {
-        rates(cai);
+        a = (Ra * ng_pow(cai, caix));
+        b = Rb;
+        tadj = 3.2093639532679714;
+        ntau = ng_div(0.3115882195229801, (a + b));
+        ninf = ng_div(a, (a + b));
         n = ninf;
     }
     NG_HD_M void states()
     {
-        rates(cai);
-        n = n + (1.0 - ng_expc(-(dt * ng_rcp(ntau)))) * (ninf - n);
+        a = (Ra * ng_pow(cai, caix));
+        b = Rb;
+        tadj = 3.2093639532679714;
+        ntau = ng_div(0.3115882195229801, (a + b));
+        ninf = ng_div(a, (a + b));
+        n = n + (1.0 - ng_expc(-(dt * ((a + b) * 3.2093639532679714)))) * (ninf - n);
     }
     NG_HD_M void breakpoint()
     {
@@ -206,7 +274,24 @@ struct Mech_km {
     }
     NG_HD_M void trates(double v)
     {
-        rates(v);
+        double efun_v1 = 0.0;
+        const double exp_0 = ng_expc(ng_div((-(v - tha)), qa));
+        if ((fabs(ng_div((-(v - tha)), qa)) < 0.0001)) {
+            efun_v1 = (1.0 - (ng_div((-(v - tha)), qa) * 0.5));
+        } else {
+            efun_v1 = ng_div(ng_div((-(v - tha)), qa), (exp_0 - 1.0));
+        }
+        a = ((Ra * qa) * efun_v1);
+        double efun_v2 = 0.0;
+        if ((fabs(ng_div((v - tha), qa)) < 0.0001)) {
+            efun_v2 = (1.0 - (ng_div((v - tha), qa) * 0.5));
+        } else {
+            efun_v2 = ng_div(ng_div((v - tha), qa), (ng_rcp(exp_0) - 1.0));
+        }
+        b = ((Rb * qa) * efun_v2);
+        tadj = 3.2093639532679714;
+        ntau = ng_div(0.3115882195229801, (a + b));
+        ninf = ng_div(a, (a + b));
     }
     NG_HD_M void rates(double v)
     {
@@ -232,13 +317,47 @@ struct Mech_km {
     NG_HD_M void initial()
     {
         tadj = 3.2093639532679714;
-        trates(v);
+        double efun_v2 = 0.0;
+        const double exp_0 = ng_expc(ng_div((-(v - tha)), qa));
+        if ((fabs(ng_div((-(v - tha)), qa)) < 0.0001)) {
+            efun_v2 = (1.0 - (ng_div((-(v - tha)), qa) * 0.5));
+        } else {
+            efun_v2 = ng_div(ng_div((-(v - tha)), qa), (exp_0 - 1.0));
+        }
+        a = ((Ra * qa) * efun_v2);
+        double efun_v3 = 0.0;
+        if ((fabs(ng_div((v - tha), qa)) < 0.0001)) {
+            efun_v3 = (1.0 - (ng_div((v - tha), qa) * 0.5));
+        } else {
+            efun_v3 = ng_div(ng_div((v - tha), qa), (ng_rcp(exp_0) - 1.0));
+        }
+        b = ((Rb * qa) * efun_v3);
+        tadj = 3.2093639532679714;
+        ntau = ng_div(0.3115882195229801, (a + b));
+        ninf = ng_div(a, (a + b));
         n = ninf;
     }
     NG_HD_M void states()
     {
-        trates(v);
-        n = n + (1.0 - ng_expc(-(dt * ng_rcp(ntau)))) * (ninf - n);
+        double efun_v2 = 0.0;
+        const double exp_0 = ng_expc(ng_div((-(v - tha)), qa));
+        if ((fabs(ng_div((-(v - tha)), qa)) < 0.0001)) {
+            efun_v2 = (1.0 - (ng_div((-(v - tha)), qa) * 0.5));
+        } else {
+            efun_v2 = ng_div(ng_div((-(v - tha)), qa), (exp_0 - 1.0));
+        }
+        a = ((Ra * qa) * efun_v2);
+        double efun_v3 = 0.0;
+        if ((fabs(ng_div((v - tha), qa)) < 0.0001)) {
+            efun_v3 = (1.0 - (ng_div((v - tha), qa) * 0.5));
+        } else {
+            efun_v3 = ng_div(ng_div((v - tha), qa), (ng_rcp(exp_0) - 1.0));
+        }
+        b = ((Rb * qa) * efun_v3);
+        tadj = 3.2093639532679714;
+        ntau = ng_div(0.3115882195229801, (a + b));
+        ninf = ng_div(a, (a + b));
+        n = n + (1.0 - ng_expc(-(dt * ((a + b) * 3.2093639532679714)))) * (ninf - n);
     }
     NG_HD_M void breakpoint()
     {
@@ -282,7 +401,24 @@ struct Mech_kv {
     }
     NG_HD_M void trates(double v)
     {
-        rates(v);
+        double efun_v1 = 0.0;
+        const double exp_0 = ng_expc(ng_div((-(v - tha)), qa));
+        if ((fabs(ng_div((-(v - tha)), qa)) < 0.0001)) {
+            efun_v1 = (1.0 - (ng_div((-(v - tha)), qa) * 0.5));
+        } else {
+            efun_v1 = ng_div(ng_div((-(v - tha)), qa), (exp_0 - 1.0));
+        }
+        a = ((Ra * qa) * efun_v1);
+        double efun_v2 = 0.0;
+        if ((fabs(ng_div((v - tha), qa)) < 0.0001)) {
+            efun_v2 = (1.0 - (ng_div((v - tha), qa) * 0.5));
+        } else {
+            efun_v2 = ng_div(ng_div((v - tha), qa), (ng_rcp(exp_0) - 1.0));
+        }
+        b = ((Rb * qa) * efun_v2);
+        tadj = 3.2093639532679714;
+        ntau = ng_div(0.3115882195229801, (a + b));
+        ninf = ng_div(a, (a + b));
     }
     NG_HD_M void rates(double v)
     {
@@ -308,13 +444,47 @@ struct Mech_kv {
     NG_HD_M void initial()
     {
         tadj = 3.2093639532679714;
-        trates(v);
+        double efun_v2 = 0.0;
+        const double exp_0 = ng_expc(ng_div((-(v - tha)), qa));
+        if ((fabs(ng_div((-(v - tha)), qa)) < 0.0001)) {
+            efun_v2 = (1.0 - (ng_div((-(v - tha)), qa) * 0.5));
+        } else {
+            efun_v2 = ng_div(ng_div((-(v - tha)), qa), (exp_0 - 1.0));
+        }
+        a = ((Ra * qa) * efun_v2);
+        double efun_v3 = 0.0;
+        if ((fabs(ng_div((v - tha), qa)) < 0.0001)) {
+            efun_v3 = (1.0 - (ng_div((v - tha), qa) * 0.5));
+        } else {
+            efun_v3 = ng_div(ng_div((v - tha), qa), (ng_rcp(exp_0) - 1.0));
+        }
+        b = ((Rb * qa) * efun_v3);
+        tadj = 3.2093639532679714;
+        ntau = ng_div(0.3115882195229801, (a + b));
+        ninf = ng_div(a, (a + b));
         n = ninf;
     }
     NG_HD_M void states()
     {
-        trates(v);
-        n = n + (1.0 - ng_expc(-(dt * ng_rcp(ntau)))) * (ninf - n);
+        double efun_v2 = 0.0;
+        const double exp_0 = ng_expc(ng_div((-(v - tha)), qa));
+        if ((fabs(ng_div((-(v - tha)), qa)) < 0.0001)) {
+            efun_v2 = (1.0 - (ng_div((-(v - tha)), qa) * 0.5));
+        } else {
+            efun_v2 = ng_div(ng_div((-(v - tha)), qa), (exp_0 - 1.0));
+        }
+        a = ((Ra * qa) * efun_v2);
+        double efun_v3 = 0.0;
+        if ((fabs(ng_div((v - tha), qa)) < 0.0001)) {
+            efun_v3 = (1.0 - (ng_div((v - tha), qa) * 0.5));
+        } else {
+            efun_v3 = ng_div(ng_div((v - tha), qa), (ng_rcp(exp_0) - 1.0));
+        }
+        b = ((Rb * qa) * efun_v3);
+        tadj = 3.2093639532679714;
+        ntau = ng_div(0.3115882195229801, (a + b));
+        ninf = ng_div(a, (a + b));
+        n = n + (1.0 - ng_expc(-(dt * ((a + b) * 3.2093639532679714)))) * (ninf - n);
     }
     NG_HD_M void breakpoint()
     {
@@ -367,7 +537,45 @@ struct Mech_na {
     }
     NG_HD_M void trates(double v)
     {
-        rates(v);
+        double a_l0 = 0.0;
+        double b_l0 = 0.0;
+        double trap0_v1 = 0.0;
+        const double exp_0 = ng_expc(ng_div((-(v - tha)), qa));
+        if ((fabs(ng_div((v - tha), qa)) > 1e-06)) {
+            trap0_v1 = ng_div((Ra * (v - tha)), (1.0 - exp_0));
+        } else {
+            trap0_v1 = (Ra * qa);
+        }
+        a_l0 = trap0_v1;
+        double trap0_v2 = 0.0;
+        if ((fabs(ng_div(((-v) - (-tha)), qa)) > 1e-06)) {
+            trap0_v2 = ng_div((Rb * ((-v) - (-tha))), (1.0 - ng_rcp(exp_0)));
+        } else {
+            trap0_v2 = (Rb * qa);
+        }
+        b_l0 = trap0_v2;
+        tadj = 3.2093639532679714;
+        mtau = ng_div(0.3115882195229801, (a_l0 + b_l0));
+        minf = ng_div(a_l0, (a_l0 + b_l0));
+        double trap0_v3 = 0.0;
+        if ((fabs(ng_div((v - thi1), qi)) > 1e-06)) {
+            const double exp_1 = ng_expc(ng_div((-(v - thi1)), qi));
+            trap0_v3 = ng_div((Rd * (v - thi1)), (1.0 - exp_1));
+        } else {
+            trap0_v3 = (Rd * qi);
+        }
+        a_l0 = trap0_v3;
+        double trap0_v4 = 0.0;
+        if ((fabs(ng_div(((-v) - (-thi2)), qi)) > 1e-06)) {
+            const double exp_2 = ng_expc(ng_div((-((-v) - (-thi2))), qi));
+            trap0_v4 = ng_div((Rg * ((-v) - (-thi2))), (1.0 - exp_2));
+        } else {
+            trap0_v4 = (Rg * qi);
+        }
+        b_l0 = trap0_v4;
+        htau = ng_div(0.3115882195229801, (a_l0 + b_l0));
+        const double exp_3 = ng_expc(ng_div((v - thinf), qinf));
+        hinf = ng_rcp((1.0 + exp_3));
     }
     NG_HD_M void rates(double vm)
     {
@@ -414,15 +622,91 @@ struct Mech_na {
     NG_HD_M void initial()
     {
         tadj = 3.2093639532679714;
-        trates((v + -5.0));
+        double a_l1 = 0.0;
+        double b_l1 = 0.0;
+        double trap0_v2 = 0.0;
+        const double exp_0 = ng_expc(ng_div((-((v + -5.0) - tha)), qa));
+        if ((fabs(ng_div(((v + -5.0) - tha), qa)) > 1e-06)) {
+            trap0_v2 = ng_div((Ra * ((v + -5.0) - tha)), (1.0 - exp_0));
+        } else {
+            trap0_v2 = (Ra * qa);
+        }
+        a_l1 = trap0_v2;
+        double trap0_v3 = 0.0;
+        if ((fabs(ng_div(((-(v + -5.0)) - (-tha)), qa)) > 1e-06)) {
+            trap0_v3 = ng_div((Rb * ((-(v + -5.0)) - (-tha))), (1.0 - ng_rcp(exp_0)));
+        } else {
+            trap0_v3 = (Rb * qa);
+        }
+        b_l1 = trap0_v3;
+        tadj = 3.2093639532679714;
+        mtau = ng_div(0.3115882195229801, (a_l1 + b_l1));
+        minf = ng_div(a_l1, (a_l1 + b_l1));
+        double trap0_v4 = 0.0;
+        if ((fabs(ng_div(((v + -5.0) - thi1), qi)) > 1e-06)) {
+            const double exp_1 = ng_expc(ng_div((-((v + -5.0) - thi1)), qi));
+            trap0_v4 = ng_div((Rd * ((v + -5.0) - thi1)), (1.0 - exp_1));
+        } else {
+            trap0_v4 = (Rd * qi);
+        }
+        a_l1 = trap0_v4;
+        double trap0_v5 = 0.0;
+        if ((fabs(ng_div(((-(v + -5.0)) - (-thi2)), qi)) > 1e-06)) {
+            const double exp_2 = ng_expc(ng_div((-((-(v + -5.0)) - (-thi2))), qi));
+            trap0_v5 = ng_div((Rg * ((-(v + -5.0)) - (-thi2))), (1.0 - exp_2));
+        } else {
+            trap0_v5 = (Rg * qi);
+        }
+        b_l1 = trap0_v5;
+        htau = ng_div(0.3115882195229801, (a_l1 + b_l1));
+        const double exp_3 = ng_expc(ng_div(((v + -5.0) - thinf), qinf));
+        hinf = ng_rcp((1.0 + exp_3));
         m = minf;
         h = hinf;
     }
     NG_HD_M void states()
     {
-        trates((v + -5.0));
+        double a_l1 = 0.0;
+        double b_l1 = 0.0;
+        double trap0_v2 = 0.0;
+        const double exp_0 = ng_expc(ng_div((-((v + -5.0) - tha)), qa));
+        if ((fabs(ng_div(((v + -5.0) - tha), qa)) > 1e-06)) {
+            trap0_v2 = ng_div((Ra * ((v + -5.0) - tha)), (1.0 - exp_0));
+        } else {
+            trap0_v2 = (Ra * qa);
+        }
+        a_l1 = trap0_v2;
+        double trap0_v3 = 0.0;
+        if ((fabs(ng_div(((-(v + -5.0)) - (-tha)), qa)) > 1e-06)) {
+            trap0_v3 = ng_div((Rb * ((-(v + -5.0)) - (-tha))), (1.0 - ng_rcp(exp_0)));
+        } else {
+            trap0_v3 = (Rb * qa);
+        }
+        b_l1 = trap0_v3;
+        tadj = 3.2093639532679714;
+        mtau = ng_div(0.3115882195229801, (a_l1 + b_l1));
+        minf = ng_div(a_l1, (a_l1 + b_l1));
+        double trap0_v4 = 0.0;
+        if ((fabs(ng_div(((v + -5.0) - thi1), qi)) > 1e-06)) {
+            const double exp_1 = ng_expc(ng_div((-((v + -5.0) - thi1)), qi));
+            trap0_v4 = ng_div((Rd * ((v + -5.0) - thi1)), (1.0 - exp_1));
+        } else {
+            trap0_v4 = (Rd * qi);
+        }
+        a_l1 = trap0_v4;
+        double trap0_v5 = 0.0;
+        if ((fabs(ng_div(((-(v + -5.0)) - (-thi2)), qi)) > 1e-06)) {
+            const double exp_2 = ng_expc(ng_div((-((-(v + -5.0)) - (-thi2))), qi));
+            trap0_v5 = ng_div((Rg * ((-(v + -5.0)) - (-thi2))), (1.0 - exp_2));
+        } else {
+            trap0_v5 = (Rg * qi);
+        }
+        b_l1 = trap0_v5;
+        htau = ng_div(0.3115882195229801, (a_l1 + b_l1));
+        const double exp_3 = ng_expc(ng_div(((v + -5.0) - thinf), qinf));
+        hinf = ng_rcp((1.0 + exp_3));
         m = m + (1.0 - ng_expc(-(dt * ng_rcp(mtau)))) * (minf - m);
-        h = h + (1.0 - ng_expc(-(dt * ng_rcp(htau)))) * (hinf - h);
+        h = h + (1.0 - ng_expc(-(dt * ((a_l1 + b_l1) * 3.2093639532679714)))) * (hinf - h);
     }
     NG_HD_M void breakpoint()
     {

@@ -48,15 +48,43 @@ struct Mech_Ca_HVA {
     }
     NG_HD_M void initial()
     {
-        rates();
+        if ((v == -27.0)) {
+            v = (v + 0.0001);
+        }
+        const double exp_0 = ng_expc(((-27.0 - v) * 0.2631578947368421));
+        mAlpha = ng_div((0.055 * (-27.0 - v)), (exp_0 - 1.0));
+        const double exp_1 = ng_expc(((-75.0 - v) * 0.058823529411764705));
+        mBeta = (0.94 * exp_1);
+        mInf = ng_div(mAlpha, (mAlpha + mBeta));
+        mTau = ng_rcp((mAlpha + mBeta));
+        const double exp_2 = ng_expc(((-13.0 - v) * 0.02));
+        hAlpha = (0.000457 * exp_2);
+        const double exp_3 = ng_expc((((-v) - 15.0) * 0.03571428571428571));
+        hBeta = ng_div(0.0065, (exp_3 + 1.0));
+        hInf = ng_div(hAlpha, (hAlpha + hBeta));
+        hTau = ng_rcp((hAlpha + hBeta));
         m = mInf;
         h = hInf;
     }
     NG_HD_M void states()
     {
-        rates();
-        m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
-        h = h + (1.0 - ng_expc(-(dt * ng_rcp(hTau)))) * (hInf - h);
+        if ((v == -27.0)) {
+            v = (v + 0.0001);
+        }
+        const double exp_0 = ng_expc(((-27.0 - v) * 0.2631578947368421));
+        mAlpha = ng_div((0.055 * (-27.0 - v)), (exp_0 - 1.0));
+        const double exp_1 = ng_expc(((-75.0 - v) * 0.058823529411764705));
+        mBeta = (0.94 * exp_1);
+        mInf = ng_div(mAlpha, (mAlpha + mBeta));
+        mTau = ng_rcp((mAlpha + mBeta));
+        const double exp_2 = ng_expc(((-13.0 - v) * 0.02));
+        hAlpha = (0.000457 * exp_2);
+        const double exp_3 = ng_expc((((-v) - 15.0) * 0.03571428571428571));
+        hBeta = ng_div(0.0065, (exp_3 + 1.0));
+        hInf = ng_div(hAlpha, (hAlpha + hBeta));
+        hTau = ng_rcp((hAlpha + hBeta));
+        m = m + (1.0 - ng_expc(-(dt * (mAlpha + mBeta)))) * (mInf - m);
+        h = h + (1.0 - ng_expc(-(dt * (hAlpha + hBeta)))) * (hInf - h);
     }
     NG_HD_M void breakpoint()
     {
@@ -95,13 +123,35 @@ struct Mech_Ca_LVAst {
     }
     NG_HD_M void initial()
     {
-        rates();
+        double qt_l0 = 0.0;
+        qt_l0 = 2.952882641412121;
+        v = (v + 10.0);
+        const double exp_0 = ng_expc(((v - -30.0) * -0.16666666666666666));
+        mInf = ng_rcp((1.0 + exp_0));
+        const double exp_1 = ng_expc(((v - -25.0) * 0.2));
+        mTau = ((5.0 + ng_div(20.0, (1.0 + exp_1))) * 0.3386521313023745);
+        const double exp_2 = ng_expc(((v - -80.0) * 0.15625));
+        hInf = ng_rcp((1.0 + exp_2));
+        const double exp_3 = ng_expc(((v - -40.0) * 0.14285714285714285));
+        hTau = ((20.0 + ng_div(50.0, (1.0 + exp_3))) * 0.3386521313023745);
+        v = (v - 10.0);
         m = mInf;
         h = hInf;
     }
     NG_HD_M void states()
     {
-        rates();
+        double qt_l0 = 0.0;
+        qt_l0 = 2.952882641412121;
+        v = (v + 10.0);
+        const double exp_0 = ng_expc(((v - -30.0) * -0.16666666666666666));
+        mInf = ng_rcp((1.0 + exp_0));
+        const double exp_1 = ng_expc(((v - -25.0) * 0.2));
+        mTau = ((5.0 + ng_div(20.0, (1.0 + exp_1))) * 0.3386521313023745);
+        const double exp_2 = ng_expc(((v - -80.0) * 0.15625));
+        hInf = ng_rcp((1.0 + exp_2));
+        const double exp_3 = ng_expc(((v - -40.0) * 0.14285714285714285));
+        hTau = ((20.0 + ng_div(50.0, (1.0 + exp_3))) * 0.3386521313023745);
+        v = (v - 10.0);
         m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
         h = h + (1.0 - ng_expc(-(dt * ng_rcp(hTau)))) * (hInf - h);
     }
@@ -160,13 +210,29 @@ struct Mech_Ih {
     }
     NG_HD_M void initial()
     {
-        rates();
+        if ((v == -154.9)) {
+            v = (v + 0.0001);
+        }
+        const double exp_0 = ng_expc(((v + 154.9) * 0.08403361344537814));
+        mAlpha = ng_div((0.00643 * (v + 154.9)), (exp_0 - 1.0));
+        const double exp_1 = ng_expc((v * 0.030211480362537763));
+        mBeta = (0.193 * exp_1);
+        mInf = ng_div(mAlpha, (mAlpha + mBeta));
+        mTau = ng_rcp((mAlpha + mBeta));
         m = mInf;
     }
     NG_HD_M void states()
     {
-        rates();
-        m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
+        if ((v == -154.9)) {
+            v = (v + 0.0001);
+        }
+        const double exp_0 = ng_expc(((v + 154.9) * 0.08403361344537814));
+        mAlpha = ng_div((0.00643 * (v + 154.9)), (exp_0 - 1.0));
+        const double exp_1 = ng_expc((v * 0.030211480362537763));
+        mBeta = (0.193 * exp_1);
+        mInf = ng_div(mAlpha, (mAlpha + mBeta));
+        mTau = ng_rcp((mAlpha + mBeta));
+        m = m + (1.0 - ng_expc(-(dt * (mAlpha + mBeta)))) * (mInf - m);
     }
     NG_HD_M void breakpoint()
     {
@@ -199,13 +265,25 @@ struct Mech_Im {
     }
     NG_HD_M void initial()
     {
-        rates();
+        double qt_l0 = 0.0;
+        qt_l0 = 2.952882641412121;
+        const double exp_0 = ng_expc((0.1 * (v - -35.0)));
+        mAlpha = (0.0033 * exp_0);
+        mBeta = (0.0033 * ng_rcp(exp_0));
+        mInf = ng_div(mAlpha, (mAlpha + mBeta));
+        mTau = (ng_rcp((mAlpha + mBeta)) * 0.3386521313023745);
         m = mInf;
     }
     NG_HD_M void states()
     {
-        rates();
-        m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
+        double qt_l0 = 0.0;
+        qt_l0 = 2.952882641412121;
+        const double exp_0 = ng_expc((0.1 * (v - -35.0)));
+        mAlpha = (0.0033 * exp_0);
+        mBeta = (0.0033 * ng_rcp(exp_0));
+        mInf = ng_div(mAlpha, (mAlpha + mBeta));
+        mTau = (ng_rcp((mAlpha + mBeta)) * 0.3386521313023745);
+        m = m + (1.0 - ng_expc(-(dt * ((mAlpha + mBeta) * 2.952882641412121)))) * (mInf - m);
     }
     NG_HD_M void breakpoint()
     {
@@ -248,13 +326,43 @@ struct Mech_K_Pst {
     }
     NG_HD_M void initial()
     {
-        rates();
+        double qt_l0 = 0.0;
+        qt_l0 = 2.952882641412121;
+        v = (v + 10.0);
+        const double exp_0 = ng_expc(((-(v + 1.0)) * 0.08333333333333333));
+        mInf = ng_rcp((1.0 + exp_0));
+        if ((v < -50.0)) {
+            const double exp_1 = ng_expc(((-v) * -0.026));
+            mTau = ((1.25 + (175.03 * exp_1)) * 0.3386521313023745);
+        } else {
+            const double exp_2 = ng_expc(((-v) * 0.026));
+            mTau = ((1.25 + (13.0 * exp_2)) * 0.3386521313023745);
+        }
+        const double exp_3 = ng_expc(((-(v + 54.0)) * -0.09090909090909091));
+        hInf = ng_rcp((1.0 + exp_3));
+        hTau = ((360.0 + ((1010.0 + (24.0 * (v + 55.0))) * ng_expc((-ng_pow(((v + 75.0) * 0.020833333333333332), 2.0))))) * 0.3386521313023745);
+        v = (v - 10.0);
         m = mInf;
         h = hInf;
     }
     NG_HD_M void states()
     {
-        rates();
+        double qt_l0 = 0.0;
+        qt_l0 = 2.952882641412121;
+        v = (v + 10.0);
+        const double exp_0 = ng_expc(((-(v + 1.0)) * 0.08333333333333333));
+        mInf = ng_rcp((1.0 + exp_0));
+        if ((v < -50.0)) {
+            const double exp_1 = ng_expc(((-v) * -0.026));
+            mTau = ((1.25 + (175.03 * exp_1)) * 0.3386521313023745);
+        } else {
+            const double exp_2 = ng_expc(((-v) * 0.026));
+            mTau = ((1.25 + (13.0 * exp_2)) * 0.3386521313023745);
+        }
+        const double exp_3 = ng_expc(((-(v + 54.0)) * -0.09090909090909091));
+        hInf = ng_rcp((1.0 + exp_3));
+        hTau = ((360.0 + ((1010.0 + (24.0 * (v + 55.0))) * ng_expc((-ng_pow(((v + 75.0) * 0.020833333333333332), 2.0))))) * 0.3386521313023745);
+        v = (v - 10.0);
         m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
         h = h + (1.0 - ng_expc(-(dt * ng_rcp(hTau)))) * (hInf - h);
     }
@@ -293,13 +401,31 @@ struct Mech_K_Tst {
     }
     NG_HD_M void initial()
     {
-        rates();
+        double qt_l0 = 0.0;
+        qt_l0 = 2.952882641412121;
+        v = (v + 10.0);
+        const double exp_0 = ng_expc(((-(v + 0.0)) * 0.05263157894736842));
+        mInf = ng_rcp((1.0 + exp_0));
+        mTau = ((0.34 + (0.92 * ng_expc((-ng_pow(((v + 71.0) * 0.01694915254237288), 2.0))))) * 0.3386521313023745);
+        const double exp_1 = ng_expc(((-(v + 66.0)) * -0.1));
+        hInf = ng_rcp((1.0 + exp_1));
+        hTau = ((8.0 + (49.0 * ng_expc((-ng_pow(((v + 73.0) * 0.043478260869565216), 2.0))))) * 0.3386521313023745);
+        v = (v - 10.0);
         m = mInf;
         h = hInf;
     }
     NG_HD_M void states()
     {
-        rates();
+        double qt_l0 = 0.0;
+        qt_l0 = 2.952882641412121;
+        v = (v + 10.0);
+        const double exp_0 = ng_expc(((-(v + 0.0)) * 0.05263157894736842));
+        mInf = ng_rcp((1.0 + exp_0));
+        mTau = ((0.34 + (0.92 * ng_expc((-ng_pow(((v + 71.0) * 0.01694915254237288), 2.0))))) * 0.3386521313023745);
+        const double exp_1 = ng_expc(((-(v + 66.0)) * -0.1));
+        hInf = ng_rcp((1.0 + exp_1));
+        hTau = ((8.0 + (49.0 * ng_expc((-ng_pow(((v + 73.0) * 0.043478260869565216), 2.0))))) * 0.3386521313023745);
+        v = (v - 10.0);
         m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
         h = h + (1.0 - ng_expc(-(dt * ng_rcp(hTau)))) * (hInf - h);
     }
@@ -356,15 +482,61 @@ struct Mech_Nap_Et2 {
     }
     NG_HD_M void initial()
     {
-        rates();
+        double qt_l0 = 0.0;
+        qt_l0 = 2.952882641412121;
+        const double exp_0 = ng_expc(((v - -52.6) * -0.2173913043478261));
+        mInf = ng_rcp((1.0 + exp_0));
+        if ((v == -38.0)) {
+            v = (v + 0.0001);
+        }
+        const double exp_1 = ng_expc(((-(v - -38.0)) * 0.16666666666666666));
+        mAlpha = ng_div((0.182 * (v - -38.0)), (1.0 - exp_1));
+        mBeta = ng_div((0.124 * ((-v) - 38.0)), (1.0 - ng_rcp(exp_1)));
+        mTau = ((6.0 * ng_rcp((mAlpha + mBeta))) * 0.3386521313023745);
+        if ((v == -17.0)) {
+            v = (v + 0.0001);
+        }
+        if ((v == -64.4)) {
+            v = (v + 0.0001);
+        }
+        const double exp_2 = ng_expc(((v - -48.8) * 0.1));
+        hInf = ng_rcp((1.0 + exp_2));
+        const double exp_3 = ng_expc(((v + 17.0) * 0.2159827213822894));
+        hAlpha = ng_div((-2.88e-06 * (v + 17.0)), (1.0 - exp_3));
+        const double exp_4 = ng_expc(((-(v + 64.4)) * 0.3802281368821293));
+        hBeta = ng_div((6.94e-06 * (v + 64.4)), (1.0 - exp_4));
+        hTau = (ng_rcp((hAlpha + hBeta)) * 0.3386521313023745);
         m = mInf;
         h = hInf;
     }
     NG_HD_M void states()
     {
-        rates();
+        double qt_l0 = 0.0;
+        qt_l0 = 2.952882641412121;
+        const double exp_0 = ng_expc(((v - -52.6) * -0.2173913043478261));
+        mInf = ng_rcp((1.0 + exp_0));
+        if ((v == -38.0)) {
+            v = (v + 0.0001);
+        }
+        const double exp_1 = ng_expc(((-(v - -38.0)) * 0.16666666666666666));
+        mAlpha = ng_div((0.182 * (v - -38.0)), (1.0 - exp_1));
+        mBeta = ng_div((0.124 * ((-v) - 38.0)), (1.0 - ng_rcp(exp_1)));
+        mTau = ((6.0 * ng_rcp((mAlpha + mBeta))) * 0.3386521313023745);
+        if ((v == -17.0)) {
+            v = (v + 0.0001);
+        }
+        if ((v == -64.4)) {
+            v = (v + 0.0001);
+        }
+        const double exp_2 = ng_expc(((v - -48.8) * 0.1));
+        hInf = ng_rcp((1.0 + exp_2));
+        const double exp_3 = ng_expc(((v + 17.0) * 0.2159827213822894));
+        hAlpha = ng_div((-2.88e-06 * (v + 17.0)), (1.0 - exp_3));
+        const double exp_4 = ng_expc(((-(v + 64.4)) * 0.3802281368821293));
+        hBeta = ng_div((6.94e-06 * (v + 64.4)), (1.0 - exp_4));
+        hTau = (ng_rcp((hAlpha + hBeta)) * 0.3386521313023745);
         m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
-        h = h + (1.0 - ng_expc(-(dt * ng_rcp(hTau)))) * (hInf - h);
+        h = h + (1.0 - ng_expc(-(dt * ((hAlpha + hBeta) * 2.952882641412121)))) * (hInf - h);
     }
     NG_HD_M void breakpoint()
     {
@@ -413,15 +585,49 @@ struct Mech_NaTa_t {
     }
     NG_HD_M void initial()
     {
-        rates();
+        double qt_l0 = 0.0;
+        qt_l0 = 2.952882641412121;
+        if ((v == -38.0)) {
+            v = (v + 0.0001);
+        }
+        const double exp_0 = ng_expc(((-(v - -38.0)) * 0.16666666666666666));
+        mAlpha = ng_div((0.182 * (v - -38.0)), (1.0 - exp_0));
+        mBeta = ng_div((0.124 * ((-v) - 38.0)), (1.0 - ng_rcp(exp_0)));
+        mTau = (ng_rcp((mAlpha + mBeta)) * 0.3386521313023745);
+        mInf = ng_div(mAlpha, (mAlpha + mBeta));
+        if ((v == -66.0)) {
+            v = (v + 0.0001);
+        }
+        const double exp_1 = ng_expc(((v - -66.0) * 0.16666666666666666));
+        hAlpha = ng_div((-0.015 * (v - -66.0)), (1.0 - exp_1));
+        hBeta = ng_div((-0.015 * ((-v) - 66.0)), (1.0 - ng_rcp(exp_1)));
+        hTau = (ng_rcp((hAlpha + hBeta)) * 0.3386521313023745);
+        hInf = ng_div(hAlpha, (hAlpha + hBeta));
         m = mInf;
         h = hInf;
     }
     NG_HD_M void states()
     {
-        rates();
-        m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
-        h = h + (1.0 - ng_expc(-(dt * ng_rcp(hTau)))) * (hInf - h);
+        double qt_l0 = 0.0;
+        qt_l0 = 2.952882641412121;
+        if ((v == -38.0)) {
+            v = (v + 0.0001);
+        }
+        const double exp_0 = ng_expc(((-(v - -38.0)) * 0.16666666666666666));
+        mAlpha = ng_div((0.182 * (v - -38.0)), (1.0 - exp_0));
+        mBeta = ng_div((0.124 * ((-v) - 38.0)), (1.0 - ng_rcp(exp_0)));
+        mTau = (ng_rcp((mAlpha + mBeta)) * 0.3386521313023745);
+        mInf = ng_div(mAlpha, (mAlpha + mBeta));
+        if ((v == -66.0)) {
+            v = (v + 0.0001);
+        }
+        const double exp_1 = ng_expc(((v - -66.0) * 0.16666666666666666));
+        hAlpha = ng_div((-0.015 * (v - -66.0)), (1.0 - exp_1));
+        hBeta = ng_div((-0.015 * ((-v) - 66.0)), (1.0 - ng_rcp(exp_1)));
+        hTau = (ng_rcp((hAlpha + hBeta)) * 0.3386521313023745);
+        hInf = ng_div(hAlpha, (hAlpha + hBeta));
+        m = m + (1.0 - ng_expc(-(dt * ((mAlpha + mBeta) * 2.952882641412121)))) * (mInf - m);
+        h = h + (1.0 - ng_expc(-(dt * ((hAlpha + hBeta) * 2.952882641412121)))) * (hInf - h);
     }
     NG_HD_M void breakpoint()
     {
@@ -470,15 +676,49 @@ struct Mech_NaTs2_t {
     }
     NG_HD_M void initial()
     {
-        rates();
+        double qt_l0 = 0.0;
+        qt_l0 = 2.952882641412121;
+        if ((v == -32.0)) {
+            v = (v + 0.0001);
+        }
+        const double exp_0 = ng_expc(((-(v - -32.0)) * 0.16666666666666666));
+        mAlpha = ng_div((0.182 * (v - -32.0)), (1.0 - exp_0));
+        mBeta = ng_div((0.124 * ((-v) - 32.0)), (1.0 - ng_rcp(exp_0)));
+        mInf = ng_div(mAlpha, (mAlpha + mBeta));
+        mTau = (ng_rcp((mAlpha + mBeta)) * 0.3386521313023745);
+        if ((v == -60.0)) {
+            v = (v + 0.0001);
+        }
+        const double exp_1 = ng_expc(((v - -60.0) * 0.16666666666666666));
+        hAlpha = ng_div((-0.015 * (v - -60.0)), (1.0 - exp_1));
+        hBeta = ng_div((-0.015 * ((-v) - 60.0)), (1.0 - ng_rcp(exp_1)));
+        hInf = ng_div(hAlpha, (hAlpha + hBeta));
+        hTau = (ng_rcp((hAlpha + hBeta)) * 0.3386521313023745);
         m = mInf;
         h = hInf;
     }
     NG_HD_M void states()
     {
-        rates();
-        m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
-        h = h + (1.0 - ng_expc(-(dt * ng_rcp(hTau)))) * (hInf - h);
+        double qt_l0 = 0.0;
+        qt_l0 = 2.952882641412121;
+        if ((v == -32.0)) {
+            v = (v + 0.0001);
+        }
+        const double exp_0 = ng_expc(((-(v - -32.0)) * 0.16666666666666666));
+        mAlpha = ng_div((0.182 * (v - -32.0)), (1.0 - exp_0));
+        mBeta = ng_div((0.124 * ((-v) - 32.0)), (1.0 - ng_rcp(exp_0)));
+        mInf = ng_div(mAlpha, (mAlpha + mBeta));
+        mTau = (ng_rcp((mAlpha + mBeta)) * 0.3386521313023745);
+        if ((v == -60.0)) {
+            v = (v + 0.0001);
+        }
+        const double exp_1 = ng_expc(((v - -60.0) * 0.16666666666666666));
+        hAlpha = ng_div((-0.015 * (v - -60.0)), (1.0 - exp_1));
+        hBeta = ng_div((-0.015 * ((-v) - 60.0)), (1.0 - ng_rcp(exp_1)));
+        hInf = ng_div(hAlpha, (hAlpha + hBeta));
+        hTau = (ng_rcp((hAlpha + hBeta)) * 0.3386521313023745);
+        m = m + (1.0 - ng_expc(-(dt * ((mAlpha + mBeta) * 2.952882641412121)))) * (mInf - m);
+        h = h + (1.0 - ng_expc(-(dt * ((hAlpha + hBeta) * 2.952882641412121)))) * (hInf - h);
     }
     NG_HD_M void breakpoint()
     {
@@ -559,13 +799,19 @@ struct Mech_SKv3_1 {
     }
     NG_HD_M void initial()
     {
-        rates();
+        const double exp_0 = ng_expc(((v - 18.7) * -0.10309278350515465));
+        mInf = ng_rcp((1.0 + exp_0));
+        const double exp_1 = ng_expc(((v - -46.56) * -0.022655188038060714));
+        mTau = ng_div(4.0, (1.0 + exp_1));
         m = mInf;
     }
     NG_HD_M void states()
     {
-        rates();
-        m = m + (1.0 - ng_expc(-(dt * ng_rcp(mTau)))) * (mInf - m);
+        const double exp_0 = ng_expc(((v - 18.7) * -0.10309278350515465));
+        mInf = ng_rcp((1.0 + exp_0));
+        const double exp_1 = ng_expc(((v - -46.56) * -0.022655188038060714));
+        mTau = ng_div(4.0, (1.0 + exp_1));
+        m = m + (1.0 - ng_expc(-(dt * ((1.0 + exp_1) * 0.25)))) * (mInf - m);
     }
     NG_HD_M void breakpoint()
     {

@@ -611,11 +611,36 @@ def _neg_unit_over(node: ast.expr) -> Optional[ast.expr]:
     return None
 
 
-def fast_cnexp(x: str, a: ast.expr, b: ast.expr, consts: Dict[str, float]) -> str:
-    """C++ of one cnexp step of x' = A + B x in the fast form."""
+def _reciprocal_of(d: ast.expr) -> Optional[ast.expr]:
+    """1/d without a division when d (a fast form) is a quotient, a refined
+    reciprocal or a refined reciprocal times a constant, else None."""
+    if isinstance(d, ast.Call) and isinstance(d.func, ast.Name):
+        if d.func.id == "ng_div" and len(d.args) == 2:
+            n, den = d.args
+            if _num(n) is not None and _num(n) != 0.0:
+                return ast.BinOp(den, ast.Mult(), _c(1.0 / _num(n)))
+            return ast.Call(ast.Name("ng_div"), [den, n], [])
+        if d.func.id == "ng_rcp" and len(d.args) == 1:
+            return d.args[0]
+    if isinstance(d, ast.BinOp) and isinstance(d.op, ast.Mult):
+        for r, c in ((d.left, d.right), (d.right, d.left)):
+            if (_num(c) not in (None, 0.0) and isinstance(r, ast.Call) and isinstance(r.func, ast.Name)
+                    and r.func.id == "ng_rcp" and len(r.args) == 1):
+                return ast.BinOp(r.args[0], ast.Mult(), _c(1.0 / _num(c)))
+    return None
+
+
+def fast_cnexp(x: str, a: ast.expr, b: ast.expr, consts: Dict[str, float],
+               defs: Optional[Dict[str, ast.expr]] = None) -> str:
+    """C++ of one cnexp step of x' = A + B x in the fast form; ``defs``: the
+    fast forms last assigned to names of the statement list (a time constant
+    tau = c / (a + b) gives the rate (a + b) / c without a reciprocal)."""
     T = _neg_unit_over(b)
     if T is not None:
-        rate = cxx(fast_expr(ast.Call(ast.Name("ng_rcp"), [T], []), consts))
+        r = None
+        if defs and isinstance(T, ast.Name) and T.id in defs:
+            r = _reciprocal_of(defs[T.id])
+        rate = cxx(fast_expr(r if r is not None else ast.Call(ast.Name("ng_rcp"), [T], []), consts))
         if isinstance(a, ast.BinOp) and isinstance(a.op, ast.Div) and ast.dump(a.right) == ast.dump(T):
             inf = fast_expr(a.left, consts)                      # A = X / T: -A/B = X
         else:
@@ -667,19 +692,21 @@ def _inlinable(fn: Block) -> bool:
     assigned = set(_assigned_names(body))
 
     def plain(st):
-        return all(x.kind in ("assign", "local") or (x.kind == "if" and all(plain(b) for _, b in x.branches))
+        return all(x.kind in ("assign", "local", "call") or (x.kind == "if" and all(plain(b) for _, b in x.branches))
                    for x in st)
     return plain(body) and not (assigned & set(fn.args))
 
 
 def _inline_functions(stmts: List[Stmt], mech: Mechanism, counter: List[int]) -> List[Stmt]:
     """The statement list with each call of one of the mechanism's FUNCTIONs
-    in an assignment replaced by the function's statements (its arguments
-    substituted as expressions, its LOCALs and its value renamed apart), so
-    that exps inside FUNCTIONs take part in the sharing.  Functions that
-    assign their arguments or call procedures stay calls."""
+    in an assignment, and each PROCEDURE call statement, replaced by the
+    block's statements (its arguments substituted as expressions, its LOCALs
+    and a function's value renamed apart), so that exps inside them take
+    part in the sharing and the cnexp sees the assignments of the time
+    constants.  Blocks that assign their arguments stay calls."""
     funcs = {b.name: b for b in mech.blocks.values() if b.kind == "FUNCTION" and _inlinable(b)}
-    if not funcs:
+    procs = {b.name: b for b in mech.blocks.values() if b.kind == "PROCEDURE" and _inlinable(b)}
+    if not funcs and not procs:
         return stmts
 
     def inline_expr(node: ast.expr, pre: List[Stmt]) -> ast.expr:
@@ -712,6 +739,22 @@ def _inline_functions(stmts: List[Stmt], mech: Mechanism, counter: List[int]) ->
             node = inline_expr(_py(s.expr), pre)
             out += pre
             out.append(Stmt("assign", s.target, ast.unparse(node)) if pre else s)
+        elif s.kind == "call" and s.target in procs:
+            pr = procs[s.target]
+            args = [a.strip() for a in s.expr.split(",") if a.strip()]
+            if len(args) != len(pr.args):
+                raise NmodlError(f"{pr.name} takes {len(pr.args)} arguments")
+            pre = []
+            argn = [inline_expr(_py(a), pre) for a in args]
+            k = counter[0]
+            counter[0] += 1
+            body = statements(pr.body)
+            mapping = dict(zip(pr.args, argn))
+            for st in body:
+                if st.kind == "local":
+                    mapping[st.target] = ast.Name(f"{st.target}_l{k}")
+            out += pre
+            out += _inline_functions(_rename(body, mapping), mech, counter)
         elif s.kind == "if":
             out.append(Stmt("if", branches=[(c, _inline_functions(body, mech, counter)) for c, body in s.branches]))
         else:
@@ -750,17 +793,24 @@ def _emit(stmts: List[Stmt], mech: Mechanism, ind: str, locals_: List[str], ret:
     out = []
     env = None if fast is None else dict(fast)
     sh = None if fast is None else _ExpShare(share)
+    defs: Dict[str, ast.expr] = {}            # fast form last assigned to a name of this list
     if fast is not None and counter is None:
         counter = [0]
         stmts = _inline_functions(stmts, mech, counter)
 
-    def ex(e: ast.expr) -> str:
+    def ex(e: ast.expr, keep: Optional[list] = None) -> str:
         if env is None:
             return cxx(e)
         sh.pre = []
-        code = cxx(fast_expr(e, env, sh))
+        fe = fast_expr(e, env, sh)
+        if keep is not None:
+            keep.append(fe)
         out.extend(ind + line for line in sh.pre)
-        return code
+        return cxx(fe)
+
+    def forget(name: str) -> None:
+        for k in [k for k, d in defs.items() if k == name or name in _names_in(d)]:
+            del defs[k]
 
     for i, s in enumerate(stmts):
         if s.kind == "local":
@@ -770,11 +820,15 @@ def _emit(stmts: List[Stmt], mech: Mechanism, ind: str, locals_: List[str], ret:
         elif s.kind == "assign":
             tgt = "ret_" if ret is not None and s.target == ret else s.target
             e = _py(s.expr)
-            code = ex(e)
+            keep: list = []
+            code = ex(e, keep)
             if env is not None:
                 val = _num(fast_expr(e, env))
                 env.pop(s.target, None)
                 sh.kill(s.target)
+                forget(s.target)
+                if tgt != "ret_":
+                    defs[s.target] = keep[0]
                 # a LOCAL assigned a constant, or a derived constant assigned its value
                 if val is not None and tgt != "ret_" and (s.target in locals_ or fast.get(s.target) == val):
                     env[s.target] = val
@@ -786,12 +840,14 @@ def _emit(stmts: List[Stmt], mech: Mechanism, ind: str, locals_: List[str], ret:
             if env is not None:
                 # a procedure may assign any field
                 sh.table = {}
+                defs.clear()
         elif s.kind == "deriv":
             a, b = linear(_py(s.expr), s.target)
             x = s.target
             if env is not None:
-                out.append(ind + fast_cnexp(x, a, b, env))
+                out.append(ind + fast_cnexp(x, a, b, env, defs))
                 sh.kill(x)
+                forget(x)
             else:
                 A, B = cxx(a), cxx(b)
                 out.append(f"{ind}{x} = {x} + (1.0 - ng_exp(dt * ({B}))) * (-({A}) / ({B}) - {x});")
@@ -827,6 +883,7 @@ def _emit(stmts: List[Stmt], mech: Mechanism, ind: str, locals_: List[str], ret:
                     for t in _assigned_names(body):
                         env.pop(t, None)
                         sh.kill(t)
+                        forget(t)
     return out
 
 
