@@ -537,3 +537,91 @@ def test_generated_set_steps_like_the_hand_written_one(sets, family):
     full = (1 << nmech) - 1
     assert H.set_ica_bits(hw, full) == H.set_ica_bits(gw, full) != 0
     print(f"{family}: worst relative state difference {worst:.2e} over {len(cols)} masks")
+
+
+def test_a_third_cell_type_generates_a_working_set(tmp_path):
+    """The BBP cortical cell's 14 mechanisms (Figures/Figure2_BBP_CC_clean,
+    with a different Ca channel and KdShu2007; branching.mod, a VERBATIM
+    block, left out) through the same generator: the set compiles for the
+    host and for gfx950, and on the host every mask (the full one and random
+    ones) keeps its states finite, gates in [0, 1] and the currents finite
+    over 300 steps of a swept voltage."""
+    d = "/root/reference/Figures/Figure2_BBP_CC_clean"
+    if not os.path.isdir(d):
+        pytest.skip("the reference's .mod files are not present here")
+    mods = sorted(os.path.join(d, f) for f in os.listdir(d) if f.endswith(".mod") and f != "branching.mod")
+    hdr = tmp_path / "gen_cc.h"
+    assert nmodl.main(["-o", str(hdr), "--set", "cc", "--celsius", "34", "--ion", "ek=-85", "--ion", "ena=50",
+                       "--ion", "cao=2", "--ion", "cai0=5e-5", "--fast"] + mods) == 0
+    (tmp_path / "cc.cpp").write_text('''#include "gen_cc.h"
+struct HP { const double *p; double operator()(int j) const { return p[j]; } };
+extern "C" int cc_shape(int *o) { o[0] = SetGen_cc::NMECH; o[1] = SetGen_cc::NPARAM; o[2] = SetGen_cc::NSTATE;
+    o[3] = SetGen_cc::CAI; o[4] = SetGen_cc::ECA; return 0; }
+extern "C" unsigned cc_bits(unsigned m) { return SetGen_cc::state_bits(m); }
+extern "C" int cc_run(unsigned m, int op, double v, double dt, double *st, const double *P, double *io)
+{
+    HP hp{P};
+    if (op == 0) SetGen_cc::init(m, v, st, hp);
+    else if (op == 1) { SetGen_cc::cnexp(m, v, dt, st, io[1], hp); SetGen_cc::eca_after_cnexp(m, st); }
+    else SetGen_cc::current(m, v, st, hp, io[0], io[1], true);
+    return 0;
+}
+''')
+    lib = tmp_path / "libcc.so"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-ffp-contract=off", "-fPIC", "-shared", "-I", CSRC, "-I",
+                    str(tmp_path), str(tmp_path / "cc.cpp"), "-o", str(lib)], check=True)
+    H = C.CDLL(str(lib))
+    H.cc_run.argtypes = [C.c_uint, C.c_int, C.c_double, C.c_double, _DP, _DP, _DP]
+    H.cc_bits.restype = C.c_uint
+    H.cc_bits.argtypes = [C.c_uint]
+    shp = (C.c_int * 8)()
+    H.cc_shape(shp)
+    nmech, npar, nst, cai, eca = list(shp)[:5]
+    assert (nmech, npar) == (14, 26)
+    # parameters: each mechanism's PARAMETER defaults, in the set's column order
+    P = []
+    for f in sorted(mods, key=lambda p: nmodl.parse(open(p).read()).suffix.lower()):
+        m = nmodl.parse(open(f).read())
+        P += [m.params[p] or 0.0 for p in nmodl.set_parameters(m)]
+    P = np.array(P, dtype=np.float64)
+    assert P.size == npar
+    rng = np.random.default_rng(3)
+    masks = [(1 << nmech) - 1] + [int(rng.integers(1, 1 << nmech)) | 0x4 for _ in range(8)]
+    for mk in masks:
+        st, io = np.zeros(nst), np.zeros(2)
+        H.cc_run(mk, 0, -75.0, 0.0, st.ctypes.data_as(_DP), P.ctypes.data_as(_DP), io.ctypes.data_as(_DP))
+        bits = H.cc_bits(mk)
+        gates = [b for b in range(nst) if bits >> b & 1 and b not in (cai, eca)]
+        for step in range(300):
+            v = -70.0 + 70.0 * math.sin(0.03 * step)
+            io[:] = 0.0
+            H.cc_run(mk, 2, v, 0.0, st.ctypes.data_as(_DP), P.ctypes.data_as(_DP), io.ctypes.data_as(_DP))
+            assert np.all(np.isfinite(io)), (hex(mk), step, io)
+            H.cc_run(mk, 1, v, 0.025, st.ctypes.data_as(_DP), P.ctypes.data_as(_DP), io.ctypes.data_as(_DP))
+            assert np.all(np.isfinite(st)), (hex(mk), step)
+            assert np.all((st[gates] >= 0.0) & (st[gates] <= 1.0)), (hex(mk), step, st[gates])
+            assert st[cai] > 0.0
+    hipcc = "/opt/rocm/bin/hipcc"
+    if not os.path.exists(hipcc):
+        return
+    (tmp_path / "k.hip").write_text('''#include "gen_cc.h"
+struct PF { const double *p; __device__ double operator()(int j) const { return p[j]; } };
+__global__ void k(double *st_g, const double *P, double *out, double dt, int n, unsigned mask)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double st[SetGen_cc::NSTATE];
+    for (int s = 0; s < SetGen_cc::NSTATE; s++) st[s] = st_g[s * n + i];
+    PF pf{P};
+    SetGen_cc::cnexp(mask, out[i], dt, st, 0.0, pf);
+    double sum = 0, ica = 0;
+    SetGen_cc::current(mask, out[i], st, pf, sum, ica, true);
+    for (int s = 0; s < SetGen_cc::NSTATE; s++) st_g[s * n + i] = st[s];
+    out[i] = sum + ica;
+}
+''')
+    r = subprocess.run([hipcc, "--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=off", "-Xclang",
+                        "-target-feature", "-Xclang", "-packed-fp32-ops", "--cuda-device-only", "-c", "-I", CSRC,
+                        "-I", str(tmp_path), str(tmp_path / "k.hip"), "-o", str(tmp_path / "k.o")],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-2000:]
