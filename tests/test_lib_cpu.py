@@ -27,8 +27,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_mechset_sizes_without_a_device():
     L = _lib.lib()
-    assert [L.ngpu_mechset_nparams(i) for i in range(3)] == [2, 19, 30]
-    assert L.ngpu_mechset_nparams(7) < 0
+    assert [L.ngpu_mechset_nparams(i) for i in range(5)] == [2, 19, 30, 19, 30]
+    assert L.ngpu_mechset_nparams(7) < 0 and L.ngpu_mechset_nparams(-1) < 0
 
 
 def test_missing_device_is_a_clean_error():
