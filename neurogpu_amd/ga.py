@@ -26,7 +26,7 @@ from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
-from ._lib import FEATURES, Simulator
+from ._lib import Simulator
 from .selection import ParetoFront, eps_indicator, ibea_fitness, tournament
 
 # Objective of an undefined feature difference, and of every objective of a
